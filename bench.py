@@ -1,0 +1,338 @@
+#!/usr/bin/env python3
+"""Benchmark of the DEC hot path on one MI355X.
+
+A step is one application of each operator of the reference operator set
+(wedge 0-1, 1-1, 0-2; d0, d1; ⋆0, ⋆1, ⋆2; one 0-form Laplacian matvec) on the
+icosphere with 8 binary subdivisions (655,362 V / 1,966,080 E / 1,310,720 T),
+float64 forms, int32 incidence tables, inputs resident in HBM.  Each step uses
+the next of several complete table + buffer sets so that the working set
+cycles through far more memory than the 256 MB Infinity Cache holds: the
+figures are HBM figures.  `value` = algorithmic bytes of the whole step /
+measured time (GB/s).  The same JSON line carries the float32 operator set,
+the Poisson solve (levels 3→8, plain float64 V(2,2) vs float32-in-float64
+iterative refinement), the roofline of the dominant kernel and the CPU
+oracle's throughput on a bounded sample of the same operators.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "combinatorialspaces.jl_amd"))
+
+METRIC = "DEC op GB/s vs HBM peak; Poisson solve s to 1e-8 resid, learned+FP32 vs plain MG"
+HBM_PEAK_GBPS = 8000.0
+OPSET = ["wedge01", "wedge11", "wedge02", "d0", "d1", "hodge0", "hodge1", "hodge2", "laplacian0"]
+
+
+def alg_bytes(name, nv, ne, nt, s, lap_nnz=None):
+    """Compulsory bytes of one application: every table / input element read
+    once, every output written once (int32 indices, s-byte values)."""
+    return {
+        "wedge01": ne * (8 + 2 * s) + nv * s,
+        "wedge11": nt * (12 + 4 * s) + 2 * ne * s,
+        "wedge02": nt * (12 + 5 * s) + nv * s,
+        "wedge_dd01": ne * (8 + 4 * s) + nt * s,
+        "d0": ne * (9 + s) + nv * s,
+        "d1": nt * (13 + s) + ne * s,
+        "boundary1": 2 * ne * 5 + nv * s + ne * s,
+        "boundary2": 3 * nt * 5 + ne * s + nt * s,
+        "hodge0": 3 * s * nv,
+        "hodge1": 3 * s * ne,
+        "hodge2": 3 * s * nt,
+        "laplacian0": (lap_nnz if lap_nnz is not None else nv + 2 * ne) * (4 + s) + 2 * nv * s,
+    }[name]
+
+
+def make_op(cs, mesh, name, td):
+    if name == "wedge01":
+        return cs.dec_wedge_product(0, 1, mesh, cast_float=td).operator
+    if name == "wedge11":
+        return cs.dec_wedge_product(1, 1, mesh, cast_float=td).operator
+    if name == "wedge02":
+        return cs.dec_wedge_product(0, 2, mesh, cast_float=td).operator
+    if name == "wedge_dd01":
+        return cs.dec_wedge_product_dd(0, 1, mesh, cast_float=td).operator
+    if name == "d0":
+        return cs.dec_differential(0, mesh, cast_float=td)
+    if name == "d1":
+        return cs.dec_differential(1, mesh, cast_float=td)
+    if name == "boundary1":
+        return cs.dec_boundary(1, mesh, cast_float=td)
+    if name == "boundary2":
+        return cs.dec_boundary(2, mesh, cast_float=td)
+    if name.startswith("hodge"):
+        return cs.dec_hodge_star(int(name[-1]), mesh, cs.DiagonalHodge, cast_float=td)
+    if name == "laplacian0":
+        return cs.fast_laplace_beltrami(mesh, cs.DiagonalHodge, cast_float=td)
+    raise KeyError(name)
+
+
+def time_opset(torch, cs, mesh, names, td, sets, steps, warmup):
+    """Per-op mean launch duration (ms) with HIP events recorded on the stream
+    the kernels run on, rotating through `sets` independent table/buffer sets."""
+    stream = torch.cuda.current_stream()
+    ops = {n: [make_op(cs, mesh, n, td) for _ in range(sets)] for n in names}
+    bufs = {}
+    seed = 100
+    for n in names:
+        bufs[n] = []
+        for s in range(sets):
+            op = ops[n][s]
+            x = cs.fill_uniform(torch.empty(op.cols, dtype=td, device="cuda"), seed); seed += 1
+            y = (cs.fill_uniform(torch.empty(op.cols2, dtype=td, device="cuda"), seed) if op.cols2 else None)
+            seed += 1
+            out = torch.empty(op.rows, dtype=td, device="cuda")
+            bufs[n].append((x, y, out))
+    for w in range(warmup):
+        s = w % sets
+        for n in names:
+            x, y, out = bufs[n][s]
+            ops[n][s].apply(x, y, out=out, stream=stream)
+    torch.cuda.synchronize()
+    ev = {n: [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+          for n in names}
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for k in range(steps):
+        s = k % sets
+        for n in names:
+            x, y, out = bufs[n][s]
+            a, b = ev[n][k]
+            a.record(stream)
+            ops[n][s].apply(x, y, out=out, stream=stream)
+            b.record(stream)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    per = {n: sum(a.elapsed_time(b) for a, b in ev[n]) / steps for n in names}
+    total = t0.elapsed_time(t1)
+    lap_nnz = None
+    if "laplacian0" in names:
+        lap_nnz = len(ops["laplacian0"][0].csr()[1])
+    # cache-resident figure: one set, back-to-back
+    cache = {}
+    for n in names:
+        x, y, out = bufs[n][0]
+        op = ops[n][0]
+        for _ in range(3):
+            op.apply(x, y, out=out, stream=stream)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(20):
+            op.apply(x, y, out=out, stream=stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        cache[n] = a.elapsed_time(b) / 20
+    del ops, bufs
+    torch.cuda.empty_cache()
+    return per, total, cache, lap_nnz
+
+
+def stream_copy_gbps(torch):
+    n = 1 << 28  # 2 GiB of float64 per buffer
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    for _ in range(3):
+        y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(10):
+        y.copy_(x)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / 10
+    del x, y
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / (ms * 1e-3) / 1e9
+
+
+def sh_rhs(points, h0, seed, nterms=16, lmax=12):
+    """f = Σ_j c_j Y_{l_j m_j} (real orthonormal spherical harmonics, l ≤ 12),
+    (l, m, c) drawn from a seeded stream, projected to ⋆0-weighted mean zero."""
+    import numpy as np
+    from scipy.special import sph_harm_y
+
+    rng = np.random.default_rng(seed)
+    x, y, z = points[:, 0], points[:, 1], points[:, 2]
+    theta = np.arccos(np.clip(z, -1, 1))
+    phi = np.arctan2(y, x)
+    f = np.zeros(len(points))
+    for _ in range(nterms):
+        l = int(rng.integers(1, lmax + 1))
+        m = int(rng.integers(-l, l + 1))
+        c = rng.uniform(-1, 1)
+        Y = sph_harm_y(l, abs(m), theta, phi)
+        if m > 0:
+            Yr = math.sqrt(2) * (-1) ** m * Y.real
+        elif m < 0:
+            Yr = math.sqrt(2) * (-1) ** m * Y.imag
+        else:
+            Yr = Y.real
+        f += c * Yr
+    return f - (h0 @ f) / h0.sum()
+
+
+def poisson(torch, cs, base_k=3, levels=5):
+    import numpy as np
+
+    t = time.perf_counter()
+    md = cs.MultigridData(cs.icosphere(base_k), cs.BinarySubdivision, levels, cs.DiagonalHodge, 2, project=True)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t
+    fine = md.finest_mesh()
+    pts = fine.export()["points"]
+    h0 = cs.dec_hodge_star(0, fine, cs.DiagonalHodge) @ torch.ones(fine.nv, dtype=torch.float64, device="cuda")
+    f = sh_rhs(pts, h0.cpu().numpy(), seed=2024)
+    fd = torch.tensor(f, device="cuda")
+    out = {"setup_s": setup, "levels": f"{base_k}->{base_k + levels}", "n_fine": fine.nv}
+    for variant in ("plain", "mixed"):
+        cs.solve(md, fd, tol=1e-8, variant=variant)       # warm-up (allocations, code objects)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        u, cyc, rr = cs.solve(md, fd, tol=1e-8, variant=variant)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        key = "plain_f64" if variant == "plain" else "mixed_f32_ir"
+        out[key] = {"seconds": dt, "cycles": cyc, "relres_f64": rr}
+    out["speedup_mixed_vs_plain"] = out["plain_f64"]["seconds"] / out["mixed_f32_ir"]["seconds"]
+    return out
+
+
+def cpu_baseline(k=7, budget_s=12.0):
+    """The oracle (single-threaded C) on the same operator set, icosphere k=7."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    lib = O.lib()
+    d = O.Dual(O.icosphere(k))
+    nv, ne, nt = d.nv, d.ne, d.nt
+    v02, c02 = d.wedge02_coeffs()
+    e11, c11 = d.wedge11_coeffs()
+    D0, D1 = d.differential(0), d.differential(1)
+    L = d.laplacian0()
+    h = [d.hodge_diag(i) for i in range(3)]
+    V, E, E2, T = (O.uniform_form(s, n) for s, n in ((1, nv), (2, ne), (3, ne), (4, nt)))
+    oE, oT, oV = np.zeros(ne), np.zeros(nt), np.zeros(nv)
+    P = O._p
+    ci = ctypes.c_int
+    D0s, D1s, Ls = D0.struct(), D1.struct(), L.struct()
+
+    def one_pass():
+        lib.om_wedge01(ci(ne), P(d.m.v0), P(d.m.v1), P(V), P(E), P(oE))
+        lib.om_wedge11(ci(nt), P(e11), P(c11), P(E), P(E2), P(oT))
+        lib.om_wedge02(ci(nt), P(v02), P(c02), P(V), P(T), P(oT))
+        lib.om_spmv(ctypes.byref(D0s), P(V), P(oE))
+        lib.om_spmv(ctypes.byref(D1s), P(E), P(oT))
+        np.multiply(h[0], V, out=oV)
+        np.multiply(h[1], E, out=oE)
+        np.multiply(h[2], T, out=oT)
+        lib.om_spmv(ctypes.byref(Ls), P(V), P(oV))
+
+    one_pass()
+    passes, t = 0, time.perf_counter()
+    while time.perf_counter() - t < budget_s:
+        one_pass()
+        passes += 1
+    dt = time.perf_counter() - t
+    nbytes = sum(alg_bytes(n, nv, ne, nt, 8, L.nnz) for n in OPSET)
+    return {"value": passes * nbytes / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (C, -O2, 1 thread) float64, icosphere k={k} ({nv} V), {passes} passes of the "
+                      f"operator set in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--k", type=int, default=8)
+    ap.add_argument("--sets", type=int, default=6)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the operators would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import cspaces as cs
+
+    torch.cuda.init()
+    t = time.perf_counter()
+    mesh = cs.icosphere(args.k)
+    cs.dualize(mesh)
+    torch.cuda.synchronize()
+    setup_mesh = time.perf_counter() - t
+    nv, ne, nt = mesh.nv, mesh.ne, mesh.nt
+    print(f"# icosphere k={args.k}: {nv} V {ne} E {nt} T, mesh+dual setup {setup_mesh:.2f} s", file=sys.stderr,
+          flush=True)
+
+    per, total, cache, lap_nnz = time_opset(torch, cs, mesh, OPSET, torch.float64, args.sets, args.steps,
+                                            args.warmup)
+    B = {n: alg_bytes(n, nv, ne, nt, 8, lap_nnz) for n in OPSET}
+    step_bytes = sum(B.values())
+    ms_per_step = total / args.steps
+    value = step_bytes / (ms_per_step * 1e-3) / 1e9
+    ops = {n: {"ms": per[n], "MB": B[n] / 1e6, "GBps": B[n] / (per[n] * 1e-3) / 1e9,
+               "frac_hbm_peak": B[n] / (per[n] * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+               "cache_resident_GBps": B[n] / (cache[n] * 1e-3) / 1e9} for n in OPSET}
+    print(f"# f64 op set: {value:.0f} GB/s aggregate", file=sys.stderr, flush=True)
+    dom = max(OPSET, key=lambda n: per[n])
+    kernel_name = {"wedge01": "k_wedge01", "wedge11": "k_wedge11", "wedge02": "k_wedge02", "d0": "k_d0",
+                   "d1": "k_d1", "hodge0": "k_diag", "hodge1": "k_diag", "hodge2": "k_diag",
+                   "laplacian0": "k_ell"}[dom]
+    roofline = {"bound": "hbm", "kernel": kernel_name, "op": dom, "achieved": ops[dom]["GBps"],
+                "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": ops[dom]["GBps"] / HBM_PEAK_GBPS,
+                "traffic": None, "alg_bytes_per_launch": B[dom], "avg_launch_ms": per[dom]}
+
+    extra = {}
+    extra_names = ["wedge_dd01", "boundary1", "boundary2"]
+    per_x, _, cache_x, _ = time_opset(torch, cs, mesh, extra_names, torch.float64, args.sets, max(10, args.steps // 2),
+                                      2)
+    extra["other_ops_f64"] = {n: {"ms": per_x[n], "GBps": alg_bytes(n, nv, ne, nt, 8) / (per_x[n] * 1e-3) / 1e9}
+                              for n in extra_names}
+    if not args.no_f32:
+        per32, total32, cache32, lap32 = time_opset(torch, cs, mesh, OPSET, torch.float32, args.sets, args.steps,
+                                                    args.warmup)
+        B32 = {n: alg_bytes(n, nv, ne, nt, 4, lap32) for n in OPSET}
+        extra["f32"] = {"value_GBps": sum(B32.values()) / (total32 / args.steps * 1e-3) / 1e9,
+                        "ops": {n: {"ms": per32[n], "GBps": B32[n] / (per32[n] * 1e-3) / 1e9} for n in OPSET}}
+        print(f"# f32 op set: {extra['f32']['value_GBps']:.0f} GB/s aggregate", file=sys.stderr, flush=True)
+    try:
+        extra["stream_copy_GBps"] = stream_copy_gbps(torch)
+    except RuntimeError as e:  # memory
+        extra["stream_copy_GBps"] = None
+        print(f"# stream copy skipped: {e}", file=sys.stderr)
+    if not args.no_solve:
+        extra["poisson"] = poisson(torch, cs)
+        print(f"# poisson: {json.dumps(extra['poisson'])}", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"icosphere k={args.k} ({nv} V / {ne} E / {nt} T), circumcentric dual, "
+                               "one application each of wedge 0-1, 1-1, 0-2, d0, d1, ⋆0, ⋆1, ⋆2 and the "
+                               "0-form Laplacian per step, float64 forms, int32 tables, "
+                               f"{args.sets} rotating table+buffer sets (HBM, not cache)",
+                   "mesh_vertices": nv, "step_bytes": step_bytes, "rotation_sets": args.sets},
+        "roofline": roofline, "cpu_baseline": cpu, "ops": ops, "setup_mesh_s": setup_mesh, **extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,91 @@
+"""ctypes binding of libdec_hip.so (the C ABI declared in include/dec_hip.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int8, c_int32, c_int64, c_uint64, c_void_p
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libdec_hip.so")
+
+DEC_OK = 0
+DEC_F64, DEC_F32 = 0, 1
+DEC_CIRCUMCENTER, DEC_BARYCENTER = 0, 1
+DEC_GEOMETRIC_HODGE, DEC_DIAGONAL_HODGE = 0, 1
+(DEC_OP_WEDGE, DEC_OP_WEDGE_DD, DEC_OP_DIFFERENTIAL, DEC_OP_BOUNDARY, DEC_OP_DUAL_DERIVATIVE,
+ DEC_OP_HODGE, DEC_OP_INV_HODGE, DEC_OP_LAPLACIAN0, DEC_OP_AVG01) = range(1, 10)
+DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI = 0, 1, 2
+DEC_SOLVE_PLAIN_F64, DEC_SOLVE_MIXED_IR = 0, 1
+
+STATUS_NAMES = {0: "DEC_OK", 1: "DEC_ERR_ARG", 2: "DEC_ERR_UNSUPPORTED", 3: "DEC_ERR_OOM",
+                4: "DEC_ERR_HIP", 5: "DEC_ERR_NOCONV", 6: "DEC_ERR_NONORIENTABLE"}
+
+# name -> (restype, argtypes); every symbol include/dec_hip.h declares.
+P = c_void_p
+PI32, PI8, PF64, PI64 = POINTER(c_int32), POINTER(c_int8), POINTER(c_double), POINTER(c_int64)
+SIGNATURES = {
+    "dec_last_error": (c_char_p, []),
+    "dec_version": (c_char_p, []),
+    "dec_fill_uniform": (c_int, [P, c_int64, c_uint64, c_int, P]),
+    "dec_mesh_create": (c_int, [c_int32, c_int32, c_int32, P, P, P, P, P, P, P, P, c_int32, POINTER(P)]),
+    "dec_mesh_icosphere": (c_int, [c_int32, POINTER(P)]),
+    "dec_mesh_triangulated_grid": (c_int, [c_double, c_double, c_double, c_double, c_int32, POINTER(P)]),
+    "dec_mesh_subdivide": (c_int, [P, c_int32, POINTER(P)]),
+    "dec_mesh_counts": (c_int, [P, PI32, PI32, PI32]),
+    "dec_mesh_export": (c_int, [P, P, P, P, P, P, P, P, P]),
+    "dec_mesh_dualize": (c_int, [P, c_int32]),
+    "dec_mesh_export_dual": (c_int, [P, P, P, P, P, P]),
+    "dec_mesh_destroy": (c_int, [P]),
+    "dec_op_create": (c_int, [P, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(P)]),
+    "dec_op_shape": (c_int, [P, PI64, PI64, PI64]),
+    "dec_op_apply": (c_int, [P, P, P, P, P]),
+    "dec_op_export_csr": (c_int, [P, PI64, P, P, P]),
+    "dec_op_export_tables": (c_int, [P, P, P]),
+    "dec_op_destroy": (c_int, [P]),
+    "dec_mg_create": (c_int, [P, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(P)]),
+    "dec_mg_nlevels": (c_int, [P, PI32]),
+    "dec_mg_level_size": (c_int, [P, c_int32, PI32]),
+    "dec_mg_finest_mesh": (c_int, [P, POINTER(P)]),
+    "dec_mg_export_level": (c_int, [P, c_int32, c_int32, PI64, P, P, P]),
+    "dec_mg_set_smoother": (c_int, [P, c_int32, c_int32, c_double, c_int32]),
+    "dec_mg_cycles": (c_int, [P, c_int32, c_int32, P, P, P]),
+    "dec_mg_full_multigrid": (c_int, [P, c_int32, c_int32, P, P, P]),
+    "dec_mg_solve": (c_int, [P, c_int32, c_double, c_int32, P, P, PI32, PF64, P]),
+    "dec_mg_destroy": (c_int, [P]),
+}
+
+
+class DecError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libdec_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != DEC_OK:
+        msg = lib().dec_last_error()
+        raise DecError(status, msg.decode() if msg else "")
+    return status
+
+
+def ptr(a):
+    """Host numpy array -> void*."""
+    return None if a is None else a.ctypes.data_as(c_void_p)

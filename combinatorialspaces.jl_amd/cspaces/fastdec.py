@@ -1,0 +1,334 @@
+"""FastDEC operators on the GPU, with CombinatorialSpaces.jl's call signatures.
+
+Mirrors src/FastDEC.jl (and its CUDA placement in
+ext/CombinatorialSpacesCUDAExt.jl) over the C ABI of libdec_hip.so.  Forms are
+torch tensors on the GPU; `backend` plays the role of `Val(:HIP)`, and
+`cast_float` (torch.float32 / torch.float64) the reference's cast_float.
+Operators that the reference returns as sparse matrices are returned as
+`Operator` objects: `op @ x` or `op(x)` applies them on the device and
+`op.csr()` exports the matrix for inspection.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import c_int32, c_int64, c_void_p
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib, ptr
+
+HIP = "HIP"
+
+
+class DiagonalHodge:
+    code = L.DEC_DIAGONAL_HODGE
+
+
+class GeometricHodge:
+    code = L.DEC_GEOMETRIC_HODGE
+
+
+class Circumcenter:
+    code = L.DEC_CIRCUMCENTER
+
+
+class Barycenter:
+    code = L.DEC_BARYCENTER
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _dtype_code(cast_float):
+    torch = _torch()
+    if cast_float is None or cast_float in (torch.float64, np.float64, "float64", "f64"):
+        return L.DEC_F64
+    if cast_float in (torch.float32, np.float32, "float32", "f32"):
+        return L.DEC_F32
+    raise ValueError(f"unsupported cast_float {cast_float!r}")
+
+
+def _torch_dtype(code):
+    torch = _torch()
+    return torch.float64 if code == L.DEC_F64 else torch.float32
+
+
+def _stream(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return c_void_p(s.cuda_stream)
+
+
+def _hodge(h):
+    if h is None:
+        return GeometricHodge.code
+    if isinstance(h, int):
+        return h
+    return h.code
+
+
+# ---------------------------------------------------------------------------
+# Meshes
+# ---------------------------------------------------------------------------
+class Mesh:
+    """A primal 2-D complex on the GPU (EmbeddedDeltaSet2D) that becomes its
+    own dual complex (EmbeddedDeltaDualComplex2D) once `dualize` has run."""
+
+    def __init__(self, handle, owned=True):
+        self._h = c_void_p(handle) if not isinstance(handle, c_void_p) else handle
+        self._owned = owned
+        nv, ne, nt = c_int32(), c_int32(), c_int32()
+        check(lib().dec_mesh_counts(self._h, ctypes.byref(nv), ctypes.byref(ne), ctypes.byref(nt)))
+        self.nv, self.ne, self.nt = nv.value, ne.value, nt.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and self._h:
+            try:
+                lib().dec_mesh_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def export(self):
+        """Host copies: dict of ∂v0, ∂v1, ∂e0, ∂e1, ∂e2 (0-based), orientations, points."""
+        out = {
+            "v0": np.zeros(self.ne, np.int32), "v1": np.zeros(self.ne, np.int32),
+            "e0": np.zeros(self.nt, np.int32), "e1": np.zeros(self.nt, np.int32),
+            "e2": np.zeros(self.nt, np.int32),
+            "edge_orientation": np.zeros(self.ne, np.int8),
+            "tri_orientation": np.zeros(self.nt, np.int8),
+            "points": np.zeros((self.nv, 3)),
+        }
+        check(lib().dec_mesh_export(self._h, ptr(out["v0"]), ptr(out["v1"]), ptr(out["e0"]),
+                                    ptr(out["e1"]), ptr(out["e2"]), ptr(out["edge_orientation"]),
+                                    ptr(out["tri_orientation"]), ptr(out["points"])))
+        return out
+
+    def export_dual(self):
+        nv, ne, nt = self.nv, self.ne, self.nt
+        out = {"dual_point": np.zeros((nv + ne + nt, 3)), "length": np.zeros(ne),
+               "area": np.zeros(nt), "dual_length": np.zeros(2 * ne + 6 * nt),
+               "dual_area": np.zeros(6 * nt)}
+        check(lib().dec_mesh_export_dual(self._h, ptr(out["dual_point"]), ptr(out["length"]),
+                                         ptr(out["area"]), ptr(out["dual_length"]),
+                                         ptr(out["dual_area"])))
+        return out
+
+
+def mesh_from_arrays(points, v0, v1, e0, e1, e2, edge_orientation=None, tri_orientation=None,
+                     index_base=0) -> Mesh:
+    """EmbeddedDeltaSet2D from boundary maps (MeshTopology / topo_to_mesh)."""
+    a = [np.ascontiguousarray(x, np.int32) for x in (v0, v1, e0, e1, e2)]
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    eo = None if edge_orientation is None else np.ascontiguousarray(edge_orientation, np.int8)
+    to = None if tri_orientation is None else np.ascontiguousarray(tri_orientation, np.int8)
+    h = c_void_p()
+    check(lib().dec_mesh_create(len(p), len(a[0]), len(a[2]), ptr(a[0]), ptr(a[1]), ptr(a[2]),
+                                ptr(a[3]), ptr(a[4]), ptr(eo), ptr(to), ptr(p), index_base,
+                                ctypes.byref(h)))
+    return Mesh(h)
+
+
+def icosphere(k: int) -> Mesh:
+    """Pole-aligned icosahedron refined k times, new points on the unit sphere."""
+    h = c_void_p()
+    check(lib().dec_mesh_icosphere(k, ctypes.byref(h)))
+    return Mesh(h)
+
+
+def triangulated_grid(lx, ly, dx, dy, compress=True) -> Mesh:
+    """triangulated_grid(lx, ly, dx, dy, Point3d, compress) (CombMeshes.jl:79-130)."""
+    h = c_void_p()
+    check(lib().dec_mesh_triangulated_grid(lx, ly, dx, dy, int(bool(compress)), ctypes.byref(h)))
+    return Mesh(h)
+
+
+def subdivision(s: Mesh, project=False) -> Mesh:
+    """subdivision(s, BinarySubdivision()) (Multigrid.jl:447-452)."""
+    h = c_void_p()
+    check(lib().dec_mesh_subdivide(s.handle, int(bool(project)), ctypes.byref(h)))
+    return Mesh(h)
+
+
+def dualize(s: Mesh, center=Circumcenter) -> Mesh:
+    """dualize(s, alg): dual complex + geometry, computed on the GPU in float64."""
+    check(lib().dec_mesh_dualize(s.handle, center.code if hasattr(center, "code") else int(center)))
+    return s
+
+
+# ---------------------------------------------------------------------------
+# Operators
+# ---------------------------------------------------------------------------
+class Operator:
+    """A device operator; unary operators behave like the reference's sparse
+    matrices (`op @ x`), binary ones (wedges) are called as `op(α, β)`."""
+
+    def __init__(self, sd: Mesh, kind, m=0, n=0, hodge=GeometricHodge, cast_float=None):
+        self.mesh = sd
+        self.kind, self.m, self.n = kind, m, n
+        self.dtype_code = _dtype_code(cast_float)
+        self.dtype = _torch_dtype(self.dtype_code)
+        h = c_void_p()
+        check(lib().dec_op_create(sd.handle, kind, m, n, _hodge(hodge), self.dtype_code, ctypes.byref(h)))
+        self._h = h
+        r, c, c2 = c_int64(), c_int64(), c_int64()
+        check(lib().dec_op_shape(h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(c2)))
+        self.rows, self.cols, self.cols2 = r.value, c.value, c2.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def shape(self):
+        return (self.rows, self.cols)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().dec_op_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _check_form(self, x, n, name):
+        torch = _torch()
+        if not isinstance(x, torch.Tensor) or not x.is_cuda:
+            raise TypeError(f"{name} must be a GPU torch tensor")
+        if x.dtype != self.dtype:
+            raise TypeError(f"{name} has dtype {x.dtype}, operator computes in {self.dtype}")
+        if x.numel() != n or not x.is_contiguous():
+            raise ValueError(f"{name} must be contiguous with {n} entries, got {tuple(x.shape)}")
+
+    def apply(self, x, y=None, out=None, stream=None):
+        torch = _torch()
+        self._check_form(x, self.cols, "x")
+        if self.cols2:
+            if y is None:
+                raise TypeError("this operator takes two forms")
+            self._check_form(y, self.cols2, "y")
+        if out is None:
+            out = torch.empty(self.rows, dtype=self.dtype, device=x.device)
+        else:
+            self._check_form(out, self.rows, "out")
+        check(lib().dec_op_apply(self._h, c_void_p(x.data_ptr()),
+                                 c_void_p(y.data_ptr()) if y is not None else None,
+                                 c_void_p(out.data_ptr()), _stream(stream)))
+        return out
+
+    __call__ = apply
+
+    def __matmul__(self, x):
+        return self.apply(x)
+
+    def csr(self):
+        """(rowptr, col, val) of the operator's matrix (float64 values)."""
+        nnz = c_int64()
+        check(lib().dec_op_export_csr(self._h, ctypes.byref(nnz), None, None, None))
+        rp = np.zeros(self.rows + 1, np.int32)
+        col = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value)
+        check(lib().dec_op_export_csr(self._h, None, ptr(rp), ptr(col), ptr(val)))
+        return rp, col, val
+
+    def dense(self):
+        rp, col, val = self.csr()
+        D = np.zeros((self.rows, self.cols))
+        for r in range(self.rows):
+            for k in range(rp[r], rp[r + 1]):
+                D[r, col[k]] += val[k]
+        return D
+
+    def tables(self, nidx, ncoeff):
+        idx = np.zeros(nidx * self.rows, np.int32)
+        coeff = np.zeros(max(ncoeff, 1) * self.rows)
+        check(lib().dec_op_export_tables(self._h, ptr(idx), ptr(coeff)))
+        return idx.reshape(nidx, self.rows), coeff.reshape(max(ncoeff, 1), self.rows)[:ncoeff]
+
+
+def _check_backend(backend):
+    if backend not in (HIP, "hip", None):
+        raise ValueError("this library only implements the HIP backend")
+
+
+def dec_wedge_product(m, n, sd: Mesh, backend=HIP, arr_cons=None, cast_float=None):
+    """dec_wedge_product(m, n, sd, Val(:HIP), arr_cons, cast_float) (FastDEC.jl:255-289):
+    returns a function (α, β) -> α ∧ β computed on the GPU."""
+    _check_backend(backend)
+    if (m, n) == (0, 0):
+        return lambda f, g: f * g
+    op = Operator(sd, L.DEC_OP_WEDGE, m, n, cast_float=cast_float)
+    fn = lambda a, b, out=None, stream=None: op.apply(a, b, out=out, stream=stream)  # noqa: E731
+    fn.operator = op
+    return fn
+
+
+def dec_c_wedge_product_(m, n, res, a, b, op: Operator, stream=None):
+    """In-place form of dec_c_wedge_product! (FastDEC.jl:217-232): res .= α ∧ β."""
+    return op.apply(a, b, out=res, stream=stream)
+
+
+def dec_wedge_product_dd(m, n, sd: Mesh, backend=HIP, cast_float=None):
+    """dec_wedge_product_dd (FastDEC.jl:313-325): dual-dual wedge 0-1 / 1-0."""
+    _check_backend(backend)
+    op = Operator(sd, L.DEC_OP_WEDGE_DD, m, n, cast_float=cast_float)
+    fn = lambda f, g, out=None, stream=None: op.apply(f, g, out=out, stream=stream)  # noqa: E731
+    fn.operator = op
+    return fn
+
+
+def dec_differential(n, sd: Mesh, backend=HIP, cast_float=None) -> Operator:
+    """dec_differential(n, sd) (FastDEC.jl:468-469)."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_DIFFERENTIAL, n, cast_float=cast_float)
+
+
+def dec_boundary(n, sd: Mesh, backend=HIP, cast_float=None) -> Operator:
+    """dec_boundary(n, sd) (FastDEC.jl:433-437)."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_BOUNDARY, n, cast_float=cast_float)
+
+
+def dec_dual_derivative(n, sd: Mesh, backend=HIP, cast_float=None) -> Operator:
+    """dec_dual_derivative(n, sd) (FastDEC.jl:443-453)."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_DUAL_DERIVATIVE, n, cast_float=cast_float)
+
+
+def dec_hodge_star(n, sd: Mesh, hodge=GeometricHodge, backend=HIP, cast_float=None) -> Operator:
+    """dec_hodge_star(n, sd, hodge) (FastDEC.jl:641-716)."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_HODGE, n, hodge=hodge, cast_float=cast_float)
+
+
+def dec_inv_hodge_star(n, sd: Mesh, hodge=GeometricHodge, backend=HIP, cast_float=None) -> Operator:
+    """dec_inv_hodge_star(n, sd, hodge) (FastDEC.jl:721-743)."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_INV_HODGE, n, hodge=hodge, cast_float=cast_float)
+
+
+def avg01_mat(sd: Mesh, backend=HIP, cast_float=None) -> Operator:
+    """avg₀₁_mat(sd) (FastDEC.jl:874-892)."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_AVG01, cast_float=cast_float)
+
+
+def fast_laplace_beltrami(sd: Mesh, hodge=GeometricHodge, backend=HIP, cast_float=None) -> Operator:
+    """-1 * ⋆0⁻¹ * d̃1 * ⋆1 * d0 (test/Multigrid.jl:46-48), i.e. ∇²(0, sd)."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_LAPLACIAN0, hodge=hodge, cast_float=cast_float)
+
+
+def fill_uniform(x, seed, stream=None):
+    """x[i] = 2u - 1 with u from splitmix64(seed * 2^32 + i) (device-side)."""
+    code = L.DEC_F64 if x.dtype == _torch().float64 else L.DEC_F32
+    check(lib().dec_fill_uniform(c_void_p(x.data_ptr()), x.numel(), seed, code, _stream(stream)))
+    return x

@@ -1,0 +1,796 @@
+// Geometric multigrid of libdec_hip.so (src/Multigrid.jl): the binary-
+// subdivision hierarchy in DirectMode with op = fast_laplace_beltrami, and the
+// μ-cycles / full multigrid / tolerance solves on the GPU.
+//
+// Per level the operator is held in slot-major ELL with rows sorted by
+// (colour, vertex): a multicolour Gauss-Seidel pass over one colour is then a
+// contiguous range of rows.  The same storage serves residuals and the CG
+// smoother (rows write back through their vertex id).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+#include "dec_internal.h"
+#include "kernels_common.h"
+
+using dec::DBuf;
+using dec::fail;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kDotBlocks = 256;
+
+// ---------------------------------------------------------------- kernels
+
+template <class T>
+__global__ void k_zero(int n, T *__restrict__ x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = static_cast<T>(0);
+}
+
+// r[v] = b[v] - (A u)[v], row sums in ascending column order (b - A*u).
+template <class T>
+__global__ void k_residual(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
+                           const int *__restrict__ rowid, const T *__restrict__ b, const T *__restrict__ u,
+                           T *__restrict__ r) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  T acc = static_cast<T>(0);
+  for (int k = 0; k < width; k++) {
+    size_t q = static_cast<size_t>(k) * n + p;
+    int c = col[q];
+    if (c < 0) break;
+    acc = acc + val[q] * u[c];
+  }
+  int v = rowid[p];
+  r[v] = b[v] - acc;
+}
+
+// y[v] = (A x)[v]
+template <class T>
+__global__ void k_spmv_rows(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
+                            const int *__restrict__ rowid, const T *__restrict__ x, T *__restrict__ y,
+                            const int *__restrict__ done) {
+  if (done && *done) return;
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  T acc = static_cast<T>(0);
+  for (int k = 0; k < width; k++) {
+    size_t q = static_cast<size_t>(k) * n + p;
+    int c = col[q];
+    if (c < 0) break;
+    acc = acc + val[q] * x[c];
+  }
+  y[rowid[p]] = acc;
+}
+
+// One colour of Gauss-Seidel: u[v] = (b[v] - Σ_{w≠v} A[v,w] u[w]) / A[v,v].
+template <class T>
+__global__ void k_gs_color(int p0, int p1, int n, int width, const int *__restrict__ col,
+                           const T *__restrict__ val, const int *__restrict__ rowid, const T *__restrict__ b,
+                           T *__restrict__ u) {
+  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p1) return;
+  int v = rowid[p];
+  T s = static_cast<T>(0), d = static_cast<T>(0);
+  for (int k = 0; k < width; k++) {
+    size_t q = static_cast<size_t>(k) * n + p;
+    int c = col[q];
+    if (c < 0) break;
+    T a = val[q];
+    if (c == v) d = a;
+    else s = s + a * u[c];
+  }
+  u[v] = (b[v] - s) / d;
+}
+
+// Damped Jacobi: un[v] = u[v] + ω (b[v] - (A u)[v]) / A[v,v].
+template <class T>
+__global__ void k_jacobi(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
+                         const int *__restrict__ rowid, const T *__restrict__ b, const T *__restrict__ u,
+                         T *__restrict__ un, T omega) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int v = rowid[p];
+  T s = static_cast<T>(0), d = static_cast<T>(0);
+  for (int k = 0; k < width; k++) {
+    size_t q = static_cast<size_t>(k) * n + p;
+    int c = col[q];
+    if (c < 0) break;
+    T a = val[q];
+    if (c == v) d = a;
+    s = s + a * u[c];
+  }
+  un[v] = u[v] + omega * ((b[v] - s) / d);
+}
+
+template <class T>
+__global__ void k_copy(int n, const T *__restrict__ x, T *__restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+
+// b_c = R r_f; rows are coarse vertices in natural order (CSC order of R:
+// the identity column first, then the midpoint columns ascending).
+template <class T>
+__global__ void k_restrict(int nc, int width, const int *__restrict__ col, const T *__restrict__ val,
+                           const T *__restrict__ rf, T *__restrict__ bc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  T acc = static_cast<T>(0);
+  for (int k = 0; k < width; k++) {
+    size_t q = static_cast<size_t>(k) * nc + i;
+    int c = col[q];
+    if (c < 0) break;
+    acc = acc + val[q] * rf[c];
+  }
+  bc[i] = acc;
+}
+
+// u_f += P z_c with P = transpose(S): coarse vertices copy, midpoints average
+// their edge's endpoints (rows (min, max) of S's column).
+template <class T>
+__global__ void k_prolong_add(int nf, int nc, const int *__restrict__ pa, const int *__restrict__ pb,
+                              const T *__restrict__ zc, T *__restrict__ uf) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nf) return;
+  T pz;
+  if (j < nc) {
+    pz = zc[j];
+  } else {
+    int e = j - nc;
+    pz = static_cast<T>(0.5) * zc[pa[e]] + static_cast<T>(0.5) * zc[pb[e]];
+  }
+  uf[j] = uf[j] + pz;
+}
+
+// u += Z r, one 64-lane wavefront per row, fixed reduction order.
+template <class T>
+__global__ void k_gemv_add(int n, const T *__restrict__ Z, const T *__restrict__ r, T *__restrict__ u) {
+  int lane = threadIdx.x & 63;
+  int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const T *z = Z + static_cast<size_t>(row) * n;
+  T acc = static_cast<T>(0);
+  for (int j = lane; j < n; j += 64) acc = acc + z[j] * r[j];
+  for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
+  if (lane == 0) u[row] = u[row] + acc;
+}
+
+template <class T>
+__device__ double block_sum(double v) {
+  __shared__ double sh[kBlock];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// Deterministic dot product: fixed grid of partial sums, then one block.
+template <class T>
+__global__ void k_dot_partial(int n, const T *__restrict__ x, const T *__restrict__ y, double *__restrict__ part,
+                              const int *__restrict__ done) {
+  if (done && *done) return;
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kDotBlocks * kBlock)
+    acc = acc + static_cast<double>(x[i]) * static_cast<double>(y[i]);
+  double s = block_sum<T>(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_sum_partials(const double *__restrict__ part, double *__restrict__ out,
+                               const int *__restrict__ done) {
+  if (done && *done) return;
+  double s = block_sum<double>(threadIdx.x < kDotBlocks ? part[threadIdx.x] : 0.0);
+  if (threadIdx.x == 0) *out = s;
+}
+
+// Krylov.jl cg recurrence with device-resident scalars (warm start,
+// atol = rtol = sqrt(eps), zero-curvature exit).
+struct CgState {
+  double gamma, rnorm, eps_stop, pnorm2, dot, alpha, beta;
+  int done, iter;
+};
+
+__global__ void k_cg_start(CgState *st, int n, double *__restrict__ x, const double *__restrict__ r,
+                           double *__restrict__ p) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    x[i] = 0.0;
+    p[i] = r[i];
+  }
+}
+
+__global__ void k_cg_init_scalars(CgState *st) {
+  double g = st->dot;
+  st->gamma = g;
+  st->rnorm = sqrt(g);
+  const double sq = 1.4901161193847656e-08;
+  st->eps_stop = sq + sq * st->rnorm;
+  st->pnorm2 = g;
+  st->iter = 0;
+  st->done = (g == 0.0) || (st->rnorm <= st->eps_stop);
+}
+
+__global__ void k_cg_alpha(CgState *st) {
+  if (st->done) return;
+  const double eps = 2.220446049250313e-16;
+  double pAp = st->dot;
+  if (pAp <= eps * st->pnorm2 && fabs(pAp) <= eps * st->pnorm2) {
+    st->done = 1;
+    return;
+  }
+  st->alpha = st->gamma / pAp;
+}
+
+__global__ void k_cg_xr(CgState *st, int n, double *__restrict__ x, const double *__restrict__ p,
+                        double *__restrict__ r, const double *__restrict__ Ap) {
+  if (st->done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a = st->alpha;
+  x[i] = x[i] + a * p[i];
+  r[i] = r[i] - a * Ap[i];
+}
+
+__global__ void k_cg_beta(CgState *st) {
+  if (st->done) return;
+  double gn = st->dot;
+  double rn = sqrt(gn);
+  st->rnorm = rn;
+  st->iter += 1;
+  if (rn <= st->eps_stop || rn + 1.0 <= 1.0) {
+    st->done = 2;  // solved: p is left as is
+    return;
+  }
+  double beta = gn / st->gamma;
+  st->beta = beta;
+  st->pnorm2 = gn + beta * beta * st->pnorm2;
+  st->gamma = gn;
+}
+
+__global__ void k_cg_p(CgState *st, int n, double *__restrict__ p, const double *__restrict__ r) {
+  if (st->done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = r[i] + st->beta * p[i];
+}
+
+__global__ void k_axpy_final(int n, const double *__restrict__ x, double *__restrict__ u) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[i] = x[i] + u[i];
+}
+
+__global__ void k_d2f(int n, const double *__restrict__ x, float *__restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = static_cast<float>(x[i]);
+}
+
+__global__ void k_add_f2d(int n, const float *__restrict__ c, double *__restrict__ u) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[i] = u[i] + static_cast<double>(c[i]);
+}
+
+// ---------------------------------------------------------------- storage
+
+template <class T>
+struct LevelVals {
+  DBuf<T> val;   // ELL values of A (colour-sorted rows)
+  DBuf<T> rval;  // ELL values of R (this level -> next coarser)
+  DBuf<T> Z;     // dense coarse operator (coarsest level, direct solve)
+  DBuf<T> u, b, r, t;  // work vectors
+};
+
+struct Level {
+  int n = 0, width = 0;
+  DBuf<int32_t> col, rowid;
+  std::vector<int> cstart;  // colour ranges of positions
+  int ncolors = 0;
+  // restriction to the next coarser level
+  int nc = 0, rwidth = 0;
+  DBuf<int32_t> rcol;
+  DBuf<int32_t> pa, pb;  // prolongation from the next coarser level (midpoint endpoints)
+  LevelVals<double> d;
+  LevelVals<float> f;
+  dec::HostCsr A, R, P;  // host copies for export
+  std::vector<double> Zh;
+};
+
+}  // namespace
+
+struct dec_mg_s {
+  std::vector<std::unique_ptr<dec_mesh_s>> meshes;  // finest first
+  std::vector<std::unique_ptr<Level>> lv;            // finest first
+  int alg = DEC_SMOOTH_CG, steps = 3, coarse_direct = 0;
+  double omega = 2.0 / 3.0;
+  DBuf<double> part, scal;  // reduction partials, scalar outputs
+  DBuf<CgState> cg;
+  DBuf<double> cgx, cgp, cgAp;  // CG vectors (finest size)
+};
+
+namespace {
+
+inline dim3 G(long long n) { return dim3(dec::nblocks(n, kBlock)); }
+
+template <class T>
+LevelVals<T> &vals(Level &L);
+template <>
+LevelVals<double> &vals<double>(Level &L) { return L.d; }
+template <>
+LevelVals<float> &vals<float>(Level &L) { return L.f; }
+
+// Colour-sorted ELL of a host CSR (columns ascending per row).
+void build_level_matrix(Level &L, const dec::HostCsr &A) {
+  int n = A.m;
+  L.n = n;
+  // symmetric adjacency for colouring
+  std::vector<int32_t> deg(n + 1, 0);
+  for (int r = 0; r < n; r++)
+    for (int k = A.ptr[r]; k < A.ptr[r + 1]; k++)
+      if (A.col[k] != r) {
+        deg[r + 1]++;
+        deg[A.col[k] + 1]++;
+      }
+  for (int r = 0; r < n; r++) deg[r + 1] += deg[r];
+  std::vector<int32_t> adj(deg[n]);
+  std::vector<int32_t> w(deg.begin(), deg.end() - 1);
+  for (int r = 0; r < n; r++)
+    for (int k = A.ptr[r]; k < A.ptr[r + 1]; k++)
+      if (A.col[k] != r) {
+        adj[w[r]++] = A.col[k];
+        adj[w[A.col[k]]++] = r;
+      }
+  std::vector<int32_t> color;
+  L.ncolors = dec::greedy_colors(n, deg, adj, color);
+  std::vector<int32_t> order(n);
+  L.cstart.assign(L.ncolors + 1, 0);
+  for (int v = 0; v < n; v++) L.cstart[color[v] + 1]++;
+  for (int c = 0; c < L.ncolors; c++) L.cstart[c + 1] += L.cstart[c];
+  std::vector<int32_t> pos(L.cstart.begin(), L.cstart.end() - 1);
+  for (int v = 0; v < n; v++) order[pos[color[v]]++] = v;
+  int width = 0;
+  for (int r = 0; r < n; r++) width = std::max(width, A.ptr[r + 1] - A.ptr[r]);
+  L.width = width;
+  size_t sz = static_cast<size_t>(n) * std::max(width, 1);
+  std::vector<int32_t> col(sz, -1);
+  std::vector<double> vd(sz, 0.0);
+  std::vector<float> vf(sz, 0.0f);
+  for (int p = 0; p < n; p++) {
+    int r = order[p];
+    for (int k = A.ptr[r]; k < A.ptr[r + 1]; k++) {
+      size_t q = static_cast<size_t>(k - A.ptr[r]) * n + p;
+      col[q] = A.col[k];
+      vd[q] = A.val[k];
+      vf[q] = static_cast<float>(A.val[k]);
+    }
+  }
+  L.col.upload(col);
+  L.rowid.upload(order);
+  L.d.val.upload(vd);
+  L.f.val.upload(vf);
+  L.A = A;
+}
+
+// make_restriction(transpose(S)) (Multigrid.jl:595-612) as rows of R, and the
+// endpoints of P = transpose(S) = subdivision_matrix(BinarySubdivision) (:191-219).
+void build_transfer(Level &fine, const dec::Topo &coarse) {
+  int nc = coarse.nv, ne = coarse.ne, nf = nc + ne;
+  std::vector<std::vector<int32_t>> mids(nc);
+  std::vector<int32_t> pa(ne), pb(ne);
+  for (int e = 0; e < ne; e++) {
+    int a = std::min(coarse.v0[e], coarse.v1[e]), b = std::max(coarse.v0[e], coarse.v1[e]);
+    pa[e] = a;
+    pb[e] = b;
+    mids[a].push_back(nc + e);
+    if (b != a) mids[b].push_back(nc + e);
+  }
+  dec::HostCsr R;
+  R.m = nc;
+  R.n = nf;
+  R.ptr.assign(nc + 1, 0);
+  for (int i = 0; i < nc; i++) {
+    double rs = 1.0;  // row sum in storage order: 1.0, then 0.5 per midpoint
+    for (size_t k = 0; k < mids[i].size(); k++) rs += 0.5;
+    R.col.push_back(i);
+    R.val.push_back(1.0 / rs);
+    for (int c : mids[i]) {
+      R.col.push_back(c);
+      R.val.push_back(0.5 / rs);
+    }
+    R.ptr[i + 1] = static_cast<int32_t>(R.col.size());
+  }
+  dec::HostCsr P;
+  P.m = nf;
+  P.n = nc;
+  P.ptr.assign(nf + 1, 0);
+  for (int j = 0; j < nf; j++) {
+    if (j < nc) {
+      P.col.push_back(j);
+      P.val.push_back(1.0);
+    } else {
+      P.col.push_back(pa[j - nc]);
+      P.val.push_back(0.5);
+      P.col.push_back(pb[j - nc]);
+      P.val.push_back(0.5);
+    }
+    P.ptr[j + 1] = static_cast<int32_t>(P.col.size());
+  }
+  int w = 0;
+  for (int i = 0; i < nc; i++) w = std::max(w, R.ptr[i + 1] - R.ptr[i]);
+  fine.nc = nc;
+  fine.rwidth = w;
+  std::vector<int32_t> col(static_cast<size_t>(nc) * w, -1);
+  std::vector<double> vd(col.size(), 0.0);
+  std::vector<float> vf(col.size(), 0.0f);
+  for (int i = 0; i < nc; i++)
+    for (int k = R.ptr[i]; k < R.ptr[i + 1]; k++) {
+      size_t q = static_cast<size_t>(k - R.ptr[i]) * nc + i;
+      col[q] = R.col[k];
+      vd[q] = R.val[k];
+      vf[q] = static_cast<float>(R.val[k]);
+    }
+  fine.rcol.upload(col);
+  fine.d.rval.upload(vd);
+  fine.f.rval.upload(vf);
+  fine.pa.upload(pa);
+  fine.pb.upload(pb);
+  fine.R = std::move(R);
+  fine.P = std::move(P);
+}
+
+void alloc_work(Level &L) {
+  for (auto *v : {&L.d.u, &L.d.b, &L.d.r, &L.d.t}) v->alloc(L.n);
+  for (auto *v : {&L.f.u, &L.f.b, &L.f.r, &L.f.t}) v->alloc(L.n);
+}
+
+dec_mesh_s *copy_mesh(const dec_mesh_s &m) {
+  auto *c = new dec_mesh_s();
+  c->h = m.h;
+  c->hp = m.hp;
+  c->to_dfs = m.to_dfs;
+  c->v0.upload(c->h.v0);
+  c->v1.upload(c->h.v1);
+  c->e0.upload(c->h.e0);
+  c->e1.upload(c->h.e1);
+  c->e2.upload(c->h.e2);
+  c->p.upload(c->hp);
+  return c;
+}
+
+dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge, int center, int steps) {
+  if (levels < 0 || levels > 11) fail(DEC_ERR_ARG, "dec_mg_create: levels must be in 0..11");
+  if (steps < 0) fail(DEC_ERR_ARG, "dec_mg_create: steps must be >= 0");
+  auto mg = std::make_unique<dec_mg_s>();
+  mg->steps = steps;
+  // coarse -> fine walk (Multigrid.jl:752-797), stored finest first
+  std::vector<std::unique_ptr<dec_mesh_s>> walk;
+  walk.emplace_back(copy_mesh(base));
+  for (int i = 0; i < levels; i++) walk.emplace_back(dec::subdivide(*walk.back(), project));
+  for (int i = levels; i >= 0; i--) mg->meshes.push_back(std::move(walk[i]));
+  int nl = levels + 1;
+  for (int l = 0; l < nl; l++) {
+    dec_mesh_s &m = *mg->meshes[l];
+    m.owned_by_mg = true;
+    dec::dualize(m, center);
+    auto L = std::make_unique<Level>();
+    std::vector<double> h0 = m.h0.download(), h1 = m.h1.download(), ih0(m.h.nv);
+    for (int v = 0; v < m.h.nv; v++) ih0[v] = 1 / h0[v];
+    dec::HostCsr A;
+    if (hodge == DEC_GEOMETRIC_HODGE) {
+      dec::HostCsr H;
+      dec::geo_hodge1(m, H);
+      dec::assemble_laplacian0(m.h, ih0, h1, &H, A);
+    } else {
+      dec::assemble_laplacian0(m.h, ih0, h1, nullptr, A);
+    }
+    build_level_matrix(*L, A);
+    alloc_work(*L);
+    mg->lv.push_back(std::move(L));
+  }
+  for (int l = 0; l + 1 < nl; l++) build_transfer(*mg->lv[l], mg->meshes[l + 1]->h);
+  // dense coarse operator for the direct solve
+  Level &C = *mg->lv.back();
+  int n = C.n;
+  if (static_cast<long long>(n) * n <= 64LL * 1024 * 1024) {
+    std::vector<double> Ad(static_cast<size_t>(n) * n, 0.0);
+    for (int r = 0; r < n; r++)
+      for (int k = C.A.ptr[r]; k < C.A.ptr[r + 1]; k++) Ad[static_cast<size_t>(r) * n + C.A.col[k]] += C.A.val[k];
+    dec::coarse_inverse(n, Ad, C.Zh);
+    C.d.Z.upload(C.Zh);
+    std::vector<float> zf(C.Zh.begin(), C.Zh.end());
+    C.f.Z.upload(zf);
+  }
+  mg->part.alloc(kDotBlocks);
+  mg->scal.alloc(8);
+  mg->cg.alloc(1);
+  int nf = mg->lv[0]->n;
+  mg->cgx.alloc(nf);
+  mg->cgp.alloc(nf);
+  mg->cgAp.alloc(nf);
+  dec::check_launch("dec_mg_create");
+  dec::dsync();
+  return mg.release();
+}
+
+// ---------------------------------------------------------------- cycles
+
+template <class T>
+void residual(Level &L, const T *b, const T *u, T *r, hipStream_t s) {
+  LevelVals<T> &V = vals<T>(L);
+  hipLaunchKernelGGL(k_residual<T>, G(L.n), dim3(kBlock), 0, s, L.n, L.width, L.col.p, V.val.p, L.rowid.p, b, u, r);
+}
+
+void dot(dec_mg_s &mg, int n, const double *x, const double *y, double *out, const int *done, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_partial<double>, dim3(kDotBlocks), dim3(kBlock), 0, s, n, x, y, mg.part.p, done);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, mg.part.p, out, done);
+}
+
+// cg(A, b, u; itmax = steps) as in Multigrid.jl:860 (float64 only).
+void cg_smooth(dec_mg_s &mg, Level &L, const double *b, double *u, int steps, hipStream_t s) {
+  int n = L.n;
+  CgState *st = mg.cg.p;
+  int *done = reinterpret_cast<int *>(reinterpret_cast<char *>(st) + offsetof(CgState, done));
+  double *dotp = reinterpret_cast<double *>(reinterpret_cast<char *>(st) + offsetof(CgState, dot));
+  double *x = mg.cgx.p, *p = mg.cgp.p, *Ap = mg.cgAp.p, *r = L.d.t.p;
+  residual<double>(L, b, u, r, s);
+  hipLaunchKernelGGL(k_cg_start, G(n), dim3(kBlock), 0, s, st, n, x, r, p);
+  dot(mg, n, r, r, dotp, nullptr, s);
+  hipLaunchKernelGGL(k_cg_init_scalars, dim3(1), dim3(1), 0, s, st);
+  for (int it = 0; it < steps; it++) {
+    hipLaunchKernelGGL(k_spmv_rows<double>, G(n), dim3(kBlock), 0, s, n, L.width, L.col.p, L.d.val.p, L.rowid.p, p,
+                       Ap, done);
+    dot(mg, n, p, Ap, dotp, done, s);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1), 0, s, st);
+    hipLaunchKernelGGL(k_cg_xr, G(n), dim3(kBlock), 0, s, st, n, x, p, r, Ap);
+    dot(mg, n, r, r, dotp, done, s);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1), 0, s, st);
+    hipLaunchKernelGGL(k_cg_p, G(n), dim3(kBlock), 0, s, st, n, p, r);
+  }
+  hipLaunchKernelGGL(k_axpy_final, G(n), dim3(kBlock), 0, s, n, x, u);
+}
+
+template <class T>
+void smooth(dec_mg_s &mg, Level &L, const T *b, T *u, bool post, hipStream_t s) {
+  int steps = mg.steps;
+  if (steps == 0) return;
+  LevelVals<T> &V = vals<T>(L);
+  if (mg.alg == DEC_SMOOTH_CG) {
+    if constexpr (sizeof(T) == 8) cg_smooth(mg, L, b, u, steps, s);
+    else fail(DEC_ERR_UNSUPPORTED, "the CG smoother runs in float64 only");
+    return;
+  }
+  for (int it = 0; it < steps; it++) {
+    if (mg.alg == DEC_SMOOTH_GS) {
+      for (int ci = 0; ci < L.ncolors; ci++) {
+        int c = post ? L.ncolors - 1 - ci : ci;
+        int p0 = L.cstart[c], p1 = L.cstart[c + 1];
+        if (p1 > p0)
+          hipLaunchKernelGGL(k_gs_color<T>, G(p1 - p0), dim3(kBlock), 0, s, p0, p1, L.n, L.width, L.col.p, V.val.p,
+                             L.rowid.p, b, u);
+      }
+    } else {
+      hipLaunchKernelGGL(k_jacobi<T>, G(L.n), dim3(kBlock), 0, s, L.n, L.width, L.col.p, V.val.p, L.rowid.p, b, u,
+                         V.t.p, static_cast<T>(mg.omega));
+      hipLaunchKernelGGL(k_copy<T>, G(L.n), dim3(kBlock), 0, s, L.n, V.t.p, u);
+    }
+  }
+}
+
+template <class T>
+void coarse_solve(dec_mg_s &mg, Level &L, const T *b, T *u, hipStream_t s) {
+  LevelVals<T> &V = vals<T>(L);
+  if (!V.Z.p) fail(DEC_ERR_UNSUPPORTED, "coarsest level too large for the dense direct solve");
+  residual<T>(L, b, u, V.r.p, s);
+  int rows_per_block = kBlock / 64;
+  hipLaunchKernelGGL(k_gemv_add<T>, dim3((L.n + rows_per_block - 1) / rows_per_block), dim3(kBlock), 0, s, L.n,
+                     V.Z.p, V.r.p, u);
+  (void)mg;
+}
+
+// _multigrid_μ_cycle (Multigrid.jl:857-872).
+template <class T>
+void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, hipStream_t s) {
+  int nl = static_cast<int>(mg.lv.size());
+  Level &L = *mg.lv[lvl];
+  if (lvl == nl - 1 && mg.coarse_direct) {
+    coarse_solve<T>(mg, L, b, u, s);
+    return;
+  }
+  smooth<T>(mg, L, b, u, false, s);
+  if (lvl >= nl - 1) return;
+  Level &C = *mg.lv[lvl + 1];
+  LevelVals<T> &V = vals<T>(L);
+  LevelVals<T> &CV = vals<T>(C);
+  residual<T>(L, b, u, V.r.p, s);
+  hipLaunchKernelGGL(k_restrict<T>, G(L.nc), dim3(kBlock), 0, s, L.nc, L.rwidth, L.rcol.p, V.rval.p, V.r.p, CV.b.p);
+  hipLaunchKernelGGL(k_zero<T>, G(C.n), dim3(kBlock), 0, s, C.n, CV.u.p);
+  mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu, s);
+  if (mu > 1) mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu - 1, s);
+  hipLaunchKernelGGL(k_prolong_add<T>, G(L.n), dim3(kBlock), 0, s, L.n, L.nc, L.pa.p, L.pb.p, CV.u.p, u);
+  smooth<T>(mg, L, b, u, true, s);
+}
+
+// _full_multigrid (Multigrid.jl:839-855).
+void fmg_rec(dec_mg_s &mg, int lvl, const double *b, double *z, int mu, int cycles, hipStream_t s) {
+  int nl = static_cast<int>(mg.lv.size());
+  Level &L = *mg.lv[lvl];
+  hipLaunchKernelGGL(k_zero<double>, G(L.n), dim3(kBlock), 0, s, L.n, z);
+  if (lvl < nl - 1) {
+    Level &C = *mg.lv[lvl + 1];
+    hipLaunchKernelGGL(k_restrict<double>, G(L.nc), dim3(kBlock), 0, s, L.nc, L.rwidth, L.rcol.p, L.d.rval.p, b,
+                       C.d.b.p);
+    fmg_rec(mg, lvl + 1, C.d.b.p, C.d.u.p, mu, cycles, s);
+    hipLaunchKernelGGL(k_prolong_add<double>, G(L.n), dim3(kBlock), 0, s, L.n, L.nc, L.pa.p, L.pb.p, C.d.u.p, z);
+  }
+  if (lvl > 0)
+    for (int c = 0; c < cycles; c++) mu_cycle<double>(mg, lvl, z, b, mu, s);
+}
+
+double norm2(dec_mg_s &mg, int n, const double *x, hipStream_t s) {
+  dot(mg, n, x, x, mg.scal.p, nullptr, s);
+  double v = 0.0;
+  if (hipMemcpyAsync(&v, mg.scal.p, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    fail(DEC_ERR_HIP, "norm: copy failed");
+  return std::sqrt(v);
+}
+
+void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u, const double *b, int *done_cycles,
+           double *relres, hipStream_t s) {
+  Level &F = *mg.lv[0];
+  int n = F.n;
+  double bn = norm2(mg, n, b, s);
+  if (bn == 0.0) {
+    hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, u);
+    *done_cycles = 0;
+    *relres = 0.0;
+    return;
+  }
+  double rr = 0.0;
+  int c = 0;
+  for (;;) {
+    residual<double>(F, b, u, F.d.r.p, s);
+    rr = norm2(mg, n, F.d.r.p, s) / bn;
+    if (rr <= tol || c >= max_cycles) break;
+    if (variant == DEC_SOLVE_PLAIN_F64) {
+      mu_cycle<double>(mg, 0, u, b, 1, s);
+    } else {
+      // float32 V-cycle on the float64 residual, correction added in float64
+      hipLaunchKernelGGL(k_d2f, G(n), dim3(kBlock), 0, s, n, F.d.r.p, F.f.b.p);
+      hipLaunchKernelGGL(k_zero<float>, G(n), dim3(kBlock), 0, s, n, F.f.u.p);
+      mu_cycle<float>(mg, 0, F.f.u.p, F.f.b.p, 1, s);
+      hipLaunchKernelGGL(k_add_f2d, G(n), dim3(kBlock), 0, s, n, F.f.u.p, u);
+    }
+    c++;
+  }
+  *done_cycles = c;
+  *relres = rr;
+  if (rr > tol) fail(DEC_ERR_NOCONV, "dec_mg_solve: relative residual " + std::to_string(rr) + " above tolerance after " +
+                                         std::to_string(c) + " cycles");
+}
+
+}  // namespace
+
+extern "C" {
+
+int dec_mg_create(dec_mesh base, int32_t levels, int32_t project, int32_t hodge, int32_t center, int32_t steps,
+                  dec_mg *out) {
+  DEC_TRY
+  if (!base || !out) fail(DEC_ERR_ARG, "dec_mg_create: NULL handle");
+  *out = nullptr;
+  *out = mg_create(*base, levels, project != 0, hodge, center, steps);
+  DEC_CATCH
+}
+
+int dec_mg_nlevels(dec_mg mg, int32_t *nlevels) {
+  DEC_TRY
+  if (!mg || !nlevels) fail(DEC_ERR_ARG, "dec_mg_nlevels: NULL argument");
+  *nlevels = static_cast<int32_t>(mg->lv.size());
+  DEC_CATCH
+}
+
+int dec_mg_level_size(dec_mg mg, int32_t level, int32_t *n) {
+  DEC_TRY
+  if (!mg || !n) fail(DEC_ERR_ARG, "dec_mg_level_size: NULL argument");
+  if (level < 0 || level >= static_cast<int>(mg->lv.size())) fail(DEC_ERR_ARG, "dec_mg_level_size: bad level");
+  *n = mg->lv[level]->n;
+  DEC_CATCH
+}
+
+int dec_mg_finest_mesh(dec_mg mg, dec_mesh *m) {
+  DEC_TRY
+  if (!mg || !m) fail(DEC_ERR_ARG, "dec_mg_finest_mesh: NULL argument");
+  *m = mg->meshes[0].get();
+  DEC_CATCH
+}
+
+int dec_mg_export_level(dec_mg mg, int32_t level, int32_t which, int64_t *nnz, int32_t *rowptr, int32_t *col,
+                        double *val) {
+  DEC_TRY
+  if (!mg) fail(DEC_ERR_ARG, "dec_mg_export_level: NULL handle");
+  int nl = static_cast<int>(mg->lv.size());
+  if (level < 0 || level >= nl) fail(DEC_ERR_ARG, "dec_mg_export_level: bad level");
+  if (which != 0 && level == nl - 1) fail(DEC_ERR_ARG, "dec_mg_export_level: coarsest level has no transfer");
+  const dec::HostCsr &A = which == 0 ? mg->lv[level]->A : (which == 1 ? mg->lv[level]->R : mg->lv[level]->P);
+  if (which < 0 || which > 2) fail(DEC_ERR_ARG, "dec_mg_export_level: which must be 0 (A), 1 (R) or 2 (P)");
+  if (nnz) *nnz = static_cast<int64_t>(A.col.size());
+  if (rowptr) std::copy(A.ptr.begin(), A.ptr.end(), rowptr);
+  if (col) std::copy(A.col.begin(), A.col.end(), col);
+  if (val) std::copy(A.val.begin(), A.val.end(), val);
+  DEC_CATCH
+}
+
+int dec_mg_set_smoother(dec_mg mg, int32_t alg, int32_t steps, double omega, int32_t coarse_direct) {
+  DEC_TRY
+  if (!mg) fail(DEC_ERR_ARG, "dec_mg_set_smoother: NULL handle");
+  if (alg < DEC_SMOOTH_CG || alg > DEC_SMOOTH_JACOBI) fail(DEC_ERR_ARG, "dec_mg_set_smoother: unknown smoother");
+  if (steps < 0) fail(DEC_ERR_ARG, "dec_mg_set_smoother: steps must be >= 0");
+  if (coarse_direct && !mg->lv.back()->d.Z.p)
+    fail(DEC_ERR_UNSUPPORTED, "dec_mg_set_smoother: coarsest level too large for the dense direct solve");
+  mg->alg = alg;
+  mg->steps = steps;
+  mg->omega = omega;
+  mg->coarse_direct = coarse_direct != 0;
+  DEC_CATCH
+}
+
+int dec_mg_cycles(dec_mg mg, int32_t mu, int32_t cycles, void *u, const void *b, void *stream) {
+  DEC_TRY
+  if (!mg || !u || !b) fail(DEC_ERR_ARG, "dec_mg_cycles: NULL argument");
+  if (mu < 1 || cycles < 0) fail(DEC_ERR_ARG, "dec_mg_cycles: mu >= 1 and cycles >= 0 required");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (int c = 0; c < cycles; c++)
+    mu_cycle<double>(*mg, 0, static_cast<double *>(u), static_cast<const double *>(b), mu, s);
+  dec::check_launch("dec_mg_cycles");
+  DEC_CATCH
+}
+
+int dec_mg_full_multigrid(dec_mg mg, int32_t mu, int32_t cycles, const void *b, void *u, void *stream) {
+  DEC_TRY
+  if (!mg || !u || !b) fail(DEC_ERR_ARG, "dec_mg_full_multigrid: NULL argument");
+  if (mu < 1 || cycles < 0) fail(DEC_ERR_ARG, "dec_mg_full_multigrid: mu >= 1 and cycles >= 0 required");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const double *bb = static_cast<const double *>(b);
+  double *uu = static_cast<double *>(u);
+  fmg_rec(*mg, 0, bb, uu, mu, cycles, s);
+  for (int c = 0; c < cycles; c++) mu_cycle<double>(*mg, 0, uu, bb, mu, s);
+  dec::check_launch("dec_mg_full_multigrid");
+  DEC_CATCH
+}
+
+int dec_mg_solve(dec_mg mg, int32_t variant, double tol, int32_t max_cycles, void *u, const void *b,
+                 int32_t *cycles_done, double *relres, void *stream) {
+  DEC_TRY
+  if (!mg || !u || !b) fail(DEC_ERR_ARG, "dec_mg_solve: NULL argument");
+  if (variant != DEC_SOLVE_PLAIN_F64 && variant != DEC_SOLVE_MIXED_IR) fail(DEC_ERR_ARG, "dec_mg_solve: bad variant");
+  if (variant == DEC_SOLVE_MIXED_IR && mg->alg == DEC_SMOOTH_CG)
+    fail(DEC_ERR_UNSUPPORTED, "dec_mg_solve: the mixed variant needs the gauss_seidel or jacobi smoother");
+  int c = 0;
+  double rr = 0.0;
+  try {
+    solve(*mg, variant, tol, max_cycles, static_cast<double *>(u), static_cast<const double *>(b), &c, &rr,
+          static_cast<hipStream_t>(stream));
+  } catch (...) {
+    if (cycles_done) *cycles_done = c;
+    if (relres) *relres = rr;
+    throw;
+  }
+  if (cycles_done) *cycles_done = c;
+  if (relres) *relres = rr;
+  DEC_CATCH
+}
+
+int dec_mg_destroy(dec_mg mg) {
+  DEC_TRY
+  if (mg)
+    for (auto &m : mg->meshes) m->owned_by_mg = false;
+  delete mg;
+  DEC_CATCH
+}
+
+}  // extern "C"

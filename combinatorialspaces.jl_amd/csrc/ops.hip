@@ -1,0 +1,944 @@
+// FastDEC operators of libdec_hip.so (src/FastDEC.jl): wedge products, exterior
+// derivative / boundary / dual derivative, Hodge stars and inverses, the 0-form
+// Laplace-Beltrami operator and avg₀₁, as gather kernels over int32 SoA
+// incidence tables.  Coefficient tables are computed on the GPU in float64
+// and cast once to the operator's dtype (cache_wedge's cast_float).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <utility>
+
+#include "dec_internal.h"
+#include "kernels_common.h"
+
+using dec::DBuf;
+using dec::fail;
+
+namespace {
+
+enum Impl {
+  I_W01,   // wedge 0-1
+  I_W02,   // wedge 0-2
+  I_W11,   // wedge 1-1
+  I_WDD,   // dual-dual wedge 0-1 / 1-0
+  I_D0,    // d0
+  I_D1,    // d1
+  I_SELL,  // ±1 ELL: ∂1 = d0ᵀ, d̃1 = -d0ᵀ, ∂2 = d̃0 = d1ᵀ
+  I_DIAG,  // diagonal ⋆, ⋆⁻¹
+  I_ELL,   // general sparse: geometric ⋆1, ∇²
+  I_AVG    // avg₀₁
+};
+
+}  // namespace
+
+struct dec_op_s {
+  int kind = 0, m = 0, n = 0, hodge = 0, dtype = 0;
+  int impl = 0;
+  bool swap = false;  // apply(x, y) evaluates the kernel on (y, x)
+  int64_t rows = 0, cols = 0, cols2 = 0;
+  // index tables
+  DBuf<int32_t> i0, i1, i2;
+  DBuf<int8_t> s8;
+  // coefficient tables, stored as raw bytes of the op dtype
+  DBuf<unsigned char> c0, c1, c2;
+  // sparse forms
+  dec::Ell<int8_t> sell;
+  dec::Ell<double> ell64;
+  dec::Ell<float> ell32;
+  dec::HostCsr csr;  // host copy for export (sparse-matrix operators)
+};
+
+namespace {
+
+template <class T>
+__global__ void k_wedge01(int ne, const int *__restrict__ v0, const int *__restrict__ v1,
+                          const T *__restrict__ f, const T *__restrict__ a, T *__restrict__ res) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  // wedge_kernel_01! (FastDEC.jl:159-163): half * α[i] * (f[p1] + f[p2])
+  res[e] = (static_cast<T>(0.5) * a[e]) * (f[v0[e]] + f[v1[e]]);
+}
+
+template <class T>
+__global__ void k_wedge02(int nt, const int *__restrict__ i0, const int *__restrict__ i1,
+                          const int *__restrict__ i2, const T *__restrict__ c0, const T *__restrict__ c1,
+                          const T *__restrict__ c2, const T *__restrict__ f, const T *__restrict__ a,
+                          T *__restrict__ res) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  // wedge_kernel_02! (FastDEC.jl:165-170) with the six coefficients summed
+  // pairwise: the six vertex slots are (v0, v0, v1, v1, v2, v2).
+  T s = c0[t] * f[i0[t]] + c1[t] * f[i1[t]];
+  s = s + c2[t] * f[i2[t]];
+  res[t] = a[t] * s;
+}
+
+template <class T>
+__global__ void k_wedge11(int nt, const int *__restrict__ i0, const int *__restrict__ i1,
+                          const int *__restrict__ i2, const T *__restrict__ c0, const T *__restrict__ c1,
+                          const T *__restrict__ c2, const T *__restrict__ a, const T *__restrict__ b,
+                          T *__restrict__ res) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  // wedge_kernel_11! (FastDEC.jl:184-191)
+  int e0 = i0[t], e1 = i1[t], e2 = i2[t];
+  T ae0 = a[e0], ae1 = a[e1], ae2 = a[e2];
+  T be0 = b[e0], be1 = b[e1], be2 = b[e2];
+  T x = c0[t] * (ae2 * be1 - ae1 * be2);
+  T y = c1[t] * (ae2 * be0 - ae0 * be2);
+  T z = c2[t] * (ae1 * be0 - ae0 * be1);
+  res[t] = (x + y) + z;
+}
+
+template <class T>
+__global__ void k_wedge_dd(int ne, const int *__restrict__ ta, const int *__restrict__ tb,
+                           const T *__restrict__ wa, const T *__restrict__ wb, const T *__restrict__ f,
+                           const T *__restrict__ g, T *__restrict__ res) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  // (m * f)[e] .* g[e], m = wedge_dd_01_mat (FastDEC.jl:293-305); the row's
+  // triangles are ascending (ta < tb, tb < 0 when e has one coface).
+  T s = static_cast<T>(0);
+  int a = ta[e], b = tb[e];
+  if (a >= 0) s += wa[e] * f[a];
+  if (b >= 0) s += wb[e] * f[b];
+  res[e] = s * g[e];
+}
+
+template <class T>
+__global__ void k_d0(int ne, const int *__restrict__ v0, const int *__restrict__ v1,
+                     const int8_t *__restrict__ sgn, const T *__restrict__ x, T *__restrict__ res) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  // row e of d0 (FastDEC.jl:471-492): σ at ∂v0, -σ at ∂v1
+  T d = x[v0[e]] - x[v1[e]];
+  res[e] = sgn[e] > 0 ? d : -d;
+}
+
+template <class T>
+__global__ void k_d1(int nt, const int *__restrict__ ia, const int *__restrict__ ib, const int *__restrict__ ic,
+                     const int8_t *__restrict__ sg, const T *__restrict__ x, T *__restrict__ res) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  // row t of d1 (FastDEC.jl:496-524), its three edges in ascending order with
+  // signs packed in bits 0..2 (set = +1)
+  int s = sg[t];
+  T xa = x[ia[t]], xb = x[ib[t]], xc = x[ic[t]];
+  T r = (s & 1) ? xa : -xa;
+  r = r + ((s & 2) ? xb : -xb);
+  r = r + ((s & 4) ? xc : -xc);
+  res[t] = r;
+}
+
+template <class T>
+__global__ void k_sign_ell(int n, int width, const int *__restrict__ col, const int8_t *__restrict__ val,
+                           const T *__restrict__ x, T *__restrict__ y) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  T acc = static_cast<T>(0);
+  for (int k = 0; k < width; k++) {
+    int c = col[static_cast<size_t>(k) * n + r];
+    if (c < 0) break;
+    T v = x[c];
+    acc = acc + (val[static_cast<size_t>(k) * n + r] > 0 ? v : -v);
+  }
+  y[r] = acc;
+}
+
+template <class T>
+__global__ void k_ell(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
+                      const T *__restrict__ x, T *__restrict__ y) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  T acc = static_cast<T>(0);
+  for (int k = 0; k < width; k++) {
+    size_t q = static_cast<size_t>(k) * n + r;
+    int c = col[q];
+    if (c < 0) break;
+    acc = acc + val[q] * x[c];
+  }
+  y[r] = acc;
+}
+
+template <class T>
+__global__ void k_diag(int n, const T *__restrict__ h, const T *__restrict__ x, T *__restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = h[i] * x[i];
+}
+
+template <class T>
+__global__ void k_avg01(int ne, const int *__restrict__ v0, const int *__restrict__ v1, const T *__restrict__ x,
+                        T *__restrict__ y) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  y[e] = static_cast<T>(0.5) * x[v0[e]] + static_cast<T>(0.5) * x[v1[e]];
+}
+
+// ---- coefficient precomputation (float64) ----
+
+// wedge_kernel_coeffs(Val(0), Val(2)) (FastDEC.jl:50-62): dual_area / area for
+// the six dual triangles of t, summed pairwise.
+template <class T>
+__global__ void k_coeff02(int nt, const double *__restrict__ da, const double *__restrict__ area,
+                          T *__restrict__ c0, T *__restrict__ c1, T *__restrict__ c2) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  double A = area[t];
+  double k[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) k[j] = da[static_cast<size_t>(j) * nt + t] / A;
+  c0[t] = static_cast<T>(k[0] + k[1]);
+  c1[t] = static_cast<T>(k[2] + k[3]);
+  c2[t] = static_cast<T>(k[4] + k[5]);
+}
+
+// wedge_kernel_coeffs(Val(1), Val(1)) (FastDEC.jl:78-93).
+template <class T>
+__global__ void k_coeff11(int nt, const int *__restrict__ e0, const int *__restrict__ e1,
+                          const int *__restrict__ e2, const int8_t *__restrict__ eo,
+                          const int8_t *__restrict__ to, const double *__restrict__ da,
+                          const double *__restrict__ area, T *__restrict__ c0, T *__restrict__ c1,
+                          T *__restrict__ c2) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  double ar = ((to[t] ? 1.0 : -1.0) * 2) * area[t];
+  double s0 = eo[e0[t]] ? 1.0 : -1.0, s1 = eo[e1[t]] ? 1.0 : -1.0, s2 = eo[e2[t]] ? 1.0 : -1.0;
+  size_t n = static_cast<size_t>(nt);
+  c0[t] = static_cast<T>(((s1 * s2) * (da[t] + da[t + n])) / ar);
+  c1[t] = static_cast<T>(((s0 * s2) * (da[t + 2 * n] + da[t + 3 * n])) / ar);
+  c2[t] = static_cast<T>(((s0 * s1) * (da[t + 4 * n] + da[t + 5 * n])) / ar);
+}
+
+// wedge_dd_01_mat weights: dual_length / Σ dual_length over e's elementary duals.
+template <class T>
+__global__ void k_coeff_dd(int ne, const int *__restrict__ ptr, const int *__restrict__ idx,
+                           const double *__restrict__ dlB, T *__restrict__ wa, T *__restrict__ wb) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int a = ptr[e], cnt = ptr[e + 1] - a;
+  double la = cnt > 0 ? dlB[idx[a]] : 0.0, lb = cnt > 1 ? dlB[idx[a + 1]] : 0.0;
+  double s = cnt > 1 ? la + lb : la;
+  // weights are listed in ascending triangle order (the CSC row order)
+  wa[e] = static_cast<T>(cnt > 0 ? la / s : 0.0);
+  wb[e] = static_cast<T>(cnt > 1 ? lb / s : 0.0);
+}
+
+template <class T>
+__global__ void k_cast(size_t n, const double *__restrict__ x, T *__restrict__ y) {
+  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = static_cast<T>(x[i]);
+}
+
+template <class T>
+__global__ void k_inv_diag(int n, const double *__restrict__ h, double mult, T *__restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // dec_inv_hodge_star: (1 ./ hdg) .* (-1)^(k(2-k)) (FastDEC.jl:725-730)
+  if (i < n) y[i] = static_cast<T>((1 / h[i]) * mult);
+}
+
+// Geometric ⋆1, per triangle (FastDEC.jl:660-716): nine candidate entries
+// (three diagonal, six off-diagonal in the reference's permutation order);
+// an entry equal to 0.0 is not stored by the reference and is dropped here.
+struct D3 {
+  double x, y, z;
+};
+__device__ __forceinline__ D3 sub3(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 cross3(D3 a, D3 b) {
+  return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double dot3(D3 a, D3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ double sgnd(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); }
+// crossdot (DiscreteExteriorCalculus.jl:1667-1670)
+__device__ __forceinline__ double crossdot(D3 a, D3 b) {
+  D3 c = cross3(a, b);
+  return sqrt(dot3(c, c)) * (c.z == 0 ? 1.0 : sgnd(c.z));
+}
+__device__ __forceinline__ D3 ld(const double *p, size_t i) { return D3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+
+__global__ void k_geo_hodge1_tri(int nv, int ne, int nt, const int *__restrict__ v0, const int *__restrict__ v1,
+                                 const int *__restrict__ e0, const int *__restrict__ e1,
+                                 const int *__restrict__ e2, const double *__restrict__ dpt,
+                                 double *__restrict__ out9) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  int te[3] = {e2[t], e1[t], e0[t]};  // reversed by construction
+  D3 tc = ld(dpt, static_cast<size_t>(nv) + ne + t);
+  D3 evt[3], dvt[3];
+  for (int i = 0; i < 3; i++) {
+    int e = te[i];
+    evt[i] = sub3(ld(dpt, v0[e]), ld(dpt, v1[e]));
+    dvt[i] = sub3(tc, ld(dpt, static_cast<size_t>(nv) + e));
+  }
+  dvt[1] = D3{-1 * dvt[1].x, -1 * dvt[1].y, -1 * dvt[1].z};
+  D3 cr = cross3(evt[0], dvt[0]);
+  double rel = cr.z == 0 ? 1.0 : sgnd(cr.z);
+  double *o = out9 + static_cast<size_t>(9) * t;
+  for (int i = 0; i < 3; i++) {
+    double dc = crossdot(evt[i], dvt[i]);
+    o[i] = dc != 0.0 ? rel * dc / dot3(evt[i], evt[i]) : 0.0;
+  }
+  const int P[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  for (int q = 0; q < 6; q++) {
+    int a = P[q][0], b = P[q][1], c = P[q][2];
+    double ddot = dot3(evt[a], dvt[a]) / dot3(evt[a], evt[a]);
+    double val = ddot * dot3(evt[a], evt[c]);
+    o[3 + q] = val != 0.0 ? rel * val / crossdot(evt[b], evt[c]) : 0.0;
+  }
+}
+
+template <class T>
+void launch_cast(const DBuf<double> &src, DBuf<unsigned char> &dst) {
+  dst.alloc(sizeof(T) * src.n);
+  if (src.n)
+    hipLaunchKernelGGL(k_cast<T>, dim3(dec::nblocks(static_cast<long long>(src.n))), dim3(256), 0, 0, src.n, src.p,
+                       reinterpret_cast<T *>(dst.p));
+}
+
+}  // namespace
+
+namespace dec {
+
+// Per-row restatement of the SpGEMM chain ((((-1 * D(inv_h0)) * d̃1) * H) * d0)
+// in Gustavson order:
+//   A[v,e] = inv_h0[v] * d0[e,v]              (exact: d0 = ±1)
+//   X[v,f] = Σ_{e ∋ v, ascending} A[v,e] * H[e,f]
+//   L[v,w] = Σ_{f ∋ w, ascending} X[v,f] * d0[f,w]
+void assemble_laplacian0(const Topo &t, const std::vector<double> &inv_h0, const std::vector<double> &h1,
+                         const HostCsr *H, HostCsr &L) {
+  Csr ve;
+  vertex_edges(t, ve);
+  int nv = t.nv;
+  L.m = L.n = nv;
+  L.ptr.assign(nv + 1, 0);
+  L.col.clear();
+  L.val.clear();
+  std::vector<std::pair<int, double>> X;   // (f, value), f in first-touch order
+  std::vector<std::pair<int, double>> C;   // (w, value)
+  std::vector<int> xpos;
+  for (int v = 0; v < nv; v++) {
+    X.clear();
+    C.clear();
+    double iv = inv_h0[v];
+    for (int k = ve.ptr[v]; k < ve.ptr[v + 1]; k++) {
+      int e = ve.idx[k];
+      double a = iv * static_cast<double>(ve.val[k]);
+      auto add = [&](int f, double hval) {
+        double prod = a * hval;
+        for (auto &x : X)
+          if (x.first == f) {
+            x.second += prod;
+            return;
+          }
+        X.emplace_back(f, prod);
+      };
+      if (H) {
+        for (int q = H->ptr[e]; q < H->ptr[e + 1]; q++) add(H->col[q], H->val[q]);
+      } else {
+        add(e, h1[e]);
+      }
+    }
+    std::sort(X.begin(), X.end(), [](const auto &p, const auto &q) { return p.first < q.first; });
+    for (auto &x : X) {
+      int f = x.first;
+      double s = t.eo[f] == 1 ? 1.0 : -1.0;
+      int ws[2] = {t.v0[f], t.v1[f]};
+      double ds[2] = {s, -s};
+      for (int j = 0; j < 2; j++) {
+        int w = ws[j];
+        double prod = x.second * ds[j];
+        bool found = false;
+        for (auto &c : C)
+          if (c.first == w) {
+            c.second += prod;
+            found = true;
+            break;
+          }
+        if (!found) C.emplace_back(w, prod);
+      }
+    }
+    std::sort(C.begin(), C.end(), [](const auto &p, const auto &q) { return p.first < q.first; });
+    for (auto &c : C) {
+      L.col.push_back(c.first);
+      L.val.push_back(c.second);
+    }
+    L.ptr[v + 1] = static_cast<int32_t>(L.col.size());
+  }
+}
+
+template <class V>
+void upload_ell(const HostCsr &A, Ell<V> &E) {
+  int n = A.m, w = 0;
+  for (int r = 0; r < n; r++) w = std::max(w, A.ptr[r + 1] - A.ptr[r]);
+  E.nrows = n;
+  E.width = w;
+  std::vector<int32_t> col(static_cast<size_t>(n) * std::max(w, 1), -1);
+  std::vector<V> val(static_cast<size_t>(n) * std::max(w, 1), static_cast<V>(0));
+  for (int r = 0; r < n; r++)
+    for (int k = A.ptr[r]; k < A.ptr[r + 1]; k++) {
+      size_t q = static_cast<size_t>(k - A.ptr[r]) * n + r;
+      col[q] = A.col[k];
+      val[q] = static_cast<V>(A.val[k]);
+    }
+  E.col.upload(col);
+  E.val.upload(val);
+}
+template void upload_ell<double>(const HostCsr &, Ell<double> &);
+template void upload_ell<float>(const HostCsr &, Ell<float> &);
+
+template <class T>
+void ell_spmv(const Ell<T> &E, const T *x, T *y, hipStream_t s) {
+  if (E.nrows == 0) return;
+  hipLaunchKernelGGL(k_ell<T>, dim3(nblocks(E.nrows)), dim3(256), 0, s, E.nrows, E.width, E.col.p, E.val.p, x, y);
+}
+template void ell_spmv<double>(const Ell<double> &, const double *, double *, hipStream_t);
+template void ell_spmv<float>(const Ell<float> &, const float *, float *, hipStream_t);
+
+// Host CSR of the geometric ⋆1 from the per-triangle entries: rows ascending
+// columns, duplicates (diagonals shared by two triangles) summed in the
+// reference's input order (triangle, then slot), zero entries dropped.
+void geo_hodge1_csr(const Topo &t, const std::vector<double> &v9, HostCsr &H) {
+  int ne = t.ne, nt = t.nt;
+  struct Ent {
+    int row, col;
+    long long order;
+    double val;
+  };
+  std::vector<Ent> ents;
+  ents.reserve(static_cast<size_t>(9) * nt);
+  const int P[6][2] = {{0, 1}, {0, 2}, {1, 0}, {1, 2}, {2, 0}, {2, 1}};
+  for (int i = 0; i < nt; i++) {
+    int te[3] = {t.e2[i], t.e1[i], t.e0[i]};
+    for (int a = 0; a < 3; a++) {
+      double v = v9[static_cast<size_t>(9) * i + a];
+      if (v != 0.0) ents.push_back({te[a], te[a], 9LL * i + a, v});
+    }
+    for (int q = 0; q < 6; q++) {
+      double v = v9[static_cast<size_t>(9) * i + 3 + q];
+      if (v != 0.0) ents.push_back({te[P[q][0]], te[P[q][1]], 9LL * i + 3 + q, v});
+    }
+  }
+  std::sort(ents.begin(), ents.end(), [](const Ent &a, const Ent &b) {
+    if (a.row != b.row) return a.row < b.row;
+    if (a.col != b.col) return a.col < b.col;
+    return a.order < b.order;
+  });
+  H.m = H.n = ne;
+  H.ptr.assign(ne + 1, 0);
+  H.col.clear();
+  H.val.clear();
+  for (size_t k = 0; k < ents.size(); k++) {
+    if (k > 0 && ents[k].row == ents[k - 1].row && ents[k].col == ents[k - 1].col) {
+      H.val.back() += ents[k].val;
+      continue;
+    }
+    H.col.push_back(ents[k].col);
+    H.val.push_back(ents[k].val);
+    H.ptr[ents[k].row + 1] = static_cast<int32_t>(H.col.size());
+  }
+  for (int r = 0; r < ne; r++) H.ptr[r + 1] = std::max(H.ptr[r + 1], H.ptr[r]);
+}
+
+void geo_hodge1(dec_mesh_s &m, HostCsr &H) {
+  const Topo &t = m.h;
+  DBuf<double> v9;
+  v9.alloc(static_cast<size_t>(9) * t.nt);
+  if (t.nt > 0)
+    hipLaunchKernelGGL(k_geo_hodge1_tri, dim3(nblocks(t.nt)), dim3(256), 0, 0, t.nv, t.ne, t.nt, m.v0.p, m.v1.p,
+                       m.e0.p, m.e1.p, m.e2.p, m.dpt.p, v9.p);
+  check_launch("geometric hodge");
+  geo_hodge1_csr(t, v9.download(), H);
+}
+
+}  // namespace dec
+
+namespace {
+
+void to_sign_ell(const dec::Csr &c, dec::Ell<int8_t> &E, bool negate) {
+  int n = c.n, w = 0;
+  for (int r = 0; r < n; r++) w = std::max(w, c.ptr[r + 1] - c.ptr[r]);
+  E.nrows = n;
+  E.width = w;
+  std::vector<int32_t> col(static_cast<size_t>(n) * std::max(w, 1), -1);
+  std::vector<int8_t> val(static_cast<size_t>(n) * std::max(w, 1), 0);
+  for (int r = 0; r < n; r++)
+    for (int k = c.ptr[r]; k < c.ptr[r + 1]; k++) {
+      size_t q = static_cast<size_t>(k - c.ptr[r]) * n + r;
+      col[q] = c.idx[k];
+      val[q] = static_cast<int8_t>(negate ? -c.val[k] : c.val[k]);
+    }
+  E.col.upload(col);
+  E.val.upload(val);
+}
+
+void csr_from_signs(const dec::Csr &c, int ncols, bool negate, dec::HostCsr &h) {
+  h.m = c.n;
+  h.n = ncols;
+  h.ptr = c.ptr;
+  h.col = c.idx;
+  h.val.resize(c.val.size());
+  for (size_t k = 0; k < c.val.size(); k++) h.val[k] = negate ? -c.val[k] : c.val[k];
+}
+
+void csr_d0(const dec::Topo &t, dec::HostCsr &h) {
+  h.m = t.ne;
+  h.n = t.nv;
+  h.ptr.assign(t.ne + 1, 0);
+  h.col.clear();
+  h.val.clear();
+  for (int e = 0; e < t.ne; e++) {
+    double s = t.eo[e] == 1 ? 1.0 : -1.0;
+    int a = t.v0[e], b = t.v1[e];
+    if (a < b) {
+      h.col.insert(h.col.end(), {a, b});
+      h.val.insert(h.val.end(), {s, -s});
+    } else {
+      h.col.insert(h.col.end(), {b, a});
+      h.val.insert(h.val.end(), {-s, s});
+    }
+    h.ptr[e + 1] = static_cast<int32_t>(h.col.size());
+  }
+}
+
+void csr_d1(const dec::Topo &t, dec::HostCsr &h) {
+  h.m = t.nt;
+  h.n = t.ne;
+  h.ptr.assign(t.nt + 1, 0);
+  h.col.clear();
+  h.val.clear();
+  for (int i = 0; i < t.nt; i++) {
+    double ts = t.to[i] == 1 ? 1.0 : -1.0;
+    std::pair<int, double> ent[3] = {{t.e0[i], (t.eo[t.e0[i]] == 1 ? 1.0 : -1.0) * ts},
+                                     {t.e1[i], -1 * (t.eo[t.e1[i]] == 1 ? 1.0 : -1.0) * ts},
+                                     {t.e2[i], (t.eo[t.e2[i]] == 1 ? 1.0 : -1.0) * ts}};
+    std::sort(ent, ent + 3, [](auto &a, auto &b) { return a.first < b.first; });
+    for (auto &x : ent) {
+      h.col.push_back(x.first);
+      h.val.push_back(x.second);
+    }
+    h.ptr[i + 1] = static_cast<int32_t>(h.col.size());
+  }
+}
+
+void csr_diag(const std::vector<double> &d, dec::HostCsr &h) {
+  int n = static_cast<int>(d.size());
+  h.m = h.n = n;
+  h.ptr.resize(n + 1);
+  h.col.resize(n);
+  h.val = d;
+  for (int i = 0; i <= n; i++) h.ptr[i] = i;
+  for (int i = 0; i < n; i++) h.col[i] = i;
+}
+
+template <class T>
+void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream_t s) {
+  const T *x = static_cast<const T *>(op.swap ? yv : xv);
+  const T *y = static_cast<const T *>(op.swap ? xv : yv);
+  T *out = static_cast<T *>(outv);
+  int rows = static_cast<int>(op.rows);
+  if (rows == 0) return;
+  dim3 g(dec::nblocks(rows)), b(256);
+  auto C = [](DBuf<unsigned char> &c) { return reinterpret_cast<const T *>(c.p); };
+  switch (op.impl) {
+    case I_W01:
+      hipLaunchKernelGGL(k_wedge01<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, x, y, out);
+      break;
+    case I_W02:
+      hipLaunchKernelGGL(k_wedge02<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, op.i2.p, C(op.c0), C(op.c1), C(op.c2), x,
+                         y, out);
+      break;
+    case I_W11:
+      hipLaunchKernelGGL(k_wedge11<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, op.i2.p, C(op.c0), C(op.c1), C(op.c2), x,
+                         y, out);
+      break;
+    case I_WDD:
+      if (op.m == 0)  // (m*f) .* g : f = x (dual 0-form), g = y
+        hipLaunchKernelGGL(k_wedge_dd<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, C(op.c0), C(op.c1), x, y, out);
+      else  // f .* (m*g)
+        hipLaunchKernelGGL(k_wedge_dd<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, C(op.c0), C(op.c1), y, x, out);
+      break;
+    case I_D0:
+      hipLaunchKernelGGL(k_d0<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, op.s8.p, x, out);
+      break;
+    case I_D1:
+      hipLaunchKernelGGL(k_d1<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, op.i2.p, op.s8.p, x, out);
+      break;
+    case I_SELL:
+      hipLaunchKernelGGL(k_sign_ell<T>, g, b, 0, s, rows, op.sell.width, op.sell.col.p, op.sell.val.p, x, out);
+      break;
+    case I_DIAG:
+      hipLaunchKernelGGL(k_diag<T>, g, b, 0, s, rows, C(op.c0), x, out);
+      break;
+    case I_ELL:
+      if constexpr (sizeof(T) == 8) dec::ell_spmv(op.ell64, x, out, s);
+      else dec::ell_spmv(op.ell32, x, out, s);
+      break;
+    case I_AVG:
+      hipLaunchKernelGGL(k_avg01<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, x, out);
+      break;
+    default:
+      fail(DEC_ERR_ARG, "dec_op_apply: corrupt operator");
+  }
+  dec::check_launch("dec_op_apply");
+}
+
+void unsupported(int m, int n) {
+  fail(DEC_ERR_UNSUPPORTED, "Unsupported combination of degrees " + std::to_string(m) + " and " + std::to_string(n) +
+                                ". Ensure that their sum is not greater than the degree of the complex.");
+}
+
+template <class T>
+void make_diag(dec_op_s &op, const DBuf<double> &h, bool inverse, int k) {
+  if (inverse) {
+    op.c0.alloc(sizeof(T) * h.n);
+    double mult = ((k * (2 - k)) % 2 == 0) ? 1.0 : -1.0;
+    if (h.n)
+      hipLaunchKernelGGL(k_inv_diag<T>, dim3(dec::nblocks(static_cast<long long>(h.n))), dim3(256), 0, 0,
+                         static_cast<int>(h.n), h.p, mult, reinterpret_cast<T *>(op.c0.p));
+  } else {
+    launch_cast<T>(h, op.c0);
+  }
+}
+
+dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) {
+  if (dtype != DEC_F64 && dtype != DEC_F32) fail(DEC_ERR_ARG, "dec_op_create: dtype must be DEC_F64 or DEC_F32");
+  if (hodge != DEC_GEOMETRIC_HODGE && hodge != DEC_DIAGONAL_HODGE) fail(DEC_ERR_ARG, "dec_op_create: unknown hodge");
+  dec::ensure_dual(m);
+  const dec::Topo &t = m.h;
+  int nv = t.nv, ne = t.ne, nt = t.nt;
+  bool f64 = dtype == DEC_F64;
+  auto op = std::make_unique<dec_op_s>();
+  op->kind = kind;
+  op->m = dm;
+  op->n = dn;
+  op->hodge = hodge;
+  op->dtype = dtype;
+  switch (kind) {
+    case DEC_OP_WEDGE: {
+      int lo = std::min(dm, dn), hi = std::max(dm, dn);
+      op->swap = dm > dn;  // (k,0) evaluates the (0,k) kernel on (β, α) (FastDEC.jl:266-269)
+      if (lo == 0 && hi == 1) {
+        op->impl = I_W01;
+        op->rows = ne;
+        op->i0.upload(t.v0);
+        op->i1.upload(t.v1);
+        op->cols = dm == 0 ? nv : ne;
+        op->cols2 = dm == 0 ? ne : nv;
+      } else if (lo == 0 && hi == 2) {
+        op->impl = I_W02;
+        op->rows = nt;
+        // slots (v0,v0,v1,v1,v2,v2) = ∂v_ev[∂e_e] for schema (v,e,ev) (FastDEC.jl:56)
+        std::vector<int32_t> a(nt), b(nt), c(nt);
+        for (int i = 0; i < nt; i++) {
+          int s0 = t.v1[t.e1[i]], s1 = t.v1[t.e2[i]], s2 = t.v0[t.e2[i]], s3 = t.v1[t.e0[i]],
+              s4 = t.v0[t.e0[i]], s5 = t.v0[t.e1[i]];
+          if (s0 != s1 || s2 != s3 || s4 != s5)
+            fail(DEC_ERR_UNSUPPORTED, "wedge 0-2: triangle violates the simplicial identities");
+          a[i] = s0;
+          b[i] = s2;
+          c[i] = s4;
+        }
+        op->i0.upload(a);
+        op->i1.upload(b);
+        op->i2.upload(c);
+        size_t bytes = (f64 ? 8 : 4) * static_cast<size_t>(nt);
+        op->c0.alloc(bytes);
+        op->c1.alloc(bytes);
+        op->c2.alloc(bytes);
+        if (nt) {
+          if (f64)
+            hipLaunchKernelGGL(k_coeff02<double>, dim3(dec::nblocks(nt)), dim3(256), 0, 0, nt, m.da.p, m.area.p,
+                               reinterpret_cast<double *>(op->c0.p), reinterpret_cast<double *>(op->c1.p),
+                               reinterpret_cast<double *>(op->c2.p));
+          else
+            hipLaunchKernelGGL(k_coeff02<float>, dim3(dec::nblocks(nt)), dim3(256), 0, 0, nt, m.da.p, m.area.p,
+                               reinterpret_cast<float *>(op->c0.p), reinterpret_cast<float *>(op->c1.p),
+                               reinterpret_cast<float *>(op->c2.p));
+        }
+        op->cols = dm == 0 ? nv : nt;
+        op->cols2 = dm == 0 ? nt : nv;
+      } else if (lo == 1 && hi == 1) {
+        op->impl = I_W11;
+        op->rows = nt;
+        op->i0.upload(t.e0);
+        op->i1.upload(t.e1);
+        op->i2.upload(t.e2);
+        size_t bytes = (f64 ? 8 : 4) * static_cast<size_t>(nt);
+        op->c0.alloc(bytes);
+        op->c1.alloc(bytes);
+        op->c2.alloc(bytes);
+        if (nt) {
+          if (f64)
+            hipLaunchKernelGGL(k_coeff11<double>, dim3(dec::nblocks(nt)), dim3(256), 0, 0, nt, m.e0.p, m.e1.p, m.e2.p,
+                               m.eo.p, m.to.p, m.da.p, m.area.p, reinterpret_cast<double *>(op->c0.p),
+                               reinterpret_cast<double *>(op->c1.p), reinterpret_cast<double *>(op->c2.p));
+          else
+            hipLaunchKernelGGL(k_coeff11<float>, dim3(dec::nblocks(nt)), dim3(256), 0, 0, nt, m.e0.p, m.e1.p, m.e2.p,
+                               m.eo.p, m.to.p, m.da.p, m.area.p, reinterpret_cast<float *>(op->c0.p),
+                               reinterpret_cast<float *>(op->c1.p), reinterpret_cast<float *>(op->c2.p));
+        }
+        op->cols = ne;
+        op->cols2 = ne;
+      } else {
+        unsupported(dm, dn);
+      }
+      break;
+    }
+    case DEC_OP_WEDGE_DD: {
+      if (!((dm == 0 && dn == 1) || (dm == 1 && dn == 0)))
+        fail(DEC_ERR_UNSUPPORTED, "Unsupported combination of degrees " + std::to_string(dm) + " and " +
+                                      std::to_string(dn) +
+                                      ". The currently supported dual-dual wedges are 0-1 and 1-0.");
+      op->impl = I_WDD;
+      op->rows = ne;
+      dec::Csr eb;
+      dec::edge_bduals(t, eb);
+      std::vector<int32_t> ta(ne, -1), tb(ne, -1);
+      std::vector<int32_t> order_ptr(ne + 1, 0), order_idx;
+      for (int e = 0; e < ne; e++) {
+        int cnt = eb.ptr[e + 1] - eb.ptr[e];
+        if (cnt > 2) fail(DEC_ERR_UNSUPPORTED, "dual-dual wedge: edge with more than two cofaces");
+        // weights in ascending triangle order
+        std::vector<int32_t> ids(eb.idx.begin() + eb.ptr[e], eb.idx.begin() + eb.ptr[e + 1]);
+        std::sort(ids.begin(), ids.end(), [&](int a, int b) { return a % nt < b % nt; });
+        if (cnt > 0) ta[e] = ids[0] % nt;
+        if (cnt > 1) tb[e] = ids[1] % nt;
+        for (int id : ids) order_idx.push_back(id);
+        order_ptr[e + 1] = static_cast<int32_t>(order_idx.size());
+      }
+      op->i0.upload(ta);
+      op->i1.upload(tb);
+      DBuf<int32_t> dp, di;
+      dp.upload(order_ptr);
+      di.upload(order_idx);
+      size_t bytes = (f64 ? 8 : 4) * static_cast<size_t>(ne);
+      op->c0.alloc(bytes);
+      op->c1.alloc(bytes);
+      const double *dlB = m.dl.p + 2 * static_cast<size_t>(ne);
+      if (ne) {
+        if (f64)
+          hipLaunchKernelGGL(k_coeff_dd<double>, dim3(dec::nblocks(ne)), dim3(256), 0, 0, ne, dp.p, di.p, dlB,
+                             reinterpret_cast<double *>(op->c0.p), reinterpret_cast<double *>(op->c1.p));
+        else
+          hipLaunchKernelGGL(k_coeff_dd<float>, dim3(dec::nblocks(ne)), dim3(256), 0, 0, ne, dp.p, di.p, dlB,
+                             reinterpret_cast<float *>(op->c0.p), reinterpret_cast<float *>(op->c1.p));
+      }
+      dec::check_launch("wedge_dd coefficients");
+      dec::dsync();
+      op->cols = dm == 0 ? nt : ne;
+      op->cols2 = dm == 0 ? ne : nt;
+      break;
+    }
+    case DEC_OP_DIFFERENTIAL:
+    case DEC_OP_BOUNDARY:
+    case DEC_OP_DUAL_DERIVATIVE: {
+      // ∂_n = d_{n-1}ᵀ; d̃0 = d1ᵀ; d̃1 = -d0ᵀ (FastDEC.jl:433-453)
+      int which;        // 0: d0, 1: d1, 2: d0ᵀ, 3: d1ᵀ
+      bool neg = false;
+      if (kind == DEC_OP_DIFFERENTIAL) {
+        if (dm != 0 && dm != 1) fail(DEC_ERR_UNSUPPORTED, "dec_differential: n must be 0 or 1 on a 2-D complex");
+        which = dm;
+      } else if (kind == DEC_OP_BOUNDARY) {
+        if (dm != 1 && dm != 2) fail(DEC_ERR_UNSUPPORTED, "dec_boundary: n must be 1 or 2 on a 2-D complex");
+        which = dm == 1 ? 2 : 3;
+      } else {
+        if (dm != 0 && dm != 1) fail(DEC_ERR_UNSUPPORTED, "dec_dual_derivative: n must be 0 or 1 on a 2-D complex");
+        which = dm == 0 ? 3 : 2;
+        neg = dm == 1;
+      }
+      if (which == 0) {
+        op->impl = I_D0;
+        op->rows = ne;
+        op->cols = nv;
+        op->i0.upload(t.v0);
+        op->i1.upload(t.v1);
+        std::vector<int8_t> sg(ne);
+        for (int e = 0; e < ne; e++) sg[e] = t.eo[e] == 1 ? 1 : -1;
+        op->s8.upload(sg);
+        csr_d0(t, op->csr);
+      } else if (which == 1) {
+        op->impl = I_D1;
+        op->rows = nt;
+        op->cols = ne;
+        csr_d1(t, op->csr);
+        std::vector<int32_t> a(nt), b(nt), c(nt);
+        std::vector<int8_t> sg(nt);
+        for (int i = 0; i < nt; i++) {
+          int p = op->csr.ptr[i];
+          if (op->csr.ptr[i + 1] - p != 3) fail(DEC_ERR_UNSUPPORTED, "d1: triangle with a repeated edge");
+          a[i] = op->csr.col[p];
+          b[i] = op->csr.col[p + 1];
+          c[i] = op->csr.col[p + 2];
+          sg[i] = static_cast<int8_t>((op->csr.val[p] > 0 ? 1 : 0) | (op->csr.val[p + 1] > 0 ? 2 : 0) |
+                                      (op->csr.val[p + 2] > 0 ? 4 : 0));
+        }
+        op->i0.upload(a);
+        op->i1.upload(b);
+        op->i2.upload(c);
+        op->s8.upload(sg);
+      } else if (which == 2) {
+        op->impl = I_SELL;
+        op->rows = nv;
+        op->cols = ne;
+        dec::Csr ve;
+        dec::vertex_edges(t, ve);
+        to_sign_ell(ve, op->sell, neg);
+        csr_from_signs(ve, ne, neg, op->csr);
+      } else {
+        op->impl = I_SELL;
+        op->rows = ne;
+        op->cols = nt;
+        dec::Csr et;
+        dec::edge_triangles(t, et);
+        to_sign_ell(et, op->sell, false);
+        csr_from_signs(et, nt, false, op->csr);
+      }
+      break;
+    }
+    case DEC_OP_HODGE:
+    case DEC_OP_INV_HODGE: {
+      if (dm < 0 || dm > 2) fail(DEC_ERR_UNSUPPORTED, "dec_hodge_star: n must be 0, 1 or 2 on a 2-D complex");
+      bool inv = kind == DEC_OP_INV_HODGE;
+      if (hodge == DEC_GEOMETRIC_HODGE && dm == 1) {
+        if (inv)
+          fail(DEC_ERR_UNSUPPORTED,
+               "dec_inv_hodge_star(1, GeometricHodge()) is a linear solve; use DiagonalHodge() for an operator");
+        op->impl = I_ELL;
+        op->rows = op->cols = ne;
+        dec::geo_hodge1(m, op->csr);
+        if (f64) dec::upload_ell(op->csr, op->ell64);
+        else dec::upload_ell(op->csr, op->ell32);
+        break;
+      }
+      op->impl = I_DIAG;
+      const DBuf<double> &h = dm == 0 ? m.h0 : (dm == 1 ? m.h1 : m.h2);
+      op->rows = op->cols = static_cast<int64_t>(h.n);
+      if (f64) make_diag<double>(*op, h, inv, dm);
+      else make_diag<float>(*op, h, inv, dm);
+      std::vector<double> hv = h.download();
+      if (inv) {
+        double mult = ((dm * (2 - dm)) % 2 == 0) ? 1.0 : -1.0;
+        for (auto &x : hv) x = (1 / x) * mult;
+      }
+      csr_diag(hv, op->csr);
+      break;
+    }
+    case DEC_OP_LAPLACIAN0: {
+      op->impl = I_ELL;
+      op->rows = op->cols = nv;
+      std::vector<double> h0 = m.h0.download(), h1 = m.h1.download(), ih0(nv);
+      for (int v = 0; v < nv; v++) ih0[v] = 1 / h0[v];
+      if (hodge == DEC_GEOMETRIC_HODGE) {
+        dec::HostCsr H;
+        dec::geo_hodge1(m, H);
+        dec::assemble_laplacian0(t, ih0, h1, &H, op->csr);
+      } else {
+        dec::assemble_laplacian0(t, ih0, h1, nullptr, op->csr);
+      }
+      if (f64) dec::upload_ell(op->csr, op->ell64);
+      else dec::upload_ell(op->csr, op->ell32);
+      break;
+    }
+    case DEC_OP_AVG01: {
+      op->impl = I_AVG;
+      op->rows = ne;
+      op->cols = nv;
+      op->i0.upload(t.v0);
+      op->i1.upload(t.v1);
+      csr_d0(t, op->csr);
+      for (auto &x : op->csr.val) x = 0.5;
+      break;
+    }
+    default:
+      fail(DEC_ERR_ARG, "dec_op_create: unknown operator kind");
+  }
+  dec::check_launch("dec_op_create");
+  dec::dsync();
+  return op.release();
+}
+
+}  // namespace
+
+extern "C" {
+
+int dec_op_create(dec_mesh m, int32_t kind, int32_t deg_m, int32_t deg_n, int32_t hodge, int32_t dtype, dec_op *out) {
+  DEC_TRY
+  if (!m || !out) fail(DEC_ERR_ARG, "dec_op_create: NULL handle");
+  *out = nullptr;
+  *out = create(*m, kind, deg_m, deg_n, hodge, dtype);
+  DEC_CATCH
+}
+
+int dec_op_shape(dec_op op, int64_t *rows, int64_t *cols, int64_t *cols2) {
+  DEC_TRY
+  if (!op) fail(DEC_ERR_ARG, "dec_op_shape: NULL handle");
+  if (rows) *rows = op->rows;
+  if (cols) *cols = op->cols;
+  if (cols2) *cols2 = op->cols2;
+  DEC_CATCH
+}
+
+int dec_op_apply(dec_op op, const void *x, const void *y, void *out, void *stream) {
+  DEC_TRY
+  if (!op) fail(DEC_ERR_ARG, "dec_op_apply: NULL handle");
+  if (op->rows > 0 && (!x || !out)) fail(DEC_ERR_ARG, "dec_op_apply: NULL form");
+  if (op->cols2 > 0 && !y) fail(DEC_ERR_ARG, "dec_op_apply: this operator takes two forms");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (op->dtype == DEC_F64) apply_t<double>(*op, x, y, out, s);
+  else apply_t<float>(*op, x, y, out, s);
+  DEC_CATCH
+}
+
+int dec_op_export_csr(dec_op op, int64_t *nnz, int32_t *rowptr, int32_t *col, double *val) {
+  DEC_TRY
+  if (!op) fail(DEC_ERR_ARG, "dec_op_export_csr: NULL handle");
+  if (op->csr.ptr.empty()) fail(DEC_ERR_UNSUPPORTED, "dec_op_export_csr: operator has no matrix form");
+  if (nnz) *nnz = static_cast<int64_t>(op->csr.col.size());
+  if (rowptr) std::copy(op->csr.ptr.begin(), op->csr.ptr.end(), rowptr);
+  if (col) std::copy(op->csr.col.begin(), op->csr.col.end(), col);
+  if (val) std::copy(op->csr.val.begin(), op->csr.val.end(), val);
+  DEC_CATCH
+}
+
+int dec_op_export_tables(dec_op op, int32_t *idx, double *coeff) {
+  DEC_TRY
+  if (!op) fail(DEC_ERR_ARG, "dec_op_export_tables: NULL handle");
+  size_t n = static_cast<size_t>(op->rows);
+  auto dl_i = [&](DBuf<int32_t> &b, size_t slot) {
+    if (idx && b.p && n) dec::d2h(idx + slot * n, b.p, sizeof(int32_t) * n);
+  };
+  auto dl_c = [&](DBuf<unsigned char> &b, size_t slot) {
+    if (!coeff || !b.p || !n) return;
+    if (op->dtype == DEC_F64) {
+      dec::d2h(coeff + slot * n, b.p, sizeof(double) * n);
+    } else {
+      std::vector<float> tmp(n);
+      dec::d2h(tmp.data(), b.p, sizeof(float) * n);
+      for (size_t i = 0; i < n; i++) coeff[slot * n + i] = tmp[i];
+    }
+  };
+  switch (op->impl) {
+    case I_W01: dl_i(op->i0, 0); dl_i(op->i1, 1); break;
+    case I_W02:
+    case I_W11:
+      dl_i(op->i0, 0); dl_i(op->i1, 1); dl_i(op->i2, 2);
+      dl_c(op->c0, 0); dl_c(op->c1, 1); dl_c(op->c2, 2);
+      break;
+    case I_WDD:
+      dl_i(op->i0, 0); dl_i(op->i1, 1); dl_c(op->c0, 0); dl_c(op->c1, 1);
+      break;
+    default:
+      fail(DEC_ERR_UNSUPPORTED, "dec_op_export_tables: operator has no wedge tables");
+  }
+  DEC_CATCH
+}
+
+int dec_op_destroy(dec_op op) {
+  DEC_TRY
+  delete op;
+  DEC_CATCH
+}
+
+}  // extern "C"

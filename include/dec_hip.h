@@ -1,0 +1,174 @@
+/*
+ * dec_hip.h -- C ABI of libdec_hip.so, the MI355X (gfx950) implementation of
+ * CombinatorialSpaces.jl's DEC hot path: FastDEC wedge products, exterior
+ * derivative / boundary / dual derivative, Hodge stars and inverses, the
+ * 0-form Laplace-Beltrami operator, and the geometric multigrid V-cycle.
+ *
+ * Conventions
+ *  - Status: every entry point returns int32 status (DEC_OK == 0); on error
+ *    dec_last_error() returns a thread-local message.  Nothing throws.
+ *  - Indices are int32.  Host index arrays passed in are 0-based unless the
+ *    index_base argument says 1 (Julia callers pass 1).
+ *  - Forms (x, y, out) are caller-owned DEVICE pointers (ROCArray, torch,
+ *    hipMalloc).  Meshes, operators and multigrid hierarchies own their device
+ *    tables and are freed by the matching *_destroy.
+ *  - stream is a hipStream_t (NULL = default stream).  Operators are
+ *    immutable after creation: concurrent applies on different streams are
+ *    safe.  A multigrid handle owns mutable work vectors: one stream at a time.
+ *  - dtype selects the arithmetic type of form values: DEC_F64 or DEC_F32.
+ */
+#ifndef DEC_HIP_H
+#define DEC_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dec_mesh_s *dec_mesh;
+typedef struct dec_op_s *dec_op;
+typedef struct dec_mg_s *dec_mg;
+
+enum dec_status {
+  DEC_OK = 0,
+  DEC_ERR_ARG = 1,          /* bad argument / shape mismatch */
+  DEC_ERR_UNSUPPORTED = 2,  /* "Unsupported combination of degrees ..." (FastDEC.jl:229, 256) */
+  DEC_ERR_OOM = 3,
+  DEC_ERR_HIP = 4,
+  DEC_ERR_NOCONV = 5,       /* solver did not reach the tolerance */
+  DEC_ERR_NONORIENTABLE = 6 /* orient! returned false (DiscreteExteriorCalculus.jl:576) */
+};
+enum dec_dtype { DEC_F64 = 0, DEC_F32 = 1 };
+enum dec_center { DEC_CIRCUMCENTER = 0, DEC_BARYCENTER = 1 };
+enum dec_hodge { DEC_GEOMETRIC_HODGE = 0, DEC_DIAGONAL_HODGE = 1 };
+enum dec_op_kind {
+  DEC_OP_WEDGE = 1,           /* primal-primal wedge, degrees (m,n) */
+  DEC_OP_WEDGE_DD = 2,        /* dual-dual wedge (0,1) / (1,0) */
+  DEC_OP_DIFFERENTIAL = 3,    /* d_n */
+  DEC_OP_BOUNDARY = 4,        /* ∂_n */
+  DEC_OP_DUAL_DERIVATIVE = 5, /* d̃_n */
+  DEC_OP_HODGE = 6,           /* ⋆_n */
+  DEC_OP_INV_HODGE = 7,       /* ⋆_n⁻¹ (diagonal) */
+  DEC_OP_LAPLACIAN0 = 8,      /* -⋆0⁻¹ d̃1 ⋆1 d0 */
+  DEC_OP_AVG01 = 9            /* avg₀₁ */
+};
+enum dec_smoother { DEC_SMOOTH_CG = 0, DEC_SMOOTH_GS = 1, DEC_SMOOTH_JACOBI = 2 };
+enum dec_solve_variant { DEC_SOLVE_PLAIN_F64 = 0, DEC_SOLVE_MIXED_IR = 1 };
+
+/* ---------------------------------------------------------------- library */
+const char *dec_last_error(void);
+const char *dec_version(void);
+/* Fill a device vector with the seeded uniform [-1,1) form
+ * x[i] = 2 u - 1, u = (splitmix64(seed * 2^32 + i) >> 11) * 2^-53. */
+int dec_fill_uniform(void *x, int64_t n, uint64_t seed, int dtype, void *stream);
+
+/* ------------------------------------------------------------------ meshes */
+/* EmbeddedDeltaSet2D from boundary maps (Multigrid.jl:27-64 MeshTopology /
+ * topo_to_mesh; SimplicialSets.jl:85-411).  edge_orient / tri_orient may be
+ * NULL or hold -1 for "unset": unset edge orientations become false and unset
+ * triangle orientations are set by orient! when the dual complex is built
+ * (DiscreteExteriorCalculus.jl:311-321, 573-587). points: nv x 3 float64. */
+int dec_mesh_create(int32_t nv, int32_t ne, int32_t nt, const int32_t *v0,
+                    const int32_t *v1, const int32_t *e0, const int32_t *e1,
+                    const int32_t *e2, const int8_t *edge_orient,
+                    const int8_t *tri_orient, const double *points,
+                    int32_t index_base, dec_mesh *out);
+/* Pole-aligned icosahedron refined k times by binary subdivision with the new
+ * points projected to the unit sphere (replaces loadmesh(Icosphere(n)),
+ * CombMeshes.jl:31-36, whose artifact is a downloaded file). */
+int dec_mesh_icosphere(int32_t k, dec_mesh *out);
+/* triangulated_grid(lx, ly, dx, dy, Point3d, compress) (CombMeshes.jl:79-130). */
+int dec_mesh_triangulated_grid(double lx, double ly, double dx, double dy,
+                               int32_t compress, dec_mesh *out);
+/* subdivision(s, BinarySubdivision()) (Multigrid.jl:447-452): refine +
+ * propagate_points; project=1 also rescales new points to |p| = 1. */
+int dec_mesh_subdivide(dec_mesh coarse, int32_t project, dec_mesh *fine);
+int dec_mesh_counts(dec_mesh m, int32_t *nv, int32_t *ne, int32_t *nt);
+/* Host copies of the primal complex (after orientation). Any pointer may be NULL. */
+int dec_mesh_export(dec_mesh m, int32_t *v0, int32_t *v1, int32_t *e0, int32_t *e1,
+                    int32_t *e2, int8_t *edge_orient, int8_t *tri_orient,
+                    double *points);
+/* dualize(s, center) geometry on the GPU in float64 (subdivide_duals_2d! +
+ * precompute_volumes_2d!, DiscreteExteriorCalculus.jl:941-990). */
+int dec_mesh_dualize(dec_mesh m, int32_t center);
+/* Host copies of the dual geometry in the reference's DualV / DualE /
+ * DualTri numbering: dual_point (nv+ne+nt) x 3, length ne, area nt,
+ * dual_length 2ne+6nt, dual_area 6nt.  Any pointer may be NULL. */
+int dec_mesh_export_dual(dec_mesh m, double *dual_point, double *length,
+                         double *area, double *dual_length, double *dual_area);
+int dec_mesh_destroy(dec_mesh m);
+
+/* --------------------------------------------------------------- operators */
+/* Create an operator on a dualized mesh.
+ *  DEC_OP_WEDGE, (m,n) in {(0,1),(1,0),(0,2),(2,0),(1,1)}:
+ *     dec_wedge_product(m, n, sd, Val(:HIP), ROCArray, T) (FastDEC.jl:255-289,
+ *     CUDA counterpart ext/CombinatorialSpacesCUDAExt.jl:16-21); apply(x=α, y=β).
+ *  DEC_OP_WEDGE_DD, (0,1) / (1,0): dec_wedge_product_dd (FastDEC.jl:313-325).
+ *  DEC_OP_DIFFERENTIAL n=0,1: dec_differential (FastDEC.jl:468-469, CUDA ext :55-61).
+ *  DEC_OP_BOUNDARY n=1,2: dec_boundary (FastDEC.jl:433-437, CUDA ext :30-36).
+ *  DEC_OP_DUAL_DERIVATIVE n=0,1: dec_dual_derivative (FastDEC.jl:443-453, CUDA ext :42-48).
+ *  DEC_OP_HODGE n=0,1,2 with hodge: dec_hodge_star (FastDEC.jl:641-716, CUDA ext :70-77).
+ *  DEC_OP_INV_HODGE n=0,1,2 (diagonal; geometric ⋆1⁻¹ is a solve and is
+ *     reported as DEC_ERR_UNSUPPORTED): dec_inv_hodge_star (FastDEC.jl:721-743, CUDA ext :83-91).
+ *  DEC_OP_LAPLACIAN0 (hodge selects ⋆1): fast_laplace_beltrami
+ *     (test/Multigrid.jl:46-48) = ∇²(0, sd) form (DiscreteExteriorCalculus.jl:1887-1890).
+ *  DEC_OP_AVG01: avg₀₁_mat (FastDEC.jl:874-892).
+ * deg_n is ignored for single-degree kinds. */
+int dec_op_create(dec_mesh m, int32_t kind, int32_t deg_m, int32_t deg_n,
+                  int32_t hodge, int32_t dtype, dec_op *out);
+/* rows = output length; cols = length of the first input, cols2 of the
+ * second input (0 if unary). */
+int dec_op_shape(dec_op op, int64_t *rows, int64_t *cols, int64_t *cols2);
+/* out = op(x[, y]).  Unary operators ignore y.  Same dtype as the operator. */
+int dec_op_apply(dec_op op, const void *x, const void *y, void *out, void *stream);
+/* Sparse-matrix operators (d, ∂, d̃, ⋆ (geometric or diagonal), ⋆⁻¹, ∇²,
+ * avg01, wedge_dd's averaging matrix): CSR export for parity checks, columns
+ * ascending.  Call with NULL arrays to read nnz first. */
+int dec_op_export_csr(dec_op op, int64_t *nnz, int32_t *rowptr, int32_t *col,
+                      double *val);
+/* Wedge coefficient tables as the kernels read them (SoA, float64 copies):
+ * wedge 0-1: idx 2 x ne; wedge 0-2: idx 3 x nt, coeff 3 x nt (pairs of the
+ * reference's six coefficients pre-summed); wedge 1-1: idx 3 x nt, coeff 3 x nt. */
+int dec_op_export_tables(dec_op op, int32_t *idx, double *coeff);
+int dec_op_destroy(dec_op op);
+
+/* --------------------------------------------------------------- multigrid */
+/* MultigridData(s, BinarySubdivision(), levels, op, steps; alg, mode=DirectMode())
+ * (Multigrid.jl:698-702, 752-797) with op = fast_laplace_beltrami using the
+ * given Hodge for ⋆1 and dual center.  Levels are stored finest first.
+ * project=1 rescales every new point to the unit sphere (icosphere family). */
+int dec_mg_create(dec_mesh base, int32_t levels, int32_t project, int32_t hodge,
+                  int32_t center, int32_t steps, dec_mg *out);
+int dec_mg_nlevels(dec_mg mg, int32_t *nlevels);
+int dec_mg_level_size(dec_mg mg, int32_t level, int32_t *n);
+/* The fine mesh of the hierarchy (owned by the hierarchy; do not destroy). */
+int dec_mg_finest_mesh(dec_mg mg, dec_mesh *m);
+/* CSR exports of A_l, R_l (level l -> l+1) and P_l = transpose(S_l). */
+int dec_mg_export_level(dec_mg mg, int32_t level, int32_t which, int64_t *nnz,
+                        int32_t *rowptr, int32_t *col, double *val);
+/* Smoother per level: alg (cg = Krylov.jl's cg as in Multigrid.jl:860,
+ * gauss_seidel = multicolour GS, jacobi = damped Jacobi), steps (0 = skip,
+ * Multigrid.jl:860, 871), omega (Jacobi damping).  coarse_direct=1 solves the
+ * coarsest level with a dense float64 operator instead of smoothing it. */
+int dec_mg_set_smoother(dec_mg mg, int32_t alg, int32_t steps, double omega,
+                        int32_t coarse_direct);
+/* multigrid_vcycles / multigrid_wcycles (Multigrid.jl:814-824): mu = 1 (V)
+ * or 2 (W); u (in/out) and b are float64 device vectors of the finest size. */
+int dec_mg_cycles(dec_mg mg, int32_t mu, int32_t cycles, void *u, const void *b,
+                  void *stream);
+/* full_multigrid (Multigrid.jl:830-855); dec_Δ⁻¹ (FastDEC.jl:866-869) is
+ * full_multigrid with mu = 2, cycles = 5, steps = 3, alg = cg. */
+int dec_mg_full_multigrid(dec_mg mg, int32_t mu, int32_t cycles, const void *b,
+                          void *u, void *stream);
+/* Solve A u = b to ||b - A u||_2 / ||b||_2 <= tol (checked in float64).
+ * variant DEC_SOLVE_PLAIN_F64: float64 V-cycles; DEC_SOLVE_MIXED_IR: float32
+ * V-cycles inside float64 iterative refinement.  Returns the cycle count and
+ * the final float64 relative residual. */
+int dec_mg_solve(dec_mg mg, int32_t variant, double tol, int32_t max_cycles,
+                 void *u, const void *b, int32_t *cycles_done, double *relres,
+                 void *stream);
+int dec_mg_destroy(dec_mg mg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

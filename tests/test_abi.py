@@ -1,0 +1,39 @@
+"""The C-ABI library loads on the CPU and exports every symbol include/dec_hip.h
+declares (no compute calls: those need a GPU)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    hdr = open(os.path.join(ROOT, "include", "dec_hip.h")).read()
+    return set(re.findall(r"^(?:int|const char \*)\s*(dec_\w+)\(", hdr, re.M))
+
+
+def test_header_declares_the_boundary():
+    syms = declared_symbols()
+    for s in ("dec_mesh_create", "dec_op_create", "dec_op_apply", "dec_mg_create", "dec_mg_cycles",
+              "dec_mg_solve", "dec_last_error"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    from cspaces import _lib
+
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing
+    assert set(_lib.SIGNATURES) == declared_symbols()
+
+
+def test_version_and_error_without_gpu():
+    from cspaces import _lib
+
+    lib = _lib.lib()
+    assert lib.dec_version().startswith(b"dec_hip")
+    # argument errors are reported before any device work
+    h = ctypes.c_void_p()
+    assert lib.dec_mesh_icosphere(-1, ctypes.byref(h)) == 1
+    assert b"k must be" in lib.dec_last_error()

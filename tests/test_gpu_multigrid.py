@@ -1,0 +1,111 @@
+"""GPU multigrid against the oracle and the reference's own residual thresholds
+(test/Multigrid.jl:54-90, 92-102, 151-170)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def csr_equal(a, b, exact=True, tol=1e-12):
+    (rp, ci, vv), (orp, oci, ovv) = a, b
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    if exact:
+        assert np.array_equal(vv, ovv)
+    else:
+        assert np.max(np.abs(vv - ovv)) <= tol * np.max(np.abs(ovv))
+
+
+@pytest.fixture(scope="module")
+def grid_pair(O, cs):
+    base_o = O.triangulated_grid(1, 1, 1 / 4, math.sqrt(3) / 2 * 1 / 4, compress=False)
+    H = O.Hierarchy(base_o, 4, geometric=True)
+    base_g = cs.triangulated_grid(1, 1, 1 / 4, math.sqrt(3) / 2 * 1 / 4, False)
+    md = cs.MultigridData(base_g, cs.BinarySubdivision, 4, cs.GeometricHodge, 3)
+    return H, md
+
+
+@pytest.fixture(scope="module")
+def sphere_pair(O, cs):
+    H = O.Hierarchy(O.icosahedron(), 4, project=True, geometric=False)
+    md = cs.MultigridData(cs.icosphere(0), cs.BinarySubdivision, 4, cs.DiagonalHodge, 2, project=True)
+    return H, md
+
+
+@pytest.mark.parametrize("which", ["grid", "sphere"])
+def test_hierarchy_matches_oracle(which, grid_pair, sphere_pair):
+    H, md = grid_pair if which == "grid" else sphere_pair
+    assert md.nlevels == len(H.A) and md.sizes == [a.m for a in H.A]
+    for lv in range(md.nlevels):
+        csr_equal(md.export(lv, "A"), H.A[lv].csr(), exact=(which == "sphere"))
+        if lv + 1 < md.nlevels:
+            csr_equal(md.export(lv, "R"), H.R[lv].csr())
+            csr_equal(md.export(lv, "P"), H.S[lv].transpose().csr())
+
+
+def rhs(O, L, seed):
+    return L.matvec(0.5 * O.uniform_form(seed, L.n) + 0.5)   # b = L * rand(nv)
+
+
+def test_reference_thresholds_cg(O, cs, grid_pair):
+    import torch
+
+    H, md = grid_pair
+    L = H.A[0]
+    b = rhs(O, L, 7)
+    db = torch.tensor(b, device="cuda")
+    res = lambda u: np.linalg.norm(L.matvec(u.cpu().numpy()) - b) / np.linalg.norm(b)  # noqa: E731
+    z = lambda: torch.zeros(L.n, dtype=torch.float64, device="cuda")  # noqa: E731
+    assert res(cs.dec_Delta_inv(md)(db)) < 1e-6
+    assert res(cs.multigrid_vcycles(z(), db, md, 5)) < 1e-7
+    assert res(cs.multigrid_wcycles(z(), db, md, 5)) < 1e-7
+    assert res(cs.full_multigrid(db, md, 5)) < 1e-6
+    assert res(cs.full_multigrid(db, md, 5, cs.cg, 2)) < 1e-7
+    r0 = res(cs.multigrid_vcycles(z(), db, md, 5, steps=0))
+    r1 = res(cs.multigrid_vcycles(z(), db, md, 5, steps=1))
+    assert r1 < r0
+    # CG iterates are third-party arithmetic (Krylov.jl): compared with the
+    # oracle's restatement only to a tolerance
+    u_o = H.mu_cycles(np.zeros(L.n), b, 2, alg=O.CG)
+    u_g = cs.multigrid_vcycles(z(), db, md, 2, steps=3).cpu().numpy()
+    assert np.linalg.norm(u_g - u_o) <= 1e-8 * np.linalg.norm(u_o)
+
+
+@pytest.mark.parametrize("alg", ["gs", "jacobi"])
+def test_smoothers_match_oracle(O, cs, sphere_pair, alg):
+    import torch
+
+    H, md = sphere_pair
+    L = H.A[0]
+    b = rhs(O, L, 9)
+    b = b - b.mean()
+    oalg, galg = (O.GS, cs.gauss_seidel) if alg == "gs" else (O.JACOBI, cs.jacobi)
+    u_o = H.mu_cycles(np.zeros(L.n), b, 3, alg=oalg, steps=2, coarse_direct=True)
+    u = torch.zeros(L.n, dtype=torch.float64, device="cuda")
+    cs.multigrid_vcycles(u, torch.tensor(b, device="cuda"), md, 3, galg, steps=2, coarse_direct=True)
+    assert np.linalg.norm(u.cpu().numpy() - u_o) <= 1e-10 * np.linalg.norm(u_o)
+
+
+def sphere_rhs(O, d, seed):
+    """Seeded smooth right-hand side, projected to ⋆0-weighted mean zero."""
+    p = d.m.p
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1, 1, 4)
+    f = c[0] * p[:, 2] + c[1] * p[:, 0] * p[:, 1] + c[2] * (3 * p[:, 2] ** 2 - 1) + c[3] * p[:, 0] ** 3
+    h0 = d.hodge_diag(0)
+    return f - (h0 @ f) / h0.sum()
+
+
+@pytest.mark.parametrize("variant", ["plain", "mixed"])
+def test_solve_to_tolerance(O, cs, sphere_pair, variant):
+    import torch
+
+    H, md = sphere_pair
+    L = H.A[0]
+    f = sphere_rhs(O, H.duals[0], 3)
+    u, cycles, rr = cs.solve(md, torch.tensor(f, device="cuda"), tol=1e-8, variant=variant)
+    u = u.cpu().numpy()
+    true = np.linalg.norm(f - L.matvec(u)) / np.linalg.norm(f)   # float64 oracle check
+    assert rr <= 1e-8 and true <= 1.01e-8
+    assert cycles <= 40

@@ -1,0 +1,301 @@
+"""GPU parity of meshes, dual geometry and every FastDEC operator against the
+CPU oracle, through the C ABI (libdec_hip.so).
+
+Bars: integer / index / orientation work bit-exact; float64 operator outputs
+within 1e-12 normwise relative error (‖x - x_ref‖∞ / ‖x_ref‖∞) of the oracle,
+and bit-exact where the reference's arithmetic fixes every rounding; float32
+outputs report their error against the float64 oracle (bound 1e-5).
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+F32_TOL = 1e-5
+
+
+def rel(a, b):
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    d = np.max(np.abs(b)) if b.size else 0.0
+    return 0.0 if b.size == 0 else float(np.max(np.abs(a - b)) / (d if d > 0 else 1.0))
+
+
+def to_dev(torch, x, dtype):
+    return torch.tensor(np.ascontiguousarray(x), dtype=dtype, device="cuda")
+
+
+# ----------------------------------------------------------------------------- cases
+def oracle_icosphere(O, k):
+    return O.icosphere(k)
+
+
+def gpu_mesh_like(cs, m):
+    """Product mesh from an oracle mesh's arrays (orientations as given)."""
+    return cs.mesh_from_arrays(m.p, m.v0, m.v1, m.e0, m.e1, m.e2, m.eo, m.to)
+
+
+def grid345_arrays(O):
+    P = np.array([[0, 4, 0], [3, 4, 0], [6, 4, 0], [0, 0, 0], [3, 0, 0], [6, 0, 0],
+                  [0, -4, 0], [3, -4, 0], [6, -4, 0]], float)
+    tris = [[0, 1, 3], [4, 1, 3], [4, 1, 2], [4, 5, 2], [4, 6, 3], [4, 6, 7], [4, 5, 7], [8, 5, 7]]
+    m = O.glue_triangles(P, tris, sorted_=True)
+    m.eo[:] = 1
+    assert O.orient(m)
+    m.to = (1 - m.to).astype(np.int8)
+    return m
+
+
+CASES = ["ico3", "ico5", "grid", "grid345_bary", "square_unoriented"]
+
+
+@pytest.fixture(scope="module", params=CASES)
+def case(request, O, cs):
+    name = request.param
+    circ = True
+    if name.startswith("ico"):
+        k = int(name[3:])
+        om = O.icosphere(k)
+        gm = cs.icosphere(k)
+    elif name == "grid":
+        om = O.triangulated_grid(1, 1, 1 / 8, math.sqrt(3) / 16, compress=True)
+        gm = cs.triangulated_grid(1, 1, 1 / 8, math.sqrt(3) / 16, True)
+    elif name == "grid345_bary":
+        om = grid345_arrays(O)
+        gm = gpu_mesh_like(cs, om)
+        circ = False
+    else:
+        om = O.glue_triangles(np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0.2]], float),
+                              [[0, 1, 2], [0, 2, 3]])
+        gm = gpu_mesh_like(cs, om)
+    od = O.Dual(om, circumcenter=circ)
+    cs.dualize(gm, cs.Circumcenter if circ else cs.Barycenter)
+    return name, od, gm
+
+
+def test_topology_and_orientation_bit_exact(case):
+    name, od, gm = case
+    g = gm.export()
+    m = od.m
+    for a in ("v0", "v1", "e0", "e1", "e2"):
+        assert np.array_equal(g[a], getattr(m, a)), a
+    assert np.array_equal(g["edge_orientation"], od.eo)
+    assert np.array_equal(g["tri_orientation"], od.to)        # orient! (DFS) vs the HIP rule
+    assert np.array_equal(g["points"], m.p)                   # midpoint + projection, bitwise
+
+
+def test_dual_geometry(case):
+    name, od, gm = case
+    g = gm.export_dual()
+    assert rel(g["dual_point"], od.dual_point) <= F64_TOL
+    for a in ("length", "area", "dual_length", "dual_area"):
+        assert rel(g[a], getattr(od, a)) <= F64_TOL, a
+    # lengths and edge-center duals involve only correctly rounded operations
+    assert np.array_equal(g["length"], od.length)
+    assert np.array_equal(g["dual_length"][:2 * od.ne], od.dual_length[:2 * od.ne])
+
+
+def forms(O, od, seed):
+    return (O.uniform_form(seed, od.nv), O.uniform_form(seed + 1, od.ne),
+            O.uniform_form(seed + 2, od.ne), O.uniform_form(seed + 3, od.nt))
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_wedges(case, O, cs, dtype):
+    import torch
+
+    name, od, gm = case
+    td = torch.float64 if dtype == "f64" else torch.float32
+    tol = F64_TOL if dtype == "f64" else F32_TOL
+    V, E, E2, T = forms(O, od, 11)
+    dV, dE, dE2, dT = (to_dev(torch, x, td) for x in (V, E, E2, T))
+    w01 = cs.dec_wedge_product(0, 1, gm, cast_float=td)
+    w10 = cs.dec_wedge_product(1, 0, gm, cast_float=td)
+    w02 = cs.dec_wedge_product(0, 2, gm, cast_float=td)
+    w20 = cs.dec_wedge_product(2, 0, gm, cast_float=td)
+    w11 = cs.dec_wedge_product(1, 1, gm, cast_float=td)
+    ref01 = od.wedge01(V, E)
+    ref02 = od.wedge02(V, T)
+    ref11 = od.wedge11(E, E2)
+    r01 = w01(dV, dE).cpu().numpy()
+    assert rel(r01, ref01) <= tol
+    assert np.array_equal(w10(dE, dV).cpu().numpy(), r01)           # (1,0) swaps its arguments
+    assert rel(w02(dV, dT).cpu().numpy(), ref02) <= tol
+    assert rel(w20(dT, dV).cpu().numpy(), ref02) <= tol
+    assert rel(w11(dE, dE2).cpu().numpy(), ref11) <= tol
+    if dtype == "f64":
+        assert np.array_equal(r01, ref01)                             # wedge 0-1 is bit-exact
+        assert np.array_equal(w11(dE, dE).cpu().numpy(), np.zeros(od.nt))
+        assert np.array_equal(w11(dE, dE2).cpu().numpy(), -w11(dE2, dE).cpu().numpy())
+    # the coefficient tables themselves
+    idx, co = w11.operator.tables(3, 3)
+    e3, c3 = od.wedge11_coeffs()
+    assert np.array_equal(idx.T.reshape(-1), e3)
+    assert rel(co.T.reshape(-1), c3) <= (F64_TOL if dtype == "f64" else 1e-7)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_wedge_dd(case, O, cs, dtype):
+    import torch
+
+    name, od, gm = case
+    td = torch.float64 if dtype == "f64" else torch.float32
+    tol = F64_TOL if dtype == "f64" else F32_TOL
+    F = O.uniform_form(21, od.nt)
+    G = O.uniform_form(22, od.ne)
+    dd01 = cs.dec_wedge_product_dd(0, 1, gm, cast_float=td)
+    dd10 = cs.dec_wedge_product_dd(1, 0, gm, cast_float=td)
+    r01 = dd01(to_dev(torch, F, td), to_dev(torch, G, td)).cpu().numpy()
+    r10 = dd10(to_dev(torch, G, td), to_dev(torch, F, td)).cpu().numpy()
+    ref01 = od.wedge_dd01(F, G)
+    ref10 = od.wedge_dd10(G, F)
+    assert rel(r01, ref01) <= tol and rel(r10, ref10) <= tol
+    if dtype == "f64":
+        assert np.array_equal(r01, ref01) and np.array_equal(r10, ref10)
+
+
+def csr_of(O, A):
+    rp, ci, vv = A.csr()
+    return rp, ci, vv
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_derivatives(case, O, cs, dtype):
+    import torch
+
+    name, od, gm = case
+    td = torch.float64 if dtype == "f64" else torch.float32
+    V, E, _, T = forms(O, od, 31)
+    cases = [
+        (cs.dec_differential(0, gm, cast_float=td), od.differential(0), V),
+        (cs.dec_differential(1, gm, cast_float=td), od.differential(1), E),
+        (cs.dec_boundary(1, gm, cast_float=td), od.boundary(1), E),
+        (cs.dec_boundary(2, gm, cast_float=td), od.boundary(2), T),
+        (cs.dec_dual_derivative(0, gm, cast_float=td), od.dual_derivative(0), T),
+        (cs.dec_dual_derivative(1, gm, cast_float=td), od.dual_derivative(1), E),
+    ]
+    for op, A, x in cases:
+        rp, ci, vv = op.csr()
+        orp, oci, ovv = csr_of(O, A)
+        assert np.array_equal(rp, orp) and np.array_equal(ci, oci) and np.array_equal(vv, ovv)
+        y = op(to_dev(torch, x, td)).cpu().numpy()
+        ref = A.matvec(x)
+        if dtype == "f64":
+            assert np.array_equal(y, ref)        # ±1 sums in the reference's column order
+        else:
+            assert rel(y, ref) <= F32_TOL
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_hodge(case, O, cs, dtype):
+    import torch
+
+    name, od, gm = case
+    td = torch.float64 if dtype == "f64" else torch.float32
+    tol = F64_TOL if dtype == "f64" else F32_TOL
+    for k, n in ((0, od.nv), (1, od.ne), (2, od.nt)):
+        x = O.uniform_form(40 + k, n)
+        h = od.hodge_diag(k)
+        ih = od.inv_hodge_diag(k)
+        hs = cs.dec_hodge_star(k, gm, cs.DiagonalHodge, cast_float=td)
+        assert rel(hs.csr()[2], h) <= F64_TOL
+        assert rel(hs(to_dev(torch, x, td)).cpu().numpy(), h * x) <= tol
+        if np.all(np.isfinite(ih)):
+            ihs = cs.dec_inv_hodge_star(k, gm, cs.DiagonalHodge, cast_float=td)
+            assert rel(ihs(to_dev(torch, x, td)).cpu().numpy(), ih * x) <= tol
+    # geometric ⋆1: sparse matrix of the same pattern and values as the oracle's
+    H = od.geo_hodge1()
+    gh = cs.dec_hodge_star(1, gm, cs.GeometricHodge, cast_float=td)
+    rp, ci, vv = gh.csr()
+    orp, oci, ovv = H.csr()
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    assert rel(vv, ovv) <= F64_TOL
+    x = O.uniform_form(45, od.ne)
+    assert rel(gh(to_dev(torch, x, td)).cpu().numpy(), H.matvec(x)) <= tol
+    # geometric ⋆1⁻¹ is a solve, not an operator
+    with pytest.raises(cs.DecError):
+        cs.dec_inv_hodge_star(1, gm, cs.GeometricHodge)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("hodge", ["diagonal", "geometric"])
+def test_laplacian(case, O, cs, dtype, hodge):
+    import torch
+
+    name, od, gm = case
+    td = torch.float64 if dtype == "f64" else torch.float32
+    tol = F64_TOL if dtype == "f64" else F32_TOL
+    geo = hodge == "geometric"
+    L = od.laplacian0(geometric=geo)
+    op = cs.fast_laplace_beltrami(gm, cs.GeometricHodge if geo else cs.DiagonalHodge, cast_float=td)
+    rp, ci, vv = op.csr()
+    orp, oci, ovv = L.csr()
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    assert rel(vv, ovv) <= F64_TOL
+    if not geo:
+        assert np.array_equal(vv, ovv)         # reference SpGEMM order reproduced
+    x = O.uniform_form(51, od.nv)
+    y = op(to_dev(torch, x, td)).cpu().numpy()
+    ref = L.matvec(x)
+    assert rel(y, ref) <= tol
+    if dtype == "f64" and not geo:
+        assert np.array_equal(y, ref)
+
+
+def test_avg01(case, O, cs):
+    import torch
+
+    name, od, gm = case
+    V, E, _, _ = forms(O, od, 61)
+    A = cs.avg01_mat(gm)
+    dV = to_dev(torch, V, torch.float64)
+    dE = to_dev(torch, E, torch.float64)
+    w = cs.dec_wedge_product(0, 1, gm)(dV, dE)
+    assert torch.equal(A @ dV * dE, w)         # avg₀₁_mat*V .* E == wedge01 (Operators.jl:339-359)
+
+
+def test_argument_errors(cs):
+    import torch
+
+    gm = cs.icosphere(1)
+    with pytest.raises(cs.DecError, match="Unsupported combination of degrees 1 and 2"):
+        cs.dec_wedge_product(1, 2, gm)
+    with pytest.raises(cs.DecError, match="dual-dual wedges are 0-1 and 1-0"):
+        cs.dec_wedge_product_dd(1, 1, gm)
+    op = cs.dec_differential(0, gm)
+    with pytest.raises(ValueError):
+        op(torch.zeros(5, dtype=torch.float64, device="cuda"))
+    with pytest.raises(TypeError):
+        op(torch.zeros(gm.nv, dtype=torch.float32, device="cuda"))
+
+
+def test_large_mesh_properties(cs):
+    """icosphere k=8 (655,362 V): size-independent identities of the kernels."""
+    import torch
+
+    gm = cs.icosphere(8)
+    cs.dualize(gm)
+    assert (gm.nv, gm.ne, gm.nt) == (655362, 1966080, 1310720)
+    f64 = torch.float64
+    onesV = torch.ones(gm.nv, dtype=f64, device="cuda")
+    onesE = torch.ones(gm.ne, dtype=f64, device="cuda")
+    w01 = cs.dec_wedge_product(0, 1, gm)
+    assert torch.equal(w01(onesV, onesE), onesE)
+    E = cs.fill_uniform(torch.empty(gm.ne, dtype=f64, device="cuda"), 5)
+    w11 = cs.dec_wedge_product(1, 1, gm)
+    assert torch.count_nonzero(w11(E, E)).item() == 0
+    # d1 d0 = 0 exactly on integer-valued 0-forms
+    x = torch.randint(-1000, 1000, (gm.nv,), device="cuda").to(f64)
+    d0 = cs.dec_differential(0, gm)
+    d1 = cs.dec_differential(1, gm)
+    assert torch.count_nonzero(d1 @ (d0 @ x)).item() == 0
+    # L 1 = 0 (up to rounding) and ⋆0 sums to the surface area
+    L = cs.fast_laplace_beltrami(gm, cs.DiagonalHodge)
+    assert (L @ onesV).abs().max().item() < 1e-8
+    h0 = cs.dec_hodge_star(0, gm, cs.DiagonalHodge) @ onesV
+    area = (cs.dec_hodge_star(2, gm, cs.DiagonalHodge) @ torch.ones(gm.nt, dtype=f64, device="cuda"))
+    assert abs(h0.sum().item() - (1 / area).sum().item()) < 1e-10 * 4 * math.pi
+    assert abs(h0.sum().item() - 4 * math.pi) < 1e-4
