@@ -10,3 +10,4 @@ from .fastdec import (  # noqa: F401
 from .multigrid import (  # noqa: F401
     BinarySubdivision, MGData, MultigridData, cg, dec_Delta_inv, full_multigrid, gauss_seidel, jacobi,
     multigrid_mu_cycles, multigrid_vcycles, multigrid_wcycles, solve)
+from .batch import HeatBatch  # noqa: F401

@@ -52,6 +52,11 @@ SIGNATURES = {
     "dec_mg_full_multigrid": (c_int, [P, c_int32, c_int32, P, P, P]),
     "dec_mg_solve": (c_int, [P, c_int32, c_double, c_int32, P, P, PI32, PF64, P]),
     "dec_mg_destroy": (c_int, [P]),
+    "dec_batch_create": (c_int, [c_int32, P, c_int32, POINTER(P)]),
+    "dec_batch_layout": (c_int, [P, PI64, P, P]),
+    "dec_batch_tune": (c_int, [P, P, c_char_p, PI32, PI32, PI32]),
+    "dec_batch_heat_steps": (c_int, [P, P, c_int32, c_int32, P]),
+    "dec_batch_destroy": (c_int, [P]),
 }
 
 

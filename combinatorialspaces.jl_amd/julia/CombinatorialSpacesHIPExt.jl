@@ -1,0 +1,234 @@
+"""
+Package extension adding `Val(:HIP)` methods to CombinatorialSpaces.jl, backed by
+libdec_hip.so (HIP, gfx950) through `ccall`.  It mirrors the method list of
+ext/CombinatorialSpacesCUDAExt.jl: the same positional signatures, with the
+backend value selecting this library.  Forms are AMDGPU.jl `ROCArray`s; the
+library never copies them.  Register it in Project.toml as
+
+    [weakdeps]   AMDGPU = "21141c5a-9bdb-4563-92ae-f87d6854732e"
+    [extensions] CombinatorialSpacesHIPExt = "AMDGPU"
+
+This file is not executed by this repository's tests (no Julia in the build
+image); tests drive the same C ABI from Python.
+"""
+module CombinatorialSpacesHIPExt
+
+using CombinatorialSpaces
+using CombinatorialSpaces.DiscreteExteriorCalculus: DiscreteHodge
+using CombinatorialSpaces.Multigrid: BinarySubdivision, AbstractSubdivisionScheme
+using AMDGPU
+using Krylov
+using LinearAlgebra
+import CombinatorialSpaces: cache_wedge, dec_wedge_product, dec_wedge_product_dd,
+  dec_boundary, dec_differential, dec_dual_derivative,
+  dec_hodge_star, dec_inv_hodge_star, avg₀₁_mat,
+  multigrid_vcycles, multigrid_wcycles, full_multigrid
+
+const libdec_hip = get(ENV, "DEC_HIP_LIB", "libdec_hip")
+
+# C ABI constants (include/dec_hip.h)
+const DEC_F64, DEC_F32 = Int32(0), Int32(1)
+const DEC_CIRCUMCENTER = Int32(0)
+const DEC_GEOMETRIC_HODGE, DEC_DIAGONAL_HODGE = Int32(0), Int32(1)
+const DEC_OP_WEDGE, DEC_OP_WEDGE_DD, DEC_OP_DIFFERENTIAL, DEC_OP_BOUNDARY,
+      DEC_OP_DUAL_DERIVATIVE, DEC_OP_HODGE, DEC_OP_INV_HODGE, DEC_OP_LAPLACIAN0,
+      DEC_OP_AVG01 = Int32.(1:9)
+const DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI = Int32.(0:2)
+
+function check(status::Cint)
+  status == 0 && return nothing
+  msg = unsafe_string(ccall((:dec_last_error, libdec_hip), Cstring, ()))
+  status == 2 ? error(msg) : throw(ErrorException("dec_hip status $status: $msg"))
+end
+
+dtype_code(::Type{Float64}) = DEC_F64
+dtype_code(::Type{Float32}) = DEC_F32
+hodge_code(::GeometricHodge) = DEC_GEOMETRIC_HODGE
+hodge_code(::DiagonalHodge) = DEC_DIAGONAL_HODGE
+stream_ptr() = AMDGPU.stream().stream
+
+# ---------------------------------------------------------------- meshes
+
+mutable struct HIPMesh
+  handle::Ptr{Cvoid}
+end
+
+const MESHES = WeakKeyDict{Any, HIPMesh}()
+
+"""Upload (once per dual complex) the primal boundary maps, orientations and
+points; the dual geometry is then computed on the GPU in Float64."""
+function hip_mesh(sd::EmbeddedDeltaDualComplex2D)
+  get!(MESHES, sd) do
+    nv_, ne_, nt_ = nv(sd), ne(sd), ntriangles(sd)
+    v0 = Int32.(sd[:∂v0]); v1 = Int32.(sd[:∂v1])
+    e0 = Int32.(sd[:∂e0]); e1 = Int32.(sd[:∂e1]); e2 = Int32.(sd[:∂e2])
+    eo = Int8[isnothing(o) ? -1 : Int8(o) for o in sd[:edge_orientation]]
+    to = Int8[isnothing(o) ? -1 : Int8(o) for o in sd[:tri_orientation]]
+    pts = Vector{Float64}(undef, 3 * nv_)
+    for (i, p) in enumerate(sd[:point])
+      pts[3i-2] = p[1]; pts[3i-1] = p[2]; pts[3i] = length(p) > 2 ? p[3] : 0.0
+    end
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:dec_mesh_create, libdec_hip), Cint,
+      (Int32, Int32, Int32, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32},
+       Ptr{Int8}, Ptr{Int8}, Ptr{Float64}, Int32, Ptr{Ptr{Cvoid}}),
+      nv_, ne_, nt_, v0, v1, e0, e1, e2, eo, to, pts, Int32(1), h))
+    check(ccall((:dec_mesh_dualize, libdec_hip), Cint, (Ptr{Cvoid}, Int32), h[], DEC_CIRCUMCENTER))
+    m = HIPMesh(h[])
+    finalizer(x -> ccall((:dec_mesh_destroy, libdec_hip), Cint, (Ptr{Cvoid},), x.handle), m)
+    m
+  end
+end
+
+# ---------------------------------------------------------------- operators
+
+mutable struct HIPOperator{T}
+  handle::Ptr{Cvoid}
+  rows::Int
+  cols::Int
+  cols2::Int
+end
+
+function HIPOperator{T}(sd, kind, m, n, hodge) where T
+  mesh = hip_mesh(sd)
+  h = Ref{Ptr{Cvoid}}(C_NULL)
+  check(ccall((:dec_op_create, libdec_hip), Cint,
+    (Ptr{Cvoid}, Int32, Int32, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
+    mesh.handle, Int32(kind), Int32(m), Int32(n), hodge, dtype_code(T), h))
+  r, c, c2 = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
+  check(ccall((:dec_op_shape, libdec_hip), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}),
+    h[], r, c, c2))
+  op = HIPOperator{T}(h[], r[], c[], c2[])
+  finalizer(x -> ccall((:dec_op_destroy, libdec_hip), Cint, (Ptr{Cvoid},), x.handle), op)
+  op
+end
+
+function apply!(res::ROCVector{T}, op::HIPOperator{T}, x::ROCVector{T}, y=nothing) where T
+  length(x) == op.cols || throw(DimensionMismatch("first form has length $(length(x)), expected $(op.cols)"))
+  op.cols2 > 0 && length(y) != op.cols2 && throw(DimensionMismatch("second form has wrong length"))
+  check(ccall((:dec_op_apply, libdec_hip), Cint,
+    (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{Cvoid}),
+    op.handle, pointer(x), isnothing(y) ? Ptr{T}(C_NULL) : pointer(y), pointer(res), stream_ptr()))
+  res
+end
+
+Base.:*(op::HIPOperator{T}, x::ROCVector{T}) where T = apply!(ROCVector{T}(undef, op.rows), op, x)
+Base.size(op::HIPOperator) = (op.rows, op.cols)
+
+float_of(sd::EmbeddedDeltaDualComplex2D{Bool, F, P}) where {F, P} = F
+
+# Wedge products: same closure protocol as FastDEC (src/FastDEC.jl:255-289).
+function cache_wedge(::Val{m}, ::Val{n}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP},
+                     arr_cons=ROCArray, cast_float=nothing) where {m, n}
+  T = isnothing(cast_float) ? float_of(sd) : cast_float
+  HIPOperator{T}(sd, DEC_OP_WEDGE, m, n, DEC_GEOMETRIC_HODGE)
+end
+
+function dec_wedge_product(::Val{m}, ::Val{n}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP},
+                           arr_cons=ROCArray, cast_float=nothing) where {m, n}
+  (m, n) == (0, 0) && return (f, g) -> f .* g
+  op = cache_wedge(Val(m), Val(n), sd, Val(:HIP), arr_cons, cast_float)
+  (α, β) -> apply!(similar(α, op.rows), op, α, β)
+end
+
+function dec_wedge_product_dd(::Val{m}, ::Val{n}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP},
+                              cast_float=nothing) where {m, n}
+  T = isnothing(cast_float) ? float_of(sd) : cast_float
+  op = HIPOperator{T}(sd, DEC_OP_WEDGE_DD, m, n, DEC_GEOMETRIC_HODGE)
+  (f, g) -> apply!(similar(f, op.rows), op, f, g)
+end
+
+# Sparse operators (ext/CombinatorialSpacesCUDAExt.jl:30-61).
+dec_boundary(n::Int, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) =
+  HIPOperator{float_of(sd)}(sd, DEC_OP_BOUNDARY, n, 0, DEC_GEOMETRIC_HODGE)
+dec_dual_derivative(n::Int, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) =
+  HIPOperator{float_of(sd)}(sd, DEC_OP_DUAL_DERIVATIVE, n, 0, DEC_GEOMETRIC_HODGE)
+dec_differential(n::Int, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) =
+  HIPOperator{float_of(sd)}(sd, DEC_OP_DIFFERENTIAL, n, 0, DEC_GEOMETRIC_HODGE)
+avg₀₁_mat(sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) =
+  HIPOperator{float_of(sd)}(sd, DEC_OP_AVG01, 0, 0, DEC_GEOMETRIC_HODGE)
+
+# Hodge stars (ext/CombinatorialSpacesCUDAExt.jl:70-96): diagonal stars stay
+# vectors on the device (no dense n×n matrix), geometric ⋆1 is a sparse operator.
+dec_hodge_star(n::Int, sd::EmbeddedDeltaDualComplex2D, h::DiscreteHodge, ::Val{:HIP}) =
+  HIPOperator{float_of(sd)}(sd, DEC_OP_HODGE, n, 0, hodge_code(h))
+dec_inv_hodge_star(n::Int, sd::EmbeddedDeltaDualComplex2D, h::DiscreteHodge, ::Val{:HIP}) =
+  n == 1 && h isa GeometricHodge ? inv_geometric_hodge1(sd) :
+  HIPOperator{float_of(sd)}(sd, DEC_OP_INV_HODGE, n, 0, hodge_code(h))
+
+# ⋆1⁻¹ geometric as a GMRES solve on the device operator, as the CUDA extension
+# does (Krylov.gmres only needs size, eltype and mul!).
+struct NegHodge{T}
+  op::HIPOperator{T}
+end
+Base.size(A::NegHodge) = (A.op.rows, A.op.cols)
+Base.size(A::NegHodge, i::Int) = size(A)[i]
+Base.eltype(::NegHodge{T}) where T = T
+function LinearAlgebra.mul!(y::ROCVector{T}, A::NegHodge{T}, x::ROCVector{T}) where T
+  apply!(y, A.op, x)
+  y .= -y
+end
+
+function inv_geometric_hodge1(sd)
+  A = NegHodge(HIPOperator{float_of(sd)}(sd, DEC_OP_HODGE, 1, 0, DEC_GEOMETRIC_HODGE))
+  x -> Krylov.gmres(A, x, atol = 1e-14)[1]
+end
+
+# ---------------------------------------------------------------- multigrid
+
+mutable struct HIPMultigridData
+  handle::Ptr{Cvoid}
+  n::Int
+  steps::Int
+end
+
+"""HIP counterpart of `MultigridData(s, BinarySubdivision(), levels, op, steps)`
+(src/Multigrid.jl:698-702) with op = fast_laplace_beltrami using `hodge` for ⋆1."""
+function HIPMultigridData(s::EmbeddedDeltaSet2D, ::BinarySubdivision, levels::Int, steps::Int;
+                          hodge::DiscreteHodge=GeometricHodge(), project::Bool=false)
+  base = hip_mesh(dualize(s, Circumcenter()))
+  h = Ref{Ptr{Cvoid}}(C_NULL)
+  check(ccall((:dec_mg_create, libdec_hip), Cint,
+    (Ptr{Cvoid}, Int32, Int32, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
+    base.handle, Int32(levels), Int32(project), hodge_code(hodge), DEC_CIRCUMCENTER, Int32(steps), h))
+  n = Ref{Int32}(0)
+  check(ccall((:dec_mg_level_size, libdec_hip), Cint, (Ptr{Cvoid}, Int32, Ref{Int32}), h[], 0, n))
+  md = HIPMultigridData(h[], n[], steps)
+  finalizer(x -> ccall((:dec_mg_destroy, libdec_hip), Cint, (Ptr{Cvoid},), x.handle), md)
+  md
+end
+
+smoother_code(alg) = alg === Krylov.cg ? DEC_SMOOTH_CG :
+  alg === :gauss_seidel ? DEC_SMOOTH_GS : alg === :jacobi ? DEC_SMOOTH_JACOBI :
+  error("unsupported smoother $alg")
+
+function set_smoother!(md::HIPMultigridData, alg; omega=2/3, coarse_direct=false)
+  check(ccall((:dec_mg_set_smoother, libdec_hip), Cint, (Ptr{Cvoid}, Int32, Int32, Float64, Int32),
+    md.handle, smoother_code(alg), Int32(md.steps), omega, Int32(coarse_direct)))
+end
+
+function multigrid_μ_cycles(u::ROCVector{Float64}, b::ROCVector{Float64}, md::HIPMultigridData,
+                            cycles, alg, μ)
+  set_smoother!(md, alg)
+  u = copy(u)
+  check(ccall((:dec_mg_cycles, libdec_hip), Cint,
+    (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+    md.handle, Int32(μ), Int32(cycles), pointer(u), pointer(b), stream_ptr()))
+  u
+end
+
+multigrid_vcycles(u::ROCVector{Float64}, b::ROCVector{Float64}, md::HIPMultigridData, cycles, alg=Krylov.cg) =
+  multigrid_μ_cycles(u, b, md, cycles, alg, 1)
+multigrid_wcycles(u::ROCVector{Float64}, b::ROCVector{Float64}, md::HIPMultigridData, cycles, alg=Krylov.cg) =
+  multigrid_μ_cycles(u, b, md, cycles, alg, 2)
+
+function full_multigrid(b::ROCVector{Float64}, md::HIPMultigridData, cycles, alg=Krylov.cg, μ=1)
+  set_smoother!(md, alg)
+  u = similar(b)
+  check(ccall((:dec_mg_full_multigrid, libdec_hip), Cint,
+    (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+    md.handle, Int32(μ), Int32(cycles), pointer(b), pointer(u), stream_ptr()))
+  u
+end
+
+end

@@ -27,6 +27,7 @@ extern "C" {
 typedef struct dec_mesh_s *dec_mesh;
 typedef struct dec_op_s *dec_op;
 typedef struct dec_mg_s *dec_mg;
+typedef struct dec_batch_s *dec_batch;
 
 enum dec_status {
   DEC_OK = 0,
@@ -167,6 +168,28 @@ int dec_mg_solve(dec_mg mg, int32_t variant, double tol, int32_t max_cycles,
                  void *u, const void *b, int32_t *cycles_done, double *relres,
                  void *stream);
 int dec_mg_destroy(dec_mg mg);
+
+/* ------------------------------------------------------------------ batches */
+/* Many explicit heat-equation instances u_t = Δu on meshes of uneven size,
+ * stepped together: u ← u - dt L u with L the 0-form Laplace-Beltrami
+ * operator (fast_laplace_beltrami with the diagonal Hodge, test/Multigrid.jl:46-48)
+ * and dt = 0.1 h², h the mesh's mean edge length.  No counterpart in the
+ * reference, which applies its operators one mesh at a time
+ * (FastDEC.jl:208-238).  meshes[i] is instance i's mesh (repeats share one
+ * operator).  The state is one device vector holding the instances back to
+ * back; dec_batch_layout returns the offsets (ninst + 1 entries) and steps. */
+int dec_batch_create(int32_t ninst, const dec_mesh *meshes, int32_t dtype, dec_batch *out);
+int dec_batch_layout(dec_batch b, int64_t *total, int64_t *offsets, double *dt);
+/* Autotune the launch shape (block size, rows per thread, workgroups per CU)
+ * on the batch itself, starting from the state u (left unchanged); the winner
+ * is cached in the text file cache_path (NULL: no file) keyed by the batch shape. */
+int dec_batch_tune(dec_batch b, void *u, const char *cache_path, int32_t *block,
+                   int32_t *rows_per_thread, int32_t *grid_mult);
+/* nsteps explicit Euler steps of every instance, one persistent launch per
+ * step; use_graph=1 captures the nsteps launches in a hipGraph (kept for the
+ * same state pointer and step count) and replays it. */
+int dec_batch_heat_steps(dec_batch b, void *u, int32_t nsteps, int32_t use_graph, void *stream);
+int dec_batch_destroy(dec_batch b);
 
 #ifdef __cplusplus
 }

@@ -38,7 +38,9 @@ def test_hierarchy_matches_oracle(which, grid_pair, sphere_pair):
     H, md = grid_pair if which == "grid" else sphere_pair
     assert md.nlevels == len(H.A) and md.sizes == [a.m for a in H.A]
     for lv in range(md.nlevels):
-        csr_equal(md.export(lv, "A"), H.A[lv].csr(), exact=(which == "sphere"))
+        # operators depend on circumcenters (closed form here, elimination in
+        # the oracle): equal to rounding; R and P are exact
+        csr_equal(md.export(lv, "A"), H.A[lv].csr(), exact=False)
         if lv + 1 < md.nlevels:
             csr_equal(md.export(lv, "R"), H.R[lv].csr())
             csr_equal(md.export(lv, "P"), H.S[lv].transpose().csr())
