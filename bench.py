@@ -74,9 +74,11 @@ def make_op(cs, mesh, name, td):
     raise KeyError(name)
 
 
-def time_opset(torch, cs, mesh, names, td, sets, steps, warmup):
-    """Per-op mean launch duration (ms) with HIP events recorded on the stream
-    the kernels run on, rotating through `sets` independent table/buffer sets."""
+def time_opset(torch, cs, mesh, names, td, sets, steps, warmup, cache_resident=True):
+    """Per-op mean launch duration (ms): `steps` back-to-back launches of the
+    op over `sets` independent table/buffer sets, between two HIP events
+    recorded on the stream the kernels run on.  Then the whole step (every op
+    once per step, rotating sets) timed the same way."""
     stream = torch.cuda.current_stream()
     ops = {n: [make_op(cs, mesh, n, td) for _ in range(sets)] for n in names}
     bufs = {}
@@ -90,46 +92,48 @@ def time_opset(torch, cs, mesh, names, td, sets, steps, warmup):
             seed += 1
             out = torch.empty(op.rows, dtype=td, device="cuda")
             bufs[n].append((x, y, out))
+
+    def launch(n, s):
+        x, y, out = bufs[n][s]
+        ops[n][s].apply(x, y, out=out, stream=stream)
+
     for w in range(warmup):
-        s = w % sets
         for n in names:
-            x, y, out = bufs[n][s]
-            ops[n][s].apply(x, y, out=out, stream=stream)
+            launch(n, w % sets)
     torch.cuda.synchronize()
-    ev = {n: [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-          for n in names}
-    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    per = {}
+    for n in names:
+        a, b = ev(), ev()
+        a.record(stream)
+        for k in range(steps):
+            launch(n, k % sets)
+        b.record(stream)
+        torch.cuda.synchronize()
+        per[n] = a.elapsed_time(b) / steps
+    t0, t1 = ev(), ev()
     t0.record(stream)
     for k in range(steps):
-        s = k % sets
         for n in names:
-            x, y, out = bufs[n][s]
-            a, b = ev[n][k]
-            a.record(stream)
-            ops[n][s].apply(x, y, out=out, stream=stream)
-            b.record(stream)
+            launch(n, k % sets)
     t1.record(stream)
     torch.cuda.synchronize()
-    per = {n: sum(a.elapsed_time(b) for a, b in ev[n]) / steps for n in names}
     total = t0.elapsed_time(t1)
     lap_nnz = None
     if "laplacian0" in names:
         lap_nnz = len(ops["laplacian0"][0].csr()[1])
-    # cache-resident figure: one set, back-to-back
     cache = {}
-    for n in names:
-        x, y, out = bufs[n][0]
-        op = ops[n][0]
-        for _ in range(3):
-            op.apply(x, y, out=out, stream=stream)
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        for _ in range(20):
-            op.apply(x, y, out=out, stream=stream)
-        b.record(stream)
-        torch.cuda.synchronize()
-        cache[n] = a.elapsed_time(b) / 20
+    if cache_resident:   # one set, back-to-back: the Infinity-Cache-resident figure
+        for n in names:
+            for _ in range(3):
+                launch(n, 0)
+            a, b = ev(), ev()
+            a.record(stream)
+            for _ in range(20):
+                launch(n, 0)
+            b.record(stream)
+            torch.cuda.synchronize()
+            cache[n] = a.elapsed_time(b) / 20
     del ops, bufs
     torch.cuda.empty_cache()
     return per, total, cache, lap_nnz
@@ -253,13 +257,15 @@ def cpu_baseline(k=7, budget_s=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--sets", type=int, default=6)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the ops outside the headline set")
+    ap.add_argument("--no-cache-resident", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -280,14 +286,14 @@ def main():
           flush=True)
 
     per, total, cache, lap_nnz = time_opset(torch, cs, mesh, OPSET, torch.float64, args.sets, args.steps,
-                                            args.warmup)
+                                            args.warmup, not args.no_cache_resident)
     B = {n: alg_bytes(n, nv, ne, nt, 8, lap_nnz) for n in OPSET}
     step_bytes = sum(B.values())
     ms_per_step = total / args.steps
     value = step_bytes / (ms_per_step * 1e-3) / 1e9
     ops = {n: {"ms": per[n], "MB": B[n] / 1e6, "GBps": B[n] / (per[n] * 1e-3) / 1e9,
                "frac_hbm_peak": B[n] / (per[n] * 1e-3) / 1e9 / HBM_PEAK_GBPS,
-               "cache_resident_GBps": B[n] / (cache[n] * 1e-3) / 1e9} for n in OPSET}
+               "cache_resident_GBps": (B[n] / (cache[n] * 1e-3) / 1e9) if n in cache else None} for n in OPSET}
     print(f"# f64 op set: {value:.0f} GB/s aggregate", file=sys.stderr, flush=True)
     dom = max(OPSET, key=lambda n: per[n])
     kernel_name = {"wedge01": "k_wedge01", "wedge11": "k_wedge11", "wedge02": "k_wedge02", "d0": "k_d0",
@@ -299,13 +305,14 @@ def main():
 
     extra = {}
     extra_names = ["wedge_dd01", "boundary1", "boundary2"]
-    per_x, _, cache_x, _ = time_opset(torch, cs, mesh, extra_names, torch.float64, args.sets, max(10, args.steps // 2),
-                                      2)
-    extra["other_ops_f64"] = {n: {"ms": per_x[n], "GBps": alg_bytes(n, nv, ne, nt, 8) / (per_x[n] * 1e-3) / 1e9}
-                              for n in extra_names}
+    if not args.no_extra:
+        per_x, _, _, _ = time_opset(torch, cs, mesh, extra_names, torch.float64, args.sets,
+                                    max(10, args.steps // 2), 2, False)
+        extra["other_ops_f64"] = {n: {"ms": per_x[n], "GBps": alg_bytes(n, nv, ne, nt, 8) / (per_x[n] * 1e-3) / 1e9}
+                                  for n in extra_names}
     if not args.no_f32:
         per32, total32, cache32, lap32 = time_opset(torch, cs, mesh, OPSET, torch.float32, args.sets, args.steps,
-                                                    args.warmup)
+                                                    args.warmup, False)
         B32 = {n: alg_bytes(n, nv, ne, nt, 4, lap32) for n in OPSET}
         extra["f32"] = {"value_GBps": sum(B32.values()) / (total32 / args.steps * 1e-3) / 1e9,
                         "ops": {n: {"ms": per32[n], "GBps": B32[n] / (per32[n] * 1e-3) / 1e9} for n in OPSET}}

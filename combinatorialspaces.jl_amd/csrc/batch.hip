@@ -18,6 +18,7 @@
 #include <string>
 
 #include "dec_internal.h"
+#include "ell_rows.h"
 #include "kernels_common.h"
 
 using dec::DBuf;
@@ -69,13 +70,8 @@ __global__ void __launch_bounds__(1024) k_heat_step(int nitems, const Item *__re
     for (int j = 0; j < RPT; j++) {
       int r = w.row0 + j * blockDim.x + threadIdx.x;
       if (r >= w.row0 + w.rows) break;
-      T acc = static_cast<T>(0);
-      for (int k = 0; k < O.width; k++) {
-        size_t q = static_cast<size_t>(k) * O.n + r;
-        int c = O.col[q];
-        if (c < 0) break;
-        acc = acc + O.val[q] * u[c];
-      }
+      T acc = O.width == 7 ? dec_ell::row<7, T, T>(O.n, O.width, r, O.col, O.val, u)
+                           : dec_ell::row<0, T, T>(O.n, O.width, r, O.col, O.val, u);
       v[r] = u[r] - dt * acc;
     }
   }

@@ -1,0 +1,105 @@
+// Row sums over a slot-major ELL (entry k of row r at [k * n + r], col < 0
+// pads), in ascending column order (the order of a CSC SpMV in SparseArrays).
+// The width is a template parameter when it is small: all loads of a row are
+// issued before the sum, which keeps many gathers in flight per wavefront; the
+// padding is predicated, not branched on.  W == 0 selects the runtime-width loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace dec_ell {
+
+template <class V, class T>
+__device__ __forceinline__ T mul(V v, T x) {
+  return static_cast<T>(v) * x;
+}
+template <>
+__device__ __forceinline__ double mul<int8_t, double>(int8_t v, double x) {
+  return v > 0 ? x : -x;
+}
+template <>
+__device__ __forceinline__ float mul<int8_t, float>(int8_t v, float x) {
+  return v > 0 ? x : -x;
+}
+
+template <int W, class T, class V>
+__device__ __forceinline__ T row(int n, int width, int r, const int *__restrict__ col, const V *__restrict__ val,
+                                 const T *__restrict__ x) {
+  T acc = static_cast<T>(0);
+  if constexpr (W > 0) {
+    int c[W];
+    V v[W];
+    T xv[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      size_t q = static_cast<size_t>(k) * n + r;
+      c[k] = col[q];
+      v[k] = val[q];
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) xv[k] = x[c[k] < 0 ? 0 : c[k]];
+#pragma unroll
+    for (int k = 0; k < W; k++)
+      if (c[k] >= 0) acc = acc + mul<V, T>(v[k], xv[k]);
+  } else {
+    for (int k = 0; k < width; k++) {
+      size_t q = static_cast<size_t>(k) * n + r;
+      int c = col[q];
+      if (c < 0) break;
+      acc = acc + mul<V, T>(val[q], x[c]);
+    }
+  }
+  return acc;
+}
+
+// Gauss-Seidel row: s = Σ_{c ≠ self} a x[c] in ascending order, d = diagonal.
+template <int W, class T>
+__device__ __forceinline__ void gs_row(int n, int width, int p, int self, const int *__restrict__ col,
+                                       const T *__restrict__ val, const T *__restrict__ x, T &s, T &d) {
+  s = static_cast<T>(0);
+  d = static_cast<T>(0);
+  if constexpr (W > 0) {
+    int c[W];
+    T v[W], xv[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      size_t q = static_cast<size_t>(k) * n + p;
+      c[k] = col[q];
+      v[k] = val[q];
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) xv[k] = x[c[k] < 0 ? 0 : c[k]];
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      if (c[k] == self) d = v[k];
+      else if (c[k] >= 0) s = s + v[k] * xv[k];
+    }
+  } else {
+    for (int k = 0; k < width; k++) {
+      size_t q = static_cast<size_t>(k) * n + p;
+      int c = col[q];
+      if (c < 0) break;
+      if (c == self) d = val[q];
+      else s = s + val[q] * x[c];
+    }
+  }
+}
+
+}  // namespace dec_ell
+
+// Instantiate `LAUNCH` (a statement using the constexpr int W) for the ELL width.
+#define DEC_ELL_WIDTH_SWITCH(width, LAUNCH)   \
+  switch (width) {                            \
+    case 1: { constexpr int W = 1; LAUNCH; } break; \
+    case 2: { constexpr int W = 2; LAUNCH; } break; \
+    case 3: { constexpr int W = 3; LAUNCH; } break; \
+    case 4: { constexpr int W = 4; LAUNCH; } break; \
+    case 5: { constexpr int W = 5; LAUNCH; } break; \
+    case 6: { constexpr int W = 6; LAUNCH; } break; \
+    case 7: { constexpr int W = 7; LAUNCH; } break; \
+    case 8: { constexpr int W = 8; LAUNCH; } break; \
+    default: { constexpr int W = 0; LAUNCH; } break; \
+  }

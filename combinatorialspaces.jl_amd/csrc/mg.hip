@@ -10,9 +10,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <memory>
 
 #include "dec_internal.h"
+#include "ell_rows.h"
 #include "kernels_common.h"
 
 using dec::DBuf;
@@ -32,79 +34,55 @@ __global__ void k_zero(int n, T *__restrict__ x) {
 }
 
 // r[v] = b[v] - (A u)[v], row sums in ascending column order (b - A*u).
-template <class T>
+template <int W, class T>
 __global__ void k_residual(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
                            const int *__restrict__ rowid, const T *__restrict__ b, const T *__restrict__ u,
                            T *__restrict__ r) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  T acc = static_cast<T>(0);
-  for (int k = 0; k < width; k++) {
-    size_t q = static_cast<size_t>(k) * n + p;
-    int c = col[q];
-    if (c < 0) break;
-    acc = acc + val[q] * u[c];
-  }
+  T acc = dec_ell::row<W, T, T>(n, width, p, col, val, u);
   int v = rowid[p];
   r[v] = b[v] - acc;
 }
 
 // y[v] = (A x)[v]
-template <class T>
+template <int W, class T>
 __global__ void k_spmv_rows(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
                             const int *__restrict__ rowid, const T *__restrict__ x, T *__restrict__ y,
                             const int *__restrict__ done) {
   if (done && *done) return;
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  T acc = static_cast<T>(0);
-  for (int k = 0; k < width; k++) {
-    size_t q = static_cast<size_t>(k) * n + p;
-    int c = col[q];
-    if (c < 0) break;
-    acc = acc + val[q] * x[c];
-  }
-  y[rowid[p]] = acc;
+  y[rowid[p]] = dec_ell::row<W, T, T>(n, width, p, col, val, x);
 }
 
 // One colour of Gauss-Seidel: u[v] = (b[v] - Σ_{w≠v} A[v,w] u[w]) / A[v,v].
-template <class T>
+template <int W, class T>
 __global__ void k_gs_color(int p0, int p1, int n, int width, const int *__restrict__ col,
                            const T *__restrict__ val, const int *__restrict__ rowid, const T *__restrict__ b,
                            T *__restrict__ u) {
   int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= p1) return;
   int v = rowid[p];
-  T s = static_cast<T>(0), d = static_cast<T>(0);
-  for (int k = 0; k < width; k++) {
-    size_t q = static_cast<size_t>(k) * n + p;
-    int c = col[q];
-    if (c < 0) break;
-    T a = val[q];
-    if (c == v) d = a;
-    else s = s + a * u[c];
-  }
+  T s, d;
+  dec_ell::gs_row<W, T>(n, width, p, v, col, val, u, s, d);
   u[v] = (b[v] - s) / d;
 }
 
 // Damped Jacobi: un[v] = u[v] + ω (b[v] - (A u)[v]) / A[v,v].
-template <class T>
+template <int W, class T>
 __global__ void k_jacobi(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
                          const int *__restrict__ rowid, const T *__restrict__ b, const T *__restrict__ u,
                          T *__restrict__ un, T omega) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   int v = rowid[p];
-  T s = static_cast<T>(0), d = static_cast<T>(0);
-  for (int k = 0; k < width; k++) {
-    size_t q = static_cast<size_t>(k) * n + p;
-    int c = col[q];
-    if (c < 0) break;
-    T a = val[q];
-    if (c == v) d = a;
-    s = s + a * u[c];
-  }
-  un[v] = u[v] + omega * ((b[v] - s) / d);
+  T s, d;
+  dec_ell::gs_row<W, T>(n, width, p, v, col, val, u, s, d);
+  // (A u)[v] in ascending column order: the diagonal term enters at its place
+  T full = dec_ell::row<W, T, T>(n, width, p, col, val, u);
+  (void)s;
+  un[v] = u[v] + omega * ((b[v] - full) / d);
 }
 
 template <class T>
@@ -115,19 +93,11 @@ __global__ void k_copy(int n, const T *__restrict__ x, T *__restrict__ y) {
 
 // b_c = R r_f; rows are coarse vertices in natural order (CSC order of R:
 // the identity column first, then the midpoint columns ascending).
-template <class T>
+template <int W, class T>
 __global__ void k_restrict(int nc, int width, const int *__restrict__ col, const T *__restrict__ val,
                            const T *__restrict__ rf, T *__restrict__ bc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nc) return;
-  T acc = static_cast<T>(0);
-  for (int k = 0; k < width; k++) {
-    size_t q = static_cast<size_t>(k) * nc + i;
-    int c = col[q];
-    if (c < 0) break;
-    acc = acc + val[q] * rf[c];
-  }
-  bc[i] = acc;
+  if (i < nc) bc[i] = dec_ell::row<W, T, T>(nc, width, i, col, val, rf);
 }
 
 // u_f += P z_c with P = transpose(S): coarse vertices copy, midpoints average
@@ -147,17 +117,15 @@ __global__ void k_prolong_add(int nf, int nc, const int *__restrict__ pa, const 
   uf[j] = uf[j] + pz;
 }
 
-// u += Z r, one 64-lane wavefront per row, fixed reduction order.
+// u += Z r with Z stored column-major (ZT[j * n + i] = Z[i, j]): one thread
+// per row, sequential sum over j (coalesced across the rows of a wavefront).
 template <class T>
-__global__ void k_gemv_add(int n, const T *__restrict__ Z, const T *__restrict__ r, T *__restrict__ u) {
-  int lane = threadIdx.x & 63;
-  int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (row >= n) return;
-  const T *z = Z + static_cast<size_t>(row) * n;
+__global__ void k_gemv_add(int n, const T *__restrict__ ZT, const T *__restrict__ r, T *__restrict__ u) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   T acc = static_cast<T>(0);
-  for (int j = lane; j < n; j += 64) acc = acc + z[j] * r[j];
-  for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
-  if (lane == 0) u[row] = u[row] + acc;
+  for (int j = 0; j < n; j++) acc = acc + ZT[static_cast<size_t>(j) * n + i] * r[j];
+  u[i] = u[i] + acc;
 }
 
 template <class T>
@@ -276,13 +244,134 @@ __global__ void k_add_f2d(int n, const float *__restrict__ c, double *__restrict
   if (i < n) u[i] = u[i] + static_cast<double>(c[i]);
 }
 
+
+// ------------------------------------------------------------- fused coarse tail
+// The levels whose size is at most kFuseRows run as one workgroup of 1024
+// threads: every phase of the V-cycle below that level (colour sweeps,
+// residual, restriction, the dense coarse solve, prolongation) is a strided
+// loop over the level's rows followed by a workgroup barrier, in place of one
+// kernel launch per phase.  Per-row arithmetic is the same as the multi-kernel
+// path (same expressions, same order).
+constexpr int kFuseRows = 16384;
+constexpr int kFuseThreads = 1024;
+constexpr int kMaxColors = 16;
+
+template <class T>
+struct LevelDesc {
+  int n, width, ncolors, nc, rwidth;
+  int cstart[kMaxColors + 1];
+  const int *col, *rowid, *rcol, *pa, *pb;
+  const T *val, *rval, *ZT;
+  T *u, *b, *r;
+};
+
+template <class T>
+__device__ __forceinline__ T fused_row(const LevelDesc<T> &L, int p, const T *x) {
+  return L.width == 7 ? dec_ell::row<7, T, T>(L.n, L.width, p, L.col, L.val, x)
+                      : dec_ell::row<0, T, T>(L.n, L.width, p, L.col, L.val, x);
+}
+
+template <class T>
+__device__ void fused_smooth(const LevelDesc<T> &L, const T *b, T *u, int steps, int alg, T omega, bool post) {
+  for (int it = 0; it < steps; it++) {
+    if (alg == DEC_SMOOTH_GS) {
+      for (int ci = 0; ci < L.ncolors; ci++) {
+        int c = post ? L.ncolors - 1 - ci : ci;
+        for (int p = L.cstart[c] + threadIdx.x; p < L.cstart[c + 1]; p += blockDim.x) {
+          int v = L.rowid[p];
+          T sum, d;
+          if (L.width == 7) dec_ell::gs_row<7, T>(L.n, L.width, p, v, L.col, L.val, u, sum, d);
+          else dec_ell::gs_row<0, T>(L.n, L.width, p, v, L.col, L.val, u, sum, d);
+          u[v] = (b[v] - sum) / d;
+        }
+        __syncthreads();
+      }
+    } else {
+      for (int p = threadIdx.x; p < L.n; p += blockDim.x) {
+        int v = L.rowid[p];
+        T sum, d;
+        if (L.width == 7) dec_ell::gs_row<7, T>(L.n, L.width, p, v, L.col, L.val, u, sum, d);
+        else dec_ell::gs_row<0, T>(L.n, L.width, p, v, L.col, L.val, u, sum, d);
+        T full = fused_row(L, p, u);
+        L.r[v] = u[v] + omega * ((b[v] - full) / d);
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < L.n; i += blockDim.x) u[i] = L.r[i];
+      __syncthreads();
+    }
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kFuseThreads)
+    k_fused_vcycle(const LevelDesc<T> *__restrict__ Ls, int l0, int nl, T *u0, const T *b0, int steps, int alg,
+                   T omega, int direct) {
+  int last = l0;
+  for (int l = l0; l < nl; l++) {
+    const LevelDesc<T> &L = Ls[l];
+    T *u = l == l0 ? u0 : L.u;
+    const T *b = l == l0 ? b0 : L.b;
+    last = l;
+    if (l > l0) {
+      for (int i = threadIdx.x; i < L.n; i += blockDim.x) u[i] = static_cast<T>(0);
+      __syncthreads();
+    }
+    if (l == nl - 1 && direct) {
+      for (int p = threadIdx.x; p < L.n; p += blockDim.x) {
+        int v = L.rowid[p];
+        L.r[v] = b[v] - fused_row(L, p, u);
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < L.n; i += blockDim.x) {
+        T acc = static_cast<T>(0);
+        for (int j = 0; j < L.n; j++) acc = acc + L.ZT[static_cast<size_t>(j) * L.n + i] * L.r[j];
+        u[i] = u[i] + acc;
+      }
+      __syncthreads();
+      break;
+    }
+    fused_smooth(L, b, u, steps, alg, omega, false);
+    if (l == nl - 1) break;
+    for (int p = threadIdx.x; p < L.n; p += blockDim.x) {
+      int v = L.rowid[p];
+      L.r[v] = b[v] - fused_row(L, p, u);
+    }
+    __syncthreads();
+    const LevelDesc<T> &C = Ls[l + 1];
+    for (int i = threadIdx.x; i < L.nc; i += blockDim.x) {
+      T acc = L.rwidth == 7 ? dec_ell::row<7, T, T>(L.nc, L.rwidth, i, L.rcol, L.rval, L.r)
+                            : dec_ell::row<0, T, T>(L.nc, L.rwidth, i, L.rcol, L.rval, L.r);
+      C.b[i] = acc;
+    }
+    __syncthreads();
+  }
+  for (int l = last - 1; l >= l0; l--) {
+    const LevelDesc<T> &L = Ls[l];
+    const LevelDesc<T> &C = Ls[l + 1];
+    T *u = l == l0 ? u0 : L.u;
+    const T *b = l == l0 ? b0 : L.b;
+    for (int j = threadIdx.x; j < L.n; j += blockDim.x) {
+      T pz;
+      if (j < L.nc) {
+        pz = C.u[j];
+      } else {
+        int e = j - L.nc;
+        pz = static_cast<T>(0.5) * C.u[L.pa[e]] + static_cast<T>(0.5) * C.u[L.pb[e]];
+      }
+      u[j] = u[j] + pz;
+    }
+    __syncthreads();
+    fused_smooth(L, b, u, steps, alg, omega, true);
+  }
+}
+
 // ---------------------------------------------------------------- storage
 
 template <class T>
 struct LevelVals {
   DBuf<T> val;   // ELL values of A (colour-sorted rows)
   DBuf<T> rval;  // ELL values of R (this level -> next coarser)
-  DBuf<T> Z;     // dense coarse operator (coarsest level, direct solve)
+  DBuf<T> Z;     // dense coarse operator, column-major (coarsest level, direct solve)
   DBuf<T> u, b, r, t;  // work vectors
 };
 
@@ -311,6 +400,24 @@ struct dec_mg_s {
   DBuf<double> part, scal;  // reduction partials, scalar outputs
   DBuf<CgState> cg;
   DBuf<double> cgx, cgp, cgAp;  // CG vectors (finest size)
+  int fuse_from = -1;           // first level handled by the fused coarse kernel
+  DBuf<LevelDesc<double>> desc64;
+  DBuf<LevelDesc<float>> desc32;
+  bool use_fused = true;
+  // graph of one solve cycle (per variant) for fixed u, b
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  const void *gu[2] = {nullptr, nullptr}, *gb[2] = {nullptr, nullptr};
+  int gkey[2] = {-1, -1};
+  hipStream_t cap = nullptr;
+  bool use_graph = true;
+  ~dec_mg_s() {
+    for (int i = 0; i < 2; i++) {
+      if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
+      if (graph[i]) (void)hipGraphDestroy(graph[i]);
+    }
+    if (cap) (void)hipStreamDestroy(cap);
+  }
 };
 
 namespace {
@@ -467,6 +574,9 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
   if (steps < 0) fail(DEC_ERR_ARG, "dec_mg_create: steps must be >= 0");
   auto mg = std::make_unique<dec_mg_s>();
   mg->steps = steps;
+  // debugging switches: run every phase as its own launch / launch cycles eagerly
+  mg->use_fused = std::getenv("DEC_MG_NO_FUSE") == nullptr;
+  mg->use_graph = std::getenv("DEC_MG_NO_GRAPH") == nullptr;
   // coarse -> fine walk (Multigrid.jl:752-797), stored finest first
   std::vector<std::unique_ptr<dec_mesh_s>> walk;
   walk.emplace_back(copy_mesh(base));
@@ -501,9 +611,52 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
     for (int r = 0; r < n; r++)
       for (int k = C.A.ptr[r]; k < C.A.ptr[r + 1]; k++) Ad[static_cast<size_t>(r) * n + C.A.col[k]] += C.A.val[k];
     dec::coarse_inverse(n, Ad, C.Zh);
-    C.d.Z.upload(C.Zh);
-    std::vector<float> zf(C.Zh.begin(), C.Zh.end());
+    std::vector<double> zt(C.Zh.size());
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) zt[static_cast<size_t>(j) * n + i] = C.Zh[static_cast<size_t>(i) * n + j];
+    C.d.Z.upload(zt);
+    std::vector<float> zf(zt.begin(), zt.end());
     C.f.Z.upload(zf);
+  }
+  // descriptors of the coarse tail for the fused kernel
+  for (int l = 0; l < nl; l++)
+    if (mg->lv[l]->n <= kFuseRows && mg->lv[l]->ncolors <= kMaxColors) {
+      bool ok = true;
+      for (int q = l; q < nl; q++) ok = ok && mg->lv[q]->n <= kFuseRows && mg->lv[q]->ncolors <= kMaxColors;
+      if (ok) {
+        mg->fuse_from = l;
+        break;
+      }
+    }
+  {
+    std::vector<LevelDesc<double>> d64(nl);
+    std::vector<LevelDesc<float>> d32(nl);
+    for (int l = 0; l < nl; l++) {
+      Level &L = *mg->lv[l];
+      auto fill = [&](auto &D, auto &V) {
+        D.n = L.n;
+        D.width = L.width;
+        D.ncolors = L.ncolors;
+        D.nc = L.nc;
+        D.rwidth = L.rwidth;
+        for (int c = 0; c <= kMaxColors; c++) D.cstart[c] = c < static_cast<int>(L.cstart.size()) ? L.cstart[c] : L.n;
+        D.col = L.col.p;
+        D.rowid = L.rowid.p;
+        D.rcol = L.rcol.p;
+        D.pa = L.pa.p;
+        D.pb = L.pb.p;
+        D.val = V.val.p;
+        D.rval = V.rval.p;
+        D.ZT = V.Z.p;
+        D.u = V.u.p;
+        D.b = V.b.p;
+        D.r = V.r.p;
+      };
+      fill(d64[l], L.d);
+      fill(d32[l], L.f);
+    }
+    mg->desc64.upload(d64);
+    mg->desc32.upload(d32);
   }
   mg->part.alloc(kDotBlocks);
   mg->scal.alloc(8);
@@ -522,7 +675,8 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
 template <class T>
 void residual(Level &L, const T *b, const T *u, T *r, hipStream_t s) {
   LevelVals<T> &V = vals<T>(L);
-  hipLaunchKernelGGL(k_residual<T>, G(L.n), dim3(kBlock), 0, s, L.n, L.width, L.col.p, V.val.p, L.rowid.p, b, u, r);
+  DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_residual<W, T>), G(L.n), dim3(kBlock), 0, s, L.n, L.width,
+                                                   L.col.p, V.val.p, L.rowid.p, b, u, r));
 }
 
 void dot(dec_mg_s &mg, int n, const double *x, const double *y, double *out, const int *done, hipStream_t s) {
@@ -542,8 +696,8 @@ void cg_smooth(dec_mg_s &mg, Level &L, const double *b, double *u, int steps, hi
   dot(mg, n, r, r, dotp, nullptr, s);
   hipLaunchKernelGGL(k_cg_init_scalars, dim3(1), dim3(1), 0, s, st);
   for (int it = 0; it < steps; it++) {
-    hipLaunchKernelGGL(k_spmv_rows<double>, G(n), dim3(kBlock), 0, s, n, L.width, L.col.p, L.d.val.p, L.rowid.p, p,
-                       Ap, done);
+    DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_spmv_rows<W, double>), G(n), dim3(kBlock), 0, s, n, L.width,
+                                                     L.col.p, L.d.val.p, L.rowid.p, p, Ap, done));
     dot(mg, n, p, Ap, dotp, done, s);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1), 0, s, st);
     hipLaunchKernelGGL(k_cg_xr, G(n), dim3(kBlock), 0, s, st, n, x, p, r, Ap);
@@ -570,12 +724,13 @@ void smooth(dec_mg_s &mg, Level &L, const T *b, T *u, bool post, hipStream_t s) 
         int c = post ? L.ncolors - 1 - ci : ci;
         int p0 = L.cstart[c], p1 = L.cstart[c + 1];
         if (p1 > p0)
-          hipLaunchKernelGGL(k_gs_color<T>, G(p1 - p0), dim3(kBlock), 0, s, p0, p1, L.n, L.width, L.col.p, V.val.p,
-                             L.rowid.p, b, u);
+          DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_gs_color<W, T>), G(p1 - p0), dim3(kBlock), 0, s, p0, p1,
+                                                           L.n, L.width, L.col.p, V.val.p, L.rowid.p, b, u));
       }
     } else {
-      hipLaunchKernelGGL(k_jacobi<T>, G(L.n), dim3(kBlock), 0, s, L.n, L.width, L.col.p, V.val.p, L.rowid.p, b, u,
-                         V.t.p, static_cast<T>(mg.omega));
+      DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_jacobi<W, T>), G(L.n), dim3(kBlock), 0, s, L.n, L.width,
+                                                     L.col.p, V.val.p, L.rowid.p, b, u, V.t.p,
+                                                     static_cast<T>(mg.omega)));
       hipLaunchKernelGGL(k_copy<T>, G(L.n), dim3(kBlock), 0, s, L.n, V.t.p, u);
     }
   }
@@ -586,9 +741,7 @@ void coarse_solve(dec_mg_s &mg, Level &L, const T *b, T *u, hipStream_t s) {
   LevelVals<T> &V = vals<T>(L);
   if (!V.Z.p) fail(DEC_ERR_UNSUPPORTED, "coarsest level too large for the dense direct solve");
   residual<T>(L, b, u, V.r.p, s);
-  int rows_per_block = kBlock / 64;
-  hipLaunchKernelGGL(k_gemv_add<T>, dim3((L.n + rows_per_block - 1) / rows_per_block), dim3(kBlock), 0, s, L.n,
-                     V.Z.p, V.r.p, u);
+  hipLaunchKernelGGL(k_gemv_add<T>, G(L.n), dim3(kBlock), 0, s, L.n, V.Z.p, V.r.p, u);
   (void)mg;
 }
 
@@ -597,6 +750,15 @@ template <class T>
 void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, hipStream_t s) {
   int nl = static_cast<int>(mg.lv.size());
   Level &L = *mg.lv[lvl];
+  if (mg.use_fused && mu == 1 && mg.alg != DEC_SMOOTH_CG && mg.fuse_from >= 0 && lvl >= mg.fuse_from &&
+      (mg.coarse_direct == 0 || vals<T>(*mg.lv.back()).Z.p)) {
+    const LevelDesc<T> *D;
+    if constexpr (sizeof(T) == 8) D = mg.desc64.p;
+    else D = mg.desc32.p;
+    hipLaunchKernelGGL(k_fused_vcycle<T>, dim3(1), dim3(kFuseThreads), 0, s, D, lvl, nl, u, b, mg.steps, mg.alg,
+                       static_cast<T>(mg.omega), mg.coarse_direct);
+    return;
+  }
   if (lvl == nl - 1 && mg.coarse_direct) {
     coarse_solve<T>(mg, L, b, u, s);
     return;
@@ -607,7 +769,8 @@ void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, hipStream_t s) {
   LevelVals<T> &V = vals<T>(L);
   LevelVals<T> &CV = vals<T>(C);
   residual<T>(L, b, u, V.r.p, s);
-  hipLaunchKernelGGL(k_restrict<T>, G(L.nc), dim3(kBlock), 0, s, L.nc, L.rwidth, L.rcol.p, V.rval.p, V.r.p, CV.b.p);
+  DEC_ELL_WIDTH_SWITCH(L.rwidth, hipLaunchKernelGGL((k_restrict<W, T>), G(L.nc), dim3(kBlock), 0, s, L.nc, L.rwidth,
+                                                    L.rcol.p, V.rval.p, V.r.p, CV.b.p));
   hipLaunchKernelGGL(k_zero<T>, G(C.n), dim3(kBlock), 0, s, C.n, CV.u.p);
   mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu, s);
   if (mu > 1) mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu - 1, s);
@@ -622,8 +785,8 @@ void fmg_rec(dec_mg_s &mg, int lvl, const double *b, double *z, int mu, int cycl
   hipLaunchKernelGGL(k_zero<double>, G(L.n), dim3(kBlock), 0, s, L.n, z);
   if (lvl < nl - 1) {
     Level &C = *mg.lv[lvl + 1];
-    hipLaunchKernelGGL(k_restrict<double>, G(L.nc), dim3(kBlock), 0, s, L.nc, L.rwidth, L.rcol.p, L.d.rval.p, b,
-                       C.d.b.p);
+    DEC_ELL_WIDTH_SWITCH(L.rwidth, hipLaunchKernelGGL((k_restrict<W, double>), G(L.nc), dim3(kBlock), 0, s, L.nc,
+                                                      L.rwidth, L.rcol.p, L.d.rval.p, b, C.d.b.p));
     fmg_rec(mg, lvl + 1, C.d.b.p, C.d.u.p, mu, cycles, s);
     hipLaunchKernelGGL(k_prolong_add<double>, G(L.n), dim3(kBlock), 0, s, L.n, L.nc, L.pa.p, L.pb.p, C.d.u.p, z);
   }
@@ -638,6 +801,50 @@ double norm2(dec_mg_s &mg, int n, const double *x, hipStream_t s) {
       hipStreamSynchronize(s) != hipSuccess)
     fail(DEC_ERR_HIP, "norm: copy failed");
   return std::sqrt(v);
+}
+
+void cycle_body(dec_mg_s &mg, int variant, double *u, const double *b, hipStream_t s) {
+  Level &F = *mg.lv[0];
+  int n = F.n;
+  if (variant == DEC_SOLVE_PLAIN_F64) {
+    mu_cycle<double>(mg, 0, u, b, 1, s);
+  } else {
+    // float32 V-cycle on the float64 residual, correction added in float64
+    hipLaunchKernelGGL(k_d2f, G(n), dim3(kBlock), 0, s, n, F.d.r.p, F.f.b.p);
+    hipLaunchKernelGGL(k_zero<float>, G(n), dim3(kBlock), 0, s, n, F.f.u.p);
+    mu_cycle<float>(mg, 0, F.f.u.p, F.f.b.p, 1, s);
+    hipLaunchKernelGGL(k_add_f2d, G(n), dim3(kBlock), 0, s, n, F.f.u.p, u);
+  }
+}
+
+void gcheck(hipError_t e, const char *what) {
+  if (e != hipSuccess) fail(DEC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// One cycle of the solve, replayed from a hipGraph captured for (u, b) and
+// the current smoother settings.
+void run_cycle(dec_mg_s &mg, int variant, double *u, const double *b, hipStream_t s) {
+  if (!mg.use_graph) {
+    cycle_body(mg, variant, u, b, s);
+    return;
+  }
+  int key = mg.alg * 1000003 + mg.steps * 1009 + mg.coarse_direct * 7 + static_cast<int>(mg.omega * 1e6);
+  int i = variant == DEC_SOLVE_PLAIN_F64 ? 0 : 1;
+  if (!mg.gexec[i] || mg.gu[i] != u || mg.gb[i] != b || mg.gkey[i] != key) {
+    if (mg.gexec[i]) (void)hipGraphExecDestroy(mg.gexec[i]);
+    if (mg.graph[i]) (void)hipGraphDestroy(mg.graph[i]);
+    mg.gexec[i] = nullptr;
+    mg.graph[i] = nullptr;
+    if (!mg.cap) gcheck(hipStreamCreateWithFlags(&mg.cap, hipStreamNonBlocking), "hipStreamCreate");
+    gcheck(hipStreamBeginCapture(mg.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    cycle_body(mg, variant, u, b, mg.cap);
+    gcheck(hipStreamEndCapture(mg.cap, &mg.graph[i]), "hipStreamEndCapture");
+    gcheck(hipGraphInstantiate(&mg.gexec[i], mg.graph[i], nullptr, nullptr, 0), "hipGraphInstantiate");
+    mg.gu[i] = u;
+    mg.gb[i] = b;
+    mg.gkey[i] = key;
+  }
+  gcheck(hipGraphLaunch(mg.gexec[i], s), "hipGraphLaunch");
 }
 
 void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u, const double *b, int *done_cycles,
@@ -657,15 +864,7 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u, con
     residual<double>(F, b, u, F.d.r.p, s);
     rr = norm2(mg, n, F.d.r.p, s) / bn;
     if (rr <= tol || c >= max_cycles) break;
-    if (variant == DEC_SOLVE_PLAIN_F64) {
-      mu_cycle<double>(mg, 0, u, b, 1, s);
-    } else {
-      // float32 V-cycle on the float64 residual, correction added in float64
-      hipLaunchKernelGGL(k_d2f, G(n), dim3(kBlock), 0, s, n, F.d.r.p, F.f.b.p);
-      hipLaunchKernelGGL(k_zero<float>, G(n), dim3(kBlock), 0, s, n, F.f.u.p);
-      mu_cycle<float>(mg, 0, F.f.u.p, F.f.b.p, 1, s);
-      hipLaunchKernelGGL(k_add_f2d, G(n), dim3(kBlock), 0, s, n, F.f.u.p, u);
-    }
+    run_cycle(mg, variant, u, b, s);
     c++;
   }
   *done_cycles = c;

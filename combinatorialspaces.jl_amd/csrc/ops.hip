@@ -12,6 +12,7 @@
 #include <utility>
 
 #include "dec_internal.h"
+#include "ell_rows.h"
 #include "kernels_common.h"
 
 using dec::DBuf;
@@ -53,114 +54,188 @@ struct dec_op_s {
 
 namespace {
 
+// Apply kernels: each thread handles the pair of simplices (2i, 2i+1), loading
+// index and coefficient tables as int2 / T2 vectors; a trailing odd element is
+// handled by a scalar path.  Caller forms must be 2·sizeof(T)-aligned for the
+// vector path (checked at apply time; otherwise the scalar kernels run).
 template <class T>
-__global__ void k_wedge01(int ne, const int *__restrict__ v0, const int *__restrict__ v1,
-                          const T *__restrict__ f, const T *__restrict__ a, T *__restrict__ res) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  // wedge_kernel_01! (FastDEC.jl:159-163): half * α[i] * (f[p1] + f[p2])
-  res[e] = (static_cast<T>(0.5) * a[e]) * (f[v0[e]] + f[v1[e]]);
-}
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <class T>
+using V2 = typename Vec2<T>::type;
 
 template <class T>
-__global__ void k_wedge02(int nt, const int *__restrict__ i0, const int *__restrict__ i1,
+__device__ __forceinline__ V2<T> ld2(const T *p, int i) {
+  return *reinterpret_cast<const V2<T> *>(p + i);
+}
+template <class T>
+__device__ __forceinline__ void st2(T *p, int i, T a, T b) {
+  V2<T> v;
+  v.x = a;
+  v.y = b;
+  *reinterpret_cast<V2<T> *>(p + i) = v;
+}
+__device__ __forceinline__ int2 ldi2(const int *p, int i) { return *reinterpret_cast<const int2 *>(p + i); }
+
+// wedge_kernel_01! (FastDEC.jl:159-163): half * α[i] * (f[p1] + f[p2])
+template <class T>
+__device__ __forceinline__ T w01(T a, T f0, T f1) {
+  return (static_cast<T>(0.5) * a) * (f0 + f1);
+}
+template <class T>
+__global__ void k_wedge01(int ne, int vec, const int *__restrict__ v0, const int *__restrict__ v1,
+                          const T *__restrict__ f, const T *__restrict__ a, T *__restrict__ res) {
+  int e = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (vec && e + 1 < ne) {
+    int2 p = ldi2(v0, e), q = ldi2(v1, e);
+    V2<T> av = ld2(a, e);
+    T f0 = f[p.x], f1 = f[q.x], f2 = f[p.y], f3 = f[q.y];
+    st2(res, e, w01(av.x, f0, f1), w01(av.y, f2, f3));
+  } else {
+    for (int j = e; j < e + 2 && j < ne; j++) {
+      res[j] = w01(a[j], f[v0[j]], f[v1[j]]);
+    }
+  }
+}
+
+// wedge_kernel_02! (FastDEC.jl:165-170) with the six coefficients summed
+// pairwise: the six vertex slots are (v0, v0, v1, v1, v2, v2).
+template <class T>
+__device__ __forceinline__ T w02(T a, T c0, T c1, T c2, T f0, T f1, T f2) {
+  T s = c0 * f0 + c1 * f1;
+  s = s + c2 * f2;
+  return a * s;
+}
+template <class T>
+__global__ void k_wedge02(int nt, int vec, const int *__restrict__ i0, const int *__restrict__ i1,
                           const int *__restrict__ i2, const T *__restrict__ c0, const T *__restrict__ c1,
                           const T *__restrict__ c2, const T *__restrict__ f, const T *__restrict__ a,
                           T *__restrict__ res) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt) return;
-  // wedge_kernel_02! (FastDEC.jl:165-170) with the six coefficients summed
-  // pairwise: the six vertex slots are (v0, v0, v1, v1, v2, v2).
-  T s = c0[t] * f[i0[t]] + c1[t] * f[i1[t]];
-  s = s + c2[t] * f[i2[t]];
-  res[t] = a[t] * s;
+  int t = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (vec && t + 1 < nt) {
+    int2 p = ldi2(i0, t), q = ldi2(i1, t), u = ldi2(i2, t);
+    V2<T> k0 = ld2(c0, t), k1 = ld2(c1, t), k2 = ld2(c2, t), av = ld2(a, t);
+    T fa = f[p.x], fb = f[q.x], fc = f[u.x], fd = f[p.y], fe = f[q.y], ff = f[u.y];
+    st2(res, t, w02(av.x, k0.x, k1.x, k2.x, fa, fb, fc), w02(av.y, k0.y, k1.y, k2.y, fd, fe, ff));
+  } else {
+    for (int j = t; j < t + 2 && j < nt; j++) {
+      res[j] = w02(a[j], c0[j], c1[j], c2[j], f[i0[j]], f[i1[j]], f[i2[j]]);
+    }
+  }
 }
 
+// wedge_kernel_11! (FastDEC.jl:184-191)
 template <class T>
-__global__ void k_wedge11(int nt, const int *__restrict__ i0, const int *__restrict__ i1,
+__device__ __forceinline__ T w11(T c0, T c1, T c2, T ae0, T ae1, T ae2, T be0, T be1, T be2) {
+  T x = c0 * (ae2 * be1 - ae1 * be2);
+  T y = c1 * (ae2 * be0 - ae0 * be2);
+  T z = c2 * (ae1 * be0 - ae0 * be1);
+  return (x + y) + z;
+}
+template <class T>
+__global__ void k_wedge11(int nt, int vec, const int *__restrict__ i0, const int *__restrict__ i1,
                           const int *__restrict__ i2, const T *__restrict__ c0, const T *__restrict__ c1,
                           const T *__restrict__ c2, const T *__restrict__ a, const T *__restrict__ b,
                           T *__restrict__ res) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt) return;
-  // wedge_kernel_11! (FastDEC.jl:184-191)
-  int e0 = i0[t], e1 = i1[t], e2 = i2[t];
-  T ae0 = a[e0], ae1 = a[e1], ae2 = a[e2];
-  T be0 = b[e0], be1 = b[e1], be2 = b[e2];
-  T x = c0[t] * (ae2 * be1 - ae1 * be2);
-  T y = c1[t] * (ae2 * be0 - ae0 * be2);
-  T z = c2[t] * (ae1 * be0 - ae0 * be1);
-  res[t] = (x + y) + z;
+  int t = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (vec && t + 1 < nt) {
+    int2 p = ldi2(i0, t), q = ldi2(i1, t), u = ldi2(i2, t);
+    V2<T> k0 = ld2(c0, t), k1 = ld2(c1, t), k2 = ld2(c2, t);
+    T a0 = a[p.x], a1 = a[q.x], a2 = a[u.x], b0 = b[p.x], b1 = b[q.x], b2 = b[u.x];
+    T a3 = a[p.y], a4 = a[q.y], a5 = a[u.y], b3 = b[p.y], b4 = b[q.y], b5 = b[u.y];
+    st2(res, t, w11(k0.x, k1.x, k2.x, a0, a1, a2, b0, b1, b2), w11(k0.y, k1.y, k2.y, a3, a4, a5, b3, b4, b5));
+  } else {
+    for (int j = t; j < t + 2 && j < nt; j++) {
+      int e0 = i0[j], e1 = i1[j], e2 = i2[j];
+      res[j] = w11(c0[j], c1[j], c2[j], a[e0], a[e1], a[e2], b[e0], b[e1], b[e2]);
+    }
+  }
 }
 
+// (m * f)[e] .* g[e], m = wedge_dd_01_mat (FastDEC.jl:293-305); the row's
+// triangles are ascending (ta < tb, tb < 0 when e has one coface).
 template <class T>
-__global__ void k_wedge_dd(int ne, const int *__restrict__ ta, const int *__restrict__ tb,
+__device__ __forceinline__ T wdd(int a, int b, T wa, T wb, const T *__restrict__ f, T g) {
+  T s = static_cast<T>(0);
+  if (a >= 0) s += wa * f[a];
+  if (b >= 0) s += wb * f[b];
+  return s * g;
+}
+template <class T>
+__global__ void k_wedge_dd(int ne, int vec, const int *__restrict__ ta, const int *__restrict__ tb,
                            const T *__restrict__ wa, const T *__restrict__ wb, const T *__restrict__ f,
                            const T *__restrict__ g, T *__restrict__ res) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  // (m * f)[e] .* g[e], m = wedge_dd_01_mat (FastDEC.jl:293-305); the row's
-  // triangles are ascending (ta < tb, tb < 0 when e has one coface).
-  T s = static_cast<T>(0);
-  int a = ta[e], b = tb[e];
-  if (a >= 0) s += wa[e] * f[a];
-  if (b >= 0) s += wb[e] * f[b];
-  res[e] = s * g[e];
+  int e = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (vec && e + 1 < ne) {
+    int2 p = ldi2(ta, e), q = ldi2(tb, e);
+    V2<T> x = ld2(wa, e), y = ld2(wb, e), gg = ld2(g, e);
+    st2(res, e, wdd(p.x, q.x, x.x, y.x, f, gg.x), wdd(p.y, q.y, x.y, y.y, f, gg.y));
+  } else {
+    for (int j = e; j < e + 2 && j < ne; j++) {
+      res[j] = wdd(ta[j], tb[j], wa[j], wb[j], f, g[j]);
+    }
+  }
 }
 
+// row e of d0 (FastDEC.jl:471-492): σ at ∂v0, -σ at ∂v1
 template <class T>
-__global__ void k_d0(int ne, const int *__restrict__ v0, const int *__restrict__ v1,
+__device__ __forceinline__ T d0v(int8_t s, T x0, T x1) {
+  T d = x0 - x1;
+  return s > 0 ? d : -d;
+}
+template <class T>
+__global__ void k_d0(int ne, int vec, const int *__restrict__ v0, const int *__restrict__ v1,
                      const int8_t *__restrict__ sgn, const T *__restrict__ x, T *__restrict__ res) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  // row e of d0 (FastDEC.jl:471-492): σ at ∂v0, -σ at ∂v1
-  T d = x[v0[e]] - x[v1[e]];
-  res[e] = sgn[e] > 0 ? d : -d;
+  int e = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (vec && e + 1 < ne) {
+    int2 p = ldi2(v0, e), q = ldi2(v1, e);
+    char2 s = *reinterpret_cast<const char2 *>(sgn + e);
+    T a = x[p.x], b = x[q.x], c = x[p.y], d = x[q.y];
+    st2(res, e, d0v<T>(s.x, a, b), d0v<T>(s.y, c, d));
+  } else {
+    for (int j = e; j < e + 2 && j < ne; j++) {
+      res[j] = d0v<T>(sgn[j], x[v0[j]], x[v1[j]]);
+    }
+  }
 }
 
+// row t of d1 (FastDEC.jl:496-524), its three edges in ascending order with
+// signs packed in bits 0..2 (set = +1)
 template <class T>
-__global__ void k_d1(int nt, const int *__restrict__ ia, const int *__restrict__ ib, const int *__restrict__ ic,
-                     const int8_t *__restrict__ sg, const T *__restrict__ x, T *__restrict__ res) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt) return;
-  // row t of d1 (FastDEC.jl:496-524), its three edges in ascending order with
-  // signs packed in bits 0..2 (set = +1)
-  int s = sg[t];
-  T xa = x[ia[t]], xb = x[ib[t]], xc = x[ic[t]];
+__device__ __forceinline__ T d1v(int s, T xa, T xb, T xc) {
   T r = (s & 1) ? xa : -xa;
   r = r + ((s & 2) ? xb : -xb);
   r = r + ((s & 4) ? xc : -xc);
-  res[t] = r;
+  return r;
 }
-
 template <class T>
-__global__ void k_sign_ell(int n, int width, const int *__restrict__ col, const int8_t *__restrict__ val,
-                           const T *__restrict__ x, T *__restrict__ y) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  T acc = static_cast<T>(0);
-  for (int k = 0; k < width; k++) {
-    int c = col[static_cast<size_t>(k) * n + r];
-    if (c < 0) break;
-    T v = x[c];
-    acc = acc + (val[static_cast<size_t>(k) * n + r] > 0 ? v : -v);
+__global__ void k_d1(int nt, int vec, const int *__restrict__ ia, const int *__restrict__ ib, const int *__restrict__ ic,
+                     const int8_t *__restrict__ sg, const T *__restrict__ x, T *__restrict__ res) {
+  int t = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (vec && t + 1 < nt) {
+    int2 p = ldi2(ia, t), q = ldi2(ib, t), u = ldi2(ic, t);
+    char2 s = *reinterpret_cast<const char2 *>(sg + t);
+    T a = x[p.x], b = x[q.x], c = x[u.x], d = x[p.y], e = x[q.y], g = x[u.y];
+    st2(res, t, d1v<T>(s.x, a, b, c), d1v<T>(s.y, d, e, g));
+  } else {
+    for (int j = t; j < t + 2 && j < nt; j++) {
+      res[j] = d1v<T>(sg[j], x[ia[j]], x[ib[j]], x[ic[j]]);
+    }
   }
-  y[r] = acc;
 }
 
-template <class T>
-__global__ void k_ell(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
+template <int W, class T, class V>
+__global__ void k_ell(int n, int width, const int *__restrict__ col, const V *__restrict__ val,
                       const T *__restrict__ x, T *__restrict__ y) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  T acc = static_cast<T>(0);
-  for (int k = 0; k < width; k++) {
-    size_t q = static_cast<size_t>(k) * n + r;
-    int c = col[q];
-    if (c < 0) break;
-    acc = acc + val[q] * x[c];
-  }
-  y[r] = acc;
+  if (r < n) y[r] = dec_ell::row<W, T, V>(n, width, r, col, val, x);
 }
 
 template <class T>
@@ -170,11 +245,19 @@ __global__ void k_diag(int n, const T *__restrict__ h, const T *__restrict__ x, 
 }
 
 template <class T>
-__global__ void k_avg01(int ne, const int *__restrict__ v0, const int *__restrict__ v1, const T *__restrict__ x,
+__global__ void k_avg01(int ne, int vec, const int *__restrict__ v0, const int *__restrict__ v1, const T *__restrict__ x,
                         T *__restrict__ y) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  y[e] = static_cast<T>(0.5) * x[v0[e]] + static_cast<T>(0.5) * x[v1[e]];
+  int e = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  const T h = static_cast<T>(0.5);
+  if (vec && e + 1 < ne) {
+    int2 p = ldi2(v0, e), q = ldi2(v1, e);
+    T a = x[p.x], b = x[q.x], c = x[p.y], d = x[q.y];
+    st2(y, e, h * a + h * b, h * c + h * d);
+  } else {
+    for (int j = e; j < e + 2 && j < ne; j++) {
+      y[j] = h * x[v0[j]] + h * x[v1[j]];
+    }
+  }
 }
 
 // ---- coefficient precomputation (float64) ----
@@ -391,7 +474,8 @@ template void upload_ell<float>(const HostCsr &, Ell<float> &);
 template <class T>
 void ell_spmv(const Ell<T> &E, const T *x, T *y, hipStream_t s) {
   if (E.nrows == 0) return;
-  hipLaunchKernelGGL(k_ell<T>, dim3(nblocks(E.nrows)), dim3(256), 0, s, E.nrows, E.width, E.col.p, E.val.p, x, y);
+  DEC_ELL_WIDTH_SWITCH(E.width, hipLaunchKernelGGL((k_ell<W, T, T>), dim3(nblocks(E.nrows)), dim3(256), 0, s,
+                                                    E.nrows, E.width, E.col.p, E.val.p, x, y));
 }
 template void ell_spmv<double>(const Ell<double> &, const double *, double *, hipStream_t);
 template void ell_spmv<float>(const Ell<float> &, const float *, float *, hipStream_t);
@@ -540,33 +624,39 @@ void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream
   int rows = static_cast<int>(op.rows);
   if (rows == 0) return;
   dim3 g(dec::nblocks(rows)), b(256);
+  dim3 g2(dec::nblocks((static_cast<long long>(rows) + 1) / 2));
+  // the pair kernels load the caller's forms as 2-vectors: require alignment
+  const uintptr_t A = 2 * sizeof(T);
+  int vec = (reinterpret_cast<uintptr_t>(out) % A == 0) && (!y || reinterpret_cast<uintptr_t>(y) % A == 0) &&
+            (!x || reinterpret_cast<uintptr_t>(x) % A == 0);
   auto C = [](DBuf<unsigned char> &c) { return reinterpret_cast<const T *>(c.p); };
   switch (op.impl) {
     case I_W01:
-      hipLaunchKernelGGL(k_wedge01<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, x, y, out);
+      hipLaunchKernelGGL(k_wedge01<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, x, y, out);
       break;
     case I_W02:
-      hipLaunchKernelGGL(k_wedge02<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, op.i2.p, C(op.c0), C(op.c1), C(op.c2), x,
+      hipLaunchKernelGGL(k_wedge02<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, op.i2.p, C(op.c0), C(op.c1), C(op.c2), x,
                          y, out);
       break;
     case I_W11:
-      hipLaunchKernelGGL(k_wedge11<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, op.i2.p, C(op.c0), C(op.c1), C(op.c2), x,
+      hipLaunchKernelGGL(k_wedge11<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, op.i2.p, C(op.c0), C(op.c1), C(op.c2), x,
                          y, out);
       break;
     case I_WDD:
       if (op.m == 0)  // (m*f) .* g : f = x (dual 0-form), g = y
-        hipLaunchKernelGGL(k_wedge_dd<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, C(op.c0), C(op.c1), x, y, out);
+        hipLaunchKernelGGL(k_wedge_dd<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, C(op.c0), C(op.c1), x, y, out);
       else  // f .* (m*g)
-        hipLaunchKernelGGL(k_wedge_dd<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, C(op.c0), C(op.c1), y, x, out);
+        hipLaunchKernelGGL(k_wedge_dd<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, C(op.c0), C(op.c1), y, x, out);
       break;
     case I_D0:
-      hipLaunchKernelGGL(k_d0<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, op.s8.p, x, out);
+      hipLaunchKernelGGL(k_d0<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, op.s8.p, x, out);
       break;
     case I_D1:
-      hipLaunchKernelGGL(k_d1<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, op.i2.p, op.s8.p, x, out);
+      hipLaunchKernelGGL(k_d1<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, op.i2.p, op.s8.p, x, out);
       break;
     case I_SELL:
-      hipLaunchKernelGGL(k_sign_ell<T>, g, b, 0, s, rows, op.sell.width, op.sell.col.p, op.sell.val.p, x, out);
+      DEC_ELL_WIDTH_SWITCH(op.sell.width, hipLaunchKernelGGL((k_ell<W, T, int8_t>), g, b, 0, s, rows, op.sell.width,
+                                                             op.sell.col.p, op.sell.val.p, x, out));
       break;
     case I_DIAG:
       hipLaunchKernelGGL(k_diag<T>, g, b, 0, s, rows, C(op.c0), x, out);
@@ -576,7 +666,7 @@ void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream
       else dec::ell_spmv(op.ell32, x, out, s);
       break;
     case I_AVG:
-      hipLaunchKernelGGL(k_avg01<T>, g, b, 0, s, rows, op.i0.p, op.i1.p, x, out);
+      hipLaunchKernelGGL(k_avg01<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, x, out);
       break;
     default:
       fail(DEC_ERR_ARG, "dec_op_apply: corrupt operator");
