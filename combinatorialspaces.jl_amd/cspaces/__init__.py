@@ -11,3 +11,4 @@ from .multigrid import (  # noqa: F401
     BinarySubdivision, MGData, MultigridData, cg, dec_Delta_inv, full_multigrid, gauss_seidel, jacobi,
     multigrid_mu_cycles, multigrid_vcycles, multigrid_wcycles, solve)
 from .batch import HeatBatch  # noqa: F401
+from .learned import LearnedSmoother, learned_stats, make_pairs, sh_basis  # noqa: F401

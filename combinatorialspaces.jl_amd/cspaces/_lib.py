@@ -52,6 +52,16 @@ SIGNATURES = {
     "dec_mg_full_multigrid": (c_int, [P, c_int32, c_int32, P, P, P]),
     "dec_mg_solve": (c_int, [P, c_int32, c_double, c_int32, P, P, PI32, PF64, P]),
     "dec_mg_destroy": (c_int, [P]),
+    "dec_nn_create": (c_int, [P, c_uint64, POINTER(P)]),
+    "dec_nn_nparams": (c_int, [P, PI32]),
+    "dec_nn_get_params": (c_int, [P, P]),
+    "dec_nn_set_params": (c_int, [P, P]),
+    "dec_nn_forward": (c_int, [P, c_int32, c_int32, P, P, c_int32, P]),
+    "dec_nn_loss_grad": (c_int, [P, c_int32, c_int32, P, c_int32, PF64, P, P]),
+    "dec_nn_train": (c_int, [P, c_int32, c_int32, P, c_int32, c_int32, ctypes.c_float, c_uint64, P, P]),
+    "dec_nn_destroy": (c_int, [P]),
+    "dec_mg_set_learned": (c_int, [P, P, c_int32, c_int32, c_int32]),
+    "dec_mg_learned_stats": (c_int, [P, PI32, PI32]),
     "dec_batch_create": (c_int, [c_int32, P, c_int32, POINTER(P)]),
     "dec_batch_layout": (c_int, [P, PI64, P, P]),
     "dec_batch_tune": (c_int, [P, P, c_char_p, PI32, PI32, PI32]),

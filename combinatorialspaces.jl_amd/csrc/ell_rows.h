@@ -12,6 +12,16 @@
 
 namespace dec_ell {
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
+// i % 8), and each XCD has its own L2.  Remapping the workgroup index so that
+// XCD k processes the k-th contiguous eighth of the rows keeps the gathers of
+// neighbouring rows in the same L2.  The grid must be a multiple of 8 (see
+// xcd_grid); workgroups past the end find no rows.
+__device__ __forceinline__ int xcd_block() {
+  int nb = gridDim.x;
+  return static_cast<int>((blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3));
+}
+
 template <class V, class T>
 __device__ __forceinline__ T mul(V v, T x) {
   return static_cast<T>(v) * x;

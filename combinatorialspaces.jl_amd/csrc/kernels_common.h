@@ -15,6 +15,8 @@ inline void check_launch(const char *what) {
 }
 
 inline int nblocks(long long n, int b = 256) { return static_cast<int>((n + b - 1) / b); }
+// grid for kernels using xcd_block(): rounded up to a multiple of the 8 XCDs
+inline int xcd_grid(long long n, int b = 256) { return (nblocks(n, b) + 7) / 8 * 8; }
 
 // mesh.hip
 void dualize(dec_mesh_s &m, int center);

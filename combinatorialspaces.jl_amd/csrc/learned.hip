@@ -1,0 +1,833 @@
+// Learned multigrid smoother of libdec_hip.so: a 3-layer edge message-passing
+// network over a level's operator graph, with hand-written forward, backward
+// and Adam kernels.  No counterpart in the reference (its smoother is Krylov
+// CG, src/Multigrid.jl:860).
+//
+// For a residual r on a level with operator A (D = diag A, P = D⁻¹(D - A)):
+//   s = sqrt(mean r²), x = r / s, f0 = [x, P x]
+//   h1 = relu(f0 W1 + b1)             2 -> 32   (VALU)
+//   h2 = relu([h1, P h1] W2 + b2)     64 -> 32  (BF16 MFMA, FP32 accumulate)
+//   h3 = relu([h2, P h2] W3 + b3)     64 -> 32  (BF16 MFMA, FP32 accumulate)
+//   y  = alpha x + h3 wo + bo, c = (s / D) y
+// trained on mean((r - A c)² / s²).  Activations are stored in BF16 (FP32 in
+// the "precise" mode, whose dense layers run on the VALU in FP32 and which
+// exists to check the kernels against the float64 oracle).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+
+#include "dec_internal.h"
+#include "kernels_common.h"
+#include "learned.h"
+
+using dec::DBuf;
+using dec::fail;
+
+namespace {
+
+constexpr int H = 32;          // hidden width
+constexpr int NP = 4290;       // parameter count
+// parameter offsets (flat float vector)
+constexpr int OW1 = 0, OB1 = 64, OW2 = 96, OB2 = 2144, OW3 = 2176, OB3 = 4224, OWO = 4256, OBO = 4288, OAL = 4289;
+constexpr int TB = 256;
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned int u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return static_cast<unsigned short>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(static_cast<unsigned int>(h) << 16); }
+
+// activation storage: BF16 (as raw bits) or FP32
+template <class AT>
+__device__ __forceinline__ float ald(const AT *p, size_t i);
+template <>
+__device__ __forceinline__ float ald<unsigned short>(const unsigned short *p, size_t i) { return bf2f(p[i]); }
+template <>
+__device__ __forceinline__ float ald<float>(const float *p, size_t i) { return p[i]; }
+template <class AT>
+__device__ __forceinline__ void ast(AT *p, size_t i, float v);
+template <>
+__device__ __forceinline__ void ast<unsigned short>(unsigned short *p, size_t i, float v) { p[i] = f2bf(v); }
+template <>
+__device__ __forceinline__ void ast<float>(float *p, size_t i, float v) { p[i] = v; }
+
+__device__ __forceinline__ float ell_row(int n, int width, int r, const int *__restrict__ col,
+                                         const float *__restrict__ val, const float *__restrict__ x) {
+  float acc = 0.0f;
+  for (int k = 0; k < width; k++) {
+    size_t q = static_cast<size_t>(k) * n + r;
+    int c = col[q];
+    if (c < 0) break;
+    acc = acc + val[q] * x[c];
+  }
+  return acc;
+}
+
+// ---- forward ------------------------------------------------------------
+
+// Per-sample scale s_b = sqrt(mean r²) (one block per sample, fixed order).
+__global__ void k_scale(int n, const float *__restrict__ r, float *__restrict__ s) {
+  __shared__ double sh[TB];
+  const float *rb = r + static_cast<size_t>(blockIdx.x) * n;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += TB) acc += static_cast<double>(rb[i]) * rb[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double m = sh[0] / n;
+    s[blockIdx.x] = m > 0 ? static_cast<float>(sqrt(m)) : 1.0f;
+  }
+}
+
+// x = r / s, f0 = [x, P x], h1 = relu(f0 W1 + b1)
+template <class AT>
+__global__ void k_input(int n, int pw, const int *__restrict__ pcol, const float *__restrict__ pval,
+                        const float *__restrict__ r, const float *__restrict__ s, const float *__restrict__ th,
+                        float *__restrict__ x, float *__restrict__ px, AT *__restrict__ h1) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (v >= n) return;
+  size_t o = static_cast<size_t>(b) * n;
+  float inv = 1.0f / s[b];
+  float xv = r[o + v] * inv;
+  float acc = 0.0f;
+  for (int k = 0; k < pw; k++) {
+    size_t q = static_cast<size_t>(k) * n + v;
+    int c = pcol[q];
+    if (c < 0) break;
+    acc = acc + pval[q] * (r[o + c] * inv);
+  }
+  x[o + v] = xv;
+  px[o + v] = acc;
+  for (int j = 0; j < H; j++) {
+    float z = xv * th[OW1 + j] + acc * th[OW1 + H + j] + th[OB1 + j];
+    ast(h1, (o + v) * H + j, z > 0.0f ? z : 0.0f);
+  }
+}
+
+// m = P h (32 features): 32 lanes per vertex, one feature per lane.
+template <class AT>
+__global__ void k_spmm(int n, int pw, const int *__restrict__ pcol, const float *__restrict__ pval,
+                       const AT *__restrict__ h, AT *__restrict__ m) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int v = t >> 5, f = t & 31;
+  int b = blockIdx.y;
+  if (v >= n) return;
+  size_t o = static_cast<size_t>(b) * n;
+  float acc = 0.0f;
+  for (int k = 0; k < pw; k++) {
+    size_t q = static_cast<size_t>(k) * n + v;
+    int c = pcol[q];
+    if (c < 0) break;
+    acc = acc + pval[q] * ald(h, (o + c) * H + f);
+  }
+  ast(m, (o + v) * H + f, acc);
+}
+
+// Dense 64 -> 32 with bias and relu on BF16 MFMA: each 64-lane wavefront
+// computes 16 rows x 32 columns as two 16x16 tiles, K = 64 in four steps of
+// v_mfma_f32_16x16x16bf16_1k.  Operand layout: lane l holds A[l%16][4(l/16)+i]
+// and B[4(l/16)+i][l%16]; the result D[4(l/16)+i][l%16].
+__global__ void k_dense_bf16(int rows, const unsigned short *__restrict__ h, const unsigned short *__restrict__ m,
+                             const float *__restrict__ W, const float *__restrict__ bias,
+                             unsigned short *__restrict__ out) {
+  int lane = threadIdx.x & 63;
+  int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int r0 = wave * 16;
+  if (r0 >= rows) return;
+  int li = lane & 15, lg = lane >> 4;
+  int row = r0 + li;
+  short4_t A[4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ks++) {
+    int k = ks * 16 + 4 * lg;  // 4 consecutive k of the concatenated [h, m]
+    const unsigned short *src = k < H ? h + static_cast<size_t>(row) * H + k : m + static_cast<size_t>(row) * H + (k - H);
+    short4_t a;
+    if (row < rows) {
+      a[0] = static_cast<short>(src[0]);
+      a[1] = static_cast<short>(src[1]);
+      a[2] = static_cast<short>(src[2]);
+      a[3] = static_cast<short>(src[3]);
+    } else {
+      a[0] = a[1] = a[2] = a[3] = 0;
+    }
+    A[ks] = a;
+  }
+  float4_t acc[2];
+#pragma unroll
+  for (int jt = 0; jt < 2; jt++) {
+    acc[jt] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) {
+      short4_t bv;
+#pragma unroll
+      for (int i = 0; i < 4; i++) bv[i] = static_cast<short>(f2bf(W[(ks * 16 + 4 * lg + i) * H + jt * 16 + li]));
+      acc[jt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], bv, acc[jt], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int jt = 0; jt < 2; jt++) {
+    int col = jt * 16 + li;
+    float bc = bias[col];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int rr = r0 + 4 * lg + i;
+      if (rr < rows) {
+        float z = acc[jt][i] + bc;
+        out[static_cast<size_t>(rr) * H + col] = f2bf(z > 0.0f ? z : 0.0f);
+      }
+    }
+  }
+}
+
+// Dense 64 -> 32 in FP32 on the VALU (precise mode), thread per (row, col).
+__global__ void k_dense_f32(int rows, const float *__restrict__ h, const float *__restrict__ m,
+                            const float *__restrict__ W, const float *__restrict__ bias, float *__restrict__ out) {
+  size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t row = t >> 5;
+  int col = static_cast<int>(t & 31);
+  if (row >= static_cast<size_t>(rows)) return;
+  float z = bias[col];
+  for (int k = 0; k < H; k++) z = z + h[row * H + k] * W[k * H + col];
+  for (int k = 0; k < H; k++) z = z + m[row * H + k] * W[(H + k) * H + col];
+  out[row * H + col] = z > 0.0f ? z : 0.0f;
+}
+
+// y = alpha x + h3 wo + bo; c = (s / D) y, optionally u += c.
+template <class AT>
+__global__ void k_output(int n, const float *__restrict__ x, const AT *__restrict__ h3, const float *__restrict__ th,
+                         const float *__restrict__ s, const float *__restrict__ D, float *__restrict__ y,
+                         float *__restrict__ c, float *__restrict__ u_add) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (v >= n) return;
+  size_t o = static_cast<size_t>(b) * n + v;
+  float acc = th[OAL] * x[o];
+  float hw = 0.0f;
+  for (int j = 0; j < H; j++) hw = hw + ald(h3, o * H + j) * th[OWO + j];
+  float yy = acc + hw + th[OBO];
+  if (y) y[o] = yy;
+  float cc = (s[b] / D[v]) * yy;
+  if (c) c[o] = cc;
+  if (u_add) u_add[o] = u_add[o] + cc;
+}
+
+// ---- loss and backward ------------------------------------------------------
+
+// rho = x - A (y / D) per sample; dρ = 2 rho / (n B); block partial of Σ rho².
+__global__ void k_loss(int n, int B, int aw, const int *__restrict__ acol, const float *__restrict__ aval,
+                       const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ D,
+                       float *__restrict__ t, float *__restrict__ drho, double *__restrict__ part) {
+  __shared__ double sh[TB];
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  size_t o = static_cast<size_t>(b) * n;
+  double sq = 0.0;
+  if (v < n) {
+    // t = y / D is formed by the neighbours' threads; compute A t on the fly
+    float acc = 0.0f;
+    for (int k = 0; k < aw; k++) {
+      size_t q = static_cast<size_t>(k) * n + v;
+      int c = acol[q];
+      if (c < 0) break;
+      acc = acc + aval[q] * (y[o + c] / D[c]);
+    }
+    float rho = x[o + v] - acc;
+    drho[o + v] = 2.0f * rho / (static_cast<float>(n) * B);
+    sq = static_cast<double>(rho) * rho;
+  }
+  sh[threadIdx.x] = sq;
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+  (void)t;
+}
+
+// dy = -(Aᵀ dρ) / D; dz3 = (dy wo) ⊙ [h3 > 0]; block partials of
+// (Σ dy x, Σ dy, Σ dy h3) for alpha, bo, wo.
+template <class AT>
+__global__ void k_out_back(int n, int atw, const int *__restrict__ atcol, const float *__restrict__ atval,
+                           const float *__restrict__ drho, const float *__restrict__ D, const float *__restrict__ x,
+                           const AT *__restrict__ h3, const float *__restrict__ th, AT *__restrict__ dz3,
+                           float *__restrict__ part) {
+  __shared__ float sh[TB][2 * H + 2];
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  size_t o = static_cast<size_t>(b) * n;
+  float dy = 0.0f, xv = 0.0f;
+  float hv[H];
+  if (v < n) {
+    float acc = 0.0f;
+    for (int k = 0; k < atw; k++) {
+      size_t q = static_cast<size_t>(k) * n + v;
+      int c = atcol[q];
+      if (c < 0) break;
+      acc = acc + atval[q] * drho[o + c];
+    }
+    dy = -acc / D[v];
+    xv = x[o + v];
+    for (int j = 0; j < H; j++) {
+      hv[j] = ald(h3, (o + v) * H + j);
+      float g = hv[j] > 0.0f ? dy * th[OWO + j] : 0.0f;
+      ast(dz3, (o + v) * H + j, g);
+      sh[threadIdx.x][H + 2 + j] = ald(dz3, (o + v) * H + j);
+    }
+  } else {
+    for (int j = 0; j < H; j++) {
+      hv[j] = 0.0f;
+      sh[threadIdx.x][H + 2 + j] = 0.0f;
+    }
+  }
+  for (int j = 0; j < H; j++) sh[threadIdx.x][j] = dy * hv[j];
+  sh[threadIdx.x][H] = dy;
+  sh[threadIdx.x][H + 1] = dy * xv;
+  __syncthreads();
+  // column sums in fixed order: [wo (32), bo, alpha, b3 (32)]
+  if (threadIdx.x < 2 * H + 2) {
+    float acc = 0.0f;
+    for (int i = 0; i < TB; i++) acc += sh[i][threadIdx.x];
+    part[(static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x) * (2 * H + 2) + threadIdx.x] = acc;
+  }
+}
+
+// dg = dz Wᵀ (rows x 64) on BF16 MFMA: 16 rows per wavefront, four 16-column
+// tiles, K = 32 in two steps.
+__global__ void k_dgrad_bf16(int rows, const unsigned short *__restrict__ dz, const float *__restrict__ W,
+                             float *__restrict__ dg) {
+  int lane = threadIdx.x & 63;
+  int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int r0 = wave * 16;
+  if (r0 >= rows) return;
+  int li = lane & 15, lg = lane >> 4;
+  int row = r0 + li;
+  short4_t A[2];
+#pragma unroll
+  for (int ks = 0; ks < 2; ks++) {
+    int k = ks * 16 + 4 * lg;
+    short4_t a;
+#pragma unroll
+    for (int i = 0; i < 4; i++) a[i] = row < rows ? static_cast<short>(dz[static_cast<size_t>(row) * H + k + i]) : 0;
+    A[ks] = a;
+  }
+#pragma unroll
+  for (int jt = 0; jt < 4; jt++) {
+    float4_t acc = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+      short4_t bv;
+      // B[k][j] = Wᵀ[k][j] = W[j][k], j = input feature (64), k = output (32)
+#pragma unroll
+      for (int i = 0; i < 4; i++) bv[i] = static_cast<short>(f2bf(W[(jt * 16 + li) * H + ks * 16 + 4 * lg + i]));
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int rr = r0 + 4 * lg + i;
+      if (rr < rows) dg[static_cast<size_t>(rr) * 2 * H + jt * 16 + li] = acc[i];
+    }
+  }
+}
+
+__global__ void k_dgrad_f32(int rows, const float *__restrict__ dz, const float *__restrict__ W, float *__restrict__ dg) {
+  size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t row = t >> 6;
+  int j = static_cast<int>(t & 63);
+  if (row >= static_cast<size_t>(rows)) return;
+  float acc = 0.0f;
+  for (int k = 0; k < H; k++) acc = acc + dz[row * H + k] * W[j * H + k];
+  dg[row * 2 * H + j] = acc;
+}
+
+// Weight gradient partials dW = gᵀ dz (64 x 32) over a chunk of 256 rows per
+// wavefront on BF16 MFMA: A = gᵀ (16 features x 16 rows), B = dz (16 rows x
+// 16 cols); eight accumulator tiles.  Partials [wave][64][32].
+__global__ void k_wgrad_bf16(int rows, const unsigned short *__restrict__ h, const unsigned short *__restrict__ m,
+                             const unsigned short *__restrict__ dz, float *__restrict__ part) {
+  int lane = threadIdx.x & 63;
+  int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int base = wave * 256;
+  int li = lane & 15, lg = lane >> 4;
+  float4_t acc[4][2];
+#pragma unroll
+  for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++) acc[ti][tj] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int kc = 0; kc < 256 && base + kc < rows; kc += 16) {
+    short4_t Bv[2];
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int rr = base + kc + 4 * lg + i;
+        Bv[tj][i] = rr < rows ? static_cast<short>(dz[static_cast<size_t>(rr) * H + tj * 16 + li]) : 0;
+      }
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++) {
+      short4_t Av;
+      int f = ti * 16 + li;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int rr = base + kc + 4 * lg + i;
+        unsigned short gv = 0;
+        if (rr < rows) gv = f < H ? h[static_cast<size_t>(rr) * H + f] : m[static_cast<size_t>(rr) * H + f - H];
+        Av[i] = static_cast<short>(gv);
+      }
+#pragma unroll
+      for (int tj = 0; tj < 2; tj++) acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Av, Bv[tj], acc[ti][tj], 0, 0, 0);
+    }
+  }
+  float *p = part + static_cast<size_t>(wave) * 2 * H * H;
+#pragma unroll
+  for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) p[(ti * 16 + 4 * lg + i) * H + tj * 16 + li] = acc[ti][tj][i];
+}
+
+// FP32 weight-gradient partials, thread per (feature, col) over 256 rows.
+__global__ void k_wgrad_f32(int rows, const float *__restrict__ h, const float *__restrict__ m,
+                            const float *__restrict__ dz, float *__restrict__ part) {
+  int chunk = blockIdx.x;
+  for (int e = threadIdx.x; e < 2 * H * H; e += blockDim.x) {
+    int f = e / H, j = e % H;
+    float acc = 0.0f;
+    for (int rr = chunk * 256; rr < chunk * 256 + 256 && rr < rows; rr++) {
+      float g = f < H ? h[static_cast<size_t>(rr) * H + f] : m[static_cast<size_t>(rr) * H + f - H];
+      acc = acc + g * dz[static_cast<size_t>(rr) * H + j];
+    }
+    part[static_cast<size_t>(chunk) * 2 * H * H + e] = acc;
+  }
+}
+
+// dh = dg[:, :32] + Pᵀ dg[:, 32:], dz = dh ⊙ [h > 0]; block partials of Σ dz (bias).
+template <class AT>
+__global__ void k_back_mix(int n, int ptw, const int *__restrict__ ptcol, const float *__restrict__ ptval,
+                           const float *__restrict__ dg, const AT *__restrict__ h, AT *__restrict__ dz,
+                           float *__restrict__ part) {
+  __shared__ float sh[TB];
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int v = t >> 5, f = t & 31;
+  int b = blockIdx.y;
+  size_t o = static_cast<size_t>(b) * n;
+  float g = 0.0f;
+  if (v < n) {
+    float acc = dg[(o + v) * 2 * H + f];
+    for (int k = 0; k < ptw; k++) {
+      size_t q = static_cast<size_t>(k) * n + v;
+      int c = ptcol[q];
+      if (c < 0) break;
+      acc = acc + ptval[q] * dg[(o + c) * 2 * H + H + f];
+    }
+    g = ald(h, (o + v) * H + f) > 0.0f ? acc : 0.0f;
+    ast(dz, (o + v) * H + f, g);
+  }
+  sh[threadIdx.x] = g;
+  __syncthreads();
+  if (threadIdx.x < H) {
+    float acc = 0.0f;
+    for (int i = threadIdx.x; i < TB; i += H) acc += sh[i];
+    part[(static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x) * H + threadIdx.x] = acc;
+  }
+}
+
+// dW1 = f0ᵀ dz1 (2 x 32) block partials.
+template <class AT>
+__global__ void k_w1_back(int n, const float *__restrict__ x, const float *__restrict__ px, const AT *__restrict__ dz1,
+                          float *__restrict__ part) {
+  __shared__ float sh[TB][2];
+  int b = blockIdx.y;
+  size_t o = static_cast<size_t>(b) * n;
+  int v0 = blockIdx.x * (TB / H);
+  // thread (vl, j): vertex v0 + vl, column j
+  int vl = threadIdx.x / H, j = threadIdx.x % H;
+  float a0 = 0.0f, a1 = 0.0f;
+  int v = v0 + vl;
+  if (v < n) {
+    float d = ald(dz1, (o + v) * H + j);
+    a0 = x[o + v] * d;
+    a1 = px[o + v] * d;
+  }
+  sh[threadIdx.x][0] = a0;
+  sh[threadIdx.x][1] = a1;
+  __syncthreads();
+  if (threadIdx.x < 2 * H) {
+    int jj = threadIdx.x % H, which = threadIdx.x / H;
+    float acc = 0.0f;
+    for (int q = 0; q < TB / H; q++) acc += sh[q * H + jj][which];
+    part[(static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x) * 2 * H + which * H + jj] = acc;
+  }
+}
+
+// out[j] = Σ_p part[p * stride + off + j] for j < width, sequential over p.
+__global__ void k_reduce_parts(const float *__restrict__ part, int nparts, int stride, int off, int width,
+                               float *__restrict__ out) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= width) return;
+  float acc = 0.0f;
+  for (int p = 0; p < nparts; p++) acc += part[static_cast<size_t>(p) * stride + off + j];
+  out[j] = acc;
+}
+
+__global__ void k_reduce_parts_d(const double *__restrict__ part, int nparts, double scale, double *__restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double acc = 0.0;
+  for (int p = 0; p < nparts; p++) acc += part[p];
+  *out = acc * scale;
+}
+
+__global__ void k_adam(int np, float *__restrict__ th, const float *__restrict__ g, float *__restrict__ m1,
+                       float *__restrict__ m2, float lr, float b1, float b2, float eps, float c1, float c2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  float gi = g[i];
+  float a = b1 * m1[i] + (1 - b1) * gi;
+  float v = b2 * m2[i] + (1 - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = v;
+  th[i] = th[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+}
+
+__global__ void k_gather_rows(int n, const float *__restrict__ src, const int *__restrict__ idx, float *__restrict__ dst) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (v < n) dst[static_cast<size_t>(b) * n + v] = src[static_cast<size_t>(idx[b]) * n + v];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+
+namespace dec {
+
+static void to_ell(const HostCsr &A, NnLevel::Op &E) {
+  Ell<float> e;
+  upload_ell(A, e);
+  E.n = e.nrows;
+  E.width = e.width;
+  E.col = std::move(e.col);
+  E.val = std::move(e.val);
+}
+
+static HostCsr transpose(const HostCsr &A) {
+  HostCsr T;
+  T.m = A.n;
+  T.n = A.m;
+  T.ptr.assign(A.n + 1, 0);
+  for (int c : A.col) T.ptr[c + 1]++;
+  for (int i = 0; i < A.n; i++) T.ptr[i + 1] += T.ptr[i];
+  T.col.resize(A.col.size());
+  T.val.resize(A.val.size());
+  std::vector<int32_t> w(T.ptr.begin(), T.ptr.end() - 1);
+  for (int r = 0; r < A.m; r++)
+    for (int k = A.ptr[r]; k < A.ptr[r + 1]; k++) {
+      int q = w[A.col[k]]++;
+      T.col[q] = r;
+      T.val[q] = A.val[k];
+    }
+  return T;
+}
+
+NnLevel::NnLevel(const HostCsr &A) {
+  n = A.m;
+  HostCsr P;
+  P.m = P.n = n;
+  P.ptr.assign(n + 1, 0);
+  std::vector<float> d(n, 0.0f);
+  for (int r = 0; r < n; r++) {
+    double dd = 0.0;
+    for (int k = A.ptr[r]; k < A.ptr[r + 1]; k++)
+      if (A.col[k] == r) dd = A.val[k];
+    d[r] = static_cast<float>(dd);
+    for (int k = A.ptr[r]; k < A.ptr[r + 1]; k++)
+      if (A.col[k] != r) {
+        P.col.push_back(A.col[k]);
+        P.val.push_back(-A.val[k] / dd);
+      }
+    P.ptr[r + 1] = static_cast<int32_t>(P.col.size());
+  }
+  D.upload(d);
+  to_ell(A, a);
+  to_ell(transpose(A), at);
+  to_ell(P, p);
+  to_ell(transpose(P), pt);
+}
+
+}  // namespace dec
+
+struct dec_nn_s {
+  std::vector<std::unique_ptr<dec::NnLevel>> levels;  // one per hierarchy level (finest first)
+  DBuf<float> th, grad, m1, m2;
+  int adam_t = 0;
+  // work buffers (grown on demand)
+  DBuf<float> x, px, y, c, s, drho, dg, part, r;
+  DBuf<double> dpart, dloss;
+  DBuf<unsigned char> h1, hm1, h2, hm2, h3, dz;  // activation storage (bf16 or f32)
+  DBuf<int32_t> idx;
+  size_t cap_rows = 0;
+  int cap_prec = -1;
+};
+
+namespace {
+
+void ensure(dec_nn_s &nn, size_t rows, int precise) {
+  if (rows <= nn.cap_rows && precise == nn.cap_prec) return;
+  size_t as = precise ? 4 : 2;
+  nn.x.alloc(rows);
+  nn.px.alloc(rows);
+  nn.y.alloc(rows);
+  nn.c.alloc(rows);
+  nn.drho.alloc(rows);
+  nn.dg.alloc(rows * 2 * H);
+  for (auto *b : {&nn.h1, &nn.hm1, &nn.h2, &nn.hm2, &nn.h3, &nn.dz}) b->alloc(rows * H * as);
+  size_t parts = (rows + 255) / 256 + 64;
+  nn.part.alloc(parts * 2 * H * H);
+  nn.dpart.alloc(parts + 64);
+  nn.cap_rows = rows;
+  nn.cap_prec = precise;
+}
+
+template <class AT>
+struct Acts {
+  AT *h1, *m1, *h2, *m2, *h3, *dz;
+};
+template <class AT>
+Acts<AT> acts(dec_nn_s &nn) {
+  return Acts<AT>{reinterpret_cast<AT *>(nn.h1.p), reinterpret_cast<AT *>(nn.hm1.p), reinterpret_cast<AT *>(nn.h2.p),
+                  reinterpret_cast<AT *>(nn.hm2.p), reinterpret_cast<AT *>(nn.h3.p), reinterpret_cast<AT *>(nn.dz.p)};
+}
+
+template <class AT>
+void dense(int R, const AT *h, const AT *m, const float *W, const float *b, AT *out, hipStream_t st) {
+  if constexpr (sizeof(AT) == 2) {
+    int waves = (R + 15) / 16;
+    hipLaunchKernelGGL(k_dense_bf16, dim3((waves + 3) / 4), dim3(TB), 0, st, R, h, m, W, b, out);
+  } else {
+    hipLaunchKernelGGL(k_dense_f32, dim3(dec::nblocks(static_cast<long long>(R) * H, TB)), dim3(TB), 0, st, R, h, m,
+                       W, b, out);
+  }
+}
+
+template <class AT>
+void forward_t(dec_nn_s &nn, const dec::NnLevel &L, int B, const float *r, float *c_out, float *u_add,
+               hipStream_t st) {
+  int n = L.n;
+  size_t rows = static_cast<size_t>(n) * B;
+  ensure(nn, rows, sizeof(AT) == 4);
+  if (nn.s.n < static_cast<size_t>(B)) nn.s.alloc(B);
+  Acts<AT> A = acts<AT>(nn);
+  const float *th = nn.th.p;
+  hipLaunchKernelGGL(k_scale, dim3(B), dim3(TB), 0, st, n, r, nn.s.p);
+  dim3 gv(dec::nblocks(n, TB), B), gf(dec::nblocks(static_cast<long long>(n) * H, TB), B);
+  hipLaunchKernelGGL(k_input<AT>, gv, dim3(TB), 0, st, n, L.p.width, L.p.col.p, L.p.val.p, r, nn.s.p, th, nn.x.p,
+                     nn.px.p, A.h1);
+  int R = static_cast<int>(rows);
+  hipLaunchKernelGGL(k_spmm<AT>, gf, dim3(TB), 0, st, n, L.p.width, L.p.col.p, L.p.val.p, A.h1, A.m1);
+  dense<AT>(R, A.h1, A.m1, th + OW2, th + OB2, A.h2, st);
+  hipLaunchKernelGGL(k_spmm<AT>, gf, dim3(TB), 0, st, n, L.p.width, L.p.col.p, L.p.val.p, A.h2, A.m2);
+  dense<AT>(R, A.h2, A.m2, th + OW3, th + OB3, A.h3, st);
+  hipLaunchKernelGGL(k_output<AT>, gv, dim3(TB), 0, st, n, nn.x.p, A.h3, th, nn.s.p, L.D.p, nn.y.p, c_out, u_add);
+}
+
+// Loss (into *loss_out, device) and gradient (into nn.grad) of a batch of B
+// residuals stacked in r.
+template <class AT>
+void loss_grad_t(dec_nn_s &nn, const dec::NnLevel &L, int B, const float *r, double *loss_out, hipStream_t st) {
+  forward_t<AT>(nn, L, B, r, nullptr, nullptr, st);
+  int n = L.n;
+  int R = n * B;
+  Acts<AT> A = acts<AT>(nn);
+  const float *th = nn.th.p;
+  float *g = nn.grad.p;
+  dim3 gv(dec::nblocks(n, TB), B), gf(dec::nblocks(static_cast<long long>(n) * H, TB), B);
+  int nbv = gv.x * B, nbf = gf.x * B;
+  hipLaunchKernelGGL(k_loss, gv, dim3(TB), 0, st, n, B, L.a.width, L.a.col.p, L.a.val.p, nn.x.p, nn.y.p, L.D.p,
+                     nullptr, nn.drho.p, nn.dpart.p);
+  hipLaunchKernelGGL(k_reduce_parts_d, dim3(1), dim3(1), 0, st, nn.dpart.p, nbv, 1.0 / (static_cast<double>(n) * B),
+                     loss_out);
+  // output head: dz3, and the (wo, bo, alpha, b3) partials
+  hipLaunchKernelGGL(k_out_back<AT>, gv, dim3(TB), 0, st, n, L.at.width, L.at.col.p, L.at.val.p, nn.drho.p, L.D.p,
+                     nn.x.p, A.h3, th, A.dz, nn.part.p);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(64), 0, st, nn.part.p, nbv, 2 * H + 2, 0, H + 2, g + OWO);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(64), 0, st, nn.part.p, nbv, 2 * H + 2, H + 2, H, g + OB3);
+  // dense layers 3 and 2: weight gradient, input gradient, mix through Pᵀ
+  auto layer = [&](const AT *gh, const AT *gm, int ow, const AT *h_below, int ob_below) {
+    if constexpr (sizeof(AT) == 2) {
+      int waves = (R + 255) / 256;
+      hipLaunchKernelGGL(k_wgrad_bf16, dim3((waves + 3) / 4), dim3(TB), 0, st, R, gh, gm, A.dz, nn.part.p);
+      hipLaunchKernelGGL(k_reduce_parts, dim3(dec::nblocks(2 * H * H, TB)), dim3(TB), 0, st, nn.part.p, waves,
+                         2 * H * H, 0, 2 * H * H, g + ow);
+      int dw = (R + 15) / 16;
+      hipLaunchKernelGGL(k_dgrad_bf16, dim3((dw + 3) / 4), dim3(TB), 0, st, R, A.dz, th + ow, nn.dg.p);
+    } else {
+      int chunks = (R + 255) / 256;
+      hipLaunchKernelGGL(k_wgrad_f32, dim3(chunks), dim3(TB), 0, st, R, gh, gm, A.dz, nn.part.p);
+      hipLaunchKernelGGL(k_reduce_parts, dim3(dec::nblocks(2 * H * H, TB)), dim3(TB), 0, st, nn.part.p, chunks,
+                         2 * H * H, 0, 2 * H * H, g + ow);
+      hipLaunchKernelGGL(k_dgrad_f32, dim3(dec::nblocks(static_cast<long long>(R) * 2 * H, TB)), dim3(TB), 0, st, R,
+                         A.dz, th + ow, nn.dg.p);
+    }
+    // dz of the layer below (overwrites dz) and its bias partials
+    hipLaunchKernelGGL(k_back_mix<AT>, gf, dim3(TB), 0, st, n, L.pt.width, L.pt.col.p, L.pt.val.p, nn.dg.p, h_below,
+                       A.dz, nn.part.p);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(64), 0, st, nn.part.p, nbf, H, 0, H, g + ob_below);
+  };
+  layer(A.h2, A.m2, OW3, A.h2, OB2);
+  layer(A.h1, A.m1, OW2, A.h1, OB1);
+  // first layer: dW1 = f0ᵀ dz1
+  dim3 g1(dec::nblocks(n, TB / H), B);
+  hipLaunchKernelGGL(k_w1_back<AT>, g1, dim3(TB), 0, st, n, nn.x.p, nn.px.p, A.dz, nn.part.p);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(64), 0, st, nn.part.p, static_cast<int>(g1.x) * B, 2 * H, 0, 2 * H,
+                     g + OW1);
+}
+
+void lcheck(hipError_t e, const char *what) {
+  if (e != hipSuccess) fail(DEC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+const dec::NnLevel &level_of(dec_nn_s &nn, int lvl) {
+  if (lvl < 0 || lvl >= static_cast<int>(nn.levels.size())) fail(DEC_ERR_ARG, "learned smoother: bad level");
+  return *nn.levels[lvl];
+}
+
+}  // namespace
+
+namespace dec {
+
+void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s) {
+  forward_t<unsigned short>(nn, level_of(nn, lvl), 1, r, nullptr, u, s);
+}
+
+dec_nn_s *nn_create(const std::vector<const HostCsr *> &ops, uint64_t seed) {
+  auto nn = std::make_unique<dec_nn_s>();
+  for (const HostCsr *A : ops) nn->levels.push_back(std::make_unique<NnLevel>(*A));
+  // initialisation: W ~ N(0, 2/fan_in) (seeded), biases and output head zero,
+  // alpha = 2/3: the untrained network is damped Jacobi
+  std::vector<float> th(NP, 0.0f);
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  for (int i = 0; i < 2 * H; i++) th[OW1 + i] = static_cast<float>(nd(rng) * std::sqrt(2.0 / 2));
+  for (int i = 0; i < 2 * H * H; i++) th[OW2 + i] = static_cast<float>(nd(rng) * std::sqrt(2.0 / (2 * H)));
+  for (int i = 0; i < 2 * H * H; i++) th[OW3 + i] = static_cast<float>(nd(rng) * std::sqrt(2.0 / (2 * H)));
+  th[OAL] = 2.0f / 3.0f;
+  nn->th.upload(th);
+  std::vector<float> z(NP, 0.0f);
+  nn->grad.upload(z);
+  nn->m1.upload(z);
+  nn->m2.upload(z);
+  nn->dloss.alloc(1);
+  return nn.release();
+}
+
+}  // namespace dec
+
+extern "C" {
+
+int dec_nn_nparams(dec_nn nn, int32_t *n) {
+  DEC_TRY
+  if (!nn || !n) fail(DEC_ERR_ARG, "dec_nn_nparams: NULL argument");
+  *n = NP;
+  DEC_CATCH
+}
+
+int dec_nn_get_params(dec_nn nn, float *host) {
+  DEC_TRY
+  if (!nn || !host) fail(DEC_ERR_ARG, "dec_nn_get_params: NULL argument");
+  dec::d2h(host, nn->th.p, sizeof(float) * NP);
+  DEC_CATCH
+}
+
+int dec_nn_set_params(dec_nn nn, const float *host) {
+  DEC_TRY
+  if (!nn || !host) fail(DEC_ERR_ARG, "dec_nn_set_params: NULL argument");
+  dec::h2d(nn->th.p, host, sizeof(float) * NP);
+  nn->adam_t = 0;
+  std::vector<float> z(NP, 0.0f);
+  dec::h2d(nn->m1.p, z.data(), sizeof(float) * NP);
+  dec::h2d(nn->m2.p, z.data(), sizeof(float) * NP);
+  DEC_CATCH
+}
+
+int dec_nn_forward(dec_nn nn, int32_t level, int32_t nsamples, const float *r, float *c, int32_t precise,
+                   void *stream) {
+  DEC_TRY
+  if (!nn || !r || !c || nsamples <= 0) fail(DEC_ERR_ARG, "dec_nn_forward: bad argument");
+  const dec::NnLevel &L = level_of(*nn, level);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precise) forward_t<float>(*nn, L, nsamples, r, c, nullptr, s);
+  else forward_t<unsigned short>(*nn, L, nsamples, r, c, nullptr, s);
+  dec::check_launch("dec_nn_forward");
+  DEC_CATCH
+}
+
+int dec_nn_loss_grad(dec_nn nn, int32_t level, int32_t nsamples, const float *r, int32_t precise, double *loss,
+                     float *grad, void *stream) {
+  DEC_TRY
+  if (!nn || !r || nsamples <= 0) fail(DEC_ERR_ARG, "dec_nn_loss_grad: bad argument");
+  const dec::NnLevel &L = level_of(*nn, level);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precise) loss_grad_t<float>(*nn, L, nsamples, r, nn->dloss.p, s);
+  else loss_grad_t<unsigned short>(*nn, L, nsamples, r, nn->dloss.p, s);
+  dec::check_launch("dec_nn_loss_grad");
+  lcheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (loss) dec::d2h(loss, nn->dloss.p, sizeof(double));
+  if (grad) dec::d2h(grad, nn->grad.p, sizeof(float) * NP);
+  DEC_CATCH
+}
+
+int dec_nn_train(dec_nn nn, int32_t level, int32_t npairs, const float *R, int32_t steps, int32_t batch, float lr,
+                 uint64_t seed, double *loss_hist, void *stream) {
+  DEC_TRY
+  if (!nn || !R || npairs <= 0 || steps < 0 || batch <= 0) fail(DEC_ERR_ARG, "dec_nn_train: bad argument");
+  const dec::NnLevel &L = level_of(*nn, level);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int n = L.n;
+  std::mt19937_64 rng(seed);
+  std::uniform_int_distribution<int> pick(0, npairs - 1);
+  std::vector<int32_t> ids(static_cast<size_t>(steps) * batch);
+  for (auto &i : ids) i = pick(rng);
+  DBuf<int32_t> dids;
+  dids.upload(ids);
+  DBuf<float> rb;
+  rb.alloc(static_cast<size_t>(n) * batch);
+  DBuf<double> hist;
+  hist.alloc(std::max(steps, 1));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int k = 0; k < steps; k++) {
+    hipLaunchKernelGGL(k_gather_rows, dim3(dec::nblocks(n, TB), batch), dim3(TB), 0, s, n, R,
+                       dids.p + static_cast<size_t>(k) * batch, rb.p);
+    loss_grad_t<unsigned short>(*nn, L, batch, rb.p, hist.p + k, s);
+    nn->adam_t++;
+    float c1 = 1.0f - std::pow(b1, static_cast<float>(nn->adam_t));
+    float c2 = 1.0f - std::pow(b2, static_cast<float>(nn->adam_t));
+    hipLaunchKernelGGL(k_adam, dim3(dec::nblocks(NP, TB)), dim3(TB), 0, s, NP, nn->th.p, nn->grad.p, nn->m1.p,
+                       nn->m2.p, lr, b1, b2, eps, c1, c2);
+  }
+  dec::check_launch("dec_nn_train");
+  lcheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (loss_hist && steps > 0) dec::d2h(loss_hist, hist.p, sizeof(double) * steps);
+  DEC_CATCH
+}
+
+int dec_nn_destroy(dec_nn nn) {
+  DEC_TRY
+  delete nn;
+  DEC_CATCH
+}
+
+}  // extern "C"

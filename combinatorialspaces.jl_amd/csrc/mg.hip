@@ -16,6 +16,7 @@
 #include "dec_internal.h"
 #include "ell_rows.h"
 #include "kernels_common.h"
+#include "learned.h"
 
 using dec::DBuf;
 using dec::fail;
@@ -38,7 +39,7 @@ template <int W, class T>
 __global__ void k_residual(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
                            const int *__restrict__ rowid, const T *__restrict__ b, const T *__restrict__ u,
                            T *__restrict__ r) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int p = dec_ell::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   T acc = dec_ell::row<W, T, T>(n, width, p, col, val, u);
   int v = rowid[p];
@@ -61,7 +62,7 @@ template <int W, class T>
 __global__ void k_gs_color(int p0, int p1, int n, int width, const int *__restrict__ col,
                            const T *__restrict__ val, const int *__restrict__ rowid, const T *__restrict__ b,
                            T *__restrict__ u) {
-  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  int p = p0 + dec_ell::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= p1) return;
   int v = rowid[p];
   T s, d;
@@ -411,6 +412,11 @@ struct dec_mg_s {
   int gkey[2] = {-1, -1};
   hipStream_t cap = nullptr;
   bool use_graph = true;
+  // learned smoother (float32 cycles, levels with at least learned_min_rows)
+  dec_nn_s *nn = nullptr;
+  int learned = 0, learned_min_rows = 16385, learned_apps = 1;
+  bool learned_active = false;
+  int fallbacks = 0, learned_cycles = 0;
   ~dec_mg_s() {
     for (int i = 0; i < 2; i++) {
       if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
@@ -675,7 +681,8 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
 template <class T>
 void residual(Level &L, const T *b, const T *u, T *r, hipStream_t s) {
   LevelVals<T> &V = vals<T>(L);
-  DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_residual<W, T>), G(L.n), dim3(kBlock), 0, s, L.n, L.width,
+  DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_residual<W, T>), dim3(dec::xcd_grid(L.n)), dim3(kBlock), 0, s,
+                                                   L.n, L.width,
                                                    L.col.p, V.val.p, L.rowid.p, b, u, r));
 }
 
@@ -709,10 +716,19 @@ void cg_smooth(dec_mg_s &mg, Level &L, const double *b, double *u, int steps, hi
 }
 
 template <class T>
-void smooth(dec_mg_s &mg, Level &L, const T *b, T *u, bool post, hipStream_t s) {
+void smooth(dec_mg_s &mg, int lvl, Level &L, const T *b, T *u, bool post, hipStream_t s) {
   int steps = mg.steps;
   if (steps == 0) return;
   LevelVals<T> &V = vals<T>(L);
+  if constexpr (sizeof(T) == 4) {
+    if (mg.nn && mg.learned && mg.learned_active && L.n >= mg.learned_min_rows) {
+      for (int a = 0; a < mg.learned_apps; a++) {
+        residual<float>(L, b, u, V.r.p, s);
+        dec::nn_smooth(*mg.nn, lvl, V.r.p, u, s);
+      }
+      return;
+    }
+  }
   if (mg.alg == DEC_SMOOTH_CG) {
     if constexpr (sizeof(T) == 8) cg_smooth(mg, L, b, u, steps, s);
     else fail(DEC_ERR_UNSUPPORTED, "the CG smoother runs in float64 only");
@@ -724,7 +740,8 @@ void smooth(dec_mg_s &mg, Level &L, const T *b, T *u, bool post, hipStream_t s) 
         int c = post ? L.ncolors - 1 - ci : ci;
         int p0 = L.cstart[c], p1 = L.cstart[c + 1];
         if (p1 > p0)
-          DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_gs_color<W, T>), G(p1 - p0), dim3(kBlock), 0, s, p0, p1,
+          DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_gs_color<W, T>), dim3(dec::xcd_grid(p1 - p0)), dim3(kBlock),
+                                                           0, s, p0, p1,
                                                            L.n, L.width, L.col.p, V.val.p, L.rowid.p, b, u));
       }
     } else {
@@ -763,7 +780,7 @@ void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, hipStream_t s) {
     coarse_solve<T>(mg, L, b, u, s);
     return;
   }
-  smooth<T>(mg, L, b, u, false, s);
+  smooth<T>(mg, lvl, L, b, u, false, s);
   if (lvl >= nl - 1) return;
   Level &C = *mg.lv[lvl + 1];
   LevelVals<T> &V = vals<T>(L);
@@ -775,7 +792,7 @@ void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, hipStream_t s) {
   mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu, s);
   if (mu > 1) mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu - 1, s);
   hipLaunchKernelGGL(k_prolong_add<T>, G(L.n), dim3(kBlock), 0, s, L.n, L.nc, L.pa.p, L.pb.p, CV.u.p, u);
-  smooth<T>(mg, L, b, u, true, s);
+  smooth<T>(mg, lvl, L, b, u, true, s);
 }
 
 // _full_multigrid (Multigrid.jl:839-855).
@@ -828,7 +845,8 @@ void run_cycle(dec_mg_s &mg, int variant, double *u, const double *b, hipStream_
     cycle_body(mg, variant, u, b, s);
     return;
   }
-  int key = mg.alg * 1000003 + mg.steps * 1009 + mg.coarse_direct * 7 + static_cast<int>(mg.omega * 1e6);
+  int key = mg.alg * 1000003 + mg.steps * 1009 + mg.coarse_direct * 7 + static_cast<int>(mg.omega * 1e6) +
+            (mg.learned && mg.learned_active ? 1 << 29 : 0) + mg.learned_apps * 31 + mg.learned_min_rows;
   int i = variant == DEC_SOLVE_PLAIN_F64 ? 0 : 1;
   if (!mg.gexec[i] || mg.gu[i] != u || mg.gb[i] != b || mg.gkey[i] != key) {
     if (mg.gexec[i]) (void)hipGraphExecDestroy(mg.gexec[i]);
@@ -858,12 +876,23 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u, con
     *relres = 0.0;
     return;
   }
-  double rr = 0.0;
+  double rr = 0.0, prev = 0.0;
   int c = 0;
+  mg.learned_active = mg.learned && mg.nn && variant == DEC_SOLVE_MIXED_IR;
+  mg.fallbacks = 0;
+  mg.learned_cycles = 0;
   for (;;) {
     residual<double>(F, b, u, F.d.r.p, s);
     rr = norm2(mg, n, F.d.r.p, s) / bn;
+    // a learned cycle that cut the residual by less than 0.5x hands the rest
+    // of the solve to Gauss-Seidel
+    if (c > 0 && mg.learned_active && rr > 0.5 * prev) {
+      mg.learned_active = false;
+      mg.fallbacks++;
+    }
     if (rr <= tol || c >= max_cycles) break;
+    if (mg.learned_active) mg.learned_cycles++;
+    prev = rr;
     run_cycle(mg, variant, u, b, s);
     c++;
   }
@@ -981,6 +1010,35 @@ int dec_mg_solve(dec_mg mg, int32_t variant, double tol, int32_t max_cycles, voi
   }
   if (cycles_done) *cycles_done = c;
   if (relres) *relres = rr;
+  DEC_CATCH
+}
+
+int dec_nn_create(dec_mg mg, uint64_t seed, dec_nn *out) {
+  DEC_TRY
+  if (!mg || !out) fail(DEC_ERR_ARG, "dec_nn_create: NULL argument");
+  std::vector<const dec::HostCsr *> ops;
+  for (auto &L : mg->lv) ops.push_back(&L->A);
+  *out = dec::nn_create(ops, seed);
+  DEC_CATCH
+}
+
+int dec_mg_set_learned(dec_mg mg, dec_nn nn, int32_t enable, int32_t min_rows, int32_t applications) {
+  DEC_TRY
+  if (!mg) fail(DEC_ERR_ARG, "dec_mg_set_learned: NULL handle");
+  if (enable && !nn) fail(DEC_ERR_ARG, "dec_mg_set_learned: no network");
+  if (applications < 1) fail(DEC_ERR_ARG, "dec_mg_set_learned: applications must be >= 1");
+  mg->nn = nn;
+  mg->learned = enable != 0;
+  mg->learned_min_rows = min_rows;
+  mg->learned_apps = applications;
+  DEC_CATCH
+}
+
+int dec_mg_learned_stats(dec_mg mg, int32_t *fallbacks, int32_t *learned_cycles) {
+  DEC_TRY
+  if (!mg) fail(DEC_ERR_ARG, "dec_mg_learned_stats: NULL handle");
+  if (fallbacks) *fallbacks = mg->fallbacks;
+  if (learned_cycles) *learned_cycles = mg->learned_cycles;
   DEC_CATCH
 }
 

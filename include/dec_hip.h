@@ -28,6 +28,7 @@ typedef struct dec_mesh_s *dec_mesh;
 typedef struct dec_op_s *dec_op;
 typedef struct dec_mg_s *dec_mg;
 typedef struct dec_batch_s *dec_batch;
+typedef struct dec_nn_s *dec_nn;
 
 enum dec_status {
   DEC_OK = 0,
@@ -168,6 +169,37 @@ int dec_mg_solve(dec_mg mg, int32_t variant, double tol, int32_t max_cycles,
                  void *u, const void *b, int32_t *cycles_done, double *relres,
                  void *stream);
 int dec_mg_destroy(dec_mg mg);
+
+/* ------------------------------------------------------- learned smoother */
+/* A 3-layer edge message-passing network (hidden 32) used as the multigrid
+ * smoother: c = N(r) for a residual r on a level of a hierarchy, with dense
+ * layers on BF16 MFMA (FP32 accumulate).  No counterpart in the reference,
+ * whose smoother is Krylov.jl's cg (Multigrid.jl:860).  Parameters are a flat
+ * float vector (dec_nn_nparams entries); the untrained network (seeded
+ * initialisation) is damped Jacobi.  r, c are float32 device arrays of
+ * nsamples x n (samples back to back, vertex order).  precise=1 runs the
+ * dense layers in FP32 on the VALU with FP32 activations (for checking). */
+int dec_nn_create(dec_mg mg, uint64_t seed, dec_nn *out);
+int dec_nn_nparams(dec_nn nn, int32_t *n);
+int dec_nn_get_params(dec_nn nn, float *host);
+int dec_nn_set_params(dec_nn nn, const float *host);
+int dec_nn_forward(dec_nn nn, int32_t level, int32_t nsamples, const float *r, float *c,
+                   int32_t precise, void *stream);
+/* Loss mean((r - A c)² / s²) over the samples and its gradient (host copies). */
+int dec_nn_loss_grad(dec_nn nn, int32_t level, int32_t nsamples, const float *r,
+                     int32_t precise, double *loss, float *grad, void *stream);
+/* Adam on minibatches drawn (seeded) from the npairs residuals in R
+ * (npairs x n, device); loss_hist (host, steps entries) may be NULL. */
+int dec_nn_train(dec_nn nn, int32_t level, int32_t npairs, const float *R, int32_t steps,
+                 int32_t batch, float lr, uint64_t seed, double *loss_hist, void *stream);
+int dec_nn_destroy(dec_nn nn);
+/* Use the network as the smoother of the float32 V-cycles (DEC_SOLVE_MIXED_IR)
+ * on the levels with at least min_rows vertices (Gauss-Seidel below).  During
+ * dec_mg_solve, a cycle whose residual drops by less than 0.5x switches the
+ * rest of the solve back to Gauss-Seidel; dec_mg_learned_stats reports those
+ * fallbacks and the cycles run with the network. */
+int dec_mg_set_learned(dec_mg mg, dec_nn nn, int32_t enable, int32_t min_rows, int32_t applications);
+int dec_mg_learned_stats(dec_mg mg, int32_t *fallbacks, int32_t *learned_cycles);
 
 /* ------------------------------------------------------------------ batches */
 /* Many explicit heat-equation instances u_t = Δu on meshes of uneven size,

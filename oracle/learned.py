@@ -1,0 +1,182 @@
+"""CPU oracle of the learned multigrid smoother (float64 numpy).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP network kernels.
+
+The network has no reference counterpart; it is this build's smoother.  For a
+residual r on a level with operator A (CSR rows, columns ascending) it returns
+a correction c:
+
+    s   = sqrt(mean(r²))                     (scale; c is linear in s)
+    x   = r / s
+    P   = D⁻¹ (D - A) with D = diag(A)        (normalized neighbour weights)
+    f0  = [x, P x]                           (N x 2)
+    h1  = relu(f0 W1 + b1)                   (N x 32)
+    h2  = relu([h1, P h1] W2 + b2)           (N x 32)   dense 64 -> 32
+    h3  = relu([h2, P h2] W3 + b3)           (N x 32)   dense 64 -> 32
+    y   = alpha x + h3 wo + bo               (N)
+    c   = (s / D) y
+
+Training loss over a batch of residuals: mean over samples and vertices of
+((r - A c) / s)², the residual left after the correction.
+
+Gradients are pinned by central finite differences in the tests; nothing else
+fixes them (parity of the network is against this restatement only).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+HIDDEN = 32
+
+# parameter layout (flat float vector), shared with the HIP kernels
+SHAPES = [("W1", (2, HIDDEN)), ("b1", (HIDDEN,)), ("W2", (2 * HIDDEN, HIDDEN)), ("b2", (HIDDEN,)),
+          ("W3", (2 * HIDDEN, HIDDEN)), ("b3", (HIDDEN,)), ("wo", (HIDDEN,)), ("bo", (1,)), ("alpha", (1,))]
+
+
+def n_params():
+    return sum(int(np.prod(s)) for _, s in SHAPES)
+
+
+def unpack(theta):
+    out, k = {}, 0
+    for name, shp in SHAPES:
+        n = int(np.prod(shp))
+        out[name] = np.asarray(theta[k:k + n], dtype=np.float64).reshape(shp)
+        k += n
+    return out
+
+
+def pack(p):
+    return np.concatenate([np.asarray(p[name], np.float64).reshape(-1) for name, _ in SHAPES])
+
+
+def init_params(seed=0, alpha=2 / 3):
+    """Deterministic initialisation: small weights, output head zero, alpha =
+    2/3 so the untrained network is exactly damped Jacobi."""
+    rng = np.random.default_rng(seed)
+    p = {}
+    for name, shp in SHAPES:
+        if name.startswith("W"):
+            fan_in = shp[0]
+            p[name] = rng.standard_normal(shp) * np.sqrt(2.0 / fan_in)
+        else:
+            p[name] = np.zeros(shp)
+    p["alpha"] = np.array([alpha])
+    return pack(p)
+
+
+class Level:
+    """A level's operator in CSR (rowptr, col, val) with its P = D⁻¹(D - A)."""
+
+    def __init__(self, rowptr, col, val):
+        self.rp, self.ci, self.vv = (np.asarray(rowptr), np.asarray(col), np.asarray(val, np.float64))
+        n = len(self.rp) - 1
+        self.n = n
+        rows = np.repeat(np.arange(n), np.diff(self.rp))
+        self.rows = rows
+        diag = np.zeros(n)
+        m = self.ci == rows
+        diag[rows[m]] = self.vv[m]
+        self.D = diag
+        self.pval = np.where(m, 0.0, -self.vv / diag[rows])
+
+    def A(self, x):
+        return np.bincount(self.rows, self.vv * x[self.ci], minlength=self.n) if x.ndim == 1 else \
+            np.stack([self.A(x[:, j]) for j in range(x.shape[1])], axis=1)
+
+    def AT(self, x):
+        return np.bincount(self.ci, self.vv * x[self.rows], minlength=self.n)
+
+    def P(self, X):
+        """Σ_w a_vw X[w] for X of shape (N,) or (N, F)."""
+        if X.ndim == 1:
+            return np.bincount(self.rows, self.pval * X[self.ci], minlength=self.n)
+        out = np.zeros_like(X)
+        np.add.at(out, self.rows, self.pval[:, None] * X[self.ci])
+        return out
+
+    def PT(self, X):
+        out = np.zeros_like(X)
+        np.add.at(out, self.ci, self.pval[:, None] * X[self.rows]) if X.ndim > 1 else \
+            np.add.at(out, self.ci, self.pval * X[self.rows])
+        return out
+
+
+def relu(z):
+    return np.maximum(z, 0.0)
+
+
+def forward(theta, lv: Level, r):
+    p = unpack(theta)
+    s = np.sqrt(np.mean(r * r))
+    x = r / s
+    f0 = np.stack([x, lv.P(x)], axis=1)
+    z1 = f0 @ p["W1"] + p["b1"]
+    h1 = relu(z1)
+    g1 = np.concatenate([h1, lv.P(h1)], axis=1)
+    z2 = g1 @ p["W2"] + p["b2"]
+    h2 = relu(z2)
+    g2 = np.concatenate([h2, lv.P(h2)], axis=1)
+    z3 = g2 @ p["W3"] + p["b3"]
+    h3 = relu(z3)
+    y = p["alpha"][0] * x + h3 @ p["wo"] + p["bo"][0]
+    c = (s / lv.D) * y
+    cache = dict(s=s, x=x, f0=f0, h1=h1, g1=g1, h2=h2, g2=g2, h3=h3, y=y, p=p)
+    return c, cache
+
+
+def loss(theta, lv: Level, R):
+    """Mean over samples (rows of R) and vertices of ((r - A c)/s)²."""
+    tot = 0.0
+    for r in R:
+        c, k = forward(theta, lv, r)
+        rho = (r - lv.A(c)) / k["s"]
+        tot += np.mean(rho * rho)
+    return tot / len(R)
+
+
+def loss_grad(theta, lv: Level, R):
+    g = np.zeros_like(theta)
+    tot = 0.0
+    B = len(R)
+    for r in R:
+        c, k = forward(theta, lv, r)
+        p = k["p"]
+        s, x = k["s"], k["x"]
+        rho = (r - lv.A(c)) / s
+        tot += np.mean(rho * rho)
+        N = lv.n
+        drho = 2.0 * rho / (N * B)
+        # rho = x - A (y / D)  (c / s = y / D)
+        dt = -lv.AT(drho)
+        dy = dt / lv.D
+        d = {name: np.zeros(shp) for name, shp in SHAPES}
+        d["alpha"][0] = dy @ x
+        d["bo"][0] = dy.sum()
+        d["wo"] = k["h3"].T @ dy
+        dh3 = np.outer(dy, p["wo"])
+        dz3 = dh3 * (k["h3"] > 0)
+        d["W3"] = k["g2"].T @ dz3
+        d["b3"] = dz3.sum(axis=0)
+        dg2 = dz3 @ p["W3"].T
+        dh2 = dg2[:, :HIDDEN] + lv.PT(dg2[:, HIDDEN:])
+        dz2 = dh2 * (k["h2"] > 0)
+        d["W2"] = k["g1"].T @ dz2
+        d["b2"] = dz2.sum(axis=0)
+        dg1 = dz2 @ p["W2"].T
+        dh1 = dg1[:, :HIDDEN] + lv.PT(dg1[:, HIDDEN:])
+        dz1 = dh1 * (k["h1"] > 0)
+        d["W1"] = k["f0"].T @ dz1
+        d["b1"] = dz1.sum(axis=0)
+        g += pack(d)
+    return tot / B, g
+
+
+def fd_grad(theta, lv, R, idx, h=1e-6):
+    out = []
+    for i in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        out.append((loss(tp, lv, R) - loss(tm, lv, R)) / (2 * h))
+    return np.array(out)
