@@ -207,6 +207,61 @@ def poisson(torch, cs, base_k=3, levels=5):
         key = "plain_f64" if variant == "plain" else "mixed_f32_ir"
         out[key] = {"seconds": dt, "cycles": cyc, "relres_f64": rr}
     out["speedup_mixed_vs_plain"] = out["plain_f64"]["seconds"] / out["mixed_f32_ir"]["seconds"]
+    return out, md, fd
+
+
+def learned(torch, cs, md, fd, npairs=10000, train_levels=3, steps=600, batch=16):
+    """The learned smoother in the level 3->8 mixed solve.  Training pairs come
+    from plain V-cycles on seeded spherical-harmonic right-hand sides on the
+    3->(3+train_levels) hierarchy; the trained parameters (level independent)
+    are loaded into the network bound to md."""
+    t = time.perf_counter()
+    mdt = cs.MultigridData(cs.icosphere(3), cs.BinarySubdivision, train_levels, cs.DiagonalHodge, 2, project=True)
+    R, E = cs.make_pairs(mdt, npairs, cycles_per_rhs=6, seed=11)
+    del E
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t
+    nn_t = cs.LearnedSmoother(mdt, seed=5)
+    val = R[-512:].contiguous()
+    l0, _ = nn_t.loss_grad(0, val)
+    t = time.perf_counter()
+    hist = nn_t.train(0, R[:-512], steps=steps, batch=batch, lr=2e-3, seed=6)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t
+    l1, _ = nn_t.loss_grad(0, val)
+    params = nn_t.get_params()
+    del R, val
+    nn = cs.LearnedSmoother(md, seed=5)
+    nn.set_params(params)
+    out = {"pairs": npairs, "train_mesh_vertices": mdt.sizes[0], "pairs_s": t_pairs, "train_steps": steps,
+           "train_batch": batch, "train_s": t_train, "val_loss_untrained": l0, "val_loss_trained": l1,
+           "loss_first": float(hist[0]), "loss_last": float(hist[-1])}
+    # one forward pass over a batch at the finest level (the BF16 MFMA layers)
+    B = 8
+    Rf = torch.randn(B, md.sizes[0], dtype=torch.float32, device="cuda")
+    for _ in range(2):
+        nn.forward(0, Rf)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    reps = 10
+    for _ in range(reps):
+        nn.forward(0, Rf)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t) / reps * 1e3
+    out["forward_ms_per_vector_fine"] = fwd_ms / B
+    for min_rows in (0, 16385, 40963):
+        nn.attach(True, min_rows=min_rows)
+        cs.solve(md, fd, tol=1e-8, variant="mixed")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        u, cyc, rr = cs.solve(md, fd, tol=1e-8, variant="mixed")
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        st = cs.learned_stats(md)
+        out[f"mixed_learned_min_rows_{min_rows}"] = {
+            "seconds": dt, "cycles": cyc, "relres_f64": rr, "fallbacks": st["fallbacks"],
+            "learned_cycles": st["learned_cycles"]}
+    nn.attach(False)
     return out
 
 
@@ -254,6 +309,126 @@ def cpu_baseline(k=7, budget_s=12.0):
                       f"operator set in {dt:.1f} s"}
 
 
+def batched_heat(torch, cs, ninst=64, nsteps=100, seed=7):
+    """64 explicit heat instances u ← u - dt L u on a seeded uneven mix of
+    icosphere levels 4-7, FP32, in one persistent launch per step replayed
+    from a hipGraph, launch shape from the autotuner; checked against the same
+    steps in FP64."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    levels = [int(x) for x in rng.choice([4, 5, 6, 7], size=ninst, p=[0.4, 0.3, 0.2, 0.1])]
+    meshes = {k: cs.dualize(cs.icosphere(k)) for k in sorted(set(levels))}
+    inst = [meshes[k] for k in levels]
+    b32 = cs.HeatBatch(inst, torch.float32)
+    b64 = cs.HeatBatch(inst, torch.float64)
+    u64 = torch.empty(b64.total, dtype=torch.float64, device="cuda")
+    for i in range(ninst):
+        cs.fill_uniform(b64.instance(u64, i), 500 + i)
+    u32 = u64.float()
+    t = time.perf_counter()
+    shape = b32.tune(u32.clone())
+    tune_s = time.perf_counter() - t
+    u = u32.clone()
+    b32.steps(u, nsteps, graph=True)       # captures the graph for (u, nsteps)
+    u.copy_(u32)
+    stream = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    b32.steps(u, nsteps, graph=True)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b)
+    v = u64.clone()
+    b64.steps(v, nsteps, graph=True)
+    err = max(float((b32.instance(u, i).double() - b64.instance(v, i)).norm() / b64.instance(v, i).norm())
+              for i in range(ninst))
+    step_bytes = 0
+    for k in levels:
+        nv, ne = 10 * 4 ** k + 2, 30 * 4 ** k
+        step_bytes += (nv + 2 * ne) * (4 + 4) + 2 * nv * 4
+    return {"instances": ninst, "levels": {str(k): levels.count(k) for k in sorted(set(levels))},
+            "total_vertices": int(b32.total), "steps": nsteps, "ms_per_step": ms / nsteps,
+            "instance_steps_per_s": ninst * nsteps / (ms * 1e-3), "GBps": step_bytes / (ms / nsteps * 1e-3) / 1e9,
+            "launch_shape": shape, "tune_s": tune_s, "max_rel_err_f32_vs_f64": err}
+
+
+COPY_MIB = 256  # calibration copy: 256 MiB read + 256 MiB written per launch
+
+
+def pmc_probe(op_name, k, sets, launches):
+    """The program that the counter passes profile: one plain copy of known
+    size (calibration) and `launches` back-to-back launches of one operator
+    over `sets` rotating table/buffer sets, as in the timed run."""
+    import torch
+    import cspaces as cs
+
+    torch.cuda.init()
+    x = torch.ones(COPY_MIB * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    mesh = cs.icosphere(k)
+    cs.dualize(mesh)
+    time_opset(torch, cs, mesh, [op_name], torch.float64, sets, launches, 2, False)
+    torch.cuda.synchronize()
+
+
+def pmc_counter(counter, op_name, kernel, k, sets, launches, timeout_s=240):
+    """Mean per-dispatch value (KiB) of one rocprofv3 counter for `kernel` and
+    for the calibration copy, from a child `rocprofv3 --pmc <counter>` run of
+    pmc_probe.  Returns (kernel KiB, copy KiB) or raises RuntimeError."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        raise RuntimeError("rocprofv3 not found")
+    out = tempfile.mkdtemp(prefix="dec_pmc_", dir="/tmp")
+    cmd = ["timeout", "-s", "KILL", str(timeout_s), exe, "--pmc", counter, "--output-format", "csv", "-d", out,
+           "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--pmc-probe", op_name, "--k", str(k),
+           "--sets", str(sets), "--steps", str(launches)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} exited {r.returncode}: {r.stdout[-400:]}")
+    files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"rocprofv3 --pmc {counter}: no counter file")
+    kv, cv = [], []
+    for row in csv.DictReader(open(files[0])):
+        if row.get("Counter_Name") != counter:
+            continue
+        name = row["Kernel_Name"]
+        v = float(row["Counter_Value"])
+        if kernel + "<" in name or kernel + "(" in name:
+            kv.append(v)
+        elif "elementwise_kernel" in name and "copy" in name.lower():
+            cv.append(v)
+    shutil.rmtree(out, ignore_errors=True)
+    if not kv or not cv:
+        raise RuntimeError(f"rocprofv3 --pmc {counter}: kernel rows {len(kv)}, copy rows {len(cv)}")
+    return sum(kv) / len(kv), sum(cv) / len(cv)
+
+
+def traffic(op_name, kernel, k, sets):
+    """HBM bytes per launch of the dominant kernel from two counter passes
+    (FETCH_SIZE, then WRITE_SIZE), each scaled by what the counter reads for
+    the calibration copy (COPY_MIB MiB each way)."""
+    fk, fc = pmc_counter("FETCH_SIZE", op_name, kernel, k, sets, 24)
+    wk, wc = pmc_counter("WRITE_SIZE", op_name, kernel, k, sets, 24)
+    true_kib = COPY_MIB * 1024.0
+    f_fetch, f_write = fc / true_kib, wc / true_kib
+    return {"bytes": fk * 1024 / f_fetch + wk * 1024 / f_write, "fetch_KiB": fk, "write_KiB": wk,
+            "fetch_factor": f_fetch, "write_factor": f_write, "copy_MiB": COPY_MIB}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,7 +441,14 @@ def main():
     ap.add_argument("--no-f32", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the ops outside the headline set")
     ap.add_argument("--no-cache-resident", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--pmc-probe", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_probe:
+        pmc_probe(args.pmc_probe, args.k, args.sets, args.steps)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(see DESIGN.md for how the operators would shard).", file=sys.stderr)
@@ -302,6 +484,16 @@ def main():
     roofline = {"bound": "hbm", "kernel": kernel_name, "op": dom, "achieved": ops[dom]["GBps"],
                 "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": ops[dom]["GBps"] / HBM_PEAK_GBPS,
                 "traffic": None, "alg_bytes_per_launch": B[dom], "avg_launch_ms": per[dom]}
+    if not args.no_traffic:
+        try:
+            t = traffic(dom, kernel_name, args.k, args.sets)
+            roofline["traffic"] = t["bytes"]
+            roofline["traffic_detail"] = t
+            print(f"# traffic {dom}: {t['bytes'] / 1e6:.2f} MB per launch (algorithmic {B[dom] / 1e6:.2f} MB)",
+                  file=sys.stderr, flush=True)
+        except (RuntimeError, OSError) as e:
+            roofline["traffic_error"] = str(e)
+            print(f"# traffic: {e}", file=sys.stderr, flush=True)
 
     extra = {}
     extra_names = ["wedge_dd01", "boundary1", "boundary2"]
@@ -323,8 +515,15 @@ def main():
         extra["stream_copy_GBps"] = None
         print(f"# stream copy skipped: {e}", file=sys.stderr)
     if not args.no_solve:
-        extra["poisson"] = poisson(torch, cs)
+        extra["poisson"], md, fd = poisson(torch, cs)
         print(f"# poisson: {json.dumps(extra['poisson'])}", file=sys.stderr, flush=True)
+        if not args.no_learned:
+            extra["learned"] = learned(torch, cs, md, fd)
+            print(f"# learned: {json.dumps(extra['learned'])}", file=sys.stderr, flush=True)
+        del md, fd
+    if not args.no_batch:
+        extra["batched_heat"] = batched_heat(torch, cs)
+        print(f"# batched heat: {json.dumps(extra['batched_heat'])}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
 
     line = {

@@ -6,7 +6,9 @@ import os
 from ctypes import POINTER, c_char_p, c_double, c_int, c_int8, c_int32, c_int64, c_uint64, c_void_p
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "libdec_hip.so")
+# DEC_HIP_LIB names another build of the library (the Julia extension reads
+# the same variable)
+LIB_PATH = os.environ.get("DEC_HIP_LIB") or os.path.join(_PKG, "libdec_hip.so")
 
 DEC_OK = 0
 DEC_F64, DEC_F32 = 0, 1
@@ -47,6 +49,7 @@ SIGNATURES = {
     "dec_mg_level_size": (c_int, [P, c_int32, PI32]),
     "dec_mg_finest_mesh": (c_int, [P, POINTER(P)]),
     "dec_mg_export_level": (c_int, [P, c_int32, c_int32, PI64, P, P, P]),
+    "dec_mg_export_colors": (c_int, [P, c_int32, P, PI32]),
     "dec_mg_set_smoother": (c_int, [P, c_int32, c_int32, c_double, c_int32]),
     "dec_mg_cycles": (c_int, [P, c_int32, c_int32, P, P, P]),
     "dec_mg_full_multigrid": (c_int, [P, c_int32, c_int32, P, P, P]),
@@ -57,6 +60,7 @@ SIGNATURES = {
     "dec_nn_get_params": (c_int, [P, P]),
     "dec_nn_set_params": (c_int, [P, P]),
     "dec_nn_forward": (c_int, [P, c_int32, c_int32, P, P, c_int32, P]),
+    "dec_nn_smooth": (c_int, [P, c_int32, P, P, P]),
     "dec_nn_loss_grad": (c_int, [P, c_int32, c_int32, P, c_int32, PF64, P, P]),
     "dec_nn_train": (c_int, [P, c_int32, c_int32, P, c_int32, c_int32, ctypes.c_float, c_uint64, P, P]),
     "dec_nn_destroy": (c_int, [P]),

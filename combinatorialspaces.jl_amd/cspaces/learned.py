@@ -70,6 +70,17 @@ class LearnedSmoother:
                                    int(bool(precise)), _stream(stream)))
         return c
 
+    def smooth(self, level, r, u, stream=None):
+        """u += c(r) for one residual (float32 GPU vectors of the level's
+        size), with the inference kernels of the V-cycles."""
+        n = self.md.sizes[level]
+        for t in (r, u):
+            if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 4 and t.is_contiguous()
+                    and t.numel() == n):
+                raise TypeError(f"r and u must be contiguous float32 GPU vectors of {n} entries")
+        check(lib().dec_nn_smooth(self._h, level, c_void_p(r.data_ptr()), c_void_p(u.data_ptr()), _stream(stream)))
+        return u
+
     def loss_grad(self, level, R, precise=False, stream=None):
         B = self._R(R, level)
         loss = c_double()

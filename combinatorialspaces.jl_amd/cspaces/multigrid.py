@@ -98,6 +98,14 @@ class MultigridData:
         check(lib().dec_mg_export_level(self._h, level, w, None, ptr(rp), ptr(ci), ptr(vv)))
         return rp, ci, vv
 
+    def colors(self, level: int):
+        """Gauss-Seidel colour of each vertex of operators[level] and the
+        number of colours."""
+        c = np.zeros(self.sizes[level], np.int32)
+        k = c_int32()
+        check(lib().dec_mg_export_colors(self._h, level, ptr(c), ctypes.byref(k)))
+        return c, k.value
+
 
 MGData = MultigridData
 

@@ -375,7 +375,14 @@ int dec_batch_heat_steps(dec_batch b, void *u, int32_t nsteps, int32_t use_graph
     drop_graph(*b);
     if (!b->cap) check(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking), "hipStreamCreate");
     check(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    run(*b, u, nsteps, b->cap);
+    try {
+      run(*b, u, nsteps, b->cap);
+    } catch (...) {
+      hipGraph_t partial = nullptr;
+      (void)hipStreamEndCapture(b->cap, &partial);
+      if (partial) (void)hipGraphDestroy(partial);
+      throw;
+    }
     check(hipStreamEndCapture(b->cap, &b->graph), "hipStreamEndCapture");
     check(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     b->gu = u;

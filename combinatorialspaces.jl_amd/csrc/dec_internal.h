@@ -60,6 +60,15 @@ void vertex_dual_tris(const Topo &t, Csr &out);
 // greedy colouring of a symmetric sparsity pattern given as CSR adjacency
 int greedy_colors(int n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &adj,
                   std::vector<int32_t> &color);
+// Four colours for the vertices of a planar graph (CSR adjacency, symmetric):
+// greedy in smallest-last order, freeing a colour by a Kempe-chain swap where
+// a vertex sees all four; false when no swap frees one or the chains visited
+// exceed `budget` vertices in total.
+bool four_colors(int n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &adj,
+                 std::vector<int32_t> &color, long budget);
+// No two adjacent vertices share a colour and every colour is in [0, ncolors).
+bool proper_coloring(int n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &adj,
+                     const std::vector<int32_t> &color, int ncolors);
 // Dense coarse operator Z (row-major n x n): z = Z r solves A z = r, bordered
 // with constants when A annihilates them.  A given as dense row-major.
 void coarse_inverse(int n, const std::vector<double> &A, std::vector<double> &Z);

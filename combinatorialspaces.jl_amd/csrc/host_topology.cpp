@@ -2,6 +2,8 @@
 // orientation, binary refinement, incidence lists and colouring.  All
 // floating-point work (points, geometry, coefficients) runs on the GPU.
 #include <algorithm>
+#include <queue>
+#include <tuple>
 #include <cmath>
 #include <cstring>
 
@@ -330,6 +332,116 @@ int greedy_colors(int n, const std::vector<int32_t> &ptr, const std::vector<int3
     nc = std::max(nc, c + 1);
   }
   return nc;
+}
+
+namespace {
+
+bool four_colors_once(int n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &adj,
+                      std::vector<int32_t> &color, long budget, uint32_t salt) {
+  color.assign(n, -1);
+  // ties of the removal order are broken by a hash of (vertex, salt)
+  auto key = [salt](int v) {
+    uint32_t h = static_cast<uint32_t>(v) * 2654435761u ^ salt * 0x9e3779b9u;
+    h ^= h >> 15;
+    return static_cast<int>(h & 0x7fffffff);
+  };
+  // smallest-last order: repeatedly remove a vertex of least remaining degree
+  std::vector<int32_t> deg(n), order;
+  order.reserve(n);
+  std::vector<char> gone(n, 0);
+  using Item = std::tuple<int, int, int>;  // (remaining degree, tie key, vertex)
+  std::priority_queue<Item, std::vector<Item>, std::greater<>> pq;
+  for (int v = 0; v < n; v++) {
+    for (int k = ptr[v]; k < ptr[v + 1]; k++) deg[v] += adj[k] != v;
+    pq.push({deg[v], salt ? key(v) : v, v});
+  }
+  while (!pq.empty()) {
+    auto [d, t, v] = pq.top();
+    pq.pop();
+    if (gone[v] || d != deg[v]) continue;
+    gone[v] = 1;
+    order.push_back(v);
+    for (int k = ptr[v]; k < ptr[v + 1]; k++) {
+      int w = adj[k];
+      if (w != v && !gone[w]) {
+        --deg[w];
+        pq.push({deg[w], salt ? key(w) : w, w});
+      }
+    }
+  }
+  // colour in reverse removal order; a vertex that sees all four colours is
+  // freed by swapping the two colours a, b on the Kempe chains through its
+  // a-coloured neighbours, when those chains reach none of its b-coloured ones
+  std::vector<int32_t> stamp(n, 0), queue;
+  int gen = 0;
+  long work = 0;
+  for (int i = n - 1; i >= 0; i--) {
+    int v = order[i];
+    int used = 0;
+    for (int k = ptr[v]; k < ptr[v + 1]; k++)
+      if (adj[k] != v && color[adj[k]] >= 0) used |= 1 << color[adj[k]];
+    int c = 0;
+    while (c < 4 && (used >> c & 1)) c++;
+    if (c < 4) {
+      color[v] = c;
+      continue;
+    }
+    bool freed = false;
+    for (int a = 0; a < 4 && !freed; a++)
+      for (int b = 0; b < 4 && !freed; b++) {
+        if (a == b) continue;
+        ++gen;
+        queue.clear();
+        for (int k = ptr[v]; k < ptr[v + 1]; k++) {
+          int w = adj[k];
+          if (w != v && color[w] == a && stamp[w] != gen) {
+            stamp[w] = gen;
+            queue.push_back(w);
+          }
+        }
+        bool blocked = false;
+        for (size_t q = 0; q < queue.size() && !blocked; q++) {
+          int x = queue[q];
+          for (int k = ptr[x]; k < ptr[x + 1]; k++) {
+            int y = adj[k];
+            if (y == x || stamp[y] == gen || (color[y] != a && color[y] != b)) continue;
+            stamp[y] = gen;
+            queue.push_back(y);
+          }
+        }
+        work += static_cast<long>(queue.size());
+        if (work > budget) return false;
+        for (int k = ptr[v]; k < ptr[v + 1] && !blocked; k++)
+          if (adj[k] != v && color[adj[k]] == b && stamp[adj[k]] == gen) blocked = true;
+        if (blocked) continue;
+        for (int x : queue) color[x] = color[x] == a ? b : a;
+        color[v] = a;
+        freed = true;
+      }
+    if (!freed) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+bool four_colors(int n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &adj,
+                 std::vector<int32_t> &color, long budget) {
+  for (uint32_t salt = 0; salt < 8; salt++)
+    if (four_colors_once(n, ptr, adj, color, budget, salt)) return true;
+  color.clear();
+  return false;
+}
+
+bool proper_coloring(int n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &adj,
+                     const std::vector<int32_t> &color, int ncolors) {
+  if (static_cast<int>(color.size()) != n) return false;
+  for (int v = 0; v < n; v++) {
+    if (color[v] < 0 || color[v] >= ncolors) return false;
+    for (int k = ptr[v]; k < ptr[v + 1]; k++)
+      if (adj[k] != v && color[adj[k]] == color[v]) return false;
+  }
+  return true;
 }
 
 void coarse_inverse(int n, const std::vector<double> &A, std::vector<double> &Z) {

@@ -22,6 +22,9 @@ struct NnLevel {
 // u += N(r) for one residual r (both float, vertex order) on level `lvl` of
 // the network's hierarchy.
 void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s);
+// Check that the network was built for a hierarchy with these level sizes
+// (finest first) and size its inference buffers for the largest level.
+void nn_bind(dec_nn_s &nn, const std::vector<int> &sizes);
 dec_nn_s *nn_create(const std::vector<const HostCsr *> &ops, uint64_t seed);
 
 }  // namespace dec

@@ -73,22 +73,32 @@ __device__ __forceinline__ float ell_row(int n, int width, int r, const int *__r
 
 // ---- forward ------------------------------------------------------------
 
-// Per-sample scale s_b = sqrt(mean r²) (one block per sample, fixed order).
-__global__ void k_scale(int n, const float *__restrict__ r, float *__restrict__ s) {
+// Per-sample scale s_b = sqrt(mean r²): kScaleBlocks partial sums per sample
+// (block j sums i ≡ j·TB + t mod kScaleBlocks·TB in double, then a fixed
+// tree), summed in block order by one thread per sample.
+constexpr int kScaleBlocks = 128;
+
+__global__ void k_scale_partial(int n, const float *__restrict__ r, double *__restrict__ part) {
   __shared__ double sh[TB];
-  const float *rb = r + static_cast<size_t>(blockIdx.x) * n;
+  const float *rb = r + static_cast<size_t>(blockIdx.y) * n;
   double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += TB) acc += static_cast<double>(rb[i]) * rb[i];
+  for (int i = blockIdx.x * TB + threadIdx.x; i < n; i += kScaleBlocks * TB) acc += static_cast<double>(rb[i]) * rb[i];
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int w = TB / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    double m = sh[0] / n;
-    s[blockIdx.x] = m > 0 ? static_cast<float>(sqrt(m)) : 1.0f;
-  }
+  if (threadIdx.x == 0) part[static_cast<size_t>(blockIdx.y) * kScaleBlocks + blockIdx.x] = sh[0];
+}
+
+__global__ void k_scale_final(int n, int B, const double *__restrict__ part, float *__restrict__ s) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  double acc = 0.0;
+  for (int j = 0; j < kScaleBlocks; j++) acc += part[static_cast<size_t>(b) * kScaleBlocks + j];
+  double m = acc / n;
+  s[b] = m > 0 ? static_cast<float>(sqrt(m)) : 1.0f;
 }
 
 // x = r / s, f0 = [x, P x], h1 = relu(f0 W1 + b1)
@@ -222,6 +232,214 @@ __global__ void k_output(int n, const float *__restrict__ x, const AT *__restric
   float cc = (s[b] / D[v]) * yy;
   if (c) c[o] = cc;
   if (u_add) u_add[o] = u_add[o] + cc;
+}
+
+// ---- inference: the smoother inside V-cycles -------------------------------
+// Three kernels for one residual (B = 1): x and P x; layer 1 recomputed for
+// every neighbour from (x, P x) and layer 2 on MFMA; layer 3 on MFMA with the
+// output head, adding c to u.  h1 and h3 never reach memory.  Each wavefront
+// takes 16-row tiles; lane (li, lg) owns row r0 + li and the features
+// F(lg) = {4lg..4lg+3} ∪ {16+4lg..16+4lg+3}, which are exactly its A-operand
+// entries of v_mfma_f32_16x16x16bf16 for the K steps 0, 1 ([h]) and 2, 3
+// ([P h]).  Expressions, summation orders and BF16 rounding points are those
+// of k_input / k_spmm / k_dense_bf16 / k_output, so that u gains exactly the
+// c of the BF16 forward.
+
+__device__ __forceinline__ int feat(int lg, int q) { return q < 4 ? 4 * lg + q : 16 + 4 * lg + (q - 4); }
+
+__device__ __forceinline__ void load_wB(const float *__restrict__ W, int li, int lg, short4_t B[2][4]) {
+#pragma unroll
+  for (int jt = 0; jt < 2; jt++)
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) B[jt][ks][i] = static_cast<short>(f2bf(W[(ks * 16 + 4 * lg + i) * H + jt * 16 + li]));
+}
+
+__device__ __forceinline__ short4_t pack4(const unsigned short *h) {
+  short4_t a;
+  a[0] = static_cast<short>(h[0]);
+  a[1] = static_cast<short>(h[1]);
+  a[2] = static_cast<short>(h[2]);
+  a[3] = static_cast<short>(h[3]);
+  return a;
+}
+
+__device__ __forceinline__ void unpack4(uint2 w, unsigned short *h) {
+  h[0] = static_cast<unsigned short>(w.x & 0xffffu);
+  h[1] = static_cast<unsigned short>(w.x >> 16);
+  h[2] = static_cast<unsigned short>(w.y & 0xffffu);
+  h[3] = static_cast<unsigned short>(w.y >> 16);
+}
+
+// xp[v] = (x, P x) with x = r / s
+__global__ void k_inf_input(int n, int pw, const int *__restrict__ pcol, const float *__restrict__ pval,
+                            const float *__restrict__ r, const float *__restrict__ s, float2 *__restrict__ xp) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  float inv = 1.0f / s[0];
+  float xv = r[v] * inv;
+  float acc = 0.0f;
+  for (int k = 0; k < pw; k++) {
+    size_t q = static_cast<size_t>(k) * n + v;
+    int c = pcol[q];
+    if (c < 0) break;
+    acc = acc + pval[q] * (r[c] * inv);
+  }
+  xp[v] = make_float2(xv, acc);
+}
+
+// h2 = relu([h1, P h1] W2 + b2), h1 = relu([x, P x] W1 + b1) evaluated per
+// neighbour.  All waves of a block run the same number of tile steps.
+__global__ void __launch_bounds__(TB) k_inf_layer2(int n, int pw, const int *__restrict__ pcol,
+                                                   const float *__restrict__ pval, const float2 *__restrict__ xp,
+                                                   const float *__restrict__ th, unsigned short *__restrict__ h2) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int li = lane & 15, lg = lane >> 4;
+  float w0[8], w1[8], b1[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    int f = feat(lg, q);
+    w0[q] = th[OW1 + f];
+    w1[q] = th[OW1 + H + f];
+    b1[q] = th[OB1 + f];
+  }
+  short4_t Bw[2][4];
+  load_wB(th + OW2, li, lg, Bw);
+  float bias0 = th[OB2 + li], bias1 = th[OB2 + 16 + li];
+  int ntiles = (n + 15) / 16;
+  constexpr int WPB = TB / 64;
+  for (int base = blockIdx.x * WPB; base < ntiles; base += gridDim.x * WPB) {
+    int t = base + w;
+    if (t >= ntiles) continue;
+    int r0 = t * 16, v = r0 + li;
+    short4_t A[4];
+    if (v < n) {
+      float2 f = xp[v];
+      unsigned short hb[8];
+      float m[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        float z = f.x * w0[q] + f.y * w1[q] + b1[q];
+        hb[q] = f2bf(z > 0.0f ? z : 0.0f);
+        m[q] = 0.0f;
+      }
+      for (int k = 0; k < pw; k++) {
+        size_t qq = static_cast<size_t>(k) * n + v;
+        int c = pcol[qq];
+        if (c < 0) break;
+        float pv = pval[qq];
+        float2 fc = xp[c];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          float z = fc.x * w0[q] + fc.y * w1[q] + b1[q];
+          m[q] = m[q] + pv * bf2f(f2bf(z > 0.0f ? z : 0.0f));
+        }
+      }
+      unsigned short mb[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) mb[q] = f2bf(m[q]);
+      A[0] = pack4(hb);
+      A[1] = pack4(hb + 4);
+      A[2] = pack4(mb);
+      A[3] = pack4(mb + 4);
+    } else {
+      A[0] = A[1] = A[2] = A[3] = short4_t{0, 0, 0, 0};
+    }
+    float4_t acc0 = float4_t{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[0][ks], acc0, 0, 0, 0);
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[1][ks], acc1, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int rr = r0 + 4 * lg + i;
+      if (rr < n) {
+        float z0 = acc0[i] + bias0, z1 = acc1[i] + bias1;
+        h2[static_cast<size_t>(rr) * H + li] = f2bf(z0 > 0.0f ? z0 : 0.0f);
+        h2[static_cast<size_t>(rr) * H + 16 + li] = f2bf(z1 > 0.0f ? z1 : 0.0f);
+      }
+    }
+  }
+}
+
+// h3 = relu([h2, P h2] W3 + b3); y = alpha x + h3 wo + bo (ascending sum over
+// the 32 features, staged through LDS); u += (s / D) y.
+__global__ void __launch_bounds__(TB) k_inf_layer3(int n, int pw, const int *__restrict__ pcol,
+                                                   const float *__restrict__ pval, const unsigned short *__restrict__ h2,
+                                                   const float2 *__restrict__ xp, const float *__restrict__ th,
+                                                   const float *__restrict__ s, const float *__restrict__ D,
+                                                   float *__restrict__ u) {
+  constexpr int WPB = TB / 64;
+  __shared__ float prod[WPB][16][H + 1];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int li = lane & 15, lg = lane >> 4;
+  short4_t Bw[2][4];
+  load_wB(th + OW3, li, lg, Bw);
+  float bias0 = th[OB3 + li], bias1 = th[OB3 + 16 + li];
+  float wo0 = th[OWO + li], wo1 = th[OWO + 16 + li];
+  float alpha = th[OAL], bo = th[OBO], sc = s[0];
+  int ntiles = (n + 15) / 16;
+  for (int base = blockIdx.x * WPB; base < ntiles; base += gridDim.x * WPB) {
+    int t = base + w;
+    bool active = t < ntiles;
+    int r0 = t * 16, v = r0 + li;
+    if (active) {
+      short4_t A[4];
+      if (v < n) {
+        const unsigned short *hv = h2 + static_cast<size_t>(v) * H;
+        unsigned short hb[8];
+        unpack4(*reinterpret_cast<const uint2 *>(hv + 4 * lg), hb);
+        unpack4(*reinterpret_cast<const uint2 *>(hv + 16 + 4 * lg), hb + 4);
+        float m[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) m[q] = 0.0f;
+        for (int k = 0; k < pw; k++) {
+          size_t qq = static_cast<size_t>(k) * n + v;
+          int c = pcol[qq];
+          if (c < 0) break;
+          float pv = pval[qq];
+          const unsigned short *hc = h2 + static_cast<size_t>(c) * H;
+          unsigned short nb[8];
+          unpack4(*reinterpret_cast<const uint2 *>(hc + 4 * lg), nb);
+          unpack4(*reinterpret_cast<const uint2 *>(hc + 16 + 4 * lg), nb + 4);
+#pragma unroll
+          for (int q = 0; q < 8; q++) m[q] = m[q] + pv * bf2f(nb[q]);
+        }
+        unsigned short mb[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) mb[q] = f2bf(m[q]);
+        A[0] = pack4(hb);
+        A[1] = pack4(hb + 4);
+        A[2] = pack4(mb);
+        A[3] = pack4(mb + 4);
+      } else {
+        A[0] = A[1] = A[2] = A[3] = short4_t{0, 0, 0, 0};
+      }
+      float4_t acc0 = float4_t{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++) acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[0][ks], acc0, 0, 0, 0);
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++) acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[1][ks], acc1, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        float z0 = acc0[i] + bias0, z1 = acc1[i] + bias1;
+        prod[w][4 * lg + i][li] = bf2f(f2bf(z0 > 0.0f ? z0 : 0.0f)) * wo0;
+        prod[w][4 * lg + i][16 + li] = bf2f(f2bf(z1 > 0.0f ? z1 : 0.0f)) * wo1;
+      }
+    }
+    __syncthreads();
+    if (active && lane < 16) {
+      int row = r0 + lane;
+      if (row < n) {
+        float hw = 0.0f;
+        for (int j = 0; j < H; j++) hw = hw + prod[w][lane][j];
+        float yy = alpha * xp[row].x + hw + bo;
+        u[row] = u[row] + (sc / D[row]) * yy;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ---- loss and backward ------------------------------------------------------
@@ -576,7 +794,9 @@ struct dec_nn_s {
   int adam_t = 0;
   // work buffers (grown on demand)
   DBuf<float> x, px, y, c, s, drho, dg, part, r;
-  DBuf<double> dpart, dloss;
+  DBuf<double> dpart, dloss, spart;
+  DBuf<float2> xp;                                 // inference: (x, P x)
+  DBuf<unsigned short> ih2;                        // inference: layer-2 output
   DBuf<unsigned char> h1, hm1, h2, hm2, h3, dz;  // activation storage (bf16 or f32)
   DBuf<int32_t> idx;
   size_t cap_rows = 0;
@@ -623,16 +843,22 @@ void dense(int R, const AT *h, const AT *m, const float *W, const float *b, AT *
   }
 }
 
+void scale(dec_nn_s &nn, int n, int B, const float *r, hipStream_t st) {
+  if (nn.s.n < static_cast<size_t>(B)) nn.s.alloc(B);
+  if (nn.spart.n < static_cast<size_t>(B) * kScaleBlocks) nn.spart.alloc(static_cast<size_t>(B) * kScaleBlocks);
+  hipLaunchKernelGGL(k_scale_partial, dim3(kScaleBlocks, B), dim3(TB), 0, st, n, r, nn.spart.p);
+  hipLaunchKernelGGL(k_scale_final, dim3(dec::nblocks(B, 64)), dim3(64), 0, st, n, B, nn.spart.p, nn.s.p);
+}
+
 template <class AT>
 void forward_t(dec_nn_s &nn, const dec::NnLevel &L, int B, const float *r, float *c_out, float *u_add,
                hipStream_t st) {
   int n = L.n;
   size_t rows = static_cast<size_t>(n) * B;
   ensure(nn, rows, sizeof(AT) == 4);
-  if (nn.s.n < static_cast<size_t>(B)) nn.s.alloc(B);
   Acts<AT> A = acts<AT>(nn);
   const float *th = nn.th.p;
-  hipLaunchKernelGGL(k_scale, dim3(B), dim3(TB), 0, st, n, r, nn.s.p);
+  scale(nn, n, B, r, st);
   dim3 gv(dec::nblocks(n, TB), B), gf(dec::nblocks(static_cast<long long>(n) * H, TB), B);
   hipLaunchKernelGGL(k_input<AT>, gv, dim3(TB), 0, st, n, L.p.width, L.p.col.p, L.p.val.p, r, nn.s.p, th, nn.x.p,
                      nn.px.p, A.h1);
@@ -709,8 +935,38 @@ const dec::NnLevel &level_of(dec_nn_s &nn, int lvl) {
 
 namespace dec {
 
+void reserve_inference(dec_nn_s &nn, int n) {
+  if (nn.xp.n < static_cast<size_t>(n)) nn.xp.alloc(n);
+  if (nn.ih2.n < static_cast<size_t>(n) * H) nn.ih2.alloc(static_cast<size_t>(n) * H);
+  if (nn.s.n < 1) nn.s.alloc(1);
+  if (nn.spart.n < static_cast<size_t>(kScaleBlocks)) nn.spart.alloc(kScaleBlocks);
+}
+
+void nn_bind(dec_nn_s &nn, const std::vector<int> &sizes) {
+  if (sizes.size() != nn.levels.size())
+    fail(DEC_ERR_ARG, "learned smoother: the network was created for another hierarchy (level count)");
+  int nmax = 0;
+  for (size_t l = 0; l < sizes.size(); l++) {
+    if (sizes[l] != nn.levels[l]->n)
+      fail(DEC_ERR_ARG, "learned smoother: the network was created for another hierarchy (level sizes)");
+    nmax = std::max(nmax, sizes[l]);
+  }
+  reserve_inference(nn, nmax);
+}
+
 void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s) {
-  forward_t<unsigned short>(nn, level_of(nn, lvl), 1, r, nullptr, u, s);
+  const NnLevel &L = level_of(nn, lvl);
+  int n = L.n;
+  reserve_inference(nn, n);
+  scale(nn, n, 1, r, s);
+  hipLaunchKernelGGL(k_inf_input, dim3(dec::nblocks(n, TB)), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, r,
+                     nn.s.p, nn.xp.p);
+  int tiles = (n + 15) / 16;
+  int blocks = std::max(1, std::min((tiles + TB / 64 - 1) / (TB / 64), 2048));
+  hipLaunchKernelGGL(k_inf_layer2, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.xp.p,
+                     nn.th.p, nn.ih2.p);
+  hipLaunchKernelGGL(k_inf_layer3, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ih2.p,
+                     nn.xp.p, nn.th.p, nn.s.p, L.D.p, u);
 }
 
 dec_nn_s *nn_create(const std::vector<const HostCsr *> &ops, uint64_t seed) {
@@ -772,6 +1028,14 @@ int dec_nn_forward(dec_nn nn, int32_t level, int32_t nsamples, const float *r, f
   if (precise) forward_t<float>(*nn, L, nsamples, r, c, nullptr, s);
   else forward_t<unsigned short>(*nn, L, nsamples, r, c, nullptr, s);
   dec::check_launch("dec_nn_forward");
+  DEC_CATCH
+}
+
+int dec_nn_smooth(dec_nn nn, int32_t level, const float *r, float *u, void *stream) {
+  DEC_TRY
+  if (!nn || !r || !u) fail(DEC_ERR_ARG, "dec_nn_smooth: bad argument");
+  dec::nn_smooth(*nn, level, r, u, static_cast<hipStream_t>(stream));
+  dec::check_launch("dec_nn_smooth");
   DEC_CATCH
 }
 

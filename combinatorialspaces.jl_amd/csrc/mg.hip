@@ -94,11 +94,14 @@ __global__ void k_copy(int n, const T *__restrict__ x, T *__restrict__ y) {
 
 // b_c = R r_f; rows are coarse vertices in natural order (CSC order of R:
 // the identity column first, then the midpoint columns ascending).
+// uc (when given) is the coarse iterate, zeroed here for the coarse cycle.
 template <int W, class T>
 __global__ void k_restrict(int nc, int width, const int *__restrict__ col, const T *__restrict__ val,
-                           const T *__restrict__ rf, T *__restrict__ bc) {
+                           const T *__restrict__ rf, T *__restrict__ bc, T *__restrict__ uc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nc) bc[i] = dec_ell::row<W, T, T>(nc, width, i, col, val, rf);
+  if (i >= nc) return;
+  bc[i] = dec_ell::row<W, T, T>(nc, width, i, col, val, rf);
+  if (uc) uc[i] = static_cast<T>(0);
 }
 
 // u_f += P z_c with P = transpose(S): coarse vertices copy, midpoints average
@@ -118,16 +121,41 @@ __global__ void k_prolong_add(int nf, int nc, const int *__restrict__ pa, const 
   uf[j] = uf[j] + pz;
 }
 
-// u += Z r with Z stored column-major (ZT[j * n + i] = Z[i, j]): one thread
-// per row, sequential sum over j (coalesced across the rows of a wavefront).
+// u += Z r with Z stored column-major (ZT[j * n + i] = Z[i, j]), in two
+// launches so that the whole chip streams Z: the columns are cut into chunks of
+// kCoarseChunk, one workgroup of 64 rows per (row tile, chunk) writes
+// part[c * n + i] = Σ_{j in chunk c, ascending} Z[i, j] r[j], and the second
+// pass adds Σ_c part[c * n + i] (c ascending) to u.  The fused coarse tail
+// uses the same two sums in the same order.
+constexpr int kCoarseChunk = 64;
+
 template <class T>
-__global__ void k_gemv_add(int n, const T *__restrict__ ZT, const T *__restrict__ r, T *__restrict__ u) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64)
+    k_coarse_partial(int n, const T *__restrict__ ZT, const T *__restrict__ r, T *__restrict__ part) {
+  int i = blockIdx.x * 64 + threadIdx.x;
+  int j0 = blockIdx.y * kCoarseChunk;
+  int j1 = min(n, j0 + kCoarseChunk);
   if (i >= n) return;
   T acc = static_cast<T>(0);
-  for (int j = 0; j < n; j++) acc = acc + ZT[static_cast<size_t>(j) * n + i] * r[j];
-  u[i] = u[i] + acc;
+#pragma unroll 16
+  for (int j = j0; j < j1; j++) acc = acc + ZT[static_cast<size_t>(j) * n + i] * r[j];
+  part[static_cast<size_t>(blockIdx.y) * n + i] = acc;
 }
+
+template <class T>
+__device__ __forceinline__ T coarse_sum(int n, int nchunks, int i, const T *__restrict__ part) {
+  T acc = static_cast<T>(0);
+  for (int c = 0; c < nchunks; c++) acc = acc + part[static_cast<size_t>(c) * n + i];
+  return acc;
+}
+
+template <class T>
+__global__ void k_coarse_finish(int n, int nchunks, const T *__restrict__ part, T *__restrict__ u) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[i] = u[i] + coarse_sum(n, nchunks, i, part);
+}
+
+inline int coarse_chunks(int n) { return (n + kCoarseChunk - 1) / kCoarseChunk; }
 
 template <class T>
 __device__ double block_sum(double v) {
@@ -247,13 +275,16 @@ __global__ void k_add_f2d(int n, const float *__restrict__ c, double *__restrict
 
 
 // ------------------------------------------------------------- fused coarse tail
-// The levels whose size is at most kFuseRows run as one workgroup of 1024
+// The levels whose size is at most fuse_rows (DEC_MG_FUSE_ROWS, default
+// kFuseRowsDefault, at most kFuseRowsMax) run as one workgroup of 1024
 // threads: every phase of the V-cycle below that level (colour sweeps,
-// residual, restriction, the dense coarse solve, prolongation) is a strided
-// loop over the level's rows followed by a workgroup barrier, in place of one
-// kernel launch per phase.  Per-row arithmetic is the same as the multi-kernel
-// path (same expressions, same order).
-constexpr int kFuseRows = 16384;
+// residual, restriction, prolongation) is a strided loop over the level's
+// rows followed by a workgroup barrier, in place of one kernel launch per
+// phase.  Per-row arithmetic is the same as the multi-kernel path (same
+// expressions, same order).  One workgroup streams at a small fraction of
+// the chip's bandwidth, so this pays only for levels of a few thousand rows.
+constexpr int kFuseRowsDefault = 3000;
+constexpr int kFuseRowsMax = 16384;
 constexpr int kFuseThreads = 1024;
 constexpr int kMaxColors = 16;
 
@@ -262,7 +293,7 @@ struct LevelDesc {
   int n, width, ncolors, nc, rwidth;
   int cstart[kMaxColors + 1];
   const int *col, *rowid, *rcol, *pa, *pb;
-  const T *val, *rval, *ZT;
+  const T *val, *rval;
   T *u, *b, *r;
 };
 
@@ -303,50 +334,55 @@ __device__ void fused_smooth(const LevelDesc<T> &L, const T *b, T *u, int steps,
   }
 }
 
+// phase 0: the whole V-cycle below l0, the coarsest level smoothed.
+// phase 1: the down sweep; at the coarsest level only its residual is formed
+//          (into Ls[nl-1].r) for the dense direct solve, which runs as
+//          k_coarse_partial on the whole chip.
+// phase 2: the coarsest correction u += Σ_c part[c] (k_coarse_finish's sum),
+//          then the up sweep.
 template <class T>
 __global__ void __launch_bounds__(kFuseThreads)
     k_fused_vcycle(const LevelDesc<T> *__restrict__ Ls, int l0, int nl, T *u0, const T *b0, int steps, int alg,
-                   T omega, int direct) {
-  int last = l0;
-  for (int l = l0; l < nl; l++) {
-    const LevelDesc<T> &L = Ls[l];
-    T *u = l == l0 ? u0 : L.u;
-    const T *b = l == l0 ? b0 : L.b;
-    last = l;
-    if (l > l0) {
-      for (int i = threadIdx.x; i < L.n; i += blockDim.x) u[i] = static_cast<T>(0);
-      __syncthreads();
-    }
-    if (l == nl - 1 && direct) {
+                   T omega, int phase, const T *__restrict__ part) {
+  if (phase != 2) {
+    for (int l = l0; l < nl; l++) {
+      const LevelDesc<T> &L = Ls[l];
+      T *u = l == l0 ? u0 : L.u;
+      const T *b = l == l0 ? b0 : L.b;
+      if (l > l0) {
+        for (int i = threadIdx.x; i < L.n; i += blockDim.x) u[i] = static_cast<T>(0);
+        __syncthreads();
+      }
+      if (l == nl - 1 && phase == 1) {
+        for (int p = threadIdx.x; p < L.n; p += blockDim.x) {
+          int v = L.rowid[p];
+          L.r[v] = b[v] - fused_row(L, p, u);
+        }
+        return;
+      }
+      fused_smooth(L, b, u, steps, alg, omega, false);
+      if (l == nl - 1) break;
       for (int p = threadIdx.x; p < L.n; p += blockDim.x) {
         int v = L.rowid[p];
         L.r[v] = b[v] - fused_row(L, p, u);
       }
       __syncthreads();
-      for (int i = threadIdx.x; i < L.n; i += blockDim.x) {
-        T acc = static_cast<T>(0);
-        for (int j = 0; j < L.n; j++) acc = acc + L.ZT[static_cast<size_t>(j) * L.n + i] * L.r[j];
-        u[i] = u[i] + acc;
+      const LevelDesc<T> &C = Ls[l + 1];
+      for (int i = threadIdx.x; i < L.nc; i += blockDim.x) {
+        T acc = L.rwidth == 7 ? dec_ell::row<7, T, T>(L.nc, L.rwidth, i, L.rcol, L.rval, L.r)
+                              : dec_ell::row<0, T, T>(L.nc, L.rwidth, i, L.rcol, L.rval, L.r);
+        C.b[i] = acc;
       }
       __syncthreads();
-      break;
     }
-    fused_smooth(L, b, u, steps, alg, omega, false);
-    if (l == nl - 1) break;
-    for (int p = threadIdx.x; p < L.n; p += blockDim.x) {
-      int v = L.rowid[p];
-      L.r[v] = b[v] - fused_row(L, p, u);
-    }
-    __syncthreads();
-    const LevelDesc<T> &C = Ls[l + 1];
-    for (int i = threadIdx.x; i < L.nc; i += blockDim.x) {
-      T acc = L.rwidth == 7 ? dec_ell::row<7, T, T>(L.nc, L.rwidth, i, L.rcol, L.rval, L.r)
-                            : dec_ell::row<0, T, T>(L.nc, L.rwidth, i, L.rcol, L.rval, L.r);
-      C.b[i] = acc;
-    }
+  } else {
+    const LevelDesc<T> &C = Ls[nl - 1];
+    T *uc = nl - 1 == l0 ? u0 : C.u;
+    int nch = (C.n + kCoarseChunk - 1) / kCoarseChunk;
+    for (int i = threadIdx.x; i < C.n; i += blockDim.x) uc[i] = uc[i] + coarse_sum(C.n, nch, i, part);
     __syncthreads();
   }
-  for (int l = last - 1; l >= l0; l--) {
+  for (int l = nl - 2; l >= l0; l--) {
     const LevelDesc<T> &L = Ls[l];
     const LevelDesc<T> &C = Ls[l + 1];
     T *u = l == l0 ? u0 : L.u;
@@ -373,6 +409,7 @@ struct LevelVals {
   DBuf<T> val;   // ELL values of A (colour-sorted rows)
   DBuf<T> rval;  // ELL values of R (this level -> next coarser)
   DBuf<T> Z;     // dense coarse operator, column-major (coarsest level, direct solve)
+  DBuf<T> zpart; // chunk partial products of the direct solve (coarsest level)
   DBuf<T> u, b, r, t;  // work vectors
 };
 
@@ -388,6 +425,7 @@ struct Level {
   LevelVals<double> d;
   LevelVals<float> f;
   dec::HostCsr A, R, P;  // host copies for export
+  std::vector<int32_t> color;  // Gauss-Seidel colour of each vertex
   std::vector<double> Zh;
 };
 
@@ -405,6 +443,7 @@ struct dec_mg_s {
   DBuf<LevelDesc<double>> desc64;
   DBuf<LevelDesc<float>> desc32;
   bool use_fused = true;
+  int fuse_rows = kFuseRowsDefault;
   // graph of one solve cycle (per variant) for fixed u, b
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   hipGraph_t graph[2] = {nullptr, nullptr};
@@ -437,12 +476,10 @@ LevelVals<double> &vals<double>(Level &L) { return L.d; }
 template <>
 LevelVals<float> &vals<float>(Level &L) { return L.f; }
 
-// Colour-sorted ELL of a host CSR (columns ascending per row).
-void build_level_matrix(Level &L, const dec::HostCsr &A) {
+// Symmetric adjacency (CSR, no diagonal) of the sparsity pattern of A.
+void adjacency(const dec::HostCsr &A, std::vector<int32_t> &deg, std::vector<int32_t> &adj) {
   int n = A.m;
-  L.n = n;
-  // symmetric adjacency for colouring
-  std::vector<int32_t> deg(n + 1, 0);
+  deg.assign(n + 1, 0);
   for (int r = 0; r < n; r++)
     for (int k = A.ptr[r]; k < A.ptr[r + 1]; k++)
       if (A.col[k] != r) {
@@ -450,7 +487,7 @@ void build_level_matrix(Level &L, const dec::HostCsr &A) {
         deg[A.col[k] + 1]++;
       }
   for (int r = 0; r < n; r++) deg[r + 1] += deg[r];
-  std::vector<int32_t> adj(deg[n]);
+  adj.assign(deg[n], 0);
   std::vector<int32_t> w(deg.begin(), deg.end() - 1);
   for (int r = 0; r < n; r++)
     for (int k = A.ptr[r]; k < A.ptr[r + 1]; k++)
@@ -458,8 +495,22 @@ void build_level_matrix(Level &L, const dec::HostCsr &A) {
         adj[w[r]++] = A.col[k];
         adj[w[A.col[k]]++] = r;
       }
-  std::vector<int32_t> color;
-  L.ncolors = dec::greedy_colors(n, deg, adj, color);
+}
+
+// Colour-sorted ELL of a host CSR (columns ascending per row).  `color` is the
+// proposed Gauss-Seidel colouring (empty: none); a proposal that is not a
+// proper colouring of A's pattern is replaced by greedy colours.  On return
+// `color` holds the colouring used.
+void build_level_matrix(Level &L, const dec::HostCsr &A, std::vector<int32_t> &color) {
+  int n = A.m;
+  L.n = n;
+  std::vector<int32_t> deg, adj;
+  adjacency(A, deg, adj);
+  int nc = 0;
+  for (int c : color) nc = std::max(nc, c + 1);
+  if (color.empty() || !dec::proper_coloring(n, deg, adj, color, nc)) nc = dec::greedy_colors(n, deg, adj, color);
+  L.ncolors = nc;
+  L.color = color;
   std::vector<int32_t> order(n);
   L.cstart.assign(L.ncolors + 1, 0);
   for (int v = 0; v < n; v++) L.cstart[color[v] + 1]++;
@@ -575,6 +626,37 @@ dec_mesh_s *copy_mesh(const dec_mesh_s &m) {
   return c;
 }
 
+// Gauss-Seidel colours of every level (finest first), four per level where
+// possible.  The base level is 4-coloured by search.  A proper colouring c of
+// a level with values in {0,1,2,3} colours the next finer level properly:
+// coarse vertices take 0 and the midpoint of coarse edge (a, b) takes
+// c(a) ^ c(b), which is nonzero because a and b are adjacent; two midpoints
+// are adjacent only when their edges (a, b), (b, z) bound a common coarse
+// triangle, and c(a) ^ c(b) = c(b) ^ c(z) would force c(a) = c(z) for the
+// adjacent a, z; coarse vertices are never adjacent on the finer level.
+// A level without a proposal (or whose proposal fails the check in
+// build_level_matrix) is coloured greedily.
+std::vector<std::vector<int32_t>> level_colors(const dec_mg_s &mg, const std::vector<dec::HostCsr> &As) {
+  int nl = static_cast<int>(As.size());
+  std::vector<std::vector<int32_t>> colors(nl);
+  const dec::HostCsr &Ab = As[nl - 1];
+  if (Ab.m <= 65536) {
+    std::vector<int32_t> deg, adj;
+    adjacency(Ab, deg, adj);
+    if (!dec::four_colors(Ab.m, deg, adj, colors[nl - 1], 64L * Ab.m + 4096)) colors[nl - 1].clear();
+  }
+  for (int l = nl - 2; l >= 0; l--) {
+    const std::vector<int32_t> &cc = colors[l + 1];
+    if (cc.empty()) break;
+    const dec::Topo &coarse = mg.meshes[l + 1]->h;
+    int nc = coarse.nv;
+    std::vector<int32_t> &cf = colors[l];
+    cf.assign(static_cast<size_t>(nc) + coarse.ne, 0);
+    for (int e = 0; e < coarse.ne; e++) cf[nc + e] = cc[coarse.v0[e]] ^ cc[coarse.v1[e]];
+  }
+  return colors;
+}
+
 dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge, int center, int steps) {
   if (levels < 0 || levels > 11) fail(DEC_ERR_ARG, "dec_mg_create: levels must be in 0..11");
   if (steps < 0) fail(DEC_ERR_ARG, "dec_mg_create: steps must be >= 0");
@@ -582,6 +664,7 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
   mg->steps = steps;
   // debugging switches: run every phase as its own launch / launch cycles eagerly
   mg->use_fused = std::getenv("DEC_MG_NO_FUSE") == nullptr;
+  if (const char *e = std::getenv("DEC_MG_FUSE_ROWS")) mg->fuse_rows = std::min(std::max(std::atoi(e), 0), kFuseRowsMax);
   mg->use_graph = std::getenv("DEC_MG_NO_GRAPH") == nullptr;
   // coarse -> fine walk (Multigrid.jl:752-797), stored finest first
   std::vector<std::unique_ptr<dec_mesh_s>> walk;
@@ -589,22 +672,25 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
   for (int i = 0; i < levels; i++) walk.emplace_back(dec::subdivide(*walk.back(), project));
   for (int i = levels; i >= 0; i--) mg->meshes.push_back(std::move(walk[i]));
   int nl = levels + 1;
+  std::vector<dec::HostCsr> As(nl);
   for (int l = 0; l < nl; l++) {
     dec_mesh_s &m = *mg->meshes[l];
     m.owned_by_mg = true;
     dec::dualize(m, center);
-    auto L = std::make_unique<Level>();
     std::vector<double> h0 = m.h0.download(), h1 = m.h1.download(), ih0(m.h.nv);
     for (int v = 0; v < m.h.nv; v++) ih0[v] = 1 / h0[v];
-    dec::HostCsr A;
     if (hodge == DEC_GEOMETRIC_HODGE) {
       dec::HostCsr H;
       dec::geo_hodge1(m, H);
-      dec::assemble_laplacian0(m.h, ih0, h1, &H, A);
+      dec::assemble_laplacian0(m.h, ih0, h1, &H, As[l]);
     } else {
-      dec::assemble_laplacian0(m.h, ih0, h1, nullptr, A);
+      dec::assemble_laplacian0(m.h, ih0, h1, nullptr, As[l]);
     }
-    build_level_matrix(*L, A);
+  }
+  std::vector<std::vector<int32_t>> colors = level_colors(*mg, As);
+  for (int l = 0; l < nl; l++) {
+    auto L = std::make_unique<Level>();
+    build_level_matrix(*L, As[l], colors[l]);
     alloc_work(*L);
     mg->lv.push_back(std::move(L));
   }
@@ -623,12 +709,14 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
     C.d.Z.upload(zt);
     std::vector<float> zf(zt.begin(), zt.end());
     C.f.Z.upload(zf);
+    C.d.zpart.alloc(static_cast<size_t>(coarse_chunks(n)) * n);
+    C.f.zpart.alloc(static_cast<size_t>(coarse_chunks(n)) * n);
   }
   // descriptors of the coarse tail for the fused kernel
   for (int l = 0; l < nl; l++)
-    if (mg->lv[l]->n <= kFuseRows && mg->lv[l]->ncolors <= kMaxColors) {
+    if (mg->lv[l]->n <= mg->fuse_rows && mg->lv[l]->ncolors <= kMaxColors) {
       bool ok = true;
-      for (int q = l; q < nl; q++) ok = ok && mg->lv[q]->n <= kFuseRows && mg->lv[q]->ncolors <= kMaxColors;
+      for (int q = l; q < nl; q++) ok = ok && mg->lv[q]->n <= mg->fuse_rows && mg->lv[q]->ncolors <= kMaxColors;
       if (ok) {
         mg->fuse_from = l;
         break;
@@ -653,7 +741,6 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
         D.pb = L.pb.p;
         D.val = V.val.p;
         D.rval = V.rval.p;
-        D.ZT = V.Z.p;
         D.u = V.u.p;
         D.b = V.b.p;
         D.r = V.r.p;
@@ -715,13 +802,15 @@ void cg_smooth(dec_mg_s &mg, Level &L, const double *b, double *u, int steps, hi
   hipLaunchKernelGGL(k_axpy_final, G(n), dim3(kBlock), 0, s, n, x, u);
 }
 
+inline bool learned_on(const dec_mg_s &mg) { return mg.nn && mg.learned && mg.learned_active; }
+
 template <class T>
 void smooth(dec_mg_s &mg, int lvl, Level &L, const T *b, T *u, bool post, hipStream_t s) {
   int steps = mg.steps;
   if (steps == 0) return;
   LevelVals<T> &V = vals<T>(L);
   if constexpr (sizeof(T) == 4) {
-    if (mg.nn && mg.learned && mg.learned_active && L.n >= mg.learned_min_rows) {
+    if (learned_on(mg) && L.n >= mg.learned_min_rows) {
       for (int a = 0; a < mg.learned_apps; a++) {
         residual<float>(L, b, u, V.r.p, s);
         dec::nn_smooth(*mg.nn, lvl, V.r.p, u, s);
@@ -753,12 +842,26 @@ void smooth(dec_mg_s &mg, int lvl, Level &L, const T *b, T *u, bool post, hipStr
   }
 }
 
+// u += Z r (r = the level's residual vector) by the chunked two-pass product.
+template <class T>
+void coarse_partials(Level &L, hipStream_t s) {
+  LevelVals<T> &V = vals<T>(L);
+  dim3 grid(static_cast<unsigned>((L.n + 63) / 64), static_cast<unsigned>(coarse_chunks(L.n)));
+  hipLaunchKernelGGL(k_coarse_partial<T>, grid, dim3(64), 0, s, L.n, V.Z.p, V.r.p, V.zpart.p);
+}
+
+template <class T>
+void coarse_gemv(Level &L, T *u, hipStream_t s) {
+  coarse_partials<T>(L, s);
+  hipLaunchKernelGGL(k_coarse_finish<T>, G(L.n), dim3(kBlock), 0, s, L.n, coarse_chunks(L.n), vals<T>(L).zpart.p, u);
+}
+
 template <class T>
 void coarse_solve(dec_mg_s &mg, Level &L, const T *b, T *u, hipStream_t s) {
   LevelVals<T> &V = vals<T>(L);
   if (!V.Z.p) fail(DEC_ERR_UNSUPPORTED, "coarsest level too large for the dense direct solve");
   residual<T>(L, b, u, V.r.p, s);
-  hipLaunchKernelGGL(k_gemv_add<T>, G(L.n), dim3(kBlock), 0, s, L.n, V.Z.p, V.r.p, u);
+  coarse_gemv<T>(L, u, s);
   (void)mg;
 }
 
@@ -767,13 +870,25 @@ template <class T>
 void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, hipStream_t s) {
   int nl = static_cast<int>(mg.lv.size());
   Level &L = *mg.lv[lvl];
+  // the fused tail has no learned smoother: a level that uses it runs unfused
+  bool learned_here = sizeof(T) == 4 && learned_on(mg) && L.n >= mg.learned_min_rows;
   if (mg.use_fused && mu == 1 && mg.alg != DEC_SMOOTH_CG && mg.fuse_from >= 0 && lvl >= mg.fuse_from &&
-      (mg.coarse_direct == 0 || vals<T>(*mg.lv.back()).Z.p)) {
+      !learned_here && (mg.coarse_direct == 0 || vals<T>(*mg.lv.back()).Z.p)) {
     const LevelDesc<T> *D;
     if constexpr (sizeof(T) == 8) D = mg.desc64.p;
     else D = mg.desc32.p;
-    hipLaunchKernelGGL(k_fused_vcycle<T>, dim3(1), dim3(kFuseThreads), 0, s, D, lvl, nl, u, b, mg.steps, mg.alg,
-                       static_cast<T>(mg.omega), mg.coarse_direct);
+    T om = static_cast<T>(mg.omega);
+    if (!mg.coarse_direct) {
+      hipLaunchKernelGGL(k_fused_vcycle<T>, dim3(1), dim3(kFuseThreads), 0, s, D, lvl, nl, u, b, mg.steps, mg.alg, om,
+                         0, static_cast<const T *>(nullptr));
+      return;
+    }
+    Level &C = *mg.lv.back();
+    hipLaunchKernelGGL(k_fused_vcycle<T>, dim3(1), dim3(kFuseThreads), 0, s, D, lvl, nl, u, b, mg.steps, mg.alg, om,
+                       1, static_cast<const T *>(nullptr));
+    coarse_partials<T>(C, s);
+    hipLaunchKernelGGL(k_fused_vcycle<T>, dim3(1), dim3(kFuseThreads), 0, s, D, lvl, nl, u, b, mg.steps, mg.alg, om,
+                       2, static_cast<const T *>(vals<T>(C).zpart.p));
     return;
   }
   if (lvl == nl - 1 && mg.coarse_direct) {
@@ -787,8 +902,7 @@ void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, hipStream_t s) {
   LevelVals<T> &CV = vals<T>(C);
   residual<T>(L, b, u, V.r.p, s);
   DEC_ELL_WIDTH_SWITCH(L.rwidth, hipLaunchKernelGGL((k_restrict<W, T>), G(L.nc), dim3(kBlock), 0, s, L.nc, L.rwidth,
-                                                    L.rcol.p, V.rval.p, V.r.p, CV.b.p));
-  hipLaunchKernelGGL(k_zero<T>, G(C.n), dim3(kBlock), 0, s, C.n, CV.u.p);
+                                                    L.rcol.p, V.rval.p, V.r.p, CV.b.p, CV.u.p));
   mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu, s);
   if (mu > 1) mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu - 1, s);
   hipLaunchKernelGGL(k_prolong_add<T>, G(L.n), dim3(kBlock), 0, s, L.n, L.nc, L.pa.p, L.pb.p, CV.u.p, u);
@@ -803,7 +917,8 @@ void fmg_rec(dec_mg_s &mg, int lvl, const double *b, double *z, int mu, int cycl
   if (lvl < nl - 1) {
     Level &C = *mg.lv[lvl + 1];
     DEC_ELL_WIDTH_SWITCH(L.rwidth, hipLaunchKernelGGL((k_restrict<W, double>), G(L.nc), dim3(kBlock), 0, s, L.nc,
-                                                      L.rwidth, L.rcol.p, L.d.rval.p, b, C.d.b.p));
+                                                      L.rwidth, L.rcol.p, L.d.rval.p, b, C.d.b.p,
+                                                      static_cast<double *>(nullptr)));
     fmg_rec(mg, lvl + 1, C.d.b.p, C.d.u.p, mu, cycles, s);
     hipLaunchKernelGGL(k_prolong_add<double>, G(L.n), dim3(kBlock), 0, s, L.n, L.nc, L.pa.p, L.pb.p, C.d.u.p, z);
   }
@@ -855,7 +970,15 @@ void run_cycle(dec_mg_s &mg, int variant, double *u, const double *b, hipStream_
     mg.graph[i] = nullptr;
     if (!mg.cap) gcheck(hipStreamCreateWithFlags(&mg.cap, hipStreamNonBlocking), "hipStreamCreate");
     gcheck(hipStreamBeginCapture(mg.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    cycle_body(mg, variant, u, b, mg.cap);
+    try {
+      cycle_body(mg, variant, u, b, mg.cap);
+    } catch (...) {
+      // close the capture, or every later allocation on this thread fails
+      hipGraph_t partial = nullptr;
+      (void)hipStreamEndCapture(mg.cap, &partial);
+      if (partial) (void)hipGraphDestroy(partial);
+      throw;
+    }
     gcheck(hipStreamEndCapture(mg.cap, &mg.graph[i]), "hipStreamEndCapture");
     gcheck(hipGraphInstantiate(&mg.gexec[i], mg.graph[i], nullptr, nullptr, 0), "hipGraphInstantiate");
     mg.gu[i] = u;
@@ -878,7 +1001,9 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u, con
   }
   double rr = 0.0, prev = 0.0;
   int c = 0;
-  mg.learned_active = mg.learned && mg.nn && variant == DEC_SOLVE_MIXED_IR;
+  // the network smooths the float32 cycles on the levels of at least
+  // learned_min_rows rows; with no such level the solve is a plain mixed one
+  mg.learned_active = mg.learned && mg.nn && variant == DEC_SOLVE_MIXED_IR && n >= mg.learned_min_rows;
   mg.fallbacks = 0;
   mg.learned_cycles = 0;
   for (;;) {
@@ -950,6 +1075,16 @@ int dec_mg_export_level(dec_mg mg, int32_t level, int32_t which, int64_t *nnz, i
   if (rowptr) std::copy(A.ptr.begin(), A.ptr.end(), rowptr);
   if (col) std::copy(A.col.begin(), A.col.end(), col);
   if (val) std::copy(A.val.begin(), A.val.end(), val);
+  DEC_CATCH
+}
+
+int dec_mg_export_colors(dec_mg mg, int32_t level, int32_t *color, int32_t *ncolors) {
+  DEC_TRY
+  if (!mg) fail(DEC_ERR_ARG, "dec_mg_export_colors: NULL handle");
+  if (level < 0 || level >= static_cast<int>(mg->lv.size())) fail(DEC_ERR_ARG, "dec_mg_export_colors: bad level");
+  const Level &L = *mg->lv[level];
+  if (color) std::copy(L.color.begin(), L.color.end(), color);
+  if (ncolors) *ncolors = L.ncolors;
   DEC_CATCH
 }
 
@@ -1027,6 +1162,13 @@ int dec_mg_set_learned(dec_mg mg, dec_nn nn, int32_t enable, int32_t min_rows, i
   if (!mg) fail(DEC_ERR_ARG, "dec_mg_set_learned: NULL handle");
   if (enable && !nn) fail(DEC_ERR_ARG, "dec_mg_set_learned: no network");
   if (applications < 1) fail(DEC_ERR_ARG, "dec_mg_set_learned: applications must be >= 1");
+  if (nn) {
+    std::vector<int> sizes;
+    for (auto &L : mg->lv) sizes.push_back(L->n);
+    // the network must carry this hierarchy's operators; its work buffers
+    // are sized here, outside any graph capture of the cycles
+    dec::nn_bind(*nn, sizes);
+  }
   mg->nn = nn;
   mg->learned = enable != 0;
   mg->learned_min_rows = min_rows;

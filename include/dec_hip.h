@@ -147,6 +147,13 @@ int dec_mg_finest_mesh(dec_mg mg, dec_mesh *m);
 /* CSR exports of A_l, R_l (level l -> l+1) and P_l = transpose(S_l). */
 int dec_mg_export_level(dec_mg mg, int32_t level, int32_t which, int64_t *nnz,
                         int32_t *rowptr, int32_t *col, double *val);
+/* The Gauss-Seidel colour of each vertex of level l (n_l entries) and the
+ * number of colours.  No counterpart in the reference, whose smoother is
+ * Krylov cg: the multicolour Gauss-Seidel smoother is this library's, and its
+ * colouring is exported so that callers and tests can check it and reproduce
+ * the sweep order.  Four colours where the base mesh admits them (found by
+ * search), else greedy colours. */
+int dec_mg_export_colors(dec_mg mg, int32_t level, int32_t *color, int32_t *ncolors);
 /* Smoother per level: alg (cg = Krylov.jl's cg as in Multigrid.jl:860,
  * gauss_seidel = multicolour GS, jacobi = damped Jacobi), steps (0 = skip,
  * Multigrid.jl:860, 871), omega (Jacobi damping).  coarse_direct=1 solves the
@@ -185,6 +192,11 @@ int dec_nn_get_params(dec_nn nn, float *host);
 int dec_nn_set_params(dec_nn nn, const float *host);
 int dec_nn_forward(dec_nn nn, int32_t level, int32_t nsamples, const float *r, float *c,
                    int32_t precise, void *stream);
+/* One smoothing application u += c(r) for a single residual r (n floats,
+ * device) with the inference kernels the V-cycles use: BF16 MFMA dense
+ * layers, hidden layers 1 and 3 kept in registers.  Bit-identical to
+ * dec_nn_forward (precise = 0) followed by u += c. */
+int dec_nn_smooth(dec_nn nn, int32_t level, const float *r, float *u, void *stream);
 /* Loss mean((r - A c)² / s²) over the samples and its gradient (host copies). */
 int dec_nn_loss_grad(dec_nn nn, int32_t level, int32_t nsamples, const float *r,
                      int32_t precise, double *loss, float *grad, void *stream);

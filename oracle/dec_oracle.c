@@ -1143,16 +1143,30 @@ static void mu_cycle(om_level *lv, int nlev, int lvl, const int *steps, int alg,
   free(r); free(rc); free(z); free(pz);
 }
 
+/* Per-level Gauss-Seidel colours: `colors` holds every level's colour array,
+ * finest first, concatenated (NULL: greedy colours, om_greedy_colors). */
 static om_level *levels_setup(int nlev, const om_csc *A, const om_csc *R,
-                              const om_csc *S, int alg, int coarse_direct) {
+                              const om_csc *S, int alg, int coarse_direct,
+                              const int *colors) {
   om_level *lv = XALLOC(om_level, nlev);
+  size_t off = 0;
   for (int l = 0; l < nlev; l++) {
     lv[l].A = A + l;
     lv[l].R = l < nlev - 1 ? R + l : NULL;
     lv[l].S = l < nlev - 1 ? S + l : NULL;
     om_transpose(A + l, &lv[l].At);
     lv[l].color = XALLOC(int, A[l].n);
-    lv[l].ncolors = (alg == OM_GS) ? om_greedy_colors(A + l, lv[l].color) : 1;
+    if (alg == OM_GS && colors) {
+      int nc = 0;
+      for (int v = 0; v < A[l].n; v++) {
+        lv[l].color[v] = colors[off + v];
+        if (colors[off + v] + 1 > nc) nc = colors[off + v] + 1;
+      }
+      lv[l].ncolors = nc;
+    } else {
+      lv[l].ncolors = (alg == OM_GS) ? om_greedy_colors(A + l, lv[l].color) : 1;
+    }
+    off += (size_t)A[l].n;
     lv[l].Z = NULL;
     if (l == nlev - 1 && coarse_direct) {
       lv[l].Z = XALLOC(double, (size_t)A[l].m * A[l].m);
@@ -1174,8 +1188,8 @@ static void levels_free(om_level *lv, int nlev) {
 /* multigrid_μ_cycles (src/Multigrid.jl:819-824). */
 void om_mu_cycles(int nlev, const om_csc *A, const om_csc *R, const om_csc *S,
                   const int *steps, int alg, int mu, int cycles, int coarse_direct,
-                  double omega, double *u, const double *b) {
-  om_level *lv = levels_setup(nlev, A, R, S, alg, coarse_direct);
+                  double omega, double *u, const double *b, const int *colors) {
+  om_level *lv = levels_setup(nlev, A, R, S, alg, coarse_direct, colors);
   for (int c = 0; c < cycles; c++) mu_cycle(lv, nlev, 0, steps, alg, mu, coarse_direct, omega, u, b);
   levels_free(lv, nlev);
 }
@@ -1200,8 +1214,9 @@ static void fmg_rec(om_level *lv, int nlev, int lvl, const int *steps, int alg, 
 
 void om_full_multigrid(int nlev, const om_csc *A, const om_csc *R, const om_csc *S,
                        const int *steps, int alg, int mu, int cycles,
-                       int coarse_direct, double omega, const double *b, double *u) {
-  om_level *lv = levels_setup(nlev, A, R, S, alg, coarse_direct);
+                       int coarse_direct, double omega, const double *b, double *u,
+                       const int *colors) {
+  om_level *lv = levels_setup(nlev, A, R, S, alg, coarse_direct, colors);
   fmg_rec(lv, nlev, 0, steps, alg, mu, cycles, coarse_direct, omega, b, u);
   for (int c = 0; c < cycles; c++) mu_cycle(lv, nlev, 0, steps, alg, mu, coarse_direct, omega, u, b);
   levels_free(lv, nlev);

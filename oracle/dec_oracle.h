@@ -110,10 +110,13 @@ void om_cg(const om_csc *A, const double *b, double *x, int itmax);
 void om_coarse_inverse(const om_csc *A, double *Z);
 void om_mu_cycles(int nlev, const om_csc *A, const om_csc *R, const om_csc *S,
                   const int *steps, int alg, int mu, int cycles, int coarse_direct,
-                  double omega, double *u, const double *b);
+                  double omega, double *u, const double *b, const int *colors);
 void om_full_multigrid(int nlev, const om_csc *A, const om_csc *R, const om_csc *S,
                        const int *steps, int alg, int mu, int cycles,
-                       int coarse_direct, double omega, const double *b, double *u);
+                       int coarse_direct, double omega, const double *b, double *u,
+                       const int *colors);
+/* colors: NULL, or every level's Gauss-Seidel colour array (finest first)
+ * concatenated; NULL colours each level with om_greedy_colors. */
 
 /* ---- inputs ---- */
 void om_uniform_form(unsigned long long seed, int n, double *x);

@@ -432,20 +432,31 @@ class Hierarchy:
         S = (_OCSC * max(nl - 1, 1))(*[s.struct() for s in self.S])
         return nl, A, R, S
 
-    def mu_cycles(self, u, b, cycles, alg=CG, mu=1, steps=3, coarse_direct=False, omega=2 / 3):
+    def _colors(self, colors):
+        """colors: None (greedy colours per level) or one colour array per
+        level, finest first."""
+        if colors is None:
+            return None, None
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(c, np.int32) for c in colors]), np.int32)
+        assert flat.size == sum(a.m for a in self.A)
+        return flat, flat.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+    def mu_cycles(self, u, b, cycles, alg=CG, mu=1, steps=3, coarse_direct=False, omega=2 / 3, colors=None):
         nl, A, R, S = self._structs()
         st = np.full(nl, steps, np.int32) if np.isscalar(steps) else np.ascontiguousarray(steps, np.int32)
         u = _f64(u).copy()
+        keep, cp = self._colors(colors)
         lib().om_mu_cycles(c_int(nl), A, R, S, _p(st), c_int(alg), c_int(mu), c_int(cycles),
-                           c_int(int(coarse_direct)), c_double(omega), _p(u), _p(_f64(b)))
+                           c_int(int(coarse_direct)), c_double(omega), _p(u), _p(_f64(b)), cp)
         return u
 
-    def full_multigrid(self, b, cycles, alg=CG, mu=1, steps=3, coarse_direct=False, omega=2 / 3):
+    def full_multigrid(self, b, cycles, alg=CG, mu=1, steps=3, coarse_direct=False, omega=2 / 3, colors=None):
         nl, A, R, S = self._structs()
         st = np.full(nl, steps, np.int32) if np.isscalar(steps) else np.ascontiguousarray(steps, np.int32)
         u = np.zeros(self.A[0].m)
+        keep, cp = self._colors(colors)
         lib().om_full_multigrid(c_int(nl), A, R, S, _p(st), c_int(alg), c_int(mu), c_int(cycles),
-                                c_int(int(coarse_direct)), c_double(omega), _p(_f64(b)), _p(u))
+                                c_int(int(coarse_direct)), c_double(omega), _p(_f64(b)), _p(u), cp)
         return u
 
 

@@ -76,6 +76,25 @@ def test_gradient_matches_oracle(O, small, precise):
     assert rel(g, gref) <= (1e-4 if precise else 0.1), rel(g, gref)
 
 
+@pytest.mark.parametrize("base,levels", [(0, 3), (2, 4)])
+def test_inference_kernels_match_forward(O, cs, base, levels):
+    """The V-cycle's inference kernels (layers 1 and 3 in registers) add to u
+    exactly the c of the BF16 forward, on every level (642 and 40962 rows
+    are not multiples of the 16-row tile)."""
+    import torch
+
+    md = cs.MultigridData(cs.icosphere(base), cs.BinarySubdivision, levels, cs.DiagonalHodge, 2, project=True)
+    nn = cs.LearnedSmoother(md, seed=8)
+    nn.set_params(random_params(nn, 4))
+    for lv in range(md.nlevels):
+        n = md.sizes[lv]
+        r = torch.tensor(O.uniform_form(30 + lv, n), dtype=torch.float32, device="cuda")
+        u0 = torch.tensor(O.uniform_form(60 + lv, n), dtype=torch.float32, device="cuda")
+        c = nn.forward(lv, r.clone())
+        u = nn.smooth(lv, r, u0.clone())
+        assert torch.equal(u, u0 + c), (lv, n, (u - (u0 + c)).abs().max().item())
+
+
 def test_training_gain_and_solve(O, cs):
     """Training lowers the residual-reduction loss on held-out V-cycle pairs
     compared with the untrained network (damped Jacobi), and the learned

@@ -83,10 +83,34 @@ def test_smoothers_match_oracle(O, cs, sphere_pair, alg):
     b = rhs(O, L, 9)
     b = b - b.mean()
     oalg, galg = (O.GS, cs.gauss_seidel) if alg == "gs" else (O.JACOBI, cs.jacobi)
-    u_o = H.mu_cycles(np.zeros(L.n), b, 3, alg=oalg, steps=2, coarse_direct=True)
+    # the oracle sweeps the colours the library chose (checked proper below)
+    cols = [md.colors(lv)[0] for lv in range(md.nlevels)]
+    u_o = H.mu_cycles(np.zeros(L.n), b, 3, alg=oalg, steps=2, coarse_direct=True, colors=cols)
     u = torch.zeros(L.n, dtype=torch.float64, device="cuda")
     cs.multigrid_vcycles(u, torch.tensor(b, device="cuda"), md, 3, galg, steps=2, coarse_direct=True)
     assert np.linalg.norm(u.cpu().numpy() - u_o) <= 1e-10 * np.linalg.norm(u_o)
+
+
+@pytest.mark.parametrize("which", ["grid", "sphere"])
+def test_gauss_seidel_colouring(which, grid_pair, sphere_pair):
+    """Every level's colouring is proper for the level's operator pattern and
+    uses at most four colours (four on refined levels); on each refined level the coarse vertices (the first
+    n_{l+1}) form colour 0 and the midpoint of coarse edge (a, b) has colour
+    c(a) xor c(b)."""
+    H, md = grid_pair if which == "grid" else sphere_pair
+    for lv in range(md.nlevels):
+        c, k = md.colors(lv)
+        rp, ci, _ = md.export(lv, "A")
+        rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+        off = rows != ci
+        assert k <= 4 and c.min() == 0 and c.max() == k - 1 and (k == 4 or lv == md.nlevels - 1)
+        assert not np.any(c[rows[off]] == c[ci[off]])
+        if lv + 1 < md.nlevels:
+            cc, _ = md.colors(lv + 1)
+            nc = md.sizes[lv + 1]
+            assert not c[:nc].any()
+            m = H.meshes[lv + 1]
+            assert np.array_equal(c[nc:], cc[m.v0] ^ cc[m.v1])
 
 
 def sphere_rhs(O, d, seed):
