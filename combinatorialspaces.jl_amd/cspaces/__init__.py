@@ -3,7 +3,7 @@ FastDEC / Multigrid call signatures.  Compute runs in libdec_hip.so (HIP,
 gfx950); this package is the host-side mirror of the reference interface."""
 from ._lib import DecError, LIB_PATH, lib  # noqa: F401
 from .fastdec import (  # noqa: F401
-    HIP, Barycenter, Circumcenter, DiagonalHodge, GeometricHodge, Mesh, Operator, avg01_mat,
+    HIP, Barycenter, Circumcenter, DiagonalHodge, GeometricHodge, Mesh, Operator, apply_many, avg01_mat,
     dec_boundary, dec_c_wedge_product_, dec_differential, dec_dual_derivative, dec_hodge_star,
     dec_inv_hodge_star, dec_wedge_product, dec_wedge_product_dd, dualize, fast_laplace_beltrami,
     fill_uniform, icosphere, mesh_from_arrays, subdivision, triangulated_grid)

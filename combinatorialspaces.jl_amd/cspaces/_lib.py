@@ -41,6 +41,7 @@ SIGNATURES = {
     "dec_op_create": (c_int, [P, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(P)]),
     "dec_op_shape": (c_int, [P, PI64, PI64, PI64]),
     "dec_op_apply": (c_int, [P, P, P, P, P]),
+    "dec_op_apply_many": (c_int, [c_int32, P, P, P, P, P]),
     "dec_op_export_csr": (c_int, [P, PI64, P, P, P]),
     "dec_op_export_tables": (c_int, [P, P, P]),
     "dec_op_destroy": (c_int, [P]),

@@ -226,6 +226,15 @@ class Operator:
 
     __call__ = apply
 
+    def _args(self, x, y, out):
+        self._check_form(x, self.cols, "x")
+        if self.cols2:
+            if y is None:
+                raise TypeError("this operator takes two forms")
+            self._check_form(y, self.cols2, "y")
+        self._check_form(out, self.rows, "out")
+        return self._h.value, x.data_ptr(), (y.data_ptr() if y is not None else None), out.data_ptr()
+
     def __matmul__(self, x):
         return self.apply(x)
 
@@ -332,3 +341,17 @@ def fill_uniform(x, seed, stream=None):
     code = L.DEC_F64 if x.dtype == _torch().float64 else L.DEC_F32
     check(lib().dec_fill_uniform(c_void_p(x.data_ptr()), x.numel(), seed, code, _stream(stream)))
     return x
+
+
+def apply_many(items, stream=None):
+    """One launch for several applications: items = [(op, x, y, out), ...]
+    (y None for one-form operators), all operators of one dtype.  Same
+    results as op.apply(x, y, out=out) for each item."""
+    items = list(items)
+    if not items:
+        return
+    args = [op._args(x, y, out) for op, x, y, out in items]
+    n = len(args)
+    arr = lambda vals: (c_void_p * n)(*vals)  # noqa: E731
+    check(lib().dec_op_apply_many(n, arr([a[0] for a in args]), arr([a[1] for a in args]),
+                                  arr([a[2] for a in args]), arr([a[3] for a in args]), _stream(stream)))

@@ -122,6 +122,13 @@ int dec_op_create(dec_mesh m, int32_t kind, int32_t deg_m, int32_t deg_n,
 int dec_op_shape(dec_op op, int64_t *rows, int64_t *cols, int64_t *cols2);
 /* out = op(x[, y]).  Unary operators ignore y.  Same dtype as the operator. */
 int dec_op_apply(dec_op op, const void *x, const void *y, void *out, void *stream);
+/* Several applications in one launch: out[i] = ops[i](x[i], y[i]) for
+ * i < count (y[i] ignored by one-form operators).  All operators must share
+ * a dtype.  Results are bit-identical to count dec_op_apply calls; the single
+ * grid removes the per-launch ramp-up and drain.  No counterpart in the
+ * reference (each FastDEC operator is its own kernel, FastDEC.jl:255-289). */
+int dec_op_apply_many(int32_t count, const dec_op *ops, const void *const *x, const void *const *y,
+                      void *const *out, void *stream);
 /* Sparse-matrix operators (d, ∂, d̃, ⋆ (geometric or diagonal), ⋆⁻¹, ∇²,
  * avg01, wedge_dd's averaging matrix): CSR export for parity checks, columns
  * ascending.  Call with NULL arrays to read nnz first. */

@@ -334,3 +334,37 @@ def test_large_mesh_properties(cs):
     area = (cs.dec_hodge_star(2, gm, cs.DiagonalHodge) @ torch.ones(gm.nt, dtype=f64, device="cuda"))
     assert abs(h0.sum().item() - (1 / area).sum().item()) < 1e-10 * 4 * math.pi
     assert abs(h0.sum().item() - 4 * math.pi) < 1e-4
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_apply_many_matches_single_applies(cs, dtype):
+    """One grouped launch of the whole operator set (plus the dual-dual wedge,
+    the boundaries and avg01) gives bit-identical outputs to one launch per
+    operator, including a misaligned form that takes the scalar path."""
+    import torch
+
+    td = torch.float64 if dtype == "f64" else torch.float32
+    m = cs.dualize(cs.icosphere(4))
+    ops = [cs.dec_wedge_product(0, 1, m, cast_float=td).operator,
+           cs.dec_wedge_product(1, 1, m, cast_float=td).operator,
+           cs.dec_wedge_product(0, 2, m, cast_float=td).operator,
+           cs.dec_wedge_product_dd(0, 1, m, cast_float=td).operator,
+           cs.dec_wedge_product_dd(1, 0, m, cast_float=td).operator,
+           cs.dec_differential(0, m, cast_float=td), cs.dec_differential(1, m, cast_float=td),
+           cs.dec_boundary(1, m, cast_float=td), cs.dec_boundary(2, m, cast_float=td),
+           cs.dec_hodge_star(0, m, cs.DiagonalHodge, cast_float=td),
+           cs.dec_hodge_star(1, m, cs.GeometricHodge, cast_float=td),
+           cs.dec_inv_hodge_star(2, m, cs.DiagonalHodge, cast_float=td),
+           cs.fast_laplace_beltrami(m, cs.DiagonalHodge, cast_float=td),
+           cs.avg01_mat(m, cast_float=td)]
+    items, seed = [], 40
+    for k, op in enumerate(ops):
+        x = cs.fill_uniform(torch.empty(op.cols + (1 if k == 1 else 0), dtype=td, device="cuda"), seed)
+        x = x[1:] if k == 1 else x               # odd offset: scalar path
+        y = cs.fill_uniform(torch.empty(op.cols2, dtype=td, device="cuda"), seed + 1) if op.cols2 else None
+        items.append((op, x, y, torch.full((op.rows,), float("nan"), dtype=td, device="cuda")))
+        seed += 2
+    cs.apply_many(items)
+    for op, x, y, out in items:
+        ref = op.apply(x, y)
+        assert torch.equal(out, ref)
