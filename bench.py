@@ -210,19 +210,22 @@ def poisson(torch, cs, base_k=3, levels=5):
     fd = torch.tensor(f, device="cuda")
     out = {"setup_s": setup, "levels": f"{base_k}->{base_k + levels}", "n_fine": fine.nv}
     for variant in ("plain", "mixed"):
-        cs.solve(md, fd, tol=1e-8, variant=variant)       # warm-up (allocations, code objects)
+        cs.solve(md, fd, tol=1e-8, variant=variant)       # warm-up (allocations, graph capture)
         torch.cuda.synchronize()
-        t = time.perf_counter()
-        u, cyc, rr = cs.solve(md, fd, tol=1e-8, variant=variant)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t
+        dt = None
+        for _ in range(3):
+            t = time.perf_counter()
+            u, cyc, rr = cs.solve(md, fd, tol=1e-8, variant=variant)
+            torch.cuda.synchronize()
+            e = time.perf_counter() - t
+            dt = e if dt is None else min(dt, e)
         key = "plain_f64" if variant == "plain" else "mixed_f32_ir"
         out[key] = {"seconds": dt, "cycles": cyc, "relres_f64": rr}
     out["speedup_mixed_vs_plain"] = out["plain_f64"]["seconds"] / out["mixed_f32_ir"]["seconds"]
     return out, md, fd
 
 
-def learned(torch, cs, md, fd, npairs=10000, train_levels=3, steps=600, batch=16):
+def learned(torch, cs, md, fd, npairs=10000, train_levels=3, steps=2000, batch=16):
     """The learned smoother in the level 3->8 mixed solve.  Training pairs come
     from plain V-cycles on seeded spherical-harmonic right-hand sides on the
     3->(3+train_levels) hierarchy; the trained parameters (level independent)
@@ -248,31 +251,42 @@ def learned(torch, cs, md, fd, npairs=10000, train_levels=3, steps=600, batch=16
     out = {"pairs": npairs, "train_mesh_vertices": mdt.sizes[0], "pairs_s": t_pairs, "train_steps": steps,
            "train_batch": batch, "train_s": t_train, "val_loss_untrained": l0, "val_loss_trained": l1,
            "loss_first": float(hist[0]), "loss_last": float(hist[-1])}
-    # one forward pass over a batch at the finest level (the BF16 MFMA layers)
-    B = 8
-    Rf = torch.randn(B, md.sizes[0], dtype=torch.float32, device="cuda")
-    for _ in range(2):
-        nn.forward(0, Rf)
+    # one smoothing application at the finest level with the inference kernels
+    # (BF16 MFMA dense layers): time per application, and the dense layers'
+    # rate (2 layers x 2 x 64 x 32 flops per row)
+    n0 = md.sizes[0]
+    r0 = torch.randn(n0, dtype=torch.float32, device="cuda")
+    u0 = torch.zeros(n0, dtype=torch.float32, device="cuda")
+    for _ in range(3):
+        nn.smooth(0, r0, u0)
+    stream = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    t = time.perf_counter()
-    reps = 10
+    a.record(stream)
+    reps = 20
     for _ in range(reps):
-        nn.forward(0, Rf)
+        nn.smooth(0, r0, u0)
+    b.record(stream)
     torch.cuda.synchronize()
-    fwd_ms = (time.perf_counter() - t) / reps * 1e3
-    out["forward_ms_per_vector_fine"] = fwd_ms / B
-    for min_rows in (0, 16385, 40963):
-        nn.attach(True, min_rows=min_rows)
+    app_ms = a.elapsed_time(b) / reps
+    out["inference_ms_per_application_fine"] = app_ms
+    out["dense_layer_TFLOPs_fine"] = 2 * 2 * 64 * 32 * n0 / (app_ms * 1e-3) / 1e12
+    # levels that use the network: all; the two finest; the levels below the
+    # finest (the finest keeps Gauss-Seidel)
+    for tag, lo, hi in (("all_levels", 0, 0), ("levels_ge_40963", 40963, 0), ("levels_2563_to_163842", 2563, 163842)):
+        nn.attach(True, min_rows=lo, max_rows=hi)
         cs.solve(md, fd, tol=1e-8, variant="mixed")
         torch.cuda.synchronize()
-        t = time.perf_counter()
-        u, cyc, rr = cs.solve(md, fd, tol=1e-8, variant="mixed")
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t
+        best = None
+        for _ in range(3):
+            t = time.perf_counter()
+            u, cyc, rr = cs.solve(md, fd, tol=1e-8, variant="mixed")
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
         st = cs.learned_stats(md)
-        out[f"mixed_learned_min_rows_{min_rows}"] = {
-            "seconds": dt, "cycles": cyc, "relres_f64": rr, "fallbacks": st["fallbacks"],
-            "learned_cycles": st["learned_cycles"]}
+        out[f"mixed_learned_{tag}"] = {"seconds": best, "cycles": cyc, "relres_f64": rr, "fallbacks": st["fallbacks"],
+                                       "learned_cycles": st["learned_cycles"]}
     nn.attach(False)
     return out
 
@@ -366,7 +380,7 @@ def batched_heat(torch, cs, ninst=64, nsteps=100, seed=7):
             "launch_shape": shape, "tune_s": tune_s, "max_rel_err_f32_vs_f64": err}
 
 
-COPY_MIB = 256  # calibration copy: 256 MiB read + 256 MiB written per launch
+COPY_MIB = 256  # calibration kernel (elementwise negation): 256 MiB read + 256 MiB written
 
 
 def pmc_probe(op_name, k, sets, launches):
@@ -380,7 +394,7 @@ def pmc_probe(op_name, k, sets, launches):
     x = torch.ones(COPY_MIB * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
     for _ in range(3):
-        y.copy_(x)
+        torch.neg(x, out=y)     # an elementwise kernel: reads and writes COPY_MIB MiB
     torch.cuda.synchronize()
     del x, y
     mesh = cs.icosphere(k)
@@ -416,16 +430,14 @@ def pmc_counter(counter, op_name, kernel, k, sets, launches, timeout_s=240):
     files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
     if not files:
         raise RuntimeError(f"rocprofv3 --pmc {counter}: no counter file")
-    rows = [r for r in csv.DictReader(open(files[0])) if r.get("Counter_Name") == counter]
-    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
-    # the calibration copies are the copy dispatches before the library's first kernel
-    first_lib = min((int(r["Dispatch_Id"]) for r in rows if "::k_" in r["Kernel_Name"]), default=None)
     kv, cv = [], []
-    for r in rows:
+    for r in csv.DictReader(open(files[0])):
+        if r.get("Counter_Name") != counter:
+            continue
         name, v = r["Kernel_Name"], float(r["Counter_Value"])
         if kernel + "<" in name or kernel + "(" in name:
             kv.append(v)
-        elif "copy" in name.lower() and first_lib is not None and int(r["Dispatch_Id"]) < first_lib:
+        elif "elementwise_kernel" in name and "neg" in name.lower():
             cv.append(v)
     shutil.rmtree(out, ignore_errors=True)
     if not kv or not cv:
@@ -459,6 +471,8 @@ def main():
     ap.add_argument("--no-cache-resident", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--learned-pairs", type=int, default=10000)
+    ap.add_argument("--learned-steps", type=int, default=2000)
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--pmc-probe", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -537,7 +551,7 @@ def main():
         extra["poisson"], md, fd = poisson(torch, cs)
         print(f"# poisson: {json.dumps(extra['poisson'])}", file=sys.stderr, flush=True)
         if not args.no_learned:
-            extra["learned"] = learned(torch, cs, md, fd)
+            extra["learned"] = learned(torch, cs, md, fd, npairs=args.learned_pairs, steps=args.learned_steps)
             print(f"# learned: {json.dumps(extra['learned'])}", file=sys.stderr, flush=True)
         del md, fd
     if not args.no_batch:

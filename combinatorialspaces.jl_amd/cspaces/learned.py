@@ -31,7 +31,7 @@ class LearnedSmoother:
     def __del__(self):
         if getattr(self, "_h", None):
             try:
-                lib().dec_mg_set_learned(self.md.handle, None, 0, 0, 1)
+                lib().dec_mg_set_learned(self.md.handle, None, 0, 0, 0, 1)
                 lib().dec_nn_destroy(self._h)
             except Exception:
                 pass
@@ -96,10 +96,11 @@ class LearnedSmoother:
                                  ptr(hist), _stream(stream)))
         return hist[:steps]
 
-    def attach(self, enable=True, min_rows=16385, applications=1):
-        """Use the network as the smoother of the float32 V-cycles of md."""
+    def attach(self, enable=True, min_rows=16385, max_rows=0, applications=1):
+        """Use the network as the smoother of the float32 V-cycles of md on the
+        levels of min_rows to max_rows rows (max_rows 0: no upper bound)."""
         check(lib().dec_mg_set_learned(self.md.handle, self._h if enable else None, int(bool(enable)), min_rows,
-                                       applications))
+                                       max_rows, applications))
 
 
 def learned_stats(md: MultigridData):

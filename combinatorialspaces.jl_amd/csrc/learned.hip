@@ -238,14 +238,16 @@ __global__ void k_output(int n, const float *__restrict__ x, const AT *__restric
 // Three kernels for one residual (B = 1): x and P x; layer 1 recomputed for
 // every neighbour from (x, P x) and layer 2 on MFMA; layer 3 on MFMA with the
 // output head, adding c to u.  h1 and h3 never reach memory.  Each wavefront
-// takes 16-row tiles; lane (li, lg) owns row r0 + li and the features
-// F(lg) = {4lg..4lg+3} ∪ {16+4lg..16+4lg+3}, which are exactly its A-operand
-// entries of v_mfma_f32_16x16x16bf16 for the K steps 0, 1 ([h]) and 2, 3
-// ([P h]).  Expressions, summation orders and BF16 rounding points are those
+// takes 64 rows at a time, one row per lane for the gathers (64 rows of
+// loads in flight per wavefront), and passes the rows [h, P h] through LDS
+// into the A operands of v_mfma_f32_16x16x16bf16 (four 16-row tiles; lane
+// (li, lg) of tile q takes row 16q + li, K entries 4lg..4lg+3 of each K
+// step).  Expressions, summation orders and BF16 rounding points are those
 // of k_input / k_spmm / k_dense_bf16 / k_output, so that u gains exactly the
 // c of the BF16 forward.
 
-__device__ __forceinline__ int feat(int lg, int q) { return q < 4 ? 4 * lg + q : 16 + 4 * lg + (q - 4); }
+constexpr int kInfWaves = TB / 64;
+constexpr int kRowPitch = 2 * H + 8;  // bf16 per LDS row: [h | P h] plus padding
 
 __device__ __forceinline__ void load_wB(const float *__restrict__ W, int li, int lg, short4_t B[2][4]) {
 #pragma unroll
@@ -256,20 +258,54 @@ __device__ __forceinline__ void load_wB(const float *__restrict__ W, int li, int
       for (int i = 0; i < 4; i++) B[jt][ks][i] = static_cast<short>(f2bf(W[(ks * 16 + 4 * lg + i) * H + jt * 16 + li]));
 }
 
-__device__ __forceinline__ short4_t pack4(const unsigned short *h) {
+__device__ __forceinline__ short4_t lds4(const unsigned short *p) {
+  uint2 w = *reinterpret_cast<const uint2 *>(p);
   short4_t a;
-  a[0] = static_cast<short>(h[0]);
-  a[1] = static_cast<short>(h[1]);
-  a[2] = static_cast<short>(h[2]);
-  a[3] = static_cast<short>(h[3]);
+  a[0] = static_cast<short>(w.x & 0xffffu);
+  a[1] = static_cast<short>(w.x >> 16);
+  a[2] = static_cast<short>(w.y & 0xffffu);
+  a[3] = static_cast<short>(w.y >> 16);
   return a;
 }
 
-__device__ __forceinline__ void unpack4(uint2 w, unsigned short *h) {
-  h[0] = static_cast<unsigned short>(w.x & 0xffffu);
-  h[1] = static_cast<unsigned short>(w.x >> 16);
-  h[2] = static_cast<unsigned short>(w.y & 0xffffu);
-  h[3] = static_cast<unsigned short>(w.y >> 16);
+__device__ __forceinline__ void unpack8(uint4 w, float *f) {
+  f[0] = bf2f(static_cast<unsigned short>(w.x & 0xffffu));
+  f[1] = bf2f(static_cast<unsigned short>(w.x >> 16));
+  f[2] = bf2f(static_cast<unsigned short>(w.y & 0xffffu));
+  f[3] = bf2f(static_cast<unsigned short>(w.y >> 16));
+  f[4] = bf2f(static_cast<unsigned short>(w.z & 0xffffu));
+  f[5] = bf2f(static_cast<unsigned short>(w.z >> 16));
+  f[6] = bf2f(static_cast<unsigned short>(w.w & 0xffffu));
+  f[7] = bf2f(static_cast<unsigned short>(w.w >> 16));
+}
+
+// The four 16-row tiles of a wavefront's 64 LDS rows times W (64 x 32):
+// acc[q][jt] holds D[4lg+i][16jt+li] of tile q.
+__device__ __forceinline__ void mfma_rows(const unsigned short *rows, int li, int lg, const short4_t Bw[2][4],
+                                          float4_t acc[4][2]) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const unsigned short *r = rows + (16 * q + li) * kRowPitch + 4 * lg;
+    short4_t A0 = lds4(r), A1 = lds4(r + 16), A2 = lds4(r + 32), A3 = lds4(r + 48);
+#pragma unroll
+    for (int jt = 0; jt < 2; jt++) {
+      float4_t c = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A0, Bw[jt][0], c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1, Bw[jt][1], c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A2, Bw[jt][2], c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A3, Bw[jt][3], c, 0, 0, 0);
+      acc[q][jt] = c;
+    }
+  }
+}
+
+__device__ __forceinline__ uint4 pack8(const float *f) {
+  uint4 w;
+  w.x = static_cast<unsigned>(f2bf(f[0])) | (static_cast<unsigned>(f2bf(f[1])) << 16);
+  w.y = static_cast<unsigned>(f2bf(f[2])) | (static_cast<unsigned>(f2bf(f[3])) << 16);
+  w.z = static_cast<unsigned>(f2bf(f[4])) | (static_cast<unsigned>(f2bf(f[5])) << 16);
+  w.w = static_cast<unsigned>(f2bf(f[6])) | (static_cast<unsigned>(f2bf(f[7])) << 16);
+  return w;
 }
 
 // xp[v] = (x, P x) with x = r / s
@@ -289,40 +325,67 @@ __global__ void k_inf_input(int n, int pw, const int *__restrict__ pcol, const f
   xp[v] = make_float2(xv, acc);
 }
 
-// h2 = relu([h1, P h1] W2 + b2), h1 = relu([x, P x] W1 + b1) evaluated per
-// neighbour.  All waves of a block run the same number of tile steps.
+// One row per lane: the row's own h (BF16) and m = P h (FP32 sums over the
+// neighbours' rows, ascending ELL order, rounded to BF16) into its LDS row.
+__device__ __forceinline__ void gather_rows(int n, int pw, int v, const int *__restrict__ pcol,
+                                            const float *__restrict__ pval, const unsigned short *__restrict__ h,
+                                            unsigned short *my) {
+  const uint4 *hv = reinterpret_cast<const uint4 *>(h + static_cast<size_t>(v) * H);
+  uint4 *dst = reinterpret_cast<uint4 *>(my);
+#pragma unroll
+  for (int k = 0; k < 4; k++) dst[k] = hv[k];
+  float m[H];
+#pragma unroll
+  for (int j = 0; j < H; j++) m[j] = 0.0f;
+  for (int k = 0; k < pw; k++) {
+    size_t qq = static_cast<size_t>(k) * n + v;
+    int c = pcol[qq];
+    if (c < 0) break;
+    float pv = pval[qq];
+    const uint4 *hc = reinterpret_cast<const uint4 *>(h + static_cast<size_t>(c) * H);
+#pragma unroll
+    for (int k4 = 0; k4 < 4; k4++) {
+      float f[8];
+      unpack8(hc[k4], f);
+#pragma unroll
+      for (int e = 0; e < 8; e++) m[8 * k4 + e] = m[8 * k4 + e] + pv * f[e];
+    }
+  }
+#pragma unroll
+  for (int k4 = 0; k4 < 4; k4++) dst[4 + k4] = pack8(m + 8 * k4);
+}
+
+__device__ __forceinline__ void zero_row(unsigned short *my) {
+  uint4 *dst = reinterpret_cast<uint4 *>(my);
+#pragma unroll
+  for (int k = 0; k < 8; k++) dst[k] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 __global__ void __launch_bounds__(TB) k_inf_layer2(int n, int pw, const int *__restrict__ pcol,
                                                    const float *__restrict__ pval, const float2 *__restrict__ xp,
                                                    const float *__restrict__ th, unsigned short *__restrict__ h2) {
+  __shared__ __align__(16) unsigned short rows_s[kInfWaves][64 * kRowPitch];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int li = lane & 15, lg = lane >> 4;
-  float w0[8], w1[8], b1[8];
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    int f = feat(lg, q);
-    w0[q] = th[OW1 + f];
-    w1[q] = th[OW1 + H + f];
-    b1[q] = th[OB1 + f];
-  }
+  unsigned short *rows = rows_s[w];
   short4_t Bw[2][4];
   load_wB(th + OW2, li, lg, Bw);
   float bias0 = th[OB2 + li], bias1 = th[OB2 + 16 + li];
-  int ntiles = (n + 15) / 16;
-  constexpr int WPB = TB / 64;
-  for (int base = blockIdx.x * WPB; base < ntiles; base += gridDim.x * WPB) {
+  int nsteps = (n + 63) / 64;
+  for (int base = blockIdx.x * kInfWaves; base < nsteps; base += gridDim.x * kInfWaves) {
     int t = base + w;
-    if (t >= ntiles) continue;
-    int r0 = t * 16, v = r0 + li;
-    short4_t A[4];
-    if (v < n) {
+    int v = t * 64 + lane;
+    bool live = t < nsteps && v < n;
+    unsigned short *my = rows + lane * kRowPitch;
+    if (live) {
+      const float *w1 = th + OW1, *b1 = th + OB1;
       float2 f = xp[v];
-      unsigned short hb[8];
-      float m[8];
+      float m[H], hh[H];
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        float z = f.x * w0[q] + f.y * w1[q] + b1[q];
-        hb[q] = f2bf(z > 0.0f ? z : 0.0f);
-        m[q] = 0.0f;
+      for (int j = 0; j < H; j++) {
+        float z = f.x * w1[j] + f.y * w1[H + j] + b1[j];
+        hh[j] = z > 0.0f ? z : 0.0f;
+        m[j] = 0.0f;
       }
       for (int k = 0; k < pw; k++) {
         size_t qq = static_cast<size_t>(k) * n + v;
@@ -331,112 +394,96 @@ __global__ void __launch_bounds__(TB) k_inf_layer2(int n, int pw, const int *__r
         float pv = pval[qq];
         float2 fc = xp[c];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-          float z = fc.x * w0[q] + fc.y * w1[q] + b1[q];
-          m[q] = m[q] + pv * bf2f(f2bf(z > 0.0f ? z : 0.0f));
+        for (int j = 0; j < H; j++) {
+          float z = fc.x * w1[j] + fc.y * w1[H + j] + b1[j];
+          m[j] = m[j] + pv * bf2f(f2bf(z > 0.0f ? z : 0.0f));
         }
       }
-      unsigned short mb[8];
+      uint4 *dst = reinterpret_cast<uint4 *>(my);
 #pragma unroll
-      for (int q = 0; q < 8; q++) mb[q] = f2bf(m[q]);
-      A[0] = pack4(hb);
-      A[1] = pack4(hb + 4);
-      A[2] = pack4(mb);
-      A[3] = pack4(mb + 4);
+      for (int k4 = 0; k4 < 4; k4++) dst[k4] = pack8(hh + 8 * k4);
+#pragma unroll
+      for (int k4 = 0; k4 < 4; k4++) dst[4 + k4] = pack8(m + 8 * k4);
     } else {
-      A[0] = A[1] = A[2] = A[3] = short4_t{0, 0, 0, 0};
+      zero_row(my);
     }
-    float4_t acc0 = float4_t{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
+    __syncthreads();
+    float4_t acc[4][2];
+    mfma_rows(rows, li, lg, Bw, acc);
+    __syncthreads();
+    // D[4lg+i][16jt+li] of tile q -> LDS row 16q+4lg+i, column 16jt+li
 #pragma unroll
-    for (int ks = 0; ks < 4; ks++) acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[0][ks], acc0, 0, 0, 0);
+    for (int q = 0; q < 4; q++)
 #pragma unroll
-    for (int ks = 0; ks < 4; ks++) acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[1][ks], acc1, 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      int rr = r0 + 4 * lg + i;
-      if (rr < n) {
-        float z0 = acc0[i] + bias0, z1 = acc1[i] + bias1;
-        h2[static_cast<size_t>(rr) * H + li] = f2bf(z0 > 0.0f ? z0 : 0.0f);
-        h2[static_cast<size_t>(rr) * H + 16 + li] = f2bf(z1 > 0.0f ? z1 : 0.0f);
+      for (int i = 0; i < 4; i++) {
+        unsigned short *o = rows + (16 * q + 4 * lg + i) * kRowPitch;
+        float z0 = acc[q][0][i] + bias0, z1 = acc[q][1][i] + bias1;
+        o[li] = f2bf(z0 > 0.0f ? z0 : 0.0f);
+        o[16 + li] = f2bf(z1 > 0.0f ? z1 : 0.0f);
       }
+    __syncthreads();
+    if (live) {
+      const uint4 *src = reinterpret_cast<const uint4 *>(my);
+      uint4 *dst = reinterpret_cast<uint4 *>(h2 + static_cast<size_t>(v) * H);
+#pragma unroll
+      for (int k = 0; k < 4; k++) dst[k] = src[k];
     }
+    __syncthreads();
   }
 }
 
 // h3 = relu([h2, P h2] W3 + b3); y = alpha x + h3 wo + bo (ascending sum over
-// the 32 features, staged through LDS); u += (s / D) y.
+// the 32 features); u += (s / D) y.
 __global__ void __launch_bounds__(TB) k_inf_layer3(int n, int pw, const int *__restrict__ pcol,
                                                    const float *__restrict__ pval, const unsigned short *__restrict__ h2,
-                                                   const float2 *__restrict__ xp, const float *__restrict__ th,
+                                                   const float2 *__restrict__ xo, const float *__restrict__ th,
                                                    const float *__restrict__ s, const float *__restrict__ D,
                                                    float *__restrict__ u) {
-  constexpr int WPB = TB / 64;
-  __shared__ float prod[WPB][16][H + 1];
+  __shared__ __align__(16) unsigned short rows_s[kInfWaves][64 * kRowPitch];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int li = lane & 15, lg = lane >> 4;
+  unsigned short *rows = rows_s[w];
+  float *prod = reinterpret_cast<float *>(rows);  // 64 x (H + 1) floats, after the MFMA reads
   short4_t Bw[2][4];
   load_wB(th + OW3, li, lg, Bw);
   float bias0 = th[OB3 + li], bias1 = th[OB3 + 16 + li];
   float wo0 = th[OWO + li], wo1 = th[OWO + 16 + li];
   float alpha = th[OAL], bo = th[OBO], sc = s[0];
-  int ntiles = (n + 15) / 16;
-  for (int base = blockIdx.x * WPB; base < ntiles; base += gridDim.x * WPB) {
+  int nsteps = (n + 63) / 64;
+  for (int base = blockIdx.x * kInfWaves; base < nsteps; base += gridDim.x * kInfWaves) {
     int t = base + w;
-    bool active = t < ntiles;
-    int r0 = t * 16, v = r0 + li;
-    if (active) {
-      short4_t A[4];
-      if (v < n) {
-        const unsigned short *hv = h2 + static_cast<size_t>(v) * H;
-        unsigned short hb[8];
-        unpack4(*reinterpret_cast<const uint2 *>(hv + 4 * lg), hb);
-        unpack4(*reinterpret_cast<const uint2 *>(hv + 16 + 4 * lg), hb + 4);
-        float m[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) m[q] = 0.0f;
-        for (int k = 0; k < pw; k++) {
-          size_t qq = static_cast<size_t>(k) * n + v;
-          int c = pcol[qq];
-          if (c < 0) break;
-          float pv = pval[qq];
-          const unsigned short *hc = h2 + static_cast<size_t>(c) * H;
-          unsigned short nb[8];
-          unpack4(*reinterpret_cast<const uint2 *>(hc + 4 * lg), nb);
-          unpack4(*reinterpret_cast<const uint2 *>(hc + 16 + 4 * lg), nb + 4);
-#pragma unroll
-          for (int q = 0; q < 8; q++) m[q] = m[q] + pv * bf2f(nb[q]);
-        }
-        unsigned short mb[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) mb[q] = f2bf(m[q]);
-        A[0] = pack4(hb);
-        A[1] = pack4(hb + 4);
-        A[2] = pack4(mb);
-        A[3] = pack4(mb + 4);
-      } else {
-        A[0] = A[1] = A[2] = A[3] = short4_t{0, 0, 0, 0};
-      }
-      float4_t acc0 = float4_t{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
-#pragma unroll
-      for (int ks = 0; ks < 4; ks++) acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[0][ks], acc0, 0, 0, 0);
-#pragma unroll
-      for (int ks = 0; ks < 4; ks++) acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[1][ks], acc1, 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        float z0 = acc0[i] + bias0, z1 = acc1[i] + bias1;
-        prod[w][4 * lg + i][li] = bf2f(f2bf(z0 > 0.0f ? z0 : 0.0f)) * wo0;
-        prod[w][4 * lg + i][16 + li] = bf2f(f2bf(z1 > 0.0f ? z1 : 0.0f)) * wo1;
-      }
+    int v = t * 64 + lane;
+    bool live = t < nsteps && v < n;
+    unsigned short *my = rows + lane * kRowPitch;
+    float xv = 0.0f, dv = 1.0f, uv = 0.0f;
+    if (live) {
+      xv = xo[v].x;
+      dv = D[v];
+      uv = u[v];
+      gather_rows(n, pw, v, pcol, pval, h2, my);
+    } else {
+      zero_row(my);
     }
     __syncthreads();
-    if (active && lane < 16) {
-      int row = r0 + lane;
-      if (row < n) {
-        float hw = 0.0f;
-        for (int j = 0; j < H; j++) hw = hw + prod[w][lane][j];
-        float yy = alpha * xp[row].x + hw + bo;
-        u[row] = u[row] + (sc / D[row]) * yy;
+    float4_t acc[4][2];
+    mfma_rows(rows, li, lg, Bw, acc);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        float *o = prod + (16 * q + 4 * lg + i) * (H + 1);
+        float z0 = acc[q][0][i] + bias0, z1 = acc[q][1][i] + bias1;
+        o[li] = bf2f(f2bf(z0 > 0.0f ? z0 : 0.0f)) * wo0;
+        o[16 + li] = bf2f(f2bf(z1 > 0.0f ? z1 : 0.0f)) * wo1;
       }
+    __syncthreads();
+    if (live) {
+      const float *o = prod + lane * (H + 1);
+      float hw = 0.0f;
+      for (int j = 0; j < H; j++) hw = hw + o[j];
+      float yy = alpha * xv + hw + bo;
+      u[v] = uv + (sc / dv) * yy;
     }
     __syncthreads();
   }
@@ -693,20 +740,32 @@ __global__ void k_w1_back(int n, const float *__restrict__ x, const float *__res
 }
 
 // out[j] = Σ_p part[p * stride + off + j] for j < width, sequential over p.
-__global__ void k_reduce_parts(const float *__restrict__ part, int nparts, int stride, int off, int width,
-                               float *__restrict__ out) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+// Column sums out[j] = scale · Σ_p part[p·stride + off + j] (j < width) of
+// per-block partials, in two deterministic passes: workgroup (x, c) sums the
+// partials p of chunk c for 64 columns (coalesced over the columns, double
+// accumulators), then one thread per column adds the chunk sums in order.
+constexpr int kRedChunks = 128;
+
+template <class PT>
+__global__ void k_reduce_chunks(const PT *__restrict__ part, int nparts, int stride, int off, int width, int len,
+                                double *__restrict__ tmp) {
+  int j = blockIdx.x * 64 + threadIdx.x;
+  int c = blockIdx.y;
   if (j >= width) return;
-  float acc = 0.0f;
-  for (int p = 0; p < nparts; p++) acc += part[static_cast<size_t>(p) * stride + off + j];
-  out[j] = acc;
+  int p0 = c * len, p1 = min(nparts, p0 + len);
+  double acc = 0.0;
+  for (int p = p0; p < p1; p++) acc += static_cast<double>(part[static_cast<size_t>(p) * stride + off + j]);
+  tmp[static_cast<size_t>(c) * width + j] = acc;
 }
 
-__global__ void k_reduce_parts_d(const double *__restrict__ part, int nparts, double scale, double *__restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+template <class OT>
+__global__ void k_reduce_final(const double *__restrict__ tmp, int chunks, int width, double scale,
+                               OT *__restrict__ out) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= width) return;
   double acc = 0.0;
-  for (int p = 0; p < nparts; p++) acc += part[p];
-  *out = acc * scale;
+  for (int c = 0; c < chunks; c++) acc += tmp[static_cast<size_t>(c) * width + j];
+  out[j] = static_cast<OT>(acc * scale);
 }
 
 __global__ void k_adam(int np, float *__restrict__ th, const float *__restrict__ g, float *__restrict__ m1,
@@ -794,9 +853,9 @@ struct dec_nn_s {
   int adam_t = 0;
   // work buffers (grown on demand)
   DBuf<float> x, px, y, c, s, drho, dg, part, r;
-  DBuf<double> dpart, dloss, spart;
-  DBuf<float2> xp;                                 // inference: (x, P x)
-  DBuf<unsigned short> ih2;                        // inference: layer-2 output
+  DBuf<double> dpart, dloss, spart, rtmp;
+  DBuf<float2> ix;                                 // inference: (x, P x)
+  DBuf<unsigned short> ih1, ih2;                   // inference: layer-1 and layer-2 rows
   DBuf<unsigned char> h1, hm1, h2, hm2, h3, dz;  // activation storage (bf16 or f32)
   DBuf<int32_t> idx;
   size_t cap_rows = 0;
@@ -818,8 +877,24 @@ void ensure(dec_nn_s &nn, size_t rows, int precise) {
   size_t parts = (rows + 255) / 256 + 64;
   nn.part.alloc(parts * 2 * H * H);
   nn.dpart.alloc(parts + 64);
+  nn.rtmp.alloc(static_cast<size_t>(kRedChunks) * 2 * H * H);
   nn.cap_rows = rows;
   nn.cap_prec = precise;
+}
+
+// out[j] = scale · Σ_p part[p·stride + off + j], j < width (kernels above)
+template <class PT, class OT>
+void reduce_parts(dec_nn_s &nn, const PT *part, int nparts, int stride, int off, int width, double scale, OT *out,
+                  hipStream_t st) {
+  int chunks = std::max(1, std::min(kRedChunks, nparts));
+  int len = (nparts + chunks - 1) / chunks;
+  chunks = (nparts + len - 1) / len;
+  size_t need = static_cast<size_t>(chunks) * width;
+  if (nn.rtmp.n < need) fail(DEC_ERR_ARG, "learned smoother: reduction scratch too small");
+  hipLaunchKernelGGL((k_reduce_chunks<PT>), dim3((width + 63) / 64, chunks), dim3(64), 0, st, part, nparts, stride,
+                     off, width, len, nn.rtmp.p);
+  hipLaunchKernelGGL((k_reduce_final<OT>), dim3((width + 63) / 64), dim3(64), 0, st, nn.rtmp.p, chunks, width, scale,
+                     out);
 }
 
 template <class AT>
@@ -884,42 +959,38 @@ void loss_grad_t(dec_nn_s &nn, const dec::NnLevel &L, int B, const float *r, dou
   int nbv = gv.x * B, nbf = gf.x * B;
   hipLaunchKernelGGL(k_loss, gv, dim3(TB), 0, st, n, B, L.a.width, L.a.col.p, L.a.val.p, nn.x.p, nn.y.p, L.D.p,
                      nullptr, nn.drho.p, nn.dpart.p);
-  hipLaunchKernelGGL(k_reduce_parts_d, dim3(1), dim3(1), 0, st, nn.dpart.p, nbv, 1.0 / (static_cast<double>(n) * B),
-                     loss_out);
+  reduce_parts(nn, nn.dpart.p, nbv, 1, 0, 1, 1.0 / (static_cast<double>(n) * B), loss_out, st);
   // output head: dz3, and the (wo, bo, alpha, b3) partials
   hipLaunchKernelGGL(k_out_back<AT>, gv, dim3(TB), 0, st, n, L.at.width, L.at.col.p, L.at.val.p, nn.drho.p, L.D.p,
                      nn.x.p, A.h3, th, A.dz, nn.part.p);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(64), 0, st, nn.part.p, nbv, 2 * H + 2, 0, H + 2, g + OWO);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(64), 0, st, nn.part.p, nbv, 2 * H + 2, H + 2, H, g + OB3);
+  reduce_parts(nn, nn.part.p, nbv, 2 * H + 2, 0, H + 2, 1.0, g + OWO, st);
+  reduce_parts(nn, nn.part.p, nbv, 2 * H + 2, H + 2, H, 1.0, g + OB3, st);
   // dense layers 3 and 2: weight gradient, input gradient, mix through Pᵀ
   auto layer = [&](const AT *gh, const AT *gm, int ow, const AT *h_below, int ob_below) {
     if constexpr (sizeof(AT) == 2) {
       int waves = (R + 255) / 256;
       hipLaunchKernelGGL(k_wgrad_bf16, dim3((waves + 3) / 4), dim3(TB), 0, st, R, gh, gm, A.dz, nn.part.p);
-      hipLaunchKernelGGL(k_reduce_parts, dim3(dec::nblocks(2 * H * H, TB)), dim3(TB), 0, st, nn.part.p, waves,
-                         2 * H * H, 0, 2 * H * H, g + ow);
+      reduce_parts(nn, nn.part.p, waves, 2 * H * H, 0, 2 * H * H, 1.0, g + ow, st);
       int dw = (R + 15) / 16;
       hipLaunchKernelGGL(k_dgrad_bf16, dim3((dw + 3) / 4), dim3(TB), 0, st, R, A.dz, th + ow, nn.dg.p);
     } else {
       int chunks = (R + 255) / 256;
       hipLaunchKernelGGL(k_wgrad_f32, dim3(chunks), dim3(TB), 0, st, R, gh, gm, A.dz, nn.part.p);
-      hipLaunchKernelGGL(k_reduce_parts, dim3(dec::nblocks(2 * H * H, TB)), dim3(TB), 0, st, nn.part.p, chunks,
-                         2 * H * H, 0, 2 * H * H, g + ow);
+      reduce_parts(nn, nn.part.p, chunks, 2 * H * H, 0, 2 * H * H, 1.0, g + ow, st);
       hipLaunchKernelGGL(k_dgrad_f32, dim3(dec::nblocks(static_cast<long long>(R) * 2 * H, TB)), dim3(TB), 0, st, R,
                          A.dz, th + ow, nn.dg.p);
     }
     // dz of the layer below (overwrites dz) and its bias partials
     hipLaunchKernelGGL(k_back_mix<AT>, gf, dim3(TB), 0, st, n, L.pt.width, L.pt.col.p, L.pt.val.p, nn.dg.p, h_below,
                        A.dz, nn.part.p);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(64), 0, st, nn.part.p, nbf, H, 0, H, g + ob_below);
+    reduce_parts(nn, nn.part.p, nbf, H, 0, H, 1.0, g + ob_below, st);
   };
   layer(A.h2, A.m2, OW3, A.h2, OB2);
   layer(A.h1, A.m1, OW2, A.h1, OB1);
   // first layer: dW1 = f0ᵀ dz1
   dim3 g1(dec::nblocks(n, TB / H), B);
   hipLaunchKernelGGL(k_w1_back<AT>, g1, dim3(TB), 0, st, n, nn.x.p, nn.px.p, A.dz, nn.part.p);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(64), 0, st, nn.part.p, static_cast<int>(g1.x) * B, 2 * H, 0, 2 * H,
-                     g + OW1);
+  reduce_parts(nn, nn.part.p, static_cast<int>(g1.x) * B, 2 * H, 0, 2 * H, 1.0, g + OW1, st);
 }
 
 void lcheck(hipError_t e, const char *what) {
@@ -936,7 +1007,8 @@ const dec::NnLevel &level_of(dec_nn_s &nn, int lvl) {
 namespace dec {
 
 void reserve_inference(dec_nn_s &nn, int n) {
-  if (nn.xp.n < static_cast<size_t>(n)) nn.xp.alloc(n);
+  if (nn.ix.n < static_cast<size_t>(n)) nn.ix.alloc(n);
+  if (nn.ih1.n < static_cast<size_t>(n) * H) nn.ih1.alloc(static_cast<size_t>(n) * H);
   if (nn.ih2.n < static_cast<size_t>(n) * H) nn.ih2.alloc(static_cast<size_t>(n) * H);
   if (nn.s.n < 1) nn.s.alloc(1);
   if (nn.spart.n < static_cast<size_t>(kScaleBlocks)) nn.spart.alloc(kScaleBlocks);
@@ -960,13 +1032,13 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s) {
   reserve_inference(nn, n);
   scale(nn, n, 1, r, s);
   hipLaunchKernelGGL(k_inf_input, dim3(dec::nblocks(n, TB)), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, r,
-                     nn.s.p, nn.xp.p);
-  int tiles = (n + 15) / 16;
-  int blocks = std::max(1, std::min((tiles + TB / 64 - 1) / (TB / 64), 2048));
-  hipLaunchKernelGGL(k_inf_layer2, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.xp.p,
+                     nn.s.p, nn.ix.p);
+  int steps = (n + 63) / 64;
+  int blocks = std::max(1, (steps + kInfWaves - 1) / kInfWaves);
+  hipLaunchKernelGGL(k_inf_layer2, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ix.p,
                      nn.th.p, nn.ih2.p);
   hipLaunchKernelGGL(k_inf_layer3, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ih2.p,
-                     nn.xp.p, nn.th.p, nn.s.p, L.D.p, u);
+                     nn.ix.p, nn.th.p, nn.s.p, L.D.p, u);
 }
 
 dec_nn_s *nn_create(const std::vector<const HostCsr *> &ops, uint64_t seed) {

@@ -451,9 +451,10 @@ struct dec_mg_s {
   int gkey[2] = {-1, -1};
   hipStream_t cap = nullptr;
   bool use_graph = true;
-  // learned smoother (float32 cycles, levels with at least learned_min_rows)
+  // learned smoother (float32 cycles, levels of learned_min_rows to
+  // learned_max_rows rows; learned_max_rows <= 0: no upper bound)
   dec_nn_s *nn = nullptr;
-  int learned = 0, learned_min_rows = 16385, learned_apps = 1;
+  int learned = 0, learned_min_rows = 16385, learned_max_rows = 0, learned_apps = 1;
   bool learned_active = false;
   int fallbacks = 0, learned_cycles = 0;
   ~dec_mg_s() {
@@ -803,15 +804,23 @@ void cg_smooth(dec_mg_s &mg, Level &L, const double *b, double *u, int steps, hi
 }
 
 inline bool learned_on(const dec_mg_s &mg) { return mg.nn && mg.learned && mg.learned_active; }
+inline bool learned_rows(const dec_mg_s &mg, int n) {
+  return n >= mg.learned_min_rows && (mg.learned_max_rows <= 0 || n <= mg.learned_max_rows);
+}
 
+// u_zero: u is known to be zero on entry (the residual is then b itself).
 template <class T>
-void smooth(dec_mg_s &mg, int lvl, Level &L, const T *b, T *u, bool post, hipStream_t s) {
+void smooth(dec_mg_s &mg, int lvl, Level &L, const T *b, T *u, bool post, bool u_zero, hipStream_t s) {
   int steps = mg.steps;
   if (steps == 0) return;
   LevelVals<T> &V = vals<T>(L);
   if constexpr (sizeof(T) == 4) {
-    if (learned_on(mg) && L.n >= mg.learned_min_rows) {
+    if (learned_on(mg) && learned_rows(mg, L.n)) {
       for (int a = 0; a < mg.learned_apps; a++) {
+        if (a == 0 && u_zero) {
+          dec::nn_smooth(*mg.nn, lvl, b, u, s);
+          continue;
+        }
         residual<float>(L, b, u, V.r.p, s);
         dec::nn_smooth(*mg.nn, lvl, V.r.p, u, s);
       }
@@ -867,11 +876,11 @@ void coarse_solve(dec_mg_s &mg, Level &L, const T *b, T *u, hipStream_t s) {
 
 // _multigrid_μ_cycle (Multigrid.jl:857-872).
 template <class T>
-void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, hipStream_t s) {
+void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, bool u_zero, hipStream_t s) {
   int nl = static_cast<int>(mg.lv.size());
   Level &L = *mg.lv[lvl];
   // the fused tail has no learned smoother: a level that uses it runs unfused
-  bool learned_here = sizeof(T) == 4 && learned_on(mg) && L.n >= mg.learned_min_rows;
+  bool learned_here = sizeof(T) == 4 && learned_on(mg) && learned_rows(mg, L.n);
   if (mg.use_fused && mu == 1 && mg.alg != DEC_SMOOTH_CG && mg.fuse_from >= 0 && lvl >= mg.fuse_from &&
       !learned_here && (mg.coarse_direct == 0 || vals<T>(*mg.lv.back()).Z.p)) {
     const LevelDesc<T> *D;
@@ -895,7 +904,7 @@ void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, hipStream_t s) {
     coarse_solve<T>(mg, L, b, u, s);
     return;
   }
-  smooth<T>(mg, lvl, L, b, u, false, s);
+  smooth<T>(mg, lvl, L, b, u, false, u_zero, s);
   if (lvl >= nl - 1) return;
   Level &C = *mg.lv[lvl + 1];
   LevelVals<T> &V = vals<T>(L);
@@ -903,10 +912,10 @@ void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, hipStream_t s) {
   residual<T>(L, b, u, V.r.p, s);
   DEC_ELL_WIDTH_SWITCH(L.rwidth, hipLaunchKernelGGL((k_restrict<W, T>), G(L.nc), dim3(kBlock), 0, s, L.nc, L.rwidth,
                                                     L.rcol.p, V.rval.p, V.r.p, CV.b.p, CV.u.p));
-  mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu, s);
-  if (mu > 1) mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu - 1, s);
+  mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu, true, s);
+  if (mu > 1) mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu - 1, false, s);
   hipLaunchKernelGGL(k_prolong_add<T>, G(L.n), dim3(kBlock), 0, s, L.n, L.nc, L.pa.p, L.pb.p, CV.u.p, u);
-  smooth<T>(mg, lvl, L, b, u, true, s);
+  smooth<T>(mg, lvl, L, b, u, true, false, s);
 }
 
 // _full_multigrid (Multigrid.jl:839-855).
@@ -923,7 +932,7 @@ void fmg_rec(dec_mg_s &mg, int lvl, const double *b, double *z, int mu, int cycl
     hipLaunchKernelGGL(k_prolong_add<double>, G(L.n), dim3(kBlock), 0, s, L.n, L.nc, L.pa.p, L.pb.p, C.d.u.p, z);
   }
   if (lvl > 0)
-    for (int c = 0; c < cycles; c++) mu_cycle<double>(mg, lvl, z, b, mu, s);
+    for (int c = 0; c < cycles; c++) mu_cycle<double>(mg, lvl, z, b, mu, false, s);
 }
 
 double norm2(dec_mg_s &mg, int n, const double *x, hipStream_t s) {
@@ -939,12 +948,12 @@ void cycle_body(dec_mg_s &mg, int variant, double *u, const double *b, hipStream
   Level &F = *mg.lv[0];
   int n = F.n;
   if (variant == DEC_SOLVE_PLAIN_F64) {
-    mu_cycle<double>(mg, 0, u, b, 1, s);
+    mu_cycle<double>(mg, 0, u, b, 1, false, s);
   } else {
     // float32 V-cycle on the float64 residual, correction added in float64
     hipLaunchKernelGGL(k_d2f, G(n), dim3(kBlock), 0, s, n, F.d.r.p, F.f.b.p);
     hipLaunchKernelGGL(k_zero<float>, G(n), dim3(kBlock), 0, s, n, F.f.u.p);
-    mu_cycle<float>(mg, 0, F.f.u.p, F.f.b.p, 1, s);
+    mu_cycle<float>(mg, 0, F.f.u.p, F.f.b.p, 1, true, s);
     hipLaunchKernelGGL(k_add_f2d, G(n), dim3(kBlock), 0, s, n, F.f.u.p, u);
   }
 }
@@ -961,7 +970,8 @@ void run_cycle(dec_mg_s &mg, int variant, double *u, const double *b, hipStream_
     return;
   }
   int key = mg.alg * 1000003 + mg.steps * 1009 + mg.coarse_direct * 7 + static_cast<int>(mg.omega * 1e6) +
-            (mg.learned && mg.learned_active ? 1 << 29 : 0) + mg.learned_apps * 31 + mg.learned_min_rows;
+            (mg.learned && mg.learned_active ? 1 << 29 : 0) + mg.learned_apps * 31 + mg.learned_min_rows +
+            mg.learned_max_rows * 7919;
   int i = variant == DEC_SOLVE_PLAIN_F64 ? 0 : 1;
   if (!mg.gexec[i] || mg.gu[i] != u || mg.gb[i] != b || mg.gkey[i] != key) {
     if (mg.gexec[i]) (void)hipGraphExecDestroy(mg.gexec[i]);
@@ -1002,8 +1012,11 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u, con
   double rr = 0.0, prev = 0.0;
   int c = 0;
   // the network smooths the float32 cycles on the levels of at least
-  // learned_min_rows rows; with no such level the solve is a plain mixed one
-  mg.learned_active = mg.learned && mg.nn && variant == DEC_SOLVE_MIXED_IR && n >= mg.learned_min_rows;
+  // learned_min_rows .. learned_max_rows rows; with no such level the solve
+  // is a plain mixed one
+  bool any = false;
+  for (auto &Lv : mg.lv) any = any || learned_rows(mg, Lv->n);
+  mg.learned_active = mg.learned && mg.nn && variant == DEC_SOLVE_MIXED_IR && any;
   mg.fallbacks = 0;
   mg.learned_cycles = 0;
   for (;;) {
@@ -1108,7 +1121,7 @@ int dec_mg_cycles(dec_mg mg, int32_t mu, int32_t cycles, void *u, const void *b,
   if (mu < 1 || cycles < 0) fail(DEC_ERR_ARG, "dec_mg_cycles: mu >= 1 and cycles >= 0 required");
   hipStream_t s = static_cast<hipStream_t>(stream);
   for (int c = 0; c < cycles; c++)
-    mu_cycle<double>(*mg, 0, static_cast<double *>(u), static_cast<const double *>(b), mu, s);
+    mu_cycle<double>(*mg, 0, static_cast<double *>(u), static_cast<const double *>(b), mu, false, s);
   dec::check_launch("dec_mg_cycles");
   DEC_CATCH
 }
@@ -1121,7 +1134,7 @@ int dec_mg_full_multigrid(dec_mg mg, int32_t mu, int32_t cycles, const void *b, 
   const double *bb = static_cast<const double *>(b);
   double *uu = static_cast<double *>(u);
   fmg_rec(*mg, 0, bb, uu, mu, cycles, s);
-  for (int c = 0; c < cycles; c++) mu_cycle<double>(*mg, 0, uu, bb, mu, s);
+  for (int c = 0; c < cycles; c++) mu_cycle<double>(*mg, 0, uu, bb, mu, false, s);
   dec::check_launch("dec_mg_full_multigrid");
   DEC_CATCH
 }
@@ -1157,7 +1170,8 @@ int dec_nn_create(dec_mg mg, uint64_t seed, dec_nn *out) {
   DEC_CATCH
 }
 
-int dec_mg_set_learned(dec_mg mg, dec_nn nn, int32_t enable, int32_t min_rows, int32_t applications) {
+int dec_mg_set_learned(dec_mg mg, dec_nn nn, int32_t enable, int32_t min_rows, int32_t max_rows,
+                       int32_t applications) {
   DEC_TRY
   if (!mg) fail(DEC_ERR_ARG, "dec_mg_set_learned: NULL handle");
   if (enable && !nn) fail(DEC_ERR_ARG, "dec_mg_set_learned: no network");
@@ -1172,6 +1186,7 @@ int dec_mg_set_learned(dec_mg mg, dec_nn nn, int32_t enable, int32_t min_rows, i
   mg->nn = nn;
   mg->learned = enable != 0;
   mg->learned_min_rows = min_rows;
+  mg->learned_max_rows = max_rows;
   mg->learned_apps = applications;
   DEC_CATCH
 }

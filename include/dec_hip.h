@@ -213,11 +213,14 @@ int dec_nn_train(dec_nn nn, int32_t level, int32_t npairs, const float *R, int32
                  int32_t batch, float lr, uint64_t seed, double *loss_hist, void *stream);
 int dec_nn_destroy(dec_nn nn);
 /* Use the network as the smoother of the float32 V-cycles (DEC_SOLVE_MIXED_IR)
- * on the levels with at least min_rows vertices (Gauss-Seidel below).  During
- * dec_mg_solve, a cycle whose residual drops by less than 0.5x switches the
- * rest of the solve back to Gauss-Seidel; dec_mg_learned_stats reports those
- * fallbacks and the cycles run with the network. */
-int dec_mg_set_learned(dec_mg mg, dec_nn nn, int32_t enable, int32_t min_rows, int32_t applications);
+ * on the levels of min_rows to max_rows vertices (max_rows <= 0: no upper
+ * bound; Gauss-Seidel elsewhere), `applications` times before and after the
+ * coarse correction.  During dec_mg_solve, a cycle whose residual drops by
+ * less than 0.5x switches the rest of the solve back to Gauss-Seidel;
+ * dec_mg_learned_stats reports those fallbacks and the cycles run with the
+ * network.  The network must come from dec_nn_create on this hierarchy. */
+int dec_mg_set_learned(dec_mg mg, dec_nn nn, int32_t enable, int32_t min_rows, int32_t max_rows,
+                       int32_t applications);
 int dec_mg_learned_stats(dec_mg mg, int32_t *fallbacks, int32_t *learned_cycles);
 
 /* ------------------------------------------------------------------ batches */
