@@ -233,23 +233,27 @@ def learned(torch, cs, md, fd, npairs=10000, train_levels=3, steps=2000, batch=1
     t = time.perf_counter()
     mdt = cs.MultigridData(cs.icosphere(3), cs.BinarySubdivision, train_levels, cs.DiagonalHodge, 2, project=True)
     R, E = cs.make_pairs(mdt, npairs, cycles_per_rhs=6, seed=11)
-    del E
     torch.cuda.synchronize()
     t_pairs = time.perf_counter() - t
     nn_t = cs.LearnedSmoother(mdt, seed=5)
-    val = R[-512:].contiguous()
-    l0, _ = nn_t.loss_grad(0, val)
+    val, evl = R[-512:].contiguous(), E[-512:].contiguous()
+    l0, _ = nn_t.loss_grad(0, val, evl)
+    r0, _ = nn_t.loss_grad(0, val)
     t = time.perf_counter()
-    hist = nn_t.train(0, R[:-512], steps=steps, batch=batch, lr=2e-3, seed=6)
+    # error loss: the high-frequency part of the error left after smoothing
+    hist = nn_t.train(0, R[:-512], steps=steps, E=E[:-512], batch=batch, lr=2e-3, seed=6)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t
-    l1, _ = nn_t.loss_grad(0, val)
+    l1, _ = nn_t.loss_grad(0, val, evl)
+    r1, _ = nn_t.loss_grad(0, val)
     params = nn_t.get_params()
-    del R, val
+    del R, E, val, evl
     nn = cs.LearnedSmoother(md, seed=5)
     nn.set_params(params)
     out = {"pairs": npairs, "train_mesh_vertices": mdt.sizes[0], "pairs_s": t_pairs, "train_steps": steps,
-           "train_batch": batch, "train_s": t_train, "val_loss_untrained": l0, "val_loss_trained": l1,
+           "train_batch": batch, "train_s": t_train, "loss": "error (high-pass)",
+           "val_error_loss_untrained": l0, "val_error_loss_trained": l1,
+           "val_residual_loss_untrained": r0, "val_residual_loss_trained": r1,
            "loss_first": float(hist[0]), "loss_last": float(hist[-1])}
     # one smoothing application at the finest level with the inference kernels
     # (BF16 MFMA dense layers): time per application, and the dense layers'

@@ -62,8 +62,8 @@ SIGNATURES = {
     "dec_nn_set_params": (c_int, [P, P]),
     "dec_nn_forward": (c_int, [P, c_int32, c_int32, P, P, c_int32, P]),
     "dec_nn_smooth": (c_int, [P, c_int32, P, P, P]),
-    "dec_nn_loss_grad": (c_int, [P, c_int32, c_int32, P, c_int32, PF64, P, P]),
-    "dec_nn_train": (c_int, [P, c_int32, c_int32, P, c_int32, c_int32, ctypes.c_float, c_uint64, P, P]),
+    "dec_nn_loss_grad": (c_int, [P, c_int32, c_int32, P, P, c_int32, PF64, P, P]),
+    "dec_nn_train": (c_int, [P, c_int32, c_int32, P, P, c_int32, c_int32, ctypes.c_float, c_uint64, P, P]),
     "dec_nn_destroy": (c_int, [P]),
     "dec_mg_set_learned": (c_int, [P, P, c_int32, c_int32, c_int32, c_int32]),
     "dec_mg_learned_stats": (c_int, [P, PI32, PI32]),

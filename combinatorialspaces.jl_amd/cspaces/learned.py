@@ -81,19 +81,28 @@ class LearnedSmoother:
         check(lib().dec_nn_smooth(self._h, level, c_void_p(r.data_ptr()), c_void_p(u.data_ptr()), _stream(stream)))
         return u
 
-    def loss_grad(self, level, R, precise=False, stream=None):
+    def _E(self, E, R):
+        if E is None:
+            return None
+        if not (E.is_cuda and E.dtype == R.dtype and E.is_contiguous() and E.shape == R.shape):
+            raise TypeError("errors must be a contiguous float32 GPU tensor shaped like the residuals")
+        return c_void_p(E.data_ptr())
+
+    def loss_grad(self, level, R, E=None, precise=False, stream=None):
+        """Loss and gradient on the residuals R (residual loss) or on the
+        (residual, error) pairs (R, E) (error loss, high-pass filtered)."""
         B = self._R(R, level)
         loss = c_double()
         g = np.zeros(self.nparams, np.float32)
-        check(lib().dec_nn_loss_grad(self._h, level, B, c_void_p(R.data_ptr()), int(bool(precise)),
+        check(lib().dec_nn_loss_grad(self._h, level, B, c_void_p(R.data_ptr()), self._E(E, R), int(bool(precise)),
                                      ctypes.byref(loss), ptr(g), _stream(stream)))
         return loss.value, g
 
-    def train(self, level, R, steps, batch=16, lr=1e-3, seed=0, stream=None):
+    def train(self, level, R, steps, E=None, batch=16, lr=1e-3, seed=0, stream=None):
         npairs = self._R(R, level)
         hist = np.zeros(max(steps, 1))
-        check(lib().dec_nn_train(self._h, level, npairs, c_void_p(R.data_ptr()), steps, batch, float(lr), seed,
-                                 ptr(hist), _stream(stream)))
+        check(lib().dec_nn_train(self._h, level, npairs, c_void_p(R.data_ptr()), self._E(E, R), steps, batch,
+                                 float(lr), seed, ptr(hist), _stream(stream)))
         return hist[:steps]
 
     def attach(self, enable=True, min_rows=16385, max_rows=0, applications=1):

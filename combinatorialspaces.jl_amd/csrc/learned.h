@@ -7,16 +7,18 @@
 namespace dec {
 
 // A level's operator A, its transpose, P = D⁻¹(D - A) and Pᵀ as float ELL,
-// and D = diag(A), all with rows in vertex order.
+// and D = diag(A), all with rows in vertex order; for the error loss, the
+// restriction R to the next coarser level (nc rows), the prolongation Pr
+// back (n rows) and their transposes (nc = 0 on the coarsest level).
 struct NnLevel {
-  int n = 0;
+  int n = 0, nc = 0;
   struct Op {
     int n = 0, width = 0;
     DBuf<int32_t> col;
     DBuf<float> val;
-  } a, at, p, pt;
+  } a, at, p, pt, r, rt, pr, prt;
   DBuf<float> D;
-  explicit NnLevel(const HostCsr &A);
+  NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr);
 };
 
 // u += N(r) for one residual r (both float, vertex order) on level `lvl` of
@@ -25,6 +27,10 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s);
 // Check that the network was built for a hierarchy with these level sizes
 // (finest first) and size its inference buffers for the largest level.
 void nn_bind(dec_nn_s &nn, const std::vector<int> &sizes);
-dec_nn_s *nn_create(const std::vector<const HostCsr *> &ops, uint64_t seed);
+// ops[l] = (A_l, R_l, P_l), R and P NULL on the coarsest level.
+struct NnOps {
+  const HostCsr *A, *R, *P;
+};
+dec_nn_s *nn_create(const std::vector<NnOps> &ops, uint64_t seed);
 
 }  // namespace dec

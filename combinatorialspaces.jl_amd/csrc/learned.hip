@@ -523,13 +523,73 @@ __global__ void k_loss(int n, int B, int aw, const int *__restrict__ acol, const
   (void)t;
 }
 
-// dy = -(Aᵀ dρ) / D; dz3 = (dy wo) ⊙ [h3 > 0]; block partials of
+// ---- error loss: mean (H ε)², ε = e / s - y / D, H = I - Pr R ---------------
+
+// y[b][i] = Σ_k val[k][i] x[b][col[k][i]] for a batch of vectors (rows ascending ELL)
+__global__ void k_bspmv(int nrows, int ncols, int width, const int *__restrict__ col, const float *__restrict__ val,
+                        const float *__restrict__ x, float *__restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (i >= nrows) return;
+  const float *xb = x + static_cast<size_t>(b) * ncols;
+  float acc = 0.0f;
+  for (int k = 0; k < width; k++) {
+    size_t q = static_cast<size_t>(k) * nrows + i;
+    int c = col[q];
+    if (c < 0) break;
+    acc = acc + val[q] * xb[c];
+  }
+  y[static_cast<size_t>(b) * nrows + i] = acc;
+}
+
+__global__ void k_err(int n, const float *__restrict__ e, const float *__restrict__ s, const float *__restrict__ y,
+                      const float *__restrict__ D, float *__restrict__ eps) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (v >= n) return;
+  size_t o = static_cast<size_t>(b) * n + v;
+  eps[o] = e[o] / s[b] - y[o] / D[v];
+}
+
+// eh = eps - t (t = Pr R eps) in place of eps; block partials of Σ eh²
+__global__ void k_hp(int n, const float *__restrict__ t, float *__restrict__ eps, double *__restrict__ part) {
+  __shared__ double sh[TB];
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  double sq = 0.0;
+  if (v < n) {
+    size_t o = static_cast<size_t>(b) * n + v;
+    float eh = eps[o] - t[o];
+    eps[o] = eh;
+    sq = static_cast<double>(eh) * eh;
+  }
+  sh[threadIdx.x] = sq;
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// g = scale (eh - t), t = Rᵀ Prᵀ eh: the gradient Hᵀ H ε scaled by 2 / (n B)
+__global__ void k_hp_grad(int n, float scale, const float *__restrict__ eh, const float *__restrict__ t,
+                          float *__restrict__ g) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (v >= n) return;
+  size_t o = static_cast<size_t>(b) * n + v;
+  g[o] = scale * (eh[o] - t[o]);
+}
+
+// dy = -(Aᵀ dρ) / D (residual loss) or -g / D for a given g (error loss);
+// dz3 = (dy wo) ⊙ [h3 > 0]; block partials of
 // (Σ dy x, Σ dy, Σ dy h3) for alpha, bo, wo.
 template <class AT>
 __global__ void k_out_back(int n, int atw, const int *__restrict__ atcol, const float *__restrict__ atval,
                            const float *__restrict__ drho, const float *__restrict__ D, const float *__restrict__ x,
                            const AT *__restrict__ h3, const float *__restrict__ th, AT *__restrict__ dz3,
-                           float *__restrict__ part) {
+                           float *__restrict__ part, int direct) {
   __shared__ float sh[TB][2 * H + 2];
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y;
@@ -538,11 +598,15 @@ __global__ void k_out_back(int n, int atw, const int *__restrict__ atcol, const 
   float hv[H];
   if (v < n) {
     float acc = 0.0f;
-    for (int k = 0; k < atw; k++) {
-      size_t q = static_cast<size_t>(k) * n + v;
-      int c = atcol[q];
-      if (c < 0) break;
-      acc = acc + atval[q] * drho[o + c];
+    if (direct) {
+      acc = drho[o + v];  // the loss gradient with respect to y / D is given
+    } else {
+      for (int k = 0; k < atw; k++) {
+        size_t q = static_cast<size_t>(k) * n + v;
+        int c = atcol[q];
+        if (c < 0) break;
+        acc = acc + atval[q] * drho[o + c];
+      }
     }
     dy = -acc / D[v];
     xv = x[o + v];
@@ -820,7 +884,7 @@ static HostCsr transpose(const HostCsr &A) {
   return T;
 }
 
-NnLevel::NnLevel(const HostCsr &A) {
+NnLevel::NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr) {
   n = A.m;
   HostCsr P;
   P.m = P.n = n;
@@ -843,6 +907,13 @@ NnLevel::NnLevel(const HostCsr &A) {
   to_ell(transpose(A), at);
   to_ell(P, p);
   to_ell(transpose(P), pt);
+  if (R && Pr) {
+    nc = R->m;
+    to_ell(*R, r);
+    to_ell(transpose(*R), rt);
+    to_ell(*Pr, pr);
+    to_ell(transpose(*Pr), prt);
+  }
 }
 
 }  // namespace dec
@@ -852,7 +923,7 @@ struct dec_nn_s {
   DBuf<float> th, grad, m1, m2;
   int adam_t = 0;
   // work buffers (grown on demand)
-  DBuf<float> x, px, y, c, s, drho, dg, part, r;
+  DBuf<float> x, px, y, c, s, drho, dg, part, r, eps, tmp, crs;
   DBuf<double> dpart, dloss, spart, rtmp;
   DBuf<float2> ix;                                 // inference: (x, P x)
   DBuf<unsigned short> ih1, ih2;                   // inference: layer-1 and layer-2 rows
@@ -872,6 +943,9 @@ void ensure(dec_nn_s &nn, size_t rows, int precise) {
   nn.y.alloc(rows);
   nn.c.alloc(rows);
   nn.drho.alloc(rows);
+  nn.eps.alloc(rows);
+  nn.tmp.alloc(rows);
+  nn.crs.alloc(rows);
   nn.dg.alloc(rows * 2 * H);
   for (auto *b : {&nn.h1, &nn.hm1, &nn.h2, &nn.hm2, &nn.h3, &nn.dz}) b->alloc(rows * H * as);
   size_t parts = (rows + 255) / 256 + 64;
@@ -948,7 +1022,9 @@ void forward_t(dec_nn_s &nn, const dec::NnLevel &L, int B, const float *r, float
 // Loss (into *loss_out, device) and gradient (into nn.grad) of a batch of B
 // residuals stacked in r.
 template <class AT>
-void loss_grad_t(dec_nn_s &nn, const dec::NnLevel &L, int B, const float *r, double *loss_out, hipStream_t st) {
+// e: the B errors paired with the residuals (error loss), or NULL (residual loss).
+void loss_grad_t(dec_nn_s &nn, const dec::NnLevel &L, int B, const float *r, const float *e, double *loss_out,
+                 hipStream_t st) {
   forward_t<AT>(nn, L, B, r, nullptr, nullptr, st);
   int n = L.n;
   int R = n * B;
@@ -957,12 +1033,27 @@ void loss_grad_t(dec_nn_s &nn, const dec::NnLevel &L, int B, const float *r, dou
   float *g = nn.grad.p;
   dim3 gv(dec::nblocks(n, TB), B), gf(dec::nblocks(static_cast<long long>(n) * H, TB), B);
   int nbv = gv.x * B, nbf = gf.x * B;
-  hipLaunchKernelGGL(k_loss, gv, dim3(TB), 0, st, n, B, L.a.width, L.a.col.p, L.a.val.p, nn.x.p, nn.y.p, L.D.p,
-                     nullptr, nn.drho.p, nn.dpart.p);
+  if (!e) {
+    hipLaunchKernelGGL(k_loss, gv, dim3(TB), 0, st, n, B, L.a.width, L.a.col.p, L.a.val.p, nn.x.p, nn.y.p, L.D.p,
+                       nullptr, nn.drho.p, nn.dpart.p);
+  } else {
+    if (L.nc == 0) fail(DEC_ERR_ARG, "learned smoother: the error loss needs a coarser level below this one");
+    dim3 gc(dec::nblocks(L.nc, TB), B);
+    hipLaunchKernelGGL(k_err, gv, dim3(TB), 0, st, n, e, nn.s.p, nn.y.p, L.D.p, nn.eps.p);
+    hipLaunchKernelGGL(k_bspmv, gc, dim3(TB), 0, st, L.nc, n, L.r.width, L.r.col.p, L.r.val.p, nn.eps.p, nn.crs.p);
+    hipLaunchKernelGGL(k_bspmv, gv, dim3(TB), 0, st, n, L.nc, L.pr.width, L.pr.col.p, L.pr.val.p, nn.crs.p,
+                       nn.tmp.p);
+    hipLaunchKernelGGL(k_hp, gv, dim3(TB), 0, st, n, nn.tmp.p, nn.eps.p, nn.dpart.p);
+    hipLaunchKernelGGL(k_bspmv, gc, dim3(TB), 0, st, L.nc, n, L.prt.width, L.prt.col.p, L.prt.val.p, nn.eps.p,
+                       nn.crs.p);
+    hipLaunchKernelGGL(k_bspmv, gv, dim3(TB), 0, st, n, L.nc, L.rt.width, L.rt.col.p, L.rt.val.p, nn.crs.p, nn.tmp.p);
+    hipLaunchKernelGGL(k_hp_grad, gv, dim3(TB), 0, st, n, 2.0f / (static_cast<float>(n) * B), nn.eps.p, nn.tmp.p,
+                       nn.drho.p);
+  }
   reduce_parts(nn, nn.dpart.p, nbv, 1, 0, 1, 1.0 / (static_cast<double>(n) * B), loss_out, st);
   // output head: dz3, and the (wo, bo, alpha, b3) partials
   hipLaunchKernelGGL(k_out_back<AT>, gv, dim3(TB), 0, st, n, L.at.width, L.at.col.p, L.at.val.p, nn.drho.p, L.D.p,
-                     nn.x.p, A.h3, th, A.dz, nn.part.p);
+                     nn.x.p, A.h3, th, A.dz, nn.part.p, e ? 1 : 0);
   reduce_parts(nn, nn.part.p, nbv, 2 * H + 2, 0, H + 2, 1.0, g + OWO, st);
   reduce_parts(nn, nn.part.p, nbv, 2 * H + 2, H + 2, H, 1.0, g + OB3, st);
   // dense layers 3 and 2: weight gradient, input gradient, mix through Pᵀ
@@ -1041,9 +1132,9 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s) {
                      nn.ix.p, nn.th.p, nn.s.p, L.D.p, u);
 }
 
-dec_nn_s *nn_create(const std::vector<const HostCsr *> &ops, uint64_t seed) {
+dec_nn_s *nn_create(const std::vector<NnOps> &ops, uint64_t seed) {
   auto nn = std::make_unique<dec_nn_s>();
-  for (const HostCsr *A : ops) nn->levels.push_back(std::make_unique<NnLevel>(*A));
+  for (const NnOps &o : ops) nn->levels.push_back(std::make_unique<NnLevel>(*o.A, o.R, o.P));
   // initialisation: W ~ N(0, 2/fan_in) (seeded), biases and output head zero,
   // alpha = 2/3: the untrained network is damped Jacobi
   std::vector<float> th(NP, 0.0f);
@@ -1111,14 +1202,14 @@ int dec_nn_smooth(dec_nn nn, int32_t level, const float *r, float *u, void *stre
   DEC_CATCH
 }
 
-int dec_nn_loss_grad(dec_nn nn, int32_t level, int32_t nsamples, const float *r, int32_t precise, double *loss,
-                     float *grad, void *stream) {
+int dec_nn_loss_grad(dec_nn nn, int32_t level, int32_t nsamples, const float *r, const float *e, int32_t precise,
+                     double *loss, float *grad, void *stream) {
   DEC_TRY
   if (!nn || !r || nsamples <= 0) fail(DEC_ERR_ARG, "dec_nn_loss_grad: bad argument");
   const dec::NnLevel &L = level_of(*nn, level);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (precise) loss_grad_t<float>(*nn, L, nsamples, r, nn->dloss.p, s);
-  else loss_grad_t<unsigned short>(*nn, L, nsamples, r, nn->dloss.p, s);
+  if (precise) loss_grad_t<float>(*nn, L, nsamples, r, e, nn->dloss.p, s);
+  else loss_grad_t<unsigned short>(*nn, L, nsamples, r, e, nn->dloss.p, s);
   dec::check_launch("dec_nn_loss_grad");
   lcheck(hipStreamSynchronize(s), "hipStreamSynchronize");
   if (loss) dec::d2h(loss, nn->dloss.p, sizeof(double));
@@ -1126,8 +1217,8 @@ int dec_nn_loss_grad(dec_nn nn, int32_t level, int32_t nsamples, const float *r,
   DEC_CATCH
 }
 
-int dec_nn_train(dec_nn nn, int32_t level, int32_t npairs, const float *R, int32_t steps, int32_t batch, float lr,
-                 uint64_t seed, double *loss_hist, void *stream) {
+int dec_nn_train(dec_nn nn, int32_t level, int32_t npairs, const float *R, const float *E, int32_t steps,
+                 int32_t batch, float lr, uint64_t seed, double *loss_hist, void *stream) {
   DEC_TRY
   if (!nn || !R || npairs <= 0 || steps < 0 || batch <= 0) fail(DEC_ERR_ARG, "dec_nn_train: bad argument");
   const dec::NnLevel &L = level_of(*nn, level);
@@ -1139,15 +1230,19 @@ int dec_nn_train(dec_nn nn, int32_t level, int32_t npairs, const float *R, int32
   for (auto &i : ids) i = pick(rng);
   DBuf<int32_t> dids;
   dids.upload(ids);
-  DBuf<float> rb;
+  DBuf<float> rb, eb;
   rb.alloc(static_cast<size_t>(n) * batch);
+  if (E) eb.alloc(static_cast<size_t>(n) * batch);
   DBuf<double> hist;
   hist.alloc(std::max(steps, 1));
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   for (int k = 0; k < steps; k++) {
     hipLaunchKernelGGL(k_gather_rows, dim3(dec::nblocks(n, TB), batch), dim3(TB), 0, s, n, R,
                        dids.p + static_cast<size_t>(k) * batch, rb.p);
-    loss_grad_t<unsigned short>(*nn, L, batch, rb.p, hist.p + k, s);
+    if (E)
+      hipLaunchKernelGGL(k_gather_rows, dim3(dec::nblocks(n, TB), batch), dim3(TB), 0, s, n, E,
+                         dids.p + static_cast<size_t>(k) * batch, eb.p);
+    loss_grad_t<unsigned short>(*nn, L, batch, rb.p, E ? eb.p : nullptr, hist.p + k, s);
     nn->adam_t++;
     float c1 = 1.0f - std::pow(b1, static_cast<float>(nn->adam_t));
     float c2 = 1.0f - std::pow(b2, static_cast<float>(nn->adam_t));

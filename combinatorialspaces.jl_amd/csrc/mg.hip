@@ -1164,8 +1164,13 @@ int dec_mg_solve(dec_mg mg, int32_t variant, double tol, int32_t max_cycles, voi
 int dec_nn_create(dec_mg mg, uint64_t seed, dec_nn *out) {
   DEC_TRY
   if (!mg || !out) fail(DEC_ERR_ARG, "dec_nn_create: NULL argument");
-  std::vector<const dec::HostCsr *> ops;
-  for (auto &L : mg->lv) ops.push_back(&L->A);
+  std::vector<dec::NnOps> ops;
+  int nl = static_cast<int>(mg->lv.size());
+  for (int l = 0; l < nl; l++) {
+    Level &L = *mg->lv[l];
+    bool has = l + 1 < nl;
+    ops.push_back({&L.A, has ? &L.R : nullptr, has ? &L.P : nullptr});
+  }
   *out = dec::nn_create(ops, seed);
   DEC_CATCH
 }

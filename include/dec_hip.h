@@ -204,13 +204,20 @@ int dec_nn_forward(dec_nn nn, int32_t level, int32_t nsamples, const float *r, f
  * layers, hidden layers 1 and 3 kept in registers.  Bit-identical to
  * dec_nn_forward (precise = 0) followed by u += c. */
 int dec_nn_smooth(dec_nn nn, int32_t level, const float *r, float *u, void *stream);
-/* Loss mean((r - A c)² / s²) over the samples and its gradient (host copies). */
-int dec_nn_loss_grad(dec_nn nn, int32_t level, int32_t nsamples, const float *r,
+/* Loss and its gradient (host copies).  e == NULL: the residual loss
+ * mean((r - A c)² / s²); e given (the errors A⁻¹r paired with r, same
+ * layout): the error loss mean((H (e - c))² / s²), H = I - Pr R with R the
+ * restriction to the next coarser level and Pr the prolongation back (the
+ * error left that the coarse grid cannot represent; not on the coarsest
+ * level). */
+int dec_nn_loss_grad(dec_nn nn, int32_t level, int32_t nsamples, const float *r, const float *e,
                      int32_t precise, double *loss, float *grad, void *stream);
 /* Adam on minibatches drawn (seeded) from the npairs residuals in R
- * (npairs x n, device); loss_hist (host, steps entries) may be NULL. */
-int dec_nn_train(dec_nn nn, int32_t level, int32_t npairs, const float *R, int32_t steps,
-                 int32_t batch, float lr, uint64_t seed, double *loss_hist, void *stream);
+ * (npairs x n, device) and, if E is given, their errors (error loss, else
+ * residual loss); loss_hist (host, steps entries) may be NULL. */
+int dec_nn_train(dec_nn nn, int32_t level, int32_t npairs, const float *R, const float *E,
+                 int32_t steps, int32_t batch, float lr, uint64_t seed, double *loss_hist,
+                 void *stream);
 int dec_nn_destroy(dec_nn nn);
 /* Use the network as the smoother of the float32 V-cycles (DEC_SOLVE_MIXED_IR)
  * on the levels of min_rows to max_rows vertices (max_rows <= 0: no upper

@@ -16,8 +16,14 @@ a correction c:
     y   = alpha x + h3 wo + bo               (N)
     c   = (s / D) y
 
-Training loss over a batch of residuals: mean over samples and vertices of
-((r - A c) / s)², the residual left after the correction.
+Training losses over a batch:
+  residual loss: mean over samples and vertices of ((r - A c) / s)², the
+    residual left after the correction;
+  error loss (residual -> error pairs (r, e), e the error A⁻¹r of the
+    iterate): mean of (H (e - c) / s)² with the high-pass H = I - Pr R, R the
+    restriction to the next coarser level and Pr the prolongation back: the
+    part of the remaining error that the coarse grid cannot represent, which
+    is what a multigrid smoother has to remove.
 
 Gradients are pinned by central finite differences in the tests; nothing else
 fixes them (parity of the network is against this restatement only).
@@ -65,10 +71,28 @@ def init_params(seed=0, alpha=2 / 3):
     return pack(p)
 
 
-class Level:
-    """A level's operator in CSR (rowptr, col, val) with its P = D⁻¹(D - A)."""
+class Csr:
+    """A rectangular sparse matrix (rowptr, col, val) with products."""
 
-    def __init__(self, rowptr, col, val):
+    def __init__(self, rowptr, col, val, ncols):
+        self.rp, self.ci, self.vv = np.asarray(rowptr), np.asarray(col), np.asarray(val, np.float64)
+        self.m, self.n = len(self.rp) - 1, ncols
+        self.rows = np.repeat(np.arange(self.m), np.diff(self.rp))
+
+    def __matmul__(self, x):
+        return np.bincount(self.rows, self.vv * x[self.ci], minlength=self.m)
+
+    def T(self, y):
+        return np.bincount(self.ci, self.vv * y[self.rows], minlength=self.n)
+
+
+class Level:
+    """A level's operator in CSR (rowptr, col, val) with its P = D⁻¹(D - A);
+    R (restriction to the next coarser level) and Pr (prolongation from it)
+    as Csr for the error loss."""
+
+    def __init__(self, rowptr, col, val, R=None, Pr=None):
+        self.R, self.Pr = R, Pr
         self.rp, self.ci, self.vv = (np.asarray(rowptr), np.asarray(col), np.asarray(val, np.float64))
         n = len(self.rp) - 1
         self.n = n
@@ -172,11 +196,64 @@ def loss_grad(theta, lv: Level, R):
     return tot / B, g
 
 
-def fd_grad(theta, lv, R, idx, h=1e-6):
+def backward(theta, lv: Level, cache, dy):
+    """Parameter gradient from dloss/dy of one sample."""
+    p, k = cache["p"], cache
+    x = k["x"]
+    d = {name: np.zeros(shp) for name, shp in SHAPES}
+    d["alpha"][0] = dy @ x
+    d["bo"][0] = dy.sum()
+    d["wo"] = k["h3"].T @ dy
+    dh3 = np.outer(dy, p["wo"])
+    dz3 = dh3 * (k["h3"] > 0)
+    d["W3"] = k["g2"].T @ dz3
+    d["b3"] = dz3.sum(axis=0)
+    dg2 = dz3 @ p["W3"].T
+    dh2 = dg2[:, :HIDDEN] + lv.PT(dg2[:, HIDDEN:])
+    dz2 = dh2 * (k["h2"] > 0)
+    d["W2"] = k["g1"].T @ dz2
+    d["b2"] = dz2.sum(axis=0)
+    dg1 = dz2 @ p["W2"].T
+    dh1 = dg1[:, :HIDDEN] + lv.PT(dg1[:, HIDDEN:])
+    dz1 = dh1 * (k["h1"] > 0)
+    d["W1"] = k["f0"].T @ dz1
+    d["b1"] = dz1.sum(axis=0)
+    return pack(d)
+
+
+def high_pass(lv: Level, v):
+    return v - lv.Pr @ (lv.R @ v)
+
+
+def loss_err(theta, lv: Level, R, E):
+    """Mean over samples and vertices of (H (e - c) / s)²."""
+    tot = 0.0
+    for r, e in zip(R, E):
+        c, k = forward(theta, lv, r)
+        eh = high_pass(lv, (e - c) / k["s"])
+        tot += np.mean(eh * eh)
+    return tot / len(R)
+
+
+def loss_err_grad(theta, lv: Level, R, E):
+    g = np.zeros_like(theta)
+    tot, B, N = 0.0, len(R), lv.n
+    for r, e in zip(R, E):
+        c, k = forward(theta, lv, r)
+        eps = (e - c) / k["s"]          # = e / s - y / D
+        eh = high_pass(lv, eps)
+        tot += np.mean(eh * eh)
+        geps = 2.0 * (eh - lv.R.T(lv.Pr.T(eh))) / (N * B)   # Hᵀ H eps
+        g += backward(theta, lv, k, -geps / lv.D)
+    return tot / B, g
+
+
+def fd_grad(theta, lv, R, idx, h=1e-6, E=None):
     out = []
+    f = (lambda t: loss(t, lv, R)) if E is None else (lambda t: loss_err(t, lv, R, E))
     for i in idx:
         tp, tm = theta.copy(), theta.copy()
         tp[i] += h
         tm[i] -= h
-        out.append((loss(tp, lv, R) - loss(tm, lv, R)) / (2 * h))
+        out.append((f(tp) - f(tm)) / (2 * h))
     return np.array(out)

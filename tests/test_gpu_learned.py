@@ -13,7 +13,8 @@ def small(cs):
 
     md = cs.MultigridData(cs.icosphere(0), cs.BinarySubdivision, 3, cs.DiagonalHodge, 2, project=True)
     nn = cs.LearnedSmoother(md, seed=3)
-    lv = LO.Level(*md.export(0, "A"))
+    n, nc = md.sizes[0], md.sizes[1]
+    lv = LO.Level(*md.export(0, "A"), R=LO.Csr(*md.export(0, "R"), ncols=n), Pr=LO.Csr(*md.export(0, "P"), ncols=nc))
     return md, nn, lv
 
 
@@ -60,8 +61,9 @@ def test_forward_matches_oracle(O, small, precise):
         assert err <= (1e-5 if precise else 3e-2), err
 
 
+@pytest.mark.parametrize("kind", ["residual", "error"])
 @pytest.mark.parametrize("precise", [True, False])
-def test_gradient_matches_oracle(O, small, precise):
+def test_gradient_matches_oracle(O, small, precise, kind):
     import torch
 
     import learned as LO
@@ -70,8 +72,15 @@ def test_gradient_matches_oracle(O, small, precise):
     p = random_params(nn, 2)
     nn.set_params(p)
     R = residuals(O, lv.n, 4, 20)
-    loss, g = nn.loss_grad(0, torch.tensor(R, device="cuda"), precise=precise)
-    lref, gref = LO.loss_grad(p.astype(np.float64), lv, R.astype(np.float64))
+    if kind == "residual":
+        loss, g = nn.loss_grad(0, torch.tensor(R, device="cuda"), precise=precise)
+        lref, gref = LO.loss_grad(p.astype(np.float64), lv, R.astype(np.float64))
+    else:
+        E = residuals(O, lv.n, 4, 40)
+        loss, g = nn.loss_grad(0, torch.tensor(R, device="cuda"), torch.tensor(E, device="cuda"), precise=precise)
+        lref, gref = LO.loss_err_grad(p.astype(np.float64), lv, R.astype(np.float64), E.astype(np.float64))
+    # FP32 path: float32 rounding; BF16 path: 8-bit mantissa activations and
+    # weights in the two dense layers (errors stated here)
     assert loss == pytest.approx(lref, rel=1e-5 if precise else 3e-2)
     assert rel(g, gref) <= (1e-4 if precise else 0.1), rel(g, gref)
 
@@ -95,19 +104,21 @@ def test_inference_kernels_match_forward(O, cs, base, levels):
         assert torch.equal(u, u0 + c), (lv, n, (u - (u0 + c)).abs().max().item())
 
 
-def test_training_gain_and_solve(O, cs):
-    """Training lowers the residual-reduction loss on held-out V-cycle pairs
-    compared with the untrained network (damped Jacobi), and the learned
-    float32 V-cycle solves to 1e-8 (float64-verified)."""
+@pytest.mark.parametrize("kind", ["residual", "error"])
+def test_training_gain_and_solve(O, cs, kind):
+    """Training lowers the loss on held-out V-cycle pairs compared with the
+    untrained network (damped Jacobi), and the learned float32 V-cycle solves
+    to 1e-8 (float64-verified)."""
     import torch
 
     md = cs.MultigridData(cs.icosphere(1), cs.BinarySubdivision, 3, cs.DiagonalHodge, 2, project=True)  # k=4
     R, E = cs.make_pairs(md, 480, cycles_per_rhs=6, seed=4)
     train, val = R[:400].contiguous(), R[400:].contiguous()
+    etrain, evl = (E[:400].contiguous(), E[400:].contiguous()) if kind == "error" else (None, None)
     nn = cs.LearnedSmoother(md, seed=5)
-    l0, _ = nn.loss_grad(0, val)
-    hist = nn.train(0, train, steps=400, batch=16, lr=2e-3, seed=6)
-    l1, _ = nn.loss_grad(0, val)
+    l0, _ = nn.loss_grad(0, val, evl)
+    hist = nn.train(0, train, steps=400, E=etrain, batch=16, lr=2e-3, seed=6)
+    l1, _ = nn.loss_grad(0, val, evl)
     assert np.all(np.isfinite(hist)) and l1 < 0.9 * l0, (l0, l1)
     # solve with the network on the finest level
     nn.attach(True, min_rows=0)
