@@ -139,11 +139,14 @@ def sh_basis(points, lmax=12):
     return np.stack(cols, axis=1), lm
 
 
-def make_pairs(md: MultigridData, npairs, cycles_per_rhs=8, nterms=16, seed=0):
+def make_pairs(md: MultigridData, npairs, cycles_per_rhs=8, nterms=16, seed=0, random_start=True):
     """(residual, error) pairs from plain float64 V(2,2) Gauss-Seidel cycles on
     seeded spherical-harmonic right-hand sides (16 terms, l ≤ 12, ⋆0-weighted
-    mean zero) on the finest level of md.  Returns float32 (R, E) of shape
-    (npairs, n) on the GPU."""
+    mean zero) on the finest level of md, one pair per iterate.  The cycles
+    start from a seeded uniform random iterate scaled to the solution's RMS
+    (random_start) or from zero; a random start gives the early iterates
+    errors of every frequency, as a smoother meets them.  Returns float32
+    (R, E) of shape (npairs, n) on the GPU."""
     import torch
 
     from . import fastdec as fd
@@ -167,6 +170,9 @@ def make_pairs(md: MultigridData, npairs, cycles_per_rhs=8, nterms=16, seed=0):
         f = f - (h0 @ f) / h0.sum()
         ustar, _, _ = mgm.solve(md, f, tol=1e-11, variant="plain", max_cycles=100)
         u = torch.zeros(n, dtype=torch.float64, device="cuda")
+        if random_start:
+            fd.fill_uniform(u, int(rng.integers(1 << 30)))
+            u *= float(ustar.norm()) / float(u.norm())
         for _ in range(cycles_per_rhs):
             if k >= npairs:
                 break

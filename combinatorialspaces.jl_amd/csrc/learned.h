@@ -12,6 +12,7 @@ namespace dec {
 // back (n rows) and their transposes (nc = 0 on the coarsest level).
 struct NnLevel {
   int n = 0, nc = 0;
+  float dmean = 1.0f;  // mean of D: the error loss's scale
   struct Op {
     int n = 0, width = 0;
     DBuf<int32_t> col;

@@ -542,13 +542,14 @@ __global__ void k_bspmv(int nrows, int ncols, int width, const int *__restrict__
   y[static_cast<size_t>(b) * nrows + i] = acc;
 }
 
-__global__ void k_err(int n, const float *__restrict__ e, const float *__restrict__ s, const float *__restrict__ y,
-                      const float *__restrict__ D, float *__restrict__ eps) {
+// eps = κ (e / s - y / D), κ = mean(D): the error in residual units
+__global__ void k_err(int n, float kap, const float *__restrict__ e, const float *__restrict__ s,
+                      const float *__restrict__ y, const float *__restrict__ D, float *__restrict__ eps) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y;
   if (v >= n) return;
   size_t o = static_cast<size_t>(b) * n + v;
-  eps[o] = e[o] / s[b] - y[o] / D[v];
+  eps[o] = kap * (e[o] / s[b] - y[o] / D[v]);
 }
 
 // eh = eps - t (t = Pr R eps) in place of eps; block partials of Σ eh²
@@ -572,7 +573,7 @@ __global__ void k_hp(int n, const float *__restrict__ t, float *__restrict__ eps
   if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
 }
 
-// g = scale (eh - t), t = Rᵀ Prᵀ eh: the gradient Hᵀ H ε scaled by 2 / (n B)
+// g = scale (eh - t), t = Rᵀ Prᵀ eh: the gradient Hᵀ H ε scaled by 2 κ / (n B)
 __global__ void k_hp_grad(int n, float scale, const float *__restrict__ eh, const float *__restrict__ t,
                           float *__restrict__ g) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -903,6 +904,9 @@ NnLevel::NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr) {
     P.ptr[r + 1] = static_cast<int32_t>(P.col.size());
   }
   D.upload(d);
+  double dsum = 0.0;
+  for (int r = 0; r < n; r++) dsum += d[r];
+  dmean = n ? static_cast<float>(dsum / n) : 1.0f;
   to_ell(A, a);
   to_ell(transpose(A), at);
   to_ell(P, p);
@@ -1039,7 +1043,7 @@ void loss_grad_t(dec_nn_s &nn, const dec::NnLevel &L, int B, const float *r, con
   } else {
     if (L.nc == 0) fail(DEC_ERR_ARG, "learned smoother: the error loss needs a coarser level below this one");
     dim3 gc(dec::nblocks(L.nc, TB), B);
-    hipLaunchKernelGGL(k_err, gv, dim3(TB), 0, st, n, e, nn.s.p, nn.y.p, L.D.p, nn.eps.p);
+    hipLaunchKernelGGL(k_err, gv, dim3(TB), 0, st, n, L.dmean, e, nn.s.p, nn.y.p, L.D.p, nn.eps.p);
     hipLaunchKernelGGL(k_bspmv, gc, dim3(TB), 0, st, L.nc, n, L.r.width, L.r.col.p, L.r.val.p, nn.eps.p, nn.crs.p);
     hipLaunchKernelGGL(k_bspmv, gv, dim3(TB), 0, st, n, L.nc, L.pr.width, L.pr.col.p, L.pr.val.p, nn.crs.p,
                        nn.tmp.p);
@@ -1047,8 +1051,8 @@ void loss_grad_t(dec_nn_s &nn, const dec::NnLevel &L, int B, const float *r, con
     hipLaunchKernelGGL(k_bspmv, gc, dim3(TB), 0, st, L.nc, n, L.prt.width, L.prt.col.p, L.prt.val.p, nn.eps.p,
                        nn.crs.p);
     hipLaunchKernelGGL(k_bspmv, gv, dim3(TB), 0, st, n, L.nc, L.rt.width, L.rt.col.p, L.rt.val.p, nn.crs.p, nn.tmp.p);
-    hipLaunchKernelGGL(k_hp_grad, gv, dim3(TB), 0, st, n, 2.0f / (static_cast<float>(n) * B), nn.eps.p, nn.tmp.p,
-                       nn.drho.p);
+    hipLaunchKernelGGL(k_hp_grad, gv, dim3(TB), 0, st, n, 2.0f * L.dmean / (static_cast<float>(n) * B), nn.eps.p,
+                       nn.tmp.p, nn.drho.p);
   }
   reduce_parts(nn, nn.dpart.p, nbv, 1, 0, 1, 1.0 / (static_cast<double>(n) * B), loss_out, st);
   // output head: dz3, and the (wo, bo, alpha, b3) partials

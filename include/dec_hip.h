@@ -206,7 +206,8 @@ int dec_nn_forward(dec_nn nn, int32_t level, int32_t nsamples, const float *r, f
 int dec_nn_smooth(dec_nn nn, int32_t level, const float *r, float *u, void *stream);
 /* Loss and its gradient (host copies).  e == NULL: the residual loss
  * mean((r - A c)² / s²); e given (the errors A⁻¹r paired with r, same
- * layout): the error loss mean((H (e - c))² / s²), H = I - Pr R with R the
+ * layout): the error loss mean((κ H (e - c))² / s²), κ = mean(diag A),
+ * H = I - Pr R with R the
  * restriction to the next coarser level and Pr the prolongation back (the
  * error left that the coarse grid cannot represent; not on the coarsest
  * level). */

@@ -20,10 +20,11 @@ Training losses over a batch:
   residual loss: mean over samples and vertices of ((r - A c) / s)², the
     residual left after the correction;
   error loss (residual -> error pairs (r, e), e the error A⁻¹r of the
-    iterate): mean of (H (e - c) / s)² with the high-pass H = I - Pr R, R the
-    restriction to the next coarser level and Pr the prolongation back: the
-    part of the remaining error that the coarse grid cannot represent, which
-    is what a multigrid smoother has to remove.
+    iterate): mean of (κ H (e - c) / s)² with the high-pass H = I - Pr R, R
+    the restriction to the next coarser level and Pr the prolongation back:
+    the part of the remaining error that the coarse grid cannot represent,
+    which is what a multigrid smoother has to remove.  κ = mean(D) puts the
+    error in residual units (the loss is then O(1), as the residual loss).
 
 Gradients are pinned by central finite differences in the tests; nothing else
 fixes them (parity of the network is against this restatement only).
@@ -226,24 +227,24 @@ def high_pass(lv: Level, v):
 
 
 def loss_err(theta, lv: Level, R, E):
-    """Mean over samples and vertices of (H (e - c) / s)²."""
-    tot = 0.0
+    """Mean over samples and vertices of (κ H (e - c) / s)², κ = mean(D)."""
+    tot, kap = 0.0, np.mean(lv.D)
     for r, e in zip(R, E):
         c, k = forward(theta, lv, r)
-        eh = high_pass(lv, (e - c) / k["s"])
+        eh = high_pass(lv, kap * (e - c) / k["s"])
         tot += np.mean(eh * eh)
     return tot / len(R)
 
 
 def loss_err_grad(theta, lv: Level, R, E):
     g = np.zeros_like(theta)
-    tot, B, N = 0.0, len(R), lv.n
+    tot, B, N, kap = 0.0, len(R), lv.n, np.mean(lv.D)
     for r, e in zip(R, E):
         c, k = forward(theta, lv, r)
-        eps = (e - c) / k["s"]          # = e / s - y / D
+        eps = kap * (e - c) / k["s"]    # = κ (e / s - y / D)
         eh = high_pass(lv, eps)
         tot += np.mean(eh * eh)
-        geps = 2.0 * (eh - lv.R.T(lv.Pr.T(eh))) / (N * B)   # Hᵀ H eps
+        geps = 2.0 * kap * (eh - lv.R.T(lv.Pr.T(eh))) / (N * B)   # κ Hᵀ H eps
         g += backward(theta, lv, k, -geps / lv.D)
     return tot / B, g
 
