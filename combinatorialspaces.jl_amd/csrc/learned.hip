@@ -92,6 +92,37 @@ __global__ void k_scale_partial(int n, const float *__restrict__ r, double *__re
   if (threadIdx.x == 0) part[static_cast<size_t>(blockIdx.y) * kScaleBlocks + blockIdx.x] = sh[0];
 }
 
+// One residual: k_scale_partial's sums, and the last workgroup to finish
+// adds the partials in block order and writes s; it also resets the ticket
+// counter for the next launch (so the kernel replays from a graph).
+__global__ void k_scale_one(int n, const float *__restrict__ r, double *__restrict__ part,
+                            unsigned int *__restrict__ ticket, float *__restrict__ s) {
+  __shared__ double sh[TB];
+  __shared__ bool last;
+  double acc = 0.0;
+  for (int i = blockIdx.x * TB + threadIdx.x; i < n; i += kScaleBlocks * TB) acc += static_cast<double>(r[i]) * r[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = sh[0];
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == kScaleBlocks - 1;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    double tot = 0.0;
+    for (int j = 0; j < kScaleBlocks; j++) tot += part[j];
+    double m = tot / n;
+    s[0] = m > 0 ? static_cast<float>(sqrt(m)) : 1.0f;
+    *ticket = 0u;
+  }
+}
+
 __global__ void k_scale_final(int n, int B, const double *__restrict__ part, float *__restrict__ s) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
@@ -247,6 +278,8 @@ __global__ void k_output(int n, const float *__restrict__ x, const AT *__restric
 // c of the BF16 forward.
 
 constexpr int kInfWaves = TB / 64;
+// levels up to this many rows use the 16-rows-per-wavefront layer kernels
+constexpr int kInfSmallRows = 50000;
 constexpr int kRowPitch = 2 * H + 8;  // bf16 per LDS row: [h | P h] plus padding
 
 __device__ __forceinline__ void load_wB(const float *__restrict__ W, int li, int lg, short4_t B[2][4]) {
@@ -484,6 +517,181 @@ __global__ void __launch_bounds__(TB) k_inf_layer3(int n, int pw, const int *__r
       for (int j = 0; j < H; j++) hw = hw + o[j];
       float yy = alpha * xv + hw + bo;
       u[v] = uv + (sc / dv) * yy;
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int feat(int lg, int q) { return q < 4 ? 4 * lg + q : 16 + 4 * lg + (q - 4); }
+
+__device__ __forceinline__ short4_t pack4(const unsigned short *h) {
+  short4_t a;
+  a[0] = static_cast<short>(h[0]);
+  a[1] = static_cast<short>(h[1]);
+  a[2] = static_cast<short>(h[2]);
+  a[3] = static_cast<short>(h[3]);
+  return a;
+}
+
+__device__ __forceinline__ void unpack4(uint2 w, unsigned short *h) {
+  h[0] = static_cast<unsigned short>(w.x & 0xffffu);
+  h[1] = static_cast<unsigned short>(w.x >> 16);
+  h[2] = static_cast<unsigned short>(w.y & 0xffffu);
+  h[3] = static_cast<unsigned short>(w.y >> 16);
+}
+
+// The same two layers for small levels, 16 rows per wavefront: lane (li, lg)
+// owns row r0 + li and the features F(lg) (its MFMA A-operand entries), so a
+// row's gathers and layer-1 recomputation are split over four lanes, which
+// shortens each wavefront's dependent chain where there are few rows per CU.
+// h2 = relu([h1, P h1] W2 + b2), h1 = relu([x, P x] W1 + b1) evaluated per
+// neighbour.  All waves of a block run the same number of tile steps.
+__global__ void __launch_bounds__(TB) k_inf_layer2_t(int n, int pw, const int *__restrict__ pcol,
+                                                   const float *__restrict__ pval, const float2 *__restrict__ xp,
+                                                   const float *__restrict__ th, unsigned short *__restrict__ h2) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int li = lane & 15, lg = lane >> 4;
+  float w0[8], w1[8], b1[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    int f = feat(lg, q);
+    w0[q] = th[OW1 + f];
+    w1[q] = th[OW1 + H + f];
+    b1[q] = th[OB1 + f];
+  }
+  short4_t Bw[2][4];
+  load_wB(th + OW2, li, lg, Bw);
+  float bias0 = th[OB2 + li], bias1 = th[OB2 + 16 + li];
+  int ntiles = (n + 15) / 16;
+  constexpr int WPB = TB / 64;
+  for (int base = blockIdx.x * WPB; base < ntiles; base += gridDim.x * WPB) {
+    int t = base + w;
+    if (t >= ntiles) continue;
+    int r0 = t * 16, v = r0 + li;
+    short4_t A[4];
+    if (v < n) {
+      float2 f = xp[v];
+      unsigned short hb[8];
+      float m[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        float z = f.x * w0[q] + f.y * w1[q] + b1[q];
+        hb[q] = f2bf(z > 0.0f ? z : 0.0f);
+        m[q] = 0.0f;
+      }
+      for (int k = 0; k < pw; k++) {
+        size_t qq = static_cast<size_t>(k) * n + v;
+        int c = pcol[qq];
+        if (c < 0) break;
+        float pv = pval[qq];
+        float2 fc = xp[c];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          float z = fc.x * w0[q] + fc.y * w1[q] + b1[q];
+          m[q] = m[q] + pv * bf2f(f2bf(z > 0.0f ? z : 0.0f));
+        }
+      }
+      unsigned short mb[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) mb[q] = f2bf(m[q]);
+      A[0] = pack4(hb);
+      A[1] = pack4(hb + 4);
+      A[2] = pack4(mb);
+      A[3] = pack4(mb + 4);
+    } else {
+      A[0] = A[1] = A[2] = A[3] = short4_t{0, 0, 0, 0};
+    }
+    float4_t acc0 = float4_t{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[0][ks], acc0, 0, 0, 0);
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[1][ks], acc1, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int rr = r0 + 4 * lg + i;
+      if (rr < n) {
+        float z0 = acc0[i] + bias0, z1 = acc1[i] + bias1;
+        h2[static_cast<size_t>(rr) * H + li] = f2bf(z0 > 0.0f ? z0 : 0.0f);
+        h2[static_cast<size_t>(rr) * H + 16 + li] = f2bf(z1 > 0.0f ? z1 : 0.0f);
+      }
+    }
+  }
+}
+
+// h3 = relu([h2, P h2] W3 + b3); y = alpha x + h3 wo + bo (ascending sum over
+// the 32 features, staged through LDS); u += (s / D) y.
+__global__ void __launch_bounds__(TB) k_inf_layer3_t(int n, int pw, const int *__restrict__ pcol,
+                                                   const float *__restrict__ pval, const unsigned short *__restrict__ h2,
+                                                   const float2 *__restrict__ xp, const float *__restrict__ th,
+                                                   const float *__restrict__ s, const float *__restrict__ D,
+                                                   float *__restrict__ u) {
+  constexpr int WPB = TB / 64;
+  __shared__ float prod[WPB][16][H + 1];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int li = lane & 15, lg = lane >> 4;
+  short4_t Bw[2][4];
+  load_wB(th + OW3, li, lg, Bw);
+  float bias0 = th[OB3 + li], bias1 = th[OB3 + 16 + li];
+  float wo0 = th[OWO + li], wo1 = th[OWO + 16 + li];
+  float alpha = th[OAL], bo = th[OBO], sc = s[0];
+  int ntiles = (n + 15) / 16;
+  for (int base = blockIdx.x * WPB; base < ntiles; base += gridDim.x * WPB) {
+    int t = base + w;
+    bool active = t < ntiles;
+    int r0 = t * 16, v = r0 + li;
+    if (active) {
+      short4_t A[4];
+      if (v < n) {
+        const unsigned short *hv = h2 + static_cast<size_t>(v) * H;
+        unsigned short hb[8];
+        unpack4(*reinterpret_cast<const uint2 *>(hv + 4 * lg), hb);
+        unpack4(*reinterpret_cast<const uint2 *>(hv + 16 + 4 * lg), hb + 4);
+        float m[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) m[q] = 0.0f;
+        for (int k = 0; k < pw; k++) {
+          size_t qq = static_cast<size_t>(k) * n + v;
+          int c = pcol[qq];
+          if (c < 0) break;
+          float pv = pval[qq];
+          const unsigned short *hc = h2 + static_cast<size_t>(c) * H;
+          unsigned short nb[8];
+          unpack4(*reinterpret_cast<const uint2 *>(hc + 4 * lg), nb);
+          unpack4(*reinterpret_cast<const uint2 *>(hc + 16 + 4 * lg), nb + 4);
+#pragma unroll
+          for (int q = 0; q < 8; q++) m[q] = m[q] + pv * bf2f(nb[q]);
+        }
+        unsigned short mb[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) mb[q] = f2bf(m[q]);
+        A[0] = pack4(hb);
+        A[1] = pack4(hb + 4);
+        A[2] = pack4(mb);
+        A[3] = pack4(mb + 4);
+      } else {
+        A[0] = A[1] = A[2] = A[3] = short4_t{0, 0, 0, 0};
+      }
+      float4_t acc0 = float4_t{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++) acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[0][ks], acc0, 0, 0, 0);
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++) acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[ks], Bw[1][ks], acc1, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        float z0 = acc0[i] + bias0, z1 = acc1[i] + bias1;
+        prod[w][4 * lg + i][li] = bf2f(f2bf(z0 > 0.0f ? z0 : 0.0f)) * wo0;
+        prod[w][4 * lg + i][16 + li] = bf2f(f2bf(z1 > 0.0f ? z1 : 0.0f)) * wo1;
+      }
+    }
+    __syncthreads();
+    if (active && lane < 16) {
+      int row = r0 + lane;
+      if (row < n) {
+        float hw = 0.0f;
+        for (int j = 0; j < H; j++) hw = hw + prod[w][lane][j];
+        float yy = alpha * xp[row].x + hw + bo;
+        u[row] = u[row] + (sc / D[row]) * yy;
+      }
     }
     __syncthreads();
   }
@@ -929,6 +1137,7 @@ struct dec_nn_s {
   // work buffers (grown on demand)
   DBuf<float> x, px, y, c, s, drho, dg, part, r, eps, tmp, crs;
   DBuf<double> dpart, dloss, spart, rtmp;
+  DBuf<unsigned int> ticket;  // inference scale: workgroups finished
   DBuf<float2> ix;                                 // inference: (x, P x)
   DBuf<unsigned short> ih1, ih2;                   // inference: layer-1 and layer-2 rows
   DBuf<unsigned char> h1, hm1, h2, hm2, h3, dz;  // activation storage (bf16 or f32)
@@ -1107,6 +1316,11 @@ void reserve_inference(dec_nn_s &nn, int n) {
   if (nn.ih2.n < static_cast<size_t>(n) * H) nn.ih2.alloc(static_cast<size_t>(n) * H);
   if (nn.s.n < 1) nn.s.alloc(1);
   if (nn.spart.n < static_cast<size_t>(kScaleBlocks)) nn.spart.alloc(kScaleBlocks);
+  if (nn.ticket.n < 1) {
+    nn.ticket.alloc(1);
+    std::vector<unsigned int> z(1, 0u);
+    nn.ticket.upload(z);
+  }
 }
 
 void nn_bind(dec_nn_s &nn, const std::vector<int> &sizes) {
@@ -1125,15 +1339,24 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s) {
   const NnLevel &L = level_of(nn, lvl);
   int n = L.n;
   reserve_inference(nn, n);
-  scale(nn, n, 1, r, s);
+  hipLaunchKernelGGL(k_scale_one, dim3(kScaleBlocks), dim3(TB), 0, s, n, r, nn.spart.p, nn.ticket.p, nn.s.p);
   hipLaunchKernelGGL(k_inf_input, dim3(dec::nblocks(n, TB)), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, r,
                      nn.s.p, nn.ix.p);
-  int steps = (n + 63) / 64;
-  int blocks = std::max(1, (steps + kInfWaves - 1) / kInfWaves);
-  hipLaunchKernelGGL(k_inf_layer2, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ix.p,
-                     nn.th.p, nn.ih2.p);
-  hipLaunchKernelGGL(k_inf_layer3, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ih2.p,
-                     nn.ix.p, nn.th.p, nn.s.p, L.D.p, u);
+  if (n <= kInfSmallRows) {
+    int tiles = (n + 15) / 16;
+    int blocks = std::max(1, (tiles + kInfWaves - 1) / kInfWaves);
+    hipLaunchKernelGGL(k_inf_layer2_t, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ix.p,
+                       nn.th.p, nn.ih2.p);
+    hipLaunchKernelGGL(k_inf_layer3_t, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ih2.p,
+                       nn.ix.p, nn.th.p, nn.s.p, L.D.p, u);
+  } else {
+    int steps = (n + 63) / 64;
+    int blocks = std::max(1, (steps + kInfWaves - 1) / kInfWaves);
+    hipLaunchKernelGGL(k_inf_layer2, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ix.p,
+                       nn.th.p, nn.ih2.p);
+    hipLaunchKernelGGL(k_inf_layer3, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ih2.p,
+                       nn.ix.p, nn.th.p, nn.s.p, L.D.p, u);
+  }
 }
 
 dec_nn_s *nn_create(const std::vector<NnOps> &ops, uint64_t seed) {
