@@ -1017,7 +1017,7 @@ __global__ void k_w1_back(int n, const float *__restrict__ x, const float *__res
 // per-block partials, in two deterministic passes: workgroup (x, c) sums the
 // partials p of chunk c for 64 columns (coalesced over the columns, double
 // accumulators), then one thread per column adds the chunk sums in order.
-constexpr int kRedChunks = 128;
+constexpr int kRedChunks = 4096;
 
 template <class PT>
 __global__ void k_reduce_chunks(const PT *__restrict__ part, int nparts, int stride, int off, int width, int len,
@@ -1164,7 +1164,7 @@ void ensure(dec_nn_s &nn, size_t rows, int precise) {
   size_t parts = (rows + 255) / 256 + 64;
   nn.part.alloc(parts * 2 * H * H);
   nn.dpart.alloc(parts + 64);
-  nn.rtmp.alloc(static_cast<size_t>(kRedChunks) * 2 * H * H);
+
   nn.cap_rows = rows;
   nn.cap_prec = precise;
 }
@@ -1173,11 +1173,12 @@ void ensure(dec_nn_s &nn, size_t rows, int precise) {
 template <class PT, class OT>
 void reduce_parts(dec_nn_s &nn, const PT *part, int nparts, int stride, int off, int width, double scale, OT *out,
                   hipStream_t st) {
-  int chunks = std::max(1, std::min(kRedChunks, nparts));
+  // about 16 partials per thread in the first pass
+  int chunks = std::max(1, std::min(kRedChunks, (nparts + 15) / 16));
   int len = (nparts + chunks - 1) / chunks;
   chunks = (nparts + len - 1) / len;
   size_t need = static_cast<size_t>(chunks) * width;
-  if (nn.rtmp.n < need) fail(DEC_ERR_ARG, "learned smoother: reduction scratch too small");
+  if (nn.rtmp.n < need) nn.rtmp.alloc(need);
   hipLaunchKernelGGL((k_reduce_chunks<PT>), dim3((width + 63) / 64, chunks), dim3(64), 0, st, part, nparts, stride,
                      off, width, len, nn.rtmp.p);
   hipLaunchKernelGGL((k_reduce_final<OT>), dim3((width + 63) / 64), dim3(64), 0, st, nn.rtmp.p, chunks, width, scale,
