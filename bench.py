@@ -275,10 +275,10 @@ def learned(torch, cs, md, fd, npairs=10000, train_levels=3, steps=2000, batch=1
     app_ms = a.elapsed_time(b) / reps
     out["inference_ms_per_application_fine"] = app_ms
     out["dense_layer_TFLOPs_fine"] = 2 * 2 * 64 * 32 * n0 / (app_ms * 1e-3) / 1e12
-    # levels that use the network: all; the two finest; the levels below the
-    # finest (the finest keeps Gauss-Seidel)
+    # levels that use the network: all (one or two applications); the two
+    # finest; the three below the finest (the finest keeps Gauss-Seidel)
     for tag, lo, hi, apps in (("all_levels", 0, 0, 1), ("all_levels_2_applications", 0, 0, 2),
-                              ("levels_ge_40963", 40963, 0, 1), ("levels_2563_to_163842", 2563, 163842, 1)):
+                              ("levels_ge_40963", 40963, 0, 1), ("levels_10242_to_163842", 10242, 163842, 1)):
         nn.attach(True, min_rows=lo, max_rows=hi, applications=apps)
         cs.solve(md, fd, tol=1e-8, variant="mixed")
         torch.cuda.synchronize()

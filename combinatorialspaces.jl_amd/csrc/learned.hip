@@ -1173,8 +1173,8 @@ void ensure(dec_nn_s &nn, size_t rows, int precise) {
 template <class PT, class OT>
 void reduce_parts(dec_nn_s &nn, const PT *part, int nparts, int stride, int off, int width, double scale, OT *out,
                   hipStream_t st) {
-  // about 16 partials per thread in the first pass
-  int chunks = std::max(1, std::min(kRedChunks, (nparts + 15) / 16));
+  // about sqrt(nparts) partials per thread in each of the two passes
+  int chunks = std::max(1, std::min(kRedChunks, static_cast<int>(std::ceil(std::sqrt(static_cast<double>(nparts))))));
   int len = (nparts + chunks - 1) / chunks;
   chunks = (nparts + len - 1) / len;
   size_t need = static_cast<size_t>(chunks) * width;
