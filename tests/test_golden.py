@@ -1,0 +1,94 @@
+"""Golden fixtures (tests/golden/*.npz, written by tests/golden/make_fixtures.py
+from the oracle): the oracle must still reproduce them bit for bit (CPU), and
+the HIP library must match them to the parity bars (GPU): indices and
+orientations bit-exact, float64 outputs within 1e-12 normwise relative,
+bit-exact where both sides evaluate the same expression (wedge 0-1, d, ∂,
+and on barycentric duals every operator)."""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+OPS = ["wedge01", "wedge11", "wedge02", "wedge_dd01", "wedge_dd10", "d0", "d1", "boundary1", "boundary2",
+       "dual_d0", "dual_d1", "hodge0", "hodge1", "hodge2", "inv_hodge0", "inv_hodge1", "inv_hodge2",
+       "geo_hodge1", "laplacian0", "laplacian0_geometric"]
+EXACT_ALWAYS = {"wedge01", "d0", "d1", "boundary1", "boundary2", "dual_d0", "dual_d1"}
+# on barycentric duals the center formula is shared: these agree bit for bit too
+EXACT_BARY = {"wedge11", "wedge_dd01", "wedge_dd10", "hodge0", "hodge1", "hodge2", "geo_hodge1", "laplacian0",
+              "laplacian0_geometric"}
+
+
+def load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def test_fixtures_present():
+    assert len(FIXTURES) == 2
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=lambda p: os.path.basename(p))
+def test_oracle_reproduces_fixture(O, path):
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    import make_fixtures
+
+    f = load(path)
+    mesh = O.Mesh(f["points"], f["v0"], f["v1"], f["e0"], f["e1"], f["e2"], f["edge_orient"], f["tri_orient"])
+    now = make_fixtures.outputs(mesh, circumcenter="circumcenter" in path)
+    assert set(now) == set(f)
+    for k, v in f.items():
+        assert np.array_equal(now[k], v), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", FIXTURES, ids=lambda p: os.path.basename(p))
+def test_hip_matches_fixture(cs, path):
+    import torch
+
+    f = load(path)
+    circ = "circumcenter" in path
+    gm = cs.mesh_from_arrays(f["points"], f["v0"], f["v1"], f["e0"], f["e1"], f["e2"], f["edge_orient"],
+                             f["tri_orient"])
+    cs.dualize(gm, cs.Circumcenter if circ else cs.Barycenter)
+    g = gm.export()
+    for a in ("v0", "v1", "e0", "e1", "e2"):
+        assert np.array_equal(g[a], f[a]), a
+    assert np.array_equal(g["edge_orientation"], f["edge_orient"])
+    assert np.array_equal(g["tri_orientation"], f["tri_orient"])
+    gd = gm.export_dual()
+    for a in ("length", "area", "dual_length", "dual_area"):
+        rel = np.max(np.abs(gd[a] - f[a])) / np.max(np.abs(f[a]))
+        assert rel <= 1e-12, a
+        if not circ:
+            assert np.array_equal(gd[a], f[a]), a
+    dv = lambda k: torch.tensor(f[k], dtype=torch.float64, device="cuda")  # noqa: E731
+    got = {
+        "wedge01": cs.dec_wedge_product(0, 1, gm)(dv("V"), dv("E")),
+        "wedge11": cs.dec_wedge_product(1, 1, gm)(dv("E"), dv("E2")),
+        "wedge02": cs.dec_wedge_product(0, 2, gm)(dv("V"), dv("T")),
+        "wedge_dd01": cs.dec_wedge_product_dd(0, 1, gm)(dv("DT"), dv("DE")),
+        "wedge_dd10": cs.dec_wedge_product_dd(1, 0, gm)(dv("DE"), dv("DT")),
+        "d0": cs.dec_differential(0, gm) @ dv("V"), "d1": cs.dec_differential(1, gm) @ dv("E"),
+        "boundary1": cs.dec_boundary(1, gm) @ dv("E"), "boundary2": cs.dec_boundary(2, gm) @ dv("T"),
+        "dual_d0": cs.dec_dual_derivative(0, gm) @ dv("DT"), "dual_d1": cs.dec_dual_derivative(1, gm) @ dv("DE"),
+        "hodge0": cs.dec_hodge_star(0, gm, cs.DiagonalHodge) @ dv("V"),
+        "hodge1": cs.dec_hodge_star(1, gm, cs.DiagonalHodge) @ dv("E"),
+        "hodge2": cs.dec_hodge_star(2, gm, cs.DiagonalHodge) @ dv("T"),
+        "inv_hodge0": cs.dec_inv_hodge_star(0, gm, cs.DiagonalHodge) @ dv("V"),
+        "inv_hodge1": cs.dec_inv_hodge_star(1, gm, cs.DiagonalHodge) @ dv("E"),
+        "inv_hodge2": cs.dec_inv_hodge_star(2, gm, cs.DiagonalHodge) @ dv("T"),
+        "geo_hodge1": cs.dec_hodge_star(1, gm, cs.GeometricHodge) @ dv("E"),
+        "laplacian0": cs.fast_laplace_beltrami(gm, cs.DiagonalHodge) @ dv("V"),
+        "laplacian0_geometric": cs.fast_laplace_beltrami(gm, cs.GeometricHodge) @ dv("V"),
+    }
+    assert set(got) == set(OPS)
+    for k in OPS:
+        x, ref = got[k].cpu().numpy(), f[k]
+        rel = np.max(np.abs(x - ref)) / max(np.max(np.abs(ref)), 1e-300)
+        assert rel <= 1e-12, (k, rel)
+        if k in EXACT_ALWAYS or (not circ and k in EXACT_BARY):
+            assert np.array_equal(x, ref), k
