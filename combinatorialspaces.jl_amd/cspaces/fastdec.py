@@ -294,6 +294,32 @@ def dec_wedge_product_dd(m, n, sd: Mesh, backend=HIP, cast_float=None):
     return fn
 
 
+def dec_wedge_product_pd(m, n, sd: Mesh, backend=HIP, cast_float=None):
+    """dec_wedge_product_pd(0, 1, sd) (FastDEC.jl:379-382): (g, f) -> (M g) .* f
+    for a primal 0-form g and a dual 1-form f, M = wedge_pd_01_mat."""
+    _check_backend(backend)
+    if (m, n) != (0, 1):
+        raise DecError(L.DEC_ERR_UNSUPPORTED, f"Unsupported combination of degrees {m} and {n}. The primal-dual "
+                       "wedges supported here are 0-1 (and dual-primal 1-0).")
+    op = Operator(sd, L.DEC_OP_WEDGE_PD, 0, 1, cast_float=cast_float)
+    fn = lambda g, f, out=None, stream=None: op.apply(g, f, out=out, stream=stream)  # noqa: E731
+    fn.operator = op
+    return fn
+
+
+def dec_wedge_product_dp(m, n, sd: Mesh, backend=HIP, cast_float=None):
+    """dec_wedge_product_dp(1, 0, sd) (FastDEC.jl:374-377): (f, g) -> f .* (M g)
+    for a dual 1-form f and a primal 0-form g."""
+    _check_backend(backend)
+    if (m, n) != (1, 0):
+        raise DecError(L.DEC_ERR_UNSUPPORTED, f"Unsupported combination of degrees {m} and {n}. The dual-primal "
+                       "wedges supported here are 1-0 (and primal-dual 0-1).")
+    op = Operator(sd, L.DEC_OP_WEDGE_PD, 1, 0, cast_float=cast_float)
+    fn = lambda f, g, out=None, stream=None: op.apply(f, g, out=out, stream=stream)  # noqa: E731
+    fn.operator = op
+    return fn
+
+
 def dec_differential(n, sd: Mesh, backend=HIP, cast_float=None) -> Operator:
     """dec_differential(n, sd) (FastDEC.jl:468-469)."""
     _check_backend(backend)

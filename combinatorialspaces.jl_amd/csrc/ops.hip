@@ -30,7 +30,8 @@ enum Impl {
   I_SELL,  // ±1 ELL: ∂1 = d0ᵀ, d̃1 = -d0ᵀ, ∂2 = d̃0 = d1ᵀ
   I_DIAG,  // diagonal ⋆, ⋆⁻¹
   I_ELL,   // general sparse: geometric ⋆1, ∇²
-  I_AVG    // avg₀₁
+  I_AVG,   // avg₀₁
+  I_WPD    // primal-dual wedge 0-1 / dual-primal 1-0: (M g) .* f, M = wedge_pd_01_mat
 };
 
 }  // namespace
@@ -321,6 +322,21 @@ template <class T>
 __global__ void k_avg01(int ne, int vec, const int *__restrict__ v0, const int *__restrict__ v1, const T *__restrict__ x,
                         T *__restrict__ y) {
   b_avg01<T>(dec_ell::xcd_block(), ne, vec, v0, v1, x, y);
+}
+
+// (M g)[e] f[e], M = wedge_pd_01_mat (FastDEC.jl:329-346) in ELL with
+// ascending columns (the CSC SpMV order), one row per thread.
+template <int W, class T>
+__device__ __forceinline__ void b_wedge_pd(int blk, int ne, int width, const int *__restrict__ col,
+                                           const T *__restrict__ val, const T *__restrict__ g, const T *__restrict__ f,
+                                           T *__restrict__ res) {
+  int e = blk * blockDim.x + threadIdx.x;
+  if (e < ne) stn(res + e, dec_ell::row<W, T, T>(ne, width, e, col, val, g) * f[e]);
+}
+template <int W, class T>
+__global__ void k_wedge_pd(int ne, int width, const int *__restrict__ col, const T *__restrict__ val,
+                           const T *__restrict__ g, const T *__restrict__ f, T *__restrict__ res) {
+  b_wedge_pd<W, T>(static_cast<int>(blockIdx.x), ne, width, col, val, g, f, res);
 }
 
 // ---- coefficient precomputation (float64) ----
@@ -731,6 +747,14 @@ void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream
     case I_AVG:
       hipLaunchKernelGGL(k_avg01<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, x, out);
       break;
+    case I_WPD: {
+      const dec::Ell<T> *E;
+      if constexpr (sizeof(T) == 8) E = &op.ell64;
+      else E = &op.ell32;
+      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_wedge_pd<W, T>), g, b, 0, s, rows, E->width, E->col.p,
+                                                        E->val.p, x, y, out));
+      break;
+    }
     default:
       fail(DEC_ERR_ARG, "dec_op_apply: corrupt operator");
   }
@@ -790,6 +814,10 @@ __global__ void __launch_bounds__(256) k_apply_group(GroupArgs g) {
                                                      static_cast<const T *>(o.eval), x, out)));
       break;
     case I_AVG: b_avg01<T>(blk, o.rows, o.vec, o.i0, o.i1, x, out); break;
+    case I_WPD:
+      DEC_ELL_WIDTH_SWITCH(o.width, (b_wedge_pd<W, T>(blk, o.rows, o.width, o.ecol, static_cast<const T *>(o.eval), x, y,
+                                                      out)));
+      break;
     default: break;
   }
 }
@@ -800,7 +828,8 @@ int group_blocks(const dec_op_s &op) {
   long long rows = op.rows;
   switch (op.impl) {
     case I_SELL:
-    case I_ELL: return dec::nblocks(rows, 256);
+    case I_ELL:
+    case I_WPD: return dec::nblocks(rows, 256);
     default: return dec::nblocks((rows + 1) / 2, 256);
   }
 }
@@ -818,7 +847,7 @@ void apply_group_t(int count, dec_op_s *const *ops, const void *const *xs, const
     for (int q = base; q < count && q < base + kMaxGroup; q++) order.push_back(q);
     auto rank = [&](int q) {
       int m = ops[q]->impl;
-      return m == I_ELL || m == I_SELL ? 0 : (m == I_DIAG ? 2 : 1);
+      return m == I_ELL || m == I_SELL || m == I_WPD ? 0 : (m == I_DIAG ? 2 : 1);
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rank(a) < rank(b); });
     for (int q : order) {
@@ -847,7 +876,7 @@ void apply_group_t(int count, dec_op_s *const *ops, const void *const *xs, const
         o.width = op.sell.width;
         o.ecol = op.sell.col.p;
         o.eval = op.sell.val.p;
-      } else if (op.impl == I_ELL) {
+      } else if (op.impl == I_ELL || op.impl == I_WPD) {
         if constexpr (sizeof(T) == 8) {
           o.width = op.ell64.width;
           o.ecol = op.ell64.col.p;
@@ -885,6 +914,56 @@ void make_diag(dec_op_s &op, const DBuf<double> &h, bool inverse, int k) {
                          static_cast<int>(h.n), h.p, mult, reinterpret_cast<T *>(op.c0.p));
   } else {
     launch_cast<T>(h, op.c0);
+  }
+}
+
+// wedge_pd_01_mat (FastDEC.jl:329-346): for edge e = (α, β) = (∂v0, ∂v1) and
+// each elementary dual edge of e (B-block dual edges, ascending) with weight
+// w = dual_length / Σ dual_length, and l the third vertex of its triangle:
+// M[e,α] += w·5/12, M[e,β] += w·5/12, M[e,l] += w·2/12.  Rows in ascending
+// column order, repeated (row, column) entries summed in that input order.
+void pd01_matrix(const dec_mesh_s &m, dec::HostCsr &M) {
+  const dec::Topo &t = m.h;
+  int nv = t.nv, ne = t.ne, nt = t.nt;
+  dec::Csr eb;
+  dec::edge_bduals(t, eb);
+  std::vector<double> dlB(3 * static_cast<size_t>(nt));
+  if (nt) dec::d2h(dlB.data(), m.dl.p + 2 * static_cast<size_t>(ne), sizeof(double) * dlB.size());
+  M.m = ne;
+  M.n = nv;
+  M.ptr.assign(ne + 1, 0);
+  M.col.clear();
+  M.val.clear();
+  std::vector<std::pair<int, double>> ent;
+  for (int e = 0; e < ne; e++) {
+    int a = t.v0[e], b = t.v1[e];
+    double tot = 0.0;
+    for (int k = eb.ptr[e]; k < eb.ptr[e + 1]; k++) tot += dlB[eb.idx[k]];
+    ent.clear();
+    for (int k = eb.ptr[e]; k < eb.ptr[e + 1]; k++) {
+      int tri = eb.idx[k] % nt;
+      int tv[3] = {t.v1[t.e1[tri]], t.v0[t.e2[tri]], t.v0[t.e1[tri]]};
+      int l = -1;
+      for (int q = 0; q < 3; q++)
+        if (tv[q] != a && tv[q] != b) l = tv[q];
+      if (l < 0) fail(DEC_ERR_ARG, "wedge_pd_01_mat: degenerate triangle");
+      double w = dlB[eb.idx[k]] / tot;
+      ent.push_back({a, w * 5 / 12});
+      ent.push_back({b, w * 5 / 12});
+      ent.push_back({l, w * 2 / 12});
+    }
+    // stable sort by column, then sum runs in input order
+    std::stable_sort(ent.begin(), ent.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    for (size_t i = 0; i < ent.size();) {
+      int c = ent[i].first;
+      double v = ent[i].second;
+      size_t j = i + 1;
+      for (; j < ent.size() && ent[j].first == c; j++) v = v + ent[j].second;
+      M.col.push_back(c);
+      M.val.push_back(v);
+      i = j;
+    }
+    M.ptr[e + 1] = static_cast<int32_t>(M.col.size());
   }
 }
 
@@ -1015,6 +1094,23 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
       dec::dsync();
       op->cols = dm == 0 ? nt : ne;
       op->cols2 = dm == 0 ? ne : nt;
+      break;
+    }
+    case DEC_OP_WEDGE_PD: {
+      // dec_wedge_product_pd(0,1) / dec_wedge_product_dp(1,0) (FastDEC.jl:327-346, 379-387)
+      if (!((dm == 0 && dn == 1) || (dm == 1 && dn == 0)))
+        fail(DEC_ERR_UNSUPPORTED, "Unsupported combination of degrees " + std::to_string(dm) + " and " +
+                                      std::to_string(dn) +
+                                      ". Ensure that their sum is not greater than the degree of the complex. The "
+                                      "currently supported primal-dual wedges are 0-1 and 1-0.");
+      op->impl = I_WPD;
+      op->rows = ne;
+      op->swap = dm == 1;  // dp(1,0) takes (f, g): the kernel runs on (g, f)
+      op->cols = dm == 0 ? nv : ne;
+      op->cols2 = dm == 0 ? ne : nv;
+      pd01_matrix(m, op->csr);
+      if (f64) dec::upload_ell(op->csr, op->ell64);
+      else dec::upload_ell(op->csr, op->ell32);
       break;
     }
     case DEC_OP_DIFFERENTIAL:

@@ -51,7 +51,8 @@ enum dec_op_kind {
   DEC_OP_HODGE = 6,           /* ⋆_n */
   DEC_OP_INV_HODGE = 7,       /* ⋆_n⁻¹ (diagonal) */
   DEC_OP_LAPLACIAN0 = 8,      /* -⋆0⁻¹ d̃1 ⋆1 d0 */
-  DEC_OP_AVG01 = 9            /* avg₀₁ */
+  DEC_OP_AVG01 = 9,           /* avg₀₁ */
+  DEC_OP_WEDGE_PD = 10        /* primal-dual wedge 0-1 / dual-primal 1-0 (wedge_pd_01_mat) */
 };
 enum dec_smoother { DEC_SMOOTH_CG = 0, DEC_SMOOTH_GS = 1, DEC_SMOOTH_JACOBI = 2 };
 enum dec_solve_variant { DEC_SOLVE_PLAIN_F64 = 0, DEC_SOLVE_MIXED_IR = 1 };

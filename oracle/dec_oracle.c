@@ -847,6 +847,53 @@ void om_wedge_dd01_mat(int nv, int ne, int nt, const int *D_v0, const int *D_v1,
   free(cnt); free(pos); free(lst); free(I); free(J); free(V);
 }
 
+/* wedge_pd_01_mat (src/FastDEC.jl:329-346): edge e = (α, β) = (∂v0, ∂v1);
+ * for each elementary dual edge de of e (dual edges ending at e's center,
+ * ascending) with w = dual_length[de] / Σ dual_length, the triangle t whose
+ * center starts de and its third vertex l: m[e,α] += w*5/12,
+ * m[e,β] += w*5/12, m[e,l] += w*2/12 (the += chain is sparse()'s duplicate
+ * summation in input order). */
+void om_wedge_pd01_mat(int nv, int ne, int nt, const int *v0, const int *v1, const int *e1,
+                       const int *e2, const int *D_v0, const int *D_v1, const double *dl,
+                       om_csc *M) {
+  int nde = 2 * ne + 6 * nt;
+  int *cnt = XALLOC(int, ne + 1);
+  for (int de = 0; de < nde; de++) {
+    int s = D_v1[de] - nv;
+    if (s >= 0 && s < ne) cnt[s + 1]++;
+  }
+  for (int e = 0; e < ne; e++) cnt[e + 1] += cnt[e];
+  int tot = cnt[ne];
+  int *pos = XALLOC(int, ne + 1), *lst = XALLOC(int, tot);
+  memcpy(pos, cnt, sizeof(int) * (ne + 1));
+  for (int de = 0; de < nde; de++) {
+    int s = D_v1[de] - nv;
+    if (s >= 0 && s < ne) lst[pos[s]++] = de;
+  }
+  int *I = XALLOC(int, 3 * tot), *J = XALLOC(int, 3 * tot);
+  double *V = XALLOC(double, 3 * tot);
+  int q = 0;
+  for (int e = 0; e < ne; e++) {
+    int a = v0[e], b = v1[e];
+    double sum = 0.0;
+    for (int k = cnt[e]; k < cnt[e + 1]; k++) sum = sum + dl[lst[k]];
+    for (int k = cnt[e]; k < cnt[e + 1]; k++) {
+      int t = D_v0[lst[k]] - nv - ne;
+      /* triangle_vertices(t) = (∂v1[∂e1], ∂v0[∂e2], ∂v0[∂e1]) (src/SimplicialSets.jl:305-309) */
+      int tv[3] = {v1[e1[t]], v0[e2[t]], v0[e1[t]]};
+      int l = -1;
+      for (int i = 0; i < 3; i++)
+        if (tv[i] != a && tv[i] != b) l = tv[i];
+      double w = dl[lst[k]] / sum;
+      I[q] = e; J[q] = a; V[q] = w * 5 / 12; q++;
+      I[q] = e; J[q] = b; V[q] = w * 5 / 12; q++;
+      I[q] = e; J[q] = l; V[q] = w * 2 / 12; q++;
+    }
+  }
+  om_sparse(ne, nv, q, I, J, V, M);
+  free(cnt); free(pos); free(lst); free(I); free(J); free(V);
+}
+
 static void cross3(const double *a, const double *b, double *c) {
   c[0] = a[1] * b[2] - a[2] * b[1];
   c[1] = a[2] * b[0] - a[0] * b[2];

@@ -75,6 +75,9 @@ void om_wedge02(int nt, const int *verts6, const double *coeffs6,
                 const double *f, const double *a, double *res);
 void om_wedge11(int nt, const int *edges3, const double *coeffs3,
                 const double *a, const double *b, double *res);
+void om_wedge_pd01_mat(int nv, int ne, int nt, const int *v0, const int *v1, const int *e1,
+                       const int *e2, const int *D_v0, const int *D_v1, const double *dl,
+                       om_csc *M);
 void om_wedge_dd01_mat(int nv, int ne, int nt, const int *D_v0, const int *D_v1,
                        const double *dual_length, om_csc *M);
 void om_derivbound(int k, int nv, int ne, int nt, const int *v0, const int *v1,

@@ -360,6 +360,21 @@ class Dual:
                                 _p(self.D_v1), _p(self.dual_length), ctypes.byref(s))
         return CSC._take(s)
 
+    def wedge_pd01_mat(self) -> CSC:
+        s = _OCSC()
+        m = self.m
+        lib().om_wedge_pd01_mat(c_int(self.nv), c_int(self.ne), c_int(self.nt), _p(m.v0), _p(m.v1), _p(m.e1),
+                                _p(m.e2), _p(self.D_v0), _p(self.D_v1), _p(self.dual_length), ctypes.byref(s))
+        return CSC._take(s)
+
+    def wedge_pd01(self, g, f):
+        """dec_wedge_product_pd(0,1,sd)(g,f) = (m*g) .* f, g primal 0-form, f dual 1-form (ne)."""
+        return self.wedge_pd01_mat().matvec(g) * f
+
+    def wedge_dp10(self, f, g):
+        """dec_wedge_product_dp(1,0,sd)(f,g) = f .* (m*g)."""
+        return f * self.wedge_pd01_mat().matvec(g)
+
     def wedge_dd01(self, f, g):
         """dec_wedge_product_dd(0,1,sd)(f,g) = (m*f) .* g, f dual 0-form (nt), g dual 1-form (ne)."""
         return self.wedge_dd01_mat().matvec(f) * g

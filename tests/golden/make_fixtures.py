@@ -33,6 +33,7 @@ def outputs(mesh, circumcenter=True):
         "V": V, "E": E, "E2": E2, "T": T, "DT": DT, "DE": DE,
         "wedge01": d.wedge01(V, E), "wedge11": d.wedge11(E, E2), "wedge02": d.wedge02(V, T),
         "wedge_dd01": d.wedge_dd01(DT, DE), "wedge_dd10": d.wedge_dd10(DE, DT),
+        "wedge_pd01": d.wedge_pd01(V, DE), "wedge_dp10": d.wedge_dp10(DE, V),
         "d0": d.differential(0).matvec(V), "d1": d.differential(1).matvec(E),
         "boundary1": d.boundary(1).matvec(E), "boundary2": d.boundary(2).matvec(T),
         "dual_d0": d.dual_derivative(0).matvec(DT), "dual_d1": d.dual_derivative(1).matvec(DE),

@@ -13,12 +13,13 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
-OPS = ["wedge01", "wedge11", "wedge02", "wedge_dd01", "wedge_dd10", "d0", "d1", "boundary1", "boundary2",
+OPS = ["wedge01", "wedge11", "wedge02", "wedge_dd01", "wedge_dd10", "wedge_pd01", "wedge_dp10", "d0", "d1",
+       "boundary1", "boundary2",
        "dual_d0", "dual_d1", "hodge0", "hodge1", "hodge2", "inv_hodge0", "inv_hodge1", "inv_hodge2",
        "geo_hodge1", "laplacian0", "laplacian0_geometric"]
 EXACT_ALWAYS = {"wedge01", "d0", "d1", "boundary1", "boundary2", "dual_d0", "dual_d1"}
 # on barycentric duals the center formula is shared: these agree bit for bit too
-EXACT_BARY = {"wedge11", "wedge_dd01", "wedge_dd10", "hodge0", "hodge1", "hodge2", "geo_hodge1", "laplacian0",
+EXACT_BARY = {"wedge11", "wedge_dd01", "wedge_dd10", "wedge_pd01", "wedge_dp10", "hodge0", "hodge1", "hodge2", "geo_hodge1", "laplacian0",
               "laplacian0_geometric"}
 
 
@@ -72,6 +73,8 @@ def test_hip_matches_fixture(cs, path):
         "wedge02": cs.dec_wedge_product(0, 2, gm)(dv("V"), dv("T")),
         "wedge_dd01": cs.dec_wedge_product_dd(0, 1, gm)(dv("DT"), dv("DE")),
         "wedge_dd10": cs.dec_wedge_product_dd(1, 0, gm)(dv("DE"), dv("DT")),
+        "wedge_pd01": cs.dec_wedge_product_pd(0, 1, gm)(dv("V"), dv("DE")),
+        "wedge_dp10": cs.dec_wedge_product_dp(1, 0, gm)(dv("DE"), dv("V")),
         "d0": cs.dec_differential(0, gm) @ dv("V"), "d1": cs.dec_differential(1, gm) @ dv("E"),
         "boundary1": cs.dec_boundary(1, gm) @ dv("E"), "boundary2": cs.dec_boundary(2, gm) @ dv("T"),
         "dual_d0": cs.dec_dual_derivative(0, gm) @ dv("DT"), "dual_d1": cs.dec_dual_derivative(1, gm) @ dv("DE"),

@@ -183,6 +183,34 @@ def test_wedge_dd(case, O, cs, dtype):
         assert np.array_equal(r01, ref01) and np.array_equal(r10, ref10)
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_wedge_pd(case, O, cs, dtype):
+    """dec_wedge_product_pd(0,1) / dec_wedge_product_dp(1,0) (FastDEC.jl:327-387)."""
+    import torch
+
+    name, od, gm = case
+    td = torch.float64 if dtype == "f64" else torch.float32
+    tol = F64_TOL if dtype == "f64" else F32_TOL
+    g = O.uniform_form(61, od.nv)
+    f = O.uniform_form(62, od.ne)
+    pd = cs.dec_wedge_product_pd(0, 1, gm, cast_float=td)
+    dp = cs.dec_wedge_product_dp(1, 0, gm, cast_float=td)
+    rpd = pd(to_dev(torch, g, td), to_dev(torch, f, td)).cpu().numpy()
+    rdp = dp(to_dev(torch, f, td), to_dev(torch, g, td)).cpu().numpy()
+    ref = od.wedge_pd01(g, f)
+    assert rel(rpd, ref) <= tol and rel(rdp, ref) <= tol
+    assert np.array_equal(rpd, rdp)                   # f .* (M g) == (M g) .* f
+    M = od.wedge_pd01_mat()
+    rp, ci, vv = pd.operator.csr()
+    orp, oci, ovv = M.csr()
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    assert rel(vv, ovv) <= F64_TOL
+    if exact_case(name):
+        assert np.array_equal(vv, ovv)
+        if dtype == "f64":
+            assert np.array_equal(rpd, ref)
+
+
 def csr_of(O, A):
     rp, ci, vv = A.csr()
     return rp, ci, vv
@@ -356,7 +384,9 @@ def test_apply_many_matches_single_applies(cs, dtype):
            cs.dec_hodge_star(1, m, cs.GeometricHodge, cast_float=td),
            cs.dec_inv_hodge_star(2, m, cs.DiagonalHodge, cast_float=td),
            cs.fast_laplace_beltrami(m, cs.DiagonalHodge, cast_float=td),
-           cs.avg01_mat(m, cast_float=td)]
+           cs.avg01_mat(m, cast_float=td),
+           cs.dec_wedge_product_pd(0, 1, m, cast_float=td).operator,
+           cs.dec_wedge_product_dp(1, 0, m, cast_float=td).operator]
     items, seed = [], 40
     for k, op in enumerate(ops):
         x = cs.fill_uniform(torch.empty(op.cols + (1 if k == 1 else 0), dtype=td, device="cuda"), seed)

@@ -328,3 +328,59 @@ def test_coarse_inverse_solves_consistent_systems(O):
     z = Z @ b
     assert np.linalg.norm(L.matvec(z) - b) < 1e-12 * np.linalg.norm(b)
     assert abs(z.mean()) < 1e-14
+
+
+def _constant_primal_form(d, X):
+    """eval_constant_primal_form (src/DiscreteExteriorCalculus.jl:2179-2184):
+    X · (p[∂v0] - p[∂v1]) times the edge's orientation sign."""
+    m = d.m
+    sign = np.where(d.eo > 0, 1.0, -1.0)
+    return (m.p[m.v0] - m.p[m.v1]) @ np.asarray(X, float) * sign
+
+
+@pytest.mark.parametrize("compress", [False, True])
+def test_primal_dual_wedge_reference_cases(O, compress):
+    """test/Operators.jl:361-414 on flat triangulated grids: a constant 0-form
+    g = 5 scales ⋆f, Λ10(f, g) == Λ01(g, f), and for a = x + 4y,
+    Λ10(⋆da, a) matches ⋆(-h ∧ dx + g ∧ dy) on interior edges (atol 1e-10),
+    with g = 4x + 16y, h = -x - 4y."""
+    dx = 1 / 8
+    grid = O.triangulated_grid(1, 1, dx, math.sqrt(3) / 2 * dx, compress=compress)
+    d = O.Dual(grid)
+    H1 = d.geo_hodge1()
+    s2 = 1 / math.sqrt(2)
+    f = H1.matvec(_constant_primal_form(d, (s2, s2, 0)))
+    g = np.full(d.nv, 5.0)
+    l10, l01 = d.wedge_dp10(f, g), d.wedge_pd01(g, f)
+    ref = H1.matvec(_constant_primal_form(d, (5 * s2, 5 * s2, 0)))
+    assert np.allclose(l10, ref, rtol=1.5e-8, atol=0)
+    assert np.array_equal(l10, l01)
+    # linear case on interior edges
+    m = d.m
+    ntri = np.bincount(np.concatenate([m.e0, m.e1, m.e2]), minlength=d.ne)
+    interior = ntri == 2
+    assert interior.any()
+    x, y = m.p[:, 0], m.p[:, 1]
+    a = x + 4 * y
+    f = H1.matvec(d.differential(0).matvec(a))
+    gg, hh = 4 * x + 16 * y, -x - 4 * y
+    DX = _constant_primal_form(d, (1, 0, 0))
+    DY = _constant_primal_form(d, (0, 1, 0))
+    analytic = H1.matvec(-d.wedge01(hh, DX) + d.wedge01(gg, DY))
+    got = d.wedge_dp10(f, a)
+    assert np.max(np.abs(got[interior] - analytic[interior])) <= 1e-10
+
+
+def test_primal_dual_wedge_matrix_structure(O):
+    """Rows of wedge_pd_01_mat sum to one (the weights of an edge's elementary
+    duals sum to one, 5/12 + 5/12 + 2/12 = 1); interior rows hold four
+    entries, boundary rows three, all positive."""
+    for mesh in (O.icosphere(2), O.triangulated_grid(1, 1, 1 / 4, math.sqrt(3) / 8, compress=False)):
+        d = O.Dual(mesh)
+        rp, ci, vv = d.wedge_pd01_mat().csr()
+        rs = np.add.reduceat(vv, rp[:-1])
+        assert np.max(np.abs(rs - 1)) <= 4e-16
+        m = d.m
+        ntri = np.bincount(np.concatenate([m.e0, m.e1, m.e2]), minlength=d.ne)
+        assert np.array_equal(np.diff(rp), np.where(ntri == 2, 4, 3))
+        assert (vv > 0).all()
