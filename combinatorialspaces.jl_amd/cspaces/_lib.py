@@ -42,6 +42,7 @@ SIGNATURES = {
     "dec_op_shape": (c_int, [P, PI64, PI64, PI64]),
     "dec_op_apply": (c_int, [P, P, P, P, P]),
     "dec_op_apply_many": (c_int, [c_int32, P, P, P, P, P]),
+    "dec_op_solve_info": (c_int, [P, PI32, PF64, P]),
     "dec_op_export_csr": (c_int, [P, PI64, P, P, P]),
     "dec_op_export_tables": (c_int, [P, P, P]),
     "dec_op_destroy": (c_int, [P]),

@@ -238,6 +238,13 @@ class Operator:
     def __matmul__(self, x):
         return self.apply(x)
 
+    def solve_info(self, stream=None):
+        """(iterations, ‖x - ⋆1 z‖/‖x‖) of the last apply of the geometric
+        ⋆1⁻¹ solve (synchronises the stream)."""
+        it, rr = c_int32(), ctypes.c_double()
+        check(lib().dec_op_solve_info(self._h, ctypes.byref(it), ctypes.byref(rr), _stream(stream)))
+        return it.value, rr.value
+
     def csr(self):
         """(rowptr, col, val) of the operator's matrix (float64 values)."""
         nnz = c_int64()
@@ -345,7 +352,9 @@ def dec_hodge_star(n, sd: Mesh, hodge=GeometricHodge, backend=HIP, cast_float=No
 
 
 def dec_inv_hodge_star(n, sd: Mesh, hodge=GeometricHodge, backend=HIP, cast_float=None) -> Operator:
-    """dec_inv_hodge_star(n, sd, hodge) (FastDEC.jl:721-743)."""
+    """dec_inv_hodge_star(n, sd, hodge) (FastDEC.jl:721-748).  With
+    GeometricHodge and n = 1 the operator is the solve x -> (-⋆1) \\ x of
+    FastDEC.jl:744-748, run as a Krylov iteration on the device."""
     _check_backend(backend)
     return Operator(sd, L.DEC_OP_INV_HODGE, n, hodge=hodge, cast_float=cast_float)
 

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <unordered_map>
 #include <utility>
 
 #include "dec_internal.h"
@@ -31,7 +32,8 @@ enum Impl {
   I_DIAG,  // diagonal ⋆, ⋆⁻¹
   I_ELL,   // general sparse: geometric ⋆1, ∇²
   I_AVG,   // avg₀₁
-  I_WPD    // primal-dual wedge 0-1 / dual-primal 1-0: (M g) .* f, M = wedge_pd_01_mat
+  I_WPD,   // primal-dual wedge 0-1 / dual-primal 1-0: (M g) .* f, M = wedge_pd_01_mat
+  I_GINV   // geometric ⋆1⁻¹: a solve with ⋆1 (Jacobi-preconditioned CG on the device)
 };
 
 }  // namespace
@@ -51,6 +53,13 @@ struct dec_op_s {
   dec::Ell<double> ell64;
   dec::Ell<float> ell32;
   dec::HostCsr csr;  // host copy for export (sparse-matrix operators)
+  // geometric ⋆1⁻¹: the Jacobi scaling 1/diag(⋆1), CG vectors, reduction
+  // partials and device-resident scalars (mutable: one apply at a time)
+  DBuf<unsigned char> dinv, cr, cp, cq, cz, crh, ct;
+  DBuf<double> cpart, cscal, czero;
+  int max_iter = 0;
+  double rtol = 0.0;
+  bool symmetric = true;  // CG when ⋆1 is symmetric, BiCGSTAB otherwise
 };
 
 namespace {
@@ -337,6 +346,198 @@ template <int W, class T>
 __global__ void k_wedge_pd(int ne, int width, const int *__restrict__ col, const T *__restrict__ val,
                            const T *__restrict__ g, const T *__restrict__ f, T *__restrict__ res) {
   b_wedge_pd<W, T>(static_cast<int>(blockIdx.x), ne, width, col, val, g, f, res);
+}
+
+// ---- geometric ⋆1⁻¹: a Krylov solve with Jacobi scaling, scalars on the device ----
+// The reference factorizes -⋆1 (FastDEC.jl:745-748; GMRES on CUDA,
+// ext/CombinatorialSpacesCUDAExt.jl:93-96).  ⋆1 is symmetric positive definite
+// on a circumcentric dual (a mass matrix) and is solved by conjugate
+// gradients; on a barycentric dual it is not symmetric and BiCGSTAB
+// (van der Vorst 1992, right Jacobi preconditioning) is used.  All scalars
+// stay in device memory; every kernel of an iteration returns at once when
+// the stop flag is set, so a fixed launch sequence needs no host round trip.
+constexpr int kCgBlocks = 256;
+constexpr int kCgChunk = 8;  // iterations between host reads of the stop flag
+enum { CS_BB = 0, CS_RR, CS_DONE, CS_ITER, CS_RZ, CS_PQ, CS_ALPHA, CS_BETA, CS_OMEGA, CS_RHO, CS_COUNT };
+
+// part[blk] = Σ a·b, part[kCgBlocks + blk] = Σ c·d (c, d may be NULL)
+template <class T>
+__global__ void k_cg_dots(int n, const T *__restrict__ a, const T *__restrict__ b, const T *__restrict__ c,
+                          const T *__restrict__ d, double *__restrict__ part, const double *__restrict__ sc) {
+  if (sc[CS_DONE] != 0.0) return;
+  __shared__ double s1[256], s2[256];
+  double x = 0.0, y = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kCgBlocks * 256) {
+    x += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+    if (d) y += static_cast<double>(c[i]) * static_cast<double>(d[i]);
+  }
+  s1[threadIdx.x] = x;
+  s2[threadIdx.x] = y;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] += s2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1[0];
+    part[kCgBlocks + blockIdx.x] = s2[0];
+  }
+}
+
+enum { PH_CG_START = 0, PH_CG_ALPHA, PH_CG_BETA, PH_BI_START, PH_BI_ALPHA, PH_BI_OMEGA, PH_BI_BETA };
+
+// One thread sums the two partial arrays and advances the recurrences.
+__global__ void k_cg_scalars(int phase, const double *__restrict__ part, double *__restrict__ sc, double rtol) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (phase != PH_CG_START && phase != PH_BI_START && sc[CS_DONE] != 0.0) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int j = 0; j < kCgBlocks; j++) {
+    s1 += part[j];
+    s2 += part[kCgBlocks + j];
+  }
+  double tol2 = rtol * rtol;
+  switch (phase) {
+    case PH_CG_START:  // s1 = r·z, s2 = r·r with r = b
+    case PH_BI_START:  // s1 = r̂·r, s2 = r·r with r̂ = r = b
+      sc[CS_RZ] = sc[CS_RHO] = s1;
+      sc[CS_BB] = sc[CS_RR] = s2;
+      sc[CS_ALPHA] = sc[CS_OMEGA] = 1.0;
+      sc[CS_BETA] = 0.0;
+      sc[CS_ITER] = 0.0;
+      sc[CS_DONE] = s2 == 0.0 ? 1.0 : 0.0;
+      break;
+    case PH_CG_ALPHA:  // s1 = p·Ap
+      sc[CS_PQ] = s1;
+      if (s1 != 0.0) sc[CS_ALPHA] = sc[CS_RZ] / s1;
+      else sc[CS_DONE] = 1.0;
+      break;
+    case PH_CG_BETA:  // s1 = r·z, s2 = r·r after the update
+      sc[CS_BETA] = sc[CS_RZ] != 0.0 ? s1 / sc[CS_RZ] : 0.0;
+      sc[CS_RZ] = s1;
+      sc[CS_RR] = s2;
+      sc[CS_ITER] += 1.0;
+      if (s2 <= tol2 * sc[CS_BB]) sc[CS_DONE] = 1.0;
+      break;
+    case PH_BI_ALPHA:  // s1 = r̂·v
+      if (s1 != 0.0) sc[CS_ALPHA] = sc[CS_RHO] / s1;
+      else sc[CS_DONE] = 1.0;
+      break;
+    case PH_BI_OMEGA:  // s1 = t·s, s2 = t·t; t = 0 means s = 0: x is the solution
+      if (s2 != 0.0 && s1 != 0.0) {
+        sc[CS_OMEGA] = s1 / s2;
+      } else {
+        sc[CS_OMEGA] = 0.0;
+        sc[CS_DONE] = 1.0;
+      }
+      break;
+    case PH_BI_BETA:  // s1 = r̂·r, s2 = r·r after the update
+      sc[CS_BETA] = (sc[CS_RHO] != 0.0 ? s1 / sc[CS_RHO] : 0.0) * (sc[CS_ALPHA] / sc[CS_OMEGA]);
+      sc[CS_RHO] = s1;
+      sc[CS_RR] = s2;
+      sc[CS_ITER] += 1.0;
+      if (s2 <= tol2 * sc[CS_BB] || s1 == 0.0) sc[CS_DONE] = 1.0;
+      break;
+  }
+}
+
+// x = 0, r = b, z = D⁻¹ r (CG: p = z; BiCGSTAB: r̂ = b, p = v = 0)
+template <class T>
+__global__ void k_cg_init(int n, int bicg, const T *__restrict__ b, const T *__restrict__ dinv, T *__restrict__ x,
+                          T *__restrict__ r, T *__restrict__ z, T *__restrict__ p, T *__restrict__ rh,
+                          T *__restrict__ v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T bi = b[i];
+  x[i] = static_cast<T>(0);
+  r[i] = bi;
+  T zi = dinv[i] * bi;
+  z[i] = zi;
+  if (bicg) {
+    rh[i] = bi;
+    p[i] = static_cast<T>(0);
+    v[i] = static_cast<T>(0);
+  } else {
+    p[i] = zi;
+  }
+}
+
+template <int W, class T>
+__global__ void k_cg_spmv(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
+                          const T *__restrict__ p, T *__restrict__ q, const double *__restrict__ sc) {
+  if (sc[CS_DONE] != 0.0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) q[i] = dec_ell::row<W, T, T>(n, width, i, col, val, p);
+}
+
+// CG: x += alpha p, r -= alpha q, z = D⁻¹ r
+template <class T>
+__global__ void k_cg_update(int n, const T *__restrict__ p, const T *__restrict__ q, const T *__restrict__ dinv,
+                            T *__restrict__ x, T *__restrict__ r, T *__restrict__ z, const double *__restrict__ sc) {
+  if (sc[CS_DONE] != 0.0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T a = static_cast<T>(sc[CS_ALPHA]);
+  x[i] = x[i] + a * p[i];
+  T ri = r[i] - a * q[i];
+  r[i] = ri;
+  z[i] = dinv[i] * ri;
+}
+
+// CG: p = z + beta p
+template <class T>
+__global__ void k_cg_dir(int n, const T *__restrict__ z, T *__restrict__ p, const double *__restrict__ sc) {
+  if (sc[CS_DONE] != 0.0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + static_cast<T>(sc[CS_BETA]) * p[i];
+}
+
+// BiCGSTAB: p = r + beta (p - omega v), y = D⁻¹ p
+template <class T>
+__global__ void k_bi_dir(int n, const T *__restrict__ r, const T *__restrict__ v, const T *__restrict__ dinv,
+                         T *__restrict__ p, T *__restrict__ y, const double *__restrict__ sc) {
+  if (sc[CS_DONE] != 0.0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T be = static_cast<T>(sc[CS_BETA]), om = static_cast<T>(sc[CS_OMEGA]);
+  T pi = r[i] + be * (p[i] - om * v[i]);
+  p[i] = pi;
+  y[i] = dinv[i] * pi;
+}
+
+// BiCGSTAB: x += alpha y, s = r - alpha v (kept in r), then yz = D⁻¹ s in
+// place of y
+template <class T>
+__global__ void k_bi_half(int n, T *__restrict__ yz, const T *__restrict__ v, const T *__restrict__ dinv,
+                          T *__restrict__ x, T *__restrict__ r, const double *__restrict__ sc) {
+  if (sc[CS_DONE] != 0.0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T a = static_cast<T>(sc[CS_ALPHA]);
+  x[i] = x[i] + a * yz[i];
+  T si = r[i] - a * v[i];
+  r[i] = si;
+  yz[i] = dinv[i] * si;
+}
+
+// BiCGSTAB: x += omega z, r = s - omega t
+template <class T>
+__global__ void k_bi_full(int n, const T *__restrict__ z, const T *__restrict__ t, T *__restrict__ x,
+                          T *__restrict__ r, const double *__restrict__ sc) {
+  if (sc[CS_DONE] != 0.0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T om = static_cast<T>(sc[CS_OMEGA]);
+  x[i] = x[i] + om * z[i];
+  r[i] = r[i] - om * t[i];
+}
+
+template <class T>
+__global__ void k_negate(int n, T *__restrict__ x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = -x[i];
 }
 
 // ---- coefficient precomputation (float64) ----
@@ -695,6 +896,73 @@ void csr_diag(const std::vector<double> &d, dec::HostCsr &h) {
   for (int i = 0; i < n; i++) h.col[i] = i;
 }
 
+inline void hip_status(hipError_t e, const char *what) {
+  if (e != hipSuccess) fail(DEC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// out = -⋆1⁻¹ x, the solve of ⋆1 z = x negated as dec_inv_hodge_star's
+// (-⋆1)⁻¹ x.  Iterations go out in chunks; between chunks the host reads the
+// device stop flag (one 8-byte copy and a stream synchronisation).  While the
+// stream is being captured into a graph the whole iteration budget is
+// recorded instead, and iterations past convergence return at once.
+template <class T>
+void ginv_apply(dec_op_s &op, const T *bvec, T *x, hipStream_t s) {
+  int n = static_cast<int>(op.rows);
+  const dec::Ell<T> *E;
+  if constexpr (sizeof(T) == 8) E = &op.ell64;
+  else E = &op.ell32;
+  auto P = [](DBuf<unsigned char> &v) { return reinterpret_cast<T *>(v.p); };
+  T *r = P(op.cr), *p = P(op.cp), *q = P(op.cq), *z = P(op.cz), *rh = P(op.crh), *t = P(op.ct);
+  const T *dinv = reinterpret_cast<const T *>(op.dinv.p);
+  const T *nul = nullptr;
+  double *part = op.cpart.p, *sc = op.cscal.p;
+  const bool bicg = !op.symmetric;
+  dim3 g(dec::nblocks(n)), blk(256), gd(kCgBlocks), one(1);
+  // the start reads the stop flag of czero (always clear), not of sc
+  hipLaunchKernelGGL(k_cg_init<T>, g, blk, 0, s, n, bicg ? 1 : 0, bvec, dinv, x, r, z, p, rh, q);
+  hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, bicg ? rh : r, bicg ? r : z, r, r, part, op.czero.p);
+  hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, bicg ? PH_BI_START : PH_CG_START, part, sc, op.rtol);
+  auto iteration = [&]() {
+    if (!bicg) {
+      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_cg_spmv<W, T>), g, blk, 0, s, n, E->width, E->col.p,
+                                                        E->val.p, p, q, sc));
+      hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, p, q, nul, nul, part, sc);
+      hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_CG_ALPHA, part, sc, op.rtol);
+      hipLaunchKernelGGL(k_cg_update<T>, g, blk, 0, s, n, p, q, dinv, x, r, z, sc);
+      hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, r, z, r, r, part, sc);
+      hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_CG_BETA, part, sc, op.rtol);
+      hipLaunchKernelGGL(k_cg_dir<T>, g, blk, 0, s, n, z, p, sc);
+    } else {
+      // q holds v = A y, z holds y and then D⁻¹ s, t holds A z
+      hipLaunchKernelGGL(k_bi_dir<T>, g, blk, 0, s, n, r, q, dinv, p, z, sc);
+      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_cg_spmv<W, T>), g, blk, 0, s, n, E->width, E->col.p,
+                                                        E->val.p, z, q, sc));
+      hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, rh, q, nul, nul, part, sc);
+      hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_BI_ALPHA, part, sc, op.rtol);
+      hipLaunchKernelGGL(k_bi_half<T>, g, blk, 0, s, n, z, q, dinv, x, r, sc);
+      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_cg_spmv<W, T>), g, blk, 0, s, n, E->width, E->col.p,
+                                                        E->val.p, z, t, sc));
+      hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, t, r, t, t, part, sc);
+      hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_BI_OMEGA, part, sc, op.rtol);
+      hipLaunchKernelGGL(k_bi_full<T>, g, blk, 0, s, n, z, t, x, r, sc);
+      hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, rh, r, r, r, part, sc);
+      hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_BI_BETA, part, sc, op.rtol);
+    }
+  };
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  hip_status(hipStreamIsCapturing(s, &cap), "hipStreamIsCapturing");
+  const int chunk = cap == hipStreamCaptureStatusNone ? kCgChunk : op.max_iter;
+  for (int done_it = 0; done_it < op.max_iter; done_it += chunk) {
+    for (int k = 0; k < chunk && done_it + k < op.max_iter; k++) iteration();
+    if (cap != hipStreamCaptureStatusNone) continue;
+    double flag = 0.0;
+    hip_status(hipMemcpyAsync(&flag, sc + CS_DONE, sizeof(double), hipMemcpyDeviceToHost, s), "stop flag copy");
+    hip_status(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (flag != 0.0) break;
+  }
+  hipLaunchKernelGGL(k_negate<T>, g, blk, 0, s, n, x);
+}
+
 template <class T>
 void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream_t s) {
   const T *x = static_cast<const T *>(op.swap ? yv : xv);
@@ -746,6 +1014,9 @@ void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream
       break;
     case I_AVG:
       hipLaunchKernelGGL(k_avg01<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, x, out);
+      break;
+    case I_GINV:
+      ginv_apply<T>(op, x, out, s);
       break;
     case I_WPD: {
       const dec::Ell<T> *E;
@@ -1184,14 +1455,61 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
       if (dm < 0 || dm > 2) fail(DEC_ERR_UNSUPPORTED, "dec_hodge_star: n must be 0, 1 or 2 on a 2-D complex");
       bool inv = kind == DEC_OP_INV_HODGE;
       if (hodge == DEC_GEOMETRIC_HODGE && dm == 1) {
-        if (inv)
-          fail(DEC_ERR_UNSUPPORTED,
-               "dec_inv_hodge_star(1, GeometricHodge()) is a linear solve; use DiagonalHodge() for an operator");
-        op->impl = I_ELL;
+        op->impl = inv ? I_GINV : I_ELL;
         op->rows = op->cols = ne;
         dec::geo_hodge1(m, op->csr);
         if (f64) dec::upload_ell(op->csr, op->ell64);
         else dec::upload_ell(op->csr, op->ell32);
+        if (inv) {
+          // Jacobi scaling from the diagonal of ⋆1; CG work vectors
+          std::vector<double> dg(ne, 0.0);
+          for (int r = 0; r < ne; r++)
+            for (int k = op->csr.ptr[r]; k < op->csr.ptr[r + 1]; k++)
+              if (op->csr.col[k] == r) dg[r] = op->csr.val[k];
+          for (int r = 0; r < ne; r++)
+            if (!(dg[r] > 0.0)) fail(DEC_ERR_UNSUPPORTED, "geometric ⋆1⁻¹: ⋆1 has a non-positive diagonal entry");
+          size_t es = f64 ? 8 : 4;
+          if (f64) {
+            std::vector<double> di(ne);
+            for (int r = 0; r < ne; r++) di[r] = 1.0 / dg[r];
+            op->dinv.alloc(es * ne);
+            if (ne) dec::h2d(op->dinv.p, di.data(), es * ne);
+          } else {
+            std::vector<float> di(ne);
+            for (int r = 0; r < ne; r++) di[r] = static_cast<float>(1.0 / dg[r]);
+            op->dinv.alloc(es * ne);
+            if (ne) dec::h2d(op->dinv.p, di.data(), es * ne);
+          }
+          // symmetric to rounding (circumcentric duals): CG; else BiCGSTAB
+          double amax = 0.0, asym = 0.0;
+          {
+            std::vector<std::unordered_map<int, double>> rowmap(ne);
+            for (int r = 0; r < ne; r++)
+              for (int k = op->csr.ptr[r]; k < op->csr.ptr[r + 1]; k++) {
+                rowmap[r][op->csr.col[k]] += op->csr.val[k];
+                amax = std::max(amax, std::fabs(op->csr.val[k]));
+              }
+            for (int r = 0; r < ne; r++)
+              for (auto &kv : rowmap[r]) {
+                auto it = rowmap[kv.first].find(r);
+                double other = it == rowmap[kv.first].end() ? 0.0 : it->second;
+                asym = std::max(asym, std::fabs(kv.second - other));
+              }
+          }
+          op->symmetric = asym <= 1e-12 * amax;
+          for (auto *v : {&op->cr, &op->cp, &op->cq, &op->cz, &op->crh, &op->ct}) v->alloc(es * ne);
+          op->cpart.alloc(2 * kCgBlocks);
+          op->cscal.alloc(CS_COUNT);
+          op->czero.alloc(CS_COUNT);  // a scalar block whose stop flag is always clear
+          {
+            std::vector<double> init(CS_COUNT, 0.0);
+            dec::h2d(op->czero.p, init.data(), sizeof(double) * CS_COUNT);
+            dec::h2d(op->cscal.p, init.data(), sizeof(double) * CS_COUNT);
+          }
+          op->max_iter = f64 ? 200 : 100;
+          op->rtol = f64 ? 1e-14 : 1e-6;
+          op->csr = dec::HostCsr();  // the inverse has no sparse matrix to export
+        }
         break;
       }
       op->impl = I_DIAG;
@@ -1262,6 +1580,19 @@ int dec_op_shape(dec_op op, int64_t *rows, int64_t *cols, int64_t *cols2) {
   DEC_CATCH
 }
 
+int dec_op_solve_info(dec_op op, int32_t *iterations, double *relres, void *stream) {
+  DEC_TRY
+  if (!op) fail(DEC_ERR_ARG, "dec_op_solve_info: NULL handle");
+  if (op->impl != I_GINV) fail(DEC_ERR_ARG, "dec_op_solve_info: the operator is not a solve");
+  double sc[CS_COUNT];
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hip_status(hipMemcpyAsync(sc, op->cscal.p, sizeof(sc), hipMemcpyDeviceToHost, s), "solve info copy");
+  hip_status(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (iterations) *iterations = static_cast<int32_t>(sc[CS_ITER]);
+  if (relres) *relres = sc[CS_BB] > 0.0 ? std::sqrt(sc[CS_RR] / sc[CS_BB]) : 0.0;
+  DEC_CATCH
+}
+
 int dec_op_apply(dec_op op, const void *x, const void *y, void *out, void *stream) {
   DEC_TRY
   if (!op) fail(DEC_ERR_ARG, "dec_op_apply: NULL handle");
@@ -1326,6 +1657,8 @@ int dec_op_apply_many(int32_t count, const dec_op *ops, const void *const *x, co
   for (int q = 0; q < count; q++) {
     dec_op_s *op = ops[q];
     if (!op) fail(DEC_ERR_ARG, "dec_op_apply_many: NULL operator");
+    if (op->impl == I_GINV)
+      fail(DEC_ERR_UNSUPPORTED, "dec_op_apply_many: the geometric ⋆1⁻¹ is a solve; apply it with dec_op_apply");
     if (dtype < 0) dtype = op->dtype;
     if (op->dtype != dtype) fail(DEC_ERR_ARG, "dec_op_apply_many: operators of different dtypes");
     if (op->rows > 0 && !out[q]) fail(DEC_ERR_ARG, "dec_op_apply_many: NULL output");

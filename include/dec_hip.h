@@ -110,8 +110,14 @@ int dec_mesh_destroy(dec_mesh m);
  *  DEC_OP_BOUNDARY n=1,2: dec_boundary (FastDEC.jl:433-437, CUDA ext :30-36).
  *  DEC_OP_DUAL_DERIVATIVE n=0,1: dec_dual_derivative (FastDEC.jl:443-453, CUDA ext :42-48).
  *  DEC_OP_HODGE n=0,1,2 with hodge: dec_hodge_star (FastDEC.jl:641-716, CUDA ext :70-77).
- *  DEC_OP_INV_HODGE n=0,1,2 (diagonal; geometric ⋆1⁻¹ is a solve and is
- *     reported as DEC_ERR_UNSUPPORTED): dec_inv_hodge_star (FastDEC.jl:721-743, CUDA ext :83-91).
+ *  DEC_OP_INV_HODGE n=0,1,2: dec_inv_hodge_star (FastDEC.jl:721-743, CUDA ext :83-91).
+ *     Geometric ⋆1⁻¹ (FastDEC.jl:744-748, CUDA ext :93-96) is a solve: apply
+ *     returns (-⋆1)⁻¹ x from a Jacobi-scaled Krylov iteration on the device
+ *     (CG where ⋆1 is symmetric, BiCGSTAB otherwise) stopped at
+ *     ‖x - ⋆1 z‖ ≤ 1e-14 ‖x‖ (float64; 1e-6 in float32).  Apply synchronises
+ *     the stream every 8 iterations to read the stop flag, except while the
+ *     stream is captured into a graph (then the whole iteration budget is
+ *     recorded).  The operator owns work vectors: one apply at a time.
  *  DEC_OP_LAPLACIAN0 (hodge selects ⋆1): fast_laplace_beltrami
  *     (test/Multigrid.jl:46-48) = ∇²(0, sd) form (DiscreteExteriorCalculus.jl:1887-1890).
  *  DEC_OP_AVG01: avg₀₁_mat (FastDEC.jl:874-892).
@@ -123,10 +129,15 @@ int dec_op_create(dec_mesh m, int32_t kind, int32_t deg_m, int32_t deg_n,
 int dec_op_shape(dec_op op, int64_t *rows, int64_t *cols, int64_t *cols2);
 /* out = op(x[, y]).  Unary operators ignore y.  Same dtype as the operator. */
 int dec_op_apply(dec_op op, const void *x, const void *y, void *out, void *stream);
+/* For the geometric ⋆1⁻¹ solve: iterations and ‖x - ⋆1 z‖/‖x‖ of the last
+ * apply on `stream` (synchronises it).  DEC_ERR_ARG for other operators.
+ * No counterpart in the reference, whose solve is a direct factorisation. */
+int dec_op_solve_info(dec_op op, int32_t *iterations, double *relres, void *stream);
 /* Several applications in one launch: out[i] = ops[i](x[i], y[i]) for
  * i < count (y[i] ignored by one-form operators).  All operators must share
  * a dtype.  Results are bit-identical to count dec_op_apply calls; the single
- * grid removes the per-launch ramp-up and drain.  No counterpart in the
+ * grid removes the per-launch ramp-up and drain.  The geometric ⋆1⁻¹
+ * solve is refused (DEC_ERR_UNSUPPORTED).  No counterpart in the
  * reference (each FastDEC operator is its own kernel, FastDEC.jl:255-289). */
 int dec_op_apply_many(int32_t count, const dec_op *ops, const void *const *x, const void *const *y,
                       void *const *out, void *stream);

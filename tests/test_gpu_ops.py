@@ -275,9 +275,46 @@ def test_hodge(case, O, cs, dtype):
         assert np.array_equal(rp, orp) and np.array_equal(ci, oci) and np.array_equal(vv, ovv)
     x = O.uniform_form(45, od.ne)
     assert rel(gh(to_dev(torch, x, td)).cpu().numpy(), H.matvec(x)) <= tol
-    # geometric ⋆1⁻¹ is a solve, not an operator
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_inv_geometric_hodge1(case, O, cs, dtype):
+    """Geometric ⋆1⁻¹ is x -> (-⋆1) \\ x (FastDEC.jl:744-748).  The expected
+    value is a sparse LU solve of the oracle's ⋆1 (itself pinned by the golden
+    fixtures); test/Operators.jl:215-220 compares with rtol = 1e-12."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    import torch
+
+    name, od, gm = case
+    td = torch.float64 if dtype == "f64" else torch.float32
+    rp, ci, vv = od.geo_hodge1().csr()
+    H = sp.csr_matrix((vv, ci, rp), shape=(od.ne, od.ne)).tocsc()
+    lu = sla.splu(-H)
+    inv = cs.dec_inv_hodge_star(1, gm, cs.GeometricHodge, cast_float=td)
+    gh = cs.dec_hodge_star(1, gm, cs.GeometricHodge, cast_float=td)
+    assert inv.shape == (od.ne, od.ne)
+    for seed in (46, 47):              # a second solve reuses the work vectors
+        x = O.uniform_form(seed, od.ne)
+        ref = lu.solve(x)
+        assert np.linalg.norm(H @ ref + x) <= 1e-13 * np.linalg.norm(x)
+        z = inv(to_dev(torch, x, td))
+        it, relres = inv.solve_info()
+        if dtype == "f64":
+            assert relres <= 1e-14 and 0 < it < 200, (it, relres)
+            assert rel(z.cpu().numpy(), ref) <= F64_TOL
+            assert rel(gh(z).cpu().numpy(), -x) <= 1e-13      # ⋆1 (⋆1⁻¹ x) = -(-x)
+        else:
+            assert relres <= 1e-6 and 0 < it < 100, (it, relres)
+            assert rel(z.cpu().numpy(), ref) <= F32_TOL
+    zero = torch.zeros(od.ne, dtype=td, device="cuda")
+    assert torch.count_nonzero(inv(zero)).item() == 0
     with pytest.raises(cs.DecError):
-        cs.dec_inv_hodge_star(1, gm, cs.GeometricHodge)
+        inv.csr()                      # a solve has no matrix to export
+    with pytest.raises(cs.DecError):
+        cs.apply_many([(inv, zero, None, torch.empty_like(zero))])
+    with pytest.raises(cs.DecError):
+        gh.solve_info()
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
