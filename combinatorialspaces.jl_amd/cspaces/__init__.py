@@ -6,8 +6,8 @@ from .fastdec import (  # noqa: F401
     HIP, Barycenter, Circumcenter, DiagonalHodge, GeometricHodge, Mesh, Operator, apply_many, avg01_mat,
     dec_boundary, dec_c_wedge_product_, dec_differential, dec_dual_derivative, dec_hodge_star,
     dec_inv_hodge_star, dec_wedge_product, dec_wedge_product_dd, dec_wedge_product_dp, dec_wedge_product_pd,
-    dualize, fast_laplace_beltrami,
-    fill_uniform, icosphere, mesh_from_arrays, subdivision, triangulated_grid)
+    dual_laplacian, dualize, fast_laplace_beltrami, fill_uniform, flat_sharp_mat, icosphere,
+    interior_product_dd, lie_derivative_dd, mesh_from_arrays, subdivision, triangulated_grid, ℒ_dd)
 from .multigrid import (  # noqa: F401
     BinarySubdivision, MGData, MultigridData, cg, dec_Delta_inv, full_multigrid, gauss_seidel, jacobi,
     multigrid_mu_cycles, multigrid_vcycles, multigrid_wcycles, solve)

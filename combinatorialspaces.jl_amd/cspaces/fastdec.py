@@ -16,7 +16,7 @@ from ctypes import c_int32, c_int64, c_void_p
 import numpy as np
 
 from . import _lib as L
-from ._lib import check, lib, ptr
+from ._lib import DecError, check, lib, ptr
 
 HIP = "HIP"
 
@@ -301,30 +301,66 @@ def dec_wedge_product_dd(m, n, sd: Mesh, backend=HIP, cast_float=None):
     return fn
 
 
-def dec_wedge_product_pd(m, n, sd: Mesh, backend=HIP, cast_float=None):
-    """dec_wedge_product_pd(0, 1, sd) (FastDEC.jl:379-382): (g, f) -> (M g) .* f
-    for a primal 0-form g and a dual 1-form f, M = wedge_pd_01_mat."""
-    _check_backend(backend)
-    if (m, n) != (0, 1):
-        raise DecError(L.DEC_ERR_UNSUPPORTED, f"Unsupported combination of degrees {m} and {n}. The primal-dual "
-                       "wedges supported here are 0-1 (and dual-primal 1-0).")
-    op = Operator(sd, L.DEC_OP_WEDGE_PD, 0, 1, cast_float=cast_float)
-    fn = lambda g, f, out=None, stream=None: op.apply(g, f, out=out, stream=stream)  # noqa: E731
+def _binary(op):
+    fn = lambda a, b, out=None, stream=None: op.apply(a, b, out=out, stream=stream)  # noqa: E731
     fn.operator = op
     return fn
+
+
+def dec_wedge_product_pd(m, n, sd: Mesh, backend=HIP, cast_float=None):
+    """dec_wedge_product_pd (FastDEC.jl:379-393): primal m-form ∧ dual n-form.
+    (0,1): (g, f) -> (M g) .* f, g primal 0-form, f dual 1-form, M = wedge_pd_01_mat.
+    (1,1): (f, g) -> ∧11(f, ♭♯ g), f primal 1-form, g dual 1-form."""
+    _check_backend(backend)
+    if (m, n) not in ((0, 1), (1, 1)):
+        raise DecError(L.DEC_ERR_UNSUPPORTED, f"Unsupported combination of degrees {m} and {n}. Ensure that their "
+                       "sum is not greater than the degree of the complex. The currently supported primal-dual "
+                       "wedges are 0-1, 1-0, and 1-1.")
+    return _binary(Operator(sd, L.DEC_OP_WEDGE_PD, m, n, cast_float=cast_float))
 
 
 def dec_wedge_product_dp(m, n, sd: Mesh, backend=HIP, cast_float=None):
-    """dec_wedge_product_dp(1, 0, sd) (FastDEC.jl:374-377): (f, g) -> f .* (M g)
-    for a dual 1-form f and a primal 0-form g."""
+    """dec_wedge_product_dp (FastDEC.jl:374-377, 396-400): dual m-form ∧ primal n-form.
+    (1,0): (f, g) -> f .* (M g), f dual 1-form, g primal 0-form.
+    (1,1): (f, g) -> ∧11(♭♯ f, g), f dual 1-form, g primal 1-form."""
     _check_backend(backend)
-    if (m, n) != (1, 0):
-        raise DecError(L.DEC_ERR_UNSUPPORTED, f"Unsupported combination of degrees {m} and {n}. The dual-primal "
-                       "wedges supported here are 1-0 (and primal-dual 0-1).")
-    op = Operator(sd, L.DEC_OP_WEDGE_PD, 1, 0, cast_float=cast_float)
-    fn = lambda f, g, out=None, stream=None: op.apply(f, g, out=out, stream=stream)  # noqa: E731
-    fn.operator = op
-    return fn
+    if (m, n) not in ((1, 0), (1, 1)):
+        raise DecError(L.DEC_ERR_UNSUPPORTED, f"Unsupported combination of degrees {m} and {n}. Ensure that their "
+                       "sum is not greater than the degree of the complex. The currently supported dual-primal "
+                       "wedges are 0-1, 1-0, and 1-1.")
+    return _binary(Operator(sd, L.DEC_OP_WEDGE_DP, m, n, cast_float=cast_float))
+
+
+def flat_sharp_mat(sd: Mesh, backend=HIP, cast_float=None) -> Operator:
+    """♭♯_mat(sd) (DiscreteExteriorCalculus.jl:1985-1992): a dual 1-form made
+    primal by ♭(DPPFlat) ∘ ♯(LLSDDSharp); `op @ x`, `op.csr()`."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_FLAT_SHARP, cast_float=cast_float)
+
+
+def interior_product_dd(m, n, sd: Mesh, backend=HIP, cast_float=None):
+    """interior_product_dd (FastDEC.jl:758-786).
+    (1,1): (f, g) -> ⋆2 ∧pd11(⋆1⁻¹ g, f), dual 1-forms -> dual 0-form.
+    (1,2): (f, g) -> ⋆1 ♭♯ ((Λ01 ⋆0⁻¹ g) .* f), dual 1-form f, dual 2-form g."""
+    _check_backend(backend)
+    return _binary(Operator(sd, L.DEC_OP_INTERIOR_PRODUCT_DD, m, n, cast_float=cast_float))
+
+
+def lie_derivative_dd(m, n, sd: Mesh, backend=HIP, cast_float=None):
+    """ℒ_dd(1, 1, sd) (FastDEC.jl:788-803): (f, g) -> -(d̃0 ι11(f, g)) - ι12(f, d̃1 g)."""
+    _check_backend(backend)
+    return _binary(Operator(sd, L.DEC_OP_LIE_DD, m, n, cast_float=cast_float))
+
+
+ℒ_dd = lie_derivative_dd
+
+
+def dual_laplacian(n, sd: Mesh, backend=HIP, cast_float=None) -> Operator:
+    """Δᵈ(Val(n), sd) (FastDEC.jl:808-860) on a 2-D complex: the dual 0-form
+    (n = 0) or dual 1-form (n = 1) Laplacian; `op @ x`.  Both contain the
+    geometric ⋆1⁻¹ solve."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_DUAL_LAPLACIAN, n, cast_float=cast_float)
 
 
 def dec_differential(n, sd: Mesh, backend=HIP, cast_float=None) -> Operator:

@@ -33,7 +33,18 @@ enum Impl {
   I_ELL,   // general sparse: geometric ⋆1, ∇²
   I_AVG,   // avg₀₁
   I_WPD,   // primal-dual wedge 0-1 / dual-primal 1-0: (M g) .* f, M = wedge_pd_01_mat
-  I_GINV   // geometric ⋆1⁻¹: a solve with ⋆1 (Jacobi-preconditioned CG on the device)
+  I_GINV,  // geometric ⋆1⁻¹: a solve with ⋆1 (Jacobi-preconditioned CG on the device)
+  I_COMP   // a composition of other operators (see Comp)
+};
+
+// Composite operators, each a fixed sequence of sub-operator applications.
+enum Comp {
+  C_WPD11,  // primal-dual 1-1: ∧11(f, ♭♯ g); dual-primal 1-1: ∧11(♭♯ f, g)
+  C_I11,    // interior_product_dd(1,1): ⋆2 ∧11(⋆1⁻¹ g, ♭♯ f)
+  C_I12,    // interior_product_dd(1,2): ⋆1 ♭♯ ((Λ01 ⋆0⁻¹ g) .* f)
+  C_LIE,    // ℒ_dd(1,1): -(d̃0 i11(f, g)) - i12(f, d̃1 g)
+  C_DL0,    // Δᵈ(0): (⋆2 d1) ⋆1⁻¹(d̃0 x)
+  C_DL1     // Δᵈ(1): M x + N ⋆1⁻¹(x), M = ⋆1 d0 ⋆0⁻¹ d̃1, N = d̃0 ⋆2 d1
 };
 
 }  // namespace
@@ -60,6 +71,10 @@ struct dec_op_s {
   int max_iter = 0;
   double rtol = 0.0;
   bool symmetric = true;  // CG when ⋆1 is symmetric, BiCGSTAB otherwise
+  // composite operators: sub-operators and work vectors (one apply at a time)
+  int comp = 0;
+  std::vector<std::unique_ptr<dec_op_s>> sub;
+  std::vector<DBuf<unsigned char>> tmp;
 };
 
 namespace {
@@ -357,7 +372,7 @@ __global__ void k_wedge_pd(int ne, int width, const int *__restrict__ col, const
 // stay in device memory; every kernel of an iteration returns at once when
 // the stop flag is set, so a fixed launch sequence needs no host round trip.
 constexpr int kCgBlocks = 256;
-constexpr int kCgChunk = 8;  // iterations between host reads of the stop flag
+constexpr int kCgChunk = 8;  // most iterations between host reads of the stop flag
 enum { CS_BB = 0, CS_RR, CS_DONE, CS_ITER, CS_RZ, CS_PQ, CS_ALPHA, CS_BETA, CS_OMEGA, CS_RHO, CS_COUNT };
 
 // part[blk] = Σ a·b, part[kCgBlocks + blk] = Σ c·d (c, d may be NULL)
@@ -532,6 +547,13 @@ __global__ void k_bi_full(int n, const T *__restrict__ z, const T *__restrict__ 
   T om = static_cast<T>(sc[CS_OMEGA]);
   x[i] = x[i] + om * z[i];
   r[i] = r[i] - om * t[i];
+}
+
+// out = sa a + sb b with sa, sb = ±1 (exact scalings)
+template <class T>
+__global__ void k_pm(int n, T sa, const T *__restrict__ a, T sb, const T *__restrict__ b, T *__restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = sa * a[i] + sb * b[i];
 }
 
 template <class T>
@@ -951,10 +973,12 @@ void ginv_apply(dec_op_s &op, const T *bvec, T *x, hipStream_t s) {
   };
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   hip_status(hipStreamIsCapturing(s, &cap), "hipStreamIsCapturing");
-  const int chunk = cap == hipStreamCaptureStatusNone ? kCgChunk : op.max_iter;
-  for (int done_it = 0; done_it < op.max_iter; done_it += chunk) {
+  // checks after 2, 6, 14, 22, ... iterations: the diagonal-dominant ⋆1
+  // converges in a few, so the first checks come early
+  int chunk = cap == hipStreamCaptureStatusNone ? 2 : op.max_iter;
+  for (int done_it = 0; done_it < op.max_iter; done_it += chunk, chunk = std::min(2 * chunk, kCgChunk)) {
     for (int k = 0; k < chunk && done_it + k < op.max_iter; k++) iteration();
-    if (cap != hipStreamCaptureStatusNone) continue;
+    if (cap != hipStreamCaptureStatusNone) break;
     double flag = 0.0;
     hip_status(hipMemcpyAsync(&flag, sc + CS_DONE, sizeof(double), hipMemcpyDeviceToHost, s), "stop flag copy");
     hip_status(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -962,6 +986,9 @@ void ginv_apply(dec_op_s &op, const T *bvec, T *x, hipStream_t s) {
   }
   hipLaunchKernelGGL(k_negate<T>, g, blk, 0, s, n, x);
 }
+
+template <class T>
+void comp_apply(dec_op_s &op, const T *x, const T *y, T *out, hipStream_t s);
 
 template <class T>
 void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream_t s) {
@@ -1018,6 +1045,9 @@ void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream
     case I_GINV:
       ginv_apply<T>(op, x, out, s);
       break;
+    case I_COMP:
+      comp_apply<T>(op, x, y, out, s);
+      break;
     case I_WPD: {
       const dec::Ell<T> *E;
       if constexpr (sizeof(T) == 8) E = &op.ell64;
@@ -1030,6 +1060,60 @@ void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream
       fail(DEC_ERR_ARG, "dec_op_apply: corrupt operator");
   }
   dec::check_launch("dec_op_apply");
+}
+
+// Composite operators: the sub-operator applications of the reference's
+// closures, in the same order, through device work vectors.
+template <class T>
+void comp_apply(dec_op_s &op, const T *x, const T *y, T *out, hipStream_t s) {
+  auto W = [&](int i) { return reinterpret_cast<T *>(op.tmp[i].p); };
+  auto A = [&](int i, const T *a, const T *b, T *o) { apply_t<T>(*op.sub[i], a, b, o, s); };
+  auto pm = [&](int64_t n, T sa, const T *a, T sb, const T *b, T *o) {
+    if (n) hipLaunchKernelGGL(k_pm<T>, dim3(dec::nblocks(n)), dim3(256), 0, s, static_cast<int>(n), sa, a, sb, b, o);
+  };
+  switch (op.comp) {
+    case C_WPD11:  // FastDEC.jl:390-400
+      if (op.kind == DEC_OP_WEDGE_PD) {  // (f, g) -> Λ11(f, ♭♯ g)
+        A(0, y, nullptr, W(0));
+        A(1, x, W(0), out);
+      } else {  // (f, g) -> Λ11(♭♯ f, g)
+        A(0, x, nullptr, W(0));
+        A(1, W(0), y, out);
+      }
+      break;
+    case C_I11:  // (f, g) -> ⋆2 Λpd11(⋆1⁻¹ g, f) (FastDEC.jl:768-773)
+      A(0, y, nullptr, W(0));
+      A(1, x, nullptr, W(1));
+      A(2, W(0), W(1), W(2));
+      A(3, W(2), nullptr, out);
+      break;
+    case C_I12:  // (f, g) -> ⋆1 ♭♯ ((Λ01 ⋆0⁻¹ g) .* f) (FastDEC.jl:779-786)
+      A(0, y, nullptr, W(0));
+      A(1, W(0), x, W(1));
+      A(2, W(1), nullptr, W(2));
+      A(3, W(2), nullptr, out);
+      break;
+    case C_LIE:  // (f, g) -> -(d̃0 i11(f, g)) - i12(f, d̃1 g) (FastDEC.jl:796-803)
+      A(0, x, y, W(0));
+      A(1, W(0), nullptr, W(1));
+      A(2, y, nullptr, W(2));
+      A(3, x, W(2), W(3));
+      pm(op.rows, static_cast<T>(-1), W(1), static_cast<T>(-1), W(3), out);
+      break;
+    case C_DL0:  // x -> (⋆2 d1) ⋆1⁻¹(d̃0 x) (FastDEC.jl:821-829)
+      A(0, x, nullptr, W(0));
+      A(1, W(0), nullptr, W(1));
+      A(2, W(1), nullptr, out);
+      break;
+    case C_DL1:  // x -> M x + N ⋆1⁻¹(x) (FastDEC.jl:845-860)
+      A(0, x, nullptr, W(0));
+      A(1, x, nullptr, W(1));
+      A(2, W(0), nullptr, W(2));
+      pm(op.rows, static_cast<T>(1), W(1), static_cast<T>(1), W(2), out);
+      break;
+    default:
+      fail(DEC_ERR_ARG, "dec_op_apply: corrupt composite operator");
+  }
 }
 
 // ---- grouped launch: several operators in one kernel -----------------------
@@ -1238,6 +1322,163 @@ void pd01_matrix(const dec_mesh_s &m, dec::HostCsr &M) {
   }
 }
 
+// ♭♯_mat = only.(♭_mat(DPPFlat) * ♯_mat(LLSDDSharp)) (DiscreteExteriorCalculus.jl:684-714,
+// 876-909, 1992).  Per triangle t with edges e_j (slot j) and B-block dual
+// edges b_j from the triangle center to the edge centers:
+//   ♯[t, e_j] = pinv(X)[:, j] · w_j,  X rows = sign(b_j) (center_t - center_{e_j}),
+//   ♭[e, t]   = w_e,t · (p[∂v0 e] - p[∂v1 e]) sign(e),
+// with w_e,t = |b_{e,t}| / Σ_{t' ∋ e} |b_{e,t'}|.  pinv(X) is taken in the
+// plane of the triangle (X has rank 2: its rows are tangent), which is the
+// reference's SVD pseudo-inverse on flat meshes and stays well defined where
+// rounding lifts the rows out of the plane on a curved surface.
+void flat_sharp_matrix(const dec_mesh_s &m, dec::HostCsr &M) {
+  const dec::Topo &t = m.h;
+  int nv = t.nv, ne = t.ne, nt = t.nt;
+  std::vector<double> dpt = m.dpt.download();
+  std::vector<double> dlB(3 * static_cast<size_t>(nt));
+  if (nt) dec::d2h(dlB.data(), m.dl.p + 2 * static_cast<size_t>(ne), sizeof(double) * dlB.size());
+  dec::Csr eb;
+  dec::edge_bduals(t, eb);
+  std::vector<double> tot(ne, 0.0);
+  for (int e = 0; e < ne; e++)
+    for (int k = eb.ptr[e]; k < eb.ptr[e + 1]; k++) tot[e] = tot[e] + dlB[eb.idx[k]];
+  const std::vector<int32_t> *E[3] = {&t.e0, &t.e1, &t.e2};
+  const double *P = m.hp.data();
+  struct Ent {
+    int row, col, tri;
+    double val;
+  };
+  std::vector<Ent> ents;
+  ents.reserve(9 * static_cast<size_t>(nt));
+  for (int tr = 0; tr < nt; tr++) {
+    const double *c = &dpt[3 * static_cast<size_t>(nv + ne + tr)];
+    double X[3][3], fl[3][3], sh[3][3];
+    int ed[3];
+    for (int j = 0; j < 3; j++) {
+      int e = (*E[j])[tr];
+      ed[j] = e;
+      const double *ce = &dpt[3 * static_cast<size_t>(nv + e)];
+      // D_edge_orientation of b_j: relative_sign(eo[e_j], j odd ? !to[t] : to[t]) (DiscreteExteriorCalculus.jl:595-599)
+      bool tj = (j % 2) ? !t.to[tr] : static_cast<bool>(t.to[tr]);
+      double sg = (static_cast<bool>(t.eo[e]) == tj) ? 1.0 : -1.0;
+      for (int k = 0; k < 3; k++) X[j][k] = sg * (c[k] - ce[k]);
+    }
+    // tangent basis of the triangle (triangle_vertices, SimplicialSets.jl:305-309)
+    int tv[3] = {t.v1[t.e1[tr]], t.v0[t.e2[tr]], t.v0[t.e1[tr]]};
+    double a[3], b[3], n[3], u[3], w[3];
+    for (int k = 0; k < 3; k++) {
+      a[k] = P[3 * tv[1] + k] - P[3 * tv[0] + k];
+      b[k] = P[3 * tv[2] + k] - P[3 * tv[0] + k];
+    }
+    n[0] = a[1] * b[2] - a[2] * b[1];
+    n[1] = a[2] * b[0] - a[0] * b[2];
+    n[2] = a[0] * b[1] - a[1] * b[0];
+    double la = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    for (int k = 0; k < 3; k++) u[k] = a[k] / la;
+    w[0] = n[1] * u[2] - n[2] * u[1];
+    w[1] = n[2] * u[0] - n[0] * u[2];
+    w[2] = n[0] * u[1] - n[1] * u[0];
+    double lw = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    if (!(la > 0.0) || !(lw > 0.0)) fail(DEC_ERR_ARG, "♭♯_mat: degenerate triangle");
+    for (int k = 0; k < 3; k++) w[k] /= lw;
+    double Y[3][2], g00 = 0, g01 = 0, g11 = 0;
+    for (int j = 0; j < 3; j++) {
+      Y[j][0] = X[j][0] * u[0] + X[j][1] * u[1] + X[j][2] * u[2];
+      Y[j][1] = X[j][0] * w[0] + X[j][1] * w[1] + X[j][2] * w[2];
+      g00 += Y[j][0] * Y[j][0];
+      g01 += Y[j][0] * Y[j][1];
+      g11 += Y[j][1] * Y[j][1];
+    }
+    double det = g00 * g11 - g01 * g01;
+    if (!(det > 0.0)) fail(DEC_ERR_ARG, "♭♯_mat: dual edges of a triangle are collinear");
+    for (int j = 0; j < 3; j++) {
+      // column j of pinv(X) = [u w] G⁻¹ Y_jᵀ, times the weight of b_j
+      double q0 = (g11 * Y[j][0] - g01 * Y[j][1]) / det;
+      double q1 = (g00 * Y[j][1] - g01 * Y[j][0]) / det;
+      double wt = dlB[static_cast<size_t>(j) * nt + tr] / tot[ed[j]];
+      for (int k = 0; k < 3; k++) sh[j][k] = (u[k] * q0 + w[k] * q1) * wt;
+      // ♭[e_j, t] = del * e_vec / dels_sum (DiscreteExteriorCalculus.jl:705-710)
+      int e = ed[j];
+      double sgn = t.eo[e] ? 1.0 : -1.0;
+      for (int k = 0; k < 3; k++) {
+        double ev = (P[3 * t.v0[e] + k] - P[3 * t.v1[e] + k]) * sgn;
+        fl[j][k] = dlB[static_cast<size_t>(j) * nt + tr] * ev / tot[e];
+      }
+    }
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        ents.push_back({ed[i], ed[j], tr, fl[i][0] * sh[j][0] + fl[i][1] * sh[j][1] + fl[i][2] * sh[j][2]});
+  }
+  // entries summed over triangles in ascending order (the SpGEMM order)
+  std::sort(ents.begin(), ents.end(), [](const Ent &x, const Ent &y) {
+    return x.row != y.row ? x.row < y.row : (x.col != y.col ? x.col < y.col : x.tri < y.tri);
+  });
+  M.m = ne;
+  M.n = ne;
+  M.ptr.assign(ne + 1, 0);
+  M.col.clear();
+  M.val.clear();
+  for (size_t i = 0; i < ents.size();) {
+    size_t j = i + 1;
+    double v = ents[i].val;
+    for (; j < ents.size() && ents[j].row == ents[i].row && ents[j].col == ents[i].col; j++) v = v + ents[j].val;
+    M.col.push_back(ents[i].col);
+    M.val.push_back(v);
+    M.ptr[ents[i].row + 1]++;
+    i = j;
+  }
+  for (int r = 0; r < ne; r++) M.ptr[r + 1] += M.ptr[r];
+}
+
+// C = A B over host CSR, each entry accumulated over k ascending (the order
+// of SparseArrays' column-by-column product), columns ascending.
+void spgemm(const dec::HostCsr &A, const dec::HostCsr &B, dec::HostCsr &C) {
+  if (A.n != B.m) fail(DEC_ERR_ARG, "spgemm: shape mismatch");
+  C.m = A.m;
+  C.n = B.n;
+  C.ptr.assign(A.m + 1, 0);
+  C.col.clear();
+  C.val.clear();
+  std::vector<double> acc(B.n, 0.0);
+  std::vector<int> mark(B.n, -1), cols;
+  for (int i = 0; i < A.m; i++) {
+    cols.clear();
+    for (int ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
+      int k = A.col[ka];
+      double av = A.val[ka];
+      for (int kb = B.ptr[k]; kb < B.ptr[k + 1]; kb++) {
+        int j = B.col[kb];
+        if (mark[j] != i) {
+          mark[j] = i;
+          acc[j] = av * B.val[kb];
+          cols.push_back(j);
+        } else {
+          acc[j] = acc[j] + av * B.val[kb];
+        }
+      }
+    }
+    std::sort(cols.begin(), cols.end());
+    for (int j : cols) {
+      C.col.push_back(j);
+      C.val.push_back(acc[j]);
+    }
+    C.ptr[i + 1] = static_cast<int32_t>(C.col.size());
+  }
+}
+
+// A sparse-matrix operator from a host CSR (products assembled at create).
+std::unique_ptr<dec_op_s> ell_op(dec::HostCsr &&A, int dtype) {
+  auto o = std::make_unique<dec_op_s>();
+  o->impl = I_ELL;
+  o->dtype = dtype;
+  o->rows = A.m;
+  o->cols = A.n;
+  o->csr = std::move(A);
+  if (dtype == DEC_F64) dec::upload_ell(o->csr, o->ell64);
+  else dec::upload_ell(o->csr, o->ell32);
+  return o;
+}
+
 dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) {
   if (dtype != DEC_F64 && dtype != DEC_F32) fail(DEC_ERR_ARG, "dec_op_create: dtype must be DEC_F64 or DEC_F32");
   if (hodge != DEC_GEOMETRIC_HODGE && hodge != DEC_DIAGONAL_HODGE) fail(DEC_ERR_ARG, "dec_op_create: unknown hodge");
@@ -1251,6 +1492,11 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
   op->n = dn;
   op->hodge = hodge;
   op->dtype = dtype;
+  const size_t es = f64 ? sizeof(double) : sizeof(float);
+  auto add_tmp = [&](int64_t n) {
+    op->tmp.emplace_back();
+    op->tmp.back().alloc(es * static_cast<size_t>(std::max<int64_t>(n, 1)));
+  };
   switch (kind) {
     case DEC_OP_WEDGE: {
       int lo = std::min(dm, dn), hi = std::max(dm, dn);
@@ -1367,13 +1613,38 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
       op->cols2 = dm == 0 ? ne : nt;
       break;
     }
-    case DEC_OP_WEDGE_PD: {
+    case DEC_OP_FLAT_SHARP: {
+      op->impl = I_ELL;
+      op->rows = op->cols = ne;
+      flat_sharp_matrix(m, op->csr);
+      if (f64) dec::upload_ell(op->csr, op->ell64);
+      else dec::upload_ell(op->csr, op->ell32);
+      break;
+    }
+    case DEC_OP_WEDGE_PD:
+    case DEC_OP_WEDGE_DP: {
+      if (dm == 1 && dn == 1) {
+        // dec_wedge_product_pd(1,1) / dec_wedge_product_dp(1,1) (FastDEC.jl:390-400)
+        op->impl = I_COMP;
+        op->comp = C_WPD11;
+        op->rows = nt;
+        op->cols = op->cols2 = ne;
+        op->sub.emplace_back(create(m, DEC_OP_FLAT_SHARP, 0, 0, hodge, dtype));
+        op->sub.emplace_back(create(m, DEC_OP_WEDGE, 1, 1, hodge, dtype));
+        add_tmp(ne);
+        break;
+      }
+      if (kind == DEC_OP_WEDGE_DP && !(dm == 1 && dn == 0))
+        fail(DEC_ERR_UNSUPPORTED, "Unsupported combination of degrees " + std::to_string(dm) + " and " +
+                                      std::to_string(dn) +
+                                      ". Ensure that their sum is not greater than the degree of the complex. The "
+                                      "currently supported dual-primal wedges are 0-1, 1-0, and 1-1.");
       // dec_wedge_product_pd(0,1) / dec_wedge_product_dp(1,0) (FastDEC.jl:327-346, 379-387)
       if (!((dm == 0 && dn == 1) || (dm == 1 && dn == 0)))
         fail(DEC_ERR_UNSUPPORTED, "Unsupported combination of degrees " + std::to_string(dm) + " and " +
                                       std::to_string(dn) +
                                       ". Ensure that their sum is not greater than the degree of the complex. The "
-                                      "currently supported primal-dual wedges are 0-1 and 1-0.");
+                                      "currently supported primal-dual wedges are 0-1, 1-0, and 1-1.");
       op->impl = I_WPD;
       op->rows = ne;
       op->swap = dm == 1;  // dp(1,0) takes (f, g): the kernel runs on (g, f)
@@ -1551,6 +1822,93 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
       for (auto &x : op->csr.val) x = 0.5;
       break;
     }
+    case DEC_OP_INTERIOR_PRODUCT_DD: {
+      op->impl = I_COMP;
+      if (dm == 1 && dn == 1) {
+        // interior_product_dd(1,1) (FastDEC.jl:762-773): dual 1-forms f, g -> dual 0-form
+        op->comp = C_I11;
+        op->rows = nt;
+        op->cols = op->cols2 = ne;
+        op->sub.emplace_back(create(m, DEC_OP_INV_HODGE, 1, 0, DEC_GEOMETRIC_HODGE, dtype));
+        op->sub.emplace_back(create(m, DEC_OP_FLAT_SHARP, 0, 0, hodge, dtype));
+        op->sub.emplace_back(create(m, DEC_OP_WEDGE, 1, 1, hodge, dtype));
+        op->sub.emplace_back(create(m, DEC_OP_HODGE, 2, 0, DEC_GEOMETRIC_HODGE, dtype));
+        add_tmp(ne);
+        add_tmp(ne);
+        add_tmp(nt);
+      } else if (dm == 1 && dn == 2) {
+        // interior_product_dd(1,2) (FastDEC.jl:775-786): dual 1-form f, dual 2-form g -> dual 1-form
+        op->comp = C_I12;
+        op->rows = ne;
+        op->cols = ne;
+        op->cols2 = nv;
+        op->sub.emplace_back(create(m, DEC_OP_INV_HODGE, 0, 0, DEC_GEOMETRIC_HODGE, dtype));
+        op->sub.emplace_back(create(m, DEC_OP_WEDGE_PD, 0, 1, hodge, dtype));
+        op->sub.emplace_back(create(m, DEC_OP_FLAT_SHARP, 0, 0, hodge, dtype));
+        op->sub.emplace_back(create(m, DEC_OP_HODGE, 1, 0, DEC_GEOMETRIC_HODGE, dtype));
+        add_tmp(nv);
+        add_tmp(ne);
+        add_tmp(ne);
+      } else {
+        fail(DEC_ERR_UNSUPPORTED, "interior_product_dd: supported degrees are (1,1) and (1,2)");
+      }
+      break;
+    }
+    case DEC_OP_LIE_DD: {
+      // ℒ_dd(1,1) (FastDEC.jl:791-803)
+      if (!(dm == 1 && dn == 1)) fail(DEC_ERR_UNSUPPORTED, "ℒ_dd: supported degrees are (1,1)");
+      op->impl = I_COMP;
+      op->comp = C_LIE;
+      op->rows = ne;
+      op->cols = op->cols2 = ne;
+      op->sub.emplace_back(create(m, DEC_OP_INTERIOR_PRODUCT_DD, 1, 1, hodge, dtype));
+      op->sub.emplace_back(create(m, DEC_OP_DUAL_DERIVATIVE, 0, 0, hodge, dtype));
+      op->sub.emplace_back(create(m, DEC_OP_DUAL_DERIVATIVE, 1, 0, hodge, dtype));
+      op->sub.emplace_back(create(m, DEC_OP_INTERIOR_PRODUCT_DD, 1, 2, hodge, dtype));
+      add_tmp(nt);
+      add_tmp(ne);
+      add_tmp(nv);
+      add_tmp(ne);
+      break;
+    }
+    case DEC_OP_DUAL_LAPLACIAN: {
+      op->impl = I_COMP;
+      auto csr_of = [&](int k, int a, int h) {
+        std::unique_ptr<dec_op_s> o(create(m, k, a, 0, h, DEC_F64));
+        return std::move(o->csr);
+      };
+      if (dm == 0) {
+        // Δᵈ(Val(0)) on a 2-D complex (FastDEC.jl:821-829): m = ⋆2 d1; x -> m ⋆1⁻¹(d̃0 x)
+        op->comp = C_DL0;
+        op->rows = op->cols = nt;
+        op->sub.emplace_back(create(m, DEC_OP_DUAL_DERIVATIVE, 0, 0, hodge, dtype));
+        op->sub.emplace_back(create(m, DEC_OP_INV_HODGE, 1, 0, DEC_GEOMETRIC_HODGE, dtype));
+        dec::HostCsr M;
+        spgemm(csr_of(DEC_OP_HODGE, 2, DEC_GEOMETRIC_HODGE), csr_of(DEC_OP_DIFFERENTIAL, 1, hodge), M);
+        op->sub.emplace_back(ell_op(std::move(M), dtype));
+        add_tmp(ne);
+        add_tmp(ne);
+      } else if (dm == 1) {
+        // Δᵈ(Val(1)) (FastDEC.jl:835-860): M = ⋆1 d0 ⋆0⁻¹ d̃1, N = d̃0 ⋆2 d1; x -> M x + N ⋆1⁻¹(x)
+        op->comp = C_DL1;
+        op->rows = op->cols = ne;
+        op->sub.emplace_back(create(m, DEC_OP_INV_HODGE, 1, 0, DEC_GEOMETRIC_HODGE, dtype));
+        dec::HostCsr t1, t2, M, N;
+        spgemm(csr_of(DEC_OP_HODGE, 1, DEC_GEOMETRIC_HODGE), csr_of(DEC_OP_DIFFERENTIAL, 0, hodge), t1);
+        spgemm(t1, csr_of(DEC_OP_INV_HODGE, 0, DEC_GEOMETRIC_HODGE), t2);
+        spgemm(t2, csr_of(DEC_OP_DUAL_DERIVATIVE, 1, hodge), M);
+        spgemm(csr_of(DEC_OP_DUAL_DERIVATIVE, 0, hodge), csr_of(DEC_OP_HODGE, 2, DEC_GEOMETRIC_HODGE), t1);
+        spgemm(t1, csr_of(DEC_OP_DIFFERENTIAL, 1, hodge), N);
+        op->sub.emplace_back(ell_op(std::move(M), dtype));
+        op->sub.emplace_back(ell_op(std::move(N), dtype));
+        add_tmp(ne);
+        add_tmp(ne);
+        add_tmp(ne);
+      } else {
+        fail(DEC_ERR_UNSUPPORTED, "Δᵈ: n must be 0 or 1 on a 2-D complex");
+      }
+      break;
+    }
     default:
       fail(DEC_ERR_ARG, "dec_op_create: unknown operator kind");
   }
@@ -1657,8 +2015,9 @@ int dec_op_apply_many(int32_t count, const dec_op *ops, const void *const *x, co
   for (int q = 0; q < count; q++) {
     dec_op_s *op = ops[q];
     if (!op) fail(DEC_ERR_ARG, "dec_op_apply_many: NULL operator");
-    if (op->impl == I_GINV)
-      fail(DEC_ERR_UNSUPPORTED, "dec_op_apply_many: the geometric ⋆1⁻¹ is a solve; apply it with dec_op_apply");
+    if (op->impl == I_GINV || op->impl == I_COMP)
+      fail(DEC_ERR_UNSUPPORTED,
+           "dec_op_apply_many: solves and composite operators run several launches; apply them with dec_op_apply");
     if (dtype < 0) dtype = op->dtype;
     if (op->dtype != dtype) fail(DEC_ERR_ARG, "dec_op_apply_many: operators of different dtypes");
     if (op->rows > 0 && !out[q]) fail(DEC_ERR_ARG, "dec_op_apply_many: NULL output");

@@ -52,7 +52,12 @@ enum dec_op_kind {
   DEC_OP_INV_HODGE = 7,       /* ⋆_n⁻¹ (diagonal) */
   DEC_OP_LAPLACIAN0 = 8,      /* -⋆0⁻¹ d̃1 ⋆1 d0 */
   DEC_OP_AVG01 = 9,           /* avg₀₁ */
-  DEC_OP_WEDGE_PD = 10        /* primal-dual wedge 0-1 / dual-primal 1-0 (wedge_pd_01_mat) */
+  DEC_OP_WEDGE_PD = 10,       /* primal-dual wedge 0-1, 1-1 (and dual-primal 1-0, as DEC_OP_WEDGE_DP) */
+  DEC_OP_FLAT_SHARP = 11,     /* ♭♯_mat: dual 1-form -> primal 1-form */
+  DEC_OP_WEDGE_DP = 12,       /* dual-primal wedge 1-0, 1-1 */
+  DEC_OP_INTERIOR_PRODUCT_DD = 13, /* interior_product_dd (1,1), (1,2) */
+  DEC_OP_LIE_DD = 14,         /* ℒ_dd (1,1) */
+  DEC_OP_DUAL_LAPLACIAN = 15  /* Δᵈ n = 0, 1 */
 };
 enum dec_smoother { DEC_SMOOTH_CG = 0, DEC_SMOOTH_GS = 1, DEC_SMOOTH_JACOBI = 2 };
 enum dec_solve_variant { DEC_SOLVE_PLAIN_F64 = 0, DEC_SOLVE_MIXED_IR = 1 };
@@ -115,9 +120,11 @@ int dec_mesh_destroy(dec_mesh m);
  *     returns (-⋆1)⁻¹ x from a Jacobi-scaled Krylov iteration on the device
  *     (CG where ⋆1 is symmetric, BiCGSTAB otherwise) stopped at
  *     ‖x - ⋆1 z‖ ≤ 1e-14 ‖x‖ (float64; 1e-6 in float32).  Apply synchronises
- *     the stream every 8 iterations to read the stop flag, except while the
- *     stream is captured into a graph (then the whole iteration budget is
- *     recorded).  The operator owns work vectors: one apply at a time.
+ *     the stream to read the stop flag after 2, 6, 14 and then every 8
+ *     iterations, except while the stream is captured into a graph (then the
+ *     whole budget of 200 (float64) or 100 (float32) iterations is recorded;
+ *     those past convergence return at once).  The operator owns work
+ *     vectors: one apply at a time.
  *  DEC_OP_LAPLACIAN0 (hodge selects ⋆1): fast_laplace_beltrami
  *     (test/Multigrid.jl:46-48) = ∇²(0, sd) form (DiscreteExteriorCalculus.jl:1887-1890).
  *  DEC_OP_AVG01: avg₀₁_mat (FastDEC.jl:874-892).

@@ -317,6 +317,37 @@ def test_inv_geometric_hodge1(case, O, cs, dtype):
         gh.solve_info()
 
 
+def test_inv_geometric_hodge1_graph_replay(O, cs):
+    """Captured into a graph, the solve records its whole iteration budget;
+    iterations past convergence return at once, so every replay equals the
+    eager solve bit for bit (barycentric dual: BiCGSTAB)."""
+    import torch
+
+    gm = cs.icosphere(4)
+    cs.dualize(gm, cs.Barycenter)
+    inv = cs.dec_inv_hodge_star(1, gm, cs.GeometricHodge, cast_float=torch.float64)
+    x = torch.empty(inv.rows, dtype=torch.float64, device="cuda")
+    out = torch.empty_like(x)
+    x.copy_(torch.from_numpy(O.uniform_form(60, inv.rows)))
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        inv.apply(x, out=out)                       # warm up outside capture
+    torch.cuda.current_stream().wait_stream(s)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        inv.apply(x, out=out)
+    for seed in (61, 62, 63):
+        x.copy_(torch.from_numpy(O.uniform_form(seed, inv.rows)))
+        g.replay()
+        torch.cuda.synchronize()
+        replayed = out.clone()
+        eager = inv(x)
+        torch.cuda.synchronize()
+        assert torch.equal(replayed, eager)
+        assert inv.solve_info()[1] <= 1e-14
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("hodge", ["diagonal", "geometric"])
 def test_laplacian(case, O, cs, dtype, hodge):
