@@ -151,6 +151,43 @@ def time_opset(torch, cs, mesh, names, td, sets, steps, warmup, cache_resident=T
     return per, total, grouped, cache, lap_nnz
 
 
+def musical_ops(torch, cs, mesh, reps=20):
+    """The composite operators on the headline mesh, FP64: milliseconds per
+    application, wall clock over `reps` back-to-back applications (the ⋆1⁻¹
+    solve inside some of them reads its stop flag from the host).  ♭♯ is a
+    plain sparse-matrix product and is also given in GB/s of its compulsory
+    traffic (nnz int32 columns and values, input and output once)."""
+    td = torch.float64
+    ne = mesh.ne
+    fs = cs.flat_sharp_mat(mesh, cast_float=td)
+    nnz = len(fs.csr()[1])
+    ops = {
+        "flat_sharp": (fs, 1),
+        "inv_hodge1_geometric": (cs.dec_inv_hodge_star(1, mesh, cs.GeometricHodge, cast_float=td), 1),
+        "wedge_pd11": (cs.dec_wedge_product_pd(1, 1, mesh, cast_float=td).operator, 2),
+        "interior_product_dd11": (cs.interior_product_dd(1, 1, mesh, cast_float=td).operator, 2),
+        "interior_product_dd12": (cs.interior_product_dd(1, 2, mesh, cast_float=td).operator, 2),
+        "lie_derivative_dd11": (cs.lie_derivative_dd(1, 1, mesh, cast_float=td).operator, 2),
+        "dual_laplacian0": (cs.dual_laplacian(0, mesh, cast_float=td), 1),
+        "dual_laplacian1": (cs.dual_laplacian(1, mesh, cast_float=td), 1),
+    }
+    res = {}
+    for name, (op, nin) in ops.items():
+        x = cs.fill_uniform(torch.empty(op.cols, dtype=td, device="cuda"), 501)
+        y = cs.fill_uniform(torch.empty(op.cols2, dtype=td, device="cuda"), 502) if nin == 2 else None
+        out = torch.empty(op.rows, dtype=td, device="cuda")
+        op.apply(x, y, out=out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            op.apply(x, y, out=out)
+        torch.cuda.synchronize()
+        res[name] = {"ms": (time.perf_counter() - t0) / reps * 1e3}
+    res["flat_sharp"]["GBps"] = (nnz * 12 + 2 * ne * 8) / (res["flat_sharp"]["ms"] * 1e-3) / 1e9
+    res["flat_sharp"]["nnz"] = nnz
+    return res
+
+
 def stream_copy_gbps(torch):
     n = 1 << 28  # 2 GiB of float64 per buffer
     x = torch.empty(n, dtype=torch.float64, device="cuda")
@@ -539,6 +576,8 @@ def main():
                                        max(10, args.steps // 2), 2, False)
         extra["other_ops_f64"] = {n: {"ms": per_x[n], "GBps": alg_bytes(n, nv, ne, nt, 8) / (per_x[n] * 1e-3) / 1e9}
                                   for n in extra_names}
+        extra["composite_ops_f64"] = musical_ops(torch, cs, mesh)
+        print(f"# composite ops: {json.dumps(extra['composite_ops_f64'])}", file=sys.stderr, flush=True)
     if not args.no_f32:
         per32, total32, grouped32, cache32, lap32 = time_opset(torch, cs, mesh, OPSET, torch.float32, args.sets,
                                                                args.steps, args.warmup, False)

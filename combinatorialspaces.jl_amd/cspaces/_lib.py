@@ -20,6 +20,8 @@ DEC_GEOMETRIC_HODGE, DEC_DIAGONAL_HODGE = 0, 1
 DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI = 0, 1, 2
 DEC_SOLVE_PLAIN_F64, DEC_SOLVE_MIXED_IR = 0, 1
 
+(DEC_OK, DEC_ERR_ARG, DEC_ERR_UNSUPPORTED, DEC_ERR_OOM, DEC_ERR_HIP, DEC_ERR_NOCONV,
+ DEC_ERR_NONORIENTABLE) = range(7)
 STATUS_NAMES = {0: "DEC_OK", 1: "DEC_ERR_ARG", 2: "DEC_ERR_UNSUPPORTED", 3: "DEC_ERR_OOM",
                 4: "DEC_ERR_HIP", 5: "DEC_ERR_NOCONV", 6: "DEC_ERR_NONORIENTABLE"}
 

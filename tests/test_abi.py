@@ -37,3 +37,15 @@ def test_version_and_error_without_gpu():
     h = ctypes.c_void_p()
     assert lib.dec_mesh_icosphere(-1, ctypes.byref(h)) == 1
     assert b"k must be" in lib.dec_last_error()
+
+
+def test_unsupported_degrees_are_reported_on_the_host():
+    """The mirror refuses degree pairs the reference refuses (FastDEC.jl:361-372),
+    with the reference's message, before touching the library."""
+    import cspaces as cs
+    import pytest
+
+    with pytest.raises(cs.DecError, match="currently supported primal-dual wedges are 0-1, 1-0, and 1-1"):
+        cs.dec_wedge_product_pd(1, 0, None)
+    with pytest.raises(cs.DecError, match="currently supported dual-primal wedges are 0-1, 1-0, and 1-1"):
+        cs.dec_wedge_product_dp(0, 1, None)
