@@ -20,6 +20,7 @@ using AMDGPU
 using Krylov
 using LinearAlgebra
 import CombinatorialSpaces: cache_wedge, dec_wedge_product, dec_wedge_product_dd,
+  dec_wedge_product_pd, dec_wedge_product_dp, ♭♯_mat, interior_product_dd, ℒ_dd, Δᵈ,
   dec_boundary, dec_differential, dec_dual_derivative,
   dec_hodge_star, dec_inv_hodge_star, avg₀₁_mat,
   multigrid_vcycles, multigrid_wcycles, full_multigrid
@@ -32,7 +33,8 @@ const DEC_CIRCUMCENTER = Int32(0)
 const DEC_GEOMETRIC_HODGE, DEC_DIAGONAL_HODGE = Int32(0), Int32(1)
 const DEC_OP_WEDGE, DEC_OP_WEDGE_DD, DEC_OP_DIFFERENTIAL, DEC_OP_BOUNDARY,
       DEC_OP_DUAL_DERIVATIVE, DEC_OP_HODGE, DEC_OP_INV_HODGE, DEC_OP_LAPLACIAN0,
-      DEC_OP_AVG01 = Int32.(1:9)
+      DEC_OP_AVG01, DEC_OP_WEDGE_PD, DEC_OP_FLAT_SHARP, DEC_OP_WEDGE_DP,
+      DEC_OP_INTERIOR_PRODUCT_DD, DEC_OP_LIE_DD, DEC_OP_DUAL_LAPLACIAN = Int32.(1:15)
 const DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI = Int32.(0:2)
 
 function check(status::Cint)
@@ -152,26 +154,39 @@ avg₀₁_mat(sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) =
 # vectors on the device (no dense n×n matrix), geometric ⋆1 is a sparse operator.
 dec_hodge_star(n::Int, sd::EmbeddedDeltaDualComplex2D, h::DiscreteHodge, ::Val{:HIP}) =
   HIPOperator{float_of(sd)}(sd, DEC_OP_HODGE, n, 0, hodge_code(h))
-dec_inv_hodge_star(n::Int, sd::EmbeddedDeltaDualComplex2D, h::DiscreteHodge, ::Val{:HIP}) =
-  n == 1 && h isa GeometricHodge ? inv_geometric_hodge1(sd) :
-  HIPOperator{float_of(sd)}(sd, DEC_OP_INV_HODGE, n, 0, hodge_code(h))
-
-# ⋆1⁻¹ geometric as a GMRES solve on the device operator, as the CUDA extension
-# does (Krylov.gmres only needs size, eltype and mul!).
-struct NegHodge{T}
-  op::HIPOperator{T}
-end
-Base.size(A::NegHodge) = (A.op.rows, A.op.cols)
-Base.size(A::NegHodge, i::Int) = size(A)[i]
-Base.eltype(::NegHodge{T}) where T = T
-function LinearAlgebra.mul!(y::ROCVector{T}, A::NegHodge{T}, x::ROCVector{T}) where T
-  apply!(y, A.op, x)
-  y .= -y
+# Geometric ⋆1⁻¹ (CUDA ext :93-96 runs GMRES on -⋆1): the library's operator is
+# the solve itself (CG or BiCGSTAB on the device), so it is returned as a closure.
+function dec_inv_hodge_star(n::Int, sd::EmbeddedDeltaDualComplex2D, h::DiscreteHodge, ::Val{:HIP})
+  op = HIPOperator{float_of(sd)}(sd, DEC_OP_INV_HODGE, n, 0, hodge_code(h))
+  n == 1 && h isa GeometricHodge ? (x -> op * x) : op
 end
 
-function inv_geometric_hodge1(sd)
-  A = NegHodge(HIPOperator{float_of(sd)}(sd, DEC_OP_HODGE, 1, 0, DEC_GEOMETRIC_HODGE))
-  x -> Krylov.gmres(A, x, atol = 1e-14)[1]
+# Primal-dual / dual-primal wedges (src/FastDEC.jl:374-400): closures as in FastDEC.
+function dec_wedge_product_pd(::Val{m}, ::Val{n}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) where {m, n}
+  op = HIPOperator{float_of(sd)}(sd, DEC_OP_WEDGE_PD, m, n, DEC_GEOMETRIC_HODGE)
+  (a, b) -> apply!(similar(a, op.rows), op, a, b)
+end
+function dec_wedge_product_dp(::Val{m}, ::Val{n}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) where {m, n}
+  op = HIPOperator{float_of(sd)}(sd, DEC_OP_WEDGE_DP, m, n, DEC_GEOMETRIC_HODGE)
+  (a, b) -> apply!(similar(a, op.rows), op, a, b)
+end
+
+# ♭♯_mat (DiscreteExteriorCalculus.jl:1992) as a device operator.
+♭♯_mat(sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) =
+  HIPOperator{float_of(sd)}(sd, DEC_OP_FLAT_SHARP, 0, 0, DEC_GEOMETRIC_HODGE)
+
+# Interior products, Lie derivative and dual Laplacians (src/FastDEC.jl:758-860).
+function interior_product_dd(::Val{m}, ::Val{n}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) where {m, n}
+  op = HIPOperator{float_of(sd)}(sd, DEC_OP_INTERIOR_PRODUCT_DD, m, n, DEC_GEOMETRIC_HODGE)
+  (f, g) -> apply!(similar(f, op.rows), op, f, g)
+end
+function ℒ_dd(::Val{1}, ::Val{1}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP})
+  op = HIPOperator{float_of(sd)}(sd, DEC_OP_LIE_DD, 1, 1, DEC_GEOMETRIC_HODGE)
+  (f, g) -> apply!(similar(f, op.rows), op, f, g)
+end
+function Δᵈ(::Val{n}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) where n
+  op = HIPOperator{float_of(sd)}(sd, DEC_OP_DUAL_LAPLACIAN, n, 0, DEC_GEOMETRIC_HODGE)
+  x -> op * x
 end
 
 # ---------------------------------------------------------------- multigrid
