@@ -9,7 +9,8 @@ from .fastdec import (  # noqa: F401
     dual_laplacian, dualize, fast_laplace_beltrami, fill_uniform, flat_sharp_mat, icosphere,
     interior_product_dd, lie_derivative_dd, mesh_from_arrays, subdivision, triangulated_grid, ℒ_dd)
 from .multigrid import (  # noqa: F401
-    BinarySubdivision, MGData, MultigridData, cg, dec_Delta_inv, full_multigrid, gauss_seidel, jacobi,
+    BinarySubdivision, DirectMode, GalerkinMode, MGData, MultigridData, cg, dec_Delta_inv, full_multigrid,
+    gauss_seidel, jacobi,
     multigrid_mu_cycles, multigrid_vcycles, multigrid_wcycles, solve)
 from .batch import HeatBatch  # noqa: F401
 from .learned import LearnedSmoother, learned_stats, make_pairs, sh_basis  # noqa: F401

@@ -18,6 +18,7 @@ DEC_GEOMETRIC_HODGE, DEC_DIAGONAL_HODGE = 0, 1
  DEC_OP_HODGE, DEC_OP_INV_HODGE, DEC_OP_LAPLACIAN0, DEC_OP_AVG01, DEC_OP_WEDGE_PD, DEC_OP_FLAT_SHARP,
  DEC_OP_WEDGE_DP, DEC_OP_INTERIOR_PRODUCT_DD, DEC_OP_LIE_DD, DEC_OP_DUAL_LAPLACIAN) = range(1, 16)
 DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI = 0, 1, 2
+DEC_MG_DIRECT, DEC_MG_GALERKIN = 0, 1
 DEC_SOLVE_PLAIN_F64, DEC_SOLVE_MIXED_IR = 0, 1
 
 (DEC_OK, DEC_ERR_ARG, DEC_ERR_UNSUPPORTED, DEC_ERR_OOM, DEC_ERR_HIP, DEC_ERR_NOCONV,
@@ -50,6 +51,7 @@ SIGNATURES = {
     "dec_op_export_tables": (c_int, [P, P, P]),
     "dec_op_destroy": (c_int, [P]),
     "dec_mg_create": (c_int, [P, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(P)]),
+    "dec_mg_create_mode": (c_int, [P, c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(P)]),
     "dec_mg_nlevels": (c_int, [P, PI32]),
     "dec_mg_level_size": (c_int, [P, c_int32, PI32]),
     "dec_mg_finest_mesh": (c_int, [P, POINTER(P)]),

@@ -36,18 +36,29 @@ gauss_seidel = _Smoother("gauss_seidel", L.DEC_SMOOTH_GS)
 jacobi = _Smoother("jacobi", L.DEC_SMOOTH_JACOBI)
 
 
+class DirectMode:
+    """Every level dualized and discretized (Multigrid.jl:566)."""
+    code = L.DEC_MG_DIRECT
+
+
+class GalerkinMode:
+    """Only the finest level dualized; Aₖ₊₁ = Rₖ Aₖ Pₖ (Multigrid.jl:567, 740-748)."""
+    code = L.DEC_MG_GALERKIN
+
+
 class MultigridData:
     """MultigridData(s, BinarySubdivision(), levels, fast_laplace_beltrami, steps;
-    alg=Circumcenter()) (Multigrid.jl:698-702, 752-797).  Levels are finest first."""
+    alg=Circumcenter(), mode=DirectMode()) (Multigrid.jl:698-702, 752-797).
+    Levels are finest first."""
 
     def __init__(self, s: Mesh, scheme=BinarySubdivision, levels: int = 1, hodge=GeometricHodge,
-                 steps: int = 3, alg=Circumcenter, project: bool = False):
+                 steps: int = 3, alg=Circumcenter, project: bool = False, mode=DirectMode):
         if scheme is not BinarySubdivision:
             raise ValueError("only BinarySubdivision is implemented")
         h = c_void_p()
         center = alg.code if hasattr(alg, "code") else int(alg)
-        check(lib().dec_mg_create(s.handle, levels, int(bool(project)), _hodge(hodge), center, steps,
-                                  ctypes.byref(h)))
+        check(lib().dec_mg_create_mode(s.handle, levels, int(bool(project)), _hodge(hodge), center, steps,
+                                       mode.code, ctypes.byref(h)))
         self._h = h
         self.steps = steps
         n = c_int32()

@@ -33,6 +33,8 @@ struct HostCsr {
 // SpGEMM order (see ops.hip).  H (geometric ⋆1) may be null: then ⋆1 = diag(h1).
 void assemble_laplacian0(const Topo &t, const std::vector<double> &inv_h0, const std::vector<double> &h1,
                          const HostCsr *H, HostCsr &L);
+// C = A B, each entry accumulated over k ascending (SparseArrays' order).
+void spgemm(const HostCsr &A, const HostCsr &B, HostCsr &C);
 // Geometric ⋆1 (FastDEC.jl:660-716) as a host CSR, per-triangle entries on the GPU.
 void geo_hodge1(dec_mesh_s &m, HostCsr &H);
 // Upload a CSR as slot-major ELL with values cast to V.

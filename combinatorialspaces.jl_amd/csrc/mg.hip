@@ -543,10 +543,14 @@ void build_level_matrix(Level &L, const dec::HostCsr &A, std::vector<int32_t> &c
 
 // make_restriction(transpose(S)) (Multigrid.jl:595-612) as rows of R, and the
 // endpoints of P = transpose(S) = subdivision_matrix(BinarySubdivision) (:191-219).
-void build_transfer(Level &fine, const dec::Topo &coarse) {
+// R = row-normalised S and P = Sᵀ of one binary refinement of `coarse`
+// (Multigrid.jl:191-219, 595-612); pa, pb are the endpoints of each midpoint.
+void transfer_csr(const dec::Topo &coarse, dec::HostCsr &R, dec::HostCsr &P, std::vector<int32_t> &pa,
+                  std::vector<int32_t> &pb) {
   int nc = coarse.nv, ne = coarse.ne, nf = nc + ne;
   std::vector<std::vector<int32_t>> mids(nc);
-  std::vector<int32_t> pa(ne), pb(ne);
+  pa.assign(ne, 0);
+  pb.assign(ne, 0);
   for (int e = 0; e < ne; e++) {
     int a = std::min(coarse.v0[e], coarse.v1[e]), b = std::max(coarse.v0[e], coarse.v1[e]);
     pa[e] = a;
@@ -554,7 +558,7 @@ void build_transfer(Level &fine, const dec::Topo &coarse) {
     mids[a].push_back(nc + e);
     if (b != a) mids[b].push_back(nc + e);
   }
-  dec::HostCsr R;
+  R = dec::HostCsr();
   R.m = nc;
   R.n = nf;
   R.ptr.assign(nc + 1, 0);
@@ -569,7 +573,7 @@ void build_transfer(Level &fine, const dec::Topo &coarse) {
     }
     R.ptr[i + 1] = static_cast<int32_t>(R.col.size());
   }
-  dec::HostCsr P;
+  P = dec::HostCsr();
   P.m = nf;
   P.n = nc;
   P.ptr.assign(nf + 1, 0);
@@ -585,6 +589,13 @@ void build_transfer(Level &fine, const dec::Topo &coarse) {
     }
     P.ptr[j + 1] = static_cast<int32_t>(P.col.size());
   }
+}
+
+void build_transfer(Level &fine, const dec::Topo &coarse) {
+  dec::HostCsr R, P;
+  std::vector<int32_t> pa, pb;
+  transfer_csr(coarse, R, P, pa, pb);
+  int nc = coarse.nv;
   int w = 0;
   for (int i = 0; i < nc; i++) w = std::max(w, R.ptr[i + 1] - R.ptr[i]);
   fine.nc = nc;
@@ -658,7 +669,8 @@ std::vector<std::vector<int32_t>> level_colors(const dec_mg_s &mg, const std::ve
   return colors;
 }
 
-dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge, int center, int steps) {
+dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge, int center, int steps,
+                    bool galerkin) {
   if (levels < 0 || levels > 11) fail(DEC_ERR_ARG, "dec_mg_create: levels must be in 0..11");
   if (steps < 0) fail(DEC_ERR_ARG, "dec_mg_create: steps must be >= 0");
   auto mg = std::make_unique<dec_mg_s>();
@@ -677,6 +689,15 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
   for (int l = 0; l < nl; l++) {
     dec_mesh_s &m = *mg->meshes[l];
     m.owned_by_mg = true;
+    if (galerkin && l > 0) {
+      // GalerkinMode: only the finest mesh is dualized; Aₖ₊₁ = (Rₖ Aₖ) Pₖ (Multigrid.jl:740-748)
+      dec::HostCsr R, P, RA;
+      std::vector<int32_t> pa, pb;
+      transfer_csr(m.h, R, P, pa, pb);
+      dec::spgemm(R, As[l - 1], RA);
+      dec::spgemm(RA, P, As[l]);
+      continue;
+    }
     dec::dualize(m, center);
     std::vector<double> h0 = m.h0.download(), h1 = m.h1.download(), ih0(m.h.nv);
     for (int v = 0; v < m.h.nv; v++) ih0[v] = 1 / h0[v];
@@ -1049,7 +1070,17 @@ int dec_mg_create(dec_mesh base, int32_t levels, int32_t project, int32_t hodge,
   DEC_TRY
   if (!base || !out) fail(DEC_ERR_ARG, "dec_mg_create: NULL handle");
   *out = nullptr;
-  *out = mg_create(*base, levels, project != 0, hodge, center, steps);
+  *out = mg_create(*base, levels, project != 0, hodge, center, steps, false);
+  DEC_CATCH
+}
+
+int dec_mg_create_mode(dec_mesh base, int32_t levels, int32_t project, int32_t hodge, int32_t center,
+                       int32_t steps, int32_t mode, dec_mg *out) {
+  DEC_TRY
+  if (!base || !out) fail(DEC_ERR_ARG, "dec_mg_create_mode: NULL handle");
+  if (mode != DEC_MG_DIRECT && mode != DEC_MG_GALERKIN) fail(DEC_ERR_ARG, "dec_mg_create_mode: unknown mode");
+  *out = nullptr;
+  *out = mg_create(*base, levels, project != 0, hodge, center, steps, mode == DEC_MG_GALERKIN);
   DEC_CATCH
 }
 

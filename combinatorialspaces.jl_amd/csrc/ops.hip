@@ -838,6 +838,42 @@ void geo_hodge1(dec_mesh_s &m, HostCsr &H) {
   geo_hodge1_csr(t, v9.download(), H);
 }
 
+// C = A B over host CSR, each entry accumulated over k ascending (the order
+// of SparseArrays' column-by-column product), columns ascending.
+void spgemm(const HostCsr &A, const HostCsr &B, HostCsr &C) {
+  if (A.n != B.m) fail(DEC_ERR_ARG, "spgemm: shape mismatch");
+  C.m = A.m;
+  C.n = B.n;
+  C.ptr.assign(A.m + 1, 0);
+  C.col.clear();
+  C.val.clear();
+  std::vector<double> acc(B.n, 0.0);
+  std::vector<int> mark(B.n, -1), cols;
+  for (int i = 0; i < A.m; i++) {
+    cols.clear();
+    for (int ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
+      int k = A.col[ka];
+      double av = A.val[ka];
+      for (int kb = B.ptr[k]; kb < B.ptr[k + 1]; kb++) {
+        int j = B.col[kb];
+        if (mark[j] != i) {
+          mark[j] = i;
+          acc[j] = av * B.val[kb];
+          cols.push_back(j);
+        } else {
+          acc[j] = acc[j] + av * B.val[kb];
+        }
+      }
+    }
+    std::sort(cols.begin(), cols.end());
+    for (int j : cols) {
+      C.col.push_back(j);
+      C.val.push_back(acc[j]);
+    }
+    C.ptr[i + 1] = static_cast<int32_t>(C.col.size());
+  }
+}
+
 }  // namespace dec
 
 namespace {
@@ -1430,42 +1466,6 @@ void flat_sharp_matrix(const dec_mesh_s &m, dec::HostCsr &M) {
   for (int r = 0; r < ne; r++) M.ptr[r + 1] += M.ptr[r];
 }
 
-// C = A B over host CSR, each entry accumulated over k ascending (the order
-// of SparseArrays' column-by-column product), columns ascending.
-void spgemm(const dec::HostCsr &A, const dec::HostCsr &B, dec::HostCsr &C) {
-  if (A.n != B.m) fail(DEC_ERR_ARG, "spgemm: shape mismatch");
-  C.m = A.m;
-  C.n = B.n;
-  C.ptr.assign(A.m + 1, 0);
-  C.col.clear();
-  C.val.clear();
-  std::vector<double> acc(B.n, 0.0);
-  std::vector<int> mark(B.n, -1), cols;
-  for (int i = 0; i < A.m; i++) {
-    cols.clear();
-    for (int ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
-      int k = A.col[ka];
-      double av = A.val[ka];
-      for (int kb = B.ptr[k]; kb < B.ptr[k + 1]; kb++) {
-        int j = B.col[kb];
-        if (mark[j] != i) {
-          mark[j] = i;
-          acc[j] = av * B.val[kb];
-          cols.push_back(j);
-        } else {
-          acc[j] = acc[j] + av * B.val[kb];
-        }
-      }
-    }
-    std::sort(cols.begin(), cols.end());
-    for (int j : cols) {
-      C.col.push_back(j);
-      C.val.push_back(acc[j]);
-    }
-    C.ptr[i + 1] = static_cast<int32_t>(C.col.size());
-  }
-}
-
 // A sparse-matrix operator from a host CSR (products assembled at create).
 std::unique_ptr<dec_op_s> ell_op(dec::HostCsr &&A, int dtype) {
   auto o = std::make_unique<dec_op_s>();
@@ -1884,7 +1884,7 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
         op->sub.emplace_back(create(m, DEC_OP_DUAL_DERIVATIVE, 0, 0, hodge, dtype));
         op->sub.emplace_back(create(m, DEC_OP_INV_HODGE, 1, 0, DEC_GEOMETRIC_HODGE, dtype));
         dec::HostCsr M;
-        spgemm(csr_of(DEC_OP_HODGE, 2, DEC_GEOMETRIC_HODGE), csr_of(DEC_OP_DIFFERENTIAL, 1, hodge), M);
+        dec::spgemm(csr_of(DEC_OP_HODGE, 2, DEC_GEOMETRIC_HODGE), csr_of(DEC_OP_DIFFERENTIAL, 1, hodge), M);
         op->sub.emplace_back(ell_op(std::move(M), dtype));
         add_tmp(ne);
         add_tmp(ne);
@@ -1894,11 +1894,11 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
         op->rows = op->cols = ne;
         op->sub.emplace_back(create(m, DEC_OP_INV_HODGE, 1, 0, DEC_GEOMETRIC_HODGE, dtype));
         dec::HostCsr t1, t2, M, N;
-        spgemm(csr_of(DEC_OP_HODGE, 1, DEC_GEOMETRIC_HODGE), csr_of(DEC_OP_DIFFERENTIAL, 0, hodge), t1);
-        spgemm(t1, csr_of(DEC_OP_INV_HODGE, 0, DEC_GEOMETRIC_HODGE), t2);
-        spgemm(t2, csr_of(DEC_OP_DUAL_DERIVATIVE, 1, hodge), M);
-        spgemm(csr_of(DEC_OP_DUAL_DERIVATIVE, 0, hodge), csr_of(DEC_OP_HODGE, 2, DEC_GEOMETRIC_HODGE), t1);
-        spgemm(t1, csr_of(DEC_OP_DIFFERENTIAL, 1, hodge), N);
+        dec::spgemm(csr_of(DEC_OP_HODGE, 1, DEC_GEOMETRIC_HODGE), csr_of(DEC_OP_DIFFERENTIAL, 0, hodge), t1);
+        dec::spgemm(t1, csr_of(DEC_OP_INV_HODGE, 0, DEC_GEOMETRIC_HODGE), t2);
+        dec::spgemm(t2, csr_of(DEC_OP_DUAL_DERIVATIVE, 1, hodge), M);
+        dec::spgemm(csr_of(DEC_OP_DUAL_DERIVATIVE, 0, hodge), csr_of(DEC_OP_HODGE, 2, DEC_GEOMETRIC_HODGE), t1);
+        dec::spgemm(t1, csr_of(DEC_OP_DIFFERENTIAL, 1, hodge), N);
         op->sub.emplace_back(ell_op(std::move(M), dtype));
         op->sub.emplace_back(ell_op(std::move(N), dtype));
         add_tmp(ne);

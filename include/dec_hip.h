@@ -166,6 +166,12 @@ int dec_op_destroy(dec_op op);
  * project=1 rescales every new point to the unit sphere (icosphere family). */
 int dec_mg_create(dec_mesh base, int32_t levels, int32_t project, int32_t hodge,
                   int32_t center, int32_t steps, dec_mg *out);
+/* The same with mode = DEC_MG_DIRECT (DirectMode(): every level dualized and
+ * discretized) or DEC_MG_GALERKIN (GalerkinMode(): only the finest level is
+ * dualized; A_{k+1} = R_k A_k P_k, Multigrid.jl:558-569, 740-748). */
+enum { DEC_MG_DIRECT = 0, DEC_MG_GALERKIN = 1 };
+int dec_mg_create_mode(dec_mesh base, int32_t levels, int32_t project, int32_t hodge,
+                       int32_t center, int32_t steps, int32_t mode, dec_mg *out);
 int dec_mg_nlevels(dec_mg mg, int32_t *nlevels);
 int dec_mg_level_size(dec_mg mg, int32_t level, int32_t *n);
 /* The fine mesh of the hierarchy (owned by the hierarchy; do not destroy). */

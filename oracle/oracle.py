@@ -412,10 +412,12 @@ CG, GS, JACOBI = 0, 1, 2
 class Hierarchy:
     """MultigridData(s, BinarySubdivision(), levels, op, steps) in DirectMode
     (src/Multigrid.jl:752-797): operators finest first, R_i / S_i between
-    level i and i+1 (P_i = transpose(S_i))."""
+    level i and i+1 (P_i = transpose(S_i)).  galerkin=True is GalerkinMode
+    (src/Multigrid.jl:740-748): only the finest level is dualized and
+    A_{i+1} = (R_i A_i) P_i."""
 
     def __init__(self, base: Mesh, levels: int, project=False, geometric=False,
-                 circumcenter=True):
+                 circumcenter=True, galerkin=False):
         meshes = [base]
         S = []
         m = base
@@ -431,6 +433,9 @@ class Hierarchy:
         self.A = []
         self.duals = []
         for k, mm in enumerate(meshes):
+            if galerkin and k > 0:
+                self.A.append((self.R[k - 1] @ self.A[k - 1]) @ self.S[k - 1].transpose())
+                continue
             mesh = mm
             if k == len(meshes) - 1:
                 # the base level is dualized from the caller's mesh, which may

@@ -74,6 +74,30 @@ def test_reference_thresholds_cg(O, cs, grid_pair):
     assert np.linalg.norm(u_g - u_o) <= 1e-8 * np.linalg.norm(u_o)
 
 
+def test_galerkin_mode(O, cs):
+    """GalerkinMode (test/Multigrid.jl:113-148, BinarySubdivision): coarse
+    operators R A P equal the oracle's (both accumulate each entry over k
+    ascending), and the reference's residual thresholds hold on the GPU."""
+    import torch
+
+    base_o = O.triangulated_grid(1, 1, 1 / 4, math.sqrt(3) / 2 * 1 / 4, compress=False)
+    H = O.Hierarchy(base_o, 4, geometric=True, galerkin=True)
+    base_g = cs.triangulated_grid(1, 1, 1 / 4, math.sqrt(3) / 2 * 1 / 4, False)
+    md = cs.MultigridData(base_g, cs.BinarySubdivision, 4, cs.GeometricHodge, 3, mode=cs.GalerkinMode)
+    assert md.sizes == [a.m for a in H.A]
+    for lv in range(md.nlevels):
+        csr_equal(md.export(lv, "A"), H.A[lv].csr(), exact=False)
+    L = H.A[0]
+    b = rhs(O, L, 9)
+    db = torch.tensor(b, device="cuda")
+    res = lambda u: np.linalg.norm(L.matvec(u.cpu().numpy()) - b) / np.linalg.norm(b)  # noqa: E731
+    z = lambda: torch.zeros(L.n, dtype=torch.float64, device="cuda")  # noqa: E731
+    assert res(cs.multigrid_vcycles(z(), db, md, 5)) < 2e-6
+    assert res(cs.multigrid_wcycles(z(), db, md, 5)) < 7e-7
+    assert res(cs.full_multigrid(db, md, 5)) < 1e-3
+    assert res(cs.full_multigrid(db, md, 5, cs.cg, 2)) < 5e-6
+
+
 @pytest.mark.parametrize("alg", ["gs", "jacobi"])
 def test_smoothers_match_oracle(O, cs, sphere_pair, alg):
     import torch

@@ -303,6 +303,30 @@ def test_multigrid_thresholds(O, grid_hierarchy):
     assert r1 < r0
 
 
+def test_galerkin_hierarchy_and_thresholds(O, grid_hierarchy):
+    """GalerkinMode (test/Multigrid.jl:113-148 with BinarySubdivision): the
+    finest operator is the direct one, coarse operators keep the coarse mesh's
+    stencil and annihilate constants (R and P preserve them), and the
+    reference's residual thresholds hold."""
+    base = O.triangulated_grid(1, 1, 1 / 4, math.sqrt(3) / 2 * 1 / 4, compress=False)
+    G = O.Hierarchy(base, 4, geometric=True, galerkin=True)
+    D = grid_hierarchy
+    assert np.array_equal(G.A[0].csr()[2], D.A[0].csr()[2])
+    for k in range(1, 5):
+        rp, ci, vv = G.A[k].csr()
+        drp, dci, _ = D.A[k].csr()
+        assert np.array_equal(rp, drp) and np.array_equal(ci, dci)
+        assert np.abs(G.A[k].matvec(np.ones(G.A[k].n))).max() <= 1e-12 * np.abs(vv).max()
+    L = G.A[0]
+    b = L.matvec(O.uniform_form(9, L.n) * 0.5 + 0.5)
+    u0 = np.zeros(L.n)
+    rel = lambda u: np.linalg.norm(L.matvec(u) - b) / np.linalg.norm(b)  # noqa: E731
+    assert rel(G.mu_cycles(u0, b, 5, alg=O.CG, mu=1)) < 2e-6
+    assert rel(G.mu_cycles(u0, b, 5, alg=O.CG, mu=2)) < 7e-7
+    assert rel(G.full_multigrid(b, 5, alg=O.CG, mu=1)) < 1e-3
+    assert rel(G.full_multigrid(b, 5, alg=O.CG, mu=2)) < 5e-6
+
+
 def test_gauss_seidel_direct_vcycle(O, grid_hierarchy):
     H = grid_hierarchy
     L = H.A[0]
