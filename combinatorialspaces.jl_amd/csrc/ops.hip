@@ -373,7 +373,7 @@ __global__ void k_wedge_pd(int ne, int width, const int *__restrict__ col, const
 // the stop flag is set, so a fixed launch sequence needs no host round trip.
 constexpr int kCgBlocks = 256;
 constexpr int kCgChunk = 8;  // most iterations between host reads of the stop flag
-enum { CS_BB = 0, CS_RR, CS_DONE, CS_ITER, CS_RZ, CS_PQ, CS_ALPHA, CS_BETA, CS_OMEGA, CS_RHO, CS_COUNT };
+enum { CS_BB = 0, CS_RR, CS_DONE, CS_ITER, CS_RZ, CS_PQ, CS_ALPHA, CS_BETA, CS_OMEGA, CS_RHO, CS_RZ0, CS_RZ1, CS_COUNT };
 
 // part[blk] = Σ a·b, part[kCgBlocks + blk] = Σ c·d (c, d may be NULL)
 template <class T>
@@ -554,6 +554,162 @@ template <class T>
 __global__ void k_pm(int n, T sa, const T *__restrict__ a, T sb, const T *__restrict__ b, T *__restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = sa * a[i] + sb * b[i];
+}
+
+// ---- fused conjugate gradients: two launches per iteration ----
+// Every block of a launch reduces the previous launch's kFcBlocks partial sums
+// itself, in one fixed order, so all blocks hold the same scalars and no
+// scalar kernel runs between the vector kernels.  The search direction is
+// double-buffered: the SpMV launch writes p_new = z + β p_old for its rows
+// and forms the same values for its neighbours from z and p_old.
+constexpr int kFcBlocks = 1024;
+
+__device__ __forceinline__ void fc_tree(double *lds) {
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      lds[threadIdx.x] += lds[threadIdx.x + w];
+      lds[256 + threadIdx.x] += lds[256 + threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+}
+
+// (Σ part[0..kFcBlocks), Σ part[kFcBlocks..2 kFcBlocks)), the same in every block
+__device__ __forceinline__ void fc_sum(const double *__restrict__ part, double &s1, double &s2, double *lds) {
+  double a = 0.0, b = 0.0;
+  for (int j = threadIdx.x; j < kFcBlocks; j += 256) {
+    a += part[j];
+    b += part[kFcBlocks + j];
+  }
+  lds[threadIdx.x] = a;
+  lds[256 + threadIdx.x] = b;
+  __syncthreads();
+  fc_tree(lds);
+  s1 = lds[0];
+  s2 = lds[256];
+  __syncthreads();
+}
+
+__device__ __forceinline__ void fc_partial(double a, double b, double *__restrict__ part, double *lds) {
+  lds[threadIdx.x] = a;
+  lds[256 + threadIdx.x] = b;
+  __syncthreads();
+  fc_tree(lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = lds[0];
+    part[kFcBlocks + blockIdx.x] = lds[256];
+  }
+}
+
+// x = 0, r = b, z = D⁻¹ b, p_old = 0, scalars cleared; partials r·z, r·r
+template <class T>
+__global__ __launch_bounds__(256) void k_fcg_init(int n, const T *__restrict__ b, const T *__restrict__ dinv,
+                                                  T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
+                                                  T *__restrict__ p0, double *__restrict__ partB,
+                                                  double *__restrict__ sc) {
+  __shared__ double lds[512];
+  if (blockIdx.x == 0 && threadIdx.x < CS_COUNT) sc[threadIdx.x] = 0.0;  // clears the stop flag
+  double a1 = 0.0, a2 = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kFcBlocks * 256) {
+    T bi = b[i];
+    T zi = dinv[i] * bi;
+    x[i] = static_cast<T>(0);
+    r[i] = bi;
+    z[i] = zi;
+    p0[i] = static_cast<T>(0);
+    a1 += static_cast<double>(bi) * static_cast<double>(zi);
+    a2 += static_cast<double>(bi) * static_cast<double>(bi);
+  }
+  fc_partial(a1, a2, partB, lds);
+}
+
+// Iteration `it`: the stopping test on the residual after `it` updates, then
+// p_new = z + β p_old, q = A p_new and partials p_new·q.  `last` only tests.
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_fcg_spmv(int n, int width, const int *__restrict__ col,
+                                                  const T *__restrict__ val, const T *__restrict__ z,
+                                                  const T *__restrict__ pold, T *__restrict__ pnew,
+                                                  T *__restrict__ q, const double *__restrict__ partB,
+                                                  double *__restrict__ partA, double *__restrict__ sc, int it,
+                                                  int last, double rtol) {
+  __shared__ double lds[512];
+  if (sc[CS_DONE] != 0.0) return;
+  double rz, rr;
+  fc_sum(partB, rz, rr, lds);
+  double bb = it == 0 ? rr : sc[CS_BB];
+  bool conv = rr <= rtol * rtol * bb;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (it == 0) sc[CS_BB] = rr;
+    sc[CS_RR] = rr;
+    sc[CS_ITER] = it;
+    sc[CS_RZ0 + (it & 1)] = rz;
+    if (conv) sc[CS_DONE] = 1.0;
+  }
+  if (conv || last) return;
+  double rzp = it == 0 ? 0.0 : sc[CS_RZ0 + ((it - 1) & 1)];
+  T beta = static_cast<T>(rzp != 0.0 ? rz / rzp : 0.0);
+  double acc = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kFcBlocks * 256) {
+    T pi = it == 0 ? z[i] : z[i] + beta * pold[i];
+    pnew[i] = pi;
+    T qi = static_cast<T>(0);
+    if constexpr (W > 0) {
+      int c[W];
+      T v[W];
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        size_t o = static_cast<size_t>(k) * n + i;
+        c[k] = col[o];
+        v[k] = val[o];
+      }
+#pragma unroll
+      for (int k = 0; k < W; k++)
+        if (c[k] >= 0) {
+          T pc = it == 0 ? z[c[k]] : z[c[k]] + beta * pold[c[k]];
+          qi = qi + v[k] * pc;
+        }
+    } else {
+      for (int k = 0; k < width; k++) {
+        size_t o = static_cast<size_t>(k) * n + i;
+        int c = col[o];
+        if (c < 0) break;
+        T pc = it == 0 ? z[c] : z[c] + beta * pold[c];
+        qi = qi + val[o] * pc;
+      }
+    }
+    q[i] = qi;
+    acc += static_cast<double>(pi) * static_cast<double>(qi);
+  }
+  fc_partial(acc, 0.0, partA, lds);
+}
+
+// x += α p, r -= α q, z = D⁻¹ r; partials r·z, r·r
+template <class T>
+__global__ __launch_bounds__(256) void k_fcg_update(int n, const T *__restrict__ p, const T *__restrict__ q,
+                                                    const T *__restrict__ dinv, T *__restrict__ x,
+                                                    T *__restrict__ r, T *__restrict__ z,
+                                                    const double *__restrict__ partA, double *__restrict__ partB,
+                                                    double *__restrict__ sc, int it) {
+  __shared__ double lds[512];
+  if (sc[CS_DONE] != 0.0) return;
+  double pq, unused;
+  fc_sum(partA, pq, unused, lds);
+  if (pq == 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[CS_DONE] = 1.0;
+    return;
+  }
+  T alpha = static_cast<T>(sc[CS_RZ0 + (it & 1)] / pq);
+  double a1 = 0.0, a2 = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kFcBlocks * 256) {
+    x[i] = x[i] + alpha * p[i];
+    T ri = r[i] - alpha * q[i];
+    T zi = dinv[i] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    a1 += static_cast<double>(ri) * static_cast<double>(zi);
+    a2 += static_cast<double>(ri) * static_cast<double>(ri);
+  }
+  fc_partial(a1, a2, partB, lds);
 }
 
 template <class T>
@@ -977,19 +1133,25 @@ void ginv_apply(dec_op_s &op, const T *bvec, T *x, hipStream_t s) {
   const bool bicg = !op.symmetric;
   dim3 g(dec::nblocks(n)), blk(256), gd(kCgBlocks), one(1);
   // the start reads the stop flag of czero (always clear), not of sc
-  hipLaunchKernelGGL(k_cg_init<T>, g, blk, 0, s, n, bicg ? 1 : 0, bvec, dinv, x, r, z, p, rh, q);
-  hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, bicg ? rh : r, bicg ? r : z, r, r, part, op.czero.p);
-  hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, bicg ? PH_BI_START : PH_CG_START, part, sc, op.rtol);
+  dim3 gf(kFcBlocks);
+  double *partA = op.cpart.p + 2 * kCgBlocks, *partB = partA + 2 * kFcBlocks;
+  T *pbuf[2] = {p, rh};  // CG: the direction, double-buffered
+  if (bicg) {
+    hipLaunchKernelGGL(k_cg_init<T>, g, blk, 0, s, n, 1, bvec, dinv, x, r, z, p, rh, q);
+    hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, rh, r, r, r, part, op.czero.p);
+    hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_BI_START, part, sc, op.rtol);
+  } else {
+    hipLaunchKernelGGL(k_fcg_init<T>, gf, blk, 0, s, n, bvec, dinv, x, r, z, pbuf[0], partB, sc);
+  }
+  int it = 0;
   auto iteration = [&]() {
     if (!bicg) {
-      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_cg_spmv<W, T>), g, blk, 0, s, n, E->width, E->col.p,
-                                                        E->val.p, p, q, sc));
-      hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, p, q, nul, nul, part, sc);
-      hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_CG_ALPHA, part, sc, op.rtol);
-      hipLaunchKernelGGL(k_cg_update<T>, g, blk, 0, s, n, p, q, dinv, x, r, z, sc);
-      hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, r, z, r, r, part, sc);
-      hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_CG_BETA, part, sc, op.rtol);
-      hipLaunchKernelGGL(k_cg_dir<T>, g, blk, 0, s, n, z, p, sc);
+      T *pold = pbuf[it & 1], *pnew = pbuf[(it + 1) & 1];
+      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_fcg_spmv<W, T>), gf, blk, 0, s, n, E->width, E->col.p,
+                                                        E->val.p, z, pold, pnew, q, partB, partA, sc, it, 0,
+                                                        op.rtol));
+      hipLaunchKernelGGL(k_fcg_update<T>, gf, blk, 0, s, n, pnew, q, dinv, x, r, z, partA, partB, sc, it);
+      it++;
     } else {
       // q holds v = A y, z holds y and then D⁻¹ s, t holds A z
       hipLaunchKernelGGL(k_bi_dir<T>, g, blk, 0, s, n, r, q, dinv, p, z, sc);
@@ -1009,11 +1171,21 @@ void ginv_apply(dec_op_s &op, const T *bvec, T *x, hipStream_t s) {
   };
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   hip_status(hipStreamIsCapturing(s, &cap), "hipStreamIsCapturing");
+  // the stopping test on the residual after the latest iteration, with no
+  // update (CG; the next iteration repeats it and goes on if it failed)
+  auto final_test = [&]() {
+    if (!bicg)
+      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_fcg_spmv<W, T>), gf, blk, 0, s, n, E->width, E->col.p,
+                                                        E->val.p, z, pbuf[it & 1], pbuf[(it + 1) & 1], q, partB,
+                                                        partA, sc, it, 1, op.rtol));
+  };
   // checks after 2, 6, 14, 22, ... iterations: the diagonal-dominant ⋆1
   // converges in a few, so the first checks come early
   int chunk = cap == hipStreamCaptureStatusNone ? 2 : op.max_iter;
   for (int done_it = 0; done_it < op.max_iter; done_it += chunk, chunk = std::min(2 * chunk, kCgChunk)) {
     for (int k = 0; k < chunk && done_it + k < op.max_iter; k++) iteration();
+    // test the latest residual now, so that the flag read below sees it
+    if (done_it + chunk >= op.max_iter || cap == hipStreamCaptureStatusNone) final_test();
     if (cap != hipStreamCaptureStatusNone) break;
     double flag = 0.0;
     hip_status(hipMemcpyAsync(&flag, sc + CS_DONE, sizeof(double), hipMemcpyDeviceToHost, s), "stop flag copy");
@@ -1769,7 +1941,7 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
           }
           op->symmetric = asym <= 1e-12 * amax;
           for (auto *v : {&op->cr, &op->cp, &op->cq, &op->cz, &op->crh, &op->ct}) v->alloc(es * ne);
-          op->cpart.alloc(2 * kCgBlocks);
+          op->cpart.alloc(2 * kCgBlocks + 4 * kFcBlocks);
           op->cscal.alloc(CS_COUNT);
           op->czero.alloc(CS_COUNT);  // a scalar block whose stop flag is always clear
           {
