@@ -34,6 +34,7 @@ SIGNATURES = {
     "dec_version": (c_char_p, []),
     "dec_fill_uniform": (c_int, [P, c_int64, c_uint64, c_int, P]),
     "dec_mesh_create": (c_int, [c_int32, c_int32, c_int32, P, P, P, P, P, P, P, P, c_int32, POINTER(P)]),
+    "dec_mesh_from_triangles": (c_int, [c_int32, c_int32, P, P, c_int32, c_int32, POINTER(P)]),
     "dec_mesh_icosphere": (c_int, [c_int32, POINTER(P)]),
     "dec_mesh_triangulated_grid": (c_int, [c_double, c_double, c_double, c_double, c_int32, POINTER(P)]),
     "dec_mesh_subdivide": (c_int, [P, c_int32, POINTER(P)]),

@@ -137,6 +137,18 @@ def mesh_from_arrays(points, v0, v1, e0, e1, e2, edge_orientation=None, tri_orie
     return Mesh(h)
 
 
+def mesh_from_triangles(points, triangles, sorted_=True, index_base=0) -> Mesh:
+    """EmbeddedDeltaSet2D from a triangle list: glue_sorted_triangle! (or
+    glue_triangle! with sorted_=False) per triangle, then orient!
+    (src/MeshInterop.jl:116-131)."""
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+    h = c_void_p()
+    check(lib().dec_mesh_from_triangles(len(p), len(t), ptr(p), ptr(t), int(bool(sorted_)), index_base,
+                                        ctypes.byref(h)))
+    return Mesh(h)
+
+
 def icosphere(k: int) -> Mesh:
     """Pole-aligned icosahedron refined k times, new points on the unit sphere."""
     h = c_void_p()

@@ -336,6 +336,30 @@ int dec_mesh_icosphere(int32_t k, dec_mesh *out) {
   DEC_CATCH
 }
 
+int dec_mesh_from_triangles(int32_t nv, int32_t nt, const double *points, const int32_t *tris, int32_t sorted,
+                            int32_t index_base, dec_mesh *out) {
+  DEC_TRY
+  if (!out) fail(DEC_ERR_ARG, "dec_mesh_from_triangles: out is NULL");
+  *out = nullptr;
+  if (nv < 0 || nt < 0) fail(DEC_ERR_ARG, "dec_mesh_from_triangles: negative count");
+  if ((nv > 0 && !points) || (nt > 0 && !tris)) fail(DEC_ERR_ARG, "dec_mesh_from_triangles: NULL array");
+  if (index_base != 0 && index_base != 1) fail(DEC_ERR_ARG, "dec_mesh_from_triangles: index_base must be 0 or 1");
+  std::vector<int32_t> tv(3 * static_cast<size_t>(nt));
+  for (size_t i = 0; i < tv.size(); i++) tv[i] = tris[i] - index_base;
+  dec::Topo t;
+  dec::glue_triangles(nv, tv, sorted != 0, t);
+  // orient!(s) right after gluing (MeshInterop.jl:129); a non-orientable
+  // complex keeps its triangles unset and is refused when dualized
+  std::vector<int8_t> to;
+  bool oriented = dec::orient_dfs(t, to);
+  if (oriented) t.to = to;
+  std::vector<double> p(points, points + 3 * static_cast<size_t>(nv));
+  dec_mesh_s *m = new_mesh_from_topo(std::move(t), p);
+  m->to_dfs = oriented;
+  *out = m;
+  DEC_CATCH
+}
+
 int dec_mesh_triangulated_grid(double lx, double ly, double dx, double dy, int32_t compress, dec_mesh *out) {
   DEC_TRY
   if (!out) fail(DEC_ERR_ARG, "dec_mesh_triangulated_grid: out is NULL");

@@ -84,6 +84,13 @@ int dec_mesh_create(int32_t nv, int32_t ne, int32_t nt, const int32_t *v0,
  * points projected to the unit sphere (replaces loadmesh(Icosphere(n)),
  * CombMeshes.jl:31-36, whose artifact is a downloaded file). */
 int dec_mesh_icosphere(int32_t k, dec_mesh *out);
+/* EmbeddedDeltaSet2D(mesh) from a triangle list (MeshInterop.jl:116-131):
+ * glue_sorted_triangle! (sorted = 1) or glue_triangle! (sorted = 0) for each
+ * triangle in order (SimplicialSets.jl:343-364), then orient!.  points: nv x 3
+ * float64; tris: nt x 3 vertex ids counted from index_base. */
+int dec_mesh_from_triangles(int32_t nv, int32_t nt, const double *points,
+                            const int32_t *tris, int32_t sorted, int32_t index_base,
+                            dec_mesh *out);
 /* triangulated_grid(lx, ly, dx, dy, Point3d, compress) (CombMeshes.jl:79-130). */
 int dec_mesh_triangulated_grid(double lx, double ly, double dx, double dy,
                                int32_t compress, dec_mesh *out);

@@ -1,0 +1,107 @@
+"""Mesh files (cspaces/meshio.py) parsed on the host: OBJ faces (fans, the
+v/vt/vn forms, negative indices), binary and ASCII STL with points merged by
+position in order of first occurrence (src/MeshInterop.jl:85-104, 154-165),
+Float32 coordinates as MeshIO reads them, and OBJ output.  The GPU test builds
+library meshes from files and compares them with the oracle's glue + orient!."""
+import struct
+
+import numpy as np
+import pytest
+
+
+def test_obj_faces_are_fans_and_indices_resolve(tmp_path):
+    from cspaces import meshio
+
+    f = tmp_path / "quad.obj"
+    f.write_text("# a quad and a triangle\n"
+                 "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0.1\nv 2 0 0\n"
+                 "vt 0 0\nvn 0 0 1\n"
+                 "f 1/1/1 2/1/1 3/1/1 4/1/1\n"
+                 "f -4//1 5//1 -3//1\n")
+    P, T = meshio.read_obj(str(f))
+    assert P.shape == (5, 3) and P.dtype == np.float64
+    assert P[3, 2] == np.float64(np.float32(0.1))               # read as Float32
+    assert T.tolist() == [[0, 1, 2], [0, 2, 3], [1, 4, 2]]
+
+
+def _square_corners():
+    # two triangles of the unit square sharing the diagonal corners 0 and 2
+    return [[(0, 0, 0), (1, 0, 0), (1, 1, 0)], [(0, 0, 0), (1, 1, 0), (0, 1, 0)]]
+
+
+def test_binary_and_ascii_stl_merge_points_in_first_occurrence_order(tmp_path):
+    from cspaces import meshio
+
+    tris = _square_corners()
+    b = tmp_path / "sq.stl"
+    with open(b, "wb") as fh:
+        fh.write(b"\0" * 80 + struct.pack("<I", len(tris)))
+        for t in tris:
+            fh.write(struct.pack("<3f", 0, 0, 1))
+            for c in t:
+                fh.write(struct.pack("<3f", *c))
+            fh.write(struct.pack("<H", 0))
+    a = tmp_path / "sq_ascii.stl"
+    a.write_text("solid sq\n" + "".join(
+        "facet normal 0 0 1\nouter loop\n" + "".join(f"vertex {x} {y} {z}\n" for x, y, z in t) +
+        "endloop\nendfacet\n" for t in tris) + "endsolid sq\n")
+    for f in (b, a):
+        P, T = meshio.read_mesh(str(f))
+        assert P.tolist() == [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]]
+        assert T.tolist() == [[0, 1, 2], [0, 2, 3]]
+        P2, T2 = meshio.read_stl(str(f), force_unique=False)       # six corners, unmerged
+        assert len(P2) == 6 and T2.tolist() == [[0, 1, 2], [3, 4, 5]]
+
+
+def test_obj_round_trip_and_glue(O, tmp_path):
+    from cspaces import meshio
+
+    m = O.icosphere(2)
+    P = m.p.astype(np.float32).astype(np.float64)                   # Float32-representable
+    tv = m.triangle_vertices()
+    f = tmp_path / "ico2.obj"
+    meshio.write_obj(str(f), P, tv)
+    P2, T2 = meshio.read_obj(str(f))
+    assert np.array_equal(P2, P) and np.array_equal(T2, tv)
+    g = O.glue_triangles(P2, T2, sorted_=True)                      # what load_mesh glues
+    assert (g.nv, g.ne, g.nt) == (162, 480, 320) and g.nv - g.ne + g.nt == 2
+    assert O.orient(g)
+
+
+def test_unknown_extension_is_refused(tmp_path):
+    from cspaces import meshio
+
+    f = tmp_path / "m.ply"
+    f.write_text("ply\n")
+    with pytest.raises(ValueError):
+        meshio.read_mesh(str(f))
+
+
+@pytest.mark.gpu
+def test_load_mesh_matches_the_oracle_glue(O, cs, tmp_path):
+    from cspaces import meshio
+
+    m = O.icosphere(2)
+    P = m.p.astype(np.float32).astype(np.float64)
+    f = tmp_path / "ico2.obj"
+    meshio.write_obj(str(f), P, m.triangle_vertices())
+    gm = cs.load_mesh(str(f))
+    P2, T2 = meshio.read_obj(str(f))
+    om = O.glue_triangles(P2, T2, sorted_=True)
+    assert O.orient(om)
+    a = gm.export()
+    for k in ("v0", "v1", "e0", "e1", "e2"):
+        assert np.array_equal(a[k], getattr(om, k)), k
+    assert np.array_equal(a["tri_orientation"], om.to)
+    assert np.array_equal(a["points"], P2)
+    # the loaded mesh dualizes and its Laplacian annihilates constants
+    cs.dualize(gm)
+    import torch
+    L = cs.fast_laplace_beltrami(gm, cs.DiagonalHodge, cast_float=torch.float64)
+    one = torch.ones(gm.nv, dtype=torch.float64, device="cuda")
+    assert (L @ one).abs().max().item() <= 1e-10
+    # written back, the library mesh gives the same triangles
+    g = tmp_path / "back.obj"
+    meshio.save_obj(str(g), gm)
+    P3, T3 = meshio.read_obj(str(g))
+    assert np.array_equal(T3, om.triangle_vertices()) and np.array_equal(P3, P2)
