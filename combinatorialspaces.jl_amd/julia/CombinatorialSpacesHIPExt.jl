@@ -15,7 +15,8 @@ module CombinatorialSpacesHIPExt
 
 using CombinatorialSpaces
 using CombinatorialSpaces.DiscreteExteriorCalculus: DiscreteHodge
-using CombinatorialSpaces.Multigrid: BinarySubdivision, AbstractSubdivisionScheme
+using CombinatorialSpaces.Multigrid: BinarySubdivision, AbstractSubdivisionScheme,
+  AbstractMultigridMode, DirectMode, GalerkinMode
 using AMDGPU
 using Krylov
 using LinearAlgebra
@@ -197,15 +198,22 @@ mutable struct HIPMultigridData
   steps::Int
 end
 
-"""HIP counterpart of `MultigridData(s, BinarySubdivision(), levels, op, steps)`
-(src/Multigrid.jl:698-702) with op = fast_laplace_beltrami using `hodge` for ⋆1."""
+const DEC_MG_DIRECT, DEC_MG_GALERKIN = Int32(0), Int32(1)
+mode_code(::DirectMode) = DEC_MG_DIRECT
+mode_code(::GalerkinMode) = DEC_MG_GALERKIN
+
+"""HIP counterpart of `MultigridData(s, BinarySubdivision(), levels, op, steps;
+mode)` (src/Multigrid.jl:698-702) with op = fast_laplace_beltrami using `hodge`
+for ⋆1; `mode` is DirectMode() or GalerkinMode()."""
 function HIPMultigridData(s::EmbeddedDeltaSet2D, ::BinarySubdivision, levels::Int, steps::Int;
-                          hodge::DiscreteHodge=GeometricHodge(), project::Bool=false)
+                          hodge::DiscreteHodge=GeometricHodge(), project::Bool=false,
+                          mode::AbstractMultigridMode=DirectMode())
   base = hip_mesh(dualize(s, Circumcenter()))
   h = Ref{Ptr{Cvoid}}(C_NULL)
-  check(ccall((:dec_mg_create, libdec_hip), Cint,
-    (Ptr{Cvoid}, Int32, Int32, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
-    base.handle, Int32(levels), Int32(project), hodge_code(hodge), DEC_CIRCUMCENTER, Int32(steps), h))
+  check(ccall((:dec_mg_create_mode, libdec_hip), Cint,
+    (Ptr{Cvoid}, Int32, Int32, Int32, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
+    base.handle, Int32(levels), Int32(project), hodge_code(hodge), DEC_CIRCUMCENTER, Int32(steps),
+    mode_code(mode), h))
   n = Ref{Int32}(0)
   check(ccall((:dec_mg_level_size, libdec_hip), Cint, (Ptr{Cvoid}, Int32, Ref{Int32}), h[], 0, n))
   md = HIPMultigridData(h[], n[], steps)

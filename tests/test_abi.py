@@ -49,3 +49,25 @@ def test_unsupported_degrees_are_reported_on_the_host():
         cs.dec_wedge_product_pd(1, 0, None)
     with pytest.raises(cs.DecError, match="currently supported dual-primal wedges are 0-1, 1-0, and 1-1"):
         cs.dec_wedge_product_dp(0, 1, None)
+
+
+def test_new_entry_points_reject_bad_arguments_before_device_work():
+    """Argument errors of the triangle-list mesh, the multigrid mode and the
+    solve statistics are reported with a status and a message, without a GPU."""
+    import numpy as np
+    from cspaces import _lib
+
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    pts = np.zeros((3, 3))
+    bad = np.array([[0, 1, 3]], np.int32)                       # vertex 3 of 3 points
+    p = pts.ctypes.data_as(ctypes.c_void_p)
+    assert lib.dec_mesh_from_triangles(3, 1, p, bad.ctypes.data_as(ctypes.c_void_p), 1, 0, ctypes.byref(h)) == 1
+    assert b"out of range" in lib.dec_last_error()
+    good = np.array([[1, 2, 3]], np.int32)
+    assert lib.dec_mesh_from_triangles(3, 1, p, good.ctypes.data_as(ctypes.c_void_p), 1, 2, ctypes.byref(h)) == 1
+    assert b"index_base" in lib.dec_last_error()
+    assert lib.dec_mesh_from_triangles(3, 1, None, None, 1, 0, ctypes.byref(h)) == 1
+    assert lib.dec_mg_create_mode(None, 2, 0, 0, 0, 2, 0, ctypes.byref(h)) == 1
+    it, rr = ctypes.c_int32(), ctypes.c_double()
+    assert lib.dec_op_solve_info(None, ctypes.byref(it), ctypes.byref(rr), None) == 1
