@@ -517,6 +517,8 @@ def main():
     ap.add_argument("--learned-steps", type=int, default=2000)
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--pmc-probe", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--composite-only", action="store_true",
+                    help="time only the composite operators (ms per application) and print their JSON")
     args = ap.parse_args()
     if args.pmc_probe:
         pmc_probe(args.pmc_probe, args.k, args.sets, args.steps)
@@ -538,6 +540,9 @@ def main():
     nv, ne, nt = mesh.nv, mesh.ne, mesh.nt
     print(f"# icosphere k={args.k}: {nv} V {ne} E {nt} T, mesh+dual setup {setup_mesh:.2f} s", file=sys.stderr,
           flush=True)
+    if args.composite_only:
+        print(json.dumps({"composite_ops_f64": musical_ops(torch, cs, mesh), "mesh_vertices": nv}), flush=True)
+        return
 
     per, total, grouped, cache, lap_nnz = time_opset(torch, cs, mesh, OPSET, torch.float64, args.sets, args.steps,
                                                      args.warmup, not args.no_cache_resident)
