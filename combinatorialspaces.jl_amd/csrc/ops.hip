@@ -67,7 +67,7 @@ struct dec_op_s {
   // geometric ⋆1⁻¹: the Jacobi scaling 1/diag(⋆1), CG vectors, reduction
   // partials and device-resident scalars (mutable: one apply at a time)
   DBuf<unsigned char> dinv, cr, cp, cq, cz, crh, ct;
-  DBuf<double> cpart, cscal, czero;
+  DBuf<double> cpart, cscal;
   int max_iter = 0;
   double rtol = 0.0;
   bool symmetric = true;  // CG when ⋆1 is symmetric, BiCGSTAB otherwise
@@ -371,190 +371,8 @@ __global__ void k_wedge_pd(int ne, int width, const int *__restrict__ col, const
 // (van der Vorst 1992, right Jacobi preconditioning) is used.  All scalars
 // stay in device memory; every kernel of an iteration returns at once when
 // the stop flag is set, so a fixed launch sequence needs no host round trip.
-constexpr int kCgBlocks = 256;
 constexpr int kCgChunk = 8;  // most iterations between host reads of the stop flag
-enum { CS_BB = 0, CS_RR, CS_DONE, CS_ITER, CS_RZ, CS_PQ, CS_ALPHA, CS_BETA, CS_OMEGA, CS_RHO, CS_RZ0, CS_RZ1, CS_COUNT };
-
-// part[blk] = Σ a·b, part[kCgBlocks + blk] = Σ c·d (c, d may be NULL)
-template <class T>
-__global__ void k_cg_dots(int n, const T *__restrict__ a, const T *__restrict__ b, const T *__restrict__ c,
-                          const T *__restrict__ d, double *__restrict__ part, const double *__restrict__ sc) {
-  if (sc[CS_DONE] != 0.0) return;
-  __shared__ double s1[256], s2[256];
-  double x = 0.0, y = 0.0;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kCgBlocks * 256) {
-    x += static_cast<double>(a[i]) * static_cast<double>(b[i]);
-    if (d) y += static_cast<double>(c[i]) * static_cast<double>(d[i]);
-  }
-  s1[threadIdx.x] = x;
-  s2[threadIdx.x] = y;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      s1[threadIdx.x] += s1[threadIdx.x + w];
-      s2[threadIdx.x] += s2[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = s1[0];
-    part[kCgBlocks + blockIdx.x] = s2[0];
-  }
-}
-
-enum { PH_CG_START = 0, PH_CG_ALPHA, PH_CG_BETA, PH_BI_START, PH_BI_ALPHA, PH_BI_OMEGA, PH_BI_BETA };
-
-// One thread sums the two partial arrays and advances the recurrences.
-__global__ void k_cg_scalars(int phase, const double *__restrict__ part, double *__restrict__ sc, double rtol) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (phase != PH_CG_START && phase != PH_BI_START && sc[CS_DONE] != 0.0) return;
-  double s1 = 0.0, s2 = 0.0;
-  for (int j = 0; j < kCgBlocks; j++) {
-    s1 += part[j];
-    s2 += part[kCgBlocks + j];
-  }
-  double tol2 = rtol * rtol;
-  switch (phase) {
-    case PH_CG_START:  // s1 = r·z, s2 = r·r with r = b
-    case PH_BI_START:  // s1 = r̂·r, s2 = r·r with r̂ = r = b
-      sc[CS_RZ] = sc[CS_RHO] = s1;
-      sc[CS_BB] = sc[CS_RR] = s2;
-      sc[CS_ALPHA] = sc[CS_OMEGA] = 1.0;
-      sc[CS_BETA] = 0.0;
-      sc[CS_ITER] = 0.0;
-      sc[CS_DONE] = s2 == 0.0 ? 1.0 : 0.0;
-      break;
-    case PH_CG_ALPHA:  // s1 = p·Ap
-      sc[CS_PQ] = s1;
-      if (s1 != 0.0) sc[CS_ALPHA] = sc[CS_RZ] / s1;
-      else sc[CS_DONE] = 1.0;
-      break;
-    case PH_CG_BETA:  // s1 = r·z, s2 = r·r after the update
-      sc[CS_BETA] = sc[CS_RZ] != 0.0 ? s1 / sc[CS_RZ] : 0.0;
-      sc[CS_RZ] = s1;
-      sc[CS_RR] = s2;
-      sc[CS_ITER] += 1.0;
-      if (s2 <= tol2 * sc[CS_BB]) sc[CS_DONE] = 1.0;
-      break;
-    case PH_BI_ALPHA:  // s1 = r̂·v
-      if (s1 != 0.0) sc[CS_ALPHA] = sc[CS_RHO] / s1;
-      else sc[CS_DONE] = 1.0;
-      break;
-    case PH_BI_OMEGA:  // s1 = t·s, s2 = t·t; t = 0 means s = 0: x is the solution
-      if (s2 != 0.0 && s1 != 0.0) {
-        sc[CS_OMEGA] = s1 / s2;
-      } else {
-        sc[CS_OMEGA] = 0.0;
-        sc[CS_DONE] = 1.0;
-      }
-      break;
-    case PH_BI_BETA:  // s1 = r̂·r, s2 = r·r after the update
-      sc[CS_BETA] = (sc[CS_RHO] != 0.0 ? s1 / sc[CS_RHO] : 0.0) * (sc[CS_ALPHA] / sc[CS_OMEGA]);
-      sc[CS_RHO] = s1;
-      sc[CS_RR] = s2;
-      sc[CS_ITER] += 1.0;
-      if (s2 <= tol2 * sc[CS_BB] || s1 == 0.0) sc[CS_DONE] = 1.0;
-      break;
-  }
-}
-
-// x = 0, r = b, z = D⁻¹ r (CG: p = z; BiCGSTAB: r̂ = b, p = v = 0)
-template <class T>
-__global__ void k_cg_init(int n, int bicg, const T *__restrict__ b, const T *__restrict__ dinv, T *__restrict__ x,
-                          T *__restrict__ r, T *__restrict__ z, T *__restrict__ p, T *__restrict__ rh,
-                          T *__restrict__ v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  T bi = b[i];
-  x[i] = static_cast<T>(0);
-  r[i] = bi;
-  T zi = dinv[i] * bi;
-  z[i] = zi;
-  if (bicg) {
-    rh[i] = bi;
-    p[i] = static_cast<T>(0);
-    v[i] = static_cast<T>(0);
-  } else {
-    p[i] = zi;
-  }
-}
-
-template <int W, class T>
-__global__ void k_cg_spmv(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
-                          const T *__restrict__ p, T *__restrict__ q, const double *__restrict__ sc) {
-  if (sc[CS_DONE] != 0.0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) q[i] = dec_ell::row<W, T, T>(n, width, i, col, val, p);
-}
-
-// CG: x += alpha p, r -= alpha q, z = D⁻¹ r
-template <class T>
-__global__ void k_cg_update(int n, const T *__restrict__ p, const T *__restrict__ q, const T *__restrict__ dinv,
-                            T *__restrict__ x, T *__restrict__ r, T *__restrict__ z, const double *__restrict__ sc) {
-  if (sc[CS_DONE] != 0.0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  T a = static_cast<T>(sc[CS_ALPHA]);
-  x[i] = x[i] + a * p[i];
-  T ri = r[i] - a * q[i];
-  r[i] = ri;
-  z[i] = dinv[i] * ri;
-}
-
-// CG: p = z + beta p
-template <class T>
-__global__ void k_cg_dir(int n, const T *__restrict__ z, T *__restrict__ p, const double *__restrict__ sc) {
-  if (sc[CS_DONE] != 0.0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = z[i] + static_cast<T>(sc[CS_BETA]) * p[i];
-}
-
-// BiCGSTAB: p = r + beta (p - omega v), y = D⁻¹ p
-template <class T>
-__global__ void k_bi_dir(int n, const T *__restrict__ r, const T *__restrict__ v, const T *__restrict__ dinv,
-                         T *__restrict__ p, T *__restrict__ y, const double *__restrict__ sc) {
-  if (sc[CS_DONE] != 0.0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  T be = static_cast<T>(sc[CS_BETA]), om = static_cast<T>(sc[CS_OMEGA]);
-  T pi = r[i] + be * (p[i] - om * v[i]);
-  p[i] = pi;
-  y[i] = dinv[i] * pi;
-}
-
-// BiCGSTAB: x += alpha y, s = r - alpha v (kept in r), then yz = D⁻¹ s in
-// place of y
-template <class T>
-__global__ void k_bi_half(int n, T *__restrict__ yz, const T *__restrict__ v, const T *__restrict__ dinv,
-                          T *__restrict__ x, T *__restrict__ r, const double *__restrict__ sc) {
-  if (sc[CS_DONE] != 0.0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  T a = static_cast<T>(sc[CS_ALPHA]);
-  x[i] = x[i] + a * yz[i];
-  T si = r[i] - a * v[i];
-  r[i] = si;
-  yz[i] = dinv[i] * si;
-}
-
-// BiCGSTAB: x += omega z, r = s - omega t
-template <class T>
-__global__ void k_bi_full(int n, const T *__restrict__ z, const T *__restrict__ t, T *__restrict__ x,
-                          T *__restrict__ r, const double *__restrict__ sc) {
-  if (sc[CS_DONE] != 0.0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  T om = static_cast<T>(sc[CS_OMEGA]);
-  x[i] = x[i] + om * z[i];
-  r[i] = r[i] - om * t[i];
-}
-
-// out = sa a + sb b with sa, sb = ±1 (exact scalings)
-template <class T>
-__global__ void k_pm(int n, T sa, const T *__restrict__ a, T sb, const T *__restrict__ b, T *__restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = sa * a[i] + sb * b[i];
-}
+enum { CS_BB = 0, CS_RR, CS_DONE, CS_ITER, CS_ALPHA, CS_OMEGA, CS_RZ0, CS_RZ1, CS_COUNT };
 
 // ---- fused conjugate gradients: two launches per iteration ----
 // Every block of a launch reduces the previous launch's kFcBlocks partial sums
@@ -710,6 +528,146 @@ __global__ __launch_bounds__(256) void k_fcg_update(int n, const T *__restrict__
     a2 += static_cast<double>(ri) * static_cast<double>(ri);
   }
   fc_partial(a1, a2, partB, lds);
+}
+
+// ---- fused BiCGSTAB: five launches per iteration, same scalar scheme ----
+// ρ_it = r̂·r sits in CS_RZ0 + (it & 1); α and ω of an iteration are written
+// by one block of the launch that forms them and read by later launches.
+
+// x = 0, r = r̂ = b, scalars cleared; partials r̂·r, r·r
+template <class T>
+__global__ __launch_bounds__(256) void k_fbi_init(int n, const T *__restrict__ b, T *__restrict__ x,
+                                                  T *__restrict__ r, T *__restrict__ rh, double *__restrict__ partR,
+                                                  double *__restrict__ sc) {
+  __shared__ double lds[512];
+  if (blockIdx.x == 0 && threadIdx.x < CS_COUNT) sc[threadIdx.x] = 0.0;
+  double a = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kFcBlocks * 256) {
+    T bi = b[i];
+    x[i] = static_cast<T>(0);
+    r[i] = bi;
+    rh[i] = bi;
+    a += static_cast<double>(bi) * static_cast<double>(bi);
+  }
+  fc_partial(a, a, partR, lds);
+}
+
+// Iteration `it`: the stopping test on the residual after `it` updates, then
+// p = r + β (p - ω v) and y = D⁻¹ p.  `last` only tests.
+template <class T>
+__global__ __launch_bounds__(256) void k_fbi_dir(int n, const T *__restrict__ r, const T *__restrict__ v,
+                                                 const T *__restrict__ dinv, T *__restrict__ p, T *__restrict__ y,
+                                                 const double *__restrict__ partR, double *__restrict__ sc, int it,
+                                                 int last, double rtol) {
+  __shared__ double lds[512];
+  if (sc[CS_DONE] != 0.0) return;
+  double rho, rr;
+  fc_sum(partR, rho, rr, lds);
+  double bb = it == 0 ? rr : sc[CS_BB];
+  bool stop = rr <= rtol * rtol * bb || rho == 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (it == 0) sc[CS_BB] = rr;
+    sc[CS_RR] = rr;
+    sc[CS_ITER] = it;
+    sc[CS_RZ0 + (it & 1)] = rho;
+    if (stop) sc[CS_DONE] = 1.0;
+  }
+  if (stop || last) return;
+  T beta = static_cast<T>(0), omega = static_cast<T>(0);
+  if (it > 0) {
+    double rhop = sc[CS_RZ0 + ((it - 1) & 1)];
+    beta = static_cast<T>(rho / rhop * (sc[CS_ALPHA] / sc[CS_OMEGA]));
+    omega = static_cast<T>(sc[CS_OMEGA]);
+  }
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kFcBlocks * 256) {
+    T pi = it == 0 ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+    p[i] = pi;
+    y[i] = dinv[i] * pi;
+  }
+}
+
+// q = A x_in and partials a·q, q·q (b may be NULL: the second sum is then q·q)
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_fbi_spmv(int n, int width, const int *__restrict__ col,
+                                                  const T *__restrict__ val, const T *__restrict__ xin,
+                                                  T *__restrict__ q, const T *__restrict__ a, int with_qq,
+                                                  double *__restrict__ part, const double *__restrict__ sc) {
+  __shared__ double lds[512];
+  if (sc[CS_DONE] != 0.0) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kFcBlocks * 256) {
+    T qi = dec_ell::row<W, T, T>(n, width, i, col, val, xin);
+    q[i] = qi;
+    s1 += static_cast<double>(a[i]) * static_cast<double>(qi);
+    if (with_qq) s2 += static_cast<double>(qi) * static_cast<double>(qi);
+  }
+  fc_partial(s1, s2, part, lds);
+}
+
+// α = ρ / r̂·v; x += α y; s = r - α v (kept in r); yz = D⁻¹ s in place of y
+template <class T>
+__global__ __launch_bounds__(256) void k_fbi_half(int n, T *__restrict__ yz, const T *__restrict__ v,
+                                                  const T *__restrict__ dinv, T *__restrict__ x, T *__restrict__ r,
+                                                  const double *__restrict__ partV, double *__restrict__ sc, int it) {
+  __shared__ double lds[512];
+  if (sc[CS_DONE] != 0.0) return;
+  double rv, unused;
+  fc_sum(partV, rv, unused, lds);
+  if (rv == 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[CS_DONE] = 1.0;
+    return;
+  }
+  double ad = sc[CS_RZ0 + (it & 1)] / rv;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[CS_ALPHA] = ad;
+  T alpha = static_cast<T>(ad);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kFcBlocks * 256) {
+    x[i] = x[i] + alpha * yz[i];
+    T si = r[i] - alpha * v[i];
+    r[i] = si;
+    yz[i] = dinv[i] * si;
+  }
+}
+
+// ω = t·s / t·t; x += ω z; r = s - ω t; partials r̂·r, r·r.  t = 0 means s = 0:
+// x already solves the system.
+template <class T>
+__global__ __launch_bounds__(256) void k_fbi_full(int n, const T *__restrict__ z, const T *__restrict__ t,
+                                                  const T *__restrict__ rh, T *__restrict__ x, T *__restrict__ r,
+                                                  const double *__restrict__ partT, double *__restrict__ partR,
+                                                  double *__restrict__ sc, int it) {
+  __shared__ double lds[512];
+  if (sc[CS_DONE] != 0.0) return;
+  double ts, tt;
+  fc_sum(partT, ts, tt, lds);
+  if (tt == 0.0 || ts == 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc[CS_DONE] = 1.0;
+      if (tt == 0.0) {
+        sc[CS_RR] = 0.0;
+        sc[CS_ITER] = it + 1;
+      }
+    }
+    return;
+  }
+  double od = ts / tt;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[CS_OMEGA] = od;
+  T omega = static_cast<T>(od);
+  double a1 = 0.0, a2 = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kFcBlocks * 256) {
+    x[i] = x[i] + omega * z[i];
+    T ri = r[i] - omega * t[i];
+    r[i] = ri;
+    a1 += static_cast<double>(rh[i]) * static_cast<double>(ri);
+    a2 += static_cast<double>(ri) * static_cast<double>(ri);
+  }
+  fc_partial(a1, a2, partR, lds);
+}
+
+// out = sa a + sb b with sa, sb = ±1 (exact scalings)
+template <class T>
+__global__ void k_pm(int n, T sa, const T *__restrict__ a, T sb, const T *__restrict__ b, T *__restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = sa * a[i] + sb * b[i];
 }
 
 template <class T>
@@ -1128,64 +1086,52 @@ void ginv_apply(dec_op_s &op, const T *bvec, T *x, hipStream_t s) {
   auto P = [](DBuf<unsigned char> &v) { return reinterpret_cast<T *>(v.p); };
   T *r = P(op.cr), *p = P(op.cp), *q = P(op.cq), *z = P(op.cz), *rh = P(op.crh), *t = P(op.ct);
   const T *dinv = reinterpret_cast<const T *>(op.dinv.p);
-  const T *nul = nullptr;
-  double *part = op.cpart.p, *sc = op.cscal.p;
+  double *sc = op.cscal.p;
   const bool bicg = !op.symmetric;
-  dim3 g(dec::nblocks(n)), blk(256), gd(kCgBlocks), one(1);
-  // the start reads the stop flag of czero (always clear), not of sc
-  dim3 gf(kFcBlocks);
-  double *partA = op.cpart.p + 2 * kCgBlocks, *partB = partA + 2 * kFcBlocks;
+  dim3 g(dec::nblocks(n)), blk(256), gf(kFcBlocks);
+  // three partial-sum pairs: A and B for CG; R, V and T for BiCGSTAB
+  double *partA = op.cpart.p, *partB = partA + 2 * kFcBlocks, *partC = partB + 2 * kFcBlocks;
   T *pbuf[2] = {p, rh};  // CG: the direction, double-buffered
-  if (bicg) {
-    hipLaunchKernelGGL(k_cg_init<T>, g, blk, 0, s, n, 1, bvec, dinv, x, r, z, p, rh, q);
-    hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, rh, r, r, r, part, op.czero.p);
-    hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_BI_START, part, sc, op.rtol);
-  } else {
-    hipLaunchKernelGGL(k_fcg_init<T>, gf, blk, 0, s, n, bvec, dinv, x, r, z, pbuf[0], partB, sc);
-  }
+  if (bicg) hipLaunchKernelGGL(k_fbi_init<T>, gf, blk, 0, s, n, bvec, x, r, rh, partA, sc);
+  else hipLaunchKernelGGL(k_fcg_init<T>, gf, blk, 0, s, n, bvec, dinv, x, r, z, pbuf[0], partB, sc);
   int it = 0;
+  // BiCGSTAB buffers: p, y = z (then D⁻¹ s), v = q, t; r holds s between the halves
+  auto spmv = [&](const T *xin, T *out, const T *a, int with_qq, double *part) {
+    DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_fbi_spmv<W, T>), gf, blk, 0, s, n, E->width, E->col.p,
+                                                      E->val.p, xin, out, a, with_qq, part, sc));
+  };
+  auto test = [&](int last) {
+    if (!bicg) {
+      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_fcg_spmv<W, T>), gf, blk, 0, s, n, E->width, E->col.p,
+                                                        E->val.p, z, pbuf[it & 1], pbuf[(it + 1) & 1], q, partB,
+                                                        partA, sc, it, last, op.rtol));
+    } else {
+      hipLaunchKernelGGL(k_fbi_dir<T>, gf, blk, 0, s, n, r, q, dinv, p, z, partA, sc, it, last, op.rtol);
+    }
+  };
   auto iteration = [&]() {
     if (!bicg) {
-      T *pold = pbuf[it & 1], *pnew = pbuf[(it + 1) & 1];
-      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_fcg_spmv<W, T>), gf, blk, 0, s, n, E->width, E->col.p,
-                                                        E->val.p, z, pold, pnew, q, partB, partA, sc, it, 0,
-                                                        op.rtol));
-      hipLaunchKernelGGL(k_fcg_update<T>, gf, blk, 0, s, n, pnew, q, dinv, x, r, z, partA, partB, sc, it);
-      it++;
+      test(0);
+      hipLaunchKernelGGL(k_fcg_update<T>, gf, blk, 0, s, n, pbuf[(it + 1) & 1], q, dinv, x, r, z, partA, partB, sc,
+                         it);
     } else {
-      // q holds v = A y, z holds y and then D⁻¹ s, t holds A z
-      hipLaunchKernelGGL(k_bi_dir<T>, g, blk, 0, s, n, r, q, dinv, p, z, sc);
-      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_cg_spmv<W, T>), g, blk, 0, s, n, E->width, E->col.p,
-                                                        E->val.p, z, q, sc));
-      hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, rh, q, nul, nul, part, sc);
-      hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_BI_ALPHA, part, sc, op.rtol);
-      hipLaunchKernelGGL(k_bi_half<T>, g, blk, 0, s, n, z, q, dinv, x, r, sc);
-      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_cg_spmv<W, T>), g, blk, 0, s, n, E->width, E->col.p,
-                                                        E->val.p, z, t, sc));
-      hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, t, r, t, t, part, sc);
-      hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_BI_OMEGA, part, sc, op.rtol);
-      hipLaunchKernelGGL(k_bi_full<T>, g, blk, 0, s, n, z, t, x, r, sc);
-      hipLaunchKernelGGL(k_cg_dots<T>, gd, blk, 0, s, n, rh, r, r, r, part, sc);
-      hipLaunchKernelGGL(k_cg_scalars, one, one, 0, s, PH_BI_BETA, part, sc, op.rtol);
+      test(0);                                          // ρ, β, p, y
+      spmv(z, q, rh, 0, partB);                         // v = A y, r̂·v
+      hipLaunchKernelGGL(k_fbi_half<T>, gf, blk, 0, s, n, z, q, dinv, x, r, partB, sc, it);
+      spmv(z, t, r, 1, partC);                          // t = A z, t·s, t·t
+      hipLaunchKernelGGL(k_fbi_full<T>, gf, blk, 0, s, n, z, t, rh, x, r, partC, partA, sc, it);
     }
+    it++;
   };
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   hip_status(hipStreamIsCapturing(s, &cap), "hipStreamIsCapturing");
-  // the stopping test on the residual after the latest iteration, with no
-  // update (CG; the next iteration repeats it and goes on if it failed)
-  auto final_test = [&]() {
-    if (!bicg)
-      DEC_ELL_WIDTH_SWITCH(E->width, hipLaunchKernelGGL((k_fcg_spmv<W, T>), gf, blk, 0, s, n, E->width, E->col.p,
-                                                        E->val.p, z, pbuf[it & 1], pbuf[(it + 1) & 1], q, partB,
-                                                        partA, sc, it, 1, op.rtol));
-  };
   // checks after 2, 6, 14, 22, ... iterations: the diagonal-dominant ⋆1
   // converges in a few, so the first checks come early
   int chunk = cap == hipStreamCaptureStatusNone ? 2 : op.max_iter;
   for (int done_it = 0; done_it < op.max_iter; done_it += chunk, chunk = std::min(2 * chunk, kCgChunk)) {
     for (int k = 0; k < chunk && done_it + k < op.max_iter; k++) iteration();
     // test the latest residual now, so that the flag read below sees it
-    if (done_it + chunk >= op.max_iter || cap == hipStreamCaptureStatusNone) final_test();
+    if (done_it + chunk >= op.max_iter || cap == hipStreamCaptureStatusNone) test(1);
     if (cap != hipStreamCaptureStatusNone) break;
     double flag = 0.0;
     hip_status(hipMemcpyAsync(&flag, sc + CS_DONE, sizeof(double), hipMemcpyDeviceToHost, s), "stop flag copy");
@@ -1941,12 +1887,10 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
           }
           op->symmetric = asym <= 1e-12 * amax;
           for (auto *v : {&op->cr, &op->cp, &op->cq, &op->cz, &op->crh, &op->ct}) v->alloc(es * ne);
-          op->cpart.alloc(2 * kCgBlocks + 4 * kFcBlocks);
+          op->cpart.alloc(6 * kFcBlocks);
           op->cscal.alloc(CS_COUNT);
-          op->czero.alloc(CS_COUNT);  // a scalar block whose stop flag is always clear
           {
             std::vector<double> init(CS_COUNT, 0.0);
-            dec::h2d(op->czero.p, init.data(), sizeof(double) * CS_COUNT);
             dec::h2d(op->cscal.p, init.data(), sizeof(double) * CS_COUNT);
           }
           op->max_iter = f64 ? 200 : 100;
