@@ -55,7 +55,26 @@ __device__ __forceinline__ T row(int n, int width, int r, const int *__restrict_
     for (int k = 0; k < W; k++)
       if (c[k] >= 0) acc = acc + mul<V, T>(v[k], xv[k]);
   } else {
-    for (int k = 0; k < width; k++) {
+    // wide rows: four slots' loads in flight at a time, summed in slot order
+    int k = 0;
+    for (; k + 4 <= width; k += 4) {
+      int c[4];
+      V v[4];
+      T xv[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        size_t q = static_cast<size_t>(k + j) * n + r;
+        c[j] = col[q];
+        v[j] = val[q];
+      }
+      if (c[0] < 0) return acc;
+#pragma unroll
+      for (int j = 0; j < 4; j++) xv[j] = x[c[j] < 0 ? 0 : c[j]];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (c[j] >= 0) acc = acc + mul<V, T>(v[j], xv[j]);
+    }
+    for (; k < width; k++) {
       size_t q = static_cast<size_t>(k) * n + r;
       int c = col[q];
       if (c < 0) break;

@@ -1758,7 +1758,7 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
                                       ". Ensure that their sum is not greater than the degree of the complex. The "
                                       "currently supported dual-primal wedges are 0-1, 1-0, and 1-1.");
       // dec_wedge_product_pd(0,1) / dec_wedge_product_dp(1,0) (FastDEC.jl:327-346, 379-387)
-      if (!((dm == 0 && dn == 1) || (dm == 1 && dn == 0)))
+      if (kind == DEC_OP_WEDGE_PD && !(dm == 0 && dn == 1))
         fail(DEC_ERR_UNSUPPORTED, "Unsupported combination of degrees " + std::to_string(dm) + " and " +
                                       std::to_string(dn) +
                                       ". Ensure that their sum is not greater than the degree of the complex. The "

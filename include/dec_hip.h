@@ -52,7 +52,7 @@ enum dec_op_kind {
   DEC_OP_INV_HODGE = 7,       /* ⋆_n⁻¹ (diagonal) */
   DEC_OP_LAPLACIAN0 = 8,      /* -⋆0⁻¹ d̃1 ⋆1 d0 */
   DEC_OP_AVG01 = 9,           /* avg₀₁ */
-  DEC_OP_WEDGE_PD = 10,       /* primal-dual wedge 0-1, 1-1 (and dual-primal 1-0, as DEC_OP_WEDGE_DP) */
+  DEC_OP_WEDGE_PD = 10,       /* primal-dual wedge 0-1, 1-1 */
   DEC_OP_FLAT_SHARP = 11,     /* ♭♯_mat: dual 1-form -> primal 1-form */
   DEC_OP_WEDGE_DP = 12,       /* dual-primal wedge 1-0, 1-1 */
   DEC_OP_INTERIOR_PRODUCT_DD = 13, /* interior_product_dd (1,1), (1,2) */
@@ -135,6 +135,21 @@ int dec_mesh_destroy(dec_mesh m);
  *  DEC_OP_LAPLACIAN0 (hodge selects ⋆1): fast_laplace_beltrami
  *     (test/Multigrid.jl:46-48) = ∇²(0, sd) form (DiscreteExteriorCalculus.jl:1887-1890).
  *  DEC_OP_AVG01: avg₀₁_mat (FastDEC.jl:874-892).
+ *  DEC_OP_WEDGE_PD (0,1): dec_wedge_product_pd(0,1), apply(x=g primal 0-form,
+ *     y=f dual 1-form) = (M g) .* f, M = wedge_pd_01_mat (FastDEC.jl:327-346, 379-382);
+ *     (1,1): apply(x=f primal, y=g dual 1-form) = ∧11(f, ♭♯ g) (FastDEC.jl:390-394).
+ *  DEC_OP_WEDGE_DP (1,0): dec_wedge_product_dp(1,0), apply(x=f dual 1-form,
+ *     y=g primal 0-form) = f .* (M g) (FastDEC.jl:374-377); (1,1): apply(x=f
+ *     dual, y=g primal 1-form) = ∧11(♭♯ f, g) (FastDEC.jl:396-400).
+ *  DEC_OP_FLAT_SHARP: ♭♯_mat (DiscreteExteriorCalculus.jl:1985-1992), ne x ne;
+ *     ♯ (LLSDDSharp) is the pseudo-inverse in each triangle's plane.
+ *  DEC_OP_INTERIOR_PRODUCT_DD (1,1): interior_product_dd, apply(f, g) dual
+ *     1-forms -> dual 0-form; (1,2): apply(f dual 1-form, g dual 2-form) ->
+ *     dual 1-form (FastDEC.jl:758-786).
+ *  DEC_OP_LIE_DD (1,1): ℒ_dd, apply(f, g) dual 1-forms (FastDEC.jl:788-803).
+ *  DEC_OP_DUAL_LAPLACIAN n=0,1: Δᵈ(Val(n)) on a 2-D complex (FastDEC.jl:808-860).
+ *  The last three and the 1-1 wedges are compositions of other operators
+ *  (and of the geometric ⋆1⁻¹ solve); they own work vectors.
  * deg_n is ignored for single-degree kinds. */
 int dec_op_create(dec_mesh m, int32_t kind, int32_t deg_m, int32_t deg_n,
                   int32_t hodge, int32_t dtype, dec_op *out);

@@ -134,3 +134,8 @@ def test_composites_are_refused_by_apply_many(cs):
         cs.interior_product_dd(2, 1, gm)
     with pytest.raises(cs.DecError):
         cs.dec_wedge_product_pd(1, 0, gm)
+    from cspaces import _lib as L
+    with pytest.raises(cs.DecError, match="primal-dual wedges are 0-1, 1-0, and 1-1"):
+        cs.Operator(gm, L.DEC_OP_WEDGE_PD, 1, 0)          # the library refuses it too
+    with pytest.raises(cs.DecError, match="dual-primal wedges are 0-1, 1-0, and 1-1"):
+        cs.Operator(gm, L.DEC_OP_WEDGE_DP, 0, 1)
