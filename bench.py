@@ -499,6 +499,21 @@ def traffic(op_name, kernel, k, sets):
             "fetch_factor": f_fetch, "write_factor": f_write, "copy_MiB": COPY_MIB}
 
 
+def traffic_per_op(k, sets):
+    """HBM traffic of each operator of the step launched on its own, against
+    its algorithmic bytes; one JSON line per operator."""
+    nv, ne, nt = 10 * 4 ** k + 2, 30 * 4 ** k, 20 * 4 ** k
+    kernels = {"wedge01": "k_wedge01", "wedge11": "k_wedge11", "wedge02": "k_wedge02", "d0": "k_d0", "d1": "k_d1",
+               "hodge0": "k_diag", "hodge1": "k_diag", "hodge2": "k_diag", "laplacian0": "k_ell"}
+    for name in OPSET:
+        t = traffic(name, kernels[name], k, sets)
+        alg = alg_bytes(name, nv, ne, nt, 8, nv + 2 * ne)
+        print(json.dumps({"op": name, "traffic_MB": t["bytes"] / 1e6, "algorithmic_MB": alg / 1e6,
+                          "ratio": t["bytes"] / alg,
+                          "fetch_MB": t["fetch_KiB"] * 1024 / t["fetch_factor"] / 1e6,
+                          "write_MB": t["write_KiB"] * 1024 / t["write_factor"] / 1e6}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -519,9 +534,14 @@ def main():
     ap.add_argument("--pmc-probe", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--composite-only", action="store_true",
                     help="time only the composite operators (ms per application) and print their JSON")
+    ap.add_argument("--traffic-per-op", action="store_true",
+                    help="measure the HBM traffic of each operator of the step (two counter passes each)")
     args = ap.parse_args()
     if args.pmc_probe:
         pmc_probe(args.pmc_probe, args.k, args.sets, args.steps)
+        return
+    if args.traffic_per_op:
+        traffic_per_op(args.k, args.sets)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
