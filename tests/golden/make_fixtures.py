@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
+import musical as MU  # noqa: E402
 import oracle as O  # noqa: E402
 
 
@@ -43,7 +44,22 @@ def outputs(mesh, circumcenter=True):
         "geo_hodge1": d.geo_hodge1().matvec(E),
         "laplacian0": d.laplacian0().matvec(V), "laplacian0_geometric": d.laplacian0(geometric=True).matvec(V),
     }
+    # the musical isomorphisms and their composites (numpy oracle; ♯ in the
+    # triangle's plane, as the library takes it on curved surfaces)
+    ops = MU.Ops(d, project=True)
+    out.update({
+        "flat_sharp": ops.FS @ DE, "wedge_pd11": ops.wedge_pd11(E, DE), "wedge_dp11": ops.wedge_dp11(DE, E),
+        "inv_hodge1_geometric": ops.inv_hodge1(E), "interior11": ops.interior11(DE, E2),
+        "interior12": ops.interior12(DE, V), "lie": ops.lie(DE, E2),
+        "dual_laplacian0": ops.dual_laplacian0(DT), "dual_laplacian1": ops.dual_laplacian1(DE),
+    })
     return {k: np.ascontiguousarray(v) for k, v in out.items()}
+
+
+# outputs computed through LAPACK / SuperLU (pinv, splu): compared to 1e-13
+# relative rather than bit for bit when the oracle is re-run
+LAPACK_KEYS = {"flat_sharp", "wedge_pd11", "wedge_dp11", "inv_hodge1_geometric", "interior11", "interior12", "lie",
+               "dual_laplacian0", "dual_laplacian1"}
 
 
 def main():
