@@ -22,6 +22,7 @@
 
 #include "dec_internal.h"
 #include "kernels_common.h"
+#include "ell_rows.h"
 #include "learned.h"
 
 using dec::DBuf;
@@ -38,10 +39,17 @@ constexpr int TB = 256;
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
+// float -> BF16, round to nearest even, by gfx950's v_cvt_pk_bf16_f32 (one
+// instruction for two values).  For every finite input and ±inf it gives the
+// bits of the integer rounding u + 0x7FFF + ((u >> 16) & 1), u = bits of f.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned int f2bf2(float a, float b) {
+  f32x2_t f = {a, b};
+  return __builtin_bit_cast(unsigned int, __builtin_convertvector(f, bf16x2_t));
+}
 __device__ __forceinline__ unsigned short f2bf(float f) {
-  unsigned int u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-  return static_cast<unsigned short>(u >> 16);
+  return __builtin_bit_cast(unsigned short, static_cast<__bf16>(f));
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(static_cast<unsigned int>(h) << 16); }
 
@@ -334,14 +342,16 @@ __device__ __forceinline__ void mfma_rows(const unsigned short *rows, int li, in
 
 __device__ __forceinline__ uint4 pack8(const float *f) {
   uint4 w;
-  w.x = static_cast<unsigned>(f2bf(f[0])) | (static_cast<unsigned>(f2bf(f[1])) << 16);
-  w.y = static_cast<unsigned>(f2bf(f[2])) | (static_cast<unsigned>(f2bf(f[3])) << 16);
-  w.z = static_cast<unsigned>(f2bf(f[4])) | (static_cast<unsigned>(f2bf(f[5])) << 16);
-  w.w = static_cast<unsigned>(f2bf(f[6])) | (static_cast<unsigned>(f2bf(f[7])) << 16);
+  w.x = f2bf2(f[0], f[1]);
+  w.y = f2bf2(f[2], f[3]);
+  w.z = f2bf2(f[4], f[5]);
+  w.w = f2bf2(f[6], f[7]);
   return w;
 }
 
-// xp[v] = (x, P x) with x = r / s
+// xp[v] = (x, P x) with x = r / s.  W > 0 is the width of P's rows: all
+// slots' loads are issued before the sums, which run in slot order.
+template <int W>
 __global__ void k_inf_input(int n, int pw, const int *__restrict__ pcol, const float *__restrict__ pval,
                             const float *__restrict__ r, const float *__restrict__ s, float2 *__restrict__ xp) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -349,17 +359,87 @@ __global__ void k_inf_input(int n, int pw, const int *__restrict__ pcol, const f
   float inv = 1.0f / s[0];
   float xv = r[v] * inv;
   float acc = 0.0f;
-  for (int k = 0; k < pw; k++) {
-    size_t q = static_cast<size_t>(k) * n + v;
-    int c = pcol[q];
-    if (c < 0) break;
-    acc = acc + pval[q] * (r[c] * inv);
+  if constexpr (W > 0) {
+    int c[W];
+    float pv[W], rc[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      size_t q = static_cast<size_t>(k) * n + v;
+      c[k] = pcol[q];
+      pv[k] = pval[q];
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) rc[k] = r[c[k] < 0 ? v : c[k]];
+#pragma unroll
+    for (int k = 0; k < W; k++)
+      if (c[k] >= 0) acc = acc + pv[k] * (rc[k] * inv);
+  } else {
+    for (int k = 0; k < pw; k++) {
+      size_t q = static_cast<size_t>(k) * n + v;
+      int c = pcol[q];
+      if (c < 0) break;
+      acc = acc + pval[q] * (r[c] * inv);
+    }
   }
   xp[v] = make_float2(xv, acc);
 }
 
+// m += Σ_k P[v, k] bf16(relu(h1 of neighbour k)), ascending slot order, with
+// the neighbours' (x, P x) loaded before the sums when W > 0
+template <int W>
+__device__ __forceinline__ void h1_messages(int n, int pw, int v, const int *__restrict__ pcol,
+                                           const float *__restrict__ pval, const float2 *__restrict__ xp,
+                                           const float *__restrict__ w1, const float *__restrict__ b1, float *m) {
+  if constexpr (W > 0) {
+    int c[W];
+    float pv[W];
+    float2 fc[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      size_t qq = static_cast<size_t>(k) * n + v;
+      c[k] = pcol[qq];
+      pv[k] = pval[qq];
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) fc[k] = xp[c[k] < 0 ? v : c[k]];
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      if (c[k] < 0) continue;
+#pragma unroll
+      for (int j = 0; j < H; j++) {
+        float z = fc[k].x * w1[j] + fc[k].y * w1[H + j] + b1[j];
+        m[j] = m[j] + pv[k] * bf2f(f2bf(z > 0.0f ? z : 0.0f));
+      }
+    }
+  } else {
+    for (int k = 0; k < pw; k++) {
+      size_t qq = static_cast<size_t>(k) * n + v;
+      int c = pcol[qq];
+      if (c < 0) break;
+      float pv = pval[qq];
+      float2 fc = xp[c];
+#pragma unroll
+      for (int j = 0; j < H; j++) {
+        float z = fc.x * w1[j] + fc.y * w1[H + j] + b1[j];
+        m[j] = m[j] + pv * bf2f(f2bf(z > 0.0f ? z : 0.0f));
+      }
+    }
+  }
+}
+
 // One row per lane: the row's own h (BF16) and m = P h (FP32 sums over the
 // neighbours' rows, ascending ELL order, rounded to BF16) into its LDS row.
+__device__ __forceinline__ void add_row(const uint4 *hc, float pv, float *m) {
+#pragma unroll
+  for (int k4 = 0; k4 < 4; k4++) {
+    float f[8];
+    unpack8(hc[k4], f);
+#pragma unroll
+    for (int e = 0; e < 8; e++) m[8 * k4 + e] = m[8 * k4 + e] + pv * f[e];
+  }
+}
+
+template <int W>
 __device__ __forceinline__ void gather_rows(int n, int pw, int v, const int *__restrict__ pcol,
                                             const float *__restrict__ pval, const unsigned short *__restrict__ h,
                                             unsigned short *my) {
@@ -370,6 +450,34 @@ __device__ __forceinline__ void gather_rows(int n, int pw, int v, const int *__r
   float m[H];
 #pragma unroll
   for (int j = 0; j < H; j++) m[j] = 0.0f;
+  if constexpr (W > 0) {
+    // all slots' columns and weights, then the neighbours' rows two at a time
+    int c[W];
+    float pv[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      size_t qq = static_cast<size_t>(k) * n + v;
+      c[k] = pcol[qq];
+      pv[k] = pval[qq];
+    }
+#pragma unroll
+    for (int k = 0; k < W; k += 2) {
+      uint4 a[4], b[4];
+      const uint4 *ha = reinterpret_cast<const uint4 *>(h + static_cast<size_t>(c[k] < 0 ? v : c[k]) * H);
+#pragma unroll
+      for (int k4 = 0; k4 < 4; k4++) a[k4] = ha[k4];
+      if (k + 1 < W) {
+        const uint4 *hb = reinterpret_cast<const uint4 *>(h + static_cast<size_t>(c[k + 1] < 0 ? v : c[k + 1]) * H);
+#pragma unroll
+        for (int k4 = 0; k4 < 4; k4++) b[k4] = hb[k4];
+      }
+      if (c[k] >= 0) add_row(a, pv[k], m);
+      if (k + 1 < W && c[k + 1] >= 0) add_row(b, pv[k + 1], m);
+    }
+#pragma unroll
+    for (int k4 = 0; k4 < 4; k4++) dst[4 + k4] = pack8(m + 8 * k4);
+    return;
+  }
   for (int k = 0; k < pw; k++) {
     size_t qq = static_cast<size_t>(k) * n + v;
     int c = pcol[qq];
@@ -394,6 +502,7 @@ __device__ __forceinline__ void zero_row(unsigned short *my) {
   for (int k = 0; k < 8; k++) dst[k] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+template <int W>
 __global__ void __launch_bounds__(TB) k_inf_layer2(int n, int pw, const int *__restrict__ pcol,
                                                    const float *__restrict__ pval, const float2 *__restrict__ xp,
                                                    const float *__restrict__ th, unsigned short *__restrict__ h2) {
@@ -420,18 +529,7 @@ __global__ void __launch_bounds__(TB) k_inf_layer2(int n, int pw, const int *__r
         hh[j] = z > 0.0f ? z : 0.0f;
         m[j] = 0.0f;
       }
-      for (int k = 0; k < pw; k++) {
-        size_t qq = static_cast<size_t>(k) * n + v;
-        int c = pcol[qq];
-        if (c < 0) break;
-        float pv = pval[qq];
-        float2 fc = xp[c];
-#pragma unroll
-        for (int j = 0; j < H; j++) {
-          float z = fc.x * w1[j] + fc.y * w1[H + j] + b1[j];
-          m[j] = m[j] + pv * bf2f(f2bf(z > 0.0f ? z : 0.0f));
-        }
-      }
+      h1_messages<W>(n, pw, v, pcol, pval, xp, w1, b1, m);
       uint4 *dst = reinterpret_cast<uint4 *>(my);
 #pragma unroll
       for (int k4 = 0; k4 < 4; k4++) dst[k4] = pack8(hh + 8 * k4);
@@ -467,6 +565,7 @@ __global__ void __launch_bounds__(TB) k_inf_layer2(int n, int pw, const int *__r
 
 // h3 = relu([h2, P h2] W3 + b3); y = alpha x + h3 wo + bo (ascending sum over
 // the 32 features); u += (s / D) y.
+template <int W>
 __global__ void __launch_bounds__(TB) k_inf_layer3(int n, int pw, const int *__restrict__ pcol,
                                                    const float *__restrict__ pval, const unsigned short *__restrict__ h2,
                                                    const float2 *__restrict__ xo, const float *__restrict__ th,
@@ -493,7 +592,7 @@ __global__ void __launch_bounds__(TB) k_inf_layer3(int n, int pw, const int *__r
       xv = xo[v].x;
       dv = D[v];
       uv = u[v];
-      gather_rows(n, pw, v, pcol, pval, h2, my);
+      gather_rows<W>(n, pw, v, pcol, pval, h2, my);
     } else {
       zero_row(my);
     }
@@ -1341,8 +1440,8 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s) {
   int n = L.n;
   reserve_inference(nn, n);
   hipLaunchKernelGGL(k_scale_one, dim3(kScaleBlocks), dim3(TB), 0, s, n, r, nn.spart.p, nn.ticket.p, nn.s.p);
-  hipLaunchKernelGGL(k_inf_input, dim3(dec::nblocks(n, TB)), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, r,
-                     nn.s.p, nn.ix.p);
+  DEC_ELL_WIDTH_SWITCH(L.p.width, hipLaunchKernelGGL((k_inf_input<W>), dim3(dec::nblocks(n, TB)), dim3(TB), 0, s, n,
+                                                      L.p.width, L.p.col.p, L.p.val.p, r, nn.s.p, nn.ix.p));
   if (n <= kInfSmallRows) {
     int tiles = (n + 15) / 16;
     int blocks = std::max(1, (tiles + kInfWaves - 1) / kInfWaves);
@@ -1353,10 +1452,12 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s) {
   } else {
     int steps = (n + 63) / 64;
     int blocks = std::max(1, (steps + kInfWaves - 1) / kInfWaves);
-    hipLaunchKernelGGL(k_inf_layer2, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ix.p,
-                       nn.th.p, nn.ih2.p);
-    hipLaunchKernelGGL(k_inf_layer3, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ih2.p,
-                       nn.ix.p, nn.th.p, nn.s.p, L.D.p, u);
+    DEC_ELL_WIDTH_SWITCH(L.p.width, {
+      hipLaunchKernelGGL((k_inf_layer2<W>), dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p,
+                         nn.ix.p, nn.th.p, nn.ih2.p);
+      hipLaunchKernelGGL((k_inf_layer3<W>), dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p,
+                         nn.ih2.p, nn.ix.p, nn.th.p, nn.s.p, L.D.p, u);
+    });
   }
 }
 
