@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdec_hip.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-HIP_SOURCES = ["runtime.hip", "mesh.hip", "ops.hip", "mg.hip", "batch.hip", "learned.hip"]
+HIP_SOURCES = ["runtime.hip", "mesh.hip", "tet.hip", "ops.hip", "mg.hip", "batch.hip", "learned.hip"]
 CXX_SOURCES = ["host_topology.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

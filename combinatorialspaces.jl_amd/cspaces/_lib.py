@@ -43,6 +43,12 @@ SIGNATURES = {
     "dec_mesh_dualize": (c_int, [P, c_int32]),
     "dec_mesh_export_dual": (c_int, [P, P, P, P, P, P]),
     "dec_mesh_destroy": (c_int, [P]),
+    "dec_mesh3_create": (c_int, [c_int32] * 4 + [P] * 13 + [c_int32, POINTER(P)]),
+    "dec_mesh3_from_tets": (c_int, [c_int32, c_int32, P, P, c_int32, c_int32, POINTER(P)]),
+    "dec_mesh_dim": (c_int, [P, PI32]),
+    "dec_mesh3_counts": (c_int, [P, PI32, PI32, PI32, PI32]),
+    "dec_mesh3_export": (c_int, [P, P, P, P, P, P]),
+    "dec_mesh3_export_dual": (c_int, [P, P, P, P, P, P, P, P]),
     "dec_op_create": (c_int, [P, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(P)]),
     "dec_op_shape": (c_int, [P, PI64, PI64, PI64]),
     "dec_op_apply": (c_int, [P, P, P, P, P]),

@@ -75,15 +75,23 @@ def _hodge(h):
 # Meshes
 # ---------------------------------------------------------------------------
 class Mesh:
-    """A primal 2-D complex on the GPU (EmbeddedDeltaSet2D) that becomes its
-    own dual complex (EmbeddedDeltaDualComplex2D) once `dualize` has run."""
+    """A primal complex on the GPU -- EmbeddedDeltaSet2D (dim 2) or
+    EmbeddedDeltaSet3D (dim 3) -- that becomes its own dual complex
+    (EmbeddedDeltaDualComplex2D / 3D) once `dualize` has run."""
 
     def __init__(self, handle, owned=True):
         self._h = c_void_p(handle) if not isinstance(handle, c_void_p) else handle
         self._owned = owned
-        nv, ne, nt = c_int32(), c_int32(), c_int32()
-        check(lib().dec_mesh_counts(self._h, ctypes.byref(nv), ctypes.byref(ne), ctypes.byref(nt)))
+        nv, ne, nt, ntet, dim = c_int32(), c_int32(), c_int32(), c_int32(), c_int32()
+        check(lib().dec_mesh_dim(self._h, ctypes.byref(dim)))
+        self.dim = dim.value
+        if self.dim == 3:
+            check(lib().dec_mesh3_counts(self._h, ctypes.byref(nv), ctypes.byref(ne), ctypes.byref(nt),
+                                         ctypes.byref(ntet)))
+        else:
+            check(lib().dec_mesh_counts(self._h, ctypes.byref(nv), ctypes.byref(ne), ctypes.byref(nt)))
         self.nv, self.ne, self.nt = nv.value, ne.value, nt.value
+        self.ntet = ntet.value
 
     @property
     def handle(self):
@@ -110,10 +118,25 @@ class Mesh:
         check(lib().dec_mesh_export(self._h, ptr(out["v0"]), ptr(out["v1"]), ptr(out["e0"]),
                                     ptr(out["e1"]), ptr(out["e2"]), ptr(out["edge_orientation"]),
                                     ptr(out["tri_orientation"]), ptr(out["points"])))
+        if self.dim == 3:
+            for k in range(4):
+                out[f"t{k}"] = np.zeros(self.ntet, np.int32)
+            out["tet_orientation"] = np.zeros(self.ntet, np.int8)
+            check(lib().dec_mesh3_export(self._h, ptr(out["t0"]), ptr(out["t1"]), ptr(out["t2"]),
+                                         ptr(out["t3"]), ptr(out["tet_orientation"])))
         return out
 
     def export_dual(self):
         nv, ne, nt = self.nv, self.ne, self.nt
+        if self.dim == 3:
+            n = self.ntet
+            out = {"dual_point": np.zeros((nv + ne + nt + n, 3)), "length": np.zeros(ne),
+                   "area": np.zeros(nt), "vol": np.zeros(n), "dual_length": np.zeros(2 * ne + 6 * nt),
+                   "dual_area": np.zeros(6 * nt), "dual_vol": np.zeros(24 * n)}
+            check(lib().dec_mesh3_export_dual(self._h, ptr(out["dual_point"]), ptr(out["length"]),
+                                              ptr(out["area"]), ptr(out["vol"]), ptr(out["dual_length"]),
+                                              ptr(out["dual_area"]), ptr(out["dual_vol"])))
+            return out
         out = {"dual_point": np.zeros((nv + ne + nt, 3)), "length": np.zeros(ne),
                "area": np.zeros(nt), "dual_length": np.zeros(2 * ne + 6 * nt),
                "dual_area": np.zeros(6 * nt)}
@@ -147,6 +170,71 @@ def mesh_from_triangles(points, triangles, sorted_=True, index_base=0) -> Mesh:
     check(lib().dec_mesh_from_triangles(len(p), len(t), ptr(p), ptr(t), int(bool(sorted_)), index_base,
                                         ctypes.byref(h)))
     return Mesh(h)
+
+
+def mesh3_from_arrays(points, v0, v1, e0, e1, e2, t0, t1, t2, t3, edge_orientation=None,
+                      tri_orientation=None, tet_orientation=None, index_base=0) -> Mesh:
+    """EmbeddedDeltaSet3D from boundary maps (SimplicialSets.jl:418-606)."""
+    a = [np.ascontiguousarray(x, np.int32) for x in (v0, v1, e0, e1, e2, t0, t1, t2, t3)]
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    o = [None if x is None else np.ascontiguousarray(x, np.int8)
+         for x in (edge_orientation, tri_orientation, tet_orientation)]
+    h = c_void_p()
+    check(lib().dec_mesh3_create(len(p), len(a[0]), len(a[2]), len(a[5]), *[ptr(x) for x in a],
+                                 *[ptr(x) for x in o], ptr(p), index_base, ctypes.byref(h)))
+    return Mesh(h)
+
+
+def mesh_from_tetrahedra(points, tets, sorted_=True, index_base=0) -> Mesh:
+    """EmbeddedDeltaSet3D(mesh) from a tetrahedron list: glue_sorted_tetrahedron!
+    (or glue_tetrahedron! with sorted_=False) per tetrahedron, then orient!
+    (src/MeshInterop.jl:141-153)."""
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+    h = c_void_p()
+    check(lib().dec_mesh3_from_tets(len(p), len(t), ptr(p), ptr(t), int(bool(sorted_)), index_base,
+                                    ctypes.byref(h)))
+    return Mesh(h)
+
+
+def kuhn_box_arrays(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, jitter=0.0, seed=0, shear=(0.0, 0.0),
+                    jitter_boundary=False):
+    """Points and tetrahedra of an nx x ny x nz box of cubes, each cut into
+    the six Kuhn tetrahedra 000 → +e_a → +e_a+e_b → 111 over the permutations
+    (a, b, c) of the axes in lexicographic order; vertex (i, j, k) has id
+    i + (nx+1)(j + (ny+1)k), cubes run with i fastest.  Interior points move by
+    up to `jitter` cell widths (numpy default_rng(seed)), boundary points too
+    with `jitter_boundary`; `shear` = (dx, dy) slides the top face, as
+    CombMeshes.parallelepiped does.  A structured
+    stand-in for the reference's TetGen meshes, which need TetGen.jl."""
+    import itertools
+
+    h = np.array([lx / nx, ly / ny, lz / nz])
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    ijk = np.stack([i.ravel(order="F"), j.ravel(order="F"), k.ravel(order="F")], 1)
+    p = ijk * h
+    if jitter:
+        rng = np.random.default_rng(seed)
+        inner = np.all((ijk > 0) & (ijk < [nx, ny, nz]), axis=1) | jitter_boundary
+        p[inner] += (rng.random((int(inner.sum()), 3)) * 2 - 1) * jitter * h
+    p[:, 0] += shear[0] * p[:, 2] / lz
+    p[:, 1] += shear[1] * p[:, 2] / lz
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    o = np.stack([ci.ravel(order="F"), cj.ravel(order="F"), ck.ravel(order="F")], 1)   # cubes, i fastest
+    vid = lambda q: q[:, 0] + (nx + 1) * (q[:, 1] + (ny + 1) * q[:, 2])  # noqa: E731
+    unit = np.eye(3, dtype=np.int64)
+    tets = np.empty((len(o), 6, 4), np.int64)
+    for s, (a, b, c) in enumerate(itertools.permutations(range(3))):
+        q1 = o + unit[a]
+        q2 = q1 + unit[b]
+        tets[:, s] = np.stack([vid(o), vid(q1), vid(q2), vid(q2 + unit[c])], 1)
+    return p, tets.reshape(-1, 4)
+
+
+def kuhn_box(nx, ny, nz, **kw) -> Mesh:
+    """The Kuhn box of kuhn_box_arrays glued as EmbeddedDeltaSet3D(mesh)."""
+    p, t = kuhn_box_arrays(nx, ny, nz, **kw)
+    return mesh_from_tetrahedra(p, t)
 
 
 def icosphere(k: int) -> Mesh:
@@ -414,7 +502,8 @@ def avg01_mat(sd: Mesh, backend=HIP, cast_float=None) -> Operator:
 
 
 def fast_laplace_beltrami(sd: Mesh, hodge=GeometricHodge, backend=HIP, cast_float=None) -> Operator:
-    """-1 * ⋆0⁻¹ * d̃1 * ⋆1 * d0 (test/Multigrid.jl:46-48), i.e. ∇²(0, sd)."""
+    """-1 * ⋆0⁻¹ * d̃1 * ⋆1 * d0 (test/Multigrid.jl:46-48), i.e. ∇²(0, sd); on a
+    3-D complex -⋆0⁻¹ d̃2 ⋆1 d0, which needs hodge=DiagonalHodge."""
     _check_backend(backend)
     return Operator(sd, L.DEC_OP_LAPLACIAN0, hodge=hodge, cast_float=cast_float)
 

@@ -258,6 +258,7 @@ dec_batch_s *create(int ninst, const dec_mesh *meshes, int dtype) {
   for (int i = 0; i < ninst; i++) {
     dec_mesh m = meshes[i];
     if (!m) fail(DEC_ERR_ARG, "dec_batch_create: NULL mesh");
+    if (m->dim != 2) fail(DEC_ERR_UNSUPPORTED, "dec_batch_create: heat steps run on 2-D complexes");
     auto it = seen.find(m);
     int oi;
     double dt;

@@ -29,8 +29,23 @@ struct Topo {
   std::vector<int8_t> eo, to;       // orientation: 1 true, 0 false, -1 unset
 };
 
+// Tetrahedra of a 3-D complex: ∂t0..∂t3 and the orientation (1, 0, -1 unset).
+struct Tets {
+  int n = 0;
+  std::vector<int32_t> t[4];
+  std::vector<int8_t> o;
+};
+
 // glue_triangle! / glue_sorted_triangle! over a triangle list (SimplicialSets.jl:323-364).
 void glue_triangles(int nv, const std::vector<int32_t> &tv, bool sorted, Topo &t);
+// glue_tetrahedron! / glue_sorted_tetrahedron! over a tetrahedron list
+// (SimplicialSets.jl:511-534): the triangle skeleton goes to t.
+void glue_tetrahedra(int nv, const std::vector<int32_t> &tv, bool sorted, Topo &t, Tets &T);
+// orient!(s, Val(3)) (SimplicialSets.jl:812-852); false if not orientable.
+bool orient3_dfs(const Topo &t, const Tets &T, std::vector<int8_t> &to);
+// tetrahedron_vertices / tetrahedron_edges of tetrahedron x (SimplicialSets.jl:474-492).
+void tet_vertices(const Topo &t, const Tets &T, int x, int v[4]);
+void tet_edges(const Topo &t, const Tets &T, int x, int e[6]);
 // orient!(s, Val(2)) (SimplicialSets.jl:812-852); false if not orientable.
 bool orient_dfs(const Topo &t, std::vector<int8_t> &to);
 // refine(::BinarySubdivision, topo) (Multigrid.jl:130-183).  Triangle
@@ -139,7 +154,9 @@ struct Ell {
 // Handles
 // ---------------------------------------------------------------------------
 struct dec_mesh_s {
+  int dim = 2;                  // 2: triangle complex; 3: tetrahedral complex
   dec::Topo h;                  // host topology (orientations resolved at dualize)
+  dec::Tets ht;                 // host tetrahedra (dim 3)
   std::vector<double> hp;       // host points, nv x 3
   dec::DBuf<int32_t> v0, v1, e0, e1, e2;
   dec::DBuf<int8_t> eo, to;
@@ -152,6 +169,11 @@ struct dec_mesh_s {
   dec::DBuf<double> da;             // 6nt
   dec::DBuf<double> dpt;            // (nv+ne+nt) x 3
   dec::DBuf<double> h0, h1, h2;     // diagonal Hodge stars
+  // 3-D: tetrahedra, volumes, ⋆3, and per tetrahedron the dual volume at each
+  // of its vertices (slot-major 4 x ntet, tetrahedron_vertices order)
+  dec::DBuf<int32_t> t0, t1, t2, t3;
+  dec::DBuf<int8_t> teo;
+  dec::DBuf<double> vol, h3, tw;
   bool owned_by_mg = false;
   bool to_dfs = false;          // h.to is the orientation orient! assigns
 };

@@ -108,6 +108,136 @@ bool orient_dfs(const Topo &t, std::vector<int8_t> &to) {
   return true;
 }
 
+// glue_tetrahedron!(s, v0, v1, v2, v3) = add_tetrahedron!(get_triangle!(v1,v2,v3),
+// get_triangle!(v0,v2,v3), get_triangle!(v0,v1,v3), get_triangle!(v0,v1,v2))
+// (SimplicialSets.jl:511-527).  get_triangle! finds the triangle whose edges
+// are v1→v2, v0→v2, v0→v1 as ∂e0, ∂e1, ∂e2; since every triangle is glued
+// from its vertex triple, that is the triangle glued from (v0, v1, v2), kept
+// here in a list per first vertex.  Otherwise glue_triangle! adds it.
+void glue_tetrahedra(int nv, const std::vector<int32_t> &tv, bool sorted, Topo &t, Tets &T) {
+  int n = static_cast<int>(tv.size() / 4);
+  t = Topo();
+  t.nv = nv;
+  T = Tets();
+  T.n = n;
+  for (auto &a : T.t) a.assign(n, 0);
+  std::vector<int32_t> ehead(nv, -1), etail(nv, -1), enext;
+  std::vector<int32_t> thead(nv, -1), ttail(nv, -1), tnext, tb, tc;
+  auto get_edge = [&](int src, int tgt) {
+    for (int e = ehead[src]; e >= 0; e = enext[e])
+      if (t.v0[e] == tgt) return e;
+    int e = static_cast<int>(t.v0.size());
+    t.v0.push_back(tgt);
+    t.v1.push_back(src);
+    enext.push_back(-1);
+    if (etail[src] < 0) ehead[src] = e;
+    else enext[etail[src]] = e;
+    etail[src] = e;
+    return e;
+  };
+  auto get_tri = [&](int a, int b, int c) {
+    for (int x = thead[a]; x >= 0; x = tnext[x])
+      if (tb[x] == b && tc[x] == c) return x;
+    int eab = get_edge(a, b), ebc = get_edge(b, c), eac = get_edge(a, c);
+    int x = static_cast<int>(t.e0.size());
+    t.e0.push_back(ebc);
+    t.e1.push_back(eac);
+    t.e2.push_back(eab);
+    tb.push_back(b);
+    tc.push_back(c);
+    tnext.push_back(-1);
+    if (ttail[a] < 0) thead[a] = x;
+    else tnext[ttail[a]] = x;
+    ttail[a] = x;
+    return x;
+  };
+  for (int i = 0; i < n; i++) {
+    int v[4] = {tv[4 * i], tv[4 * i + 1], tv[4 * i + 2], tv[4 * i + 3]};
+    for (int k = 0; k < 4; k++)
+      if (v[k] < 0 || v[k] >= nv) fail(DEC_ERR_ARG, "tetrahedron vertex out of range");
+    if (sorted) std::sort(v, v + 4);
+    for (int a = 0; a < 4; a++)
+      for (int b = a + 1; b < 4; b++)
+        if (v[a] == v[b]) fail(DEC_ERR_ARG, "tetrahedron with a repeated vertex");
+    T.t[0][i] = get_tri(v[1], v[2], v[3]);
+    T.t[1][i] = get_tri(v[0], v[2], v[3]);
+    T.t[2][i] = get_tri(v[0], v[1], v[3]);
+    T.t[3][i] = get_tri(v[0], v[1], v[2]);
+  }
+  t.ne = static_cast<int>(t.v0.size());
+  t.nt = static_cast<int>(t.e0.size());
+  t.eo.assign(t.ne, -1);
+  t.to.assign(t.nt, -1);
+  T.o.assign(n, -1);
+}
+
+// orient!(s, Val(3)): the depth-first pass of orient_dfs over tetrahedra and
+// their four faces.
+bool orient3_dfs(const Topo &t, const Tets &T, std::vector<int8_t> &to) {
+  int nt = t.nt, n = T.n;
+  to.assign(n, 1);
+  if (n == 0) return true;
+  std::vector<int32_t> ptr[4], lst[4];
+  for (int j = 0; j < 4; j++) {
+    ptr[j].assign(nt + 1, 0);
+    lst[j].assign(n, 0);
+    for (int i = 0; i < n; i++) ptr[j][T.t[j][i] + 1]++;
+    for (int f = 0; f < nt; f++) ptr[j][f + 1] += ptr[j][f];
+    std::vector<int32_t> w(ptr[j].begin(), ptr[j].end() - 1);
+    for (int i = 0; i < n; i++) lst[j][w[T.t[j][i]]++] = i;
+  }
+  std::vector<int8_t> ors(n, 0);
+  std::vector<int32_t> stack;
+  stack.reserve(n);
+  for (int seed = 0; seed < n; seed++) {
+    if (ors[seed]) continue;
+    stack.clear();
+    stack.push_back(seed);
+    ors[seed] = 1;
+    while (!stack.empty()) {
+      int x = stack.back();
+      stack.pop_back();
+      int ox = ors[x];
+      for (int i = 0; i < 4; i++) {
+        int face = T.t[i][x];
+        for (int j = 0; j < 4; j++) {
+          bool same = ((i + j) % 2) == 0;
+          for (int k = ptr[j][face]; k < ptr[j][face + 1]; k++) {
+            int y = lst[j][k];
+            if (y == x) continue;
+            if (ors[y] == 0) {
+              ors[y] = static_cast<int8_t>(same ? -ox : ox);
+              stack.push_back(y);
+            } else if (same && ors[y] == ox) {
+              return false;
+            }
+          }
+        }
+      }
+    }
+  }
+  for (int i = 0; i < n; i++) to[i] = ors[i] == 1 ? 1 : 0;
+  return true;
+}
+
+void tet_vertices(const Topo &t, const Tets &T, int x, int v[4]) {
+  int a = T.t[2][x], b = T.t[0][x];
+  v[0] = t.v1[t.e2[a]];
+  v[1] = t.v0[t.e2[a]];
+  v[2] = t.v1[t.e0[b]];
+  v[3] = t.v0[t.e0[b]];
+}
+
+void tet_edges(const Topo &t, const Tets &T, int x, int e[6]) {
+  int a = T.t[0][x], b = T.t[1][x], c = T.t[2][x];
+  e[0] = t.e0[a];
+  e[1] = t.e1[a];
+  e[2] = t.e2[a];
+  e[3] = t.e1[b];
+  e[4] = t.e2[b];
+  e[5] = t.e2[c];
+}
+
 // refine(::BinarySubdivision, topo), 0-based: coarse edge e -> fine edges 2e
 // (∂v1 = ∂v0_c[e]) and 2e+1 (∂v1 = ∂v1_c[e]), both with ∂v0 = nv + e; interior
 // edges 2ne + 3t + {0,1,2}; children 4t + {0,1,2,3}.

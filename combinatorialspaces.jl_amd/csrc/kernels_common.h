@@ -22,6 +22,10 @@ inline int xcd_grid(long long n, int b = 256) { return (nblocks(n, b) + 7) / 8 *
 void dualize(dec_mesh_s &m, int center);
 void ensure_dual(dec_mesh_s &m);
 dec_mesh_s *subdivide(const dec_mesh_s &c, bool project);
+// tet.hip: orientations of a 3-D complex before dualizing, and the tetrahedron
+// part of its dual geometry with the 3-D diagonal Hodge stars
+void resolve_orientation3(dec_mesh_s &m);
+void dualize_tets(dec_mesh_s &m, int center);
 
 // Host CSR with float64 values, columns ascending within each row.
 struct HostCsr {

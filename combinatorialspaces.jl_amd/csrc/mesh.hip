@@ -183,22 +183,28 @@ namespace dec {
 void dualize(dec_mesh_s &m, int center) {
   if (center != DEC_CIRCUMCENTER && center != DEC_BARYCENTER) fail(DEC_ERR_ARG, "dualize: unknown center");
   Topo &t = m.h;
-  for (auto &o : t.eo)
-    if (o < 0) o = 0;
-  bool unset = std::any_of(t.to.begin(), t.to.end(), [](int8_t o) { return o < 0; });
-  if (unset) {
-    std::vector<int8_t> to;
-    if (!orient_dfs(t, to)) fail(DEC_ERR_NONORIENTABLE, "The 2-simplices of the given 2D delta set are non-orientable.");
-    t.to = to;
+  if (m.dim == 3) {
+    resolve_orientation3(m);
+  } else {
+    for (auto &o : t.eo)
+      if (o < 0) o = 0;
+    bool unset = std::any_of(t.to.begin(), t.to.end(), [](int8_t o) { return o < 0; });
+    if (unset) {
+      std::vector<int8_t> to;
+      if (!orient_dfs(t, to))
+        fail(DEC_ERR_NONORIENTABLE, "The 2-simplices of the given 2D delta set are non-orientable.");
+      t.to = to;
+    }
   }
   m.eo.upload(t.eo);
   m.to.upload(t.to);
   int nv = t.nv, ne = t.ne, nt = t.nt;
+  size_t ndv = static_cast<size_t>(nv) + ne + nt + (m.dim == 3 ? m.ht.n : 0);
   m.length.alloc(ne);
   m.area.alloc(nt);
   m.dl.alloc(2 * static_cast<size_t>(ne) + 6 * static_cast<size_t>(nt));
   m.da.alloc(6 * static_cast<size_t>(nt));
-  m.dpt.alloc(3 * (static_cast<size_t>(nv) + ne + nt));
+  m.dpt.alloc(3 * ndv);
   h2d(m.dpt.p, m.hp.data(), sizeof(double) * 3 * nv);
   if (ne > 0)
     hipLaunchKernelGGL(k_edge_geometry, dim3(dec::nblocks(ne)), dim3(256), 0, 0, nv, ne, m.v0.p, m.v1.p, m.p.p,
@@ -207,6 +213,15 @@ void dualize(dec_mesh_s &m, int center) {
     hipLaunchKernelGGL(k_tri_geometry, dim3(dec::nblocks(nt)), dim3(256), 0, 0, nv, ne, nt, m.v0.p, m.v1.p, m.e0.p,
                        m.e1.p, m.e2.p, m.p.p, center == DEC_CIRCUMCENTER ? 1 : 0, m.area.p, m.dl.p, m.da.p,
                        m.dpt.p);
+  dec::check_launch("dualize");
+  if (m.dim == 3) {
+    // the 1-D and 2-D steps above are shared with a 3-D complex
+    // (DiscreteExteriorCalculus.jl:1308-1327); the tetrahedra follow
+    dualize_tets(m, center);
+    m.dualized = true;
+    m.center = center;
+    return;
+  }
   Csr vdt, eb;
   vertex_dual_tris(t, vdt);
   edge_bduals(t, eb);
@@ -374,6 +389,7 @@ int dec_mesh_triangulated_grid(double lx, double ly, double dx, double dy, int32
 int dec_mesh_subdivide(dec_mesh coarse, int32_t project, dec_mesh *fine) {
   DEC_TRY
   if (!coarse || !fine) fail(DEC_ERR_ARG, "dec_mesh_subdivide: NULL handle");
+  if (coarse->dim != 2) fail(DEC_ERR_UNSUPPORTED, "dec_mesh_subdivide: binary subdivision is defined for 2-D complexes");
   *fine = dec::subdivide(*coarse, project != 0);
   DEC_CATCH
 }
@@ -415,6 +431,7 @@ int dec_mesh_export_dual(dec_mesh m, double *dual_point, double *length, double 
   DEC_TRY
   if (!m) fail(DEC_ERR_ARG, "dec_mesh_export_dual: NULL handle");
   if (!m->dualized) fail(DEC_ERR_ARG, "dec_mesh_export_dual: mesh is not dualized");
+  if (m->dim != 2) fail(DEC_ERR_ARG, "dec_mesh_export_dual: a 3-D complex exports with dec_mesh3_export_dual");
   if (dual_point) dec::d2h(dual_point, m->dpt.p, sizeof(double) * m->dpt.n);
   if (length) dec::d2h(length, m->length.p, sizeof(double) * m->length.n);
   if (area) dec::d2h(area, m->area.p, sizeof(double) * m->area.n);

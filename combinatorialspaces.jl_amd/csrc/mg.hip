@@ -671,6 +671,7 @@ std::vector<std::vector<int32_t>> level_colors(const dec_mg_s &mg, const std::ve
 
 dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge, int center, int steps,
                     bool galerkin) {
+  if (base.dim != 2) fail(DEC_ERR_UNSUPPORTED, "dec_mg_create: the multigrid hierarchy refines 2-D complexes");
   if (levels < 0 || levels > 11) fail(DEC_ERR_ARG, "dec_mg_create: levels must be in 0..11");
   if (steps < 0) fail(DEC_ERR_ARG, "dec_mg_create: steps must be >= 0");
   auto mg = std::make_unique<dec_mg_s>();

@@ -33,6 +33,8 @@ enum Impl {
   I_ELL,   // general sparse: geometric ⋆1, ∇²
   I_AVG,   // avg₀₁
   I_WPD,   // primal-dual wedge 0-1 / dual-primal 1-0: (M g) .* f, M = wedge_pd_01_mat
+  I_W03,   // wedge 0-3 (tetrahedra)
+  I_W21,   // wedge 2-1 (tetrahedra)
   I_GINV,  // geometric ⋆1⁻¹: a solve with ⋆1 (Jacobi-preconditioned CG on the device)
   I_COMP   // a composition of other operators (see Comp)
 };
@@ -59,6 +61,11 @@ struct dec_op_s {
   DBuf<int8_t> s8;
   // coefficient tables, stored as raw bytes of the op dtype
   DBuf<unsigned char> c0, c1, c2;
+  // tetrahedron tables (slot-major: slot k of row x at k * rows + x): vertex
+  // or edge/triangle ids, per-vertex weights, and sign bits of the 2-1 terms
+  DBuf<int32_t> ix;
+  DBuf<unsigned char> cx;
+  DBuf<uint16_t> m16;
   // sparse forms
   dec::Ell<int8_t> sell;
   dec::Ell<double> ell64;
@@ -294,6 +301,91 @@ template <class T>
 __global__ void k_d1(int nt, int vec, const int *__restrict__ ia, const int *__restrict__ ib, const int *__restrict__ ic,
                      const int8_t *__restrict__ sg, const T *__restrict__ x, T *__restrict__ res) {
   b_d1<T>(dec_ell::xcd_block(), nt, vec, ia, ib, ic, sg, x, res);
+}
+
+// wedge_kernel_03! (FastDEC.jl:173-183): res = Σ_j (α c_j) f[p_j] over the 24
+// dual tetrahedra.  The six dual tetrahedra at a vertex share p_j, so their
+// coefficients are stored summed: one weight per vertex (Σ dual_vol / vol),
+// four terms per tetrahedron in tetrahedron_vertices order.
+template <class T>
+__device__ __forceinline__ void b_wedge03(int blk, int n, const int *__restrict__ vi, const T *__restrict__ c,
+                                          const T *__restrict__ f, const T *__restrict__ a, T *__restrict__ res) {
+  int x = blk * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const size_t N = static_cast<size_t>(n);
+  int p0 = vi[x], p1 = vi[N + x], p2 = vi[2 * N + x], p3 = vi[3 * N + x];
+  T f0 = f[p0], f1 = f[p1], f2 = f[p2], f3 = f[p3];
+  T c0 = c[x], c1 = c[N + x], c2 = c[2 * N + x], c3 = c[3 * N + x];
+  T av = a[x];
+  T s = (av * c0) * f0;
+  s = s + (av * c1) * f1;
+  s = s + (av * c2) * f2;
+  s = s + (av * c3) * f3;
+  stn(res + x, s);
+}
+template <class T>
+__global__ void k_wedge03(int n, const int *__restrict__ vi, const T *__restrict__ c, const T *__restrict__ f,
+                          const T *__restrict__ a, T *__restrict__ res) {
+  b_wedge03<T>(dec_ell::xcd_block(), n, vi, c, f, a, res);
+}
+
+// wedge_kernel_21! (FastDEC.jl:194-208): α a primal 2-form on the triangles
+// t0..t3, β a primal 1-form on the edges e0..e5 of each tetrahedron.  Each of
+// the 12 coefficients (FastDEC.jl:96-133) is ± the weight ws_v = (dual volume
+// at vertex v / vol) / 3 of its vertex; bit k of sg is set when coefficient
+// k + 1 is negative.  Terms and their order are the reference kernel's.
+template <class T>
+__device__ __forceinline__ void b_wedge21(int blk, int n, const int *__restrict__ et, const T *__restrict__ ws,
+                                          const uint16_t *__restrict__ sg, const T *__restrict__ a,
+                                          const T *__restrict__ b, T *__restrict__ res) {
+  int x = blk * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const size_t N = static_cast<size_t>(n);
+  int e[6], t[4];
+#pragma unroll
+  for (int k = 0; k < 6; k++) e[k] = et[k * N + x];
+#pragma unroll
+  for (int k = 0; k < 4; k++) t[k] = et[(6 + k) * N + x];
+  T be[6], at[4];
+#pragma unroll
+  for (int k = 0; k < 6; k++) be[k] = b[e[k]];
+#pragma unroll
+  for (int k = 0; k < 4; k++) at[k] = a[t[k]];
+  T w0 = ws[x], w1 = ws[N + x], w2 = ws[2 * N + x], w3 = ws[3 * N + x];
+  unsigned m = sg[x];
+  auto C = [m](int k, T w) { return ((m >> k) & 1u) ? -w : w; };
+  T r = C(0, w0) * be[3] * at[3];
+  r = r - C(1, w0) * be[4] * at[2];
+  r = r + C(2, w0) * be[5] * at[1];
+  r = r + C(3, w1) * be[1] * at[3];
+  r = r - C(4, w1) * be[2] * at[2];
+  r = r + C(5, w1) * be[5] * at[0];
+  r = r + C(6, w2) * be[0] * at[3];
+  r = r - C(7, w2) * be[2] * at[1];
+  r = r + C(8, w2) * be[4] * at[0];
+  r = r + C(9, w3) * be[0] * at[2];
+  r = r - C(10, w3) * be[1] * at[1];
+  r = r + C(11, w3) * be[3] * at[0];
+  stn(res + x, r);
+}
+template <class T>
+__global__ void k_wedge21(int n, const int *__restrict__ et, const T *__restrict__ ws, const uint16_t *__restrict__ sg,
+                          const T *__restrict__ a, const T *__restrict__ b, T *__restrict__ res) {
+  b_wedge21<T>(dec_ell::xcd_block(), n, et, ws, sg, a, b, res);
+}
+
+// Per-vertex weights of the 0-3 (div = 1) and 2-1 (div = 3) wedges from the
+// dual volume at each vertex of each tetrahedron (slot-major 4 x n).
+template <class T>
+__global__ void k_coeff_tet(int n, const double *__restrict__ tw, const double *__restrict__ vol, int div3,
+                            T *__restrict__ out) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const size_t N = static_cast<size_t>(n);
+  for (int i = 0; i < 4; i++) {
+    double w = tw[i * N + x] / vol[x];
+    out[i * N + x] = static_cast<T>(div3 ? w / 3 : w);
+  }
 }
 
 // One row per thread (two rows per thread with int2 / T2 slot loads measured
@@ -1196,6 +1288,12 @@ void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream
     case I_AVG:
       hipLaunchKernelGGL(k_avg01<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, x, out);
       break;
+    case I_W03:
+      hipLaunchKernelGGL(k_wedge03<T>, g, b, 0, s, rows, op.ix.p, C(op.cx), x, y, out);
+      break;
+    case I_W21:
+      hipLaunchKernelGGL(k_wedge21<T>, g, b, 0, s, rows, op.ix.p, C(op.cx), op.m16.p, x, y, out);
+      break;
     case I_GINV:
       ginv_apply<T>(op, x, out, s);
       break;
@@ -1284,6 +1382,9 @@ struct GroupOp {
   const void *c0, *c1, *c2;
   const int *ecol;
   const void *eval;
+  const int *ix;
+  const void *cx;
+  const uint16_t *m16;
   const void *x, *y;
   void *out;
 };
@@ -1323,6 +1424,8 @@ __global__ void __launch_bounds__(256) k_apply_group(GroupArgs g) {
                                                      static_cast<const T *>(o.eval), x, out)));
       break;
     case I_AVG: b_avg01<T>(blk, o.rows, o.vec, o.i0, o.i1, x, out); break;
+    case I_W03: b_wedge03<T>(blk, o.rows, o.ix, static_cast<const T *>(o.cx), x, y, out); break;
+    case I_W21: b_wedge21<T>(blk, o.rows, o.ix, static_cast<const T *>(o.cx), o.m16, x, y, out); break;
     case I_WPD:
       DEC_ELL_WIDTH_SWITCH(o.width, (b_wedge_pd<W, T>(blk, o.rows, o.width, o.ecol, static_cast<const T *>(o.eval), x, y,
                                                       out)));
@@ -1338,7 +1441,9 @@ int group_blocks(const dec_op_s &op) {
   switch (op.impl) {
     case I_SELL:
     case I_ELL:
-    case I_WPD: return dec::nblocks(rows, 256);
+    case I_WPD:
+    case I_W03:
+    case I_W21: return dec::nblocks(rows, 256);
     default: return dec::nblocks((rows + 1) / 2, 256);
   }
 }
@@ -1378,6 +1483,9 @@ void apply_group_t(int count, dec_op_s *const *ops, const void *const *xs, const
       o.c0 = op.c0.p;
       o.c1 = op.c1.p;
       o.c2 = op.c2.p;
+      o.ix = op.ix.p;
+      o.cx = op.cx.p;
+      o.m16 = op.m16.p;
       o.x = x;
       o.y = y;
       o.out = outs[q];
@@ -1414,10 +1522,9 @@ void unsupported(int m, int n) {
 }
 
 template <class T>
-void make_diag(dec_op_s &op, const DBuf<double> &h, bool inverse, int k) {
+void make_diag(dec_op_s &op, const DBuf<double> &h, bool inverse, double mult) {
   if (inverse) {
     op.c0.alloc(sizeof(T) * h.n);
-    double mult = ((k * (2 - k)) % 2 == 0) ? 1.0 : -1.0;
     if (h.n)
       hipLaunchKernelGGL(k_inv_diag<T>, dim3(dec::nblocks(static_cast<long long>(h.n))), dim3(256), 0, 0,
                          static_cast<int>(h.n), h.p, mult, reinterpret_cast<T *>(op.c0.p));
@@ -1597,10 +1704,251 @@ std::unique_ptr<dec_op_s> ell_op(dec::HostCsr &&A, int dtype) {
   return o;
 }
 
+// ±1 matrix of d2 (FastDEC.jl:527-548) by rows (tetrahedra, triangles
+// ascending) or, transposed, by triangles (tetrahedra ascending).
+void csr_d2(const dec::Topo &t, const dec::Tets &T, bool transposed, dec::Csr &c) {
+  int n = T.n, nt = t.nt;
+  auto sgn = [&](int x, int k) {
+    int st = t.to[T.t[k][x]] == 1 ? 1 : -1, sx = T.o[x] == 1 ? 1 : -1;
+    return static_cast<int8_t>((k % 2 ? -1 : 1) * st * sx);
+  };
+  if (!transposed) {
+    c.n = n;
+    c.ptr.assign(n + 1, 0);
+    c.idx.clear();
+    c.val.clear();
+    for (int x = 0; x < n; x++) {
+      std::pair<int, int8_t> ent[4];
+      for (int k = 0; k < 4; k++) ent[k] = {T.t[k][x], sgn(x, k)};
+      std::sort(ent, ent + 4, [](const auto &a, const auto &b) { return a.first < b.first; });
+      for (int k = 0; k < 4; k++) {
+        if (k && ent[k].first == ent[k - 1].first) fail(DEC_ERR_UNSUPPORTED, "d2: tetrahedron with a repeated triangle");
+        c.idx.push_back(ent[k].first);
+        c.val.push_back(ent[k].second);
+      }
+      c.ptr[x + 1] = static_cast<int32_t>(c.idx.size());
+    }
+    return;
+  }
+  c.n = nt;
+  c.ptr.assign(nt + 1, 0);
+  for (int x = 0; x < n; x++)
+    for (int k = 0; k < 4; k++) c.ptr[T.t[k][x] + 1]++;
+  for (int f = 0; f < nt; f++) c.ptr[f + 1] += c.ptr[f];
+  c.idx.assign(c.ptr[nt], 0);
+  c.val.assign(c.ptr[nt], 0);
+  std::vector<int32_t> w(c.ptr.begin(), c.ptr.end() - 1);
+  for (int x = 0; x < n; x++)  // ascending x => each row sorted
+    for (int k = 0; k < 4; k++) {
+      int f = T.t[k][x];
+      c.idx[w[f]] = x;
+      c.val[w[f]++] = sgn(x, k);
+    }
+}
+
+// The operators of a 3-D complex that differ from those of a 2-D one.  Returns
+// null for the operators the triangle skeleton serves unchanged (wedges 0-1,
+// 0-2, 1-1 with the 2-D dual triangles of each triangle; d0, d1; ∂1, ∂2),
+// which create() then builds exactly as on a 2-D complex (FastDEC.jl methods
+// on HasDeltaSet2D apply to a HasDeltaSet3D).
+std::unique_ptr<dec_op_s> create3(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) {
+  const dec::Topo &t = m.h;
+  const dec::Tets &T = m.ht;
+  int nv = t.nv, ne = t.ne, nt = t.nt, n = T.n;
+  const size_t N = static_cast<size_t>(n);
+  bool f64 = dtype == DEC_F64;
+  auto op = std::make_unique<dec_op_s>();
+  op->kind = kind;
+  op->m = dm;
+  op->n = dn;
+  op->hodge = hodge;
+  op->dtype = dtype;
+  auto sell = [&](dec::Csr &&c, int ncols, bool neg) {
+    op->impl = I_SELL;
+    op->rows = c.n;
+    op->cols = ncols;
+    to_sign_ell(c, op->sell, neg);
+    csr_from_signs(c, ncols, neg, op->csr);
+  };
+  auto weights = [&](int div3) {
+    op->cx.alloc((f64 ? 8 : 4) * 4 * std::max<size_t>(N, 1));
+    if (n) {
+      if (f64)
+        hipLaunchKernelGGL(k_coeff_tet<double>, dim3(dec::nblocks(n)), dim3(256), 0, 0, n, m.tw.p, m.vol.p, div3,
+                           reinterpret_cast<double *>(op->cx.p));
+      else
+        hipLaunchKernelGGL(k_coeff_tet<float>, dim3(dec::nblocks(n)), dim3(256), 0, 0, n, m.tw.p, m.vol.p, div3,
+                           reinterpret_cast<float *>(op->cx.p));
+    }
+  };
+  switch (kind) {
+    case DEC_OP_WEDGE: {
+      int lo = std::min(dm, dn), hi = std::max(dm, dn);
+      if (lo < 0 || lo + hi > 3) unsupported(dm, dn);
+      if (!((lo == 0 && hi == 3) || (lo == 1 && hi == 2))) return nullptr;
+      op->rows = n;
+      std::vector<int32_t> ix;
+      if (lo == 0) {
+        // dec_wedge_product(0,3) / (3,0): the (0,3) kernel on (α, β) / (β, α) (FastDEC.jl:266-274)
+        op->impl = I_W03;
+        op->swap = dm > dn;
+        op->cols = dm == 0 ? nv : n;
+        op->cols2 = dm == 0 ? n : nv;
+        ix.resize(4 * N);
+        for (int x = 0; x < n; x++) {
+          int v[4];
+          dec::tet_vertices(t, T, x, v);
+          for (int i = 0; i < 4; i++) ix[i * N + x] = v[i];
+        }
+        op->ix.upload(ix);
+        weights(0);
+      } else {
+        // dec_wedge_product(2,1) on (α, β); (1,2) runs it on (β, α) (FastDEC.jl:281-289)
+        op->impl = I_W21;
+        op->swap = dm < dn;
+        op->cols = dm == 2 ? nt : ne;
+        op->cols2 = dm == 2 ? ne : nt;
+        ix.resize(10 * N);
+        std::vector<uint16_t> sg(N);
+        // (edge slot, triangle slot) of coefficients 1..12 (FastDEC.jl:117-130)
+        const int pe[12] = {3, 4, 5, 1, 2, 5, 0, 2, 4, 0, 1, 3};
+        const int pt[12] = {3, 2, 1, 3, 2, 0, 3, 1, 0, 2, 1, 0};
+        for (int x = 0; x < n; x++) {
+          int e[6];
+          dec::tet_edges(t, T, x, e);
+          for (int k = 0; k < 6; k++) ix[k * N + x] = e[k];
+          for (int k = 0; k < 4; k++) ix[(6 + k) * N + x] = T.t[k][x];
+          int sx = T.o[x] == 1 ? 1 : -1;
+          uint16_t bits = 0;
+          for (int k = 0; k < 12; k++) {
+            int s = sx * (t.eo[e[pe[k]]] == 1 ? 1 : -1) * (t.to[T.t[pt[k]][x]] == 1 ? 1 : -1);
+            if (s < 0) bits = static_cast<uint16_t>(bits | (1u << k));
+          }
+          sg[x] = bits;
+        }
+        op->ix.upload(ix);
+        op->m16.upload(sg);
+        weights(1);
+      }
+      break;
+    }
+    case DEC_OP_DIFFERENTIAL:
+      if (dm == 0 || dm == 1) return nullptr;
+      if (dm != 2) fail(DEC_ERR_UNSUPPORTED, "dec_differential: n must be 0, 1 or 2 on a 3-D complex");
+      {
+        dec::Csr c;
+        csr_d2(t, T, false, c);
+        sell(std::move(c), nt, false);
+      }
+      break;
+    case DEC_OP_BOUNDARY:
+      if (dm == 1 || dm == 2) return nullptr;
+      if (dm != 3) fail(DEC_ERR_UNSUPPORTED, "dec_boundary: n must be 1, 2 or 3 on a 3-D complex");
+      {
+        dec::Csr c;
+        csr_d2(t, T, true, c);
+        sell(std::move(c), n, false);
+      }
+      break;
+    case DEC_OP_DUAL_DERIVATIVE: {
+      // d̃0 = -∂3, d̃1 = ∂2, d̃2 = -∂1 (FastDEC.jl:455-462)
+      dec::Csr c;
+      if (dm == 0) {
+        csr_d2(t, T, true, c);
+        sell(std::move(c), n, true);
+      } else if (dm == 1) {
+        dec::edge_triangles(t, c);
+        sell(std::move(c), nt, false);
+      } else if (dm == 2) {
+        dec::vertex_edges(t, c);
+        sell(std::move(c), ne, true);
+      } else {
+        fail(DEC_ERR_UNSUPPORTED, "dec_dual_derivative: n must be 0, 1 or 2 on a 3-D complex");
+      }
+      break;
+    }
+    case DEC_OP_HODGE:
+    case DEC_OP_INV_HODGE: {
+      if (dm < 0 || dm > 3) fail(DEC_ERR_UNSUPPORTED, "dec_hodge_star: n must be 0, 1, 2 or 3 on a 3-D complex");
+      bool inv = kind == DEC_OP_INV_HODGE;
+      if (hodge == DEC_GEOMETRIC_HODGE) {
+        // dec_hodge_star has no geometric method on a 3-D complex; the inverse
+        // falls back to the diagonal one for n = 0, 3 (FastDEC.jl:750-757)
+        if (!inv)
+          fail(DEC_ERR_UNSUPPORTED,
+               "dec_hodge_star: the geometric Hodge star is not defined on a 3-D complex; use the diagonal one");
+        if (dm == 1 || dm == 2)
+          fail(DEC_ERR_UNSUPPORTED,
+               "The Geometric Hodge star in 3D for 1-forms and 2-forms has not yet been implemented. Please use the "
+               "Diagonal Hodge star instead.");
+      }
+      op->impl = I_DIAG;
+      const DBuf<double> &h = dm == 0 ? m.h0 : (dm == 1 ? m.h1 : (dm == 2 ? m.h2 : m.h3));
+      op->rows = op->cols = static_cast<int64_t>(h.n);
+      // (-1)^(k(3-k)) = 1 for every k (FastDEC.jl:721-727)
+      if (f64) make_diag<double>(*op, h, inv, 1.0);
+      else make_diag<float>(*op, h, inv, 1.0);
+      std::vector<double> hv = h.download();
+      if (inv)
+        for (auto &x : hv) x = (1 / x) * 1.0;
+      csr_diag(hv, op->csr);
+      break;
+    }
+    case DEC_OP_LAPLACIAN0: {
+      // ∇²(0, s) = -⋆0⁻¹ d̃2 ⋆1 d0 (DiscreteExteriorCalculus.jl:1836-1890); d̃2 = -d0ᵀ as d̃1 in 2-D
+      if (hodge != DEC_DIAGONAL_HODGE)
+        fail(DEC_ERR_UNSUPPORTED, "∇²: a 3-D complex has only the diagonal Hodge star; pass DEC_DIAGONAL_HODGE");
+      op->impl = I_ELL;
+      op->rows = op->cols = nv;
+      std::vector<double> h0 = m.h0.download(), h1 = m.h1.download(), ih0(nv);
+      for (int v = 0; v < nv; v++) ih0[v] = 1 / h0[v];
+      dec::assemble_laplacian0(t, ih0, h1, nullptr, op->csr);
+      if (f64) dec::upload_ell(op->csr, op->ell64);
+      else dec::upload_ell(op->csr, op->ell32);
+      break;
+    }
+    case DEC_OP_DUAL_LAPLACIAN: {
+      // Δᵈ(Val(0)) on a 3-D complex: m = ⋆3 d2 ⋆2⁻¹ d̃0 as one matrix (FastDEC.jl:832-839)
+      if (dm != 0) fail(DEC_ERR_UNSUPPORTED, "Δᵈ: n must be 0 on a 3-D complex");
+      std::vector<double> h3 = m.h3.download(), h2 = m.h2.download();
+      dec::Csr rows, cols;
+      csr_d2(t, T, false, rows);
+      csr_d2(t, T, true, cols);
+      dec::HostCsr A, B, M;
+      A.m = n;
+      A.n = nt;
+      A.ptr = rows.ptr;
+      A.col = rows.idx;
+      A.val.resize(rows.idx.size());
+      for (int x = 0; x < n; x++)
+        for (int k = rows.ptr[x]; k < rows.ptr[x + 1]; k++)
+          A.val[k] = (h3[x] * rows.val[k]) * ((1 / h2[rows.idx[k]]) * 1.0);
+      csr_from_signs(cols, n, true, B);
+      dec::spgemm(A, B, M);
+      auto e = ell_op(std::move(M), dtype);
+      e->kind = kind;
+      e->m = dm;
+      e->hodge = hodge;
+      return e;
+    }
+    default:
+      fail(DEC_ERR_UNSUPPORTED, "dec_op_create: operator kind " + std::to_string(kind) +
+                                    " is defined on 2-D complexes only");
+  }
+  return op;
+}
+
 dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) {
   if (dtype != DEC_F64 && dtype != DEC_F32) fail(DEC_ERR_ARG, "dec_op_create: dtype must be DEC_F64 or DEC_F32");
   if (hodge != DEC_GEOMETRIC_HODGE && hodge != DEC_DIAGONAL_HODGE) fail(DEC_ERR_ARG, "dec_op_create: unknown hodge");
   dec::ensure_dual(m);
+  if (m.dim == 3) {
+    if (auto o3 = create3(m, kind, dm, dn, hodge, dtype)) {
+      dec::check_launch("dec_op_create");
+      dec::dsync();
+      return o3.release();
+    }
+  }
   const dec::Topo &t = m.h;
   int nv = t.nv, ne = t.ne, nt = t.nt;
   bool f64 = dtype == DEC_F64;
@@ -1902,13 +2250,13 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
       op->impl = I_DIAG;
       const DBuf<double> &h = dm == 0 ? m.h0 : (dm == 1 ? m.h1 : m.h2);
       op->rows = op->cols = static_cast<int64_t>(h.n);
-      if (f64) make_diag<double>(*op, h, inv, dm);
-      else make_diag<float>(*op, h, inv, dm);
+      // dec_inv_hodge_star: (1 ./ hdg) .* (-1)^(k(n-k)) (FastDEC.jl:721-727)
+      double mult = ((dm * (2 - dm)) % 2 == 0) ? 1.0 : -1.0;
+      if (f64) make_diag<double>(*op, h, inv, mult);
+      else make_diag<float>(*op, h, inv, mult);
       std::vector<double> hv = h.download();
-      if (inv) {
-        double mult = ((dm * (2 - dm)) % 2 == 0) ? 1.0 : -1.0;
+      if (inv)
         for (auto &x : hv) x = (1 / x) * mult;
-      }
       csr_diag(hv, op->csr);
       break;
     }
@@ -2089,6 +2437,24 @@ int dec_op_export_csr(dec_op op, int64_t *nnz, int32_t *rowptr, int32_t *col, do
   DEC_CATCH
 }
 
+}  // extern "C"
+
+namespace {
+// slots x rows weights of a tetrahedron wedge as float64
+void export_weights(const dec_op_s &op, int slots, double *out) {
+  size_t cnt = static_cast<size_t>(slots) * static_cast<size_t>(op.rows);
+  if (op.dtype == DEC_F64) {
+    dec::d2h(out, op.cx.p, sizeof(double) * cnt);
+  } else {
+    std::vector<float> tmp(cnt);
+    dec::d2h(tmp.data(), op.cx.p, sizeof(float) * cnt);
+    for (size_t i = 0; i < cnt; i++) out[i] = tmp[i];
+  }
+}
+}  // namespace
+
+extern "C" {
+
 int dec_op_export_tables(dec_op op, int32_t *idx, double *coeff) {
   DEC_TRY
   if (!op) fail(DEC_ERR_ARG, "dec_op_export_tables: NULL handle");
@@ -2116,6 +2482,26 @@ int dec_op_export_tables(dec_op op, int32_t *idx, double *coeff) {
     case I_WDD:
       dl_i(op->i0, 0); dl_i(op->i1, 1); dl_c(op->c0, 0); dl_c(op->c1, 1);
       break;
+    case I_W03:  // vertex slots 0..3 and their weights
+      if (idx && n) dec::d2h(idx, op->ix.p, sizeof(int32_t) * 4 * n);
+      if (coeff && n) export_weights(*op, 4, coeff);
+      break;
+    case I_W21: {  // edges e0..e5 and triangles t0..t3; the 12 signed coefficients
+      if (idx && n) dec::d2h(idx, op->ix.p, sizeof(int32_t) * 10 * n);
+      if (coeff && n) {
+        std::vector<double> ws(4 * n);
+        export_weights(*op, 4, ws.data());
+        std::vector<uint16_t> sg(n);
+        dec::d2h(sg.data(), op->m16.p, sizeof(uint16_t) * n);
+        const int vert[12] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3};
+        for (int k = 0; k < 12; k++)
+          for (size_t x = 0; x < n; x++) {
+            double w = ws[vert[k] * n + x];
+            coeff[k * n + x] = ((sg[x] >> k) & 1u) ? -w : w;
+          }
+      }
+      break;
+    }
     default:
       fail(DEC_ERR_UNSUPPORTED, "dec_op_export_tables: operator has no wedge tables");
   }

@@ -112,6 +112,44 @@ int dec_mesh_export_dual(dec_mesh m, double *dual_point, double *length,
                          double *area, double *dual_length, double *dual_area);
 int dec_mesh_destroy(dec_mesh m);
 
+/* ------------------------------------------------------- 3-D (tetrahedra) */
+/* EmbeddedDeltaSet3D from boundary maps (SimplicialSets.jl:418-606): edges
+ * (∂v0, ∂v1), triangles (∂e0, ∂e1, ∂e2), tetrahedra (∂t0..∂t3).  Orientation
+ * arrays may be NULL or hold -1 for "unset": when the dual complex is built,
+ * unset edges and triangles become false and unset tetrahedra are set by
+ * orient!(s, Val(3)) (DiscreteExteriorCalculus.jl:311-321, 573-584, 1198-1207).
+ * The handle is a dec_mesh: dec_mesh_dualize, dec_mesh_export (the triangle
+ * skeleton), dec_op_create and dec_mesh_destroy take it. */
+int dec_mesh3_create(int32_t nv, int32_t ne, int32_t nt, int32_t ntet,
+                     const int32_t *v0, const int32_t *v1, const int32_t *e0,
+                     const int32_t *e1, const int32_t *e2, const int32_t *t0,
+                     const int32_t *t1, const int32_t *t2, const int32_t *t3,
+                     const int8_t *edge_orient, const int8_t *tri_orient,
+                     const int8_t *tet_orient, const double *points,
+                     int32_t index_base, dec_mesh *out);
+/* EmbeddedDeltaSet3D(mesh) from a tetrahedron list (MeshInterop.jl:141-153):
+ * glue_sorted_tetrahedron! (sorted = 1) or glue_tetrahedron! (sorted = 0) of
+ * each tetrahedron in order (SimplicialSets.jl:511-534), then orient!.
+ * points: nv x 3 float64; tets: ntet x 4 vertex ids counted from index_base. */
+int dec_mesh3_from_tets(int32_t nv, int32_t ntet, const double *points,
+                        const int32_t *tets, int32_t sorted, int32_t index_base,
+                        dec_mesh *out);
+/* 2 for a triangle complex, 3 for a tetrahedral one. */
+int dec_mesh_dim(dec_mesh m, int32_t *dim);
+int dec_mesh3_counts(dec_mesh m, int32_t *nv, int32_t *ne, int32_t *nt, int32_t *ntet);
+/* Host copies of ∂t0..∂t3 and the tetrahedron orientation (after orient!). */
+int dec_mesh3_export(dec_mesh m, int32_t *t0, int32_t *t1, int32_t *t2, int32_t *t3,
+                     int8_t *tet_orient);
+/* Host copies of the 3-D dual geometry (subdivide_duals_3d! +
+ * precompute_volumes_3d!, DiscreteExteriorCalculus.jl:1303-1335):
+ * dual_point (nv+ne+nt+ntet) x 3, length ne, area nt, vol ntet, dual_length
+ * and dual_area of the first 2ne+6nt dual edges and 6nt dual triangles (the
+ * 1-D and 2-D blocks, numbered as on a 2-D complex), and dual_vol 24 ntet
+ * (DualTet 24x + q, D_tetrahedron_schemas order).  Any pointer may be NULL. */
+int dec_mesh3_export_dual(dec_mesh m, double *dual_point, double *length, double *area,
+                          double *vol, double *dual_length, double *dual_area,
+                          double *dual_vol);
+
 /* --------------------------------------------------------------- operators */
 /* Create an operator on a dualized mesh.
  *  DEC_OP_WEDGE, (m,n) in {(0,1),(1,0),(0,2),(2,0),(1,1)}:
@@ -150,6 +188,17 @@ int dec_mesh_destroy(dec_mesh m);
  *  DEC_OP_DUAL_LAPLACIAN n=0,1: Δᵈ(Val(n)) on a 2-D complex (FastDEC.jl:808-860).
  *  The last three and the 1-1 wedges are compositions of other operators
  *  (and of the geometric ⋆1⁻¹ solve); they own work vectors.
+ * On a 3-D complex (FastDEC.jl methods for HasDeltaSet3D):
+ *  DEC_OP_WEDGE (0,1),(1,0),(0,2),(2,0),(1,1) as above on the triangle
+ *     skeleton; (0,3),(3,0) (FastDEC.jl:64-76, 173-183) and (2,1),(1,2)
+ *     (FastDEC.jl:96-133, 194-208, 281-289).
+ *  DEC_OP_DIFFERENTIAL n=0,1,2; DEC_OP_BOUNDARY n=1,2,3;
+ *  DEC_OP_DUAL_DERIVATIVE n=0,1,2: d̃0 = -∂3, d̃1 = ∂2, d̃2 = -∂1 (FastDEC.jl:455-462).
+ *  DEC_OP_HODGE n=0..3 with DEC_DIAGONAL_HODGE (FastDEC.jl:606-636);
+ *  DEC_OP_INV_HODGE n=0..3 diagonal, and geometric for n=0,3 (FastDEC.jl:750-757).
+ *  DEC_OP_LAPLACIAN0 with DEC_DIAGONAL_HODGE: -⋆0⁻¹ d̃2 ⋆1 d0.
+ *  DEC_OP_DUAL_LAPLACIAN n=0: ⋆3 d2 ⋆2⁻¹ d̃0 as one matrix (FastDEC.jl:832-839).
+ *  Other kinds return DEC_ERR_UNSUPPORTED.
  * deg_n is ignored for single-degree kinds. */
 int dec_op_create(dec_mesh m, int32_t kind, int32_t deg_m, int32_t deg_n,
                   int32_t hodge, int32_t dtype, dec_op *out);
@@ -177,7 +226,11 @@ int dec_op_export_csr(dec_op op, int64_t *nnz, int32_t *rowptr, int32_t *col,
                       double *val);
 /* Wedge coefficient tables as the kernels read them (SoA, float64 copies):
  * wedge 0-1: idx 2 x ne; wedge 0-2: idx 3 x nt, coeff 3 x nt (pairs of the
- * reference's six coefficients pre-summed); wedge 1-1: idx 3 x nt, coeff 3 x nt. */
+ * reference's six coefficients pre-summed); wedge 1-1: idx 3 x nt, coeff 3 x nt;
+ * wedge 0-3: idx 4 x ntet (tetrahedron_vertices), coeff 4 x ntet (the
+ * reference's 24 coefficients summed per vertex); wedge 2-1: idx 10 x ntet
+ * (tetrahedron_edges, then tetrahedron_triangles), coeff 12 x ntet (the
+ * reference's signed coefficients). */
 int dec_op_export_tables(dec_op op, int32_t *idx, double *coeff);
 int dec_op_destroy(dec_op op);
 

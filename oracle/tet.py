@@ -202,13 +202,14 @@ def blessent_cube():
     return m
 
 
-def kuhn_grid_arrays(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, jitter=0.0, seed=0, shear=(0.0, 0.0)):
+def kuhn_grid_arrays(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, jitter=0.0, seed=0, shear=(0.0, 0.0),
+                     jitter_boundary=False):
     """Points and tetrahedra of an nx x ny x nz box of cubes, each cut into the
     six Kuhn tetrahedra 000 → +e_a → +e_a+e_b → 111 over the permutations
     (a, b, c) of the axes in lexicographic order; vertex (i, j, k) has id
     i + (nx+1)(j + (ny+1)k).  Interior points move by up to `jitter` cell
-    widths (seeded); `shear` = (dx, dy) slides the top face as the
-    reference's parallelepiped does.  A synthetic mesh: the reference makes
+    widths (seeded), boundary points too with `jitter_boundary`; `shear` =
+    (dx, dy) slides the top face as the reference's parallelepiped does.  A synthetic mesh: the reference makes
     its 3-D test meshes with TetGen, which is not available here."""
     gx, gy, gz = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
     ijk = np.stack([gx.ravel(order="F"), gy.ravel(order="F"), gz.ravel(order="F")], 1)
@@ -216,7 +217,7 @@ def kuhn_grid_arrays(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, jitter=0.0, seed=0, she
     p = ijk * h
     if jitter:
         rng = np.random.default_rng(seed)
-        inner = np.all((ijk > 0) & (ijk < [nx, ny, nz]), axis=1)
+        inner = np.all((ijk > 0) & (ijk < [nx, ny, nz]), axis=1) | jitter_boundary
         p[inner] += (rng.random((inner.sum(), 3)) * 2 - 1) * jitter * h
     p[:, 0] += shear[0] * p[:, 2] / lz
     p[:, 1] += shear[1] * p[:, 2] / lz

@@ -71,3 +71,39 @@ def test_new_entry_points_reject_bad_arguments_before_device_work():
     assert lib.dec_mg_create_mode(None, 2, 0, 0, 0, 2, 0, ctypes.byref(h)) == 1
     it, rr = ctypes.c_int32(), ctypes.c_double()
     assert lib.dec_op_solve_info(None, ctypes.byref(it), ctypes.byref(rr), None) == 1
+
+
+def test_tetrahedral_entry_points_reject_bad_arguments_before_device_work():
+    """The 3-D mesh constructors and accessors report argument errors with a
+    status and a message, without a GPU."""
+    import numpy as np
+    from cspaces import _lib
+
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    pts = np.zeros((4, 3))
+    p = pts.ctypes.data_as(ctypes.c_void_p)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    bad = np.array([[0, 1, 2, 4]], np.int32)                    # vertex 4 of 4 points
+    assert lib.dec_mesh3_from_tets(4, 1, p, vp(bad), 1, 0, ctypes.byref(h)) == _lib.DEC_ERR_ARG
+    assert b"out of range" in lib.dec_last_error()
+    rep = np.array([[0, 1, 1, 2]], np.int32)
+    assert lib.dec_mesh3_from_tets(4, 1, p, vp(rep), 1, 0, ctypes.byref(h)) == _lib.DEC_ERR_ARG
+    assert b"repeated vertex" in lib.dec_last_error()
+    good = np.array([[1, 2, 3, 4]], np.int32)
+    assert lib.dec_mesh3_from_tets(4, 1, p, vp(good), 1, 2, ctypes.byref(h)) == _lib.DEC_ERR_ARG
+    assert b"index_base" in lib.dec_last_error()
+    assert lib.dec_mesh3_from_tets(4, 1, None, None, 1, 0, ctypes.byref(h)) == _lib.DEC_ERR_ARG
+    z = np.zeros(1, np.int32)
+    args = [1, 1, 1, 1] + [None] * 9 + [None, None, None, p, 0, ctypes.byref(h)]
+    assert lib.dec_mesh3_create(*args) == _lib.DEC_ERR_ARG
+    assert b"NULL array" in lib.dec_last_error()
+    args = [4, 1, 1, 1, vp(z), vp(z), vp(z), vp(z), vp(z), vp(np.full(1, 5, np.int32)), vp(z), vp(z), vp(z),
+            None, None, None, p, 0, ctypes.byref(h)]
+    assert lib.dec_mesh3_create(*args) == _lib.DEC_ERR_ARG
+    assert b"tetrahedron triangle out of range" in lib.dec_last_error()
+    dim = ctypes.c_int32()
+    assert lib.dec_mesh_dim(None, ctypes.byref(dim)) == _lib.DEC_ERR_ARG
+    assert lib.dec_mesh3_counts(None, None, None, None, None) == _lib.DEC_ERR_ARG
+    assert lib.dec_mesh3_export(None, None, None, None, None, None) == _lib.DEC_ERR_ARG
+    assert lib.dec_mesh3_export_dual(None, None, None, None, None, None, None, None) == _lib.DEC_ERR_ARG
