@@ -31,10 +31,14 @@ HBM_PEAK_GBPS = 8000.0
 OPSET = ["wedge01", "wedge11", "wedge02", "d0", "d1", "hodge0", "hodge1", "hodge2", "laplacian0"]
 
 
-def alg_bytes(name, nv, ne, nt, s, lap_nnz=None):
+def alg_bytes(name, nv, ne, nt, s, lap_nnz=None, ntet=0):
     """Compulsory bytes of one application: every table / input element read
     once, every output written once (int32 indices, s-byte values)."""
     return {
+        "wedge03": ntet * (16 + 6 * s) + nv * s,
+        "wedge21": ntet * (42 + 5 * s) + (nt + ne) * s,
+        "d2": ntet * (20 + s) + nt * s,
+        "hodge3": 3 * s * ntet,
         "wedge01": ne * (8 + 2 * s) + nv * s,
         "wedge11": nt * (12 + 4 * s) + 2 * ne * s,
         "wedge02": nt * (12 + 5 * s) + nv * s,
@@ -63,6 +67,12 @@ def make_op(cs, mesh, name, td):
         return cs.dec_differential(0, mesh, cast_float=td)
     if name == "d1":
         return cs.dec_differential(1, mesh, cast_float=td)
+    if name == "wedge03":
+        return cs.dec_wedge_product(0, 3, mesh, cast_float=td).operator
+    if name == "wedge21":
+        return cs.dec_wedge_product(2, 1, mesh, cast_float=td).operator
+    if name == "d2":
+        return cs.dec_differential(2, mesh, cast_float=td)
     if name == "boundary1":
         return cs.dec_boundary(1, mesh, cast_float=td)
     if name == "boundary2":
@@ -185,6 +195,36 @@ def musical_ops(torch, cs, mesh, reps=20):
         res[name] = {"ms": (time.perf_counter() - t0) / reps * 1e3}
     res["flat_sharp"]["GBps"] = (nnz * 12 + 2 * ne * 8) / (res["flat_sharp"]["ms"] * 1e-3) / 1e9
     res["flat_sharp"]["nnz"] = nnz
+    return res
+
+
+OPSET3 = ["wedge01", "wedge11", "wedge02", "wedge03", "wedge21", "d0", "d1", "d2", "hodge0", "hodge1", "hodge2",
+          "hodge3", "laplacian0"]
+
+
+def ops_3d(torch, cs, n=64, sets=3, steps=30, warmup=3):
+    """The 3-D operator set on an n x n x n Kuhn box (6 n³ tetrahedra) with
+    its barycentric dual, float64: one application each of the wedges 0-1,
+    1-1, 0-2, 0-3 and 2-1, d0, d1, d2, ⋆0..⋆3 and the 0-form Laplacian per
+    step, timed as in the 2-D step (rotating sets, HIP events on the kernels'
+    stream); GB/s of the compulsory bytes."""
+    t = time.perf_counter()
+    mesh = cs.kuhn_box(n, n, n)
+    cs.dualize(mesh, cs.Barycenter)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t
+    nv, ne, nt, ntet = mesh.nv, mesh.ne, mesh.nt, mesh.ntet
+    per, total, grouped, _, lap_nnz = time_opset(torch, cs, mesh, OPSET3, torch.float64, sets, steps, warmup,
+                                                 cache_resident=False)
+    B = {k: alg_bytes(k, nv, ne, nt, 8, lap_nnz, ntet) for k in OPSET3}
+    step = sum(B.values())
+    res = {"mesh": f"Kuhn box {n}^3 cubes ({nv} V / {ne} E / {nt} T / {ntet} Tet), barycentric dual",
+           "setup_s": setup, "step_MB": step / 1e6,
+           "grouped_GBps": step / (grouped / steps * 1e-3) / 1e9,
+           "one_launch_per_op_GBps": step / (total / steps * 1e-3) / 1e9,
+           "ops": {k: {"ms": per[k], "GBps": B[k] / (per[k] * 1e-3) / 1e9} for k in OPSET3}}
+    del mesh
+    torch.cuda.empty_cache()
     return res
 
 
@@ -534,6 +574,10 @@ def main():
     ap.add_argument("--pmc-probe", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--composite-only", action="store_true",
                     help="time only the composite operators (ms per application) and print their JSON")
+    ap.add_argument("--no-3d", action="store_true", help="skip the operator set on a tetrahedral mesh")
+    ap.add_argument("--ops3d-only", action="store_true",
+                    help="time only the operator set on a tetrahedral mesh and print its JSON")
+    ap.add_argument("--n3d", type=int, default=64, help="cubes per side of the Kuhn box of the 3-D leg")
     ap.add_argument("--traffic-per-op", action="store_true",
                     help="measure the HBM traffic of each operator of the step (two counter passes each)")
     args = ap.parse_args()
@@ -552,6 +596,10 @@ def main():
     import cspaces as cs
 
     torch.cuda.init()
+    if args.ops3d_only:
+        print(json.dumps({"ops_3d_f64": ops_3d(torch, cs, args.n3d, steps=args.steps, warmup=args.warmup)}),
+              flush=True)
+        return
     t = time.perf_counter()
     mesh = cs.icosphere(args.k)
     cs.dualize(mesh)
@@ -611,6 +659,10 @@ def main():
                         "one_launch_per_op_GBps": sum(B32.values()) / (total32 / args.steps * 1e-3) / 1e9,
                         "ops": {n: {"ms": per32[n], "GBps": B32[n] / (per32[n] * 1e-3) / 1e9} for n in OPSET}}
         print(f"# f32 op set: {extra['f32']['value_GBps']:.0f} GB/s aggregate", file=sys.stderr, flush=True)
+    if not args.no_3d:
+        extra["ops_3d_f64"] = ops_3d(torch, cs, args.n3d)
+        print(f"# 3-D op set: {extra['ops_3d_f64']['grouped_GBps']:.0f} GB/s grouped launch", file=sys.stderr,
+              flush=True)
     try:
         extra["stream_copy_GBps"] = stream_copy_gbps(torch)
     except RuntimeError as e:  # memory

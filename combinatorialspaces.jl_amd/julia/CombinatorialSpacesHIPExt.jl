@@ -14,7 +14,7 @@ image); tests drive the same C ABI from Python.
 module CombinatorialSpacesHIPExt
 
 using CombinatorialSpaces
-using CombinatorialSpaces.DiscreteExteriorCalculus: DiscreteHodge
+using CombinatorialSpaces.DiscreteExteriorCalculus: DiscreteHodge, Circumcenter, Barycenter
 using CombinatorialSpaces.Multigrid: BinarySubdivision, AbstractSubdivisionScheme,
   AbstractMultigridMode, DirectMode, GalerkinMode
 using AMDGPU
@@ -30,7 +30,7 @@ const libdec_hip = get(ENV, "DEC_HIP_LIB", "libdec_hip")
 
 # C ABI constants (include/dec_hip.h)
 const DEC_F64, DEC_F32 = Int32(0), Int32(1)
-const DEC_CIRCUMCENTER = Int32(0)
+const DEC_CIRCUMCENTER, DEC_BARYCENTER = Int32(0), Int32(1)
 const DEC_GEOMETRIC_HODGE, DEC_DIAGONAL_HODGE = Int32(0), Int32(1)
 const DEC_OP_WEDGE, DEC_OP_WEDGE_DD, DEC_OP_DIFFERENTIAL, DEC_OP_BOUNDARY,
       DEC_OP_DUAL_DERIVATIVE, DEC_OP_HODGE, DEC_OP_INV_HODGE, DEC_OP_LAPLACIAN0,
@@ -48,6 +48,12 @@ dtype_code(::Type{Float64}) = DEC_F64
 dtype_code(::Type{Float32}) = DEC_F32
 hodge_code(::GeometricHodge) = DEC_GEOMETRIC_HODGE
 hodge_code(::DiagonalHodge) = DEC_DIAGONAL_HODGE
+center_code(::Circumcenter) = DEC_CIRCUMCENTER
+center_code(::Barycenter) = DEC_BARYCENTER
+
+# The FastDEC methods below that CombinatorialSpaces defines for every
+# HasDeltaSet2D also serve tetrahedral complexes (FastDEC.jl:42-133, 433-462, 606-636).
+const DualComplex = Union{EmbeddedDeltaDualComplex2D, EmbeddedDeltaDualComplex3D}
 stream_ptr() = AMDGPU.stream().stream
 
 # ---------------------------------------------------------------- meshes
@@ -58,28 +64,60 @@ end
 
 const MESHES = WeakKeyDict{Any, HIPMesh}()
 
+function pack_points(sd)
+  pts = Vector{Float64}(undef, 3 * nv(sd))
+  for (i, p) in enumerate(sd[:point])
+    pts[3i-2] = p[1]; pts[3i-1] = p[2]; pts[3i] = length(p) > 2 ? p[3] : 0.0
+  end
+  pts
+end
+orientation_codes(xs) = Int8[isnothing(o) ? -1 : Int8(o) for o in xs]
+
+function register_mesh(sd, h, center)
+  check(ccall((:dec_mesh_dualize, libdec_hip), Cint, (Ptr{Cvoid}, Int32), h, center_code(center)))
+  m = HIPMesh(h)
+  finalizer(x -> ccall((:dec_mesh_destroy, libdec_hip), Cint, (Ptr{Cvoid},), x.handle), m)
+  m
+end
+
 """Upload (once per dual complex) the primal boundary maps, orientations and
-points; the dual geometry is then computed on the GPU in Float64."""
-function hip_mesh(sd::EmbeddedDeltaDualComplex2D)
+points; the dual geometry is then computed on the GPU in Float64 with the
+given center (call `hip_mesh(sd; center=Barycenter())` before the first
+operator for a barycentric dual)."""
+function hip_mesh(sd::EmbeddedDeltaDualComplex2D; center=Circumcenter())
   get!(MESHES, sd) do
     nv_, ne_, nt_ = nv(sd), ne(sd), ntriangles(sd)
     v0 = Int32.(sd[:∂v0]); v1 = Int32.(sd[:∂v1])
     e0 = Int32.(sd[:∂e0]); e1 = Int32.(sd[:∂e1]); e2 = Int32.(sd[:∂e2])
-    eo = Int8[isnothing(o) ? -1 : Int8(o) for o in sd[:edge_orientation]]
-    to = Int8[isnothing(o) ? -1 : Int8(o) for o in sd[:tri_orientation]]
-    pts = Vector{Float64}(undef, 3 * nv_)
-    for (i, p) in enumerate(sd[:point])
-      pts[3i-2] = p[1]; pts[3i-1] = p[2]; pts[3i] = length(p) > 2 ? p[3] : 0.0
-    end
+    eo = orientation_codes(sd[:edge_orientation])
+    to = orientation_codes(sd[:tri_orientation])
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:dec_mesh_create, libdec_hip), Cint,
       (Int32, Int32, Int32, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32},
        Ptr{Int8}, Ptr{Int8}, Ptr{Float64}, Int32, Ptr{Ptr{Cvoid}}),
-      nv_, ne_, nt_, v0, v1, e0, e1, e2, eo, to, pts, Int32(1), h))
-    check(ccall((:dec_mesh_dualize, libdec_hip), Cint, (Ptr{Cvoid}, Int32), h[], DEC_CIRCUMCENTER))
-    m = HIPMesh(h[])
-    finalizer(x -> ccall((:dec_mesh_destroy, libdec_hip), Cint, (Ptr{Cvoid},), x.handle), m)
-    m
+      nv_, ne_, nt_, v0, v1, e0, e1, e2, eo, to, pack_points(sd), Int32(1), h))
+    register_mesh(sd, h[], center)
+  end
+end
+
+# A tetrahedral complex: dec_mesh3_create with ∂t0..∂t3 and the tetrahedron
+# orientation as well (SimplicialSets.jl:418-606).
+function hip_mesh(sd::EmbeddedDeltaDualComplex3D; center=Circumcenter())
+  get!(MESHES, sd) do
+    v0 = Int32.(sd[:∂v0]); v1 = Int32.(sd[:∂v1])
+    e0 = Int32.(sd[:∂e0]); e1 = Int32.(sd[:∂e1]); e2 = Int32.(sd[:∂e2])
+    t0 = Int32.(sd[:∂t0]); t1 = Int32.(sd[:∂t1]); t2 = Int32.(sd[:∂t2]); t3 = Int32.(sd[:∂t3])
+    eo = orientation_codes(sd[:edge_orientation])
+    to = orientation_codes(sd[:tri_orientation])
+    teo = orientation_codes(sd[:tet_orientation])
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:dec_mesh3_create, libdec_hip), Cint,
+      (Int32, Int32, Int32, Int32, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32},
+       Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int8}, Ptr{Int8}, Ptr{Int8},
+       Ptr{Float64}, Int32, Ptr{Ptr{Cvoid}}),
+      nv(sd), ne(sd), ntriangles(sd), ntetrahedra(sd), v0, v1, e0, e1, e2, t0, t1, t2, t3,
+      eo, to, teo, pack_points(sd), Int32(1), h))
+    register_mesh(sd, h[], center)
   end
 end
 
@@ -119,15 +157,16 @@ Base.:*(op::HIPOperator{T}, x::ROCVector{T}) where T = apply!(ROCVector{T}(undef
 Base.size(op::HIPOperator) = (op.rows, op.cols)
 
 float_of(sd::EmbeddedDeltaDualComplex2D{Bool, F, P}) where {F, P} = F
+float_of(sd::EmbeddedDeltaDualComplex3D{Bool, F, P}) where {F, P} = F
 
 # Wedge products: same closure protocol as FastDEC (src/FastDEC.jl:255-289).
-function cache_wedge(::Val{m}, ::Val{n}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP},
+function cache_wedge(::Val{m}, ::Val{n}, sd::DualComplex, ::Val{:HIP},
                      arr_cons=ROCArray, cast_float=nothing) where {m, n}
   T = isnothing(cast_float) ? float_of(sd) : cast_float
   HIPOperator{T}(sd, DEC_OP_WEDGE, m, n, DEC_GEOMETRIC_HODGE)
 end
 
-function dec_wedge_product(::Val{m}, ::Val{n}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP},
+function dec_wedge_product(::Val{m}, ::Val{n}, sd::DualComplex, ::Val{:HIP},
                            arr_cons=ROCArray, cast_float=nothing) where {m, n}
   (m, n) == (0, 0) && return (f, g) -> f .* g
   op = cache_wedge(Val(m), Val(n), sd, Val(:HIP), arr_cons, cast_float)
@@ -142,22 +181,22 @@ function dec_wedge_product_dd(::Val{m}, ::Val{n}, sd::EmbeddedDeltaDualComplex2D
 end
 
 # Sparse operators (ext/CombinatorialSpacesCUDAExt.jl:30-61).
-dec_boundary(n::Int, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) =
+dec_boundary(n::Int, sd::DualComplex, ::Val{:HIP}) =
   HIPOperator{float_of(sd)}(sd, DEC_OP_BOUNDARY, n, 0, DEC_GEOMETRIC_HODGE)
-dec_dual_derivative(n::Int, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) =
+dec_dual_derivative(n::Int, sd::DualComplex, ::Val{:HIP}) =
   HIPOperator{float_of(sd)}(sd, DEC_OP_DUAL_DERIVATIVE, n, 0, DEC_GEOMETRIC_HODGE)
-dec_differential(n::Int, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) =
+dec_differential(n::Int, sd::DualComplex, ::Val{:HIP}) =
   HIPOperator{float_of(sd)}(sd, DEC_OP_DIFFERENTIAL, n, 0, DEC_GEOMETRIC_HODGE)
 avg₀₁_mat(sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) =
   HIPOperator{float_of(sd)}(sd, DEC_OP_AVG01, 0, 0, DEC_GEOMETRIC_HODGE)
 
 # Hodge stars (ext/CombinatorialSpacesCUDAExt.jl:70-96): diagonal stars stay
 # vectors on the device (no dense n×n matrix), geometric ⋆1 is a sparse operator.
-dec_hodge_star(n::Int, sd::EmbeddedDeltaDualComplex2D, h::DiscreteHodge, ::Val{:HIP}) =
+dec_hodge_star(n::Int, sd::DualComplex, h::DiscreteHodge, ::Val{:HIP}) =
   HIPOperator{float_of(sd)}(sd, DEC_OP_HODGE, n, 0, hodge_code(h))
 # Geometric ⋆1⁻¹ (CUDA ext :93-96 runs GMRES on -⋆1): the library's operator is
 # the solve itself (CG or BiCGSTAB on the device), so it is returned as a closure.
-function dec_inv_hodge_star(n::Int, sd::EmbeddedDeltaDualComplex2D, h::DiscreteHodge, ::Val{:HIP})
+function dec_inv_hodge_star(n::Int, sd::DualComplex, h::DiscreteHodge, ::Val{:HIP})
   op = HIPOperator{float_of(sd)}(sd, DEC_OP_INV_HODGE, n, 0, hodge_code(h))
   n == 1 && h isa GeometricHodge ? (x -> op * x) : op
 end
@@ -185,7 +224,7 @@ function ℒ_dd(::Val{1}, ::Val{1}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP})
   op = HIPOperator{float_of(sd)}(sd, DEC_OP_LIE_DD, 1, 1, DEC_GEOMETRIC_HODGE)
   (f, g) -> apply!(similar(f, op.rows), op, f, g)
 end
-function Δᵈ(::Val{n}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) where n
+function Δᵈ(::Val{n}, sd::DualComplex, ::Val{:HIP}) where n
   op = HIPOperator{float_of(sd)}(sd, DEC_OP_DUAL_LAPLACIAN, n, 0, DEC_GEOMETRIC_HODGE)
   x -> op * x
 end
