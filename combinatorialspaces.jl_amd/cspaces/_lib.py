@@ -16,7 +16,9 @@ DEC_CIRCUMCENTER, DEC_BARYCENTER = 0, 1
 DEC_GEOMETRIC_HODGE, DEC_DIAGONAL_HODGE = 0, 1
 (DEC_OP_WEDGE, DEC_OP_WEDGE_DD, DEC_OP_DIFFERENTIAL, DEC_OP_BOUNDARY, DEC_OP_DUAL_DERIVATIVE,
  DEC_OP_HODGE, DEC_OP_INV_HODGE, DEC_OP_LAPLACIAN0, DEC_OP_AVG01, DEC_OP_WEDGE_PD, DEC_OP_FLAT_SHARP,
- DEC_OP_WEDGE_DP, DEC_OP_INTERIOR_PRODUCT_DD, DEC_OP_LIE_DD, DEC_OP_DUAL_LAPLACIAN) = range(1, 16)
+ DEC_OP_WEDGE_DP, DEC_OP_INTERIOR_PRODUCT_DD, DEC_OP_LIE_DD, DEC_OP_DUAL_LAPLACIAN,
+ DEC_OP_P2_D2_INTERPOLATION, DEC_OP_P3_D3_INTERPOLATION, DEC_OP_D0_P0_INTERPOLATION,
+ DEC_OP_P0_D0_INTERPOLATION) = range(1, 20)
 DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI = 0, 1, 2
 DEC_MG_DIRECT, DEC_MG_GALERKIN = 0, 1
 DEC_SOLVE_PLAIN_F64, DEC_SOLVE_MIXED_IR = 0, 1

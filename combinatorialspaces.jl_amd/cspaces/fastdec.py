@@ -495,6 +495,34 @@ def dec_inv_hodge_star(n, sd: Mesh, hodge=GeometricHodge, backend=HIP, cast_floa
     return Operator(sd, L.DEC_OP_INV_HODGE, n, hodge=hodge, cast_float=cast_float)
 
 
+def p2_d2_interpolation(sd: Mesh, backend=HIP, cast_float=None) -> Operator:
+    """p2_d2_interpolation(sd) (DiscreteExteriorCalculus.jl:2016-2031): primal
+    2-forms to dual 2-forms, nv x nt."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_P2_D2_INTERPOLATION, cast_float=cast_float)
+
+
+def p3_d3_interpolation(sd: Mesh, backend=HIP, cast_float=None) -> Operator:
+    """p3_d3_interpolation(sd) (DiscreteExteriorCalculus.jl:2038-2053): primal
+    3-forms to dual 3-forms on a tetrahedral complex, nv x ntet."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_P3_D3_INTERPOLATION, cast_float=cast_float)
+
+
+def d0_p0_interpolation(sd: Mesh, hodge=GeometricHodge, backend=HIP, cast_float=None) -> Operator:
+    """d0_p0_interpolation(sd; hodge) (FastDEC.jl:915-926): dual 0-forms to
+    primal 0-forms, ⋆0⁻¹ p2_d2 ⋆2⁻¹ (⋆0⁻¹ p3_d3 ⋆3⁻¹ on tetrahedra)."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_D0_P0_INTERPOLATION, hodge=hodge, cast_float=cast_float)
+
+
+def p0_d0_interpolation(sd: Mesh, backend=HIP, cast_float=None) -> Operator:
+    """p0_d0_interpolation(sd) (FastDEC.jl:928-941): primal 0-forms to the
+    dual 0-forms at triangle centers, nt x nv."""
+    _check_backend(backend)
+    return Operator(sd, L.DEC_OP_P0_D0_INTERPOLATION, cast_float=cast_float)
+
+
 def avg01_mat(sd: Mesh, backend=HIP, cast_float=None) -> Operator:
     """avg₀₁_mat(sd) (FastDEC.jl:874-892)."""
     _check_backend(backend)

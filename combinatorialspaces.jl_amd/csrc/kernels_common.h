@@ -26,6 +26,8 @@ dec_mesh_s *subdivide(const dec_mesh_s &c, bool project);
 // part of its dual geometry with the 3-D diagonal Hodge stars
 void resolve_orientation3(dec_mesh_s &m);
 void dualize_tets(dec_mesh_s &m, int center);
+// the 24 dual volumes of each tetrahedron (DualTet 24x + q), host copy
+void tet_dual_volumes(dec_mesh_s &m, std::vector<double> &dvol);
 
 // Host CSR with float64 values, columns ascending within each row.
 struct HostCsr {

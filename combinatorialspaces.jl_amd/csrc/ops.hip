@@ -1592,6 +1592,106 @@ void pd01_matrix(const dec_mesh_s &m, dec::HostCsr &M) {
 // plane of the triangle (X has rank 2: its rows are tangent), which is the
 // reference's SVD pseudo-inverse on flat meshes and stays well defined where
 // rounding lifts the rows out of the plane on a curved surface.
+// p2_d2_interpolation (DiscreteExteriorCalculus.jl:2016-2031), nv x nt: for each
+// triangle t ascending and each of its six dual triangles k·nt + t in order,
+// mat[v, t] += dual_area / area with v the dual triangle's primal vertex
+// (slots v0, v0, v1, v1, v2, v2 of triangle_vertices).  transposed = 1 gives
+// p0_d0_interpolation (FastDEC.jl:928-941), nt x nv, the same sums by rows of t.
+void p2_d2_matrix(const dec_mesh_s &m, bool transposed, dec::HostCsr &M) {
+  const dec::Topo &t = m.h;
+  int nv = t.nv, nt = t.nt;
+  std::vector<double> da = m.da.download(), area = m.area.download();
+  const int sv[6] = {0, 0, 1, 1, 2, 2};
+  // per triangle: three (vertex, weight) pairs, each the sum of two dual triangles
+  std::vector<int32_t> tv(3 * static_cast<size_t>(nt));
+  std::vector<double> w(3 * static_cast<size_t>(nt));
+  for (int i = 0; i < nt; i++) {
+    int vs[3] = {t.v1[t.e1[i]], t.v0[t.e2[i]], t.v0[t.e1[i]]};
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 6; k++) acc[sv[k]] = acc[sv[k]] + da[static_cast<size_t>(k) * nt + i] / area[i];
+    for (int j = 0; j < 3; j++) {
+      tv[3 * static_cast<size_t>(i) + j] = vs[j];
+      w[3 * static_cast<size_t>(i) + j] = acc[j];
+    }
+  }
+  M = dec::HostCsr();
+  if (transposed) {
+    M.m = nt;
+    M.n = nv;
+    M.ptr.assign(nt + 1, 0);
+    for (int i = 0; i < nt; i++) {
+      std::pair<int, double> e[3];
+      for (int j = 0; j < 3; j++) e[j] = {tv[3 * static_cast<size_t>(i) + j], w[3 * static_cast<size_t>(i) + j]};
+      std::sort(e, e + 3, [](const auto &a, const auto &b) { return a.first < b.first; });
+      for (int j = 0; j < 3; j++) {
+        if (j && e[j].first == e[j - 1].first) fail(DEC_ERR_UNSUPPORTED, "interpolation: triangle with a repeated vertex");
+        M.col.push_back(e[j].first);
+        M.val.push_back(e[j].second);
+      }
+      M.ptr[i + 1] = static_cast<int32_t>(M.col.size());
+    }
+    return;
+  }
+  M.m = nv;
+  M.n = nt;
+  M.ptr.assign(nv + 1, 0);
+  for (size_t q = 0; q < tv.size(); q++) M.ptr[tv[q] + 1]++;
+  for (int v = 0; v < nv; v++) M.ptr[v + 1] += M.ptr[v];
+  M.col.assign(M.ptr[nv], 0);
+  M.val.assign(M.ptr[nv], 0.0);
+  std::vector<int32_t> pos(M.ptr.begin(), M.ptr.end() - 1);
+  for (int i = 0; i < nt; i++)  // ascending t => each row sorted
+    for (int j = 0; j < 3; j++) {
+      int v = tv[3 * static_cast<size_t>(i) + j];
+      M.col[pos[v]] = i;
+      M.val[pos[v]++] = w[3 * static_cast<size_t>(i) + j];
+    }
+}
+
+// p3_d3_interpolation (DiscreteExteriorCalculus.jl:2038-2053), nv x ntet: for
+// each tetrahedron and its 24 dual tetrahedra in order, mat[v, x] += dual_vol /
+// vol, v the dual tetrahedron's primal vertex (six consecutive per vertex).
+void p3_d3_matrix(dec_mesh_s &m, dec::HostCsr &M) {
+  const dec::Topo &t = m.h;
+  int nv = t.nv, n = m.ht.n;
+  std::vector<double> dvol, vol = m.vol.download();
+  dec::tet_dual_volumes(m, dvol);
+  M = dec::HostCsr();
+  M.m = nv;
+  M.n = n;
+  M.ptr.assign(nv + 1, 0);
+  std::vector<int32_t> tvs(4 * static_cast<size_t>(n));
+  for (int x = 0; x < n; x++) {
+    dec::tet_vertices(t, m.ht, x, &tvs[4 * static_cast<size_t>(x)]);
+    for (int i = 0; i < 4; i++) M.ptr[tvs[4 * static_cast<size_t>(x) + i] + 1]++;
+  }
+  for (int v = 0; v < nv; v++) M.ptr[v + 1] += M.ptr[v];
+  M.col.assign(M.ptr[nv], 0);
+  M.val.assign(M.ptr[nv], 0.0);
+  std::vector<int32_t> pos(M.ptr.begin(), M.ptr.end() - 1);
+  for (int x = 0; x < n; x++)  // ascending x => each row sorted
+    for (int i = 0; i < 4; i++) {
+      double acc = 0.0;
+      for (int q = 6 * i; q < 6 * i + 6; q++) acc = acc + dvol[24 * static_cast<size_t>(x) + q] / vol[x];
+      int v = tvs[4 * static_cast<size_t>(x) + i];
+      M.col[pos[v]] = x;
+      M.val[pos[v]++] = acc;
+    }
+}
+
+// SparseMatrixCSC(⋆0⁻¹) * P * SparseMatrixCSC(⋆k⁻¹): entry (v, c) = (ih0[v] P[v, c]) ihk[c]
+// (d0_p0_interpolation, FastDEC.jl:915-926).
+void scale_rows_cols(dec::HostCsr &P, const std::vector<double> &rs, const std::vector<double> &cs) {
+  for (int r = 0; r < P.m; r++)
+    for (int k = P.ptr[r]; k < P.ptr[r + 1]; k++) P.val[k] = (rs[r] * P.val[k]) * cs[P.col[k]];
+}
+
+std::vector<double> inv_diag(const DBuf<double> &h) {
+  std::vector<double> v = h.download();
+  for (auto &x : v) x = (1 / x) * 1.0;  // dec_inv_hodge_star with (-1)^(k(n-k)) = 1 for these k
+  return v;
+}
+
 void flat_sharp_matrix(const dec_mesh_s &m, dec::HostCsr &M) {
   const dec::Topo &t = m.h;
   int nv = t.nv, ne = t.ne, nt = t.nt;
@@ -1928,6 +2028,19 @@ std::unique_ptr<dec_op_s> create3(dec_mesh_s &m, int kind, int dm, int dn, int h
       auto e = ell_op(std::move(M), dtype);
       e->kind = kind;
       e->m = dm;
+      e->hodge = hodge;
+      return e;
+    }
+    case DEC_OP_P0_D0_INTERPOLATION:
+      return nullptr;  // the triangle skeleton's p0_d0 (FastDEC.jl:928 takes any HasDeltaSet2D)
+    case DEC_OP_P3_D3_INTERPOLATION:
+    case DEC_OP_D0_P0_INTERPOLATION: {
+      // p3_d3_interpolation; d0_p0_interpolation = ⋆0⁻¹ p3_d3 ⋆3⁻¹ (FastDEC.jl:924-926)
+      dec::HostCsr P;
+      p3_d3_matrix(m, P);
+      if (kind == DEC_OP_D0_P0_INTERPOLATION) scale_rows_cols(P, inv_diag(m.h0), inv_diag(m.h3));
+      auto e = ell_op(std::move(P), dtype);
+      e->kind = kind;
       e->hodge = hodge;
       return e;
     }
@@ -2276,6 +2389,21 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
       else dec::upload_ell(op->csr, op->ell32);
       break;
     }
+    case DEC_OP_P2_D2_INTERPOLATION:
+    case DEC_OP_P0_D0_INTERPOLATION:
+    case DEC_OP_D0_P0_INTERPOLATION: {
+      dec::HostCsr P;
+      p2_d2_matrix(m, kind == DEC_OP_P0_D0_INTERPOLATION, P);
+      if (kind == DEC_OP_D0_P0_INTERPOLATION) scale_rows_cols(P, inv_diag(m.h0), inv_diag(m.h2));
+      auto e = ell_op(std::move(P), dtype);
+      e->kind = kind;
+      e->hodge = hodge;
+      dec::check_launch("dec_op_create");
+      dec::dsync();
+      return e.release();
+    }
+    case DEC_OP_P3_D3_INTERPOLATION:
+      fail(DEC_ERR_UNSUPPORTED, "p3_d3_interpolation is defined on 3-D complexes");
     case DEC_OP_AVG01: {
       op->impl = I_AVG;
       op->rows = ne;

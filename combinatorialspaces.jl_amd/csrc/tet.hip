@@ -273,6 +273,20 @@ void dualize_tets(dec_mesh_s &m, int center) {
   dsync();
 }
 
+void tet_dual_volumes(dec_mesh_s &m, std::vector<double> &dvol) {
+  const size_t N = static_cast<size_t>(m.ht.n);
+  dvol.assign(24 * N, 0.0);
+  if (!N) return;
+  DBuf<double> v, w, a, l, dv;
+  v.alloc(N);
+  w.alloc(4 * N);
+  a.alloc(12 * N);
+  l.alloc(4 * N);
+  dv.alloc(24 * N);
+  launch_geometry(m, m.center == DEC_CIRCUMCENTER ? 1 : 0, v.p, w.p, a.p, l.p, dv.p);
+  d2h(dvol.data(), dv.p, sizeof(double) * 24 * N);
+}
+
 }  // namespace dec
 
 extern "C" {
@@ -383,15 +397,9 @@ int dec_mesh3_export_dual(dec_mesh m, double *dual_point, double *length, double
   if (dual_length) dec::d2h(dual_length, s.dl.p, sizeof(double) * s.dl.n);
   if (dual_area) dec::d2h(dual_area, s.da.p, sizeof(double) * s.da.n);
   if (dual_vol && s.ht.n) {
-    const size_t N = static_cast<size_t>(s.ht.n);
-    DBuf<double> v, w, a, l, dv;
-    v.alloc(N);
-    w.alloc(4 * N);
-    a.alloc(12 * N);
-    l.alloc(4 * N);
-    dv.alloc(24 * N);
-    launch_geometry(s, s.center == DEC_CIRCUMCENTER ? 1 : 0, v.p, w.p, a.p, l.p, dv.p);
-    dec::d2h(dual_vol, dv.p, sizeof(double) * 24 * N);
+    std::vector<double> dv;
+    dec::tet_dual_volumes(s, dv);
+    std::copy(dv.begin(), dv.end(), dual_vol);
   }
   DEC_CATCH
 }

@@ -24,6 +24,7 @@ import CombinatorialSpaces: cache_wedge, dec_wedge_product, dec_wedge_product_dd
   dec_wedge_product_pd, dec_wedge_product_dp, ♭♯_mat, interior_product_dd, ℒ_dd, Δᵈ,
   dec_boundary, dec_differential, dec_dual_derivative,
   dec_hodge_star, dec_inv_hodge_star, avg₀₁_mat,
+  d0_p0_interpolation, p0_d0_interpolation, p2_d2_interpolation, p3_d3_interpolation,
   multigrid_vcycles, multigrid_wcycles, full_multigrid
 
 const libdec_hip = get(ENV, "DEC_HIP_LIB", "libdec_hip")
@@ -35,7 +36,9 @@ const DEC_GEOMETRIC_HODGE, DEC_DIAGONAL_HODGE = Int32(0), Int32(1)
 const DEC_OP_WEDGE, DEC_OP_WEDGE_DD, DEC_OP_DIFFERENTIAL, DEC_OP_BOUNDARY,
       DEC_OP_DUAL_DERIVATIVE, DEC_OP_HODGE, DEC_OP_INV_HODGE, DEC_OP_LAPLACIAN0,
       DEC_OP_AVG01, DEC_OP_WEDGE_PD, DEC_OP_FLAT_SHARP, DEC_OP_WEDGE_DP,
-      DEC_OP_INTERIOR_PRODUCT_DD, DEC_OP_LIE_DD, DEC_OP_DUAL_LAPLACIAN = Int32.(1:15)
+      DEC_OP_INTERIOR_PRODUCT_DD, DEC_OP_LIE_DD, DEC_OP_DUAL_LAPLACIAN,
+      DEC_OP_P2_D2_INTERPOLATION, DEC_OP_P3_D3_INTERPOLATION, DEC_OP_D0_P0_INTERPOLATION,
+      DEC_OP_P0_D0_INTERPOLATION = Int32.(1:19)
 const DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI = Int32.(0:2)
 
 function check(status::Cint)
@@ -228,6 +231,17 @@ function Δᵈ(::Val{n}, sd::DualComplex, ::Val{:HIP}) where n
   op = HIPOperator{float_of(sd)}(sd, DEC_OP_DUAL_LAPLACIAN, n, 0, DEC_GEOMETRIC_HODGE)
   x -> op * x
 end
+
+# Interpolation matrices (FastDEC.jl:915-941, DiscreteExteriorCalculus.jl:2016-2053)
+# as device operators.
+p2_d2_interpolation(sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP}) =
+  HIPOperator{float_of(sd)}(sd, DEC_OP_P2_D2_INTERPOLATION, 0, 0, DEC_GEOMETRIC_HODGE)
+p3_d3_interpolation(sd::EmbeddedDeltaDualComplex3D, ::Val{:HIP}) =
+  HIPOperator{float_of(sd)}(sd, DEC_OP_P3_D3_INTERPOLATION, 0, 0, DEC_GEOMETRIC_HODGE)
+d0_p0_interpolation(sd::DualComplex, ::Val{:HIP}; hodge=GeometricHodge()) =
+  HIPOperator{float_of(sd)}(sd, DEC_OP_D0_P0_INTERPOLATION, 0, 0, hodge_code(hodge))
+p0_d0_interpolation(sd::DualComplex, ::Val{:HIP}) =
+  HIPOperator{float_of(sd)}(sd, DEC_OP_P0_D0_INTERPOLATION, 0, 0, DEC_GEOMETRIC_HODGE)
 
 # ---------------------------------------------------------------- multigrid
 

@@ -57,7 +57,11 @@ enum dec_op_kind {
   DEC_OP_WEDGE_DP = 12,       /* dual-primal wedge 1-0, 1-1 */
   DEC_OP_INTERIOR_PRODUCT_DD = 13, /* interior_product_dd (1,1), (1,2) */
   DEC_OP_LIE_DD = 14,         /* ℒ_dd (1,1) */
-  DEC_OP_DUAL_LAPLACIAN = 15  /* Δᵈ n = 0, 1 */
+  DEC_OP_DUAL_LAPLACIAN = 15, /* Δᵈ n = 0, 1 */
+  DEC_OP_P2_D2_INTERPOLATION = 16, /* primal 2-forms -> dual 2-forms, nv x nt */
+  DEC_OP_P3_D3_INTERPOLATION = 17, /* primal 3-forms -> dual 3-forms, nv x ntet */
+  DEC_OP_D0_P0_INTERPOLATION = 18, /* dual 0-forms -> primal 0-forms */
+  DEC_OP_P0_D0_INTERPOLATION = 19  /* primal 0-forms -> dual 0-forms, nt x nv */
 };
 enum dec_smoother { DEC_SMOOTH_CG = 0, DEC_SMOOTH_GS = 1, DEC_SMOOTH_JACOBI = 2 };
 enum dec_solve_variant { DEC_SOLVE_PLAIN_F64 = 0, DEC_SOLVE_MIXED_IR = 1 };
@@ -186,6 +190,10 @@ int dec_mesh3_export_dual(dec_mesh m, double *dual_point, double *length, double
  *     dual 1-form (FastDEC.jl:758-786).
  *  DEC_OP_LIE_DD (1,1): ℒ_dd, apply(f, g) dual 1-forms (FastDEC.jl:788-803).
  *  DEC_OP_DUAL_LAPLACIAN n=0,1: Δᵈ(Val(n)) on a 2-D complex (FastDEC.jl:808-860).
+ *  DEC_OP_P2_D2_INTERPOLATION: p2_d2_interpolation (DiscreteExteriorCalculus.jl:2016-2031).
+ *  DEC_OP_P0_D0_INTERPOLATION: p0_d0_interpolation (FastDEC.jl:928-941).
+ *  DEC_OP_D0_P0_INTERPOLATION: d0_p0_interpolation = ⋆0⁻¹ p2_d2 ⋆2⁻¹
+ *     (FastDEC.jl:915-917); hodge is accepted, both stars are diagonal.
  *  The last three and the 1-1 wedges are compositions of other operators
  *  (and of the geometric ⋆1⁻¹ solve); they own work vectors.
  * On a 3-D complex (FastDEC.jl methods for HasDeltaSet3D):
@@ -198,6 +206,9 @@ int dec_mesh3_export_dual(dec_mesh m, double *dual_point, double *length, double
  *  DEC_OP_INV_HODGE n=0..3 diagonal, and geometric for n=0,3 (FastDEC.jl:750-757).
  *  DEC_OP_LAPLACIAN0 with DEC_DIAGONAL_HODGE: -⋆0⁻¹ d̃2 ⋆1 d0.
  *  DEC_OP_DUAL_LAPLACIAN n=0: ⋆3 d2 ⋆2⁻¹ d̃0 as one matrix (FastDEC.jl:832-839).
+ *  DEC_OP_P3_D3_INTERPOLATION: p3_d3_interpolation (DiscreteExteriorCalculus.jl:2038-2053);
+ *  DEC_OP_D0_P0_INTERPOLATION: ⋆0⁻¹ p3_d3 ⋆3⁻¹ (FastDEC.jl:924-926);
+ *  DEC_OP_P0_D0_INTERPOLATION on the triangle skeleton as above.
  *  Other kinds return DEC_ERR_UNSUPPORTED.
  * deg_n is ignored for single-degree kinds. */
 int dec_op_create(dec_mesh m, int32_t kind, int32_t deg_m, int32_t deg_n,

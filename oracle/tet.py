@@ -25,6 +25,7 @@ own incidence definitions instead of a shortcut:
 * ∧ 2-1 / 1-2                                   DiscreteExteriorCalculus.jl:1337-1381
 * FastDEC's 0-3 and 2-1 coefficient tables      FastDEC.jl:64-76, 96-133, 173-202
 * ∇²₀ with the diagonal ⋆                       DiscreteExteriorCalculus.jl:1836-1890
+* p3_d3_interpolation, d0_p0_interpolation (3-D) DiscreteExteriorCalculus.jl:2038-2053, FastDEC.jl:924-926
 * Δᵈ(Val(0)) on a 3-D complex                   FastDEC.jl:832-839
 
 Circumcenters are taken, as in the reference, from the first row of the
@@ -616,6 +617,24 @@ class TetDual:
         return (c[0] * be[3] * at[3] - c[1] * be[4] * at[2] + c[2] * be[5] * at[1] + c[3] * be[1] * at[3]
                 - c[4] * be[2] * at[2] + c[5] * be[5] * at[0] + c[6] * be[0] * at[3] - c[7] * be[2] * at[1]
                 + c[8] * be[4] * at[0] + c[9] * be[0] * at[2] - c[10] * be[1] * at[1] + c[11] * be[3] * at[0])
+
+    # -- interpolation -------------------------------------------------------------
+    def p3_d3_interpolation(self):
+        """mat[v, x] += dual_vol / vol over x ascending and its 24 dual
+        tetrahedra, v = D_∂v1[D_∂e2[D_∂t1[dual tetrahedron]]]
+        (DiscreteExteriorCalculus.jl:2038-2053)."""
+        acc = {}
+        for x in range(self.ntet):
+            V = self.vol[x]
+            for dt in range(24 * x, 24 * x + 24):
+                v = int(self.D_v1[self.D_e2[self.D_t1[dt]]])
+                acc[(v, x)] = acc.get((v, x), 0.0) + self.dual_vol[dt] / V
+        keys = np.array(list(acc.keys()), np.int64).reshape(-1, 2)
+        return sp.csr_matrix((np.array(list(acc.values())), (keys[:, 0], keys[:, 1])), shape=(self.nv, self.ntet))
+
+    def d0_p0_interpolation(self):
+        """⋆0⁻¹ p3_d3 ⋆3⁻¹ (FastDEC.jl:924-926)."""
+        return sp.diags(self.inv_hodge_diag(0)) @ self.p3_d3_interpolation() @ sp.diags(self.inv_hodge_diag(3))
 
     # -- Laplacians ---------------------------------------------------------------
     def laplacian0(self):
