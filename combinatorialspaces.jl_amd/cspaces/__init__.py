@@ -10,7 +10,8 @@ from .fastdec import (  # noqa: F401
     interior_product_dd, kuhn_box, kuhn_box_arrays, lie_derivative_dd, mesh3_from_arrays, mesh_from_arrays,
     mesh_from_tetrahedra, mesh_from_triangles, p0_d0_interpolation, p2_d2_interpolation, p3_d3_interpolation,
     subdivision, triangulated_grid, ℒ_dd)
-from .meshio import load_mesh, read_mesh, read_obj, read_stl, save_obj, write_obj  # noqa: F401
+from .meshio import (  # noqa: F401
+    load_json_acset, load_mesh, read_json_acset, read_mesh, read_obj, read_stl, save_obj, write_json_acset, write_obj)
 from .multigrid import (  # noqa: F401
     BinarySubdivision, DirectMode, GalerkinMode, MGData, MultigridData, cg, dec_Delta_inv, full_multigrid,
     gauss_seidel, jacobi,

@@ -1,4 +1,5 @@
-"""Mesh files on either side of the DEC path: OBJ and STL in, OBJ out.
+"""Mesh files on either side of the DEC path: OBJ, STL and JSON ACSets in,
+OBJ and JSON ACSets out.
 
 The reference reads meshes with FileIO/MeshIO into a GeometryBasics.Mesh and
 glues its triangles (src/MeshInterop.jl:85-165); it writes them back through
@@ -15,11 +16,21 @@ those steps on numpy arrays:
 * the mesh is then glue_sorted_triangle! of each face in file order followed
   by orient! (dec_mesh_from_triangles).
 
+JSON ACSets (CombMeshes.loadmesh(Rectangle_30x10()) reads one with
+parse_json_acset, src/CombMeshes.jl:42-55) hold one array per part type (V, E,
+Tri, and Tet for a 3-D set), one object per part with its face maps (1-based)
+and attributes: `point` as a coordinate array, orientations as booleans or
+null (unset).  `_id` keys and dual parts (DualV, DualE, ... of a saved dual
+complex) are ignored: the library recomputes the dual.  No ACSet file of the
+reference is available offline, so the reader follows ACSets.jl's documented
+layout and is checked by round trips only.
+
 Parsing is host work and needs no GPU; `load_mesh` then builds the library's
 mesh handle.
 """
 from __future__ import annotations
 
+import json
 import os
 import struct
 
@@ -94,6 +105,85 @@ def read_stl(path, force_unique=True):
     corners = _stl_corners(path)
     ind_map, uniques = unique_index_map(corners, force_unique)
     return corners[uniques], ind_map.reshape(-1, 3).astype(np.int32)
+
+
+def _orient(rows, key):
+    out = np.full(len(rows), -1, np.int8)
+    for i, r in enumerate(rows):
+        o = r.get(key)
+        if o is not None:
+            out[i] = 1 if bool(o) else 0
+    return out
+
+
+def read_json_acset(path_or_text):
+    """The primal parts of a JSON ACSet (EmbeddedDeltaSet2D / 3D or a dual
+    complex of one): dict with points (nv x 3), 0-based ∂v0, ∂v1, ∂e0..∂e2,
+    ∂t0..∂t3 (3-D) and the orientations (1, 0, -1 unset)."""
+    text = path_or_text
+    if not text.lstrip().startswith("{"):
+        with open(path_or_text, "r", encoding="utf-8") as fh:
+            text = fh.read()
+    data = json.loads(text)
+    V, E, T = data.get("V", []), data.get("E", []), data.get("Tri", [])
+    pts = np.zeros((len(V), 3))
+    for i, r in enumerate(V):
+        p = r["point"]
+        if isinstance(p, dict):                      # {"x": .., "y": .., "z": ..} or {"data": [...]}
+            p = p.get("data", [p.get("x", 0.0), p.get("y", 0.0), p.get("z", 0.0)])
+        pts[i, :len(p)] = p[:3]
+    hom = lambda rows, key: np.array([int(r[key]) - 1 for r in rows], np.int32)  # noqa: E731
+    out = {"points": pts, "v0": hom(E, "∂v0"), "v1": hom(E, "∂v1"),
+           "e0": hom(T, "∂e0"), "e1": hom(T, "∂e1"), "e2": hom(T, "∂e2"),
+           "edge_orientation": _orient(E, "edge_orientation"), "tri_orientation": _orient(T, "tri_orientation")}
+    if "Tet" in data:
+        X = data["Tet"]
+        for k in range(4):
+            out[f"t{k}"] = hom(X, f"∂t{k}")
+        out["tet_orientation"] = _orient(X, "tet_orientation")
+    for key in ("v0", "v1"):
+        if out[key].size and (out[key].min() < 0 or out[key].max() >= len(pts)):
+            raise ValueError("JSON ACSet: an edge refers to a vertex that does not exist")
+    return out
+
+
+def write_json_acset(path, arrays):
+    """A JSON ACSet of the primal parts in `arrays` (the layout read_json_acset
+    returns, or Mesh.export()); 1-based face maps, `_id` on every part."""
+    def orient(a, i):
+        o = int(a[i])
+        return None if o < 0 else bool(o)
+
+    P = np.asarray(arrays["points"], np.float64).reshape(-1, 3)
+    eo = arrays.get("edge_orientation", np.full(len(arrays["v0"]), -1))
+    to = arrays.get("tri_orientation", np.full(len(arrays["e0"]), -1))
+    data = {
+        "V": [{"_id": i + 1, "point": p} for i, p in enumerate(P.tolist())],
+        "E": [{"_id": i + 1, "∂v0": int(a) + 1, "∂v1": int(b) + 1, "edge_orientation": orient(eo, i)}
+              for i, (a, b) in enumerate(zip(arrays["v0"], arrays["v1"]))],
+        "Tri": [{"_id": i + 1, "∂e0": int(a) + 1, "∂e1": int(b) + 1, "∂e2": int(c) + 1,
+                 "tri_orientation": orient(to, i)}
+                for i, (a, b, c) in enumerate(zip(arrays["e0"], arrays["e1"], arrays["e2"]))],
+    }
+    if "t0" in arrays:
+        teo = arrays.get("tet_orientation", np.full(len(arrays["t0"]), -1))
+        data["Tet"] = [{"_id": i + 1, **{f"∂t{k}": int(arrays[f"t{k}"][i]) + 1 for k in range(4)},
+                        "tet_orientation": orient(teo, i)} for i in range(len(arrays["t0"]))]
+    with open(path, "w", encoding="utf-8") as fh:
+        json.dump(data, fh, ensure_ascii=False)
+
+
+def load_json_acset(path_or_text):
+    """parse_json_acset(EmbeddedDeltaSet2D/3D, text): the library mesh of a JSON
+    ACSet, with its boundary maps and orientations as stored."""
+    from .fastdec import mesh3_from_arrays, mesh_from_arrays
+
+    a = read_json_acset(path_or_text)
+    if "t0" in a:
+        return mesh3_from_arrays(a["points"], a["v0"], a["v1"], a["e0"], a["e1"], a["e2"], a["t0"], a["t1"],
+                                 a["t2"], a["t3"], a["edge_orientation"], a["tri_orientation"], a["tet_orientation"])
+    return mesh_from_arrays(a["points"], a["v0"], a["v1"], a["e0"], a["e1"], a["e2"], a["edge_orientation"],
+                            a["tri_orientation"])
 
 
 def read_mesh(path, force_unique=None):

@@ -105,3 +105,56 @@ def test_load_mesh_matches_the_oracle_glue(O, cs, tmp_path):
     meshio.save_obj(str(g), gm)
     P3, T3 = meshio.read_obj(str(g))
     assert np.array_equal(T3, om.triangle_vertices()) and np.array_equal(P3, P2)
+
+
+def test_json_acset_round_trip_2d_and_3d(O, tmp_path):
+    """JSON ACSets in ACSets.jl's layout: parts as arrays of objects, 1-based
+    face maps, null for an unset orientation; `_id` and dual parts ignored."""
+    import json
+
+    import tet
+    from cspaces import meshio
+
+    m = O.icosphere(1)
+    a = {"points": m.p, "v0": m.v0, "v1": m.v1, "e0": m.e0, "e1": m.e1, "e2": m.e2,
+         "edge_orientation": np.where(np.arange(m.ne) % 3 == 0, -1, 1).astype(np.int8),
+         "tri_orientation": np.full(m.nt, -1, np.int8)}
+    f = tmp_path / "ico1.json"
+    meshio.write_json_acset(str(f), a)
+    b = meshio.read_json_acset(str(f))
+    for k, v in a.items():
+        assert np.array_equal(np.asarray(b[k]), np.asarray(v)), k
+    raw = json.loads(f.read_text())
+    assert raw["E"][0]["edge_orientation"] is None and raw["E"][1]["edge_orientation"] is True
+    assert raw["Tri"][0]["∂e0"] == int(m.e0[0]) + 1
+    # a saved dual complex carries dual parts as well; the reader keeps the primal ones
+    raw["DualV"] = [{"_id": 1, "dual_point": [0, 0, 0]}]
+    c = meshio.read_json_acset(json.dumps(raw))
+    assert np.array_equal(c["e2"], m.e2)
+    tm = tet.kuhn_grid(2, 1, 1)
+    a3 = {"points": tm.p, "v0": tm.v0, "v1": tm.v1, "e0": tm.e0, "e1": tm.e1, "e2": tm.e2,
+          "t0": tm.t0, "t1": tm.t1, "t2": tm.t2, "t3": tm.t3, "tet_orientation": tm.teo}
+    g = tmp_path / "kuhn.json"
+    meshio.write_json_acset(str(g), a3)
+    b3 = meshio.read_json_acset(str(g))
+    for k, v in a3.items():
+        assert np.array_equal(np.asarray(b3[k]), np.asarray(v)), k
+    assert (b3["edge_orientation"] == -1).all()
+    with pytest.raises(ValueError):
+        raw["E"][0]["∂v0"] = 10_000
+        meshio.read_json_acset(json.dumps(raw))
+
+
+@pytest.mark.gpu
+def test_load_json_acset_builds_the_same_mesh(O, cs, tmp_path):
+    from cspaces import meshio
+
+    m = O.icosphere(2)
+    f = tmp_path / "ico2.json"
+    meshio.write_json_acset(str(f), {"points": m.p, "v0": m.v0, "v1": m.v1, "e0": m.e0, "e1": m.e1, "e2": m.e2})
+    gm = cs.load_json_acset(str(f))
+    cs.dualize(gm)
+    a = gm.export()
+    for k in ("v0", "v1", "e0", "e1", "e2"):
+        assert np.array_equal(a[k], getattr(m, k)), k
+    assert np.array_equal(a["points"], m.p)
