@@ -4,7 +4,8 @@ small meshes for seeded forms, stored as .npz (no pickles).
     python tests/golden/make_fixtures.py
 
 Each file holds the mesh (topology, orientations, points), the dual geometry
-(lengths, areas), the seeded input forms and every operator's output.  The
+(lengths, areas; volumes for the tetrahedral file tet_*.npz), the seeded
+input forms and every operator's output.  The
 CPU tests check that the oracle still reproduces them and the GPU tests
 compare the HIP library against them.
 """
@@ -18,8 +19,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
+import interp as IP  # noqa: E402
 import musical as MU  # noqa: E402
 import oracle as O  # noqa: E402
+import tet as TT  # noqa: E402
 
 
 def outputs(mesh, circumcenter=True):
@@ -56,6 +59,35 @@ def outputs(mesh, circumcenter=True):
     return {k: np.ascontiguousarray(v) for k, v in out.items()}
 
 
+def _mv(A, x):
+    """A x summed over each row's columns in ascending order (the library's order)."""
+    A = A.tocsr()
+    A.sort_indices()
+    return A @ x
+
+
+def outputs3(mesh, circumcenter=False):
+    """The 3-D operators of a tetrahedral complex (oracle/tet.py) on seeded forms."""
+    d = TT.TetDual(mesh, circumcenter_=circumcenter)
+    m = d.m
+    V, E, T, X = (O.uniform_form(s, n) for s, n in ((11, d.nv), (12, d.ne), (13, d.nt), (14, d.ntet)))
+    out = {
+        "v0": m.v0, "v1": m.v1, "e0": m.e0, "e1": m.e1, "e2": m.e2, "t0": m.t0, "t1": m.t1, "t2": m.t2,
+        "t3": m.t3, "edge_orient": m.eo, "tri_orient": m.to, "tet_orient": m.teo, "points": m.p,
+        "vol": d.vol, "dual_vol": d.dual_vol, "V": V, "E": E, "T": T, "X": X,
+        "wedge01": d.wedge0k(1, V, E), "wedge02": d.wedge0k(2, V, T), "wedge03": d.wedge0k(3, V, X),
+        "wedge11": d.wedge11(E, E), "wedge21": d.wedge21(T, E),
+        "d0": _mv(d.differential(0), V), "d1": _mv(d.differential(1), E), "d2": _mv(d.differential(2), T),
+        "boundary3": _mv(d.boundary(3), X), "dual_d0": _mv(d.dual_derivative(0), X),
+        "dual_d1": _mv(d.dual_derivative(1), T), "dual_d2": _mv(d.dual_derivative(2), E),
+        "hodge0": d.hodge_diag(0) * V, "hodge1": d.hodge_diag(1) * E, "hodge2": d.hodge_diag(2) * T,
+        "hodge3": d.hodge_diag(3) * X, "laplacian0": d.laplacian0() @ V, "dual_laplacian0": d.dual_laplacian0() @ X,
+        "p3_d3": d.p3_d3_interpolation() @ X, "d0_p0": d.d0_p0_interpolation() @ X,
+        "p0_d0": IP.p0_d0_interpolation(d) @ V,
+    }
+    return {k: np.ascontiguousarray(v) for k, v in out.items()}
+
+
 # outputs computed through LAPACK / SuperLU (pinv, splu): compared to 1e-13
 # relative rather than bit for bit when the oracle is re-run
 LAPACK_KEYS = {"flat_sharp", "wedge_pd11", "wedge_dp11", "inv_hodge1_geometric", "interior11", "interior12", "lie",
@@ -66,6 +98,8 @@ def main():
     np.savez(os.path.join(HERE, "icosphere2_circumcenter.npz"), **outputs(O.icosphere(2)))
     grid = O.triangulated_grid(1, 1, 1 / 4, math.sqrt(3) / 2 * 1 / 4, compress=False)
     np.savez(os.path.join(HERE, "grid4_barycenter.npz"), **outputs(grid, circumcenter=False))
+    km = TT.kuhn_grid(2, 2, 1, lx=2.0, ly=2.0, lz=1.0, jitter=0.2, seed=4)
+    np.savez(os.path.join(HERE, "tet_kuhn221_barycenter.npz"), **outputs3(km, circumcenter=False))
     print("fixtures written to", HERE)
 
 
