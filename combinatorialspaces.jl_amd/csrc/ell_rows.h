@@ -84,6 +84,34 @@ __device__ __forceinline__ T row(int n, int width, int r, const int *__restrict_
   return acc;
 }
 
+// A row of at most WMAX slots (width ≤ WMAX, not a compile-time constant):
+// every slot's loads are issued before the sum, as for an exact W, and the
+// sum runs in slot order over the row's columns, so the result equals row<0>'s.
+template <int WMAX, class T, class V>
+__device__ __forceinline__ T row_upto(int n, int width, int r, const int *__restrict__ col,
+                                      const V *__restrict__ val, const T *__restrict__ x) {
+  int c[WMAX];
+  V v[WMAX];
+  T xv[WMAX];
+#pragma unroll
+  for (int k = 0; k < WMAX; k++) {
+    c[k] = -1;
+    v[k] = V(0);
+    if (k < width) {
+      size_t q = static_cast<size_t>(k) * n + r;
+      c[k] = col[q];
+      v[k] = val[q];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < WMAX; k++) xv[k] = c[k] >= 0 ? x[c[k]] : static_cast<T>(0);
+  T acc = static_cast<T>(0);
+#pragma unroll
+  for (int k = 0; k < WMAX; k++)
+    if (c[k] >= 0) acc = acc + mul<V, T>(v[k], xv[k]);
+  return acc;
+}
+
 // Gauss-Seidel row: s = Σ_{c ≠ self} a x[c] in ascending order, d = diagonal.
 template <int W, class T>
 __device__ __forceinline__ void gs_row(int n, int width, int p, int self, const int *__restrict__ col,
