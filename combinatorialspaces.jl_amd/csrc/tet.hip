@@ -242,15 +242,16 @@ void dualize_tets(dec_mesh_s &m, int center) {
     int v[4], e[6];
     tet_vertices(t, T, x, v);
     tet_edges(t, T, x, e);
+    const size_t X = static_cast<size_t>(x);
     for (int i = 0; i < 4; i++) {
-      kv[4 * x + i] = v[i];
-      sv[4 * x + i] = i;
-      kt[4 * x + i] = T.t[i][x];
-      st[4 * x + i] = i;
+      kv[4 * X + i] = v[i];
+      sv[4 * X + i] = i;
+      kt[4 * X + i] = T.t[i][x];
+      st[4 * X + i] = i;
     }
     for (int j = 0; j < 6; j++) {
-      ke[6 * x + j] = e[j];
-      se[6 * x + j] = j;
+      ke[6 * X + j] = e[j];
+      se[6 * X + j] = j;
     }
   }
   Lists Lv, Le, Lt;
