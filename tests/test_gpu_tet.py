@@ -23,7 +23,7 @@ from test_gpu_ops import F32_TOL, F64_TOL, rel, sparse_close, to_dev
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["single", "blessent_circ", "blessent_bary", "kuhn_bary", "kuhn_circ", "kuhn_jitter_circ"]
+CASES = ["single", "blessent_circ", "blessent_bary", "kuhn_bary", "kuhn_circ", "kuhn_jitter_circ", "delaunay_bary"]
 DEGENERATE = {"blessent_circ"}
 SQRT_TOL = 1e-7
 
@@ -43,7 +43,19 @@ def oracle_mesh(T, name):
         return T.kuhn_grid(3, 3, 2, jitter=0.2, seed=7), False
     if name == "kuhn_circ":
         return T.kuhn_grid(3, 2, 2, lx=3.0, ly=2.0, lz=1.5, shear=(0.5, 0.25)), True
-    return T.kuhn_grid(3, 3, 3, jitter=0.15, seed=2, jitter_boundary=True), True
+    if name == "kuhn_jitter_circ":
+        return T.kuhn_grid(3, 3, 3, jitter=0.15, seed=2, jitter_boundary=True), True
+    # unstructured: scipy's Delaunay tetrahedralization of a jittered 5 x 5 x 5 lattice
+    # (682 tetrahedra of varying shape and vertex degree), glued and oriented as
+    # EmbeddedDeltaSet3D(mesh) does
+    from scipy.spatial import Delaunay
+
+    rng = np.random.default_rng(3)
+    g = np.stack(np.meshgrid(*[np.arange(5.0)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    p = g + (rng.random(g.shape) - 0.5) * 0.5
+    m = T.glue_tetrahedra(p, Delaunay(p).simplices)
+    assert T.orient3(m)
+    return m, False
 
 
 def gpu_mesh3_like(cs, m):
