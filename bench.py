@@ -302,6 +302,76 @@ def poisson(torch, cs, base_k=3, levels=5):
     return out, md, fd
 
 
+def mg_reference_calls(torch, cs, fd, base_k=3, levels=5, reps=3):
+    """The reference's own multigrid calls on the level 3->8 hierarchy, with its
+    default Krylov CG smoother (3 steps): multigrid_vcycles / wcycles (5
+    cycles, Multigrid.jl:814-817), full_multigrid (5 cycles, μ = 1 and the μ = 2
+    of dec_Δ⁻¹, Multigrid.jl:830-837, FastDEC.jl:866-869), and V-cycles on a
+    Galerkin hierarchy (Aₖ₊₁ = Rₖ Aₖ Pₖ).  Wall-clock ms per call (best of
+    `reps`, after a warm-up call) and the FP64 relative residual reached.  The
+    hierarchies are built here (DirectMode and GalerkinMode, 3 smoothing
+    steps), so no other leg's smoother settings carry over; fd is the
+    right-hand side of the Poisson leg on the same finest mesh."""
+    md = cs.MultigridData(cs.icosphere(base_k), cs.BinarySubdivision, levels, cs.DiagonalHodge, 3, project=True)
+    fine = md.finest_mesh()
+    L = cs.fast_laplace_beltrami(fine, cs.DiagonalHodge)
+    bn = torch.linalg.norm(fd).item()
+
+    def relres(u):
+        return torch.linalg.norm(fd - L @ u).item() / bn
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        best, u = None, None
+        for _ in range(reps):
+            t = time.perf_counter()
+            u = fn()
+            torch.cuda.synchronize()
+            e = time.perf_counter() - t
+            best = e if best is None else min(best, e)
+        return {"ms": best * 1e3, "relres_f64": relres(u)}
+
+    zero = lambda: torch.zeros_like(fd)  # noqa: E731
+    out = {
+        "vcycles_cg_5": timed(lambda: cs.multigrid_vcycles(zero(), fd, md, 5, cs.cg, steps=3)),
+        "wcycles_cg_5": timed(lambda: cs.multigrid_wcycles(zero(), fd, md, 5, cs.cg, steps=3)),
+        "full_multigrid_cg_5": timed(lambda: cs.full_multigrid(fd, md, 5, cs.cg, 1, steps=3)),
+        "dec_Delta_inv_fmg_mu2_5": timed(lambda: cs.full_multigrid(fd, md, 5, cs.cg, 2, steps=3)),
+    }
+    mg = cs.MultigridData(cs.icosphere(base_k), cs.BinarySubdivision, levels, cs.DiagonalHodge, 3, project=True,
+                          mode=cs.GalerkinMode)
+    out["galerkin_vcycles_cg_5"] = timed(lambda: cs.multigrid_vcycles(zero(), fd, mg, 5, cs.cg, steps=3))
+    del mg, md
+    return out
+
+
+def interpolation_ops(torch, cs, mesh, reps=20):
+    """The interpolation matrices on the headline mesh, FP64: µs per
+    application and GB/s of their compulsory bytes (nnz int32 columns and
+    float64 values, input and output once)."""
+    td = torch.float64
+    res = {}
+    for name, op in (("p2_d2", cs.p2_d2_interpolation(mesh, cast_float=td)),
+                     ("d0_p0", cs.d0_p0_interpolation(mesh, cast_float=td)),
+                     ("p0_d0", cs.p0_d0_interpolation(mesh, cast_float=td))):
+        nnz = len(op.csr()[1])
+        x = cs.fill_uniform(torch.empty(op.cols, dtype=td, device="cuda"), 601)
+        out = torch.empty(op.rows, dtype=td, device="cuda")
+        op.apply(x, out=out)
+        stream = torch.cuda.current_stream()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            op.apply(x, out=out, stream=stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        byt = nnz * 12 + (op.rows + op.cols) * 8
+        res[name] = {"us": ms * 1e3, "GBps": byt / (ms * 1e-3) / 1e9, "nnz": nnz}
+    return res
+
+
 def learned(torch, cs, md, fd, npairs=10000, train_levels=3, steps=2000, batch=16):
     """The learned smoother in the level 3->8 mixed solve.  Training pairs come
     from plain V-cycles on seeded spherical-harmonic right-hand sides on the
@@ -609,7 +679,9 @@ def main():
     print(f"# icosphere k={args.k}: {nv} V {ne} E {nt} T, mesh+dual setup {setup_mesh:.2f} s", file=sys.stderr,
           flush=True)
     if args.composite_only:
-        print(json.dumps({"composite_ops_f64": musical_ops(torch, cs, mesh), "mesh_vertices": nv}), flush=True)
+        print(json.dumps({"composite_ops_f64": musical_ops(torch, cs, mesh),
+                          "interpolation_ops_f64": interpolation_ops(torch, cs, mesh), "mesh_vertices": nv}),
+              flush=True)
         return
 
     per, total, grouped, cache, lap_nnz = time_opset(torch, cs, mesh, OPSET, torch.float64, args.sets, args.steps,
@@ -651,6 +723,8 @@ def main():
                                   for n in extra_names}
         extra["composite_ops_f64"] = musical_ops(torch, cs, mesh)
         print(f"# composite ops: {json.dumps(extra['composite_ops_f64'])}", file=sys.stderr, flush=True)
+        extra["interpolation_ops_f64"] = interpolation_ops(torch, cs, mesh)
+        print(f"# interpolation ops: {json.dumps(extra['interpolation_ops_f64'])}", file=sys.stderr, flush=True)
     if not args.no_f32:
         per32, total32, grouped32, cache32, lap32 = time_opset(torch, cs, mesh, OPSET, torch.float32, args.sets,
                                                                args.steps, args.warmup, False)
@@ -674,6 +748,9 @@ def main():
         if not args.no_learned:
             extra["learned"] = learned(torch, cs, md, fd, npairs=args.learned_pairs, steps=args.learned_steps)
             print(f"# learned: {json.dumps(extra['learned'])}", file=sys.stderr, flush=True)
+        extra["multigrid_reference_calls"] = mg_reference_calls(torch, cs, fd)
+        print(f"# multigrid reference calls: {json.dumps(extra['multigrid_reference_calls'])}", file=sys.stderr,
+              flush=True)
         del md, fd
     if not args.no_batch:
         extra["batched_heat"] = batched_heat(torch, cs)
