@@ -189,73 +189,98 @@ __global__ void k_sum_partials(const double *__restrict__ part, double *__restri
 }
 
 // Krylov.jl cg recurrence with device-resident scalars (warm start,
-// atol = rtol = sqrt(eps), zero-curvature exit).
+// atol = rtol = sqrt(eps), zero-curvature exit).  γ and ‖p‖² are kept in two
+// slots: iteration `it` reads slot it & 1 and writes the other, so that the
+// blocks of one launch never see a value written in that launch.
 struct CgState {
-  double gamma, rnorm, eps_stop, pnorm2, dot, alpha, beta;
+  double gamma[2], pnorm2[2], rnorm, eps_stop, alpha, beta;
   int done, iter;
 };
 
-__global__ void k_cg_start(CgState *st, int n, double *__restrict__ x, const double *__restrict__ r,
-                           double *__restrict__ p) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
+// The scalar steps run inside the vector kernels: every block reduces the
+// kDotBlocks partial sums of the previous launch itself with block_sum's tree,
+// so each holds the value a separate k_sum_partials launch would give, and
+// block 0 alone writes the scalars back.  The partial sums keep k_dot_partial's
+// partition (kDotBlocks blocks, grid-stride over the form index) and every
+// element update is the unfused one, so the iterates are those of the
+// sequence dot → scalar kernel → update, bit for bit, in fewer launches.
+
+// x = 0, p = r, and the partial sums of r·r.
+__global__ void k_cgf_start(int n, double *__restrict__ x, const double *__restrict__ r, double *__restrict__ p,
+                            double *__restrict__ part) {
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kDotBlocks * kBlock) {
+    double ri = r[i];
     x[i] = 0.0;
-    p[i] = r[i];
+    p[i] = ri;
+    acc = acc + ri * ri;
   }
+  double sum = block_sum<double>(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = sum;
 }
 
-__global__ void k_cg_init_scalars(CgState *st) {
-  double g = st->dot;
-  st->gamma = g;
+// γ = r·r, ‖r‖, the stopping threshold, and the immediate exit.
+__global__ void k_cgf_init(CgState *st, const double *__restrict__ part) {
+  double g = block_sum<double>(threadIdx.x < kDotBlocks ? part[threadIdx.x] : 0.0);
+  if (threadIdx.x != 0) return;
+  st->gamma[0] = g;
   st->rnorm = sqrt(g);
   const double sq = 1.4901161193847656e-08;
   st->eps_stop = sq + sq * st->rnorm;
-  st->pnorm2 = g;
+  st->pnorm2[0] = g;
   st->iter = 0;
   st->done = (g == 0.0) || (st->rnorm <= st->eps_stop);
 }
 
-__global__ void k_cg_alpha(CgState *st) {
+// α = γ / p·Ap (zero-curvature exit), x += α p, r -= α Ap, partials of r·r.
+__global__ void k_cgf_xr(CgState *st, int c, int n, double *__restrict__ x, const double *__restrict__ p,
+                         double *__restrict__ r, const double *__restrict__ Ap, const double *__restrict__ part,
+                         double *__restrict__ part2) {
   if (st->done) return;
   const double eps = 2.220446049250313e-16;
-  double pAp = st->dot;
-  if (pAp <= eps * st->pnorm2 && fabs(pAp) <= eps * st->pnorm2) {
-    st->done = 1;
+  double pAp = block_sum<double>(threadIdx.x < kDotBlocks ? part[threadIdx.x] : 0.0);
+  double pn2 = st->pnorm2[c];
+  if (pAp <= eps * pn2 && fabs(pAp) <= eps * pn2) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
     return;
   }
-  st->alpha = st->gamma / pAp;
+  double a = st->gamma[c] / pAp;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = a;
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kDotBlocks * kBlock) {
+    x[i] = x[i] + a * p[i];
+    double ri = r[i] - a * Ap[i];
+    r[i] = ri;
+    acc = acc + ri * ri;
+  }
+  double sum = block_sum<double>(acc);
+  if (threadIdx.x == 0) part2[blockIdx.x] = sum;
 }
 
-__global__ void k_cg_xr(CgState *st, int n, double *__restrict__ x, const double *__restrict__ p,
-                        double *__restrict__ r, const double *__restrict__ Ap) {
+// ‖r‖ and the solved exit, β = γ_new / γ, ‖p‖² and γ to the other slot, p = r + β p.
+__global__ void k_cgf_p(CgState *st, int c, int n, double *__restrict__ p, const double *__restrict__ r,
+                        const double *__restrict__ part2) {
   if (st->done) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double a = st->alpha;
-  x[i] = x[i] + a * p[i];
-  r[i] = r[i] - a * Ap[i];
-}
-
-__global__ void k_cg_beta(CgState *st) {
-  if (st->done) return;
-  double gn = st->dot;
+  double gn = block_sum<double>(threadIdx.x < kDotBlocks ? part2[threadIdx.x] : 0.0);
   double rn = sqrt(gn);
-  st->rnorm = rn;
-  st->iter += 1;
+  bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (rn <= st->eps_stop || rn + 1.0 <= 1.0) {
-    st->done = 2;  // solved: p is left as is
+    if (lead) {
+      st->rnorm = rn;
+      st->iter += 1;
+      st->done = 2;  // solved: p is left as is
+    }
     return;
   }
-  double beta = gn / st->gamma;
-  st->beta = beta;
-  st->pnorm2 = gn + beta * beta * st->pnorm2;
-  st->gamma = gn;
-}
-
-__global__ void k_cg_p(CgState *st, int n, double *__restrict__ p, const double *__restrict__ r) {
-  if (st->done) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = r[i] + st->beta * p[i];
+  double beta = gn / st->gamma[c];
+  if (lead) {
+    st->rnorm = rn;
+    st->iter += 1;
+    st->beta = beta;
+    st->pnorm2[c ^ 1] = gn + beta * beta * st->pnorm2[c];
+    st->gamma[c ^ 1] = gn;
+  }
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kDotBlocks * kBlock) p[i] = r[i] + beta * p[i];
 }
 
 __global__ void k_axpy_final(int n, const double *__restrict__ x, double *__restrict__ u) {
@@ -436,7 +461,7 @@ struct dec_mg_s {
   std::vector<std::unique_ptr<Level>> lv;            // finest first
   int alg = DEC_SMOOTH_CG, steps = 3, coarse_direct = 0;
   double omega = 2.0 / 3.0;
-  DBuf<double> part, scal;  // reduction partials, scalar outputs
+  DBuf<double> part, part2, scal;  // reduction partials, scalar outputs
   DBuf<CgState> cg;
   DBuf<double> cgx, cgp, cgAp;  // CG vectors (finest size)
   int fuse_from = -1;           // first level handled by the fused coarse kernel
@@ -775,6 +800,7 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
     mg->desc32.upload(d32);
   }
   mg->part.alloc(kDotBlocks);
+  mg->part2.alloc(kDotBlocks);
   mg->scal.alloc(8);
   mg->cg.alloc(1);
   int nf = mg->lv[0]->n;
@@ -801,26 +827,24 @@ void dot(dec_mg_s &mg, int n, const double *x, const double *y, double *out, con
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, mg.part.p, out, done);
 }
 
-// cg(A, b, u; itmax = steps) as in Multigrid.jl:860 (float64 only).
+// cg(A, b, u; itmax = steps) as in Multigrid.jl:860 (float64 only).  Four
+// launches per iteration: the product, the partial sums of p·Ap, then the
+// α and β steps each with its update and the next partial sums.
 void cg_smooth(dec_mg_s &mg, Level &L, const double *b, double *u, int steps, hipStream_t s) {
   int n = L.n;
   CgState *st = mg.cg.p;
   int *done = reinterpret_cast<int *>(reinterpret_cast<char *>(st) + offsetof(CgState, done));
-  double *dotp = reinterpret_cast<double *>(reinterpret_cast<char *>(st) + offsetof(CgState, dot));
   double *x = mg.cgx.p, *p = mg.cgp.p, *Ap = mg.cgAp.p, *r = L.d.t.p;
   residual<double>(L, b, u, r, s);
-  hipLaunchKernelGGL(k_cg_start, G(n), dim3(kBlock), 0, s, st, n, x, r, p);
-  dot(mg, n, r, r, dotp, nullptr, s);
-  hipLaunchKernelGGL(k_cg_init_scalars, dim3(1), dim3(1), 0, s, st);
+  hipLaunchKernelGGL(k_cgf_start, dim3(kDotBlocks), dim3(kBlock), 0, s, n, x, r, p, mg.part.p);
+  hipLaunchKernelGGL(k_cgf_init, dim3(1), dim3(kBlock), 0, s, st, mg.part.p);
   for (int it = 0; it < steps; it++) {
     DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_spmv_rows<W, double>), G(n), dim3(kBlock), 0, s, n, L.width,
                                                      L.col.p, L.d.val.p, L.rowid.p, p, Ap, done));
-    dot(mg, n, p, Ap, dotp, done, s);
-    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1), 0, s, st);
-    hipLaunchKernelGGL(k_cg_xr, G(n), dim3(kBlock), 0, s, st, n, x, p, r, Ap);
-    dot(mg, n, r, r, dotp, done, s);
-    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1), 0, s, st);
-    hipLaunchKernelGGL(k_cg_p, G(n), dim3(kBlock), 0, s, st, n, p, r);
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(kDotBlocks), dim3(kBlock), 0, s, n, p, Ap, mg.part.p, done);
+    hipLaunchKernelGGL(k_cgf_xr, dim3(kDotBlocks), dim3(kBlock), 0, s, st, it & 1, n, x, p, r, Ap, mg.part.p,
+                       mg.part2.p);
+    hipLaunchKernelGGL(k_cgf_p, dim3(kDotBlocks), dim3(kBlock), 0, s, st, it & 1, n, p, r, mg.part2.p);
   }
   hipLaunchKernelGGL(k_axpy_final, G(n), dim3(kBlock), 0, s, n, x, u);
 }
