@@ -658,8 +658,8 @@ def main():
         traffic_per_op(args.k, args.sets)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the operators would shard).", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path; run it with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
