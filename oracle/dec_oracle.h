@@ -1,7 +1,7 @@
 /*
  * dec_oracle.h -- CPU restatement of CombinatorialSpaces.jl's DEC hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP library:
+ * TEST INFRASTRUCTURE ONLY.  The HIP library is tested against this oracle:
  * only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may load
  * it.  The product path (libdec_hip.so) never links or calls it.
  *

@@ -1,7 +1,7 @@
 """Interpolation matrices between primal and dual 0-forms / top forms of a
 2-D complex, restated in numpy over an oracle.Dual (0-based indices).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
+TEST INFRASTRUCTURE ONLY: what the HIP library is tested against.  Only tests/,
 __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module.
 
 * p2_d2_interpolation   src/DiscreteExteriorCalculus.jl:2016-2031

@@ -1,8 +1,8 @@
 """CPU oracle of the learned multigrid smoother (float64 numpy).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP network kernels.
+TEST INFRASTRUCTURE ONLY: what the HIP network kernels are tested against.
 
-The network has no reference counterpart; it is this build's smoother.  For a
+The network has no reference counterpart; it is the library's own smoother.  For a
 residual r on a level with operator A (CSR rows, columns ascending) it returns
 a correction c:
 

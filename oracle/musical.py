@@ -1,6 +1,6 @@
 """Musical isomorphisms and the operators built on them, restated in numpy.
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
+TEST INFRASTRUCTURE ONLY: what the HIP library is tested against.  Only tests/,
 __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module.
 
 Each function follows a reference routine of CombinatorialSpaces.jl step by

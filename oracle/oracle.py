@@ -1,6 +1,6 @@
 """Python face of the CPU oracle (liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
+TEST INFRASTRUCTURE ONLY: what the HIP library is tested against.  Only tests/,
 __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module.
 
 Every routine mirrors a reference routine of CombinatorialSpaces.jl v0.10.1;

@@ -1,7 +1,7 @@
 """Tetrahedral complexes, their dual complex and the 3-D DEC operators,
 restated in numpy and plain Python.
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
+TEST INFRASTRUCTURE ONLY: what the HIP library is tested against.  Only tests/,
 __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module.
 
 The dual complex is built simplex by simplex as the reference builds it, with
