@@ -1,7 +1,7 @@
 /*
  * dec_oracle.c -- CPU restatement of CombinatorialSpaces.jl's DEC hot path.
  *
- * TEST INFRASTRUCTURE ONLY: this is the checker the HIP library is compared
+ * TEST INFRASTRUCTURE ONLY: this is the restatement the HIP library is compared
  * against.  It is never linked into, called by, or shipped with the product.
  *
  * Every function restates one reference routine and cites it (paths are inside
