@@ -1392,7 +1392,7 @@ void comp_apply(dec_op_s &op, const T *x, const T *y, T *out, hipStream_t s) {
 // run operator i, with that operator's own grid); each workgroup runs the
 // same device body as the single-operator kernel, so results are identical.
 // One launch replaces `count` launches and their ramp-up and drain.
-constexpr int kMaxGroup = 12;
+constexpr int kMaxGroup = 16;
 
 struct GroupOp {
   int impl, rows, vec, width, swap_wdd;
