@@ -102,9 +102,12 @@ __global__ void k_scale_partial(int n, const float *__restrict__ r, double *__re
 
 // One residual: k_scale_partial's sums, and the last workgroup to finish
 // adds the partials in block order and writes s; it also resets the ticket
-// counter for the next launch (so the kernel replays from a graph).
+// counter for the next launch (so the kernel replays from a graph).  The
+// last workgroup loads the partials with one thread each into LDS, then sums
+// them serially there, in the same order as k_scale_final.
 __global__ void k_scale_one(int n, const float *__restrict__ r, double *__restrict__ part,
                             unsigned int *__restrict__ ticket, float *__restrict__ s) {
+  static_assert(kScaleBlocks <= TB, "one partial per thread of the last workgroup");
   __shared__ double sh[TB];
   __shared__ bool last;
   double acc = 0.0;
@@ -121,10 +124,13 @@ __global__ void k_scale_one(int n, const float *__restrict__ r, double *__restri
     last = atomicAdd(ticket, 1u) == kScaleBlocks - 1;
   }
   __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x < kScaleBlocks) sh[threadIdx.x] = part[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {
     double tot = 0.0;
-    for (int j = 0; j < kScaleBlocks; j++) tot += part[j];
+    for (int j = 0; j < kScaleBlocks; j++) tot += sh[j];
     double m = tot / n;
     s[0] = m > 0 ? static_cast<float>(sqrt(m)) : 1.0f;
     *ticket = 0u;
