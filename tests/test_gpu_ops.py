@@ -466,3 +466,25 @@ def test_apply_many_matches_single_applies(cs, dtype):
     for op, x, y, out in items:
         ref = op.apply(x, y)
         assert torch.equal(out, ref)
+
+
+def test_apply_many_splits_long_lists(cs):
+    """A list longer than one grouped launch holds (37 applications of six
+    operators, repeats included) is split over several launches and still
+    matches one launch per application bit for bit."""
+    import torch
+
+    f64 = torch.float64
+    m = cs.dualize(cs.icosphere(3))
+    ops = [cs.dec_wedge_product(0, 1, m).operator, cs.dec_wedge_product(1, 1, m).operator,
+           cs.dec_differential(0, m), cs.dec_differential(1, m),
+           cs.dec_hodge_star(1, m, cs.DiagonalHodge), cs.fast_laplace_beltrami(m, cs.DiagonalHodge)]
+    items = []
+    for k in range(37):
+        op = ops[k % len(ops)]
+        x = cs.fill_uniform(torch.empty(op.cols, dtype=f64, device="cuda"), 100 + 2 * k)
+        y = cs.fill_uniform(torch.empty(op.cols2, dtype=f64, device="cuda"), 101 + 2 * k) if op.cols2 else None
+        items.append((op, x, y, torch.full((op.rows,), float("nan"), dtype=f64, device="cuda")))
+    cs.apply_many(items)
+    for op, x, y, out in items:
+        assert torch.equal(out, op.apply(x, y))

@@ -2,7 +2,10 @@
 v/vt/vn forms, negative indices), binary and ASCII STL with points merged by
 position in order of first occurrence (src/MeshInterop.jl:85-104, 154-165),
 Float32 coordinates as MeshIO reads them, and OBJ output.  The GPU test builds
-library meshes from files and compares them with the oracle's glue + orient!."""
+library meshes from files and compares them with the oracle's glue + orient!.
+tests/golden/assets holds the reference's own mesh files (test/assets/square.obj
+and square.stl, data only), read as test/MeshInterop.jl:16-25 reads them."""
+import os
 import struct
 
 import numpy as np
@@ -66,6 +69,63 @@ def test_obj_round_trip_and_glue(O, tmp_path):
     g = O.glue_triangles(P2, T2, sorted_=True)                      # what load_mesh glues
     assert (g.nv, g.ne, g.nt) == (162, 480, 320) and g.nv - g.ne + g.nt == 2
     assert O.orient(g)
+
+
+ASSETS = os.path.join(os.path.dirname(__file__), "golden", "assets")
+
+
+def _triangle_vertices(v0, v1, e0, e1, e2):
+    """triangle_vertices (src/SimplicialSets.jl:305-309) from ∂ arrays."""
+    return np.stack([v1[e1], v0[e2], v0[e1]], axis=1)
+
+
+@pytest.mark.parametrize("name", ["square.obj", "square.stl"])
+def test_reference_assets_glue_as_the_reference_tests_expect(O, name):
+    """test/MeshInterop.jl:16-22: both files load, and triangle_vertices(s, 1)
+    == [1, 2, 3] (0-based [0, 1, 2]); the STL's six corners merge into four
+    points in order of first occurrence."""
+    from cspaces import meshio
+
+    P, T = meshio.read_mesh(os.path.join(ASSETS, name))
+    assert P.tolist() == [[0, 0, 0], [0, 1, 0], [1, 1, 0], [1, 0, 0]]
+    assert T.tolist() == [[0, 1, 2], [0, 3, 2]]
+    g = O.glue_triangles(P, T, sorted_=True)
+    assert O.orient(g)
+    assert (g.nv, g.ne, g.nt) == (4, 5, 2)
+    assert _triangle_vertices(g.v0, g.v1, g.e0, g.e1, g.e2)[0].tolist() == [0, 1, 2]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["square.obj", "square.stl"])
+def test_reference_assets_load_on_the_library(O, cs, name):
+    """The same files through cs.load_mesh: the oracle's glue + orient!
+    index for index, triangle_vertices(s, 1) == [1, 2, 3], and the barycentric
+    dual of test/MeshInterop.jl:24-25 (areas 1/2 each, ⋆0 summing to 1)."""
+    import torch
+    from cspaces import meshio
+
+    f = os.path.join(ASSETS, name)
+    gm = cs.load_mesh(f)
+    P, T = meshio.read_mesh(f)
+    om = O.glue_triangles(P, T, sorted_=True)
+    assert O.orient(om)
+    a = gm.export()
+    for k in ("v0", "v1", "e0", "e1", "e2"):
+        assert np.array_equal(a[k], getattr(om, k)), k
+    assert np.array_equal(a["tri_orientation"], om.to)
+    assert _triangle_vertices(a["v0"], a["v1"], a["e0"], a["e1"], a["e2"])[0].tolist() == [0, 1, 2]
+    cs.dualize(gm, cs.Barycenter)
+    f64 = torch.float64
+    area = 1.0 / (cs.dec_hodge_star(2, gm, cs.DiagonalHodge, cast_float=f64)
+                  @ torch.ones(gm.nt, dtype=f64, device="cuda")).cpu().numpy()
+    assert np.allclose(area, 0.5, rtol=0, atol=1e-15)
+    h0 = (cs.dec_hodge_star(0, gm, cs.DiagonalHodge, cast_float=f64)
+          @ torch.ones(gm.nv, dtype=f64, device="cuda")).sum().item()
+    assert abs(h0 - 1.0) < 1e-15
+    od = O.Dual(om, circumcenter=False)
+    assert np.allclose((cs.dec_hodge_star(0, gm, cs.DiagonalHodge, cast_float=f64)
+                        @ torch.ones(gm.nv, dtype=f64, device="cuda")).cpu().numpy(),
+                       od.hodge_diag(0), rtol=1e-14, atol=0)
 
 
 def test_unknown_extension_is_refused(tmp_path):
