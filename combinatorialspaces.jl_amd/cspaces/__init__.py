@@ -7,8 +7,8 @@ from .fastdec import (  # noqa: F401
     dec_boundary, dec_c_wedge_product_, dec_differential, dec_dual_derivative, dec_hodge_star,
     dec_inv_hodge_star, dec_wedge_product, dec_wedge_product_dd, dec_wedge_product_dp, dec_wedge_product_pd,
     d0_p0_interpolation, dual_laplacian, dualize, fast_laplace_beltrami, fill_uniform, flat_sharp_mat, icosphere,
-    interior_product_dd, kuhn_box, kuhn_box_arrays, lie_derivative_dd, mesh3_from_arrays, mesh_from_arrays,
-    mesh_from_tetrahedra, mesh_from_triangles, p0_d0_interpolation, p2_d2_interpolation, p3_d3_interpolation,
+    interior_product_dd, kuhn_box, kuhn_box_arrays, lie_derivative_dd, make_sphere, make_sphere_arrays,
+    mesh3_from_arrays, mesh_from_arrays, mesh_from_tetrahedra, mesh_from_triangles, p0_d0_interpolation, p2_d2_interpolation, p3_d3_interpolation,
     subdivision, triangulated_grid, ℒ_dd)
 from .meshio import (  # noqa: F401
     load_json_acset, load_mesh, read_json_acset, read_mesh, read_obj, read_stl, save_obj, write_json_acset, write_obj)

@@ -237,6 +237,128 @@ def kuhn_box(nx, ny, nz, **kw) -> Mesh:
     return mesh_from_tetrahedra(p, t)
 
 
+def _range_len(a, d, b):
+    """length(a:d:b) for the integer-valued steps makeSphere is called with."""
+    return int(np.floor((b - a) / d + 1e-9)) + 1 if b >= a else 0
+
+
+def _sind(x):
+    """sin of x degrees, exact at multiples of 90."""
+    r = np.fmod(np.asarray(x, np.float64), 360.0)
+    out = np.sin(np.radians(r))
+    q = np.fmod(r, 90.0) == 0
+    out[q] = np.array([0.0, 1.0, 0.0, -1.0])[(np.round(r[q] / 90).astype(int)) % 4]
+    return out
+
+
+def _cosd(x):
+    return _sind(np.asarray(x, np.float64) + 90.0)
+
+
+def make_sphere_arrays(min_lat, max_lat, d_lat, min_long, max_long, d_long, radius):
+    """Points and boundary maps of makeSphere (src/CombMeshes.jl:238-342): a
+    latitude-longitude sphere or band of radius `radius`, latitudes measured
+    from the north pole in degrees.  Vertices run parallel by parallel, then
+    the north and south poles when min_lat = 0 / max_lat = 180.  Edges are
+    numbered as the reference creates them: each parallel's closing edge (when
+    max_long = 360) and the first parallel's own edges are added explicitly,
+    every other edge on first use by a triangle (glue_sorted_triangle!), which
+    fixes edge and triangle ids index for index.  Returns (points, v0, v1, e0,
+    e1, e2, north_pole, south_pole) with 0-based ids, -1 for a missing pole.
+    Coordinates use numpy's sin/cos of radians with exact values at multiples
+    of 90 degrees (the reference uses Julia's sind/cosd, which may round
+    differently in the last bit)."""
+    if (not (0 <= min_lat <= 180) or not (0 <= max_lat <= 180) or not (0 <= min_long <= 360)
+            or max_lat < min_lat or max_long < min_long):
+        raise ValueError("Mins must be less than Maxs, lats must be in [0,180], and longs must be in [0,360].")
+    if min_lat == max_lat and d_lat == 0:
+        d_lat = 1
+    north = min_lat == 0
+    south = max_lat == 180
+    if north:
+        min_lat += d_lat
+    if south:
+        max_lat -= d_lat
+    close = max_long == 360
+    if close:
+        max_long -= d_long
+    npar = _range_len(min_lat, d_lat, max_lat)
+    nm = _range_len(min_long, d_long, max_long)
+    if npar < 1 or nm < 2 or (close and nm < 3):
+        raise ValueError("makeSphere: too few parallels or meridians for a triangulated surface")
+    theta = min_lat + d_lat * np.arange(npar, dtype=np.float64)
+    phi = min_long + d_long * np.arange(nm, dtype=np.float64)
+    st, ct = _sind(theta)[:, None], _cosd(theta)[:, None]
+    pts = np.stack([(radius * st) * _cosd(phi)[None, :], (radius * st) * _sind(phi)[None, :],
+                    np.broadcast_to(radius * ct, (npar, nm))], axis=2).reshape(-1, 3)
+    extra = []
+    if north:
+        extra.append([0.0, 0.0, radius * 1.0])
+    if south:
+        extra.append([radius * 0.0, 0.0, radius * -1.0])
+    if extra:
+        pts = np.vstack([pts, np.array(extra)])
+    npole = npar * nm if north else -1
+    spole = npar * nm + (1 if north else 0) if south else -1
+    # The creation stream: explicit edges (kind 0, always new) and sorted
+    # triangles (kind 1, three get_edge! lookups each), in reference order.
+    # events: rows (kind, a, b, c), with a < b (< c for triangles)
+    ev = []
+    i = np.arange(nm - 1)
+    for p in range(npar):
+        o = p * nm
+        if close:
+            ev.append(np.array([[0, o, o + nm - 1, -1]]))
+        if p == 0:
+            ev.append(np.stack([np.zeros(nm - 1, int), o + i, o + i + 1, -np.ones(nm - 1, int)], 1))
+            continue
+        ta = np.stack([o + i, o + i + 1, o - nm + i], 1)
+        tb = np.stack([o + 1 + i, o - nm + i, o - nm + 1 + i], 1)
+        tl = np.array([[o + nm - 1, o, o - 1], [o - nm, o, o - 1]]) if close else np.zeros((0, 3), int)
+        t = np.sort(np.vstack([ta, tb, tl]), axis=1)
+        ev.append(np.hstack([np.ones((len(t), 1), int), t]))
+    for pole, start in ((npole, 0), (spole, nm * (npar - 1))):
+        if pole < 0:
+            continue
+        t = np.stack([np.full(nm - 1, pole), start + i, start + i + 1], 1)
+        if close:
+            t = np.vstack([t, [[pole, start, start + nm - 1]]])
+        t = np.sort(t, axis=1)
+        ev.append(np.hstack([np.ones((len(t), 1), int), t]))
+    ev = np.vstack(ev)
+    tri = ev[:, 0] == 1
+    # every edge reference in stream order: explicit edges once, triangles as
+    # (v0 v1), (v1 v2), (v0 v2) -- the order of glue_triangle!'s get_edge! calls
+    ref = np.where(tri[:, None], ev[:, [1, 2, 1]], ev[:, [1, 1, 1]])
+    ref = np.stack([ref, np.where(tri[:, None], ev[:, [2, 3, 3]], ev[:, [2, 2, 2]])], 2)
+    used = np.where(tri[:, None], True, np.array([True, False, False])[None, :])
+    pairs = ref[used]                                   # (src, tgt), stream order
+    key = pairs[:, 0].astype(np.int64) * (len(pts) + 1) + pairs[:, 1]
+    if len(np.unique(key[~np.repeat(tri, used.sum(1))])) != int((~tri).sum()):
+        raise ValueError("makeSphere: these parameters repeat an explicit edge")
+    uniq, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")             # edge ids by first appearance
+    rank = np.empty(len(uniq), np.int64)
+    rank[order] = np.arange(len(uniq))
+    eid = rank[inv.ravel()]
+    v1 = pairs[first[order], 0]
+    v0 = pairs[first[order], 1]
+    slot = np.cumsum(used.ravel()).reshape(used.shape) - 1
+    te = eid[slot[tri]]                                  # (ea, eb, ec) per triangle
+    # add_triangle!(ea, eb, ec) stores ∂e0 = eb, ∂e1 = ec, ∂e2 = ea
+    return pts, v0, v1, te[:, 1], te[:, 2], te[:, 0], npole, spole
+
+
+def make_sphere(min_lat, max_lat, d_lat, min_long, max_long, d_long, radius):
+    """makeSphere(minLat, maxLat, dLat, minLong, maxLong, dLong, radius)
+    (src/CombMeshes.jl:238-342): (mesh, north_pole, south_pole), 0-based pole
+    ids or -1.  Orientations are left unset, as the reference leaves them:
+    dualize sets edge orientations to false and orients the triangles."""
+    p, v0, v1, e0, e1, e2, npole, spole = make_sphere_arrays(min_lat, max_lat, d_lat, min_long, max_long,
+                                                             d_long, radius)
+    return mesh_from_arrays(p, v0, v1, e0, e1, e2), npole, spole
+
+
 def icosphere(k: int) -> Mesh:
     """Pole-aligned icosahedron refined k times, new points on the unit sphere."""
     h = c_void_p()

@@ -10,6 +10,7 @@ the citations live beside the C restatement in dec_oracle.c.  Indices are
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 from ctypes import POINTER, c_double, c_int, c_ulonglong, c_void_p
 
@@ -234,6 +235,114 @@ def make_restriction(S: CSC) -> CSC:
     s = _OCSC()
     lib().om_make_restriction(ctypes.byref(S.struct()), ctypes.byref(s))
     return CSC._take(s)
+
+
+def make_sphere(min_lat, max_lat, d_lat, min_long, max_long, d_long, radius):
+    """makeSphere (src/CombMeshes.jl:238-342), restated call by call on a
+    growing delta set: add_vertices!, add_sorted_edge(s)!, and
+    glue_sorted_triangle! = add_triangle!(get_edge!(a,b), get_edge!(b,c),
+    get_edge!(a,c)) on the sorted ids, get_edge! reusing the first edge with
+    that (∂v1, ∂v0) (src/SimplicialSets.jl:323-364).  The argument check
+    tests min_long twice, as the reference does.  Coordinates: sin/cos of
+    radians, exact at multiples of 90 degrees.  Returns (Mesh, north_pole,
+    south_pole) with 0-based pole ids, -1 when absent."""
+    if (not (0 <= min_lat <= 180) or not (0 <= max_lat <= 180) or not (0 <= min_long <= 360)
+            or not (0 <= min_long <= 360) or max_lat < min_lat or max_long < min_long):
+        raise ValueError("Mins must be less than Maxs, lats must be in [0,180], and longs must be in [0,360].")
+
+    def sind(x):
+        r = math.fmod(x, 360.0)
+        if math.fmod(r, 90.0) == 0:
+            return (0.0, 1.0, 0.0, -1.0)[int(round(r / 90)) % 4]
+        return math.sin(math.radians(r))
+
+    def cosd(x):
+        return sind(x + 90.0)
+
+    def steps(a, d, b):                       # a:d:b
+        out, k = [], 0
+        while a + k * d <= b + 1e-9 * abs(d):
+            out.append(a + k * d)
+            k += 1
+        return out
+
+    pts, V0, V1, E0, E1, E2, first = [], [], [], [], [], [], {}
+
+    def add_vertex(p):                        # 1-based ids, as in Julia
+        pts.append(p)
+        return len(pts)
+
+    def add_edge(src, tgt):
+        V1.append(src)
+        V0.append(tgt)
+        first.setdefault((src, tgt), len(V0))
+        return len(V0)
+
+    def add_sorted_edge(a, b):
+        return add_edge(min(a, b), max(a, b))
+
+    def get_edge(src, tgt):
+        return first[(src, tgt)] if (src, tgt) in first else add_edge(src, tgt)
+
+    def glue_sorted_triangle(i, j, k):
+        a, b, c = sorted((i, j, k))
+        x, y, z = get_edge(a, b), get_edge(b, c), get_edge(a, c)
+        E0.append(y)
+        E1.append(z)
+        E2.append(x)
+
+    def sph2car(rho, phi, theta):
+        return (rho * sind(theta) * cosd(phi), rho * sind(theta) * sind(phi), rho * cosd(theta))
+
+    if min_lat == max_lat and d_lat == 0:
+        d_lat = 1
+    north = south = False
+    if min_lat == 0:
+        min_lat += d_lat
+        north = True
+    if max_lat == 180:
+        max_lat -= d_lat
+        south = True
+    close = False
+    if max_long == 360:
+        max_long -= d_long
+        close = True
+    lats, longs = steps(min_lat, d_lat, max_lat), steps(min_long, d_long, max_long)
+    npar, nm = len(lats), len(longs)
+    for theta in lats:
+        off = len(pts) + 1
+        for phi in longs:
+            add_vertex(sph2car(radius, phi, theta))
+        if close:
+            add_sorted_edge(off + nm - 1, off)
+        if off == 1:
+            for a in range(off, off + nm - 1):
+                add_sorted_edge(a, a + 1)
+            continue
+        for i, j, k in zip(range(off, off + nm - 1), range(off + 1, off + nm), range(off - nm, off - 1)):
+            glue_sorted_triangle(i, j, k)
+        for i, j, k in zip(range(off + 1, off + nm), range(off - nm, off - 1), range(off - nm + 1, off)):
+            glue_sorted_triangle(i, j, k)
+        if close:
+            glue_sorted_triangle(off + nm - 1, off, off - 1)
+            glue_sorted_triangle(off - nm, off, off - 1)
+    npole = spole = 0
+    if north:
+        npole = add_vertex(sph2car(radius, 0, 0))
+        for i in range(1, nm):
+            glue_sorted_triangle(npole, i, i + 1)
+        if close:
+            glue_sorted_triangle(npole, 1, nm)
+    if south:
+        start = nm * (npar - 1) + 1
+        spole = add_vertex(sph2car(radius, 0, 180))
+        for i in range(start, start + nm - 1):
+            glue_sorted_triangle(spole, i, i + 1)
+        if close:
+            glue_sorted_triangle(spole, start, start + nm - 1)
+    z = lambda v: np.asarray(v, np.int64) - 1  # noqa: E731
+    m = Mesh(np.array(pts, np.float64), z(V0), z(V1), z(E0), z(E1), z(E2))
+    return m, npole - 1, spole - 1
 
 
 def icosphere(k: int) -> Mesh:

@@ -55,7 +55,9 @@ def grid345_arrays(O):
     return m
 
 
-CASES = ["ico3", "ico5", "grid", "grid345_bary", "ico3_bary", "square_unoriented"]
+# uvsphere_bary: makeSphere(5, 175, 5, 0, 360, 5, 6371+90) with barycentric
+# duals, a mesh of test/Operators.jl:60 and test/Backends.jl:53
+CASES = ["ico3", "ico5", "grid", "grid345_bary", "ico3_bary", "square_unoriented", "uvsphere_bary"]
 
 
 @pytest.fixture(scope="module", params=CASES)
@@ -73,6 +75,11 @@ def case(request, O, cs):
     elif name == "grid345_bary":
         om = grid345_arrays(O)
         gm = gpu_mesh_like(cs, om)
+        circ = False
+    elif name == "uvsphere_bary":
+        args = (5, 175, 5, 0, 360, 5, 6371 + 90)
+        om = O.make_sphere(*args)[0]
+        gm = cs.make_sphere(*args)[0]
         circ = False
     else:
         om = O.glue_triangles(np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0.2]], float),
