@@ -56,8 +56,10 @@ def grid345_arrays(O):
 
 
 # uvsphere_bary: makeSphere(5, 175, 5, 0, 360, 5, 6371+90) with barycentric
-# duals, a mesh of test/Operators.jl:60 and test/Backends.jl:53
-CASES = ["ico3", "ico5", "grid", "grid345_bary", "ico3_bary", "square_unoriented", "uvsphere_bary"]
+# duals, and tg10_bary / tg10_circ: triangulated_grid(10, 10, 8, 8) with either
+# dual, the meshes of test/Operators.jl:60 and test/Backends.jl:49-57
+CASES = ["ico3", "ico5", "grid", "grid345_bary", "ico3_bary", "square_unoriented", "uvsphere_bary", "tg10_bary",
+         "tg10_circ"]
 
 
 @pytest.fixture(scope="module", params=CASES)
@@ -76,6 +78,10 @@ def case(request, O, cs):
         om = grid345_arrays(O)
         gm = gpu_mesh_like(cs, om)
         circ = False
+    elif name.startswith("tg10"):
+        om = O.triangulated_grid(10, 10, 8, 8, compress=True)
+        gm = cs.triangulated_grid(10, 10, 8, 8, True)
+        circ = name.endswith("circ")
     elif name == "uvsphere_bary":
         args = (5, 175, 5, 0, 360, 5, 6371 + 90)
         om = O.make_sphere(*args)[0]
