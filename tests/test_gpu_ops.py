@@ -163,6 +163,13 @@ def test_wedges(case, O, cs, dtype):
     assert rel(w02(dV, dT).cpu().numpy(), ref02) <= tol
     assert rel(w20(dT, dV).cpu().numpy(), ref02) <= tol
     assert rel(w11(dE, dE2).cpu().numpy(), ref11) <= tol
+    if dtype == "f32":
+        # test/Backends.jl:131-135 (Float32 on the device): mean((got - ref)²) < 1e-15
+        # against the FP64 wedge of the same, FP32-rounded, inputs
+        V3, E3, F3, T3 = (x.astype(np.float32).astype(np.float64) for x in (V, E, E2, T))
+        for got, ref in ((r01, od.wedge01(V3, E3)), (w02(dV, dT).cpu().numpy(), od.wedge02(V3, T3)),
+                         (w11(dE, dE2).cpu().numpy(), od.wedge11(E3, F3))):
+            assert np.mean((got.astype(np.float64) - ref) ** 2) < 1e-15
     if dtype == "f64" and exact_case(name):
         assert np.array_equal(w11(dE, dE2).cpu().numpy(), ref11)
     if dtype == "f64":
