@@ -159,6 +159,13 @@ def test_wedges(case, O, cs, dtype):
     ref11 = od.wedge11(E, E2)
     r01 = w01(dV, dE).cpu().numpy()
     assert rel(r01, ref01) <= tol
+    # wedge 0-0 is the pointwise product (test/Operators.jl:229, 239 compare with ==)
+    V2 = O.uniform_form(17, od.nv)
+    w00 = cs.dec_wedge_product(0, 0, gm, cast_float=td)(dV, to_dev(torch, V2, td)).cpu().numpy()
+    if dtype == "f64":
+        assert np.array_equal(w00, V * V2)
+    else:
+        assert np.array_equal(w00, V.astype(np.float32) * V2.astype(np.float32))
     assert np.array_equal(w10(dE, dV).cpu().numpy(), r01)           # (1,0) swaps its arguments
     assert rel(w02(dV, dT).cpu().numpy(), ref02) <= tol
     assert rel(w20(dT, dV).cpu().numpy(), ref02) <= tol
