@@ -13,7 +13,7 @@ from test_gpu_tet import csr3, gpu_mesh3_like
 
 pytestmark = pytest.mark.gpu
 
-CASES2 = ["tri_345", "grid_345", "tg", "ico3", "ico3_bary", "rect"]
+CASES2 = ["tri_345", "grid_345", "tg", "ico3", "ico3_bary", "rect", "uvsphere_bary"]
 CASES3 = ["single", "kuhn_bary", "kuhn_jitter_circ"]
 
 
@@ -26,6 +26,8 @@ def case2(request, O, cs):
         om, circ = O.icosphere(3), not name.endswith("bary")
     elif name == "rect":
         om, circ = O.triangulated_grid(30, 10, 0.5, 0.5, True), True
+    elif name == "uvsphere_bary":
+        om, circ = O.make_sphere(5, 175, 5, 0, 360, 5, 6371 + 90)[0], False      # test/Operators.jl:60
     else:
         om, circ = flat_primal_meshes(O)[name], False
     od = O.Dual(om, circumcenter=circ)
