@@ -228,23 +228,41 @@ def ops_3d(torch, cs, n=64, sets=3, steps=30, warmup=3):
     return res
 
 
-def stream_copy_gbps(torch):
-    n = 1 << 28  # 2 GiB of float64 per buffer
-    x = torch.empty(n, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
-    x.fill_(1.0)
-    for _ in range(3):
-        y.copy_(x)
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(10):
-        y.copy_(x)
-    b.record()
-    torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / 10
-    del x, y
-    torch.cuda.empty_cache()
-    return 2 * n * 8 / (ms * 1e-3) / 1e9
+def stream_gbps(torch):
+    """Hand-written STREAM copy (b = a) and triad (a = b + s c) kernels
+    (libdec_stream.so), float64, at traffic sizes of the operators (16-120 MB
+    per launch: read + written bytes) and at 2 GiB.  Below 2 GiB the arrays
+    rotate over enough sets that one rotation exceeds 1 GiB, four times the
+    256 MB Infinity Cache, as the operator legs do.  Each figure is the best
+    of plain and nontemporal stores."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "combinatorialspaces.jl_amd", "libdec_stream.so"))
+    lib.dec_stream_run.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                   ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                   ctypes.POINTER(ctypes.c_float)]
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for kind, name, arrays in ((0, "copy", 2), (1, "triad", 3)):
+        res = {}
+        for mb in (16, 32, 64, 120, 2048):
+            nbytes = mb * (1 << 20) if mb == 2048 else mb * 10 ** 6     # traffic per launch
+            n = (nbytes // (8 * arrays)) // 2 * 2
+            sets = 1 if mb == 2048 else max(1, -(-(1 << 30) // (n * 8 * arrays)))
+            bufs = [torch.ones(n, dtype=torch.float64, device="cuda") for _ in range(arrays * sets)]
+            ptrs = (ctypes.c_void_p * len(bufs))(*[b.data_ptr() for b in bufs])
+            best = None
+            for nt in (0, 1):
+                ms = ctypes.c_float()
+                reps = 20 if mb == 2048 else 200
+                rc = lib.dec_stream_run(kind, nt, n, sets, ptrs, 2 * sets, reps, stream, ctypes.byref(ms))
+                if rc != 0:
+                    raise RuntimeError(f"dec_stream_run: hip error {rc}")
+                gbps = arrays * n * 8 / (ms.value * 1e-3) / 1e9
+                best = gbps if best is None else max(best, gbps)
+            res[f"{mb}MB" if mb != 2048 else "2GiB"] = best
+            del bufs
+            torch.cuda.empty_cache()
+        out[name] = res
+    return out
 
 
 def sh_rhs(points, h0, seed, nterms=16, lmax=12):
@@ -443,48 +461,52 @@ def learned(torch, cs, md, fd, npairs=10000, train_levels=3, steps=2000, batch=1
     return out
 
 
-def cpu_baseline(k=7, budget_s=12.0):
-    """The oracle (single-threaded C) on the same operator set, icosphere k=7."""
-    import numpy as np
+def _host_cpus():
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return avail, model
 
+
+def cpu_baseline(k=8, budget_s=8.0):
+    """The oracle's operator loops (C, -O3, OpenMP; oracle/cpu_baseline.c) on
+    the operator set of the GPU step, icosphere k, float64: passes for about
+    budget_s seconds at one thread and again at all host cores the process may
+    use (at most 16, the box's CPU share of one GPU)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    import cpubase
 
-    lib = O.lib()
+    t = time.perf_counter()
     d = O.Dual(O.icosphere(k))
-    nv, ne, nt = d.nv, d.ne, d.nt
-    v02, c02 = d.wedge02_coeffs()
-    e11, c11 = d.wedge11_coeffs()
-    D0, D1 = d.differential(0), d.differential(1)
-    L = d.laplacian0()
-    h = [d.hodge_diag(i) for i in range(3)]
-    V, E, E2, T = (O.uniform_form(s, n) for s, n in ((1, nv), (2, ne), (3, ne), (4, nt)))
-    oE, oT, oV = np.zeros(ne), np.zeros(nt), np.zeros(nv)
-    P = O._p
-    ci = ctypes.c_int
-    D0s, D1s, Ls = D0.struct(), D1.struct(), L.struct()
-
-    def one_pass():
-        lib.om_wedge01(ci(ne), P(d.m.v0), P(d.m.v1), P(V), P(E), P(oE))
-        lib.om_wedge11(ci(nt), P(e11), P(c11), P(E), P(E2), P(oT))
-        lib.om_wedge02(ci(nt), P(v02), P(c02), P(V), P(T), P(oT))
-        lib.om_spmv(ctypes.byref(D0s), P(V), P(oE))
-        lib.om_spmv(ctypes.byref(D1s), P(E), P(oT))
-        np.multiply(h[0], V, out=oV)
-        np.multiply(h[1], E, out=oE)
-        np.multiply(h[2], T, out=oT)
-        lib.om_spmv(ctypes.byref(Ls), P(V), P(oV))
-
-    one_pass()
-    passes, t = 0, time.perf_counter()
-    while time.perf_counter() - t < budget_s:
-        one_pass()
-        passes += 1
-    dt = time.perf_counter() - t
-    nbytes = sum(alg_bytes(n, nv, ne, nt, 8, L.nnz) for n in OPSET)
-    return {"value": passes * nbytes / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (C, -O2, 1 thread) float64, icosphere k={k} ({nv} V), {passes} passes of the "
-                      f"operator set in {dt:.1f} s"}
+    s = cpubase.OpSet(d)
+    setup = time.perf_counter() - t
+    nbytes = sum(alg_bytes(n, d.nv, d.ne, d.nt, 8, s.L.nnz) for n in OPSET)
+    avail, model = _host_cpus()
+    threads = max(1, min(avail, 16))
+    rates = {}
+    for th in sorted({1, threads}):
+        dt1 = s.run(th, 1)
+        passes = max(2, int(budget_s / max(dt1, 1e-4)))
+        dt = s.run(th, passes)
+        rates[th] = (passes * nbytes / dt / 1e9, passes, dt)
+    v, passes, dt = rates[threads]
+    return {"value": v, "unit": "GB/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "cpus_available": avail, "cpu_model": model,
+            "one_thread_GBps": rates[1][0],
+            "sample": f"the oracle's operator loops (C, -O3, OpenMP) on the same nine operators, float64, "
+                      f"icosphere k={k} ({d.nv} V): {passes} passes in {dt:.1f} s at {threads} threads, "
+                      f"{rates[1][1]} passes in {rates[1][2]:.1f} s at 1 thread; set-up {setup:.1f} s"}
 
 
 def batched_heat(torch, cs, ninst=64, nsteps=100, seed=7):
@@ -738,10 +760,11 @@ def main():
         print(f"# 3-D op set: {extra['ops_3d_f64']['grouped_GBps']:.0f} GB/s grouped launch", file=sys.stderr,
               flush=True)
     try:
-        extra["stream_copy_GBps"] = stream_copy_gbps(torch)
-    except RuntimeError as e:  # memory
-        extra["stream_copy_GBps"] = None
-        print(f"# stream copy skipped: {e}", file=sys.stderr)
+        extra["stream_GBps"] = stream_gbps(torch)
+        print(f"# stream: {json.dumps(extra['stream_GBps'])}", file=sys.stderr, flush=True)
+    except (RuntimeError, OSError) as e:
+        extra["stream_GBps"] = None
+        print(f"# stream skipped: {e}", file=sys.stderr)
     if not args.no_solve:
         extra["poisson"], md, fd = poisson(torch, cs)
         print(f"# poisson: {json.dumps(extra['poisson'])}", file=sys.stderr, flush=True)

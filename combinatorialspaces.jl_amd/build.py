@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdec_hip.so")
+STREAM_LIB = os.path.join(HERE, "libdec_stream.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIP_SOURCES = ["runtime.hip", "mesh.hip", "tet.hip", "ops.hip", "mg.hip", "batch.hip", "learned.hip"]
@@ -60,6 +61,11 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # the bandwidth-ceiling kernels the benchmark measures against (not part
+    # of the operator library)
+    tmp = STREAM_LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "stream_bench.hip")])
+    os.replace(tmp, STREAM_LIB)
     return LIB
 
 

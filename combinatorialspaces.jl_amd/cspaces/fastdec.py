@@ -667,8 +667,10 @@ def fill_uniform(x, seed, stream=None):
 
 def apply_many(items, stream=None):
     """One launch for several applications: items = [(op, x, y, out), ...]
-    (y None for one-form operators), all operators of one dtype.  Same
-    results as op.apply(x, y, out=out) for each item."""
+    (y None for one-form operators), all operators of one dtype.  The items
+    run concurrently: no out may overlap another item's x, y or out (the
+    library raises DecError).  Same results as op.apply(x, y, out=out) for
+    each item."""
     items = list(items)
     if not items:
         return

@@ -84,14 +84,20 @@ def _stl_corners(path):
 
 def unique_index_map(coords, force_unique):
     """(ind_map, uniques): with force_unique, equal points share the index of
-    their first occurrence and `uniques` lists those first occurrences."""
+    their first occurrence and `uniques` lists those first occurrences.
+    Points are equal as Julia's isequal compares them (the Dict of
+    MeshInterop.jl:84-103): bit for bit, so -0.0 and 0.0 differ, while every
+    NaN equals every other NaN."""
     n = len(coords)
     if not force_unique:
         return np.arange(n), np.arange(n)
+    c64 = np.array(coords, np.float64).reshape(n, -1)
+    c64[np.isnan(c64)] = np.nan                      # one canonical NaN payload
+    keys = c64.view(np.uint64)
     seen = {}
     ind_map = np.empty(n, np.int64)
     uniques = []
-    for j, c in enumerate(map(tuple, np.asarray(coords))):
+    for j, c in enumerate(map(tuple, keys)):
         k = seen.get(c)
         if k is None:
             k = seen[c] = len(uniques)

@@ -177,10 +177,27 @@ void check(hipError_t e, const char *what) {
 }
 
 // Time candidate shapes on the batch's own work and keep the fastest.
+// The autotuner's candidate launch shapes; a cached shape outside these sets
+// is not trusted.
+constexpr int kBlocks[] = {64, 128, 256, 512};
+constexpr int kRpts[] = {1, 2, 4};
+constexpr int kMults[] = {2, 4, 8};
+
+template <size_t N>
+bool one_of(const int (&set)[N], int v) {
+  for (int x : set)
+    if (x == v) return true;
+  return false;
+}
+
+bool valid_shape(const Shape &s) {
+  return one_of(kBlocks, s.block) && one_of(kRpts, s.rpt) && one_of(kMults, s.grid_mult);
+}
+
 void tune(dec_batch_s &b, void *u) {
-  const int blocks[] = {64, 128, 256, 512};
-  const int rpts[] = {1, 2, 4};
-  const int mults[] = {2, 4, 8};
+  const auto &blocks = kBlocks;
+  const auto &rpts = kRpts;
+  const auto &mults = kMults;
   hipEvent_t e0, e1;
   check(hipEventCreate(&e0), "hipEventCreate");
   check(hipEventCreate(&e1), "hipEventCreate");
@@ -212,14 +229,21 @@ void tune(dec_batch_s &b, void *u) {
 }
 
 std::string cache_key(const dec_batch_s &b) {
-  // sizes of the instances and the dtype determine the workload shape
+  // the device's architecture, the sizes of the instances and the dtype
+  // determine the best shape
   unsigned long long h = 1469598103934665603ULL;
   for (auto &i : b.inst) {
     h ^= static_cast<unsigned long long>(i.n);
     h *= 1099511628211ULL;
   }
-  char buf[96];
-  snprintf(buf, sizeof buf, "heat:%d:%lld:%d:%016llx", b.ninst, b.total, b.dtype, h);
+  int dev = 0;
+  hipDeviceProp_t prop;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+  std::string arch(prop.gcnArchName);
+  arch = arch.substr(0, arch.find(':'));  // drop the feature suffix (":sramecc+:xnack-")
+  char buf[160];
+  snprintf(buf, sizeof buf, "heat:%s:%d:%lld:%d:%016llx", arch.c_str(), b.ninst, b.total, b.dtype, h);
   return buf;
 }
 
@@ -231,7 +255,7 @@ bool cache_lookup(const char *path, const std::string &key, Shape &s) {
   int bl, rp, gm;
   bool found = false;
   while (fscanf(f, "%127s %d %d %d", k, &bl, &rp, &gm) == 4)
-    if (key == k) {
+    if (key == k && valid_shape(Shape{bl, rp, gm})) {  // a later valid line wins
       s = Shape{bl, rp, gm};
       found = true;
     }

@@ -2673,6 +2673,30 @@ int dec_op_apply_many(int32_t count, const dec_op *ops, const void *const *x, co
     if (op->rows > 0 && op->cols > 0 && !x[q]) fail(DEC_ERR_ARG, "dec_op_apply_many: NULL input");
     if (op->rows > 0 && op->cols2 > 0 && !y[q]) fail(DEC_ERR_ARG, "dec_op_apply_many: NULL second input");
   }
+  // The items of one call run concurrently: an output that overlaps another
+  // item's input or output would race, so such lists are refused.
+  const size_t es = dtype == DEC_F64 ? sizeof(double) : sizeof(float);
+  auto span = [es](const void *p, int64_t n) {
+    uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    return std::make_pair(a, a + static_cast<uintptr_t>(n > 0 ? n : 0) * es);
+  };
+  auto overlap = [](std::pair<uintptr_t, uintptr_t> a, std::pair<uintptr_t, uintptr_t> b) {
+    return a.first < a.second && b.first < b.second && a.first < b.second && b.first < a.second;
+  };
+  for (int q = 0; q < count; q++) {
+    if (ops[q]->rows <= 0) continue;
+    auto o = span(out[q], ops[q]->rows);
+    for (int r = 0; r < count; r++) {
+      if (r == q) continue;
+      const dec_op_s *opr = ops[r];
+      if (opr->rows <= 0) continue;
+      if (overlap(o, span(out[r], opr->rows)) || (opr->cols > 0 && overlap(o, span(x[r], opr->cols))) ||
+          (opr->cols2 > 0 && overlap(o, span(y[r], opr->cols2))))
+        fail(DEC_ERR_ARG, "dec_op_apply_many: the output of item " + std::to_string(q) +
+                              " overlaps an input or output of item " + std::to_string(r) +
+                              "; items of one call run concurrently");
+    }
+  }
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dtype == DEC_F64) apply_group_t<double>(count, ops, x, y, out, s);
   else apply_group_t<float>(count, ops, x, y, out, s);

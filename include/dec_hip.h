@@ -224,9 +224,12 @@ int dec_op_apply(dec_op op, const void *x, const void *y, void *out, void *strea
 int dec_op_solve_info(dec_op op, int32_t *iterations, double *relres, void *stream);
 /* Several applications in one launch: out[i] = ops[i](x[i], y[i]) for
  * i < count (y[i] ignored by one-form operators).  All operators must share
- * a dtype.  Results are bit-identical to count dec_op_apply calls; the single
- * grid removes the per-launch ramp-up and drain.  The geometric ⋆1⁻¹
- * solve is refused (DEC_ERR_UNSUPPORTED).  No counterpart in the
+ * a dtype.  The items run concurrently, so no output may overlap another
+ * item's input or output (DEC_ERR_ARG otherwise; a chain such as ⋆ then d
+ * takes two calls).  With that, results are bit-identical to count
+ * dec_op_apply calls; the single grid removes the per-launch ramp-up and
+ * drain.  The geometric ⋆1⁻¹ solve and composite operators are refused
+ * (DEC_ERR_UNSUPPORTED).  No counterpart in the
  * reference (each FastDEC operator is its own kernel, FastDEC.jl:255-289). */
 int dec_op_apply_many(int32_t count, const dec_op *ops, const void *const *x, const void *const *y,
                       void *const *out, void *stream);

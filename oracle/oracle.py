@@ -24,7 +24,7 @@ def build() -> str:
     """Compile the oracle with its Makefile (gcc)."""
     import subprocess
 
-    subprocess.run(["make", "-s", "-B", "-C", _HERE, "liboracle.so"], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "all"], check=True)
     return _LIB_PATH
 
 

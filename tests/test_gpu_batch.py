@@ -112,3 +112,25 @@ def test_f32_against_f64_trajectory(cs, batch_setup):
     b32.steps(u32, 20)
     err = (u32.double() - u64).abs().max().item() / u64.abs().max().item()
     assert err < 1e-5
+
+
+def test_cache_entries_outside_the_candidates_are_retuned(cs, batch_setup, tmp_path):
+    """A cache line whose shape is not one the tuner tries (block 0, a grid
+    multiplier of 0) is ignored: the batch tunes again, appends a valid line,
+    and its steps still match the single-mesh operator."""
+    import torch
+
+    levels, meshes, inst, b64, b32, u0 = batch_setup
+    u32 = u0.to(torch.float32)
+    path = tmp_path / "tune.txt"
+    b32.tune(u32, str(path))
+    key = path.read_text().split()[0]
+    assert ":gfx" in key                                    # the device architecture is part of the key
+    path.write_text(f"{key} 0 1 4\n{key} 256 1 0\n")
+    shape = b32.tune(u32, str(path))
+    assert shape["block"] in (64, 128, 256, 512) and shape["grid_mult"] in (2, 4, 8)
+    assert len(path.read_text().splitlines()) == 3
+    a = u32.clone()
+    b32.steps(a, 2, graph=False)
+    ref = torch.cat([eager_steps(cs, m, b32.dt[i], b32.instance(u32, i).clone(), 2) for i, m in enumerate(inst)])
+    assert torch.equal(a, ref)

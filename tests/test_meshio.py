@@ -56,6 +56,23 @@ def test_binary_and_ascii_stl_merge_points_in_first_occurrence_order(tmp_path):
         assert len(P2) == 6 and T2.tolist() == [[0, 1, 2], [3, 4, 5]]
 
 
+def test_stl_keeps_signed_zeros_apart(tmp_path):
+    """Corners merge when Julia's isequal holds (MeshInterop.jl:84-103):
+    -0.0 and 0.0 are different keys there, so a corner written as -0.0 is a
+    vertex of its own."""
+    from cspaces import meshio
+
+    a = tmp_path / "signed_zero.stl"
+    a.write_text("solid z\nfacet normal 0 0 1\nouter loop\n"
+                 "vertex 0 0 0\nvertex 1 0 0\nvertex 1 1 0\nendloop\nendfacet\n"
+                 "facet normal 0 0 1\nouter loop\n"
+                 "vertex -0.0 0 0\nvertex 1 1 0\nvertex 0 1 0\nendloop\nendfacet\nendsolid z\n")
+    P, T = meshio.read_stl(str(a))
+    assert len(P) == 5
+    assert T.tolist() == [[0, 1, 2], [3, 2, 4]]
+    assert np.signbit(P[3, 0]) and not np.signbit(P[0, 0])
+
+
 def test_obj_round_trip_and_glue(O, tmp_path):
     from cspaces import meshio
 

@@ -185,6 +185,162 @@ def test_wedge_dd_weights(O):
     assert np.all((D > 0).sum(axis=1) == 2)                  # closed surface: two cofaces
 
 
+# --- brute force from the points alone ---------------------------------------
+def _tri_centers(m, circumcenter):
+    """Triangle centers from the three points, independent of the oracle's
+    dual complex: the centroid, or the circumcenter in closed form
+    a + (|u|² v×(u×v) + |v|² (u×v)×u) / (2|u×v|²), u = b - a, v = c - a."""
+    V = m.triangle_vertices()
+    a, b, c = m.p[V[:, 0]], m.p[V[:, 1]], m.p[V[:, 2]]
+    if not circumcenter:
+        return (a + b + c) / 3
+    u, v = b - a, c - a
+    w = np.cross(u, v)
+    num = (u * u).sum(1)[:, None] * np.cross(v, w) + (v * v).sum(1)[:, None] * np.cross(w, u)
+    return a + num / (2 * (w * w).sum(1))[:, None]
+
+
+def _area(a, b, c):
+    return 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=-1)
+
+
+def brute_wedge_dd_weights(m, circumcenter):
+    """m[e, t] = |c_t - mid_e| / Σ_{t' ∋ e} |c_t' - mid_e| (wedge_dd_01_mat,
+    src/FastDEC.jl:293-305: the elementary duals of e run from e's midpoint to
+    the centers of its cofaces), as a dense ne x nt array."""
+    ct = _tri_centers(m, circumcenter)
+    mid = 0.5 * m.p[m.v0] + 0.5 * m.p[m.v1]
+    W = np.zeros((m.ne, m.nt))
+    for E in (m.e0, m.e1, m.e2):
+        t = np.arange(m.nt)
+        W[E, t] = np.linalg.norm(ct - mid[E], axis=1)
+    return W / W.sum(axis=1, keepdims=True)
+
+
+def dd_weights_match(M, m, circumcenter):
+    return np.allclose(M.dense(), brute_wedge_dd_weights(m, circumcenter), rtol=0, atol=1e-13)
+
+
+def _swap_two_weights(M):
+    """A planted mistake: the two weights of the two-coface edge whose weights
+    differ most are exchanged between its triangles (row sums and positivity
+    unchanged)."""
+    D = M.dense().copy()
+    two = np.flatnonzero((D > 0).sum(axis=1) == 2)
+    spread = D[two].max(axis=1) - np.where(D[two] > 0, D[two], np.inf).min(axis=1)
+    e = int(two[np.argmax(spread)])
+    t1, t2 = np.flatnonzero(D[e])
+    D[e, t1], D[e, t2] = D[e, t2], D[e, t1]
+    class _Dense:
+        def dense(self):
+            return D
+    return _Dense()
+
+
+def _jittered_grid(O):
+    """triangulated_grid(1, 1, 1/4, √3/8) with its points moved by a seeded
+    in-plane jitter of up to 15% of the spacing, so that no two cofaces of an
+    edge are mirror images."""
+    m = O.triangulated_grid(1, 1, 1 / 4, math.sqrt(3) / 8, compress=False)
+    rng = np.random.default_rng(5)
+    m.p[:, :2] += rng.uniform(-0.15, 0.15, (m.nv, 2)) * (1 / 4)
+    return m
+
+
+@pytest.mark.parametrize("case", ["icosphere2_circumcenter", "grid_barycenter"])
+def test_wedge_dd_weights_brute_force(O, case):
+    """The oracle's dual-dual 0-1 matrix equals the weights computed from the
+    points alone, on a curved closed surface (circumcentric, two cofaces per
+    edge) and a jittered flat grid with boundary edges (barycentric, one coface
+    there, weight 1); a copy with two weights of one edge swapped fails the check."""
+    if case == "icosphere2_circumcenter":
+        m, circ = O.icosphere(2), True
+    else:
+        m, circ = _jittered_grid(O), False
+    d = O.Dual(m, circumcenter=circ)
+    M = d.wedge_dd01_mat()
+    assert dd_weights_match(M, m, circ)
+    W = brute_wedge_dd_weights(m, circ)
+    assert len(np.unique(np.round(W[W > 0], 12))) > 2       # weights are not all equal
+    assert not dd_weights_match(_swap_two_weights(M), m, circ)
+
+
+def test_wedge_dd_reference_cases(O):
+    """test/Operators.jl:419-437 on the flat meshes: f = ⋆(1/√2 dx + 1/√2 dy)
+    (geometric ⋆1, the default Hodge), g = 5 on triangles; Λ10(f, g) ≈
+    ⋆(5/√2 dx + 5/√2 dy) and Λ10(f, g) == Λ01(g, f) exactly."""
+    from test_oracle_musical import flat_primal_meshes
+
+    s2 = 1 / math.sqrt(2)
+    for name, m in flat_primal_meshes(O).items():
+        d = O.Dual(m, circumcenter=False)
+        H1 = d.geo_hodge1()
+        f = H1.matvec(_constant_primal_form(d, (s2, s2, 0)))
+        g = np.full(d.nt, 5.0)
+        l10, l01 = d.wedge_dd10(f, g), d.wedge_dd01(g, f)
+        ref = H1.matvec(_constant_primal_form(d, (5 * s2, 5 * s2, 0)))
+        assert np.allclose(l10, ref, rtol=1.5e-8, atol=1e-14), name
+        assert np.array_equal(l10, l01), name
+
+
+def brute_wedge02(m, circumcenter, f, a):
+    """∧(0, 2) of the reference's pedagogical wedge (wedge_product_zero,
+    src/DiscreteExteriorCalculus.jl:2084-2090) from the points alone: a
+    triangle's six dual triangles are (p_v, mid_e, c_t) for its vertices v and
+    the two edges e of t at v; the weight of f[v] is the sum of its two dual
+    areas over the triangle's area."""
+    ct = _tri_centers(m, circumcenter)
+    V = m.triangle_vertices()
+    P = m.p[V]                                          # nt x 3 x 3
+    area = _area(P[:, 0], P[:, 1], P[:, 2])
+    res = np.zeros(m.nt)
+    for i in range(3):
+        pv = P[:, i]
+        wsum = np.zeros(m.nt)
+        for j in range(3):
+            if j != i:
+                wsum += _area(pv, 0.5 * pv + 0.5 * P[:, j], ct)
+        res += wsum / area * f[V[:, i]]
+    return res * a
+
+
+@pytest.mark.parametrize("case", ["icosphere2_circumcenter", "grid_barycenter", "tri_345"])
+def test_wedge02_equals_generic_wedge(O, case):
+    """dec_wedge_product(0, 2) == ∧(0, 2) (test/Operators.jl:245) with random
+    0-forms: the oracle's FastDEC table (six vertex/coefficient slots) agrees
+    with the per-simplex wedge built from the points; a copy of the table with
+    two vertex slots exchanged does not."""
+    if case == "icosphere2_circumcenter":
+        m, circ = O.icosphere(2), True
+    elif case == "grid_barycenter":
+        m, circ = _jittered_grid(O), False
+    else:
+        from test_oracle_musical import flat_primal_meshes
+
+        m, circ = flat_primal_meshes(O)["tri_345"], True
+    d = O.Dual(m, circumcenter=circ)
+    f, a = O.uniform_form(11, d.nv), O.uniform_form(12, d.nt)
+    got, ref = d.wedge02(f, a), brute_wedge02(m, circ, f, a)
+    assert np.allclose(got, ref, rtol=1e-13, atol=1e-15)
+    if not circ:
+        # barycentric: all six dual areas are area/6, so 0-2 is the plain
+        # vertex mean and no vertex permutation can change it
+        assert np.allclose(got, a * f[m.triangle_vertices()].mean(axis=1), rtol=1e-13, atol=1e-15)
+        return
+    # planted mistake: in the triangle where it matters most, the first slot
+    # points at the vertex of the third slot and vice versa
+    verts, c = d.wedge02_coeffs()
+    V6, C6 = verts.reshape(d.nt, 6), c.reshape(d.nt, 6)
+    t = int(np.argmax(np.abs((C6[:, 0] - C6[:, 2]) * (f[V6[:, 0]] - f[V6[:, 2]]) * a)))
+    bad = V6.copy()
+    assert bad[t, 0] != bad[t, 2]
+    bad[t, [0, 2]] = bad[t, [2, 0]]
+    res = np.zeros(d.nt)
+    O.lib().om_wedge02(O.c_int(d.nt), O._p(bad.reshape(-1)), O._p(c), O._p(f), O._p(a), O._p(res))
+    assert np.array_equal(np.delete(res, t), np.delete(got, t))
+    assert not np.isclose(res[t], ref[t], rtol=1e-13, atol=1e-15)
+
+
 # --- meshes and subdivision: test/CombMeshes.jl:14-84, test/Multigrid.jl:14-41 -
 def test_icosphere_counts(O):
     expect = [(12, 30, 20), (42, 120, 80), (162, 480, 320), (642, 1920, 1280),
