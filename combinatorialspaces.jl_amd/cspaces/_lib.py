@@ -86,6 +86,7 @@ SIGNATURES = {
     "dec_batch_layout": (c_int, [P, PI64, P, P]),
     "dec_batch_tune": (c_int, [P, P, c_char_p, PI32, PI32, PI32]),
     "dec_batch_heat_steps": (c_int, [P, P, c_int32, c_int32, P]),
+    "dec_batch_heat_steps_noalloc": (c_int, [P, P, c_int32, P]),
     "dec_batch_destroy": (c_int, [P]),
 }
 

@@ -53,5 +53,11 @@ class HeatBatch:
         check(lib().dec_batch_heat_steps(self._h, self._u(u), nsteps, int(bool(graph)), _stream(stream)))
         return u
 
+    def steps_noalloc(self, u, nsteps, stream=None):
+        """The same steps with the library's fixed scratch array as the
+        partner buffer: no device allocation (states up to 32 MiB)."""
+        check(lib().dec_batch_heat_steps_noalloc(self._h, self._u(u), nsteps, _stream(stream)))
+        return u
+
     def instance(self, u, i):
         return u[self.offsets[i]:self.offsets[i + 1]]

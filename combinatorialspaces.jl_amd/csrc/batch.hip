@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "dec_internal.h"
@@ -83,6 +84,15 @@ __global__ void k_copy_rows(long long n, const T *__restrict__ x, T *__restrict_
   if (i < n) y[i] = x[i];
 }
 
+// Partner buffer of dec_batch_heat_steps_noalloc: a fixed device array, so
+// that entry point allocates nothing.  Each GPU the library runs on holds its
+// own copy until the process ends.  Host code never passes `scratch` itself
+// (in host code the name is a stand-in, not a device address): it passes the
+// address hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 struct Shape {
   int block = 256;
   int rpt = 1;
@@ -147,9 +157,10 @@ void launch_step(dec_batch_s &b, const T *src, T *dst, hipStream_t s) {
   }
 }
 
+// nsteps steps ping-ponging between the caller's state u and the partner
+// buffer t (at least b.total entries); an odd count ends with a copy back.
 template <class T>
-void run_steps(dec_batch_s &b, T *u, int nsteps, hipStream_t s) {
-  T *t = reinterpret_cast<T *>(b.tmp.p);
+void run_steps(dec_batch_s &b, T *u, T *t, int nsteps, hipStream_t s) {
   for (int k = 0; k < nsteps; k++) {
     if (k % 2 == 0) launch_step<T>(b, u, t, s);
     else launch_step<T>(b, t, u, s);
@@ -158,9 +169,41 @@ void run_steps(dec_batch_s &b, T *u, int nsteps, hipStream_t s) {
     hipLaunchKernelGGL(k_copy_rows<T>, dim3(dec::nblocks(b.total)), dim3(256), 0, s, b.total, t, u);
 }
 
+void run_with(dec_batch_s &b, void *u, void *partner, int nsteps, hipStream_t s) {
+  if (b.dtype == DEC_F64) run_steps<double>(b, static_cast<double *>(u), static_cast<double *>(partner), nsteps, s);
+  else run_steps<float>(b, static_cast<float *>(u), static_cast<float *>(partner), nsteps, s);
+}
+
+size_t state_bytes(const dec_batch_s &b) { return static_cast<size_t>(b.total) * (b.dtype == DEC_F64 ? 8 : 4); }
+
+// The library-owned partner buffer, allocated on first use (never under a
+// capture: callers reach this before hipStreamBeginCapture).
+void ensure_tmp(dec_batch_s &b) {
+  if (!b.tmp.p) b.tmp.alloc(state_bytes(b));
+}
+
+// The device address of `scratch` on the current device, looked up on the
+// first call made with that device current and kept per device.
+void *scratch_ptr() {
+  static std::mutex mu;
+  static void *ptrs[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) fail(DEC_ERR_HIP, "hipGetDevice failed");
+  if (dev < 0 || dev >= kMaxDevices) fail(DEC_ERR_UNSUPPORTED, "device index above the scratch table");
+  std::lock_guard<std::mutex> g(mu);
+  if (!ptrs[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      fail(DEC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ptrs[dev] = p;
+  }
+  return ptrs[dev];
+}
+
 void run(dec_batch_s &b, void *u, int nsteps, hipStream_t s) {
-  if (b.dtype == DEC_F64) run_steps<double>(b, static_cast<double *>(u), nsteps, s);
-  else run_steps<float>(b, static_cast<float *>(u), nsteps, s);
+  ensure_tmp(b);
+  run_with(b, u, b.tmp.p, nsteps, s);
 }
 
 void drop_graph(dec_batch_s &b) {
@@ -328,7 +371,6 @@ dec_batch_s *create(int ninst, const dec_mesh *meshes, int dtype) {
   b->dops64.upload(p64);
   b->dops32.upload(p32);
   b->dinst.upload(b->inst);
-  b->tmp.alloc(static_cast<size_t>(b->total) * (dtype == DEC_F64 ? 8 : 4));
   build_items(*b);
   dec::check_launch("dec_batch_create");
   dec::dsync();
@@ -372,11 +414,11 @@ int dec_batch_tune(dec_batch b, void *u, const char *cache_path, int32_t *block,
     drop_graph(*b);
   } else {
     // tuning runs steps on u: work on a copy so the caller's state is kept
-    size_t bytes = static_cast<size_t>(b->total) * (b->dtype == DEC_F64 ? 8 : 4);
-    DBuf<unsigned char> scratch;
-    scratch.alloc(bytes);
-    check(hipMemcpy(scratch.p, u, bytes, hipMemcpyDeviceToDevice), "hipMemcpy");
-    tune(*b, scratch.p);
+    size_t bytes = state_bytes(*b);
+    DBuf<unsigned char> work;
+    work.alloc(bytes);
+    check(hipMemcpy(work.p, u, bytes, hipMemcpyDeviceToDevice), "hipMemcpy");
+    tune(*b, work.p);
     dec::dsync();
     cache_store(cache_path, key, b->shape);
   }
@@ -391,6 +433,7 @@ int dec_batch_heat_steps(dec_batch b, void *u, int32_t nsteps, int32_t use_graph
   if (!b || !u) fail(DEC_ERR_ARG, "dec_batch_heat_steps: NULL argument");
   if (nsteps < 0) fail(DEC_ERR_ARG, "dec_batch_heat_steps: nsteps must be >= 0");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  ensure_tmp(*b);
   if (!use_graph) {
     run(*b, u, nsteps, s);
     dec::check_launch("dec_batch_heat_steps");
@@ -414,6 +457,19 @@ int dec_batch_heat_steps(dec_batch b, void *u, int32_t nsteps, int32_t use_graph
     b->gsteps = nsteps;
   }
   check(hipGraphLaunch(b->gexec, s), "hipGraphLaunch");
+  DEC_CATCH
+}
+
+int dec_batch_heat_steps_noalloc(dec_batch b, void *u, int32_t nsteps, void *stream) {
+  DEC_TRY
+  if (!b || !u) fail(DEC_ERR_ARG, "dec_batch_heat_steps_noalloc: NULL argument");
+  if (nsteps < 0) fail(DEC_ERR_ARG, "dec_batch_heat_steps_noalloc: nsteps must be >= 0");
+  if (state_bytes(*b) > kScratchBytes)
+    fail(DEC_ERR_ARG, "dec_batch_heat_steps_noalloc: the state takes " + std::to_string(state_bytes(*b)) +
+                          " bytes, above the " + std::to_string(kScratchBytes) + "-byte scratch buffer");
+  void *t = scratch_ptr();
+  run_with(*b, u, t, nsteps, static_cast<hipStream_t>(stream));
+  dec::check_launch("dec_batch_heat_steps_noalloc");
   DEC_CATCH
 }
 

@@ -365,6 +365,16 @@ int dec_batch_tune(dec_batch b, void *u, const char *cache_path, int32_t *block,
  * step; use_graph=1 captures the nsteps launches in a hipGraph (kept for the
  * same state pointer and step count) and replays it. */
 int dec_batch_heat_steps(dec_batch b, void *u, int32_t nsteps, int32_t use_graph, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the steps of dec_batch_heat_steps, same kernels and
+ * results, with the ping-pong partner of u in a fixed 32 MiB device array of
+ * the library instead of a library allocation, and no graph of its own
+ * (callers may capture the call).  Calls no hipMalloc/hipFree.  States above
+ * 32 MiB (b's total rows times 8 bytes in float64, 4 in float32) are rejected
+ * with DEC_ERR_ARG.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other threads) race. */
+int dec_batch_heat_steps_noalloc(dec_batch b, void *u, int32_t nsteps, void *stream);
 int dec_batch_destroy(dec_batch b);
 
 #ifdef __cplusplus
