@@ -304,20 +304,53 @@ def poisson(torch, cs, base_k=3, levels=5):
     f = sh_rhs(pts, h0.cpu().numpy(), seed=2024)
     fd = torch.tensor(f, device="cuda")
     out = {"setup_s": setup, "levels": f"{base_k}->{base_k + levels}", "n_fine": fine.nv}
-    for variant in ("plain", "mixed"):
-        cs.solve(md, fd, tol=1e-8, variant=variant)       # warm-up (allocations, graph capture)
-        torch.cuda.synchronize()
-        dt = None
-        for _ in range(3):
-            t = time.perf_counter()
-            u, cyc, rr = cs.solve(md, fd, tol=1e-8, variant=variant)
+    cyc_bytes = {"plain": vcycle_bytes(md, 8), "mixed": vcycle_bytes(md, 4)}
+    out["bytes_per_cycle_f64"] = cyc_bytes["plain"]
+    out["bytes_per_cycle_f32"] = cyc_bytes["mixed"]
+    for alg, tag in ((cs.block_gauss_seidel, ""), (cs.gauss_seidel, "_multicolour_gs")):
+        for variant in ("plain", "mixed"):
+            cs.solve(md, fd, tol=1e-8, variant=variant, alg=alg)   # warm-up (allocations, graph capture)
             torch.cuda.synchronize()
-            e = time.perf_counter() - t
-            dt = e if dt is None else min(dt, e)
-        key = "plain_f64" if variant == "plain" else "mixed_f32_ir"
-        out[key] = {"seconds": dt, "cycles": cyc, "relres_f64": rr}
+            dt = None
+            for _ in range(3):
+                t = time.perf_counter()
+                u, cyc, rr = cs.solve(md, fd, tol=1e-8, variant=variant, alg=alg)
+                torch.cuda.synchronize()
+                e = time.perf_counter() - t
+                dt = e if dt is None else min(dt, e)
+            key = ("plain_f64" if variant == "plain" else "mixed_f32_ir") + tag
+            gbps = cyc * cyc_bytes[variant] / dt / 1e9
+            out[key] = {"seconds": dt, "cycles": cyc, "relres_f64": rr, "smoother": alg.name,
+                        "cycle_GBps": gbps, "frac_hbm_peak": gbps / 8000.0}
     out["speedup_mixed_vs_plain"] = out["plain_f64"]["seconds"] / out["mixed_f32_ir"]["seconds"]
     return out, md, fd
+
+
+def vcycle_bytes(md, s):
+    """Algorithmic bytes of one V(2,2) cycle on levels above the dense coarse
+    solve, s = 8 (float64) or 4 (float32), int32 indices: per level four
+    sweeps (matrix entries, b, the iterate in and out, the row index), the
+    residual (matrix, b, u, r, row index), the restriction (R entries, r, b_c,
+    the zeroed u_c) and the prolongation (P entries, z_c, u in and out); the
+    coarsest level's dense solve (n² entries, r, u)."""
+    total = 0
+    nl = md.nlevels
+    for lv in range(nl):
+        n = md.sizes[lv]
+        rp, _, _ = md.export(lv, "A")
+        nnz = int(rp[-1])
+        if lv == nl - 1:
+            total += n * n * s + 2 * n * s
+            continue
+        nc = md.sizes[lv + 1]
+        rnnz = int(md.export(lv, "R")[0][-1])
+        pnnz = int(md.export(lv, "P")[0][-1])
+        sweep = nnz * (4 + s) + n * (3 * s + 4)
+        total += 4 * sweep
+        total += nnz * (4 + s) + n * (3 * s + 4)
+        total += rnnz * (4 + s) + n * s + 2 * nc * s
+        total += pnnz * (4 + s) + nc * s + 2 * n * s
+    return total
 
 
 def mg_reference_calls(torch, cs, fd, base_k=3, levels=5, reps=3):
@@ -509,22 +542,54 @@ def cpu_baseline(k=8, budget_s=8.0):
                       f"{rates[1][1]} passes in {rates[1][2]:.1f} s at 1 thread; set-up {setup:.1f} s"}
 
 
-def batched_heat(torch, cs, ninst=64, nsteps=100, seed=7):
-    """64 explicit heat instances u ← u - dt L u on a seeded uneven mix of
-    icosphere levels 4-7, FP32, in one persistent launch per step replayed
-    from a hipGraph, launch shape from the autotuner; checked against the same
-    steps in FP64."""
+def heat_instances(torch, cs, ninst=64, seed=7):
+    """The batch of the batched leg: a seeded uneven mix of icosphere levels
+    4-7, each instance on its own mesh (the level's points moved tangentially
+    by a seeded jitter of up to 5% of the mean edge length and put back on the
+    unit sphere), so no two instances share a Laplacian; seeded uniform states."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
     levels = [int(x) for x in rng.choice([4, 5, 6, 7], size=ninst, p=[0.4, 0.3, 0.2, 0.1])]
-    meshes = {k: cs.dualize(cs.icosphere(k)) for k in sorted(set(levels))}
-    inst = [meshes[k] for k in levels]
+    base = {k: cs.icosphere(k).export() for k in sorted(set(levels))}
+    inst = []
+    for i, k in enumerate(levels):
+        e = base[k]
+        p = e["points"].copy()
+        h = np.linalg.norm(p[e["v0"]] - p[e["v1"]], axis=1).mean()
+        p += np.random.default_rng(1000 + i).uniform(-0.05, 0.05, p.shape) * h
+        p /= np.linalg.norm(p, axis=1, keepdims=True)
+        m = cs.mesh_from_arrays(p, e["v0"], e["v1"], e["e0"], e["e1"], e["e2"], e["edge_orientation"],
+                                e["tri_orientation"])
+        inst.append(cs.dualize(m))
     b32 = cs.HeatBatch(inst, torch.float32)
     b64 = cs.HeatBatch(inst, torch.float64)
     u64 = torch.empty(b64.total, dtype=torch.float64, device="cuda")
     for i in range(ninst):
         cs.fill_uniform(b64.instance(u64, i), 500 + i)
+    return levels, inst, b32, b64, u64
+
+
+def heat_step_bytes(levels, s):
+    """Algorithmic bytes of one batched step: each instance's Laplacian rows
+    (nnz = nv + 2 ne entries of int32 column + value) and its state read and
+    written once."""
+    total = 0
+    for k in levels:
+        nv, ne = 10 * 4 ** k + 2, 30 * 4 ** k
+        total += (nv + 2 * ne) * (4 + s) + 2 * nv * s
+    return total
+
+
+def batched_heat(torch, cs, ninst=64, nsteps=100, seed=7, counters=True):
+    """64 explicit heat instances u ← u - dt L u on a seeded uneven mix of
+    icosphere levels 4-7 (one mesh per instance), FP32, in one persistent
+    launch per step replayed from a hipGraph, launch shape from the autotuner;
+    checked against the same steps in FP64.  The working set (tables and
+    state, about 104 MB) fits the 256 MB Infinity Cache, so across steps the
+    tables come back from that cache: the rate is of L2-miss traffic, and the
+    counter pass gives the bytes the L2 actually fetched and wrote."""
+    levels, inst, b32, b64, u64 = heat_instances(torch, cs, ninst, seed)
     u32 = u64.float()
     t = time.perf_counter()
     shape = b32.tune(u32.clone())
@@ -544,14 +609,22 @@ def batched_heat(torch, cs, ninst=64, nsteps=100, seed=7):
     b64.steps(v, nsteps, graph=True)
     err = max(float((b32.instance(u, i).double() - b64.instance(v, i)).norm() / b64.instance(v, i).norm())
               for i in range(ninst))
-    step_bytes = 0
-    for k in levels:
-        nv, ne = 10 * 4 ** k + 2, 30 * 4 ** k
-        step_bytes += (nv + 2 * ne) * (4 + 4) + 2 * nv * 4
-    return {"instances": ninst, "levels": {str(k): levels.count(k) for k in sorted(set(levels))},
-            "total_vertices": int(b32.total), "steps": nsteps, "ms_per_step": ms / nsteps,
-            "instance_steps_per_s": ninst * nsteps / (ms * 1e-3), "GBps": step_bytes / (ms / nsteps * 1e-3) / 1e9,
-            "launch_shape": shape, "tune_s": tune_s, "max_rel_err_f32_vs_f64": err}
+    step_bytes = heat_step_bytes(levels, 4)
+    out = {"instances": ninst, "meshes": "one jittered icosphere per instance (no shared tables)",
+           "levels": {str(k): levels.count(k) for k in sorted(set(levels))},
+           "total_vertices": int(b32.total), "steps": nsteps, "ms_per_step": ms / nsteps,
+           "instance_steps_per_s": ninst * nsteps / (ms * 1e-3), "step_bytes": step_bytes,
+           "GBps": step_bytes / (ms / nsteps * 1e-3) / 1e9, "working_set_MB": step_bytes / 1e6,
+           "launch_shape": shape, "tune_s": tune_s, "max_rel_err_f32_vs_f64": err}
+    if counters:
+        try:
+            t = traffic("heat", "k_heat_step", 0, 1)
+            out["counter_bytes_per_step"] = t["bytes"]
+            out["counter_GBps"] = t["bytes"] / (ms / nsteps * 1e-3) / 1e9
+            out["counter_over_algorithmic"] = t["bytes"] / step_bytes
+        except RuntimeError as e:
+            out["counter_error"] = str(e)[:200]
+    return out
 
 
 COPY_MIB = 256  # calibration kernel (elementwise negation): 256 MiB read + 256 MiB written
@@ -571,6 +644,13 @@ def pmc_probe(op_name, k, sets, launches):
         torch.neg(x, out=y)     # an elementwise kernel: reads and writes COPY_MIB MiB
     torch.cuda.synchronize()
     del x, y
+    if op_name == "heat":
+        levels, inst, b32, b64, u64 = heat_instances(torch, cs)
+        u = u64.float()
+        b32.tune(u.clone())
+        b32.steps(u, launches, graph=False)
+        torch.cuda.synchronize()
+        return
     mesh = cs.icosphere(k)
     cs.dualize(mesh)
     if op_name == "group":
@@ -776,7 +856,7 @@ def main():
               flush=True)
         del md, fd
     if not args.no_batch:
-        extra["batched_heat"] = batched_heat(torch, cs)
+        extra["batched_heat"] = batched_heat(torch, cs, counters=not args.no_traffic)
         print(f"# batched heat: {json.dumps(extra['batched_heat'])}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
 

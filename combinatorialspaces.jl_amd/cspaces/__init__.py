@@ -14,7 +14,7 @@ from .meshio import (  # noqa: F401
     load_json_acset, load_mesh, read_json_acset, read_mesh, read_obj, read_stl, save_obj, write_json_acset, write_obj)
 from .multigrid import (  # noqa: F401
     BinarySubdivision, DirectMode, GalerkinMode, MGData, MultigridData, cg, dec_Delta_inv, full_multigrid,
-    gauss_seidel, jacobi,
+    gauss_seidel, block_gauss_seidel, jacobi,
     multigrid_mu_cycles, multigrid_vcycles, multigrid_wcycles, solve)
 from .batch import HeatBatch  # noqa: F401
 from .learned import LearnedSmoother, learned_stats, make_pairs, sh_basis  # noqa: F401

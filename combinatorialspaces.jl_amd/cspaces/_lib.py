@@ -19,7 +19,7 @@ DEC_GEOMETRIC_HODGE, DEC_DIAGONAL_HODGE = 0, 1
  DEC_OP_WEDGE_DP, DEC_OP_INTERIOR_PRODUCT_DD, DEC_OP_LIE_DD, DEC_OP_DUAL_LAPLACIAN,
  DEC_OP_P2_D2_INTERPOLATION, DEC_OP_P3_D3_INTERPOLATION, DEC_OP_D0_P0_INTERPOLATION,
  DEC_OP_P0_D0_INTERPOLATION) = range(1, 20)
-DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI = 0, 1, 2
+DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI, DEC_SMOOTH_BLOCK_GS = 0, 1, 2, 3
 DEC_MG_DIRECT, DEC_MG_GALERKIN = 0, 1
 DEC_SOLVE_PLAIN_F64, DEC_SOLVE_MIXED_IR = 0, 1
 
@@ -61,11 +61,13 @@ SIGNATURES = {
     "dec_op_destroy": (c_int, [P]),
     "dec_mg_create": (c_int, [P, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(P)]),
     "dec_mg_create_mode": (c_int, [P, c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(P)]),
+    "dec_mg_create_from_csr": (c_int, [c_int32, P] + [P] * 9 + [c_int32, c_int32, POINTER(P)]),
     "dec_mg_nlevels": (c_int, [P, PI32]),
     "dec_mg_level_size": (c_int, [P, c_int32, PI32]),
     "dec_mg_finest_mesh": (c_int, [P, POINTER(P)]),
     "dec_mg_export_level": (c_int, [P, c_int32, c_int32, PI64, P, P, P]),
     "dec_mg_export_colors": (c_int, [P, c_int32, P, PI32]),
+    "dec_mg_export_blocks": (c_int, [P, c_int32, P, PI32]),
     "dec_mg_set_smoother": (c_int, [P, c_int32, c_int32, c_double, c_int32]),
     "dec_mg_cycles": (c_int, [P, c_int32, c_int32, P, P, P]),
     "dec_mg_full_multigrid": (c_int, [P, c_int32, c_int32, P, P, P]),

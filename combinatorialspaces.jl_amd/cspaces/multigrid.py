@@ -5,7 +5,8 @@ steps)` in DirectMode, `multigrid_vcycles`, `multigrid_wcycles`,
 `full_multigrid`, and FastDEC's `dec_Δ⁻¹` (here `dec_Delta_inv`).  `op` is the
 0-form Laplace-Beltrami operator (`fast_laplace_beltrami`) with the chosen
 Hodge star for ⋆1.  `alg` selects the smoother: `cg` (Krylov.jl's cg, the
-reference default), `gauss_seidel` (multicolour) or `jacobi`.
+reference default), `gauss_seidel` (multicolour), `block_gauss_seidel`
+(hybrid block GS) or `jacobi`.
 """
 from __future__ import annotations
 
@@ -34,6 +35,10 @@ class _Smoother:
 cg = _Smoother("cg", L.DEC_SMOOTH_CG)
 gauss_seidel = _Smoother("gauss_seidel", L.DEC_SMOOTH_GS)
 jacobi = _Smoother("jacobi", L.DEC_SMOOTH_JACOBI)
+# hybrid block Gauss-Seidel (this library's GPU smoother): multicolour GS
+# inside blocks of the level, neighbours in other blocks from the start of the
+# sweep; one launch per sweep
+block_gauss_seidel = _Smoother("block_gauss_seidel", L.DEC_SMOOTH_BLOCK_GS)
 
 
 class DirectMode:
@@ -69,6 +74,45 @@ class MultigridData:
             check(lib().dec_mg_level_size(h, lv, ctypes.byref(n)))
             self.sizes.append(n.value)
         self._alg = None
+
+    @classmethod
+    def from_matrices(cls, operators, restrictions, prolongations, steps: int = 3):
+        """MultigridData(ops, rs, ps, steps) (Multigrid.jl:545-546): a hierarchy
+        from host matrices, finest first.  Each matrix is a scipy sparse
+        matrix or a CSR triple (rowptr, col, val), 0-based."""
+        import ctypes as C
+
+        def csr(M):
+            if hasattr(M, "tocsr"):
+                M = M.tocsr()
+                return (np.ascontiguousarray(M.indptr, np.int32), np.ascontiguousarray(M.indices, np.int32),
+                        np.ascontiguousarray(M.data, np.float64))
+            rp, ci, vv = M
+            return (np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32),
+                    np.ascontiguousarray(vv, np.float64))
+
+        A = [csr(M) for M in operators]
+        R = [csr(M) for M in restrictions]
+        P = [csr(M) for M in prolongations]
+        nl = len(A)
+        if len(R) != nl - 1 or len(P) != nl - 1:
+            raise ValueError("need one restriction and one prolongation per pair of levels")
+        sizes = np.array([len(a[0]) - 1 for a in A], np.int32)
+
+        def arr(ms, j):
+            return (C.c_void_p * max(len(ms), 1))(*[m[j].ctypes.data for m in ms])
+
+        h = c_void_p()
+        check(lib().dec_mg_create_from_csr(nl, ptr(sizes), arr(A, 0), arr(A, 1), arr(A, 2), arr(R, 0), arr(R, 1),
+                                           arr(R, 2), arr(P, 0), arr(P, 1), arr(P, 2), 0, steps, ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self.steps = steps
+        self.nlevels = nl
+        self.sizes = [int(x) for x in sizes]
+        self._alg = None
+        self._keep = (A, R, P)
+        return self
 
     def __len__(self):
         return self.nlevels
@@ -115,6 +159,14 @@ class MultigridData:
         c = np.zeros(self.sizes[level], np.int32)
         k = c_int32()
         check(lib().dec_mg_export_colors(self._h, level, ptr(c), ctypes.byref(k)))
+        return c, k.value
+
+    def blocks(self, level: int):
+        """Block of each vertex of operators[level] for the hybrid block
+        Gauss-Seidel smoother, and the number of blocks."""
+        c = np.zeros(self.sizes[level], np.int32)
+        k = c_int32()
+        check(lib().dec_mg_export_blocks(self._h, level, ptr(c), ctypes.byref(k)))
         return c, k.value
 
 
@@ -174,7 +226,7 @@ def dec_Delta_inv(md: MultigridData, steps=3, cycles=5, alg=cg, mu=2):
     return lambda b: full_multigrid(b, md, cycles, alg, mu, steps=steps)
 
 
-def solve(md: MultigridData, b, u=None, tol=1e-8, max_cycles=200, variant="plain", alg=gauss_seidel,
+def solve(md: MultigridData, b, u=None, tol=1e-8, max_cycles=200, variant="plain", alg=block_gauss_seidel,
           steps=2, coarse_direct=True, stream=None):
     """Solve A u = b to ‖b - A u‖₂/‖b‖₂ ≤ tol (checked in float64) with float64
     V-cycles ("plain") or float32 V-cycles inside float64 iterative refinement

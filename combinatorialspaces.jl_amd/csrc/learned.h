@@ -19,18 +19,24 @@ struct NnLevel {
     DBuf<float> val;
   } a, at, p, pt, r, rt, pr, prt;
   DBuf<float> D;
-  NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr);
+  // P and D over the multigrid level's own vertex numbering (perm[i] = vertex
+  // of internal number i), for the smoother inside the V-cycles
+  Op pi;
+  DBuf<float> Di;
+  NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr, const std::vector<int32_t> *perm);
 };
 
-// u += N(r) for one residual r (both float, vertex order) on level `lvl` of
-// the network's hierarchy.
-void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s);
+// u += N(r) for one residual r (both float) on level `lvl` of the network's
+// hierarchy: in the multigrid level's own numbering (internal), else in
+// vertex order.
+void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s, bool internal = true);
 // Check that the network was built for a hierarchy with these level sizes
 // (finest first) and size its inference buffers for the largest level.
 void nn_bind(dec_nn_s &nn, const std::vector<int> &sizes);
 // ops[l] = (A_l, R_l, P_l), R and P NULL on the coarsest level.
 struct NnOps {
   const HostCsr *A, *R, *P;
+  const std::vector<int32_t> *perm;  // the level's numbering (NULL: vertex order)
 };
 dec_nn_s *nn_create(const std::vector<NnOps> &ops, uint64_t seed);
 

@@ -1198,7 +1198,7 @@ static HostCsr transpose(const HostCsr &A) {
   return T;
 }
 
-NnLevel::NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr) {
+NnLevel::NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr, const std::vector<int32_t> *perm) {
   n = A.m;
   HostCsr P;
   P.m = P.n = n;
@@ -1223,6 +1223,27 @@ NnLevel::NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr) {
   to_ell(A, a);
   to_ell(transpose(A), at);
   to_ell(P, p);
+  {
+    // P's rows renumbered (rows perm[i], columns through the inverse), slots
+    // in P's order, so each row's sum is the vertex-order row's
+    std::vector<int32_t> iperm(n);
+    for (int i = 0; i < n; i++) iperm[perm ? (*perm)[i] : i] = i;
+    HostCsr Q;
+    Q.m = Q.n = n;
+    Q.ptr.assign(n + 1, 0);
+    std::vector<float> di(n);
+    for (int i = 0; i < n; i++) {
+      int v = perm ? (*perm)[i] : i;
+      di[i] = d[v];
+      for (int k = P.ptr[v]; k < P.ptr[v + 1]; k++) {
+        Q.col.push_back(iperm[P.col[k]]);
+        Q.val.push_back(P.val[k]);
+      }
+      Q.ptr[i + 1] = static_cast<int32_t>(Q.col.size());
+    }
+    to_ell(Q, pi);
+    Di.upload(di);
+  }
   to_ell(transpose(P), pt);
   if (R && Pr) {
     nc = R->m;
@@ -1441,9 +1462,13 @@ void nn_bind(dec_nn_s &nn, const std::vector<int> &sizes) {
   reserve_inference(nn, nmax);
 }
 
-void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s) {
-  const NnLevel &L = level_of(nn, lvl);
-  int n = L.n;
+void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s, bool internal) {
+  const NnLevel &NL = level_of(nn, lvl);
+  struct {
+    const NnLevel::Op &p;
+    const DBuf<float> &D;
+  } L{internal ? NL.pi : NL.p, internal ? NL.Di : NL.D};
+  int n = NL.n;
   reserve_inference(nn, n);
   hipLaunchKernelGGL(k_scale_one, dim3(kScaleBlocks), dim3(TB), 0, s, n, r, nn.spart.p, nn.ticket.p, nn.s.p);
   DEC_ELL_WIDTH_SWITCH(L.p.width, hipLaunchKernelGGL((k_inf_input<W>), dim3(dec::nblocks(n, TB)), dim3(TB), 0, s, n,
@@ -1469,7 +1494,7 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s) {
 
 dec_nn_s *nn_create(const std::vector<NnOps> &ops, uint64_t seed) {
   auto nn = std::make_unique<dec_nn_s>();
-  for (const NnOps &o : ops) nn->levels.push_back(std::make_unique<NnLevel>(*o.A, o.R, o.P));
+  for (const NnOps &o : ops) nn->levels.push_back(std::make_unique<NnLevel>(*o.A, o.R, o.P, o.perm));
   // initialisation: W ~ N(0, 2/fan_in) (seeded), biases and output head zero,
   // alpha = 2/3: the untrained network is damped Jacobi
   std::vector<float> th(NP, 0.0f);
@@ -1532,7 +1557,7 @@ int dec_nn_forward(dec_nn nn, int32_t level, int32_t nsamples, const float *r, f
 int dec_nn_smooth(dec_nn nn, int32_t level, const float *r, float *u, void *stream) {
   DEC_TRY
   if (!nn || !r || !u) fail(DEC_ERR_ARG, "dec_nn_smooth: bad argument");
-  dec::nn_smooth(*nn, level, r, u, static_cast<hipStream_t>(stream));
+  dec::nn_smooth(*nn, level, r, u, static_cast<hipStream_t>(stream), false);
   dec::check_launch("dec_nn_smooth");
   DEC_CATCH
 }

@@ -25,6 +25,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kDotBlocks = 256;
+constexpr int kNormSlots = 4;  // residual norms of a solve in flight (at most 2 are)
 
 // ---------------------------------------------------------------- kernels
 
@@ -104,21 +105,94 @@ __global__ void k_restrict(int nc, int width, const int *__restrict__ col, const
   if (uc) uc[i] = static_cast<T>(0);
 }
 
-// u_f += P z_c with P = transpose(S): coarse vertices copy, midpoints average
-// their edge's endpoints (rows (min, max) of S's column).
-template <class T>
-__global__ void k_prolong_add(int nf, int nc, const int *__restrict__ pa, const int *__restrict__ pb,
+// u_f += P z_c, P in ELL rows (for a binary subdivision P = transpose(S):
+// a coarse vertex copies its value with weight 1, a midpoint takes 0.5 of its
+// edge's endpoints in the order (min, max) of S's column).
+template <int W, class T>
+__global__ void k_prolong_add(int nf, int width, const int *__restrict__ pcol, const T *__restrict__ pval,
                               const T *__restrict__ zc, T *__restrict__ uf) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nf) return;
-  T pz;
-  if (j < nc) {
-    pz = zc[j];
-  } else {
-    int e = j - nc;
-    pz = static_cast<T>(0.5) * zc[pa[e]] + static_cast<T>(0.5) * zc[pb[e]];
+  uf[j] = uf[j] + dec_ell::row<W, T, T>(nf, width, j, pcol, pval, zc);
+}
+
+// One hybrid block Gauss-Seidel sweep.  The level's vertices are numbered so
+// that every block is a contiguous range [bstart[k], bstart[k+1]); the ELL rows
+// are sorted by (colour, vertex), so the rows of colour c in block k are the
+// positions [seg[c*(nb+1)+k], seg[c*(nb+1)+k+1]).  One workgroup per block
+// stages the block's iterate in LDS and runs the colours in order (reversed
+// for the post-smoother) with a barrier between colours; a neighbour in
+// another block enters with its value in uin (the iterate at the start of the
+// sweep).  The result goes to uout, so no workgroup reads a value another one
+// writes in the same launch.  Per row: the sum over the off-diagonal entries
+// in slot (ascending column) order, then (b - sum) / diagonal.
+template <int W, class T>
+__global__ void __launch_bounds__(256)
+    k_block_gs(int n, int width, int ncolors, int nb, const int *__restrict__ col, const T *__restrict__ val,
+               const int *__restrict__ rowid, const int *__restrict__ bstart, const int *__restrict__ seg,
+               const T *__restrict__ b, const T *__restrict__ uin, T *__restrict__ uout, int reverse) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *x = reinterpret_cast<T *>(smem_raw);
+  int k = blockIdx.x;
+  int r0 = bstart[k], nr = bstart[k + 1] - r0;
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) x[i] = uin[r0 + i];
+  __syncthreads();
+  for (int ci = 0; ci < ncolors; ci++) {
+    int c = reverse ? ncolors - 1 - ci : ci;
+    int p0 = seg[c * (nb + 1) + k], p1 = seg[c * (nb + 1) + k + 1];
+    for (int p = p0 + static_cast<int>(threadIdx.x); p < p1; p += blockDim.x) {
+      int v = rowid[p];
+      T s = static_cast<T>(0), d = static_cast<T>(0);
+      if constexpr (W > 0) {
+        int cc[W];
+        T vv[W], xv[W];
+#pragma unroll
+        for (int q = 0; q < W; q++) {
+          size_t o = static_cast<size_t>(q) * n + p;
+          cc[q] = col[o];
+          vv[q] = val[o];
+        }
+#pragma unroll
+        for (int q = 0; q < W; q++) {
+          unsigned loc = static_cast<unsigned>(cc[q] - r0);
+          xv[q] = (cc[q] < 0 || loc < static_cast<unsigned>(nr)) ? static_cast<T>(0) : uin[cc[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < W; q++) {
+          if (cc[q] < 0) continue;
+          unsigned loc = static_cast<unsigned>(cc[q] - r0);
+          if (cc[q] == v) d = vv[q];
+          else s = s + vv[q] * (loc < static_cast<unsigned>(nr) ? x[loc] : xv[q]);
+        }
+      } else {
+        for (int q = 0; q < width; q++) {
+          size_t o = static_cast<size_t>(q) * n + p;
+          int cq = col[o];
+          if (cq < 0) break;
+          unsigned loc = static_cast<unsigned>(cq - r0);
+          if (cq == v) d = val[o];
+          else s = s + val[o] * (loc < static_cast<unsigned>(nr) ? x[loc] : uin[cq]);
+        }
+      }
+      x[v - r0] = (b[v] - s) / d;
+    }
+    __syncthreads();
   }
-  uf[j] = uf[j] + pz;
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) uout[r0 + i] = x[i];
+}
+
+// Renumbering between the caller's vertex order and the hierarchy's own:
+// perm[i] is the caller's index of internal vertex i.
+template <class T>
+__global__ void k_perm_gather(int n, const int *__restrict__ perm, const T *__restrict__ src, T *__restrict__ dst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+
+template <class T>
+__global__ void k_perm_scatter(int n, const int *__restrict__ perm, const T *__restrict__ src, T *__restrict__ dst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm[i]] = src[i];
 }
 
 // u += Z r with Z stored column-major (ZT[j * n + i] = Z[i, j]), in two
@@ -309,16 +383,16 @@ __global__ void k_add_f2d(int n, const float *__restrict__ c, double *__restrict
 // expressions, same order).  One workgroup streams at a small fraction of
 // the chip's bandwidth, so this pays only for levels of a few thousand rows.
 constexpr int kFuseRowsDefault = 3000;
-constexpr int kFuseRowsMax = 16384;
+constexpr int kFuseRowsMax = 4096;  // levels up to this size are one block (LDS)
 constexpr int kFuseThreads = 1024;
 constexpr int kMaxColors = 16;
 
 template <class T>
 struct LevelDesc {
-  int n, width, ncolors, nc, rwidth;
+  int n, width, ncolors, nc, rwidth, pwidth;
   int cstart[kMaxColors + 1];
-  const int *col, *rowid, *rcol, *pa, *pb;
-  const T *val, *rval;
+  const int *col, *rowid, *rcol, *pcol;
+  const T *val, *rval, *pval;
   T *u, *b, *r;
 };
 
@@ -331,7 +405,9 @@ __device__ __forceinline__ T fused_row(const LevelDesc<T> &L, int p, const T *x)
 template <class T>
 __device__ void fused_smooth(const LevelDesc<T> &L, const T *b, T *u, int steps, int alg, T omega, bool post) {
   for (int it = 0; it < steps; it++) {
-    if (alg == DEC_SMOOTH_GS) {
+    if (alg == DEC_SMOOTH_GS || alg == DEC_SMOOTH_BLOCK_GS) {
+      // a level in the fused tail is one block: the hybrid sweep is the
+      // multicolour sweep
       for (int ci = 0; ci < L.ncolors; ci++) {
         int c = post ? L.ncolors - 1 - ci : ci;
         for (int p = L.cstart[c] + threadIdx.x; p < L.cstart[c + 1]; p += blockDim.x) {
@@ -412,16 +488,9 @@ __global__ void __launch_bounds__(kFuseThreads)
     const LevelDesc<T> &C = Ls[l + 1];
     T *u = l == l0 ? u0 : L.u;
     const T *b = l == l0 ? b0 : L.b;
-    for (int j = threadIdx.x; j < L.n; j += blockDim.x) {
-      T pz;
-      if (j < L.nc) {
-        pz = C.u[j];
-      } else {
-        int e = j - L.nc;
-        pz = static_cast<T>(0.5) * C.u[L.pa[e]] + static_cast<T>(0.5) * C.u[L.pb[e]];
-      }
-      u[j] = u[j] + pz;
-    }
+    for (int j = threadIdx.x; j < L.n; j += blockDim.x)
+      u[j] = u[j] + (L.pwidth == 2 ? dec_ell::row<2, T, T>(L.n, L.pwidth, j, L.pcol, L.pval, C.u)
+                                   : dec_ell::row<0, T, T>(L.n, L.pwidth, j, L.pcol, L.pval, C.u));
     __syncthreads();
     fused_smooth(L, b, u, steps, alg, omega, true);
   }
@@ -433,25 +502,43 @@ template <class T>
 struct LevelVals {
   DBuf<T> val;   // ELL values of A (colour-sorted rows)
   DBuf<T> rval;  // ELL values of R (this level -> next coarser)
+  DBuf<T> pval;  // ELL values of P (next coarser -> this level)
   DBuf<T> Z;     // dense coarse operator, column-major (coarsest level, direct solve)
   DBuf<T> zpart; // chunk partial products of the direct solve (coarsest level)
   DBuf<T> u, b, r, t;  // work vectors
 };
 
+// Every level keeps its own vertex numbering (internal numbers): vertices are
+// grouped into blocks grown breadth-first over the operator's graph, each
+// block a contiguous range, so that a block is a compact patch of the mesh
+// (few neighbours outside it) and its vectors are contiguous.  The caller's
+// order is restored at the boundary (solve, cycles, full multigrid); host
+// copies and exports stay in the caller's order.  A level of at most
+// fuse_rows rows is one block and keeps the caller's order.
 struct Level {
   int n = 0, width = 0;
-  DBuf<int32_t> col, rowid;
+  DBuf<int32_t> col, rowid;  // ELL of A over internal numbers, rows sorted by (colour, vertex)
   std::vector<int> cstart;  // colour ranges of positions
   int ncolors = 0;
-  // restriction to the next coarser level
+  // restriction to the next coarser level (rows: coarse internal numbers)
   int nc = 0, rwidth = 0;
   DBuf<int32_t> rcol;
-  DBuf<int32_t> pa, pb;  // prolongation from the next coarser level (midpoint endpoints)
+  // prolongation from the next coarser level (rows: this level's internal numbers)
+  int pwidth = 0;
+  DBuf<int32_t> pcol;
   LevelVals<double> d;
   LevelVals<float> f;
-  dec::HostCsr A, R, P;  // host copies for export
-  std::vector<int32_t> color;  // Gauss-Seidel colour of each vertex
+  dec::HostCsr A, R, P;  // host copies for export (caller's numbering)
+  std::vector<int32_t> color;  // Gauss-Seidel colour of each vertex (caller's numbering)
   std::vector<double> Zh;
+  // numbering: perm[i] = caller's index of internal vertex i, iperm its inverse
+  std::vector<int32_t> perm, iperm;
+  bool identity = true;
+  DBuf<int32_t> dperm;
+  // hybrid block Gauss-Seidel: block ranges and (colour, block) ELL segments
+  int nblk = 0, bmax = 0, seg_threads = 64;
+  DBuf<int32_t> bstart, seg;
+  std::vector<int32_t> hbstart;
 };
 
 }  // namespace
@@ -476,6 +563,9 @@ struct dec_mg_s {
   int gkey[2] = {-1, -1};
   hipStream_t cap = nullptr;
   bool use_graph = true;
+  // residual norms of a solve on their way to the host (pinned ring + events)
+  double *hnorm = nullptr;
+  hipEvent_t nev[kNormSlots] = {};
   // learned smoother (float32 cycles, levels of learned_min_rows to
   // learned_max_rows rows; learned_max_rows <= 0: no upper bound)
   dec_nn_s *nn = nullptr;
@@ -488,6 +578,9 @@ struct dec_mg_s {
       if (graph[i]) (void)hipGraphDestroy(graph[i]);
     }
     if (cap) (void)hipStreamDestroy(cap);
+    for (auto &e : nev)
+      if (e) (void)hipEventDestroy(e);
+    if (hnorm) (void)hipHostFree(hnorm);
   }
 };
 
@@ -523,11 +616,73 @@ void adjacency(const dec::HostCsr &A, std::vector<int32_t> &deg, std::vector<int
       }
 }
 
-// Colour-sorted ELL of a host CSR (columns ascending per row).  `color` is the
-// proposed Gauss-Seidel colouring (empty: none); a proposal that is not a
-// proper colouring of A's pattern is replaced by greedy colours.  On return
-// `color` holds the colouring used.
-void build_level_matrix(Level &L, const dec::HostCsr &A, std::vector<int32_t> &color) {
+// Blocks of about B vertices grown breadth-first over the symmetric graph
+// (deg, adj) of a level's operator: a block grows from its seed until it
+// holds B vertices, and the next block is seeded next to it (from the
+// unassigned neighbours of the last block's BFS frontier, else the lowest
+// unassigned vertex), so consecutive blocks are neighbours.  perm lists the
+// vertices block by block in BFS order; bstart holds the block ranges.
+void grow_blocks(int n, const std::vector<int32_t> &deg, const std::vector<int32_t> &adj, int B,
+                 std::vector<int32_t> &perm, std::vector<int32_t> &bstart) {
+  std::vector<int32_t> blk(n, -1);
+  perm.clear();
+  perm.reserve(n);
+  bstart.assign(1, 0);
+  std::vector<int32_t> seeds, queue;
+  size_t sh = 0;
+  int next = 0, nb = 0;
+  while (static_cast<int>(perm.size()) < n) {
+    int seed = -1;
+    while (sh < seeds.size() && seed < 0) {
+      int c = seeds[sh++];
+      if (blk[c] < 0) seed = c;
+    }
+    if (seed < 0) {
+      while (next < n && blk[next] >= 0) next++;
+      seed = next;
+    }
+    queue.clear();
+    queue.push_back(seed);
+    blk[seed] = nb;
+    size_t qh = 0;
+    int cnt = 1;
+    while (qh < queue.size() && cnt < B) {
+      int v = queue[qh++];
+      for (int k = deg[v]; k < deg[v + 1] && cnt < B; k++) {
+        int w = adj[k];
+        if (blk[w] < 0) {
+          blk[w] = nb;
+          cnt++;
+          queue.push_back(w);
+        }
+      }
+    }
+    seeds.clear();
+    sh = 0;
+    for (size_t q = qh; q < queue.size(); q++)
+      for (int k = deg[queue[q]]; k < deg[queue[q] + 1]; k++)
+        if (blk[adj[k]] < 0) seeds.push_back(adj[k]);
+    perm.insert(perm.end(), queue.begin(), queue.end());
+    bstart.push_back(static_cast<int32_t>(perm.size()));
+    nb++;
+  }
+}
+
+// Target block size of the hybrid Gauss-Seidel: about 640 blocks (2.5 per
+// CU), 64 to 512 rows each.
+int block_rows(int n) {
+  int b = 64;
+  while (b < 512 && static_cast<long long>(b) * 640 < n) b *= 2;
+  return b;
+}
+
+// Colour-sorted ELL of a host CSR (caller's numbering) over the level's
+// internal numbers.  `color` is the proposed Gauss-Seidel colouring (empty:
+// none); a proposal that is not a proper colouring of A's pattern is replaced
+// by greedy colours.  On return `color` holds the colouring used.  A level of
+// more than `block_min` rows is renumbered into BFS-grown blocks; slots keep
+// the caller's ascending column order, so row sums are those of A.
+void build_level_matrix(Level &L, const dec::HostCsr &A, std::vector<int32_t> &color, int block_min) {
   int n = A.m;
   L.n = n;
   std::vector<int32_t> deg, adj;
@@ -537,12 +692,57 @@ void build_level_matrix(Level &L, const dec::HostCsr &A, std::vector<int32_t> &c
   if (color.empty() || !dec::proper_coloring(n, deg, adj, color, nc)) nc = dec::greedy_colors(n, deg, adj, color);
   L.ncolors = nc;
   L.color = color;
+  std::vector<int32_t> bstart;
+  if (n > block_min) {
+    grow_blocks(n, deg, adj, block_rows(n), L.perm, bstart);
+  } else {
+    L.perm.resize(n);
+    for (int v = 0; v < n; v++) L.perm[v] = v;
+    bstart = {0, n};
+  }
+  L.iperm.assign(n, 0);
+  L.identity = true;
+  for (int i = 0; i < n; i++) {
+    L.iperm[L.perm[i]] = i;
+    L.identity = L.identity && L.perm[i] == i;
+  }
+  L.dperm.upload(L.perm);
+  // positions: rows sorted by (colour, internal number)
   std::vector<int32_t> order(n);
   L.cstart.assign(L.ncolors + 1, 0);
   for (int v = 0; v < n; v++) L.cstart[color[v] + 1]++;
   for (int c = 0; c < L.ncolors; c++) L.cstart[c + 1] += L.cstart[c];
   std::vector<int32_t> pos(L.cstart.begin(), L.cstart.end() - 1);
-  for (int v = 0; v < n; v++) order[pos[color[v]]++] = v;
+  for (int i = 0; i < n; i++) order[pos[color[L.perm[i]]]++] = i;
+  // (colour, block) segments of the positions
+  int nb = static_cast<int>(bstart.size()) - 1;
+  L.nblk = nb;
+  L.bmax = 0;
+  for (int k = 0; k < nb; k++) L.bmax = std::max(L.bmax, bstart[k + 1] - bstart[k]);
+  std::vector<int32_t> seg(static_cast<size_t>(L.ncolors) * (nb + 1));
+  {
+    std::vector<int32_t> blk_of(n);
+    for (int k = 0; k < nb; k++)
+      for (int i = bstart[k]; i < bstart[k + 1]; i++) blk_of[i] = k;
+    std::vector<int32_t> cnt(static_cast<size_t>(L.ncolors) * nb, 0);
+    for (int i = 0; i < n; i++) cnt[static_cast<size_t>(color[L.perm[i]]) * nb + blk_of[i]]++;
+    for (int c = 0; c < L.ncolors; c++) {
+      int p = L.cstart[c];
+      for (int k = 0; k < nb; k++) {
+        seg[static_cast<size_t>(c) * (nb + 1) + k] = p;
+        p += cnt[static_cast<size_t>(c) * nb + k];
+      }
+      seg[static_cast<size_t>(c) * (nb + 1) + nb] = p;
+    }
+    // threads per block of the sweep kernel: one per row of the largest
+    // (colour, block) segment, in whole wavefronts, at most 256
+    int smax = 0;
+    for (int c : cnt) smax = std::max(smax, c);
+    L.seg_threads = std::min(256, std::max(64, (smax + 63) / 64 * 64));
+  }
+  L.bstart.upload(bstart);
+  L.seg.upload(seg);
+  L.hbstart = bstart;
   int width = 0;
   for (int r = 0; r < n; r++) width = std::max(width, A.ptr[r + 1] - A.ptr[r]);
   L.width = width;
@@ -551,10 +751,10 @@ void build_level_matrix(Level &L, const dec::HostCsr &A, std::vector<int32_t> &c
   std::vector<double> vd(sz, 0.0);
   std::vector<float> vf(sz, 0.0f);
   for (int p = 0; p < n; p++) {
-    int r = order[p];
+    int r = L.perm[order[p]];
     for (int k = A.ptr[r]; k < A.ptr[r + 1]; k++) {
       size_t q = static_cast<size_t>(k - A.ptr[r]) * n + p;
-      col[q] = A.col[k];
+      col[q] = L.iperm[A.col[k]];
       vd[q] = A.val[k];
       vf[q] = static_cast<float>(A.val[k]);
     }
@@ -616,32 +816,56 @@ void transfer_csr(const dec::Topo &coarse, dec::HostCsr &R, dec::HostCsr &P, std
   }
 }
 
-void build_transfer(Level &fine, const dec::Topo &coarse) {
-  dec::HostCsr R, P;
-  std::vector<int32_t> pa, pb;
-  transfer_csr(coarse, R, P, pa, pb);
-  int nc = coarse.nv;
-  int w = 0;
-  for (int i = 0; i < nc; i++) w = std::max(w, R.ptr[i + 1] - R.ptr[i]);
-  fine.nc = nc;
-  fine.rwidth = w;
-  std::vector<int32_t> col(static_cast<size_t>(nc) * w, -1);
-  std::vector<double> vd(col.size(), 0.0);
-  std::vector<float> vf(col.size(), 0.0f);
-  for (int i = 0; i < nc; i++)
-    for (int k = R.ptr[i]; k < R.ptr[i + 1]; k++) {
-      size_t q = static_cast<size_t>(k - R.ptr[i]) * nc + i;
-      col[q] = R.col[k];
-      vd[q] = R.val[k];
-      vf[q] = static_cast<float>(R.val[k]);
+// Slot-major ELL of a host CSR with rows and columns renumbered: row i is
+// CSR row rperm[i], columns map through cimap; slots keep the CSR order.
+template <class T>
+void ell_renumbered(const dec::HostCsr &M, const std::vector<int32_t> &rperm, const std::vector<int32_t> &cimap,
+                    int &width, std::vector<int32_t> &col, std::vector<T> &val) {
+  int m = M.m;
+  width = 0;
+  for (int r = 0; r < m; r++) width = std::max(width, M.ptr[r + 1] - M.ptr[r]);
+  size_t sz = static_cast<size_t>(m) * std::max(width, 1);
+  col.assign(sz, -1);
+  val.assign(sz, T(0));
+  for (int i = 0; i < m; i++) {
+    int r = rperm[i];
+    for (int k = M.ptr[r]; k < M.ptr[r + 1]; k++) {
+      size_t q = static_cast<size_t>(k - M.ptr[r]) * m + i;
+      col[q] = cimap[M.col[k]];
+      val[q] = static_cast<T>(M.val[k]);
     }
+  }
+}
+
+// R (fine -> coarse) and P (coarse -> fine) of one level pair as ELL over
+// the two levels' internal numbers; R and P are host CSRs in the caller's
+// numbering, kept for export.
+void set_transfer(Level &fine, const Level &coarse, dec::HostCsr R, dec::HostCsr P) {
+  if (R.m != coarse.n || R.n != fine.n || P.m != fine.n || P.n != coarse.n)
+    fail(DEC_ERR_ARG, "multigrid: transfer operator shapes do not match the levels");
+  fine.nc = coarse.n;
+  std::vector<int32_t> col;
+  std::vector<double> vd;
+  std::vector<float> vf;
+  ell_renumbered(R, coarse.perm, fine.iperm, fine.rwidth, col, vd);
+  ell_renumbered(R, coarse.perm, fine.iperm, fine.rwidth, col, vf);
   fine.rcol.upload(col);
   fine.d.rval.upload(vd);
   fine.f.rval.upload(vf);
-  fine.pa.upload(pa);
-  fine.pb.upload(pb);
+  ell_renumbered(P, fine.perm, coarse.iperm, fine.pwidth, col, vd);
+  ell_renumbered(P, fine.perm, coarse.iperm, fine.pwidth, col, vf);
+  fine.pcol.upload(col);
+  fine.d.pval.upload(vd);
+  fine.f.pval.upload(vf);
   fine.R = std::move(R);
   fine.P = std::move(P);
+}
+
+void build_transfer(Level &fine, const Level &coarse, const dec::Topo &coarse_topo) {
+  dec::HostCsr R, P;
+  std::vector<int32_t> pa, pb;
+  transfer_csr(coarse_topo, R, P, pa, pb);
+  set_transfer(fine, coarse, std::move(R), std::move(P));
 }
 
 void alloc_work(Level &L) {
@@ -694,6 +918,164 @@ std::vector<std::vector<int32_t>> level_colors(const dec_mg_s &mg, const std::ve
   return colors;
 }
 
+// Levels, transfers, the dense coarse operator and the fused-tail
+// descriptors of a hierarchy, from its operators (caller's numbering, finest
+// first), R_l / P_l between levels l and l+1, and proposed colourings.
+void finish_hierarchy(dec_mg_s &mg, const std::vector<dec::HostCsr> &As, std::vector<dec::HostCsr> &Rs,
+                      std::vector<dec::HostCsr> &Ps, std::vector<std::vector<int32_t>> &colors) {
+  int nl = static_cast<int>(As.size());
+  for (int l = 0; l < nl; l++) {
+    auto L = std::make_unique<Level>();
+    build_level_matrix(*L, As[l], colors[l], mg.fuse_rows);
+    alloc_work(*L);
+    mg.lv.push_back(std::move(L));
+  }
+  for (int l = 0; l + 1 < nl; l++) set_transfer(*mg.lv[l], *mg.lv[l + 1], std::move(Rs[l]), std::move(Ps[l]));
+  // dense coarse operator for the direct solve (over the level's internal numbers)
+  Level &C = *mg.lv.back();
+  int n = C.n;
+  if (static_cast<long long>(n) * n <= 64LL * 1024 * 1024) {
+    std::vector<double> Ad(static_cast<size_t>(n) * n, 0.0);
+    for (int r = 0; r < n; r++)
+      for (int k = C.A.ptr[r]; k < C.A.ptr[r + 1]; k++) Ad[static_cast<size_t>(r) * n + C.A.col[k]] += C.A.val[k];
+    dec::coarse_inverse(n, Ad, C.Zh);
+    std::vector<double> zt(C.Zh.size());
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++)
+        zt[static_cast<size_t>(j) * n + i] = C.Zh[static_cast<size_t>(C.perm[i]) * n + C.perm[j]];
+    C.d.Z.upload(zt);
+    std::vector<float> zf(zt.begin(), zt.end());
+    C.f.Z.upload(zf);
+    C.d.zpart.alloc(static_cast<size_t>(coarse_chunks(n)) * n);
+    C.f.zpart.alloc(static_cast<size_t>(coarse_chunks(n)) * n);
+  }
+  // descriptors of the coarse tail for the fused kernel
+  for (int l = 0; l < nl; l++)
+    if (mg.lv[l]->n <= mg.fuse_rows && mg.lv[l]->ncolors <= kMaxColors) {
+      bool ok = true;
+      for (int q = l; q < nl; q++) ok = ok && mg.lv[q]->n <= mg.fuse_rows && mg.lv[q]->ncolors <= kMaxColors;
+      if (ok) {
+        mg.fuse_from = l;
+        break;
+      }
+    }
+  {
+    std::vector<LevelDesc<double>> d64(nl);
+    std::vector<LevelDesc<float>> d32(nl);
+    for (int l = 0; l < nl; l++) {
+      Level &L = *mg.lv[l];
+      auto fill = [&](auto &D, auto &V) {
+        D.n = L.n;
+        D.width = L.width;
+        D.ncolors = L.ncolors;
+        D.nc = L.nc;
+        D.rwidth = L.rwidth;
+        D.pwidth = L.pwidth;
+        for (int c = 0; c <= kMaxColors; c++) D.cstart[c] = c < static_cast<int>(L.cstart.size()) ? L.cstart[c] : L.n;
+        D.col = L.col.p;
+        D.rowid = L.rowid.p;
+        D.rcol = L.rcol.p;
+        D.pcol = L.pcol.p;
+        D.val = V.val.p;
+        D.rval = V.rval.p;
+        D.pval = V.pval.p;
+        D.u = V.u.p;
+        D.b = V.b.p;
+        D.r = V.r.p;
+      };
+      fill(d64[l], L.d);
+      fill(d32[l], L.f);
+    }
+    mg.desc64.upload(d64);
+    mg.desc32.upload(d32);
+  }
+  mg.part.alloc(kDotBlocks);
+  mg.part2.alloc(kDotBlocks);
+  mg.scal.alloc(8);
+  if (hipHostMalloc(reinterpret_cast<void **>(&mg.hnorm), sizeof(double) * kNormSlots, hipHostMallocDefault) !=
+      hipSuccess)
+    fail(DEC_ERR_OOM, "multigrid: pinned host buffer");
+  for (auto &e : mg.nev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) fail(DEC_ERR_HIP, "hipEventCreate failed");
+  mg.cg.alloc(1);
+  int nf = mg.lv[0]->n;
+  mg.cgx.alloc(nf);
+  mg.cgp.alloc(nf);
+  mg.cgAp.alloc(nf);
+}
+
+void env_switches(dec_mg_s &mg) {
+  // debugging switches: run every phase as its own launch / launch cycles eagerly
+  mg.use_fused = std::getenv("DEC_MG_NO_FUSE") == nullptr;
+  if (const char *e = std::getenv("DEC_MG_FUSE_ROWS")) mg.fuse_rows = std::min(std::max(std::atoi(e), 0), kFuseRowsMax);
+  mg.use_graph = std::getenv("DEC_MG_NO_GRAPH") == nullptr;
+}
+
+// A caller's m x n CSR (index_base 0 or 1) checked and copied 0-based.
+dec::HostCsr csr_in(int m, int n, const int32_t *ptr, const int32_t *col, const double *val, int base,
+                    const char *what) {
+  if (!ptr || (!col && ptr[m] > ptr[0]) || (!val && ptr[m] > ptr[0]))
+    fail(DEC_ERR_ARG, std::string("dec_mg_create_from_csr: NULL array for ") + what);
+  dec::HostCsr A;
+  A.m = m;
+  A.n = n;
+  A.ptr.resize(m + 1);
+  for (int r = 0; r <= m; r++) {
+    A.ptr[r] = ptr[r] - ptr[0];
+    if (r > 0 && A.ptr[r] < A.ptr[r - 1])
+      fail(DEC_ERR_ARG, std::string("dec_mg_create_from_csr: row pointers of ") + what + " decrease");
+  }
+  int nnz = A.ptr[m];
+  A.col.resize(nnz);
+  A.val.assign(val ? val : static_cast<const double *>(nullptr), val ? val + nnz : nullptr);
+  for (int k = 0; k < nnz; k++) {
+    int c = col[k] - base;
+    if (c < 0 || c >= n) fail(DEC_ERR_ARG, std::string("dec_mg_create_from_csr: column out of range in ") + what);
+    A.col[k] = c;
+  }
+  for (int r = 0; r < m; r++)
+    for (int k = A.ptr[r] + 1; k < A.ptr[r + 1]; k++)
+      if (A.col[k] == A.col[k - 1])
+        fail(DEC_ERR_ARG, std::string("dec_mg_create_from_csr: repeated column in a row of ") + what);
+  return A;
+}
+
+// MultigridData(ops, restrictions, prolongations, steps) (Multigrid.jl:545-546):
+// a hierarchy from the caller's matrices, finest first.
+dec_mg_s *mg_from_csr(int nl, const int32_t *sizes, const int32_t *const *ap, const int32_t *const *ac,
+                      const double *const *av, const int32_t *const *rp, const int32_t *const *rc,
+                      const double *const *rv, const int32_t *const *pp, const int32_t *const *pc,
+                      const double *const *pv, int base, int steps) {
+  if (nl < 1 || nl > 16) fail(DEC_ERR_ARG, "dec_mg_create_from_csr: 1 to 16 levels");
+  if (!sizes || !ap || !ac || !av || (nl > 1 && (!rp || !rc || !rv || !pp || !pc || !pv)))
+    fail(DEC_ERR_ARG, "dec_mg_create_from_csr: NULL argument");
+  if (base != 0 && base != 1) fail(DEC_ERR_ARG, "dec_mg_create_from_csr: index_base must be 0 or 1");
+  if (steps < 0) fail(DEC_ERR_ARG, "dec_mg_create_from_csr: steps must be >= 0");
+  for (int l = 0; l < nl; l++)
+    if (sizes[l] < 1) fail(DEC_ERR_ARG, "dec_mg_create_from_csr: empty level");
+  auto mg = std::make_unique<dec_mg_s>();
+  mg->steps = steps;
+  env_switches(*mg);
+  std::vector<dec::HostCsr> As(nl), Rs(nl - 1), Ps(nl - 1);
+  for (int l = 0; l < nl; l++) {
+    As[l] = csr_in(sizes[l], sizes[l], ap[l], ac[l], av[l], base, "an operator");
+    bool diag = true;
+    for (int r = 0; r < sizes[l] && diag; r++) {
+      bool has = false;
+      for (int k = As[l].ptr[r]; k < As[l].ptr[r + 1]; k++) has = has || (As[l].col[k] == r && As[l].val[k] != 0.0);
+      diag = has;
+    }
+    if (!diag) fail(DEC_ERR_ARG, "dec_mg_create_from_csr: an operator has a zero diagonal entry");
+  }
+  for (int l = 0; l + 1 < nl; l++) {
+    Rs[l] = csr_in(sizes[l + 1], sizes[l], rp[l], rc[l], rv[l], base, "a restriction");
+    Ps[l] = csr_in(sizes[l], sizes[l + 1], pp[l], pc[l], pv[l], base, "a prolongation");
+  }
+  std::vector<std::vector<int32_t>> colors(nl);  // greedy colours per level
+  finish_hierarchy(*mg, As, Rs, Ps, colors);
+  return mg.release();
+}
+
 dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge, int center, int steps,
                     bool galerkin) {
   if (base.dim != 2) fail(DEC_ERR_UNSUPPORTED, "dec_mg_create: the multigrid hierarchy refines 2-D complexes");
@@ -701,10 +1083,7 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
   if (steps < 0) fail(DEC_ERR_ARG, "dec_mg_create: steps must be >= 0");
   auto mg = std::make_unique<dec_mg_s>();
   mg->steps = steps;
-  // debugging switches: run every phase as its own launch / launch cycles eagerly
-  mg->use_fused = std::getenv("DEC_MG_NO_FUSE") == nullptr;
-  if (const char *e = std::getenv("DEC_MG_FUSE_ROWS")) mg->fuse_rows = std::min(std::max(std::atoi(e), 0), kFuseRowsMax);
-  mg->use_graph = std::getenv("DEC_MG_NO_GRAPH") == nullptr;
+  env_switches(*mg);
   // coarse -> fine walk (Multigrid.jl:752-797), stored finest first
   std::vector<std::unique_ptr<dec_mesh_s>> walk;
   walk.emplace_back(copy_mesh(base));
@@ -736,81 +1115,18 @@ dec_mg_s *mg_create(const dec_mesh_s &base, int levels, bool project, int hodge,
     }
   }
   std::vector<std::vector<int32_t>> colors = level_colors(*mg, As);
-  for (int l = 0; l < nl; l++) {
-    auto L = std::make_unique<Level>();
-    build_level_matrix(*L, As[l], colors[l]);
-    alloc_work(*L);
-    mg->lv.push_back(std::move(L));
+  std::vector<dec::HostCsr> Rs(nl > 0 ? nl - 1 : 0), Ps(nl > 0 ? nl - 1 : 0);
+  for (int l = 0; l + 1 < nl; l++) {
+    std::vector<int32_t> pa, pb;
+    transfer_csr(mg->meshes[l + 1]->h, Rs[l], Ps[l], pa, pb);
   }
-  for (int l = 0; l + 1 < nl; l++) build_transfer(*mg->lv[l], mg->meshes[l + 1]->h);
-  // dense coarse operator for the direct solve
-  Level &C = *mg->lv.back();
-  int n = C.n;
-  if (static_cast<long long>(n) * n <= 64LL * 1024 * 1024) {
-    std::vector<double> Ad(static_cast<size_t>(n) * n, 0.0);
-    for (int r = 0; r < n; r++)
-      for (int k = C.A.ptr[r]; k < C.A.ptr[r + 1]; k++) Ad[static_cast<size_t>(r) * n + C.A.col[k]] += C.A.val[k];
-    dec::coarse_inverse(n, Ad, C.Zh);
-    std::vector<double> zt(C.Zh.size());
-    for (int i = 0; i < n; i++)
-      for (int j = 0; j < n; j++) zt[static_cast<size_t>(j) * n + i] = C.Zh[static_cast<size_t>(i) * n + j];
-    C.d.Z.upload(zt);
-    std::vector<float> zf(zt.begin(), zt.end());
-    C.f.Z.upload(zf);
-    C.d.zpart.alloc(static_cast<size_t>(coarse_chunks(n)) * n);
-    C.f.zpart.alloc(static_cast<size_t>(coarse_chunks(n)) * n);
-  }
-  // descriptors of the coarse tail for the fused kernel
-  for (int l = 0; l < nl; l++)
-    if (mg->lv[l]->n <= mg->fuse_rows && mg->lv[l]->ncolors <= kMaxColors) {
-      bool ok = true;
-      for (int q = l; q < nl; q++) ok = ok && mg->lv[q]->n <= mg->fuse_rows && mg->lv[q]->ncolors <= kMaxColors;
-      if (ok) {
-        mg->fuse_from = l;
-        break;
-      }
-    }
-  {
-    std::vector<LevelDesc<double>> d64(nl);
-    std::vector<LevelDesc<float>> d32(nl);
-    for (int l = 0; l < nl; l++) {
-      Level &L = *mg->lv[l];
-      auto fill = [&](auto &D, auto &V) {
-        D.n = L.n;
-        D.width = L.width;
-        D.ncolors = L.ncolors;
-        D.nc = L.nc;
-        D.rwidth = L.rwidth;
-        for (int c = 0; c <= kMaxColors; c++) D.cstart[c] = c < static_cast<int>(L.cstart.size()) ? L.cstart[c] : L.n;
-        D.col = L.col.p;
-        D.rowid = L.rowid.p;
-        D.rcol = L.rcol.p;
-        D.pa = L.pa.p;
-        D.pb = L.pb.p;
-        D.val = V.val.p;
-        D.rval = V.rval.p;
-        D.u = V.u.p;
-        D.b = V.b.p;
-        D.r = V.r.p;
-      };
-      fill(d64[l], L.d);
-      fill(d32[l], L.f);
-    }
-    mg->desc64.upload(d64);
-    mg->desc32.upload(d32);
-  }
-  mg->part.alloc(kDotBlocks);
-  mg->part2.alloc(kDotBlocks);
-  mg->scal.alloc(8);
-  mg->cg.alloc(1);
-  int nf = mg->lv[0]->n;
-  mg->cgx.alloc(nf);
-  mg->cgp.alloc(nf);
-  mg->cgAp.alloc(nf);
+  finish_hierarchy(*mg, As, Rs, Ps, colors);
   dec::check_launch("dec_mg_create");
   dec::dsync();
   return mg.release();
 }
+
+
 
 // ---------------------------------------------------------------- cycles
 
@@ -876,6 +1192,22 @@ void smooth(dec_mg_s &mg, int lvl, Level &L, const T *b, T *u, bool post, bool u
   if (mg.alg == DEC_SMOOTH_CG) {
     if constexpr (sizeof(T) == 8) cg_smooth(mg, L, b, u, steps, s);
     else fail(DEC_ERR_UNSUPPORTED, "the CG smoother runs in float64 only");
+    return;
+  }
+  if (mg.alg == DEC_SMOOTH_BLOCK_GS) {
+    // one launch per sweep, ping-ponging between u and the level's t
+    // (an odd step count ends with a copy back into u)
+    T *bufs[2] = {u, V.t.p};
+    int nthr = L.seg_threads;
+    size_t lds = static_cast<size_t>(L.bmax) * sizeof(T);
+    for (int it = 0; it < steps; it++) {
+      const T *src = bufs[it & 1];
+      T *dst = bufs[(it + 1) & 1];
+      DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_block_gs<W, T>), dim3(L.nblk), dim3(nthr), lds, s, L.n,
+                                                       L.width, L.ncolors, L.nblk, L.col.p, V.val.p, L.rowid.p,
+                                                       L.bstart.p, L.seg.p, b, src, dst, post ? 1 : 0));
+    }
+    if (steps & 1) hipLaunchKernelGGL(k_copy<T>, G(L.n), dim3(kBlock), 0, s, L.n, V.t.p, u);
     return;
   }
   for (int it = 0; it < steps; it++) {
@@ -960,7 +1292,8 @@ void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, bool u_zero, hipS
                                                     L.rcol.p, V.rval.p, V.r.p, CV.b.p, CV.u.p));
   mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu, true, s);
   if (mu > 1) mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu - 1, false, s);
-  hipLaunchKernelGGL(k_prolong_add<T>, G(L.n), dim3(kBlock), 0, s, L.n, L.nc, L.pa.p, L.pb.p, CV.u.p, u);
+  DEC_ELL_WIDTH_SWITCH(L.pwidth, hipLaunchKernelGGL((k_prolong_add<W, T>), G(L.n), dim3(kBlock), 0, s, L.n, L.pwidth,
+                                                    L.pcol.p, V.pval.p, CV.u.p, u));
   smooth<T>(mg, lvl, L, b, u, true, false, s);
 }
 
@@ -975,10 +1308,33 @@ void fmg_rec(dec_mg_s &mg, int lvl, const double *b, double *z, int mu, int cycl
                                                       L.rwidth, L.rcol.p, L.d.rval.p, b, C.d.b.p,
                                                       static_cast<double *>(nullptr)));
     fmg_rec(mg, lvl + 1, C.d.b.p, C.d.u.p, mu, cycles, s);
-    hipLaunchKernelGGL(k_prolong_add<double>, G(L.n), dim3(kBlock), 0, s, L.n, L.nc, L.pa.p, L.pb.p, C.d.u.p, z);
+    DEC_ELL_WIDTH_SWITCH(L.pwidth, hipLaunchKernelGGL((k_prolong_add<W, double>), G(L.n), dim3(kBlock), 0, s, L.n,
+                                                      L.pwidth, L.pcol.p, L.d.pval.p, C.d.u.p, z));
   }
   if (lvl > 0)
     for (int c = 0; c < cycles; c++) mu_cycle<double>(mg, lvl, z, b, mu, false, s);
+}
+
+// The caller's finest-level vectors in the hierarchy's numbering: b into
+// the finest level's b, u (when load_u) into its u; the caller's own pointers
+// when the finest level keeps the caller's order.
+struct FineIO {
+  double *u;
+  const double *b;
+};
+
+FineIO fine_in(dec_mg_s &mg, double *u, const double *b, bool load_u, hipStream_t s) {
+  Level &F = *mg.lv[0];
+  if (F.identity) return FineIO{u, b};
+  hipLaunchKernelGGL(k_perm_gather<double>, G(F.n), dim3(kBlock), 0, s, F.n, F.dperm.p, b, F.d.b.p);
+  if (load_u) hipLaunchKernelGGL(k_perm_gather<double>, G(F.n), dim3(kBlock), 0, s, F.n, F.dperm.p, u, F.d.u.p);
+  return FineIO{F.d.u.p, F.d.b.p};
+}
+
+void fine_out(dec_mg_s &mg, double *u, hipStream_t s) {
+  Level &F = *mg.lv[0];
+  if (F.identity) return;
+  hipLaunchKernelGGL(k_perm_scatter<double>, G(F.n), dim3(kBlock), 0, s, F.n, F.dperm.p, F.d.u.p, u);
 }
 
 double norm2(dec_mg_s &mg, int n, const double *x, hipStream_t s) {
@@ -1044,19 +1400,20 @@ void run_cycle(dec_mg_s &mg, int variant, double *u, const double *b, hipStream_
   gcheck(hipGraphLaunch(mg.gexec[i], s), "hipGraphLaunch");
 }
 
-void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u, const double *b, int *done_cycles,
-           double *relres, hipStream_t s) {
+void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_caller, const double *b_caller,
+           int *done_cycles, double *relres, hipStream_t s) {
   Level &F = *mg.lv[0];
   int n = F.n;
-  double bn = norm2(mg, n, b, s);
+  double bn = norm2(mg, n, b_caller, s);
   if (bn == 0.0) {
-    hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, u);
+    hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, u_caller);
     *done_cycles = 0;
     *relres = 0.0;
     return;
   }
-  double rr = 0.0, prev = 0.0;
-  int c = 0;
+  FineIO io = fine_in(mg, u_caller, b_caller, true, s);
+  double *u = io.u;
+  const double *b = io.b;
   // the network smooths the float32 cycles on the levels of at least
   // learned_min_rows .. learned_max_rows rows; with no such level the solve
   // is a plain mixed one
@@ -1065,21 +1422,50 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u, con
   mg.learned_active = mg.learned && mg.nn && variant == DEC_SOLVE_MIXED_IR && any;
   mg.fallbacks = 0;
   mg.learned_cycles = 0;
-  for (;;) {
+  // The residual norm after k cycles is copied to pinned host memory with an
+  // event per slot; the host reads it when it must decide.  While the last
+  // known residual is above 1000 tol the next cycle cannot meet the tolerance
+  // (no V-cycle cuts the residual 1000x), so the cycle after it is queued at
+  // once and the GPU does not wait for the host between cycles.
+  auto enqueue_norm = [&](int k) {
     residual<double>(F, b, u, F.d.r.p, s);
-    rr = norm2(mg, n, F.d.r.p, s) / bn;
+    double *slot = mg.scal.p + (k % kNormSlots);
+    dot(mg, n, F.d.r.p, F.d.r.p, slot, nullptr, s);
+    gcheck(hipMemcpyAsync(mg.hnorm + (k % kNormSlots), slot, sizeof(double), hipMemcpyDeviceToHost, s),
+           "hipMemcpyAsync");
+    gcheck(hipEventRecord(mg.nev[k % kNormSlots], s), "hipEventRecord");
+  };
+  auto read_norm = [&](int k) {
+    gcheck(hipEventSynchronize(mg.nev[k % kNormSlots]), "hipEventSynchronize");
+    return std::sqrt(mg.hnorm[k % kNormSlots]) / bn;
+  };
+  auto enqueue_cycle = [&](int k) {  // cycle k (1-based) and the norm after it
+    if (mg.learned_active) mg.learned_cycles++;
+    run_cycle(mg, variant, u, b, s);
+    enqueue_norm(k);
+  };
+  enqueue_norm(0);
+  int enq = 0;  // cycles queued
+  double rr = 0.0, prev = 0.0;
+  int c = 0;
+  for (;; c++) {
+    rr = read_norm(c);
     // a learned cycle that cut the residual by less than 0.5x hands the rest
-    // of the solve to Gauss-Seidel
+    // of the solve to Gauss-Seidel (from the next cycle not yet queued)
     if (c > 0 && mg.learned_active && rr > 0.5 * prev) {
       mg.learned_active = false;
       mg.fallbacks++;
     }
-    if (rr <= tol || c >= max_cycles) break;
-    if (mg.learned_active) mg.learned_cycles++;
     prev = rr;
-    run_cycle(mg, variant, u, b, s);
-    c++;
+    if (rr <= tol || c >= max_cycles) break;
+    if (enq < c + 1) enqueue_cycle(++enq);
+    if (enq == c + 1 && c + 2 <= max_cycles && rr * 1e-3 > tol) enqueue_cycle(++enq);
   }
+  if (enq > c) {  // a queued cycle ran past the stop (a drop above 1000x): report it
+    c = enq;
+    rr = read_norm(c);
+  }
+  fine_out(mg, u_caller, s);
   *done_cycles = c;
   *relres = rr;
   if (rr > tol) fail(DEC_ERR_NOCONV, "dec_mg_solve: relative residual " + std::to_string(rr) + " above tolerance after " +
@@ -1109,6 +1495,23 @@ int dec_mg_create_mode(dec_mesh base, int32_t levels, int32_t project, int32_t h
   DEC_CATCH
 }
 
+int dec_mg_create_from_csr(int32_t nlevels, const int32_t *sizes, const int32_t *const *a_rowptr,
+                           const int32_t *const *a_col, const double *const *a_val,
+                           const int32_t *const *r_rowptr, const int32_t *const *r_col,
+                           const double *const *r_val, const int32_t *const *p_rowptr,
+                           const int32_t *const *p_col, const double *const *p_val, int32_t index_base,
+                           int32_t steps, dec_mg *out) {
+  DEC_TRY
+  if (!out) fail(DEC_ERR_ARG, "dec_mg_create_from_csr: out is NULL");
+  *out = nullptr;
+  dec_mg_s *mg = mg_from_csr(nlevels, sizes, a_rowptr, a_col, a_val, r_rowptr, r_col, r_val, p_rowptr, p_col,
+                             p_val, index_base, steps);
+  dec::check_launch("dec_mg_create_from_csr");
+  dec::dsync();
+  *out = mg;
+  DEC_CATCH
+}
+
 int dec_mg_nlevels(dec_mg mg, int32_t *nlevels) {
   DEC_TRY
   if (!mg || !nlevels) fail(DEC_ERR_ARG, "dec_mg_nlevels: NULL argument");
@@ -1127,6 +1530,7 @@ int dec_mg_level_size(dec_mg mg, int32_t level, int32_t *n) {
 int dec_mg_finest_mesh(dec_mg mg, dec_mesh *m) {
   DEC_TRY
   if (!mg || !m) fail(DEC_ERR_ARG, "dec_mg_finest_mesh: NULL argument");
+  if (mg->meshes.empty()) fail(DEC_ERR_UNSUPPORTED, "dec_mg_finest_mesh: the hierarchy was built from matrices");
   *m = mg->meshes[0].get();
   DEC_CATCH
 }
@@ -1157,10 +1561,22 @@ int dec_mg_export_colors(dec_mg mg, int32_t level, int32_t *color, int32_t *ncol
   DEC_CATCH
 }
 
+int dec_mg_export_blocks(dec_mg mg, int32_t level, int32_t *block, int32_t *nblocks) {
+  DEC_TRY
+  if (!mg) fail(DEC_ERR_ARG, "dec_mg_export_blocks: NULL handle");
+  if (level < 0 || level >= static_cast<int>(mg->lv.size())) fail(DEC_ERR_ARG, "dec_mg_export_blocks: bad level");
+  const Level &L = *mg->lv[level];
+  if (block)
+    for (int k = 0; k < L.nblk; k++)
+      for (int i = L.hbstart[k]; i < L.hbstart[k + 1]; i++) block[L.perm[i]] = k;
+  if (nblocks) *nblocks = L.nblk;
+  DEC_CATCH
+}
+
 int dec_mg_set_smoother(dec_mg mg, int32_t alg, int32_t steps, double omega, int32_t coarse_direct) {
   DEC_TRY
   if (!mg) fail(DEC_ERR_ARG, "dec_mg_set_smoother: NULL handle");
-  if (alg < DEC_SMOOTH_CG || alg > DEC_SMOOTH_JACOBI) fail(DEC_ERR_ARG, "dec_mg_set_smoother: unknown smoother");
+  if (alg < DEC_SMOOTH_CG || alg > DEC_SMOOTH_BLOCK_GS) fail(DEC_ERR_ARG, "dec_mg_set_smoother: unknown smoother");
   if (steps < 0) fail(DEC_ERR_ARG, "dec_mg_set_smoother: steps must be >= 0");
   if (coarse_direct && !mg->lv.back()->d.Z.p)
     fail(DEC_ERR_UNSUPPORTED, "dec_mg_set_smoother: coarsest level too large for the dense direct solve");
@@ -1176,8 +1592,9 @@ int dec_mg_cycles(dec_mg mg, int32_t mu, int32_t cycles, void *u, const void *b,
   if (!mg || !u || !b) fail(DEC_ERR_ARG, "dec_mg_cycles: NULL argument");
   if (mu < 1 || cycles < 0) fail(DEC_ERR_ARG, "dec_mg_cycles: mu >= 1 and cycles >= 0 required");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  for (int c = 0; c < cycles; c++)
-    mu_cycle<double>(*mg, 0, static_cast<double *>(u), static_cast<const double *>(b), mu, false, s);
+  FineIO io = fine_in(*mg, static_cast<double *>(u), static_cast<const double *>(b), true, s);
+  for (int c = 0; c < cycles; c++) mu_cycle<double>(*mg, 0, io.u, io.b, mu, false, s);
+  fine_out(*mg, static_cast<double *>(u), s);
   dec::check_launch("dec_mg_cycles");
   DEC_CATCH
 }
@@ -1187,10 +1604,10 @@ int dec_mg_full_multigrid(dec_mg mg, int32_t mu, int32_t cycles, const void *b, 
   if (!mg || !u || !b) fail(DEC_ERR_ARG, "dec_mg_full_multigrid: NULL argument");
   if (mu < 1 || cycles < 0) fail(DEC_ERR_ARG, "dec_mg_full_multigrid: mu >= 1 and cycles >= 0 required");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const double *bb = static_cast<const double *>(b);
-  double *uu = static_cast<double *>(u);
-  fmg_rec(*mg, 0, bb, uu, mu, cycles, s);
-  for (int c = 0; c < cycles; c++) mu_cycle<double>(*mg, 0, uu, bb, mu, false, s);
+  FineIO io = fine_in(*mg, static_cast<double *>(u), static_cast<const double *>(b), false, s);
+  fmg_rec(*mg, 0, io.b, io.u, mu, cycles, s);
+  for (int c = 0; c < cycles; c++) mu_cycle<double>(*mg, 0, io.u, io.b, mu, false, s);
+  fine_out(*mg, static_cast<double *>(u), s);
   dec::check_launch("dec_mg_full_multigrid");
   DEC_CATCH
 }
@@ -1225,7 +1642,7 @@ int dec_nn_create(dec_mg mg, uint64_t seed, dec_nn *out) {
   for (int l = 0; l < nl; l++) {
     Level &L = *mg->lv[l];
     bool has = l + 1 < nl;
-    ops.push_back({&L.A, has ? &L.R : nullptr, has ? &L.P : nullptr});
+    ops.push_back({&L.A, has ? &L.R : nullptr, has ? &L.P : nullptr, &L.perm});
   }
   *out = dec::nn_create(ops, seed);
   DEC_CATCH

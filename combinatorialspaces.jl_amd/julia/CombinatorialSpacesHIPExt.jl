@@ -14,13 +14,16 @@ image); tests drive the same C ABI from Python.
 module CombinatorialSpacesHIPExt
 
 using CombinatorialSpaces
-using CombinatorialSpaces.DiscreteExteriorCalculus: DiscreteHodge, Circumcenter, Barycenter
+using CombinatorialSpaces.DiscreteExteriorCalculus: DiscreteHodge, Circumcenter, Barycenter,
+  geometric_center, triangle_center, tetrahedron_center
 using CombinatorialSpaces.Multigrid: BinarySubdivision, AbstractSubdivisionScheme,
-  AbstractMultigridMode, DirectMode, GalerkinMode
+  AbstractMultigridMode, DirectMode, GalerkinMode, MultigridData, MGData, PrimalGeometricMapSeries
 using AMDGPU
 using Krylov
 using LinearAlgebra
+using SparseArrays
 import CombinatorialSpaces: cache_wedge, dec_wedge_product, dec_wedge_product_dd,
+  dec_c_wedge_product!, dec_c_wedge_product, dec_Δ⁻¹, ∇²,
   dec_wedge_product_pd, dec_wedge_product_dp, ♭♯_mat, interior_product_dd, ℒ_dd, Δᵈ,
   dec_boundary, dec_differential, dec_dual_derivative,
   dec_hodge_star, dec_inv_hodge_star, avg₀₁_mat,
@@ -39,7 +42,7 @@ const DEC_OP_WEDGE, DEC_OP_WEDGE_DD, DEC_OP_DIFFERENTIAL, DEC_OP_BOUNDARY,
       DEC_OP_INTERIOR_PRODUCT_DD, DEC_OP_LIE_DD, DEC_OP_DUAL_LAPLACIAN,
       DEC_OP_P2_D2_INTERPOLATION, DEC_OP_P3_D3_INTERPOLATION, DEC_OP_D0_P0_INTERPOLATION,
       DEC_OP_P0_D0_INTERPOLATION = Int32.(1:19)
-const DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI = Int32.(0:2)
+const DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI, DEC_SMOOTH_BLOCK_GS = Int32.(0:3)
 
 function check(status::Cint)
   status == 0 && return nothing
@@ -83,12 +86,35 @@ function register_mesh(sd, h, center)
   m
 end
 
+# The center `dualize(s, center)` used for sd, read from sd's own dual points:
+# the dual point of a cell is compared with the barycenter and the
+# circumcenter of its vertices (geometric_center, DiscreteExteriorCalculus.jl:
+# 100-135), on the first cell where those two differ.  A complex whose cells
+# all have coinciding centers gets the same geometry either way.
+function dual_center(sd, cells, center_of, verts_of)
+  for x in cells
+    ps = sd[verts_of(sd, x), :point]
+    bary, circ = geometric_center(ps, Barycenter()), geometric_center(ps, Circumcenter())
+    scale = maximum(norm(p - bary) for p in ps)
+    norm(bary - circ) > 1e-6 * scale || continue
+    dp = sd[center_of(sd, x), :dual_point]
+    norm(dp - bary) <= 1e-9 * scale && return Barycenter()
+    norm(dp - circ) <= 1e-9 * scale && return Circumcenter()
+    error("HIP backend: dual points are neither barycenters nor circumcenters")
+  end
+  Circumcenter()
+end
+dual_center(sd::EmbeddedDeltaDualComplex2D) =
+  dual_center(sd, triangles(sd), triangle_center, triangle_vertices)
+dual_center(sd::EmbeddedDeltaDualComplex3D) =
+  dual_center(sd, tetrahedra(sd), tetrahedron_center, tetrahedron_vertices)
+
 """Upload (once per dual complex) the primal boundary maps, orientations and
 points; the dual geometry is then computed on the GPU in Float64 with the
-given center (call `hip_mesh(sd; center=Barycenter())` before the first
-operator for a barycentric dual)."""
-function hip_mesh(sd::EmbeddedDeltaDualComplex2D; center=Circumcenter())
+center that sd itself was dualized with (`dual_center`)."""
+function hip_mesh(sd::EmbeddedDeltaDualComplex2D)
   get!(MESHES, sd) do
+    center = dual_center(sd)
     nv_, ne_, nt_ = nv(sd), ne(sd), ntriangles(sd)
     v0 = Int32.(sd[:∂v0]); v1 = Int32.(sd[:∂v1])
     e0 = Int32.(sd[:∂e0]); e1 = Int32.(sd[:∂e1]); e2 = Int32.(sd[:∂e2])
@@ -105,8 +131,9 @@ end
 
 # A tetrahedral complex: dec_mesh3_create with ∂t0..∂t3 and the tetrahedron
 # orientation as well (SimplicialSets.jl:418-606).
-function hip_mesh(sd::EmbeddedDeltaDualComplex3D; center=Circumcenter())
+function hip_mesh(sd::EmbeddedDeltaDualComplex3D)
   get!(MESHES, sd) do
+    center = dual_center(sd)
     v0 = Int32.(sd[:∂v0]); v1 = Int32.(sd[:∂v1])
     e0 = Int32.(sd[:∂e0]); e1 = Int32.(sd[:∂e1]); e2 = Int32.(sd[:∂e2])
     t0 = Int32.(sd[:∂t0]); t1 = Int32.(sd[:∂t1]); t2 = Int32.(sd[:∂t2]); t3 = Int32.(sd[:∂t3])
@@ -162,18 +189,47 @@ Base.size(op::HIPOperator) = (op.rows, op.cols)
 float_of(sd::EmbeddedDeltaDualComplex2D{Bool, F, P}) where {F, P} = F
 float_of(sd::EmbeddedDeltaDualComplex3D{Bool, F, P}) where {F, P} = F
 
-# Wedge products: same closure protocol as FastDEC (src/FastDEC.jl:255-289).
+# Wedge products.  cache_wedge returns the reference's (tables, coefficients,
+# count) triple (src/FastDEC.jl:136-155): here the first slot is the device
+# operator holding the library's tables, the second its coefficient store
+# (the same operator), the last the result length.  dec_c_wedge_product! and
+# dec_c_wedge_product (:217-238) then run the library's kernel when handed
+# that triple, so caller code that caches tables keeps working.
+struct HIPWedgeTable{T}
+  op::HIPOperator{T}
+end
+struct HIPWedgeCache{T}
+  table::HIPWedgeTable{T}
+  n::Int
+end
+Base.getindex(c::HIPWedgeCache, i::Int) = i == 1 || i == 2 ? c.table :
+  i == 3 ? c.n : throw(BoundsError(c, i))
+Base.first(c::HIPWedgeCache) = c.table
+Base.last(c::HIPWedgeCache) = c.n
+Base.length(::HIPWedgeCache) = 3
+
 function cache_wedge(::Val{m}, ::Val{n}, sd::DualComplex, ::Val{:HIP},
                      arr_cons=ROCArray, cast_float=nothing) where {m, n}
   T = isnothing(cast_float) ? float_of(sd) : cast_float
-  HIPOperator{T}(sd, DEC_OP_WEDGE, m, n, DEC_GEOMETRIC_HODGE)
+  op = HIPOperator{T}(sd, DEC_OP_WEDGE, m, n, DEC_GEOMETRIC_HODGE)
+  HIPWedgeCache{T}(HIPWedgeTable{T}(op), op.rows)
+end
+
+function dec_c_wedge_product!(::Val{j}, ::Val{k}, res::ROCVector{T}, α, β, p::HIPWedgeTable{T}, c) where {j, k, T}
+  apply!(res, p.op, α, β)
+end
+
+function dec_c_wedge_product(::Val{m}, ::Val{n}, α, β, wc::HIPWedgeCache{T}) where {m, n, T}
+  α_data = α isa SimplexForm ? α.data : α
+  β_data = β isa SimplexForm ? β.data : β
+  apply!(ROCVector{T}(undef, last(wc)), wc.table.op, α_data, β_data)
 end
 
 function dec_wedge_product(::Val{m}, ::Val{n}, sd::DualComplex, ::Val{:HIP},
                            arr_cons=ROCArray, cast_float=nothing) where {m, n}
   (m, n) == (0, 0) && return (f, g) -> f .* g
-  op = cache_wedge(Val(m), Val(n), sd, Val(:HIP), arr_cons, cast_float)
-  (α, β) -> apply!(similar(α, op.rows), op, α, β)
+  wc = cache_wedge(Val(m), Val(n), sd, Val(:HIP), arr_cons, cast_float)
+  (α, β) -> dec_c_wedge_product(Val(m), Val(n), α, β, wc)
 end
 
 function dec_wedge_product_dd(::Val{m}, ::Val{n}, sd::EmbeddedDeltaDualComplex2D, ::Val{:HIP},
@@ -261,7 +317,7 @@ for ⋆1; `mode` is DirectMode() or GalerkinMode()."""
 function HIPMultigridData(s::EmbeddedDeltaSet2D, ::BinarySubdivision, levels::Int, steps::Int;
                           hodge::DiscreteHodge=GeometricHodge(), project::Bool=false,
                           mode::AbstractMultigridMode=DirectMode())
-  base = hip_mesh(dualize(s, Circumcenter()))
+  base = hip_mesh(dualize(s, Circumcenter()))   # the library dualizes every level itself
   h = Ref{Ptr{Cvoid}}(C_NULL)
   check(ccall((:dec_mg_create_mode, libdec_hip), Cint,
     (Ptr{Cvoid}, Int32, Int32, Int32, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
@@ -274,8 +330,59 @@ function HIPMultigridData(s::EmbeddedDeltaSet2D, ::BinarySubdivision, levels::In
   md
 end
 
+# CSR arrays (1-based, Int32) of a sparse matrix: the CSC arrays of its transpose.
+function csr_arrays(M::AbstractMatrix)
+  T = SparseMatrixCSC{Float64, Int32}(sparse(transpose(sparse(M))))
+  (Vector{Int32}(T.colptr), Vector{Int32}(T.rowval), Vector{Float64}(T.nzval), size(M, 1))
+end
+
+"""HIP counterpart of `MultigridData(ops, restrictions, prolongations, steps)`
+(src/Multigrid.jl:545-546): the hierarchy of a CPU `MultigridData` (operators,
+restrictions and prolongations finest first) uploaded as is.  The reference's
+other constructors go through it: `MultigridData(s, scheme, levels, op, steps)`
+with any `op`, and `MGData(series, op, steps)`."""
+function HIPMultigridData(md::MultigridData)
+  steps = unique(md.steps)
+  length(steps) == 1 || error("HIP backend: one smoothing step count for all levels")
+  A = map(csr_arrays, md.operators)
+  R = map(csr_arrays, md.restrictions)
+  P = map(csr_arrays, md.prolongations)
+  nl = length(A)
+  sizes = Int32[a[4] for a in A]
+  ptrs(ms, j) = Ptr{Cvoid}[pointer(m[j]) for m in ms]
+  h = Ref{Ptr{Cvoid}}(C_NULL)
+  GC.@preserve A R P begin
+    check(ccall((:dec_mg_create_from_csr, libdec_hip), Cint,
+      (Int32, Ptr{Int32}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
+       Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
+       Int32, Int32, Ptr{Ptr{Cvoid}}),
+      Int32(nl), sizes, ptrs(A, 1), ptrs(A, 2), ptrs(A, 3), ptrs(R, 1), ptrs(R, 2), ptrs(R, 3),
+      ptrs(P, 1), ptrs(P, 2), ptrs(P, 3), Int32(1), Int32(only(steps)), h))
+  end
+  out = HIPMultigridData(h[], Int(sizes[1]), Int(only(steps)))
+  finalizer(x -> ccall((:dec_mg_destroy, libdec_hip), Cint, (Ptr{Cvoid},), x.handle), out)
+  out
+end
+
+"""`MultigridData(s, scheme, levels, op, steps; alg, mode)` (src/Multigrid.jl:
+698-702) with any operator function `op`: the reference builds the operators,
+restrictions and prolongations on the host, the V-cycles run on the GPU."""
+HIPMultigridData(s::EmbeddedDeltaSet2D, scheme::AbstractSubdivisionScheme, levels::Int, op::Function, steps;
+                 alg=Circumcenter(), mode::AbstractMultigridMode=DirectMode()) =
+  HIPMultigridData(MultigridData(s, scheme, levels, op, steps; alg=alg, mode=mode))
+
+"""dec_Δ⁻¹(Val(0), series; steps, cycles, alg, μ) (src/FastDEC.jl:866-869) on
+the GPU: MGData(series, sd -> ∇²(0, sd), steps) uploaded once, then full
+multigrid with μ-cycles per call."""
+function dec_Δ⁻¹(::Val{0}, s::PrimalGeometricMapSeries, ::Val{:HIP};
+                 scheme::AbstractSubdivisionScheme=BinarySubdivision(), steps=3, cycles=5, alg=Krylov.cg, μ=2)
+  md = HIPMultigridData(MGData(s, sd -> ∇²(0, sd), steps, scheme))
+  b -> full_multigrid(b, md, cycles, alg, μ)
+end
+
 smoother_code(alg) = alg === Krylov.cg ? DEC_SMOOTH_CG :
   alg === :gauss_seidel ? DEC_SMOOTH_GS : alg === :jacobi ? DEC_SMOOTH_JACOBI :
+  alg === :block_gauss_seidel ? DEC_SMOOTH_BLOCK_GS :
   error("unsupported smoother $alg")
 
 function set_smoother!(md::HIPMultigridData, alg; omega=2/3, coarse_direct=false)

@@ -63,7 +63,7 @@ enum dec_op_kind {
   DEC_OP_D0_P0_INTERPOLATION = 18, /* dual 0-forms -> primal 0-forms */
   DEC_OP_P0_D0_INTERPOLATION = 19  /* primal 0-forms -> dual 0-forms, nt x nv */
 };
-enum dec_smoother { DEC_SMOOTH_CG = 0, DEC_SMOOTH_GS = 1, DEC_SMOOTH_JACOBI = 2 };
+enum dec_smoother { DEC_SMOOTH_CG = 0, DEC_SMOOTH_GS = 1, DEC_SMOOTH_JACOBI = 2, DEC_SMOOTH_BLOCK_GS = 3 };
 enum dec_solve_variant { DEC_SOLVE_PLAIN_F64 = 0, DEC_SOLVE_MIXED_IR = 1 };
 
 /* ---------------------------------------------------------------- library */
@@ -261,6 +261,22 @@ int dec_mg_create(dec_mesh base, int32_t levels, int32_t project, int32_t hodge,
 enum { DEC_MG_DIRECT = 0, DEC_MG_GALERKIN = 1 };
 int dec_mg_create_mode(dec_mesh base, int32_t levels, int32_t project, int32_t hodge,
                        int32_t center, int32_t steps, int32_t mode, dec_mg *out);
+/* MultigridData(ops, restrictions, prolongations, steps) (Multigrid.jl:545-546,
+ * MGData at :548-551): a hierarchy from the caller's host matrices, finest
+ * first, as CSR arrays per level (index_base 0, or 1 for Julia's ids).
+ * sizes[l] = n_l; A_l is n_l x n_l with a nonzero diagonal; R_l (level l ->
+ * l+1) is n_{l+1} x n_l and P_l is n_l x n_{l+1} (l < nlevels - 1).  Row sums
+ * follow each row's stored column order.  This also carries the reference's
+ * generic `op` (MultigridData(s, scheme, levels, op, steps), :698-702): the
+ * caller evaluates op on each level's dual complex and passes the matrices.
+ * Gauss-Seidel colours are greedy; dec_mg_finest_mesh is unavailable. */
+int dec_mg_create_from_csr(int32_t nlevels, const int32_t *sizes,
+                           const int32_t *const *a_rowptr, const int32_t *const *a_col,
+                           const double *const *a_val, const int32_t *const *r_rowptr,
+                           const int32_t *const *r_col, const double *const *r_val,
+                           const int32_t *const *p_rowptr, const int32_t *const *p_col,
+                           const double *const *p_val, int32_t index_base, int32_t steps,
+                           dec_mg *out);
 int dec_mg_nlevels(dec_mg mg, int32_t *nlevels);
 int dec_mg_level_size(dec_mg mg, int32_t level, int32_t *n);
 /* The fine mesh of the hierarchy (owned by the hierarchy; do not destroy). */
@@ -275,8 +291,18 @@ int dec_mg_export_level(dec_mg mg, int32_t level, int32_t which, int64_t *nnz,
  * the sweep order.  Four colours where the base mesh admits them (found by
  * search), else greedy colours. */
 int dec_mg_export_colors(dec_mg mg, int32_t level, int32_t *color, int32_t *ncolors);
+/* The block of each vertex of level l (n_l entries, caller's numbering) for
+ * the hybrid block Gauss-Seidel smoother, and the number of blocks.  No
+ * counterpart in the reference (the smoother is this library's); exported so
+ * that tests can reproduce the sweep.  Levels of at most 3000 rows are one
+ * block. */
+int dec_mg_export_blocks(dec_mg mg, int32_t level, int32_t *block, int32_t *nblocks);
 /* Smoother per level: alg (cg = Krylov.jl's cg as in Multigrid.jl:860,
- * gauss_seidel = multicolour GS, jacobi = damped Jacobi), steps (0 = skip,
+ * gauss_seidel = multicolour GS, jacobi = damped Jacobi, block_gs = hybrid
+ * block Gauss-Seidel: multicolour GS inside blocks of up to 512 vertices
+ * grown over the mesh, neighbours in other blocks taken from the start of
+ * the sweep, one launch per sweep; blocks are exported by
+ * dec_mg_export_blocks), steps (0 = skip,
  * Multigrid.jl:860, 871), omega (Jacobi damping).  coarse_direct=1 solves the
  * coarsest level with a dense float64 operator instead of smoothing it. */
 int dec_mg_set_smoother(dec_mg mg, int32_t alg, int32_t steps, double omega,

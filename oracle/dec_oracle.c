@@ -1104,6 +1104,7 @@ typedef struct {
   const om_csc *A, *R, *S;
   om_csc At;      /* rows of A */
   int *color, ncolors;
+  int *block;     /* OM_BLOCK_GS: block index of each vertex */
   double *Z;      /* coarsest, direct */
 } om_level;
 
@@ -1126,6 +1127,34 @@ static void gs_sweep(om_level *L, const double *b, double *u, int reverse) {
       u[v] = (b[v] - s) / d;
     }
   }
+}
+
+/* One hybrid block Gauss-Seidel sweep (the build's GPU smoother, no
+ * counterpart in the reference): the vertices are split into blocks; within a
+ * block the sweep is the multicolour sweep above, and a neighbour in another
+ * block enters with its value from the start of the sweep,
+ * u[v] = (b[v] - sum_{w != v, ascending} A[v,w] x[w]) / A[v,v] with
+ * x[w] = u[w] when block[w] == block[v], else u_start[w].
+ * With a single block this is gs_sweep. */
+static void block_gs_sweep(om_level *L, const double *b, double *u, int reverse) {
+  const om_csc *At = &L->At;
+  int n = At->n;
+  double *u0 = XALLOC(double, n);
+  memcpy(u0, u, sizeof(double) * n);
+  for (int ci = 0; ci < L->ncolors; ci++) {
+    int c = reverse ? L->ncolors - 1 - ci : ci;
+    for (int v = 0; v < n; v++) {
+      if (L->color[v] != c) continue;
+      double s = 0.0, d = 0.0;
+      for (int k = At->colptr[v]; k < At->colptr[v + 1]; k++) {
+        int w = At->rowval[k];
+        if (w == v) d = At->nzval[k];
+        else s += At->nzval[k] * (L->block[w] == L->block[v] ? u[w] : u0[w]);
+      }
+      u[v] = (b[v] - s) / d;
+    }
+  }
+  free(u0);
 }
 
 /* Damped Jacobi: u += omega * (b - A u) ./ diag(A). */
@@ -1151,6 +1180,7 @@ static void smooth(om_level *L, int alg, int steps, double omega, const double *
   if (steps == 0) return;
   if (alg == OM_CG) om_cg(L->A, b, u, steps);
   else if (alg == OM_GS) for (int s = 0; s < steps; s++) gs_sweep(L, b, u, post);
+  else if (alg == OM_BLOCK_GS) for (int s = 0; s < steps; s++) block_gs_sweep(L, b, u, post);
   else for (int s = 0; s < steps; s++) jacobi_sweep(L, b, u, omega);
 }
 
@@ -1194,16 +1224,19 @@ static void mu_cycle(om_level *lv, int nlev, int lvl, const int *steps, int alg,
  * finest first, concatenated (NULL: greedy colours, om_greedy_colors). */
 static om_level *levels_setup(int nlev, const om_csc *A, const om_csc *R,
                               const om_csc *S, int alg, int coarse_direct,
-                              const int *colors) {
+                              const int *colors, const int *blocks) {
   om_level *lv = XALLOC(om_level, nlev);
   size_t off = 0;
+  int gs = alg == OM_GS || alg == OM_BLOCK_GS;
   for (int l = 0; l < nlev; l++) {
     lv[l].A = A + l;
     lv[l].R = l < nlev - 1 ? R + l : NULL;
     lv[l].S = l < nlev - 1 ? S + l : NULL;
     om_transpose(A + l, &lv[l].At);
     lv[l].color = XALLOC(int, A[l].n);
-    if (alg == OM_GS && colors) {
+    lv[l].block = XALLOC(int, A[l].n);
+    for (int v = 0; v < A[l].n; v++) lv[l].block[v] = blocks ? blocks[off + v] : 0;
+    if (gs && colors) {
       int nc = 0;
       for (int v = 0; v < A[l].n; v++) {
         lv[l].color[v] = colors[off + v];
@@ -1211,7 +1244,7 @@ static om_level *levels_setup(int nlev, const om_csc *A, const om_csc *R,
       }
       lv[l].ncolors = nc;
     } else {
-      lv[l].ncolors = (alg == OM_GS) ? om_greedy_colors(A + l, lv[l].color) : 1;
+      lv[l].ncolors = gs ? om_greedy_colors(A + l, lv[l].color) : 1;
     }
     off += (size_t)A[l].n;
     lv[l].Z = NULL;
@@ -1227,6 +1260,7 @@ static void levels_free(om_level *lv, int nlev) {
   for (int l = 0; l < nlev; l++) {
     om_csc_free(&lv[l].At);
     free(lv[l].color);
+    free(lv[l].block);
     free(lv[l].Z);
   }
   free(lv);
@@ -1235,8 +1269,9 @@ static void levels_free(om_level *lv, int nlev) {
 /* multigrid_μ_cycles (src/Multigrid.jl:819-824). */
 void om_mu_cycles(int nlev, const om_csc *A, const om_csc *R, const om_csc *S,
                   const int *steps, int alg, int mu, int cycles, int coarse_direct,
-                  double omega, double *u, const double *b, const int *colors) {
-  om_level *lv = levels_setup(nlev, A, R, S, alg, coarse_direct, colors);
+                  double omega, double *u, const double *b, const int *colors,
+                  const int *blocks) {
+  om_level *lv = levels_setup(nlev, A, R, S, alg, coarse_direct, colors, blocks);
   for (int c = 0; c < cycles; c++) mu_cycle(lv, nlev, 0, steps, alg, mu, coarse_direct, omega, u, b);
   levels_free(lv, nlev);
 }
@@ -1262,8 +1297,8 @@ static void fmg_rec(om_level *lv, int nlev, int lvl, const int *steps, int alg, 
 void om_full_multigrid(int nlev, const om_csc *A, const om_csc *R, const om_csc *S,
                        const int *steps, int alg, int mu, int cycles,
                        int coarse_direct, double omega, const double *b, double *u,
-                       const int *colors) {
-  om_level *lv = levels_setup(nlev, A, R, S, alg, coarse_direct, colors);
+                       const int *colors, const int *blocks) {
+  om_level *lv = levels_setup(nlev, A, R, S, alg, coarse_direct, colors, blocks);
   fmg_rec(lv, nlev, 0, steps, alg, mu, cycles, coarse_direct, omega, b, u);
   for (int c = 0; c < cycles; c++) mu_cycle(lv, nlev, 0, steps, alg, mu, coarse_direct, omega, u, b);
   levels_free(lv, nlev);

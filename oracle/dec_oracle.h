@@ -107,19 +107,22 @@ void om_laplacian0(const om_csc *d0, const double *h1, const om_csc *h1_geo,
                    const double *inv_h0, om_csc *L);
 
 /* ---- multigrid (src/Multigrid.jl + project smoothers) ---- */
-enum { OM_CG = 0, OM_GS = 1, OM_JACOBI = 2 };
+enum { OM_CG = 0, OM_GS = 1, OM_JACOBI = 2, OM_BLOCK_GS = 3 };
 int om_greedy_colors(const om_csc *A, int *color);
 void om_cg(const om_csc *A, const double *b, double *x, int itmax);
 void om_coarse_inverse(const om_csc *A, double *Z);
 void om_mu_cycles(int nlev, const om_csc *A, const om_csc *R, const om_csc *S,
                   const int *steps, int alg, int mu, int cycles, int coarse_direct,
-                  double omega, double *u, const double *b, const int *colors);
+                  double omega, double *u, const double *b, const int *colors,
+                  const int *blocks);
 void om_full_multigrid(int nlev, const om_csc *A, const om_csc *R, const om_csc *S,
                        const int *steps, int alg, int mu, int cycles,
                        int coarse_direct, double omega, const double *b, double *u,
-                       const int *colors);
+                       const int *colors, const int *blocks);
 /* colors: NULL, or every level's Gauss-Seidel colour array (finest first)
- * concatenated; NULL colours each level with om_greedy_colors. */
+ * concatenated; NULL colours each level with om_greedy_colors.
+ * blocks: for OM_BLOCK_GS, every level's block index per vertex (finest
+ * first, concatenated; NULL: one block per level, which is multicolour GS). */
 
 /* ---- inputs ---- */
 void om_uniform_form(unsigned long long seed, int n, double *x);

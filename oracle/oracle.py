@@ -515,7 +515,7 @@ def _f64(x):
 # --------------------------------------------------------------------------
 # Multigrid
 # --------------------------------------------------------------------------
-CG, GS, JACOBI = 0, 1, 2
+CG, GS, JACOBI, BLOCK_GS = 0, 1, 2, 3
 
 
 class Hierarchy:
@@ -570,22 +570,28 @@ class Hierarchy:
         assert flat.size == sum(a.m for a in self.A)
         return flat, flat.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
 
-    def mu_cycles(self, u, b, cycles, alg=CG, mu=1, steps=3, coarse_direct=False, omega=2 / 3, colors=None):
+    def mu_cycles(self, u, b, cycles, alg=CG, mu=1, steps=3, coarse_direct=False, omega=2 / 3, colors=None,
+                  blocks=None):
+        """blocks (alg=BLOCK_GS): one block-index array per level, finest
+        first (None: one block per level)."""
         nl, A, R, S = self._structs()
         st = np.full(nl, steps, np.int32) if np.isscalar(steps) else np.ascontiguousarray(steps, np.int32)
         u = _f64(u).copy()
         keep, cp = self._colors(colors)
+        keepb, bp = self._colors(blocks)
         lib().om_mu_cycles(c_int(nl), A, R, S, _p(st), c_int(alg), c_int(mu), c_int(cycles),
-                           c_int(int(coarse_direct)), c_double(omega), _p(u), _p(_f64(b)), cp)
+                           c_int(int(coarse_direct)), c_double(omega), _p(u), _p(_f64(b)), cp, bp)
         return u
 
-    def full_multigrid(self, b, cycles, alg=CG, mu=1, steps=3, coarse_direct=False, omega=2 / 3, colors=None):
+    def full_multigrid(self, b, cycles, alg=CG, mu=1, steps=3, coarse_direct=False, omega=2 / 3, colors=None,
+                       blocks=None):
         nl, A, R, S = self._structs()
         st = np.full(nl, steps, np.int32) if np.isscalar(steps) else np.ascontiguousarray(steps, np.int32)
         u = np.zeros(self.A[0].m)
         keep, cp = self._colors(colors)
+        keepb, bp = self._colors(blocks)
         lib().om_full_multigrid(c_int(nl), A, R, S, _p(st), c_int(alg), c_int(mu), c_int(cycles),
-                                c_int(int(coarse_direct)), c_double(omega), _p(_f64(b)), _p(u), cp)
+                                c_int(int(coarse_direct)), c_double(omega), _p(_f64(b)), _p(u), cp, bp)
         return u
 
 

@@ -185,3 +185,127 @@ def test_fused_and_graph_paths_are_bit_identical(O, cs, variant, monkeypatch):
     slow = run()
     for (u1, c1, r1), (u2, c2, r2) in zip(fast, slow):
         assert c1 == c2 and r1 == r2 and torch.equal(u1, u2)
+
+
+@pytest.fixture(scope="module")
+def sphere5_pair(O, cs):
+    """Levels 10,242 .. 12: the finest level (above 3,000 rows) is split into
+    blocks and renumbered; the others are one block each."""
+    H = O.Hierarchy(O.icosahedron(), 5, project=True, geometric=False)
+    md = cs.MultigridData(cs.icosphere(0), cs.BinarySubdivision, 5, cs.DiagonalHodge, 2, project=True)
+    return H, md
+
+
+def test_blocks_partition_levels(sphere5_pair):
+    """Blocks cover every vertex, hold at most 512 vertices, and each block of
+    the finest level is one connected patch; small levels are one block."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+
+    H, md = sphere5_pair
+    for lv in range(md.nlevels):
+        blk, nb = md.blocks(lv)
+        n = md.sizes[lv]
+        assert blk.min() == 0 and blk.max() == nb - 1 and len(np.unique(blk)) == nb
+        if n <= 3000:
+            assert nb == 1
+            continue
+        sizes = np.bincount(blk)
+        assert sizes.max() <= 512 and nb >= n // 512
+        rp, ci, _ = md.export(lv, "A")
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        same = blk[rows] == blk[ci]
+        G = sp.csr_matrix((np.ones(same.sum()), (rows[same], ci[same])), shape=(n, n))
+        ncomp, lab = connected_components(G, directed=False)
+        assert ncomp <= nb + 8          # BFS-grown: almost every block is one patch
+        assert same.mean() > 0.6        # most couplings stay inside a block
+
+
+def test_block_gs_matches_oracle(O, cs, sphere5_pair):
+    """Hybrid block Gauss-Seidel V(2,2) cycles against the oracle's sweep with
+    the library's colours and blocks (the direct coarse solve sums in another
+    order, so to 1e-10)."""
+    import torch
+
+    H, md = sphere5_pair
+    L = H.A[0]
+    f = sphere_rhs(O, H.duals[0], 5)
+    cols = [md.colors(lv)[0] for lv in range(md.nlevels)]
+    blks = [md.blocks(lv)[0] for lv in range(md.nlevels)]
+    u_o = H.mu_cycles(np.zeros(L.n), f, 3, alg=O.BLOCK_GS, steps=2, coarse_direct=True, colors=cols, blocks=blks)
+    u = torch.zeros(L.n, dtype=torch.float64, device="cuda")
+    cs.multigrid_vcycles(u, torch.tensor(f, device="cuda"), md, 3, cs.block_gauss_seidel, steps=2,
+                         coarse_direct=True)
+    assert np.linalg.norm(u.cpu().numpy() - u_o) <= 1e-10 * np.linalg.norm(u_o)
+    # odd sweep count (the last sweep copied back) and W-cycles: same check
+    u_o = H.mu_cycles(np.zeros(L.n), f, 2, alg=O.BLOCK_GS, mu=2, steps=3, coarse_direct=True, colors=cols,
+                      blocks=blks)
+    u.zero_()
+    cs.multigrid_wcycles(u, torch.tensor(f, device="cuda"), md, 2, cs.block_gauss_seidel, steps=3,
+                         coarse_direct=True)
+    assert np.linalg.norm(u.cpu().numpy() - u_o) <= 1e-10 * np.linalg.norm(u_o)
+
+
+@pytest.mark.parametrize("variant", ["plain", "mixed"])
+def test_block_gs_solve(O, cs, sphere5_pair, variant):
+    """The solve with the hybrid smoother reaches 1e-8 (checked by the oracle's
+    float64 operator) in no more cycles than multicolour Gauss-Seidel plus one."""
+    import torch
+
+    H, md = sphere5_pair
+    L = H.A[0]
+    f = sphere_rhs(O, H.duals[0], 3)
+    df = torch.tensor(f, device="cuda")
+    u, cb, rr = cs.solve(md, df, tol=1e-8, variant=variant, alg=cs.block_gauss_seidel)
+    true = np.linalg.norm(f - L.matvec(u.cpu().numpy())) / np.linalg.norm(f)
+    assert rr <= 1e-8 and true <= 1.01e-8
+    _, cg_, _ = cs.solve(md, df, tol=1e-8, variant=variant, alg=cs.gauss_seidel)
+    assert cb <= cg_ + 1
+
+
+def test_hierarchy_from_matrices(O, cs, sphere5_pair):
+    """MultigridData(ops, rs, ps, steps) (Multigrid.jl:545-546): the oracle's own
+    CSR matrices handed to the library give the oracle's cycles (CG smoother
+    of the reference, and Gauss-Seidel with the library's greedy colours),
+    and the same CG iterates as the hierarchy the library builds itself."""
+    import torch
+
+    H, md = sphere5_pair
+    csr = lambda M: M.csr()  # noqa: E731
+    mdm = cs.MultigridData.from_matrices([csr(a) for a in H.A], [csr(r) for r in H.R],
+                                         [csr(s.transpose()) for s in H.S], steps=3)
+    assert mdm.sizes == md.sizes
+    for lv in range(mdm.nlevels):
+        csr_equal(mdm.export(lv, "A"), H.A[lv].csr())
+        if lv + 1 < mdm.nlevels:
+            csr_equal(mdm.export(lv, "R"), H.R[lv].csr())
+            csr_equal(mdm.export(lv, "P"), H.S[lv].transpose().csr())
+    L = H.A[0]
+    f = sphere_rhs(O, H.duals[0], 8)
+    df = torch.tensor(f, device="cuda")
+    u_o = H.mu_cycles(np.zeros(L.n), f, 2, alg=O.CG, steps=3)
+    u = cs.multigrid_vcycles(torch.zeros(L.n, dtype=torch.float64, device="cuda"), df, mdm, 2, steps=3)
+    assert np.linalg.norm(u.cpu().numpy() - u_o) <= 1e-8 * np.linalg.norm(u_o)
+    cols = [mdm.colors(lv)[0] for lv in range(mdm.nlevels)]
+    u_o = H.mu_cycles(np.zeros(L.n), f, 3, alg=O.GS, steps=2, coarse_direct=True, colors=cols)
+    u = cs.multigrid_vcycles(torch.zeros(L.n, dtype=torch.float64, device="cuda"), df, mdm, 3, cs.gauss_seidel,
+                             steps=2, coarse_direct=True)
+    assert np.linalg.norm(u.cpu().numpy() - u_o) <= 1e-10 * np.linalg.norm(u_o)
+    # a solve to 1e-8 on the matrix-built hierarchy, verified by the oracle
+    u, c, rr = cs.solve(mdm, df, tol=1e-8)
+    assert rr <= 1e-8 and np.linalg.norm(f - L.matvec(u.cpu().numpy())) / np.linalg.norm(f) <= 1.01e-8
+
+
+def test_from_matrices_rejects_bad_input(cs):
+    import scipy.sparse as sp
+
+    from cspaces._lib import DecError
+
+    A = sp.identity(4, format="csr") * 2.0
+    R = sp.csr_matrix(np.ones((2, 4)) * 0.25)
+    P = sp.csr_matrix(np.ones((4, 3)))          # wrong shape: 4 x 3, coarse level has 2
+    with pytest.raises(DecError):
+        cs.MultigridData.from_matrices([A, sp.identity(2, format="csr")], [R], [P])
+    Z = sp.csr_matrix((4, 4))                   # no diagonal
+    with pytest.raises(DecError):
+        cs.MultigridData.from_matrices([Z], [], [])

@@ -156,3 +156,26 @@ def test_dual_laplacians_have_the_sphere_spectrum(O):
     np.testing.assert_allclose(ev1[:6], -2.0, rtol=0.03)
     np.testing.assert_allclose(ev1[6:16], -6.0, rtol=0.03)
     assert ev1[16] < -11.0
+
+
+def test_literal_pinv_sharp_is_rounding_noise_on_spheres(O):
+    """The reference's ♯ takes LinearAlgebra.pinv (default cut 3 eps σmax) of
+    the 3x3 matrix of dual-edge vectors.  On a curved surface those rows lie
+    in the triangle's plane up to rounding, so the third singular value is
+    rounding noise above the cut and gets inverted: ♭♯ built that way moves by
+    O(1) when the dual points move by one unit in the last place, so no
+    implementation can reproduce the reference's values there.  The in-plane
+    pseudo-inverse (the library's ♯) moves by rounding only, and both forms
+    agree on flat meshes (the meshes of the reference's tests)."""
+    import musical as M
+
+    d = O.Dual(O.icosphere(3))
+    lit, proj = M.flat_sharp_mat(d, project=False), M.flat_sharp_mat(d, project=True)
+    d2 = O.Dual(O.icosphere(3))
+    rng = np.random.default_rng(0)
+    d2.dual_point = d2.dual_point * (1 + rng.choice([-1.0, 1.0], d2.dual_point.shape) * 2.0 ** -52)
+    lit2, proj2 = M.flat_sharp_mat(d2, project=False), M.flat_sharp_mat(d2, project=True)
+    rel = lambda a, b: abs(a - b).max() / abs(b).max()  # noqa: E731
+    assert rel(lit2, lit) > 0.5                 # one ulp in the points: an O(1) change
+    assert rel(proj2, proj) < 1e-12             # the in-plane form is well conditioned
+    assert rel(lit, proj) > 0.5
