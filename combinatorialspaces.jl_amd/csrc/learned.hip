@@ -100,43 +100,6 @@ __global__ void k_scale_partial(int n, const float *__restrict__ r, double *__re
   if (threadIdx.x == 0) part[static_cast<size_t>(blockIdx.y) * kScaleBlocks + blockIdx.x] = sh[0];
 }
 
-// One residual: k_scale_partial's sums, and the last workgroup to finish
-// adds the partials in block order and writes s; it also resets the ticket
-// counter for the next launch (so the kernel replays from a graph).  The
-// last workgroup loads the partials with one thread each into LDS, then sums
-// them serially there, in the same order as k_scale_final.
-__global__ void k_scale_one(int n, const float *__restrict__ r, double *__restrict__ part,
-                            unsigned int *__restrict__ ticket, float *__restrict__ s) {
-  static_assert(kScaleBlocks <= TB, "one partial per thread of the last workgroup");
-  __shared__ double sh[TB];
-  __shared__ bool last;
-  double acc = 0.0;
-  for (int i = blockIdx.x * TB + threadIdx.x; i < n; i += kScaleBlocks * TB) acc += static_cast<double>(r[i]) * r[i];
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = TB / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = sh[0];
-    __threadfence();
-    last = atomicAdd(ticket, 1u) == kScaleBlocks - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (threadIdx.x < kScaleBlocks) sh[threadIdx.x] = part[threadIdx.x];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double tot = 0.0;
-    for (int j = 0; j < kScaleBlocks; j++) tot += sh[j];
-    double m = tot / n;
-    s[0] = m > 0 ? static_cast<float>(sqrt(m)) : 1.0f;
-    *ticket = 0u;
-  }
-}
-
 __global__ void k_scale_final(int n, int B, const double *__restrict__ part, float *__restrict__ s) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
@@ -355,11 +318,14 @@ __device__ __forceinline__ uint4 pack8(const float *f) {
   return w;
 }
 
-// xp[v] = (x, P x) with x = r / s.  W > 0 is the width of P's rows: all
-// slots' loads are issued before the sums, which run in slot order.
+// xp[v] = (x, P x) with x = r / s, and (h1 non-null) the row's layer-1
+// features h1 = relu([x, P x] W1 + b1) in BF16, the values k_input stores.
+// W > 0 is the width of P's rows: all slots' loads are issued before the
+// sums, which run in slot order.
 template <int W>
 __global__ void k_inf_input(int n, int pw, const int *__restrict__ pcol, const float *__restrict__ pval,
-                            const float *__restrict__ r, const float *__restrict__ s, float2 *__restrict__ xp) {
+                            const float *__restrict__ r, const float *__restrict__ s, float2 *__restrict__ xp,
+                            const float *__restrict__ th, unsigned short *__restrict__ h1) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   float inv = 1.0f / s[0];
@@ -388,6 +354,17 @@ __global__ void k_inf_input(int n, int pw, const int *__restrict__ pcol, const f
     }
   }
   xp[v] = make_float2(xv, acc);
+  if (h1) {
+    float hh[H];
+#pragma unroll
+    for (int j = 0; j < H; j++) {
+      float z = xv * th[OW1 + j] + acc * th[OW1 + H + j] + th[OB1 + j];
+      hh[j] = z > 0.0f ? z : 0.0f;
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(h1 + static_cast<size_t>(v) * H);
+#pragma unroll
+    for (int k4 = 0; k4 < 4; k4++) dst[k4] = pack8(hh + 8 * k4);
+  }
 }
 
 // m += Σ_k P[v, k] bf16(relu(h1 of neighbour k)), ascending slot order, with
@@ -506,6 +483,52 @@ __device__ __forceinline__ void zero_row(unsigned short *my) {
   uint4 *dst = reinterpret_cast<uint4 *>(my);
 #pragma unroll
   for (int k = 0; k < 8; k++) dst[k] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// h2 = relu([h1, P h1] W2 + b2) from the stored layer-1 rows: each lane
+// gathers its row's neighbours' h1 (the sums of k_spmm, in slot order).
+template <int W>
+__global__ void __launch_bounds__(TB) k_inf_layer2s(int n, int pw, const int *__restrict__ pcol,
+                                                    const float *__restrict__ pval,
+                                                    const unsigned short *__restrict__ h1,
+                                                    const float *__restrict__ th, unsigned short *__restrict__ h2) {
+  __shared__ __align__(16) unsigned short rows_s[kInfWaves][64 * kRowPitch];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int li = lane & 15, lg = lane >> 4;
+  unsigned short *rows = rows_s[w];
+  short4_t Bw[2][4];
+  load_wB(th + OW2, li, lg, Bw);
+  float bias0 = th[OB2 + li], bias1 = th[OB2 + 16 + li];
+  int nsteps = (n + 63) / 64;
+  for (int base = blockIdx.x * kInfWaves; base < nsteps; base += gridDim.x * kInfWaves) {
+    int t = base + w;
+    int v = t * 64 + lane;
+    bool live = t < nsteps && v < n;
+    unsigned short *my = rows + lane * kRowPitch;
+    if (live) gather_rows<W>(n, pw, v, pcol, pval, h1, my);
+    else zero_row(my);
+    __syncthreads();
+    float4_t acc[4][2];
+    mfma_rows(rows, li, lg, Bw, acc);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        unsigned short *o = rows + (16 * q + 4 * lg + i) * kRowPitch;
+        float z0 = acc[q][0][i] + bias0, z1 = acc[q][1][i] + bias1;
+        o[li] = f2bf(z0 > 0.0f ? z0 : 0.0f);
+        o[16 + li] = f2bf(z1 > 0.0f ? z1 : 0.0f);
+      }
+    __syncthreads();
+    if (live) {
+      const uint4 *src = reinterpret_cast<const uint4 *>(my);
+      uint4 *dst = reinterpret_cast<uint4 *>(h2 + static_cast<size_t>(v) * H);
+#pragma unroll
+      for (int k = 0; k < 4; k++) dst[k] = src[k];
+    }
+    __syncthreads();
+  }
 }
 
 template <int W>
@@ -1263,7 +1286,6 @@ struct dec_nn_s {
   // work buffers (grown on demand)
   DBuf<float> x, px, y, c, s, drho, dg, part, r, eps, tmp, crs;
   DBuf<double> dpart, dloss, spart, rtmp;
-  DBuf<unsigned int> ticket;  // inference scale: workgroups finished
   DBuf<float2> ix;                                 // inference: (x, P x)
   DBuf<unsigned short> ih1, ih2;                   // inference: layer-1 and layer-2 rows
   DBuf<unsigned char> h1, hm1, h2, hm2, h3, dz;  // activation storage (bf16 or f32)
@@ -1443,11 +1465,6 @@ void reserve_inference(dec_nn_s &nn, int n) {
   if (nn.ih2.n < static_cast<size_t>(n) * H) nn.ih2.alloc(static_cast<size_t>(n) * H);
   if (nn.s.n < 1) nn.s.alloc(1);
   if (nn.spart.n < static_cast<size_t>(kScaleBlocks)) nn.spart.alloc(kScaleBlocks);
-  if (nn.ticket.n < 1) {
-    nn.ticket.alloc(1);
-    std::vector<unsigned int> z(1, 0u);
-    nn.ticket.upload(z);
-  }
 }
 
 void nn_bind(dec_nn_s &nn, const std::vector<int> &sizes) {
@@ -1470,10 +1487,16 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s, b
   } L{internal ? NL.pi : NL.p, internal ? NL.Di : NL.D};
   int n = NL.n;
   reserve_inference(nn, n);
-  hipLaunchKernelGGL(k_scale_one, dim3(kScaleBlocks), dim3(TB), 0, s, n, r, nn.spart.p, nn.ticket.p, nn.s.p);
+  // the scale in two launches (one with a last-workgroup ticket needs a
+  // device-scope fence per workgroup, which writes back the XCD's L2)
+  hipLaunchKernelGGL(k_scale_partial, dim3(kScaleBlocks, 1), dim3(TB), 0, s, n, r, nn.spart.p);
+  hipLaunchKernelGGL(k_scale_final, dim3(1), dim3(64), 0, s, n, 1, nn.spart.p, nn.s.p);
+  bool big = n > kInfSmallRows;
+  unsigned short *h1 = big ? nn.ih1.p : nullptr;
   DEC_ELL_WIDTH_SWITCH(L.p.width, hipLaunchKernelGGL((k_inf_input<W>), dim3(dec::nblocks(n, TB)), dim3(TB), 0, s, n,
-                                                      L.p.width, L.p.col.p, L.p.val.p, r, nn.s.p, nn.ix.p));
-  if (n <= kInfSmallRows) {
+                                                      L.p.width, L.p.col.p, L.p.val.p, r, nn.s.p, nn.ix.p, nn.th.p,
+                                                      h1));
+  if (!big) {
     int tiles = (n + 15) / 16;
     int blocks = std::max(1, (tiles + kInfWaves - 1) / kInfWaves);
     hipLaunchKernelGGL(k_inf_layer2_t, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ix.p,
@@ -1484,8 +1507,8 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s, b
     int steps = (n + 63) / 64;
     int blocks = std::max(1, (steps + kInfWaves - 1) / kInfWaves);
     DEC_ELL_WIDTH_SWITCH(L.p.width, {
-      hipLaunchKernelGGL((k_inf_layer2<W>), dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p,
-                         nn.ix.p, nn.th.p, nn.ih2.p);
+      hipLaunchKernelGGL((k_inf_layer2s<W>), dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p,
+                         nn.ih1.p, nn.th.p, nn.ih2.p);
       hipLaunchKernelGGL((k_inf_layer3<W>), dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p,
                          nn.ih2.p, nn.ix.p, nn.th.p, nn.s.p, L.D.p, u);
     });

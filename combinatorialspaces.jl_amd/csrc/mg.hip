@@ -47,6 +47,50 @@ __global__ void k_residual(int n, int width, const int *__restrict__ col, const 
   r[v] = b[v] - acc;
 }
 
+// The residual r = b - A u of k_residual and, per workgroup, the sum of its
+// rows' r² (a fixed tree) into part[blockIdx.x]; k_sum_rows adds the
+// partial sums.  (A ticket letting the last workgroup add them needs a
+// device-scope fence in every workgroup, which writes back the XCD's L2:
+// measured six times slower.)
+template <int W>
+__global__ void __launch_bounds__(256)
+    k_residual_norm(int n, int width, const int *__restrict__ col, const double *__restrict__ val,
+                    const int *__restrict__ rowid, const double *__restrict__ b, const double *__restrict__ u,
+                    double *__restrict__ r, double *__restrict__ part) {
+  __shared__ double sh[256];
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0;
+  if (p < n) {
+    double acc = dec_ell::row<W, double, double>(n, width, p, col, val, u);
+    int v = rowid[p];
+    double ri = b[v] - acc;
+    r[v] = ri;
+    rr = ri * ri;
+  }
+  sh[threadIdx.x] = rr;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// *out = Σ part[0..nb) in a fixed order: 256 consecutive runs, then a tree.
+__global__ void __launch_bounds__(256) k_sum_rows(int nb, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double sh[256];
+  int per = (nb + 255) / 256;
+  double acc = 0.0;
+  for (int j = threadIdx.x * per; j < min(nb, (static_cast<int>(threadIdx.x) + 1) * per); j++) acc = acc + part[j];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
 // y[v] = (A x)[v]
 template <int W, class T>
 __global__ void k_spmv_rows(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
@@ -105,6 +149,46 @@ __global__ void k_restrict(int nc, int width, const int *__restrict__ col, const
   if (uc) uc[i] = static_cast<T>(0);
 }
 
+// k_residual and k_restrict in one launch for levels where launches, not
+// bytes, bound the cycle.  Eight lanes per coarse row i: lane s evaluates
+// r_j = b_j - (A u)_j for the row's slot s (j = R's column, fine row j at
+// ELL position pos[j], summed as k_residual does) and multiplies it by R's
+// entry; lane 0 then adds the eight products in slot order, so b_c is
+// bit-identical to the two launches.  Fine rows shared by two coarse rows are
+// evaluated twice.  R's width is at most 8.
+template <int W, class T>
+__global__ void __launch_bounds__(256)
+    k_residual_restrict(int nc, int rwidth, const int *__restrict__ rcol, const T *__restrict__ rval, int n,
+                        int width, const int *__restrict__ col, const T *__restrict__ val, const int *__restrict__ pos,
+                        const T *__restrict__ b, const T *__restrict__ u, T *__restrict__ bc, T *__restrict__ uc) {
+  __shared__ T prod[256];
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = t >> 3, sl = t & 7;
+  T pr = static_cast<T>(0);
+  bool has = false;
+  if (i < nc && sl < rwidth) {
+    size_t q = static_cast<size_t>(sl) * nc + i;
+    int j = rcol[q];
+    if (j >= 0) {
+      T rj = b[j] - dec_ell::row<W, T, T>(n, width, pos[j], col, val, u);
+      pr = rval[q] * rj;
+      has = true;
+    }
+  }
+  prod[threadIdx.x] = pr;
+  unsigned long long m = __ballot(has);
+  __syncthreads();
+  if (sl == 0 && i < nc) {
+    T acc = static_cast<T>(0);
+    int base = threadIdx.x;
+    unsigned lanebits = static_cast<unsigned>((m >> (threadIdx.x & 63)) & 0xffu);
+    for (int k = 0; k < 8; k++)
+      if (lanebits & (1u << k)) acc = acc + prod[base + k];
+    bc[i] = acc;
+    if (uc) uc[i] = static_cast<T>(0);
+  }
+}
+
 // u_f += P z_c, P in ELL rows (for a binary subdivision P = transpose(S):
 // a coarse vertex copies its value with weight 1, a midpoint takes 0.5 of its
 // edge's endpoints in the order (min, max) of S's column).
@@ -114,6 +198,14 @@ __global__ void k_prolong_add(int nf, int width, const int *__restrict__ pcol, c
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nf) return;
   uf[j] = uf[j] + dec_ell::row<W, T, T>(nf, width, j, pcol, pval, zc);
+}
+
+// u[j], or with zc given u[j] + (P zc)[j] (P an ELL of width 2: the sum of
+// k_prolong_add), for sweeps that apply the prolongation themselves.
+template <class T>
+__device__ __forceinline__ T prolonged(int n, int j, const T *__restrict__ u, const int *__restrict__ pcol,
+                                       const T *__restrict__ pval, const T *__restrict__ zc) {
+  return zc ? u[j] + dec_ell::row<2, T, T>(n, 2, j, pcol, pval, zc) : u[j];
 }
 
 // One hybrid block Gauss-Seidel sweep.  The level's vertices are numbered so
@@ -126,16 +218,18 @@ __global__ void k_prolong_add(int nf, int width, const int *__restrict__ pcol, c
 // sweep).  The result goes to uout, so no workgroup reads a value another one
 // writes in the same launch.  Per row: the sum over the off-diagonal entries
 // in slot (ascending column) order, then (b - sum) / diagonal.
-template <int W, class T>
+template <int W, class T, bool FOLD>
 __global__ void __launch_bounds__(256)
     k_block_gs(int n, int width, int ncolors, int nb, const int *__restrict__ col, const T *__restrict__ val,
                const int *__restrict__ rowid, const int *__restrict__ bstart, const int *__restrict__ seg,
-               const T *__restrict__ b, const T *__restrict__ uin, T *__restrict__ uout, int reverse) {
+               const T *__restrict__ b, const T *__restrict__ uin, T *__restrict__ uout, int reverse,
+               const int *__restrict__ pcol, const T *__restrict__ pval, const T *__restrict__ zc_arg) {
+  const T *zc = FOLD ? zc_arg : nullptr;  // folded prolongation only in the FOLD instance
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *x = reinterpret_cast<T *>(smem_raw);
   int k = blockIdx.x;
   int r0 = bstart[k], nr = bstart[k + 1] - r0;
-  for (int i = threadIdx.x; i < nr; i += blockDim.x) x[i] = uin[r0 + i];
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) x[i] = prolonged(n, r0 + i, uin, pcol, pval, zc);
   __syncthreads();
   for (int ci = 0; ci < ncolors; ci++) {
     int c = reverse ? ncolors - 1 - ci : ci;
@@ -155,7 +249,8 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int q = 0; q < W; q++) {
           unsigned loc = static_cast<unsigned>(cc[q] - r0);
-          xv[q] = (cc[q] < 0 || loc < static_cast<unsigned>(nr)) ? static_cast<T>(0) : uin[cc[q]];
+          xv[q] = (cc[q] < 0 || loc < static_cast<unsigned>(nr)) ? static_cast<T>(0)
+                                                                 : prolonged(n, cc[q], uin, pcol, pval, zc);
         }
 #pragma unroll
         for (int q = 0; q < W; q++) {
@@ -171,13 +266,116 @@ __global__ void __launch_bounds__(256)
           if (cq < 0) break;
           unsigned loc = static_cast<unsigned>(cq - r0);
           if (cq == v) d = val[o];
-          else s = s + val[o] * (loc < static_cast<unsigned>(nr) ? x[loc] : uin[cq]);
+          else s = s + val[o] * (loc < static_cast<unsigned>(nr) ? x[loc] : prolonged(n, cq, uin, pcol, pval, zc));
         }
       }
       x[v - r0] = (b[v] - s) / d;
     }
     __syncthreads();
   }
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) uout[r0 + i] = x[i];
+}
+
+// The same sweep with every memory access issued before the colour loop, so
+// that the colour phases touch only LDS and registers.  Thread t owns the
+// block's rows t + i·blockDim.x (i < R) of the colour-major row list (the
+// concatenated (colour, block) segments) and holds their slots, values, b
+// and the values of their neighbours in other blocks.  With zc given, the
+// sweep starts from u + P zc (the prolongation of the coarse correction,
+// P an ELL of width 2, the sums of k_prolong_add), both for the block's rows
+// and for those neighbours.  The per-row arithmetic is k_block_gs's.
+template <int W, class T, int R>
+struct OwnedRows {
+  int v[R], c[R];
+  int col[R][W];
+  T val[R][W], halo[R][W], b[R];
+};
+
+template <int W, class T, int R>
+__device__ __forceinline__ void load_owned(OwnedRows<W, T, R> &o, int k, int r0, int nr, int ncolors, int nb, int n,
+                                           const int *__restrict__ col, const T *__restrict__ val,
+                                           const int *__restrict__ rowid, const int *__restrict__ seg,
+                                           const T *__restrict__ b, const T *__restrict__ uin,
+                                           const int *__restrict__ pcol, const T *__restrict__ pval,
+                                           const T *__restrict__ zc) {
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    int q = static_cast<int>(threadIdx.x) + i * static_cast<int>(blockDim.x);
+    o.v[i] = -1;
+    o.c[i] = -1;
+    if (q >= nr) continue;
+    int acc = 0, p = -1;
+    for (int c = 0; c < ncolors; c++) {
+      int s0 = seg[c * (nb + 1) + k], s1 = seg[c * (nb + 1) + k + 1];
+      if (q < acc + (s1 - s0)) {
+        p = s0 + (q - acc);
+        o.c[i] = c;
+        break;
+      }
+      acc += s1 - s0;
+    }
+    o.v[i] = rowid[p];
+#pragma unroll
+    for (int s = 0; s < W; s++) {
+      size_t off = static_cast<size_t>(s) * n + p;
+      o.col[i][s] = col[off];
+      o.val[i][s] = val[off];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    if (o.v[i] < 0) continue;
+    o.b[i] = b[o.v[i]];
+#pragma unroll
+    for (int s = 0; s < W; s++) {
+      int cq = o.col[i][s];
+      unsigned loc = static_cast<unsigned>(cq - r0);
+      o.halo[i][s] = (cq < 0 || loc < static_cast<unsigned>(nr)) ? static_cast<T>(0)
+                                                                   : prolonged(n, cq, uin, pcol, pval, zc);
+    }
+  }
+}
+
+// The colour phases of one sweep over the owned rows (x: the block's iterate in LDS).
+template <int W, class T, int R>
+__device__ __forceinline__ void sweep_owned(const OwnedRows<W, T, R> &o, T *x, int r0, int nr, int ncolors,
+                                            bool reverse) {
+  for (int ci = 0; ci < ncolors; ci++) {
+    int c = reverse ? ncolors - 1 - ci : ci;
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+      if (o.c[i] != c) continue;
+      int v = o.v[i];
+      T s = static_cast<T>(0), d = static_cast<T>(0);
+#pragma unroll
+      for (int q = 0; q < W; q++) {
+        int cq = o.col[i][q];
+        if (cq < 0) continue;
+        unsigned loc = static_cast<unsigned>(cq - r0);
+        if (cq == v) d = o.val[i][q];
+        else s = s + o.val[i][q] * (loc < static_cast<unsigned>(nr) ? x[loc] : o.halo[i][q]);
+      }
+      x[v - r0] = (o.b[i] - s) / d;
+    }
+    __syncthreads();
+  }
+}
+
+template <int W, class T, int R>
+__global__ void __launch_bounds__(256)
+    k_block_gs_pf(int n, int ncolors, int nb, const int *__restrict__ col, const T *__restrict__ val,
+                  const int *__restrict__ rowid, const int *__restrict__ bstart, const int *__restrict__ seg,
+                  const T *__restrict__ b, const T *__restrict__ uin, T *__restrict__ uout, int reverse,
+                  const int *__restrict__ pcol, const T *__restrict__ pval, const T *__restrict__ zc) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *x = reinterpret_cast<T *>(smem_raw);
+  int k = blockIdx.x;
+  int r0 = bstart[k], nr = bstart[k + 1] - r0;
+  OwnedRows<W, T, R> o;
+  load_owned<W, T, R>(o, k, r0, nr, ncolors, nb, n, col, val, rowid, seg, b, uin, pcol, pval, zc);
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) x[i] = prolonged(n, r0 + i, uin, pcol, pval, zc);
+  __syncthreads();
+  sweep_owned<W, T, R>(o, x, r0, nr, ncolors, reverse != 0);
   for (int i = threadIdx.x; i < nr; i += blockDim.x) uout[r0 + i] = x[i];
 }
 
@@ -496,6 +694,76 @@ __global__ void __launch_bounds__(kFuseThreads)
   }
 }
 
+// The coarse tail when it is one smoothed level A (at most 3 x 1024 rows,
+// ELL width 1-8) above the dense coarsest level C, in Gauss-Seidel V-cycles:
+// the same phases and per-row arithmetic as k_fused_vcycle with A's rows held
+// in registers (thread t owns positions t + 1024 i) and its vectors in LDS, so
+// the colour phases wait on no memory access.
+//   phase 1: A's pre-smoothing, residual and restriction; C's b and its
+//            residual (C's iterate starts at zero, so that residual is b).
+//   phase 2: C's correction from k_coarse_partial's chunks, the prolongation
+//            to A and A's post-smoothing.
+template <int W, class T, int R>
+__global__ void __launch_bounds__(kFuseThreads)
+    k_tail2(const LevelDesc<T> A, const LevelDesc<T> C, T *__restrict__ uA, const T *__restrict__ bA, int steps,
+            int phase, const T *__restrict__ part) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  int n = A.n;
+  T *x = reinterpret_cast<T *>(smem_raw);
+  T *w = x + n;  // phase 1: A's residual; phase 2: C's correction
+  OwnedRows<W, T, R> o;
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    int q = static_cast<int>(threadIdx.x) + i * kFuseThreads;
+    o.v[i] = -1;
+    o.c[i] = -1;
+    if (q >= n) continue;
+    int c = 0;
+    while (c + 1 < A.ncolors && q >= A.cstart[c + 1]) c++;
+    o.c[i] = c;
+    o.v[i] = A.rowid[q];
+#pragma unroll
+    for (int sl = 0; sl < W; sl++) {
+      size_t off = static_cast<size_t>(sl) * n + q;
+      o.col[i][sl] = A.col[off];
+      o.val[i][sl] = A.val[off];
+    }
+    o.b[i] = bA[o.v[i]];
+  }
+  if (phase == 1) {
+    for (int i = threadIdx.x; i < n; i += kFuseThreads) x[i] = uA[i];
+    __syncthreads();
+    for (int it = 0; it < steps; it++) sweep_owned<W, T, R>(o, x, 0, n, A.ncolors, false);
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+      if (o.v[i] < 0) continue;
+      T acc = static_cast<T>(0);
+#pragma unroll
+      for (int sl = 0; sl < W; sl++)
+        if (o.col[i][sl] >= 0) acc = acc + o.val[i][sl] * x[o.col[i][sl]];
+      w[o.v[i]] = o.b[i] - acc;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < A.nc; i += kFuseThreads) {
+      T bc = A.rwidth == 7 ? dec_ell::row<7, T, T>(A.nc, A.rwidth, i, A.rcol, A.rval, w)
+                           : dec_ell::row<0, T, T>(A.nc, A.rwidth, i, A.rcol, A.rval, w);
+      C.b[i] = bc;
+      C.r[i] = bc;
+    }
+    for (int i = threadIdx.x; i < n; i += kFuseThreads) uA[i] = x[i];
+    return;
+  }
+  int nch = (C.n + kCoarseChunk - 1) / kCoarseChunk;
+  for (int i = threadIdx.x; i < C.n; i += kFuseThreads) w[i] = static_cast<T>(0) + coarse_sum(C.n, nch, i, part);
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += kFuseThreads)
+    x[j] = uA[j] + (A.pwidth == 2 ? dec_ell::row<2, T, T>(n, A.pwidth, j, A.pcol, A.pval, w)
+                                  : dec_ell::row<0, T, T>(n, A.pwidth, j, A.pcol, A.pval, w));
+  __syncthreads();
+  for (int it = 0; it < steps; it++) sweep_owned<W, T, R>(o, x, 0, n, A.ncolors, true);
+  for (int i = threadIdx.x; i < n; i += kFuseThreads) uA[i] = x[i];
+}
+
 // ---------------------------------------------------------------- storage
 
 template <class T>
@@ -518,6 +786,7 @@ struct LevelVals {
 struct Level {
   int n = 0, width = 0;
   DBuf<int32_t> col, rowid;  // ELL of A over internal numbers, rows sorted by (colour, vertex)
+  DBuf<int32_t> pos;         // ELL position of each internal vertex (inverse of rowid)
   std::vector<int> cstart;  // colour ranges of positions
   int ncolors = 0;
   // restriction to the next coarser level (rows: coarse internal numbers)
@@ -554,6 +823,8 @@ struct dec_mg_s {
   int fuse_from = -1;           // first level handled by the fused coarse kernel
   DBuf<LevelDesc<double>> desc64;
   DBuf<LevelDesc<float>> desc32;
+  std::vector<LevelDesc<double>> hdesc64;  // host copies (kernel arguments)
+  std::vector<LevelDesc<float>> hdesc32;
   bool use_fused = true;
   int fuse_rows = kFuseRowsDefault;
   // graph of one solve cycle (per variant) for fixed u, b
@@ -564,6 +835,7 @@ struct dec_mg_s {
   hipStream_t cap = nullptr;
   bool use_graph = true;
   // residual norms of a solve on their way to the host (pinned ring + events)
+  DBuf<double> rpart;  // per-workgroup partial sums of the finest residual's r²
   double *hnorm = nullptr;
   hipEvent_t nev[kNormSlots] = {};
   // learned smoother (float32 cycles, levels of learned_min_rows to
@@ -587,6 +859,13 @@ struct dec_mg_s {
 namespace {
 
 inline dim3 G(long long n) { return dim3(dec::nblocks(n, kBlock)); }
+
+template <class T>
+const std::vector<LevelDesc<T>> &host_desc(const dec_mg_s &mg);
+template <>
+const std::vector<LevelDesc<double>> &host_desc<double>(const dec_mg_s &mg) { return mg.hdesc64; }
+template <>
+const std::vector<LevelDesc<float>> &host_desc<float>(const dec_mg_s &mg) { return mg.hdesc32; }
 
 template <class T>
 LevelVals<T> &vals(Level &L);
@@ -761,6 +1040,11 @@ void build_level_matrix(Level &L, const dec::HostCsr &A, std::vector<int32_t> &c
   }
   L.col.upload(col);
   L.rowid.upload(order);
+  {
+    std::vector<int32_t> where(n);
+    for (int p = 0; p < n; p++) where[order[p]] = p;
+    L.pos.upload(where);
+  }
   L.d.val.upload(vd);
   L.f.val.upload(vf);
   L.A = A;
@@ -988,10 +1272,13 @@ void finish_hierarchy(dec_mg_s &mg, const std::vector<dec::HostCsr> &As, std::ve
     }
     mg.desc64.upload(d64);
     mg.desc32.upload(d32);
+    mg.hdesc64 = d64;
+    mg.hdesc32 = d32;
   }
   mg.part.alloc(kDotBlocks);
   mg.part2.alloc(kDotBlocks);
   mg.scal.alloc(8);
+  mg.rpart.alloc(static_cast<size_t>(dec::nblocks(mg.lv[0]->n, kBlock)));
   if (hipHostMalloc(reinterpret_cast<void **>(&mg.hnorm), sizeof(double) * kNormSlots, hipHostMallocDefault) !=
       hipSuccess)
     fail(DEC_ERR_OOM, "multigrid: pinned host buffer");
@@ -1171,8 +1458,55 @@ inline bool learned_rows(const dec_mg_s &mg, int n) {
 }
 
 // u_zero: u is known to be zero on entry (the residual is then b itself).
+// The prefetching sweep serves levels up to this many rows, where a sweep is
+// bound by memory latency; on larger levels its register footprint lowers
+// occupancy and the streaming sweep is faster (measured at 655,362 rows:
+// 25.6 against 18.2 µs per sweep).
+constexpr int kPrefetchMaxRows = 200000;
+// Residual and restriction run as one launch on levels up to this many rows
+// (measured: 6.8 against 9.0 µs at 40,962 rows, 18.5 against 10.6 µs at
+// 163,842, where evaluating shared fine rows twice costs more than a launch).
+constexpr int kFusedRestrictMaxRows = 50000;
+
+// Rows per thread of the prefetching sweep for this level's blocks (1 or 2
+// with up to 256 threads), or 0 when it does not apply (wide rows, or a
+// single block larger than 512 rows).
+int prefetch_rows(const Level &L) {
+  if (L.width < 1 || L.width > 8) return 0;
+  if (L.bmax <= kBlock) return 1;
+  if (L.bmax <= 2 * kBlock) return 2;
+  return 0;
+}
+
 template <class T>
-void smooth(dec_mg_s &mg, int lvl, Level &L, const T *b, T *u, bool post, bool u_zero, hipStream_t s) {
+void launch_block_gs_pf(Level &L, LevelVals<T> &V, int R, int nthr, size_t lds, const T *b, const T *src, T *dst,
+                        bool post, const T *zc, hipStream_t s) {
+  const T *pv = zc ? V.pval.p : nullptr;
+  const int *pc = zc ? L.pcol.p : nullptr;
+#define DEC_BGS_PF(WW, RR)                                                                                            \
+  hipLaunchKernelGGL((k_block_gs_pf<WW, T, RR>), dim3(L.nblk), dim3(nthr), lds, s, L.n, L.ncolors, L.nblk, L.col.p, \
+                     V.val.p, L.rowid.p, L.bstart.p, L.seg.p, b, src, dst, post ? 1 : 0, pc, pv, zc)
+#define DEC_BGS_PF_R(WW) \
+  if (R == 1) DEC_BGS_PF(WW, 1); \
+  else DEC_BGS_PF(WW, 2);
+  switch (L.width) {
+    case 1: DEC_BGS_PF_R(1) break;
+    case 2: DEC_BGS_PF_R(2) break;
+    case 3: DEC_BGS_PF_R(3) break;
+    case 4: DEC_BGS_PF_R(4) break;
+    case 5: DEC_BGS_PF_R(5) break;
+    case 6: DEC_BGS_PF_R(6) break;
+    case 7: DEC_BGS_PF_R(7) break;
+    case 8: DEC_BGS_PF_R(8) break;
+    default: fail(DEC_ERR_ARG, "multigrid: prefetching sweep width");
+  }
+#undef DEC_BGS_PF_R
+#undef DEC_BGS_PF
+}
+
+template <class T>
+void smooth(dec_mg_s &mg, int lvl, Level &L, const T *b, T *u, bool post, bool u_zero, hipStream_t s,
+            const T *zc = nullptr) {
   int steps = mg.steps;
   if (steps == 0) return;
   LevelVals<T> &V = vals<T>(L);
@@ -1196,16 +1530,33 @@ void smooth(dec_mg_s &mg, int lvl, Level &L, const T *b, T *u, bool post, bool u
   }
   if (mg.alg == DEC_SMOOTH_BLOCK_GS) {
     // one launch per sweep, ping-ponging between u and the level's t
-    // (an odd step count ends with a copy back into u)
+    // (an odd step count ends with a copy back into u); zc (post-smoothing
+    // only) folds the prolongation u += P zc into the first sweep
     T *bufs[2] = {u, V.t.p};
-    int nthr = L.seg_threads;
     size_t lds = static_cast<size_t>(L.bmax) * sizeof(T);
+    int R = prefetch_rows(L);
     for (int it = 0; it < steps; it++) {
       const T *src = bufs[it & 1];
       T *dst = bufs[(it + 1) & 1];
-      DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_block_gs<W, T>), dim3(L.nblk), dim3(nthr), lds, s, L.n,
-                                                       L.width, L.ncolors, L.nblk, L.col.p, V.val.p, L.rowid.p,
-                                                       L.bstart.p, L.seg.p, b, src, dst, post ? 1 : 0));
+      const T *z = it == 0 ? zc : nullptr;
+      if (R > 0 && L.n <= kPrefetchMaxRows) {
+        int nthr = R == 1 ? (L.bmax + 63) / 64 * 64 : kBlock;
+        launch_block_gs_pf<T>(L, V, R, nthr, lds, b, src, dst, post, z, s);
+      } else {
+        const T *pv = z ? V.pval.p : nullptr;
+        const int *pc = z ? L.pcol.p : nullptr;
+        if (z) {
+          DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_block_gs<W, T, true>), dim3(L.nblk),
+                                                           dim3(L.seg_threads), lds, s, L.n, L.width, L.ncolors,
+                                                           L.nblk, L.col.p, V.val.p, L.rowid.p, L.bstart.p, L.seg.p,
+                                                           b, src, dst, post ? 1 : 0, pc, pv, z));
+        } else {
+          DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_block_gs<W, T, false>), dim3(L.nblk),
+                                                           dim3(L.seg_threads), lds, s, L.n, L.width, L.ncolors,
+                                                           L.nblk, L.col.p, V.val.p, L.rowid.p, L.bstart.p, L.seg.p,
+                                                           b, src, dst, post ? 1 : 0, pc, pv, z));
+        }
+      }
     }
     if (steps & 1) hipLaunchKernelGGL(k_copy<T>, G(L.n), dim3(kBlock), 0, s, L.n, V.t.p, u);
     return;
@@ -1271,6 +1622,29 @@ void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, bool u_zero, hipS
       return;
     }
     Level &C = *mg.lv.back();
+    // one Gauss-Seidel level above the dense coarsest level: the register/LDS tail
+    if (lvl == nl - 2 && mg.alg != DEC_SMOOTH_JACOBI && L.n <= 3 * kFuseThreads && L.width >= 1 && L.width <= 8 &&
+        static_cast<size_t>(L.n + std::max(L.n, C.n)) * sizeof(T) <= 64 * 1024) {
+      size_t lds = static_cast<size_t>(L.n + std::max(L.n, C.n)) * sizeof(T);
+      const T *zp = vals<T>(C).zpart.p;
+      const std::vector<LevelDesc<T>> &HD = host_desc<T>(mg);
+      auto tail = [&](int phase) {
+        switch (L.width) {
+#define DEC_TAIL2(WW)                                                                                                 \
+  case WW:                                                                                                           \
+    hipLaunchKernelGGL((k_tail2<WW, T, 3>), dim3(1), dim3(kFuseThreads), lds, s, HD[lvl], HD[lvl + 1], u, b,       \
+                       mg.steps, phase, zp);                                                                          \
+    break;
+          DEC_TAIL2(1) DEC_TAIL2(2) DEC_TAIL2(3) DEC_TAIL2(4) DEC_TAIL2(5) DEC_TAIL2(6) DEC_TAIL2(7) DEC_TAIL2(8)
+#undef DEC_TAIL2
+          default: break;
+        }
+      };
+      tail(1);
+      coarse_partials<T>(C, s);
+      tail(2);
+      return;
+    }
     hipLaunchKernelGGL(k_fused_vcycle<T>, dim3(1), dim3(kFuseThreads), 0, s, D, lvl, nl, u, b, mg.steps, mg.alg, om,
                        1, static_cast<const T *>(nullptr));
     coarse_partials<T>(C, s);
@@ -1287,14 +1661,27 @@ void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, bool u_zero, hipS
   Level &C = *mg.lv[lvl + 1];
   LevelVals<T> &V = vals<T>(L);
   LevelVals<T> &CV = vals<T>(C);
-  residual<T>(L, b, u, V.r.p, s);
-  DEC_ELL_WIDTH_SWITCH(L.rwidth, hipLaunchKernelGGL((k_restrict<W, T>), G(L.nc), dim3(kBlock), 0, s, L.nc, L.rwidth,
-                                                    L.rcol.p, V.rval.p, V.r.p, CV.b.p, CV.u.p));
+  if (L.n <= kFusedRestrictMaxRows && L.rwidth <= 8) {
+    DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_residual_restrict<W, T>), G(8LL * L.nc), dim3(kBlock), 0, s,
+                                                     L.nc, L.rwidth, L.rcol.p, V.rval.p, L.n, L.width, L.col.p,
+                                                     V.val.p, L.pos.p, b, u, CV.b.p, CV.u.p));
+  } else {
+    residual<T>(L, b, u, V.r.p, s);
+    DEC_ELL_WIDTH_SWITCH(L.rwidth, hipLaunchKernelGGL((k_restrict<W, T>), G(L.nc), dim3(kBlock), 0, s, L.nc,
+                                                      L.rwidth, L.rcol.p, V.rval.p, V.r.p, CV.b.p, CV.u.p));
+  }
   mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu, true, s);
   if (mu > 1) mu_cycle<T>(mg, lvl + 1, CV.u.p, CV.b.p, mu - 1, false, s);
-  DEC_ELL_WIDTH_SWITCH(L.pwidth, hipLaunchKernelGGL((k_prolong_add<W, T>), G(L.n), dim3(kBlock), 0, s, L.n, L.pwidth,
-                                                    L.pcol.p, V.pval.p, CV.u.p, u));
-  smooth<T>(mg, lvl, L, b, u, true, false, s);
+  // the hybrid smoother's first post-sweep applies the prolongation itself
+  // (on the prefetching sweep only: there the extra loads join the load
+  // phase; on the streaming sweep of the largest level they cost more than
+  // the separate launch, measured 37 against 18 + 6.5 µs at 655,362 rows)
+  bool fold = mg.alg == DEC_SMOOTH_BLOCK_GS && mg.steps > 0 && L.pwidth == 2 && prefetch_rows(L) > 0 &&
+              L.n <= kPrefetchMaxRows && !(sizeof(T) == 4 && learned_on(mg) && learned_rows(mg, L.n));
+  if (!fold)
+    DEC_ELL_WIDTH_SWITCH(L.pwidth, hipLaunchKernelGGL((k_prolong_add<W, T>), G(L.n), dim3(kBlock), 0, s, L.n,
+                                                      L.pwidth, L.pcol.p, V.pval.p, CV.u.p, u));
+  smooth<T>(mg, lvl, L, b, u, true, false, s, fold ? static_cast<const T *>(CV.u.p) : nullptr);
 }
 
 // _full_multigrid (Multigrid.jl:839-855).
@@ -1428,9 +1815,12 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_call
   // (no V-cycle cuts the residual 1000x), so the cycle after it is queued at
   // once and the GPU does not wait for the host between cycles.
   auto enqueue_norm = [&](int k) {
-    residual<double>(F, b, u, F.d.r.p, s);
     double *slot = mg.scal.p + (k % kNormSlots);
-    dot(mg, n, F.d.r.p, F.d.r.p, slot, nullptr, s);
+    int nb = dec::nblocks(n, kBlock);
+    if (mg.rpart.n < static_cast<size_t>(nb)) fail(DEC_ERR_ARG, "multigrid: residual partials");
+    DEC_ELL_WIDTH_SWITCH(F.width, hipLaunchKernelGGL((k_residual_norm<W>), dim3(nb), dim3(kBlock), 0, s, n, F.width,
+                                                     F.col.p, F.d.val.p, F.rowid.p, b, u, F.d.r.p, mg.rpart.p));
+    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(kBlock), 0, s, nb, mg.rpart.p, slot);
     gcheck(hipMemcpyAsync(mg.hnorm + (k % kNormSlots), slot, sizeof(double), hipMemcpyDeviceToHost, s),
            "hipMemcpyAsync");
     gcheck(hipEventRecord(mg.nev[k % kNormSlots], s), "hipEventRecord");
