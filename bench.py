@@ -475,8 +475,24 @@ def learned(torch, cs, md, fd, npairs=10000, train_levels=3, steps=2000, batch=1
     out["dense_layer_TFLOPs_fine"] = 2 * 2 * 64 * 32 * n0 / (app_ms * 1e-3) / 1e12
     # levels that use the network: all (one or two applications); the two
     # finest; the three below the finest (the finest keeps Gauss-Seidel)
+    # one application per level at each level size (internal numbering, as in the cycles)
+    out["inference_ms_per_application"] = {}
+    for lv, n in enumerate(md.sizes[:-1]):
+        rl = torch.randn(n, dtype=torch.float32, device="cuda")
+        ul = torch.zeros(n, dtype=torch.float32, device="cuda")
+        for _ in range(3):
+            nn.smooth(lv, rl, ul)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(reps):
+            nn.smooth(lv, rl, ul)
+        b.record(stream)
+        torch.cuda.synchronize()
+        out["inference_ms_per_application"][str(n)] = a.elapsed_time(b) / reps
     for tag, lo, hi, apps in (("all_levels", 0, 0, 1), ("all_levels_2_applications", 0, 0, 2),
-                              ("levels_ge_40963", 40963, 0, 1), ("levels_10242_to_163842", 10242, 163842, 1)):
+                              ("levels_ge_40963", 40963, 0, 1), ("levels_10242_to_163842", 10242, 163842, 1),
+                              ("levels_10242_to_40962", 10242, 40962, 1), ("level_10242", 10242, 10242, 1),
+                              ("level_40962", 40962, 40962, 1)):
         nn.attach(True, min_rows=lo, max_rows=hi, applications=apps)
         cs.solve(md, fd, tol=1e-8, variant="mixed")
         torch.cuda.synchronize()

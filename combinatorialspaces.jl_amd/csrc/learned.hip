@@ -324,11 +324,24 @@ __device__ __forceinline__ uint4 pack8(const float *f) {
 // sums, which run in slot order.
 template <int W>
 __global__ void k_inf_input(int n, int pw, const int *__restrict__ pcol, const float *__restrict__ pval,
-                            const float *__restrict__ r, const float *__restrict__ s, float2 *__restrict__ xp,
-                            const float *__restrict__ th, unsigned short *__restrict__ h1) {
+                            const float *__restrict__ r, const double *__restrict__ part, float *__restrict__ s,
+                            float2 *__restrict__ xp, const float *__restrict__ th, unsigned short *__restrict__ h1) {
+  // s = sqrt(mean r²) from k_scale_partial's sums, added in k_scale_final's
+  // order by one thread of every workgroup; workgroup 0 stores it for the
+  // layer kernels
+  __shared__ float s_sh;
+  if (threadIdx.x == 0) {
+    double acc = 0.0;
+    for (int j = 0; j < kScaleBlocks; j++) acc += part[j];
+    double m = acc / n;
+    float sv = m > 0 ? static_cast<float>(sqrt(m)) : 1.0f;
+    s_sh = sv;
+    if (blockIdx.x == 0) s[0] = sv;
+  }
+  __syncthreads();
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  float inv = 1.0f / s[0];
+  float inv = 1.0f / s_sh;
   float xv = r[v] * inv;
   float acc = 0.0f;
   if constexpr (W > 0) {
@@ -1487,15 +1500,15 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s, b
   } L{internal ? NL.pi : NL.p, internal ? NL.Di : NL.D};
   int n = NL.n;
   reserve_inference(nn, n);
-  // the scale in two launches (one with a last-workgroup ticket needs a
-  // device-scope fence per workgroup, which writes back the XCD's L2)
+  // the scale's partial sums; the input kernel adds them (a last-workgroup
+  // ticket would need a device-scope fence per workgroup, which writes back
+  // the XCD's L2)
   hipLaunchKernelGGL(k_scale_partial, dim3(kScaleBlocks, 1), dim3(TB), 0, s, n, r, nn.spart.p);
-  hipLaunchKernelGGL(k_scale_final, dim3(1), dim3(64), 0, s, n, 1, nn.spart.p, nn.s.p);
   bool big = n > kInfSmallRows;
   unsigned short *h1 = big ? nn.ih1.p : nullptr;
   DEC_ELL_WIDTH_SWITCH(L.p.width, hipLaunchKernelGGL((k_inf_input<W>), dim3(dec::nblocks(n, TB)), dim3(TB), 0, s, n,
-                                                      L.p.width, L.p.col.p, L.p.val.p, r, nn.s.p, nn.ix.p, nn.th.p,
-                                                      h1));
+                                                      L.p.width, L.p.col.p, L.p.val.p, r, nn.spart.p, nn.s.p,
+                                                      nn.ix.p, nn.th.p, h1));
   if (!big) {
     int tiles = (n + 15) / 16;
     int blocks = std::max(1, (tiles + kInfWaves - 1) / kInfWaves);
