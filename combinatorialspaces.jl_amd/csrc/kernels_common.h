@@ -48,7 +48,7 @@ template <class V>
 void upload_ell(const HostCsr &A, Ell<V> &E);
 // y = A x over a slot-major ELL, rows in storage order, sequential sums.
 template <class T>
-void ell_spmv(const Ell<T> &E, const T *x, T *y, hipStream_t s);
+void ell_spmv(const Ell<T> &E, const T *x, T *y, hipStream_t s, const int *sched = nullptr, int nsched = 0);
 
 }  // namespace dec
 

@@ -78,6 +78,9 @@ struct dec_op_s {
   int max_iter = 0;
   double rtol = 0.0;
   bool symmetric = true;  // CG when ⋆1 is symmetric, BiCGSTAB otherwise
+  // spatial tile schedule of the gather kernels (8 x tiles-per-XCD entries)
+  DBuf<int32_t> sched;
+  int nsched = 0;
   // composite operators: sub-operators and work vectors (one apply at a time)
   int comp = 0;
   std::vector<std::unique_ptr<dec_op_s>> sub;
@@ -85,6 +88,15 @@ struct dec_op_s {
 };
 
 namespace {
+
+// The tile (workgroup-sized row range) a workgroup processes: its entry in
+// the operator's spatial schedule when it has one, else the default order.
+// A schedule deals tiles to the XCDs (workgroup i runs on XCD i % 8) by
+// position on the mesh, so the neighbours one XCD gathers stay in its L2;
+// -1 pads the XCDs with fewer tiles.
+__device__ __forceinline__ int tile_index(const int *__restrict__ sched, int dflt) {
+  return sched ? sched[blockIdx.x] : dflt;
+}
 
 // Apply kernels: each thread handles the pair of simplices (2i, 2i+1), loading
 // index and coefficient tables as int2 / T2 vectors; a trailing odd element is
@@ -143,8 +155,10 @@ __device__ __forceinline__ void b_wedge01(int blk, int ne, int vec, const int *_
 }
 template <class T>
 __global__ void k_wedge01(int ne, int vec, const int *__restrict__ v0, const int *__restrict__ v1,
-                          const T *__restrict__ f, const T *__restrict__ a, T *__restrict__ res) {
-  b_wedge01<T>(dec_ell::xcd_block(), ne, vec, v0, v1, f, a, res);
+                          const T *__restrict__ f, const T *__restrict__ a, T *__restrict__ res,
+                          const int *__restrict__ sched) {
+  int blk = tile_index(sched, dec_ell::xcd_block());
+  if (blk >= 0) b_wedge01<T>(blk, ne, vec, v0, v1, f, a, res);
 }
 
 // wedge_kernel_02! (FastDEC.jl:165-170) with the six coefficients summed
@@ -176,8 +190,9 @@ template <class T>
 __global__ void k_wedge02(int nt, int vec, const int *__restrict__ i0, const int *__restrict__ i1,
                           const int *__restrict__ i2, const T *__restrict__ c0, const T *__restrict__ c1,
                           const T *__restrict__ c2, const T *__restrict__ f, const T *__restrict__ a,
-                          T *__restrict__ res) {
-  b_wedge02<T>(dec_ell::xcd_block(), nt, vec, i0, i1, i2, c0, c1, c2, f, a, res);
+                          T *__restrict__ res, const int *__restrict__ sched) {
+  int blk = tile_index(sched, dec_ell::xcd_block());
+  if (blk >= 0) b_wedge02<T>(blk, nt, vec, i0, i1, i2, c0, c1, c2, f, a, res);
 }
 
 // wedge_kernel_11! (FastDEC.jl:184-191)
@@ -211,8 +226,9 @@ template <class T>
 __global__ void k_wedge11(int nt, int vec, const int *__restrict__ i0, const int *__restrict__ i1,
                           const int *__restrict__ i2, const T *__restrict__ c0, const T *__restrict__ c1,
                           const T *__restrict__ c2, const T *__restrict__ a, const T *__restrict__ b,
-                          T *__restrict__ res) {
-  b_wedge11<T>(static_cast<int>(blockIdx.x), nt, vec, i0, i1, i2, c0, c1, c2, a, b, res);
+                          T *__restrict__ res, const int *__restrict__ sched) {
+  int blk = tile_index(sched, static_cast<int>(blockIdx.x));
+  if (blk >= 0) b_wedge11<T>(blk, nt, vec, i0, i1, i2, c0, c1, c2, a, b, res);
 }
 
 // (m * f)[e] .* g[e], m = wedge_dd_01_mat (FastDEC.jl:293-305); the row's
@@ -269,8 +285,10 @@ __device__ __forceinline__ void b_d0(int blk, int ne, int vec, const int *__rest
 }
 template <class T>
 __global__ void k_d0(int ne, int vec, const int *__restrict__ v0, const int *__restrict__ v1,
-                     const int8_t *__restrict__ sgn, const T *__restrict__ x, T *__restrict__ res) {
-  b_d0<T>(dec_ell::xcd_block(), ne, vec, v0, v1, sgn, x, res);
+                     const int8_t *__restrict__ sgn, const T *__restrict__ x, T *__restrict__ res,
+                     const int *__restrict__ sched) {
+  int blk = tile_index(sched, dec_ell::xcd_block());
+  if (blk >= 0) b_d0<T>(blk, ne, vec, v0, v1, sgn, x, res);
 }
 
 // row t of d1 (FastDEC.jl:496-524), its three edges in ascending order with
@@ -299,8 +317,10 @@ __device__ __forceinline__ void b_d1(int blk, int nt, int vec, const int *__rest
 }
 template <class T>
 __global__ void k_d1(int nt, int vec, const int *__restrict__ ia, const int *__restrict__ ib, const int *__restrict__ ic,
-                     const int8_t *__restrict__ sg, const T *__restrict__ x, T *__restrict__ res) {
-  b_d1<T>(dec_ell::xcd_block(), nt, vec, ia, ib, ic, sg, x, res);
+                     const int8_t *__restrict__ sg, const T *__restrict__ x, T *__restrict__ res,
+                     const int *__restrict__ sched) {
+  int blk = tile_index(sched, dec_ell::xcd_block());
+  if (blk >= 0) b_d1<T>(blk, nt, vec, ia, ib, ic, sg, x, res);
 }
 
 // wedge_kernel_03! (FastDEC.jl:173-183): res = Σ_j (α c_j) f[p_j] over the 24
@@ -398,8 +418,9 @@ __device__ __forceinline__ void b_ell(int blk, int n, int width, const int *__re
 }
 template <int W, class T, class V>
 __global__ void k_ell(int n, int width, const int *__restrict__ col, const V *__restrict__ val,
-                      const T *__restrict__ x, T *__restrict__ y) {
-  b_ell<W, T, V>(static_cast<int>(blockIdx.x), n, width, col, val, x, y);
+                      const T *__restrict__ x, T *__restrict__ y, const int *__restrict__ sched) {
+  int blk = tile_index(sched, static_cast<int>(blockIdx.x));
+  if (blk >= 0) b_ell<W, T, V>(blk, n, width, col, val, x, y);
 }
 template <int WMAX, class T, class V>
 __global__ void k_ell_upto(int n, int width, const int *__restrict__ col, const V *__restrict__ val,
@@ -986,7 +1007,7 @@ template void upload_ell<double>(const HostCsr &, Ell<double> &);
 template void upload_ell<float>(const HostCsr &, Ell<float> &);
 
 template <class T>
-void ell_spmv(const Ell<T> &E, const T *x, T *y, hipStream_t s) {
+void ell_spmv(const Ell<T> &E, const T *x, T *y, hipStream_t s, const int *sched, int nsched) {
   if (E.nrows == 0) return;
   // rows of 9..16 slots (tetrahedral ∇²₀, the dual Laplacians' products) issue
   // all their loads at once; the grouped launch keeps the generic loop so that
@@ -1001,11 +1022,11 @@ void ell_spmv(const Ell<T> &E, const T *x, T *y, hipStream_t s) {
                        E.val.p, x, y);
     return;
   }
-  DEC_ELL_WIDTH_SWITCH(E.width, hipLaunchKernelGGL((k_ell<W, T, T>), dim3(nblocks(E.nrows)), dim3(256), 0, s,
-                                                    E.nrows, E.width, E.col.p, E.val.p, x, y));
+  DEC_ELL_WIDTH_SWITCH(E.width, hipLaunchKernelGGL((k_ell<W, T, T>), dim3(sched ? nsched : nblocks(E.nrows)),
+                                                    dim3(256), 0, s, E.nrows, E.width, E.col.p, E.val.p, x, y, sched));
 }
-template void ell_spmv<double>(const Ell<double> &, const double *, double *, hipStream_t);
-template void ell_spmv<float>(const Ell<float> &, const float *, float *, hipStream_t);
+template void ell_spmv<double>(const Ell<double> &, const double *, double *, hipStream_t, const int *, int);
+template void ell_spmv<float>(const Ell<float> &, const float *, float *, hipStream_t, const int *, int);
 
 // Host CSR of the geometric ⋆1 from the per-triangle entries: rows ascending
 // columns, duplicates (diagonals shared by two triangles) summed in the
@@ -1269,17 +1290,19 @@ void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream
   int vec = (reinterpret_cast<uintptr_t>(out) % A == 0) && (!y || reinterpret_cast<uintptr_t>(y) % A == 0) &&
             (!x || reinterpret_cast<uintptr_t>(x) % A == 0);
   auto C = [](DBuf<unsigned char> &c) { return reinterpret_cast<const T *>(c.p); };
+  // grid of a gather kernel: one workgroup per schedule entry when it has one
+  auto gs = [&](dim3 dflt) { return op.sched.p ? dim3(op.nsched) : dflt; };
   switch (op.impl) {
     case I_W01:
-      hipLaunchKernelGGL(k_wedge01<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, x, y, out);
+      hipLaunchKernelGGL(k_wedge01<T>, gs(g2), b, 0, s, rows, vec, op.i0.p, op.i1.p, x, y, out, op.sched.p);
       break;
     case I_W02:
-      hipLaunchKernelGGL(k_wedge02<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, op.i2.p, C(op.c0), C(op.c1), C(op.c2), x,
-                         y, out);
+      hipLaunchKernelGGL(k_wedge02<T>, gs(g2), b, 0, s, rows, vec, op.i0.p, op.i1.p, op.i2.p, C(op.c0), C(op.c1),
+                         C(op.c2), x, y, out, op.sched.p);
       break;
     case I_W11:
-      hipLaunchKernelGGL(k_wedge11<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, op.i2.p, C(op.c0), C(op.c1), C(op.c2), x,
-                         y, out);
+      hipLaunchKernelGGL(k_wedge11<T>, gs(g2), b, 0, s, rows, vec, op.i0.p, op.i1.p, op.i2.p, C(op.c0), C(op.c1),
+                         C(op.c2), x, y, out, op.sched.p);
       break;
     case I_WDD:
       if (op.m == 0)  // (m*f) .* g : f = x (dual 0-form), g = y
@@ -1288,21 +1311,23 @@ void apply_t(dec_op_s &op, const void *xv, const void *yv, void *outv, hipStream
         hipLaunchKernelGGL(k_wedge_dd<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, C(op.c0), C(op.c1), y, x, out);
       break;
     case I_D0:
-      hipLaunchKernelGGL(k_d0<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, op.s8.p, x, out);
+      hipLaunchKernelGGL(k_d0<T>, gs(g2), b, 0, s, rows, vec, op.i0.p, op.i1.p, op.s8.p, x, out, op.sched.p);
       break;
     case I_D1:
-      hipLaunchKernelGGL(k_d1<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, op.i2.p, op.s8.p, x, out);
+      hipLaunchKernelGGL(k_d1<T>, gs(g2), b, 0, s, rows, vec, op.i0.p, op.i1.p, op.i2.p, op.s8.p, x, out,
+                         op.sched.p);
       break;
     case I_SELL:
       DEC_ELL_WIDTH_SWITCH(op.sell.width, hipLaunchKernelGGL((k_ell<W, T, int8_t>), g, b, 0, s, rows, op.sell.width,
-                                                             op.sell.col.p, op.sell.val.p, x, out));
+                                                             op.sell.col.p, op.sell.val.p, x, out,
+                                                             static_cast<const int *>(nullptr)));
       break;
     case I_DIAG:
       hipLaunchKernelGGL(k_diag<T>, g2, b, 0, s, rows, vec, C(op.c0), x, out);
       break;
     case I_ELL:
-      if constexpr (sizeof(T) == 8) dec::ell_spmv(op.ell64, x, out, s);
-      else dec::ell_spmv(op.ell32, x, out, s);
+      if constexpr (sizeof(T) == 8) dec::ell_spmv(op.ell64, x, out, s, op.sched.p, op.nsched);
+      else dec::ell_spmv(op.ell32, x, out, s, op.sched.p, op.nsched);
       break;
     case I_AVG:
       hipLaunchKernelGGL(k_avg01<T>, g2, b, 0, s, rows, vec, op.i0.p, op.i1.p, x, out);
@@ -1404,6 +1429,7 @@ struct GroupOp {
   const int *ix;
   const void *cx;
   const uint16_t *m16;
+  const int *sched;  // spatial tile schedule (null: tiles in order)
   const void *x, *y;
   void *out;
 };
@@ -1420,6 +1446,12 @@ __global__ void __launch_bounds__(256) k_apply_group(GroupArgs g) {
   while (i + 1 < g.count && b >= g.start[i + 1]) i++;
   const GroupOp &o = g.op[i];
   int blk = b - g.start[i];
+  // every operator's range starts at a multiple of 8, so b and its local
+  // index run on the same XCD (b % 8), as the schedule assumes
+  if (o.sched) {
+    blk = o.sched[blk];
+    if (blk < 0) return;
+  }
   const T *x = static_cast<const T *>(o.x), *y = static_cast<const T *>(o.y);
   const T *c0 = static_cast<const T *>(o.c0), *c1 = static_cast<const T *>(o.c1), *c2 = static_cast<const T *>(o.c2);
   T *out = static_cast<T *>(o.out);
@@ -1455,16 +1487,24 @@ __global__ void __launch_bounds__(256) k_apply_group(GroupArgs g) {
 
 // Workgroups of one operator in the grouped launch (its own grid, without the
 // XCD rounding, and without the grid-stride cap of the diagonal kernel).
+// Rounded up to a multiple of 8 (the XCD count), so that every operator's
+// range of workgroups starts on XCD 0; surplus workgroups find no rows.
 int group_blocks(const dec_op_s &op) {
   long long rows = op.rows;
-  switch (op.impl) {
-    case I_SELL:
-    case I_ELL:
-    case I_WPD:
-    case I_W03:
-    case I_W21: return dec::nblocks(rows, 256);
-    default: return dec::nblocks((rows + 1) / 2, 256);
+  int nb;
+  if (op.sched.p) {
+    nb = op.nsched;
+  } else {
+    switch (op.impl) {
+      case I_SELL:
+      case I_ELL:
+      case I_WPD:
+      case I_W03:
+      case I_W21: nb = dec::nblocks(rows, 256); break;
+      default: nb = dec::nblocks((rows + 1) / 2, 256); break;
+    }
   }
+  return (nb + 7) / 8 * 8;
 }
 
 template <class T>
@@ -1505,6 +1545,7 @@ void apply_group_t(int count, dec_op_s *const *ops, const void *const *xs, const
       o.ix = op.ix.p;
       o.cx = op.cx.p;
       o.m16 = op.m16.p;
+      o.sched = op.sched.p;
       o.x = x;
       o.y = y;
       o.out = outs[q];
@@ -2070,6 +2111,100 @@ std::unique_ptr<dec_op_s> create3(dec_mesh_s &m, int kind, int dm, int dn, int h
   return op;
 }
 
+// Spatial tile schedule: the rows are cut into tiles of `per` rows (one
+// workgroup each), the tiles are sorted by the Morton code of their mean row
+// position in the mesh's bounding box, and the sorted list is cut into 8
+// contiguous groups, group x going to XCD x: sched[8 j + x] is the j-th tile
+// of group x (-1 past its end).  Neighbouring rows, whose gathers overlap,
+// then run on the same XCD and hit its L2.
+void make_schedule(dec_op_s &op, const std::vector<double> &pos, int per) {
+  long long rows = op.rows;
+  int ntile = static_cast<int>((rows + per - 1) / per);
+  if (ntile < 16) return;
+  std::vector<double> c(3 * static_cast<size_t>(ntile), 0.0);
+  for (long long r = 0; r < rows; r++)
+    for (int k = 0; k < 3; k++) c[3 * (r / per) + k] += pos[3 * r + k];
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int t = 0; t < ntile; t++) {
+    long long cnt = std::min<long long>(per, rows - static_cast<long long>(t) * per);
+    for (int k = 0; k < 3; k++) {
+      c[3 * t + k] /= static_cast<double>(cnt);
+      lo[k] = std::min(lo[k], c[3 * t + k]);
+      hi[k] = std::max(hi[k], c[3 * t + k]);
+    }
+  }
+  auto spread = [](uint64_t v) {
+    v &= 0x3ff;
+    v = (v | (v << 16)) & 0x30000ff;
+    v = (v | (v << 8)) & 0x300f00f;
+    v = (v | (v << 4)) & 0x30c30c3;
+    v = (v | (v << 2)) & 0x9249249;
+    return v;
+  };
+  std::vector<std::pair<uint64_t, int>> key(ntile);
+  for (int t = 0; t < ntile; t++) {
+    uint64_t code = 0;
+    for (int k = 0; k < 3; k++) {
+      double span = hi[k] - lo[k];
+      uint64_t q = span > 0 ? static_cast<uint64_t>((c[3 * t + k] - lo[k]) / span * 1023.0) : 0;
+      code |= spread(q) << k;
+    }
+    key[t] = {code, t};
+  }
+  std::stable_sort(key.begin(), key.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+  int per_xcd = (ntile + 7) / 8;
+  std::vector<int32_t> sched(8 * static_cast<size_t>(per_xcd), -1);
+  for (int x = 0; x < 8; x++) {
+    // each XCD walks its tiles in row order (tables stream in address order)
+    int i0 = std::min(ntile, x * per_xcd), i1 = std::min(ntile, (x + 1) * per_xcd);
+    std::vector<int> mine;
+    for (int i = i0; i < i1; i++) mine.push_back(key[i].second);
+    std::sort(mine.begin(), mine.end());
+    for (size_t j = 0; j < mine.size(); j++) sched[8 * j + x] = mine[j];
+  }
+  op.sched.upload(sched);
+  op.nsched = static_cast<int>(sched.size());
+}
+
+// Schedules for the gather operators of a complex: rows at edge midpoints
+// (wedge 0-1, d0), triangle centroids (wedges 0-2, 1-1, d1) or vertices
+// (the 0-form Laplacian, rows of at most 8 slots).
+void schedule_gathers(dec_op_s &op, const dec_mesh_s &m) {
+  const dec::Topo &t = m.h;
+  const std::vector<double> &p = m.hp;
+  if (p.size() < 3 * static_cast<size_t>(t.nv)) return;
+  std::vector<double> pos;
+  int per = 512;
+  switch (op.impl) {
+    case I_W01:
+    case I_D0:
+      if (op.rows != t.ne) return;
+      pos.resize(3 * static_cast<size_t>(t.ne));
+      for (int e = 0; e < t.ne; e++)
+        for (int k = 0; k < 3; k++) pos[3 * e + k] = 0.5 * p[3 * t.v0[e] + k] + 0.5 * p[3 * t.v1[e] + k];
+      break;
+    case I_W02:
+    case I_W11:
+    case I_D1:
+      if (op.rows != t.nt) return;
+      pos.resize(3 * static_cast<size_t>(t.nt));
+      for (int i = 0; i < t.nt; i++) {
+        int a = t.v1[t.e1[i]], b = t.v0[t.e2[i]], c = t.v0[t.e1[i]];
+        for (int k = 0; k < 3; k++) pos[3 * i + k] = (p[3 * a + k] + p[3 * b + k] + p[3 * c + k]) / 3.0;
+      }
+      break;
+    case I_ELL: {
+      int width = op.dtype == DEC_F64 ? op.ell64.width : op.ell32.width;
+      if (op.kind != DEC_OP_LAPLACIAN0 || op.rows != t.nv || width > 8) return;
+      pos.assign(p.begin(), p.begin() + 3 * static_cast<size_t>(t.nv));
+      per = 256;
+      break;
+    }
+    default: return;
+  }
+  make_schedule(op, pos, per);
+}
+
 dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) {
   if (dtype != DEC_F64 && dtype != DEC_F32) fail(DEC_ERR_ARG, "dec_op_create: dtype must be DEC_F64 or DEC_F32");
   if (hodge != DEC_GEOMETRIC_HODGE && hodge != DEC_DIAGONAL_HODGE) fail(DEC_ERR_ARG, "dec_op_create: unknown hodge");
@@ -2523,6 +2658,7 @@ dec_op_s *create(dec_mesh_s &m, int kind, int dm, int dn, int hodge, int dtype) 
     default:
       fail(DEC_ERR_ARG, "dec_op_create: unknown operator kind");
   }
+  schedule_gathers(*op, m);
   dec::check_launch("dec_op_create");
   dec::dsync();
   return op.release();
