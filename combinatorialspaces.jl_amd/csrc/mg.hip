@@ -291,29 +291,36 @@ struct OwnedRows {
   T val[R][W], halo[R][W], b[R];
 };
 
+constexpr int kPfColors = 8;  // colours the prefetching sweep supports
+
 template <int W, class T, int R>
-__device__ __forceinline__ void load_owned(OwnedRows<W, T, R> &o, int k, int r0, int nr, int ncolors, int nb, int n,
+__device__ __forceinline__ void load_owned(OwnedRows<W, T, R> &o, int k, int r0, int nr, int ncolors, int n,
                                            const int *__restrict__ col, const T *__restrict__ val,
-                                           const int *__restrict__ rowid, const int *__restrict__ seg,
+                                           const int *__restrict__ rowid, const int *__restrict__ segb,
                                            const T *__restrict__ b, const T *__restrict__ uin,
                                            const int *__restrict__ pcol, const T *__restrict__ pval,
                                            const T *__restrict__ zc) {
+  // this block's (colour, block) segments, all loaded at once: segb holds
+  // for block k the first position of each colour and, after them, the
+  // running row count before each colour (ncolors <= kPfColors)
+  int sp[kPfColors], sc[kPfColors + 1];
+#pragma unroll
+  for (int c = 0; c < kPfColors; c++) sp[c] = c < ncolors ? segb[2 * k * (kPfColors + 1) + c] : 0;
+#pragma unroll
+  for (int c = 0; c <= kPfColors; c++) sc[c] = c <= ncolors ? segb[(2 * k + 1) * (kPfColors + 1) + c] : nr;
 #pragma unroll
   for (int i = 0; i < R; i++) {
     int q = static_cast<int>(threadIdx.x) + i * static_cast<int>(blockDim.x);
     o.v[i] = -1;
     o.c[i] = -1;
     if (q >= nr) continue;
-    int acc = 0, p = -1;
-    for (int c = 0; c < ncolors; c++) {
-      int s0 = seg[c * (nb + 1) + k], s1 = seg[c * (nb + 1) + k + 1];
-      if (q < acc + (s1 - s0)) {
-        p = s0 + (q - acc);
+    int p = -1;
+#pragma unroll
+    for (int c = 0; c < kPfColors; c++)
+      if (c < ncolors && q >= sc[c] && q < sc[c + 1]) {
+        p = sp[c] + (q - sc[c]);
         o.c[i] = c;
-        break;
       }
-      acc += s1 - s0;
-    }
     o.v[i] = rowid[p];
 #pragma unroll
     for (int s = 0; s < W; s++) {
@@ -363,8 +370,8 @@ __device__ __forceinline__ void sweep_owned(const OwnedRows<W, T, R> &o, T *x, i
 
 template <int W, class T, int R>
 __global__ void __launch_bounds__(256)
-    k_block_gs_pf(int n, int ncolors, int nb, const int *__restrict__ col, const T *__restrict__ val,
-                  const int *__restrict__ rowid, const int *__restrict__ bstart, const int *__restrict__ seg,
+    k_block_gs_pf(int n, int ncolors, const int *__restrict__ col, const T *__restrict__ val,
+                  const int *__restrict__ rowid, const int *__restrict__ bstart, const int *__restrict__ segb,
                   const T *__restrict__ b, const T *__restrict__ uin, T *__restrict__ uout, int reverse,
                   const int *__restrict__ pcol, const T *__restrict__ pval, const T *__restrict__ zc) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -372,7 +379,7 @@ __global__ void __launch_bounds__(256)
   int k = blockIdx.x;
   int r0 = bstart[k], nr = bstart[k + 1] - r0;
   OwnedRows<W, T, R> o;
-  load_owned<W, T, R>(o, k, r0, nr, ncolors, nb, n, col, val, rowid, seg, b, uin, pcol, pval, zc);
+  load_owned<W, T, R>(o, k, r0, nr, ncolors, n, col, val, rowid, segb, b, uin, pcol, pval, zc);
   for (int i = threadIdx.x; i < nr; i += blockDim.x) x[i] = prolonged(n, r0 + i, uin, pcol, pval, zc);
   __syncthreads();
   sweep_owned<W, T, R>(o, x, r0, nr, ncolors, reverse != 0);
@@ -731,6 +738,23 @@ __global__ void __launch_bounds__(kFuseThreads)
     o.b[i] = bA[o.v[i]];
   }
   if (phase == 1) {
+    // the restriction row this thread forms at the end, loaded up front
+    // (one coarse row per thread when R is at most 8 wide and C has at most
+    // 1024 rows; else the rows are read at the end)
+    bool rpre = A.rwidth <= 8 && A.nc <= kFuseThreads;
+    int rcl[8];
+    T rvl[8];
+    int ri = threadIdx.x;
+#pragma unroll
+    for (int sl = 0; sl < 8; sl++) {
+      rcl[sl] = -1;
+      rvl[sl] = static_cast<T>(0);
+      if (rpre && ri < A.nc && sl < A.rwidth) {
+        size_t off = static_cast<size_t>(sl) * A.nc + ri;
+        rcl[sl] = A.rcol[off];
+        rvl[sl] = A.rval[off];
+      }
+    }
     for (int i = threadIdx.x; i < n; i += kFuseThreads) x[i] = uA[i];
     __syncthreads();
     for (int it = 0; it < steps; it++) sweep_owned<W, T, R>(o, x, 0, n, A.ncolors, false);
@@ -744,11 +768,22 @@ __global__ void __launch_bounds__(kFuseThreads)
       w[o.v[i]] = o.b[i] - acc;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < A.nc; i += kFuseThreads) {
-      T bc = A.rwidth == 7 ? dec_ell::row<7, T, T>(A.nc, A.rwidth, i, A.rcol, A.rval, w)
-                           : dec_ell::row<0, T, T>(A.nc, A.rwidth, i, A.rcol, A.rval, w);
-      C.b[i] = bc;
-      C.r[i] = bc;
+    if (rpre) {
+      if (ri < A.nc) {
+        T bc = static_cast<T>(0);
+#pragma unroll
+        for (int sl = 0; sl < 8; sl++)
+          if (rcl[sl] >= 0) bc = bc + rvl[sl] * w[rcl[sl]];
+        C.b[ri] = bc;
+        C.r[ri] = bc;
+      }
+    } else {
+      for (int i = threadIdx.x; i < A.nc; i += kFuseThreads) {
+        T bc = A.rwidth == 7 ? dec_ell::row<7, T, T>(A.nc, A.rwidth, i, A.rcol, A.rval, w)
+                             : dec_ell::row<0, T, T>(A.nc, A.rwidth, i, A.rcol, A.rval, w);
+        C.b[i] = bc;
+        C.r[i] = bc;
+      }
     }
     for (int i = threadIdx.x; i < n; i += kFuseThreads) uA[i] = x[i];
     return;
@@ -807,6 +842,7 @@ struct Level {
   // hybrid block Gauss-Seidel: block ranges and (colour, block) ELL segments
   int nblk = 0, bmax = 0, seg_threads = 64;
   DBuf<int32_t> bstart, seg;
+  DBuf<int32_t> segb;  // per block: first position of each colour, then row offsets (prefetching sweep)
   std::vector<int32_t> hbstart;
 };
 
@@ -1018,6 +1054,20 @@ void build_level_matrix(Level &L, const dec::HostCsr &A, std::vector<int32_t> &c
     int smax = 0;
     for (int c : cnt) smax = std::max(smax, c);
     L.seg_threads = std::min(256, std::max(64, (smax + 63) / 64 * 64));
+    // block-major copy for the prefetching sweep (kPfColors colours at most)
+    if (L.ncolors <= kPfColors) {
+      std::vector<int32_t> sb(2 * static_cast<size_t>(nb) * (kPfColors + 1), 0);
+      for (int k = 0; k < nb; k++) {
+        int acc = 0;
+        for (int c = 0; c < L.ncolors; c++) {
+          sb[2 * static_cast<size_t>(k) * (kPfColors + 1) + c] = seg[static_cast<size_t>(c) * (nb + 1) + k];
+          sb[(2 * static_cast<size_t>(k) + 1) * (kPfColors + 1) + c] = acc;
+          acc += cnt[static_cast<size_t>(c) * nb + k];
+        }
+        sb[(2 * static_cast<size_t>(k) + 1) * (kPfColors + 1) + L.ncolors] = acc;
+      }
+      L.segb.upload(sb);
+    }
   }
   L.bstart.upload(bstart);
   L.seg.upload(seg);
@@ -1472,7 +1522,7 @@ constexpr int kFusedRestrictMaxRows = 50000;
 // with up to 256 threads), or 0 when it does not apply (wide rows, or a
 // single block larger than 512 rows).
 int prefetch_rows(const Level &L) {
-  if (L.width < 1 || L.width > 8) return 0;
+  if (L.width < 1 || L.width > 8 || L.ncolors > kPfColors) return 0;
   if (L.bmax <= kBlock) return 1;
   if (L.bmax <= 2 * kBlock) return 2;
   return 0;
@@ -1484,8 +1534,8 @@ void launch_block_gs_pf(Level &L, LevelVals<T> &V, int R, int nthr, size_t lds, 
   const T *pv = zc ? V.pval.p : nullptr;
   const int *pc = zc ? L.pcol.p : nullptr;
 #define DEC_BGS_PF(WW, RR)                                                                                            \
-  hipLaunchKernelGGL((k_block_gs_pf<WW, T, RR>), dim3(L.nblk), dim3(nthr), lds, s, L.n, L.ncolors, L.nblk, L.col.p, \
-                     V.val.p, L.rowid.p, L.bstart.p, L.seg.p, b, src, dst, post ? 1 : 0, pc, pv, zc)
+  hipLaunchKernelGGL((k_block_gs_pf<WW, T, RR>), dim3(L.nblk), dim3(nthr), lds, s, L.n, L.ncolors, L.col.p,       \
+                     V.val.p, L.rowid.p, L.bstart.p, L.segb.p, b, src, dst, post ? 1 : 0, pc, pv, zc)
 #define DEC_BGS_PF_R(WW) \
   if (R == 1) DEC_BGS_PF(WW, 1); \
   else DEC_BGS_PF(WW, 2);
@@ -1809,17 +1859,19 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_call
   mg.learned_active = mg.learned && mg.nn && variant == DEC_SOLVE_MIXED_IR && any;
   mg.fallbacks = 0;
   mg.learned_cycles = 0;
-  // The residual norm after k cycles is copied to pinned host memory with an
-  // event per slot; the host reads it when it must decide.  While the last
-  // known residual is above 1000 tol the next cycle cannot meet the tolerance
-  // (no V-cycle cuts the residual 1000x), so the cycle after it is queued at
-  // once and the GPU does not wait for the host between cycles.
-  auto enqueue_norm = [&](int k) {
-    double *slot = mg.scal.p + (k % kNormSlots);
+  // A residual norm is copied to pinned host memory with an event per slot,
+  // and the host reads it when it must decide whether to stop.
+  bool plain = variant == DEC_SOLVE_PLAIN_F64;
+  // after cycle k: the float64 residual (which the mixed variant's next cycle
+  // refines) and, when `check`, its norm on its way to the host
+  auto post = [&](int k, bool check) {
+    if (plain && !check) return;
     int nb = dec::nblocks(n, kBlock);
     if (mg.rpart.n < static_cast<size_t>(nb)) fail(DEC_ERR_ARG, "multigrid: residual partials");
     DEC_ELL_WIDTH_SWITCH(F.width, hipLaunchKernelGGL((k_residual_norm<W>), dim3(nb), dim3(kBlock), 0, s, n, F.width,
                                                      F.col.p, F.d.val.p, F.rowid.p, b, u, F.d.r.p, mg.rpart.p));
+    if (!check) return;
+    double *slot = mg.scal.p + (k % kNormSlots);
     hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(kBlock), 0, s, nb, mg.rpart.p, slot);
     gcheck(hipMemcpyAsync(mg.hnorm + (k % kNormSlots), slot, sizeof(double), hipMemcpyDeviceToHost, s),
            "hipMemcpyAsync");
@@ -1829,29 +1881,53 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_call
     gcheck(hipEventSynchronize(mg.nev[k % kNormSlots]), "hipEventSynchronize");
     return std::sqrt(mg.hnorm[k % kNormSlots]) / bn;
   };
-  auto enqueue_cycle = [&](int k) {  // cycle k (1-based) and the norm after it
+  int enq = 0;               // cycles queued
+  bool enq_checked = true;   // whether the norm after cycle `enq` was queued
+  auto enqueue_cycle = [&](bool check) {
+    enq++;
     if (mg.learned_active) mg.learned_cycles++;
     run_cycle(mg, variant, u, b, s);
-    enqueue_norm(k);
+    post(enq, check);
+    enq_checked = check;
   };
-  enqueue_norm(0);
-  int enq = 0;  // cycles queued
-  double rr = 0.0, prev = 0.0;
+  // The norm is read only where the decision needs it: after cycle 1 (the
+  // convergence rate), then at the first cycle that could meet the
+  // tolerance, then after every cycle (every cycle while the learned
+  // smoother is on: its fallback test needs each drop).  Cycles before a
+  // check run without waiting for the host; while the last known residual
+  // stays above 1000 tol after the queued cycles, the cycle after the check
+  // is queued too (no V-cycle cuts the residual 1000x).
+  post(0, true);
+  double rr = read_norm(0), prev = rr, rho = -1.0;
   int c = 0;
-  for (;; c++) {
-    rr = read_norm(c);
+  while (rr > tol && c < max_cycles) {
+    int kc = c + 1;
+    // (the prediction takes each cycle to cut twice as much as the last ones
+    // did, so the check comes no later than the first cycle that can meet
+    // the tolerance)
+    if (rho > 0.0 && !mg.learned_active)
+      while (kc < max_cycles && rr * std::pow(0.5 * rho, kc - c) > tol) kc++;
+    while (enq < kc) enqueue_cycle(enq + 1 == kc);
+    if (enq == kc && !enq_checked) {
+      // the check cycle was queued ahead without its norm; nothing after it
+      // is queued, so the iterate is still the one after cycle kc
+      post(kc, true);
+      enq_checked = true;
+    }
+    if (kc + 1 <= max_cycles && rr * std::pow(1e-3, kc + 1 - c) > tol) enqueue_cycle(false);
+    double rk = read_norm(kc);
+    rho = std::min(0.95, std::max(1e-4, std::pow(rk / rr, 1.0 / (kc - c))));
     // a learned cycle that cut the residual by less than 0.5x hands the rest
     // of the solve to Gauss-Seidel (from the next cycle not yet queued)
-    if (c > 0 && mg.learned_active && rr > 0.5 * prev) {
+    if (mg.learned_active && rk > 0.5 * prev) {
       mg.learned_active = false;
       mg.fallbacks++;
     }
-    prev = rr;
-    if (rr <= tol || c >= max_cycles) break;
-    if (enq < c + 1) enqueue_cycle(++enq);
-    if (enq == c + 1 && c + 2 <= max_cycles && rr * 1e-3 > tol) enqueue_cycle(++enq);
+    prev = rr = rk;
+    c = kc;
   }
   if (enq > c) {  // a queued cycle ran past the stop (a drop above 1000x): report it
+    if (!enq_checked) post(enq, true);
     c = enq;
     rr = read_norm(c);
   }
