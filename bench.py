@@ -765,6 +765,8 @@ def main():
     ap.add_argument("--no-3d", action="store_true", help="skip the operator set on a tetrahedral mesh")
     ap.add_argument("--ops3d-only", action="store_true",
                     help="time only the operator set on a tetrahedral mesh and print its JSON")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="time only the Poisson solves on levels 3->8 and print their JSON")
     ap.add_argument("--n3d", type=int, default=64, help="cubes per side of the Kuhn box of the 3-D leg")
     ap.add_argument("--traffic-per-op", action="store_true",
                     help="measure the HBM traffic of each operator of the step (two counter passes each)")
@@ -787,6 +789,9 @@ def main():
     if args.ops3d_only:
         print(json.dumps({"ops_3d_f64": ops_3d(torch, cs, args.n3d, steps=args.steps, warmup=args.warmup)}),
               flush=True)
+        return
+    if args.solve_only:
+        print(json.dumps({"poisson": poisson(torch, cs)[0]}), flush=True)
         return
     t = time.perf_counter()
     mesh = cs.icosphere(args.k)
