@@ -129,15 +129,38 @@ struct dec_batch_s {
 
 namespace {
 
+// The work list, laid out for the XCDs.  Workgroup w runs on XCD w mod 8,
+// and the grid is a multiple of 8, so item i is always served by XCD i mod 8.
+// Whole instances are dealt to the 8 XCDs, largest first onto the least
+// loaded, and item 8j + x is the j-th tile of XCD x's instances (an empty
+// item where x has fewer tiles).  An instance's gathers of its state then
+// stay in one XCD's L2 instead of being fetched by all eight.
+constexpr int kXcds = 8;
+
 void build_items(dec_batch_s &b) {
-  std::vector<Item> items;
   int tile = b.shape.block * b.shape.rpt;
-  for (int i = 0; i < b.ninst; i++)
-    for (int r = 0; r < b.inst[i].n; r += tile) items.push_back(Item{i, r, std::min(tile, b.inst[i].n - r)});
-  // interleave large and small instances so that a workgroup's stride visits
-  // a mix of meshes (balance over the uneven sizes)
-  std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &c) { return a.row0 < c.row0; });
-  b.items.upload(std::vector<Item>(items.begin(), items.end()));
+  std::vector<int> order(b.ninst);
+  for (int i = 0; i < b.ninst; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b.inst[a].n > b.inst[c].n; });
+  std::vector<long long> load(kXcds, 0);
+  std::vector<std::vector<int>> group(kXcds);
+  for (int i : order) {
+    int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+    group[x].push_back(i);
+    load[x] += b.inst[i].n;
+  }
+  std::vector<std::vector<Item>> per(kXcds);
+  size_t longest = 0;
+  for (int x = 0; x < kXcds; x++) {
+    std::sort(group[x].begin(), group[x].end());
+    for (int i : group[x])
+      for (int r = 0; r < b.inst[i].n; r += tile) per[x].push_back(Item{i, r, std::min(tile, b.inst[i].n - r)});
+    longest = std::max(longest, per[x].size());
+  }
+  std::vector<Item> items(longest * kXcds, Item{0, 0, 0});
+  for (int x = 0; x < kXcds; x++)
+    for (size_t j = 0; j < per[x].size(); j++) items[j * kXcds + x] = per[x][j];
+  b.items.upload(items);
   b.nitems = static_cast<int>(items.size());
 }
 
