@@ -323,7 +323,36 @@ def poisson(torch, cs, base_k=3, levels=5):
             out[key] = {"seconds": dt, "cycles": cyc, "relres_f64": rr, "smoother": alg.name,
                         "cycle_GBps": gbps, "frac_hbm_peak": gbps / 8000.0}
     out["speedup_mixed_vs_plain"] = out["plain_f64"]["seconds"] / out["mixed_f32_ir"]["seconds"]
+    # conjugate gradients preconditioned by one V-cycle (float64 or float32),
+    # the hybrid block Gauss-Seidel smoother
+    extra = pcg_extra_bytes(md)
+    for variant, key, cyc in (("pcg", "pcg_f64", "plain"), ("pcg_mixed", "pcg_mixed", "mixed")):
+        cs.solve(md, fd, tol=1e-8, variant=variant)
+        torch.cuda.synchronize()
+        dt = None
+        for _ in range(3):
+            t = time.perf_counter()
+            u, it, rr = cs.solve(md, fd, tol=1e-8, variant=variant)
+            torch.cuda.synchronize()
+            e = time.perf_counter() - t
+            dt = e if dt is None else min(dt, e)
+        per_it = cyc_bytes[cyc] + extra[cyc]
+        gbps = it * per_it / dt / 1e9
+        out[key] = {"seconds": dt, "iterations": it, "relres_f64": rr, "smoother": "block_gauss_seidel",
+                    "bytes_per_iteration": per_it, "iteration_GBps": gbps, "frac_hbm_peak": gbps / 8000.0}
     return out, md, fd
+
+
+def pcg_extra_bytes(md):
+    """Algorithmic bytes a CG iteration adds to its V-cycle on the finest level
+    (float64 vectors, int32 indices): zeroing z; the dot r·z (r, z); the
+    direction launch (the operator's entries and row index, z and p_old, p and
+    A p written); the update (x, r, p, A p read, x, r, p_old written); and for
+    the float32 preconditioner the conversions of r and z (8 + 4 bytes each)."""
+    n = md.sizes[0]
+    nnz = int(md.export(0, "A")[0][-1])
+    base = n * 8 + 2 * n * 8 + (nnz * 12 + n * 4 + 2 * n * 8 + 2 * n * 8) + 7 * n * 8
+    return {"plain": base, "mixed": base + 2 * n * 12 + n * 4}
 
 
 def vcycle_bytes(md, s):

@@ -21,7 +21,7 @@ DEC_GEOMETRIC_HODGE, DEC_DIAGONAL_HODGE = 0, 1
  DEC_OP_P0_D0_INTERPOLATION) = range(1, 20)
 DEC_SMOOTH_CG, DEC_SMOOTH_GS, DEC_SMOOTH_JACOBI, DEC_SMOOTH_BLOCK_GS = 0, 1, 2, 3
 DEC_MG_DIRECT, DEC_MG_GALERKIN = 0, 1
-DEC_SOLVE_PLAIN_F64, DEC_SOLVE_MIXED_IR = 0, 1
+DEC_SOLVE_PLAIN_F64, DEC_SOLVE_MIXED_IR, DEC_SOLVE_PCG_F64, DEC_SOLVE_PCG_MIXED = 0, 1, 2, 3
 
 (DEC_OK, DEC_ERR_ARG, DEC_ERR_UNSUPPORTED, DEC_ERR_OOM, DEC_ERR_HIP, DEC_ERR_NOCONV,
  DEC_ERR_NONORIENTABLE) = range(7)

@@ -229,15 +229,18 @@ def dec_Delta_inv(md: MultigridData, steps=3, cycles=5, alg=cg, mu=2):
 def solve(md: MultigridData, b, u=None, tol=1e-8, max_cycles=200, variant="plain", alg=block_gauss_seidel,
           steps=2, coarse_direct=True, stream=None):
     """Solve A u = b to ‖b - A u‖₂/‖b‖₂ ≤ tol (checked in float64) with float64
-    V-cycles ("plain") or float32 V-cycles inside float64 iterative refinement
-    ("mixed").  Returns (u, cycles, relative residual)."""
+    V-cycles ("plain"), float32 V-cycles inside float64 iterative refinement
+    ("mixed"), or float64 conjugate gradients preconditioned by one float64
+    ("pcg") or float32 ("pcg_mixed") V-cycle.  Returns (u, cycles or CG
+    iterations, relative residual)."""
     import torch
 
     _prepare(md, alg, steps=steps, coarse_direct=coarse_direct)
     n = md.sizes[0]
     if u is None:
         u = torch.zeros(n, dtype=torch.float64, device=b.device)
-    v = {"plain": L.DEC_SOLVE_PLAIN_F64, "mixed": L.DEC_SOLVE_MIXED_IR}[variant]
+    v = {"plain": L.DEC_SOLVE_PLAIN_F64, "mixed": L.DEC_SOLVE_MIXED_IR, "pcg": L.DEC_SOLVE_PCG_F64,
+         "pcg_mixed": L.DEC_SOLVE_PCG_MIXED}[variant]
     c = c_int32()
     r = c_double()
     check(lib().dec_mg_solve(md.handle, v, c_double(tol), max_cycles, _dev(u, n, "u"), _dev(b, n, "b"),

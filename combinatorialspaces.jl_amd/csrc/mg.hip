@@ -518,6 +518,7 @@ __global__ void k_cgf_xr(CgState *st, int c, int n, double *__restrict__ x, cons
   if (st->done) return;
   const double eps = 2.220446049250313e-16;
   double pAp = block_sum<double>(threadIdx.x < kDotBlocks ? part[threadIdx.x] : 0.0);
+  __syncthreads();  // every thread has read pAp before the second block_sum below
   double pn2 = st->pnorm2[c];
   if (pAp <= eps * pn2 && fabs(pAp) <= eps * pn2) {
     if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
@@ -575,6 +576,132 @@ __global__ void k_d2f(int n, const double *__restrict__ x, float *__restrict__ y
 __global__ void k_add_f2d(int n, const float *__restrict__ c, double *__restrict__ u) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) u[i] = u[i] + static_cast<double>(c[i]);
+}
+
+__global__ void k_f2d(int n, const float *__restrict__ c, double *__restrict__ z) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = static_cast<double>(c[i]);
+}
+
+// ---------------------------------------------- V-cycle-preconditioned CG
+// Conjugate gradients on the finest level with one V-cycle (from a zero
+// start) as the preconditioner z = M r:
+//   β = (r·z) / (r·z)_prev,  p = z + β p,  q = A p,  α = (r·z) / (p·q),
+//   x += α p,  r -= α q.
+// Three launches per iteration after the V-cycle, with the scalars on the
+// device: every workgroup adds the previous launch's kPcgBlocks partial sums
+// itself, in one fixed order (four consecutive partials per thread, then
+// block_sum's tree), so all hold the same β or α.  Vectors are indexed by
+// the hierarchy's vertex numbers; the product runs over the level's ELL
+// positions (row rowid[p]).  sc[0] = (r·z)_prev (1 before the first
+// iteration, when p = 0), sc[1] = r·z: the direction launch reads sc[0] and
+// its block 0 writes sc[1]; the update launch reads sc[1] and its block 0
+// copies it to sc[0], so no launch reads a slot it writes.
+constexpr int kPcgBlocks = 1024;
+static_assert(kPcgBlocks == 4 * kBlock, "sum_pcg_partials adds four partials per thread");
+
+// (every thread has read block_sum's result before the kernel's second
+// block_sum writes the same LDS array)
+__device__ double sum_pcg_partials(const double *__restrict__ part) {
+  const double *q = part + 4 * threadIdx.x;
+  double acc = ((q[0] + q[1]) + q[2]) + q[3];
+  double s = block_sum<double>(acc);
+  __syncthreads();
+  return s;
+}
+
+// partial sums of r·z
+__global__ void __launch_bounds__(kBlock) k_pcg_dot(int n, const double *__restrict__ r, const double *__restrict__ z,
+                                                    double *__restrict__ part) {
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kPcgBlocks * kBlock) acc = acc + r[i] * z[i];
+  double s = block_sum<double>(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// β, then for every row p_new = z + β p_old and q = A p_new (the neighbours'
+// p_new formed from z and p_old in the row sum, slots in ascending column
+// order), and the partial sums of p_new·q
+template <int W>
+__global__ void __launch_bounds__(kBlock)
+    k_pcg_dir(int n, int width, const int *__restrict__ col, const double *__restrict__ val,
+              const int *__restrict__ rowid, const double *__restrict__ z, const double *__restrict__ pold,
+              double *__restrict__ pnew, double *__restrict__ q, const double *__restrict__ part_rz,
+              double *__restrict__ sc, double *__restrict__ part_pq) {
+  double rz = sum_pcg_partials(part_rz);
+  double beta = rz / sc[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[1] = rz;
+  double acc = 0.0;
+  for (int p = blockIdx.x * kBlock + threadIdx.x; p < n; p += kPcgBlocks * kBlock) {
+    double s = 0.0;
+    if constexpr (W > 0) {
+      int c[W];
+      double v[W], zc[W], pc[W];
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        size_t o = static_cast<size_t>(k) * n + p;
+        c[k] = col[o];
+        v[k] = val[o];
+      }
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        int cc = c[k] < 0 ? 0 : c[k];
+        zc[k] = z[cc];
+        pc[k] = pold[cc];
+      }
+#pragma unroll
+      for (int k = 0; k < W; k++)
+        if (c[k] >= 0) s = s + v[k] * (zc[k] + beta * pc[k]);
+    } else {
+      for (int k = 0; k < width; k++) {
+        size_t o = static_cast<size_t>(k) * n + p;
+        int cc = col[o];
+        if (cc < 0) break;
+        s = s + val[o] * (z[cc] + beta * pold[cc]);
+      }
+    }
+    int v = rowid[p];
+    double pv = z[v] + beta * pold[v];
+    pnew[v] = pv;
+    q[v] = s;
+    acc = acc + pv * s;
+  }
+  double t = block_sum<double>(acc);
+  if (threadIdx.x == 0) part_pq[blockIdx.x] = t;
+}
+
+// α = (r·z) / (p·q) (zero when p·q is not positive), x += α p, r -= α q,
+// p_old = p, and the partial sums of r·r
+__global__ void __launch_bounds__(kBlock)
+    k_pcg_update(int n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+                 const double *__restrict__ q, double *__restrict__ pold, const double *__restrict__ part_pq,
+                 double *__restrict__ sc, double *__restrict__ part_rr) {
+  double pq = sum_pcg_partials(part_pq);
+  double rz = sc[1];
+  double alpha = pq > 0.0 ? rz / pq : 0.0;
+  if (!(alpha == alpha) || fabs(alpha) > 1e300) alpha = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[0] = rz;
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kPcgBlocks * kBlock) {
+    double pi = p[i];
+    x[i] = x[i] + alpha * pi;
+    double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    pold[i] = pi;
+    acc = acc + ri * ri;
+  }
+  double t = block_sum<double>(acc);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = t;
+}
+
+// p_old = 0, sc[0] = 1: the first direction is z
+__global__ void k_pcg_start(int n, double *__restrict__ pold, double *__restrict__ sc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) pold[i] = 0.0;
+  if (i == 0) {
+    sc[0] = 1.0;
+    sc[1] = 0.0;
+  }
 }
 
 
@@ -863,11 +990,17 @@ struct dec_mg_s {
   std::vector<LevelDesc<float>> hdesc32;
   bool use_fused = true;
   int fuse_rows = kFuseRowsDefault;
-  // graph of one solve cycle (per variant) for fixed u, b
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  const void *gu[2] = {nullptr, nullptr}, *gb[2] = {nullptr, nullptr};
-  int gkey[2] = {-1, -1};
+  // graph of one solve cycle or preconditioned-CG iteration (per variant)
+  // for fixed u, b
+  static constexpr int kGraphs = 4;
+  hipGraphExec_t gexec[kGraphs] = {};
+  hipGraph_t graph[kGraphs] = {};
+  const void *gu[kGraphs] = {}, *gb[kGraphs] = {};
+  int gkey[kGraphs] = {-1, -1, -1, -1};
+  // preconditioned CG on the finest level: residual, preconditioned residual,
+  // direction (previous and current), A times the direction, the partial sums
+  // of r·z, p·q and r·r (kPcgBlocks each) and the two scalars
+  DBuf<double> pcg_r, pcg_z, pcg_pold, pcg_p, pcg_q, pcg_part, pcg_sc;
   hipStream_t cap = nullptr;
   bool use_graph = true;
   // residual norms of a solve on their way to the host (pinned ring + events)
@@ -881,7 +1014,7 @@ struct dec_mg_s {
   bool learned_active = false;
   int fallbacks = 0, learned_cycles = 0;
   ~dec_mg_s() {
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < kGraphs; i++) {
       if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
       if (graph[i]) (void)hipGraphDestroy(graph[i]);
     }
@@ -1801,17 +1934,13 @@ void gcheck(hipError_t e, const char *what) {
   if (e != hipSuccess) fail(DEC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// One cycle of the solve, replayed from a hipGraph captured for (u, b) and
-// the current smoother settings.
-void run_cycle(dec_mg_s &mg, int variant, double *u, const double *b, hipStream_t s) {
-  if (!mg.use_graph) {
-    cycle_body(mg, variant, u, b, s);
-    return;
-  }
+// body(stream) replayed from the hipGraph in slot i, captured for (u, b) and
+// the current smoother settings (key); captured again when any of them changes.
+template <class Body>
+void replay(dec_mg_s &mg, int i, const void *u, const void *b, Body body, hipStream_t s) {
   int key = mg.alg * 1000003 + mg.steps * 1009 + mg.coarse_direct * 7 + static_cast<int>(mg.omega * 1e6) +
             (mg.learned && mg.learned_active ? 1 << 29 : 0) + mg.learned_apps * 31 + mg.learned_min_rows +
             mg.learned_max_rows * 7919;
-  int i = variant == DEC_SOLVE_PLAIN_F64 ? 0 : 1;
   if (!mg.gexec[i] || mg.gu[i] != u || mg.gb[i] != b || mg.gkey[i] != key) {
     if (mg.gexec[i]) (void)hipGraphExecDestroy(mg.gexec[i]);
     if (mg.graph[i]) (void)hipGraphDestroy(mg.graph[i]);
@@ -1820,7 +1949,7 @@ void run_cycle(dec_mg_s &mg, int variant, double *u, const double *b, hipStream_
     if (!mg.cap) gcheck(hipStreamCreateWithFlags(&mg.cap, hipStreamNonBlocking), "hipStreamCreate");
     gcheck(hipStreamBeginCapture(mg.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
-      cycle_body(mg, variant, u, b, mg.cap);
+      body(mg.cap);
     } catch (...) {
       // close the capture, or every later allocation on this thread fails
       hipGraph_t partial = nullptr;
@@ -1835,6 +1964,17 @@ void run_cycle(dec_mg_s &mg, int variant, double *u, const double *b, hipStream_
     mg.gkey[i] = key;
   }
   gcheck(hipGraphLaunch(mg.gexec[i], s), "hipGraphLaunch");
+}
+
+// One cycle of the solve, replayed from a hipGraph captured for (u, b) and
+// the current smoother settings.
+void run_cycle(dec_mg_s &mg, int variant, double *u, const double *b, hipStream_t s) {
+  if (!mg.use_graph) {
+    cycle_body(mg, variant, u, b, s);
+    return;
+  }
+  replay(mg, variant == DEC_SOLVE_PLAIN_F64 ? 0 : 1, u, b,
+         [&](hipStream_t cs) { cycle_body(mg, variant, u, b, cs); }, s);
 }
 
 void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_caller, const double *b_caller,
@@ -1936,6 +2076,156 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_call
   *relres = rr;
   if (rr > tol) fail(DEC_ERR_NOCONV, "dec_mg_solve: relative residual " + std::to_string(rr) + " above tolerance after " +
                                          std::to_string(c) + " cycles");
+}
+
+// One iteration of the preconditioned CG: z = one V-cycle on r from a zero
+// start (float32 cycles on the rounded residual for the mixed variant, z
+// their float64 value), then the three CG launches.
+void pcg_iteration(dec_mg_s &mg, bool mixed, double *x, hipStream_t s) {
+  Level &F = *mg.lv[0];
+  int n = F.n;
+  double *r = mg.pcg_r.p, *z = mg.pcg_z.p, *part = mg.pcg_part.p;
+  if (!mixed) {
+    hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, z);
+    mu_cycle<double>(mg, 0, z, r, 1, true, s);
+  } else {
+    hipLaunchKernelGGL(k_d2f, G(n), dim3(kBlock), 0, s, n, r, F.f.b.p);
+    hipLaunchKernelGGL(k_zero<float>, G(n), dim3(kBlock), 0, s, n, F.f.u.p);
+    mu_cycle<float>(mg, 0, F.f.u.p, F.f.b.p, 1, true, s);
+    hipLaunchKernelGGL(k_f2d, G(n), dim3(kBlock), 0, s, n, F.f.u.p, z);
+  }
+  hipLaunchKernelGGL(k_pcg_dot, dim3(kPcgBlocks), dim3(kBlock), 0, s, n, r, z, part);
+  DEC_ELL_WIDTH_SWITCH(F.width, hipLaunchKernelGGL((k_pcg_dir<W>), dim3(kPcgBlocks), dim3(kBlock), 0, s, n, F.width,
+                                                   F.col.p, F.d.val.p, F.rowid.p, z, mg.pcg_pold.p, mg.pcg_p.p,
+                                                   mg.pcg_q.p, part, mg.pcg_sc.p, part + kPcgBlocks));
+  hipLaunchKernelGGL(k_pcg_update, dim3(kPcgBlocks), dim3(kBlock), 0, s, n, x, r, mg.pcg_p.p, mg.pcg_q.p,
+                     mg.pcg_pold.p, part + kPcgBlocks, mg.pcg_sc.p, part + 2 * kPcgBlocks);
+}
+
+// Solve with the preconditioned CG (variants DEC_SOLVE_PCG_F64 / _MIXED).
+// The stop test reads the recursively updated residual's norm where the
+// decision needs it, as solve() does; once it meets the tolerance the true
+// residual b - A x is formed and must meet it too (otherwise CG restarts
+// from the true residual).  An iteration rate of 0.9 or worse hands the rest
+// of the solve to plain V-cycles from the current iterate.  The learned
+// smoother is not used in these variants.
+void pcg_solve(dec_mg_s &mg, bool mixed, double tol, int max_iters, double *u_caller, const double *b_caller,
+               int *done_iters, double *relres, hipStream_t s) {
+  Level &F = *mg.lv[0];
+  int n = F.n;
+  double bn = norm2(mg, n, b_caller, s);
+  if (bn == 0.0) {
+    hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, u_caller);
+    *done_iters = 0;
+    *relres = 0.0;
+    return;
+  }
+  if (mg.pcg_r.n < static_cast<size_t>(n)) {
+    for (auto *v : {&mg.pcg_r, &mg.pcg_z, &mg.pcg_pold, &mg.pcg_p, &mg.pcg_q}) v->alloc(n);
+    mg.pcg_part.alloc(3 * kPcgBlocks);
+    mg.pcg_sc.alloc(2);
+  }
+  FineIO io = fine_in(mg, u_caller, b_caller, true, s);
+  double *x = io.u;
+  const double *b = io.b;
+  mg.learned_active = false;
+  int nb = dec::nblocks(n, kBlock);
+  if (mg.rpart.n < static_cast<size_t>(nb)) fail(DEC_ERR_ARG, "multigrid: residual partials");
+  // a norm on its way to the host: the partial sums part[0..np) added into
+  // ring slot k with an event
+  auto send = [&](int k, const double *part, int np) {
+    double *slot = mg.scal.p + (k % kNormSlots);
+    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(kBlock), 0, s, np, part, slot);
+    gcheck(hipMemcpyAsync(mg.hnorm + (k % kNormSlots), slot, sizeof(double), hipMemcpyDeviceToHost, s),
+           "hipMemcpyAsync");
+    gcheck(hipEventRecord(mg.nev[k % kNormSlots], s), "hipEventRecord");
+  };
+  auto read_norm = [&](int k) {
+    gcheck(hipEventSynchronize(mg.nev[k % kNormSlots]), "hipEventSynchronize");
+    return std::sqrt(mg.hnorm[k % kNormSlots]) / bn;
+  };
+  // the true residual b - A x into F.d.r, its norm to slot k
+  auto true_residual = [&](int k) {
+    DEC_ELL_WIDTH_SWITCH(F.width, hipLaunchKernelGGL((k_residual_norm<W>), dim3(nb), dim3(kBlock), 0, s, n, F.width,
+                                                     F.col.p, F.d.val.p, F.rowid.p, b, x, F.d.r.p, mg.rpart.p));
+    send(k, mg.rpart.p, nb);
+  };
+  auto restart = [&]() {
+    hipLaunchKernelGGL(k_copy<double>, G(n), dim3(kBlock), 0, s, n, F.d.r.p, mg.pcg_r.p);
+    hipLaunchKernelGGL(k_pcg_start, G(n), dim3(kBlock), 0, s, n, mg.pcg_pold.p, mg.pcg_sc.p);
+  };
+  int enq = 0;
+  bool enq_checked = true;
+  auto enqueue = [&](bool check) {
+    enq++;
+    if (mg.use_graph)
+      replay(mg, mixed ? 3 : 2, x, b, [&](hipStream_t cs) { pcg_iteration(mg, mixed, x, cs); }, s);
+    else
+      pcg_iteration(mg, mixed, x, s);
+    if (check) send(enq, mg.pcg_part.p + 2 * kPcgBlocks, kPcgBlocks);
+    enq_checked = check;
+  };
+  true_residual(0);
+  restart();
+  double rr = read_norm(0), rho = -1.0;
+  int c = 0;
+  bool stalled = false, verified = rr <= tol;
+  while (c < max_iters && !verified) {
+    if (rr <= tol) {
+      // the recursive residual meets the tolerance: confirm it on the
+      // iterate after every queued iteration
+      c = enq;
+      true_residual(c);
+      rr = read_norm(c);
+      verified = rr <= tol;
+      if (!verified) {
+        restart();
+        rho = -1.0;
+      }
+      continue;
+    }
+    // the check comes no later than the first iteration that can meet the
+    // tolerance (each predicted to cut twice as much as the last ones did)
+    int kc = c + 1;
+    if (rho > 0.0)
+      while (kc < max_iters && rr * std::pow(0.5 * rho, kc - c) > tol) kc++;
+    while (enq < kc) enqueue(enq + 1 == kc);
+    if (enq == kc && !enq_checked) {
+      // queued ahead without its norm; nothing after it is queued, so its
+      // partial sums are still in place
+      send(kc, mg.pcg_part.p + 2 * kPcgBlocks, kPcgBlocks);
+      enq_checked = true;
+    }
+    if (kc + 1 <= max_iters && rr * std::pow(1e-3, kc + 1 - c) > tol) enqueue(false);
+    double rk = read_norm(kc);
+    double rate = std::pow(rk / rr, 1.0 / (kc - c));
+    rr = rk;
+    c = kc;
+    if (!(rate < 0.9)) {
+      stalled = true;
+      break;
+    }
+    rho = std::min(0.95, std::max(1e-4, rate));
+  }
+  if (!verified) {
+    // the true residual after every queued iteration; when CG stalled,
+    // plain V-cycles from there
+    c = enq;
+    true_residual(c);
+    rr = read_norm(c);
+    int variant = mixed ? DEC_SOLVE_MIXED_IR : DEC_SOLVE_PLAIN_F64;
+    while (stalled && rr > tol && c < max_iters) {
+      run_cycle(mg, variant, x, b, s);
+      c++;
+      true_residual(c);
+      rr = read_norm(c);
+    }
+  }
+  fine_out(mg, u_caller, s);
+  *done_iters = c;
+  *relres = rr;
+  if (rr > tol) fail(DEC_ERR_NOCONV, "dec_mg_solve: relative residual " + std::to_string(rr) + " above tolerance after " +
+                                         std::to_string(c) + " iterations");
 }
 
 }  // namespace
@@ -2082,14 +2372,20 @@ int dec_mg_solve(dec_mg mg, int32_t variant, double tol, int32_t max_cycles, voi
                  int32_t *cycles_done, double *relres, void *stream) {
   DEC_TRY
   if (!mg || !u || !b) fail(DEC_ERR_ARG, "dec_mg_solve: NULL argument");
-  if (variant != DEC_SOLVE_PLAIN_F64 && variant != DEC_SOLVE_MIXED_IR) fail(DEC_ERR_ARG, "dec_mg_solve: bad variant");
-  if (variant == DEC_SOLVE_MIXED_IR && mg->alg == DEC_SMOOTH_CG)
-    fail(DEC_ERR_UNSUPPORTED, "dec_mg_solve: the mixed variant needs the gauss_seidel or jacobi smoother");
+  if (variant < DEC_SOLVE_PLAIN_F64 || variant > DEC_SOLVE_PCG_MIXED) fail(DEC_ERR_ARG, "dec_mg_solve: bad variant");
+  if ((variant == DEC_SOLVE_MIXED_IR || variant == DEC_SOLVE_PCG_MIXED) && mg->alg == DEC_SMOOTH_CG)
+    fail(DEC_ERR_UNSUPPORTED, "dec_mg_solve: the mixed variants need a Gauss-Seidel or Jacobi smoother");
+  if ((variant == DEC_SOLVE_PCG_F64 || variant == DEC_SOLVE_PCG_MIXED) && mg->alg == DEC_SMOOTH_CG)
+    fail(DEC_ERR_UNSUPPORTED, "dec_mg_solve: the CG variants take the V-cycle of a Gauss-Seidel or Jacobi smoother");
   int c = 0;
   double rr = 0.0;
   try {
-    solve(*mg, variant, tol, max_cycles, static_cast<double *>(u), static_cast<const double *>(b), &c, &rr,
-          static_cast<hipStream_t>(stream));
+    if (variant == DEC_SOLVE_PCG_F64 || variant == DEC_SOLVE_PCG_MIXED)
+      pcg_solve(*mg, variant == DEC_SOLVE_PCG_MIXED, tol, max_cycles, static_cast<double *>(u),
+                static_cast<const double *>(b), &c, &rr, static_cast<hipStream_t>(stream));
+    else
+      solve(*mg, variant, tol, max_cycles, static_cast<double *>(u), static_cast<const double *>(b), &c, &rr,
+            static_cast<hipStream_t>(stream));
   } catch (...) {
     if (cycles_done) *cycles_done = c;
     if (relres) *relres = rr;

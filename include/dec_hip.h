@@ -64,7 +64,12 @@ enum dec_op_kind {
   DEC_OP_P0_D0_INTERPOLATION = 19  /* primal 0-forms -> dual 0-forms, nt x nv */
 };
 enum dec_smoother { DEC_SMOOTH_CG = 0, DEC_SMOOTH_GS = 1, DEC_SMOOTH_JACOBI = 2, DEC_SMOOTH_BLOCK_GS = 3 };
-enum dec_solve_variant { DEC_SOLVE_PLAIN_F64 = 0, DEC_SOLVE_MIXED_IR = 1 };
+enum dec_solve_variant {
+  DEC_SOLVE_PLAIN_F64 = 0,
+  DEC_SOLVE_MIXED_IR = 1,
+  DEC_SOLVE_PCG_F64 = 2,
+  DEC_SOLVE_PCG_MIXED = 3
+};
 
 /* ---------------------------------------------------------------- library */
 const char *dec_last_error(void);
@@ -317,8 +322,15 @@ int dec_mg_full_multigrid(dec_mg mg, int32_t mu, int32_t cycles, const void *b,
                           void *u, void *stream);
 /* Solve A u = b to ||b - A u||_2 / ||b||_2 <= tol (checked in float64).
  * variant DEC_SOLVE_PLAIN_F64: float64 V-cycles; DEC_SOLVE_MIXED_IR: float32
- * V-cycles inside float64 iterative refinement.  Returns the cycle count and
- * the final float64 relative residual. */
+ * V-cycles inside float64 iterative refinement; DEC_SOLVE_PCG_F64 /
+ * DEC_SOLVE_PCG_MIXED: conjugate gradients in float64 with one V-cycle
+ * (float64 / float32) from a zero start as the preconditioner, for operators
+ * that are symmetric up to a positive diagonal scaling (the 0-form Laplacian
+ * is); an iteration that cuts the residual by less than 0.9x hands the rest
+ * to the V-cycles of the matching first two variants.  The CG variants do not
+ * use the learned smoother.  No counterpart in the reference (its solves are
+ * multigrid_vcycles / full_multigrid, Multigrid.jl:814-855).  Returns the
+ * cycle (iteration) count and the final float64 relative residual of b - A u. */
 int dec_mg_solve(dec_mg mg, int32_t variant, double tol, int32_t max_cycles,
                  void *u, const void *b, int32_t *cycles_done, double *relres,
                  void *stream);

@@ -611,3 +611,28 @@ def cg(A: CSC, b, x0, itmax):
     x = _f64(x0).copy()
     lib().om_cg(ctypes.byref(A.struct()), _p(_f64(b)), _p(x), c_int(itmax))
     return x
+
+
+def pcg(A: CSC, b, x0, iters, precond):
+    """Preconditioned conjugate gradients, the library's DEC_SOLVE_PCG_*
+    variants (no reference counterpart; the reference's solves are V/W cycles
+    and full multigrid, src/Multigrid.jl:814-855).  The textbook recurrence
+    (Saad, Iterative Methods for Sparse Linear Systems, 2nd ed., Alg. 9.1):
+    z = M r, β = (r·z)/(r·z)_prev, p = z + β p, α = (r·z)/(p·A p),
+    x += α p, r -= α A p; α = 0 when p·A p is not positive.  `precond(r)`
+    returns z.  Runs `iters` iterations; returns (x, r)."""
+    x = _f64(x0).copy()
+    r = _f64(b) - A.matvec(x)
+    p = np.zeros_like(x)
+    rz_prev = 1.0
+    for _ in range(iters):
+        z = precond(r)
+        rz = float(r @ z)
+        p = z + (rz / rz_prev) * p
+        q = A.matvec(p)
+        pq = float(p @ q)
+        alpha = rz / pq if pq > 0 else 0.0
+        x = x + alpha * p
+        r = r - alpha * q
+        rz_prev = rz
+    return x, r

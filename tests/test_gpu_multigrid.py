@@ -263,6 +263,47 @@ def test_block_gs_solve(O, cs, sphere5_pair, variant):
     assert cb <= cg_ + 1
 
 
+def test_pcg_iterates_match_oracle(O, cs, sphere5_pair):
+    """Three iterations of CG preconditioned by one V(2,2) block Gauss-Seidel
+    cycle against the oracle's PCG with the oracle's V-cycle (the library's
+    colours and blocks) as M: to 1e-9 (the cycles themselves agree to 1e-10)."""
+    import torch
+
+    H, md = sphere5_pair
+    L = H.A[0]
+    f = sphere_rhs(O, H.duals[0], 11)
+    cols = [md.colors(lv)[0] for lv in range(md.nlevels)]
+    blks = [md.blocks(lv)[0] for lv in range(md.nlevels)]
+
+    def vcycle(r):
+        return H.mu_cycles(np.zeros(L.n), r, 1, alg=O.BLOCK_GS, steps=2, coarse_direct=True, colors=cols,
+                           blocks=blks)
+
+    x_o, _ = O.pcg(L, f, np.zeros(L.n), 3, vcycle)
+    u = torch.zeros(L.n, dtype=torch.float64, device="cuda")
+    with pytest.raises(cs.DecError):   # 1e-30 is out of reach: exactly 3 iterations run
+        cs.solve(md, torch.tensor(f, device="cuda"), u=u, tol=1e-30, max_cycles=3, variant="pcg")
+    assert np.linalg.norm(u.cpu().numpy() - x_o) <= 1e-9 * np.linalg.norm(x_o)
+
+
+@pytest.mark.parametrize("variant", ["pcg", "pcg_mixed"])
+def test_pcg_solve(O, cs, sphere5_pair, variant):
+    """CG preconditioned by one V-cycle (float64, or float32 for pcg_mixed)
+    reaches 1e-8, checked by the oracle's float64 operator, in fewer
+    iterations than the stationary cycles of the same precision."""
+    import torch
+
+    H, md = sphere5_pair
+    L = H.A[0]
+    f = sphere_rhs(O, H.duals[0], 3)
+    df = torch.tensor(f, device="cuda")
+    u, it, rr = cs.solve(md, df, tol=1e-8, variant=variant)
+    true = np.linalg.norm(f - L.matvec(u.cpu().numpy())) / np.linalg.norm(f)
+    assert rr <= 1e-8 and true <= 1.01e-8 and abs(true - rr) <= 1e-3 * rr
+    _, cyc, _ = cs.solve(md, df, tol=1e-8, variant="plain" if variant == "pcg" else "mixed")
+    assert it < cyc, (it, cyc)
+
+
 def test_hierarchy_from_matrices(O, cs, sphere5_pair):
     """MultigridData(ops, rs, ps, steps) (Multigrid.jl:545-546): the oracle's own
     CSR matrices handed to the library give the oracle's cycles (CG smoother
