@@ -35,7 +35,8 @@ __global__ void k_zero(int n, T *__restrict__ x) {
   if (i < n) x[i] = static_cast<T>(0);
 }
 
-// r[v] = b[v] - (A u)[v], row sums in ascending column order (b - A*u).
+// r[v] = b[v] - (A u)[v], row sums in ascending column order (b - A*u); the
+// rows at ELL positions p are rowid[p] (rows in vertex order: rowid null).
 template <int W, class T>
 __global__ void k_residual(int n, int width, const int *__restrict__ col, const T *__restrict__ val,
                            const int *__restrict__ rowid, const T *__restrict__ b, const T *__restrict__ u,
@@ -43,7 +44,7 @@ __global__ void k_residual(int n, int width, const int *__restrict__ col, const 
   int p = dec_ell::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   T acc = dec_ell::row<W, T, T>(n, width, p, col, val, u);
-  int v = rowid[p];
+  int v = rowid ? rowid[p] : p;
   r[v] = b[v] - acc;
 }
 
@@ -62,7 +63,7 @@ __global__ void __launch_bounds__(256)
   double rr = 0.0;
   if (p < n) {
     double acc = dec_ell::row<W, double, double>(n, width, p, col, val, u);
-    int v = rowid[p];
+    int v = rowid ? rowid[p] : p;
     double ri = b[v] - acc;
     r[v] = ri;
     rr = ri * ri;
@@ -151,16 +152,16 @@ __global__ void k_restrict(int nc, int width, const int *__restrict__ col, const
 
 // k_residual and k_restrict in one launch for levels where launches, not
 // bytes, bound the cycle.  Eight lanes per coarse row i: lane s evaluates
-// r_j = b_j - (A u)_j for the row's slot s (j = R's column, fine row j at
-// ELL position pos[j], summed as k_residual does) and multiplies it by R's
+// r_j = b_j - (A u)_j for the row's slot s (j = R's column; the fine rows in
+// vertex order, summed as k_residual does) and multiplies it by R's
 // entry; lane 0 then adds the eight products in slot order, so b_c is
 // bit-identical to the two launches.  Fine rows shared by two coarse rows are
 // evaluated twice.  R's width is at most 8.
 template <int W, class T>
 __global__ void __launch_bounds__(256)
     k_residual_restrict(int nc, int rwidth, const int *__restrict__ rcol, const T *__restrict__ rval, int n,
-                        int width, const int *__restrict__ col, const T *__restrict__ val, const int *__restrict__ pos,
-                        const T *__restrict__ b, const T *__restrict__ u, T *__restrict__ bc, T *__restrict__ uc) {
+                        int width, const int *__restrict__ col, const T *__restrict__ val, const T *__restrict__ b,
+                        const T *__restrict__ u, T *__restrict__ bc, T *__restrict__ uc) {
   __shared__ T prod[256];
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   int i = t >> 3, sl = t & 7;
@@ -170,7 +171,7 @@ __global__ void __launch_bounds__(256)
     size_t q = static_cast<size_t>(sl) * nc + i;
     int j = rcol[q];
     if (j >= 0) {
-      T rj = b[j] - dec_ell::row<W, T, T>(n, width, pos[j], col, val, u);
+      T rj = b[j] - dec_ell::row<W, T, T>(n, width, j, col, val, u);
       pr = rval[q] * rj;
       has = true;
     }
@@ -588,119 +589,149 @@ __global__ void k_f2d(int n, const float *__restrict__ c, double *__restrict__ z
 // start) as the preconditioner z = M r:
 //   β = (r·z) / (r·z)_prev,  p = z + β p,  q = A p,  α = (r·z) / (p·q),
 //   x += α p,  r -= α q.
-// Three launches per iteration after the V-cycle, with the scalars on the
-// device: every workgroup adds the previous launch's kPcgBlocks partial sums
-// itself, in one fixed order (four consecutive partials per thread, then
-// block_sum's tree), so all hold the same β or α.  Vectors are indexed by
-// the hierarchy's vertex numbers; the product runs over the level's ELL
-// positions (row rowid[p]).  sc[0] = (r·z)_prev (1 before the first
-// iteration, when p = 0), sc[1] = r·z: the direction launch reads sc[0] and
-// its block 0 writes sc[1]; the update launch reads sc[1] and its block 0
-// copies it to sc[0], so no launch reads a slot it writes.
-constexpr int kPcgBlocks = 1024;
-static_assert(kPcgBlocks == 4 * kBlock, "sum_pcg_partials adds four partials per thread");
+// Three launches per iteration after the V-cycle, one element per thread,
+// with the scalars on the device: every workgroup adds the previous launch's
+// per-workgroup partial sums itself, in one fixed order (sum_partials), so all
+// hold the same β or α.  Vectors are indexed by the hierarchy's vertex
+// numbers; the product runs over the level's rows in vertex order.
+// sc[0] = (r·z)_prev (1 before the first iteration, when p = 0),
+// sc[1] = r·z, sc[2] = α: the direction launch reads sc[0] and its block 0
+// writes sc[1]; a one-workgroup launch forms α and copies sc[1] to sc[0];
+// the update launch reads α.  No launch with several workgroups reads a slot
+// it writes.  The update launch also prepares the next V-cycle's input: z = 0
+// (float64 preconditioner), or the float32 r and a zero float32 iterate
+// (float32 preconditioner).
 
-// (every thread has read block_sum's result before the kernel's second
-// block_sum writes the same LDS array)
-__device__ double sum_pcg_partials(const double *__restrict__ part) {
-  const double *q = part + 4 * threadIdx.x;
-  double acc = ((q[0] + q[1]) + q[2]) + q[3];
+// Σ part[0..np) in a fixed order, the same in every workgroup: thread t adds
+// a run of ceil(np / kBlock) consecutive partials, then block_sum's tree
+// (every thread has read the result before a later block_sum of the kernel
+// writes the same LDS array)
+__device__ double sum_partials(const double *__restrict__ part, int np) {
+  constexpr int kRun = 16;  // runs up to this long load all their partials at once
+  int per = (np + kBlock - 1) / kBlock;
+  int j0 = min(np, static_cast<int>(threadIdx.x) * per), j1 = min(np, j0 + per);
+  double acc = 0.0;
+  if (per <= kRun) {
+    double v[kRun];
+#pragma unroll
+    for (int k = 0; k < kRun; k++) v[k] = part[min(j0 + k, np - 1)];
+#pragma unroll
+    for (int k = 0; k < kRun; k++)
+      if (j0 + k < j1) acc = acc + v[k];
+  } else {
+    for (int j = j0; j < j1; j++) acc = acc + part[j];
+  }
   double s = block_sum<double>(acc);
   __syncthreads();
   return s;
 }
 
-// partial sums of r·z
-__global__ void __launch_bounds__(kBlock) k_pcg_dot(int n, const double *__restrict__ r, const double *__restrict__ z,
-                                                    double *__restrict__ part) {
+// kPcgDotBlocks partial sums of r·z over a grid-stride loop (z32 non-null:
+// z = the float32 preconditioner's result, stored in float64 first)
+constexpr int kPcgDotBlocks = kBlock;
+
+__global__ void __launch_bounds__(kBlock) k_pcg_dot(int n, const double *__restrict__ r, const float *__restrict__ z32,
+                                                    double *__restrict__ z, double *__restrict__ part) {
   double acc = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kPcgBlocks * kBlock) acc = acc + r[i] * z[i];
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kPcgDotBlocks * kBlock) {
+    double zi;
+    if (z32) {
+      zi = static_cast<double>(z32[i]);
+      z[i] = zi;
+    } else {
+      zi = z[i];
+    }
+    acc = acc + r[i] * zi;
+  }
   double s = block_sum<double>(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// β, then for every row p_new = z + β p_old and q = A p_new (the neighbours'
-// p_new formed from z and p_old in the row sum, slots in ascending column
-// order), and the partial sums of p_new·q
+// α = (r·z) / (p·q) into sc[2] (zero when p·q is not positive), from the
+// direction launch's per-workgroup partial sums of p·q; sc[0] = sc[1] for
+// the next iteration's β
+__global__ void __launch_bounds__(kBlock) k_pcg_alpha(const double *__restrict__ part_pq, int np,
+                                                      double *__restrict__ sc) {
+  double pq = sum_partials(part_pq, np);
+  if (threadIdx.x != 0) return;
+  double rz = sc[1];
+  double alpha = pq > 0.0 ? rz / pq : 0.0;
+  if (!(alpha == alpha) || fabs(alpha) > 1e300) alpha = 0.0;
+  sc[2] = alpha;
+  sc[0] = rz;
+}
+
+// β, then per row p = z + β p and q = A z + β q (= A p, the product of the
+// previous direction carried over, so one vector is gathered; slots in
+// ascending column order), in place, and the partial sums of p·q
 template <int W>
 __global__ void __launch_bounds__(kBlock)
     k_pcg_dir(int n, int width, const int *__restrict__ col, const double *__restrict__ val,
-              const int *__restrict__ rowid, const double *__restrict__ z, const double *__restrict__ pold,
-              double *__restrict__ pnew, double *__restrict__ q, const double *__restrict__ part_rz,
-              double *__restrict__ sc, double *__restrict__ part_pq) {
-  double rz = sum_pcg_partials(part_rz);
+              const int *__restrict__ rowid, const double *__restrict__ z, double *__restrict__ p,
+              double *__restrict__ q, const double *__restrict__ part_rz, int np, double *__restrict__ sc,
+              double *__restrict__ part_pq) {
+  double rz = sum_partials(part_rz, np);
   double beta = rz / sc[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[1] = rz;
+  int pos = dec_ell::xcd_block() * kBlock + threadIdx.x;
   double acc = 0.0;
-  for (int p = blockIdx.x * kBlock + threadIdx.x; p < n; p += kPcgBlocks * kBlock) {
-    double s = 0.0;
-    if constexpr (W > 0) {
-      int c[W];
-      double v[W], zc[W], pc[W];
-#pragma unroll
-      for (int k = 0; k < W; k++) {
-        size_t o = static_cast<size_t>(k) * n + p;
-        c[k] = col[o];
-        v[k] = val[o];
-      }
-#pragma unroll
-      for (int k = 0; k < W; k++) {
-        int cc = c[k] < 0 ? 0 : c[k];
-        zc[k] = z[cc];
-        pc[k] = pold[cc];
-      }
-#pragma unroll
-      for (int k = 0; k < W; k++)
-        if (c[k] >= 0) s = s + v[k] * (zc[k] + beta * pc[k]);
-    } else {
-      for (int k = 0; k < width; k++) {
-        size_t o = static_cast<size_t>(k) * n + p;
-        int cc = col[o];
-        if (cc < 0) break;
-        s = s + val[o] * (z[cc] + beta * pold[cc]);
-      }
-    }
-    int v = rowid[p];
-    double pv = z[v] + beta * pold[v];
-    pnew[v] = pv;
-    q[v] = s;
-    acc = acc + pv * s;
+  if (pos < n) {
+    double s = dec_ell::row<W, double, double>(n, width, pos, col, val, z);
+    int v = rowid ? rowid[pos] : pos;
+    double pv = z[v] + beta * p[v];
+    double qv = s + beta * q[v];
+    p[v] = pv;
+    q[v] = qv;
+    acc = pv * qv;
   }
   double t = block_sum<double>(acc);
   if (threadIdx.x == 0) part_pq[blockIdx.x] = t;
 }
 
-// α = (r·z) / (p·q) (zero when p·q is not positive), x += α p, r -= α q,
-// p_old = p, and the partial sums of r·r
+// x += α p, r -= α q, the partial sums of r·r, and the next V-cycle's input
 __global__ void __launch_bounds__(kBlock)
     k_pcg_update(int n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
-                 const double *__restrict__ q, double *__restrict__ pold, const double *__restrict__ part_pq,
-                 double *__restrict__ sc, double *__restrict__ part_rr) {
-  double pq = sum_pcg_partials(part_pq);
-  double rz = sc[1];
-  double alpha = pq > 0.0 ? rz / pq : 0.0;
-  if (!(alpha == alpha) || fabs(alpha) > 1e300) alpha = 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[0] = rz;
+                 const double *__restrict__ q, const double *__restrict__ sc, double *__restrict__ part_rr,
+                 double *__restrict__ z, float *__restrict__ b32, float *__restrict__ u32) {
+  const double alpha = sc[2];
+  int i = blockIdx.x * kBlock + threadIdx.x;
   double acc = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kPcgBlocks * kBlock) {
-    double pi = p[i];
-    x[i] = x[i] + alpha * pi;
+  if (i < n) {
+    x[i] = x[i] + alpha * p[i];
     double ri = r[i] - alpha * q[i];
     r[i] = ri;
-    pold[i] = pi;
-    acc = acc + ri * ri;
+    acc = ri * ri;
+    if (z) z[i] = 0.0;
+    if (b32) {
+      b32[i] = static_cast<float>(ri);
+      u32[i] = 0.0f;
+    }
   }
   double t = block_sum<double>(acc);
   if (threadIdx.x == 0) part_rr[blockIdx.x] = t;
 }
 
-// p_old = 0, sc[0] = 1: the first direction is z
-__global__ void k_pcg_start(int n, double *__restrict__ pold, double *__restrict__ sc) {
+// (re)start: r = src, p = q = 0, sc[0] = 1 (the first direction is z), and
+// the first V-cycle's input as k_pcg_update prepares it
+__global__ void k_pcg_start(int n, const double *__restrict__ src, double *__restrict__ r, double *__restrict__ p,
+                            double *__restrict__ q, double *__restrict__ sc, double *__restrict__ z,
+                            float *__restrict__ b32, float *__restrict__ u32) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) pold[i] = 0.0;
+  if (i < n) {
+    double ri = src[i];
+    r[i] = ri;
+    p[i] = 0.0;
+    q[i] = 0.0;
+    if (z) z[i] = 0.0;
+    if (b32) {
+      b32[i] = static_cast<float>(ri);
+      u32[i] = 0.0f;
+    }
+  }
   if (i == 0) {
     sc[0] = 1.0;
     sc[1] = 0.0;
+    sc[2] = 0.0;
   }
 }
 
@@ -931,6 +962,7 @@ __global__ void __launch_bounds__(kFuseThreads)
 template <class T>
 struct LevelVals {
   DBuf<T> val;   // ELL values of A (colour-sorted rows)
+  DBuf<T> vval;  // the same rows in vertex order (residuals and the CG product)
   DBuf<T> rval;  // ELL values of R (this level -> next coarser)
   DBuf<T> pval;  // ELL values of P (next coarser -> this level)
   DBuf<T> Z;     // dense coarse operator, column-major (coarsest level, direct solve)
@@ -948,7 +980,7 @@ struct LevelVals {
 struct Level {
   int n = 0, width = 0;
   DBuf<int32_t> col, rowid;  // ELL of A over internal numbers, rows sorted by (colour, vertex)
-  DBuf<int32_t> pos;         // ELL position of each internal vertex (inverse of rowid)
+  DBuf<int32_t> vcol;        // the same ELL with rows in vertex order (same slots)
   std::vector<int> cstart;  // colour ranges of positions
   int ncolors = 0;
   // restriction to the next coarser level (rows: coarse internal numbers)
@@ -998,9 +1030,10 @@ struct dec_mg_s {
   const void *gu[kGraphs] = {}, *gb[kGraphs] = {};
   int gkey[kGraphs] = {-1, -1, -1, -1};
   // preconditioned CG on the finest level: residual, preconditioned residual,
-  // direction (previous and current), A times the direction, the partial sums
-  // of r·z, p·q and r·r (kPcgBlocks each) and the two scalars
-  DBuf<double> pcg_r, pcg_z, pcg_pold, pcg_p, pcg_q, pcg_part, pcg_sc;
+  // direction, A times the direction, the partial sums
+  // of r·z, p·q and r·r (at most one per workgroup of the finest level's
+  // element launches) and the scalars (r·z)_prev, r·z, α
+  DBuf<double> pcg_r, pcg_z, pcg_p, pcg_q, pcg_part, pcg_sc;
   hipStream_t cap = nullptr;
   bool use_graph = true;
   // residual norms of a solve on their way to the host (pinned ring + events)
@@ -1226,7 +1259,23 @@ void build_level_matrix(Level &L, const dec::HostCsr &A, std::vector<int32_t> &c
   {
     std::vector<int32_t> where(n);
     for (int p = 0; p < n; p++) where[order[p]] = p;
-    L.pos.upload(where);
+    // rows in vertex order, for the kernels that sweep no colours: their
+    // vector accesses are then contiguous, where colour-sorted rows touch
+    // every cache line of a vector once per colour
+    std::vector<int32_t> vc(sz, -1);
+    std::vector<double> vvd(sz, 0.0);
+    std::vector<float> vvf(sz, 0.0f);
+    int w = std::max(width, 1);
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < w; k++) {
+        size_t a = static_cast<size_t>(k) * n + i, b = static_cast<size_t>(k) * n + where[i];
+        vc[a] = col[b];
+        vvd[a] = vd[b];
+        vvf[a] = vf[b];
+      }
+    L.vcol.upload(vc);
+    L.d.vval.upload(vvd);
+    L.f.vval.upload(vvf);
   }
   L.d.val.upload(vd);
   L.f.val.upload(vf);
@@ -1604,8 +1653,8 @@ template <class T>
 void residual(Level &L, const T *b, const T *u, T *r, hipStream_t s) {
   LevelVals<T> &V = vals<T>(L);
   DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_residual<W, T>), dim3(dec::xcd_grid(L.n)), dim3(kBlock), 0, s,
-                                                   L.n, L.width,
-                                                   L.col.p, V.val.p, L.rowid.p, b, u, r));
+                                                   L.n, L.width, L.vcol.p, V.vval.p, static_cast<const int *>(nullptr),
+                                                   b, u, r));
 }
 
 void dot(dec_mg_s &mg, int n, const double *x, const double *y, double *out, const int *done, hipStream_t s) {
@@ -1846,8 +1895,8 @@ void mu_cycle(dec_mg_s &mg, int lvl, T *u, const T *b, int mu, bool u_zero, hipS
   LevelVals<T> &CV = vals<T>(C);
   if (L.n <= kFusedRestrictMaxRows && L.rwidth <= 8) {
     DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_residual_restrict<W, T>), G(8LL * L.nc), dim3(kBlock), 0, s,
-                                                     L.nc, L.rwidth, L.rcol.p, V.rval.p, L.n, L.width, L.col.p,
-                                                     V.val.p, L.pos.p, b, u, CV.b.p, CV.u.p));
+                                                     L.nc, L.rwidth, L.rcol.p, V.rval.p, L.n, L.width, L.vcol.p,
+                                                     V.vval.p, b, u, CV.b.p, CV.u.p));
   } else {
     residual<T>(L, b, u, V.r.p, s);
     DEC_ELL_WIDTH_SWITCH(L.rwidth, hipLaunchKernelGGL((k_restrict<W, T>), G(L.nc), dim3(kBlock), 0, s, L.nc,
@@ -2009,7 +2058,8 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_call
     int nb = dec::nblocks(n, kBlock);
     if (mg.rpart.n < static_cast<size_t>(nb)) fail(DEC_ERR_ARG, "multigrid: residual partials");
     DEC_ELL_WIDTH_SWITCH(F.width, hipLaunchKernelGGL((k_residual_norm<W>), dim3(nb), dim3(kBlock), 0, s, n, F.width,
-                                                     F.col.p, F.d.val.p, F.rowid.p, b, u, F.d.r.p, mg.rpart.p));
+                                                     F.vcol.p, F.d.vval.p, static_cast<const int *>(nullptr), b, u,
+                                                     F.d.r.p, mg.rpart.p));
     if (!check) return;
     double *slot = mg.scal.p + (k % kNormSlots);
     hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(kBlock), 0, s, nb, mg.rpart.p, slot);
@@ -2080,26 +2130,26 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_call
 
 // One iteration of the preconditioned CG: z = one V-cycle on r from a zero
 // start (float32 cycles on the rounded residual for the mixed variant, z
-// their float64 value), then the three CG launches.
+// their float64 value), then the three CG launches.  The V-cycle's input (a
+// zero z, or the float32 r and iterate) was prepared by the previous
+// iteration's update launch or by the start.
 void pcg_iteration(dec_mg_s &mg, bool mixed, double *x, hipStream_t s) {
   Level &F = *mg.lv[0];
-  int n = F.n;
+  int n = F.n, nb = dec::nblocks(n, kBlock), ng = dec::xcd_grid(n);
+  // partial sums: r·z in [0, kPcgDotBlocks), then p·q (one per workgroup of
+  // the direction launch, ng) and r·r (one per workgroup of the update, nb)
   double *r = mg.pcg_r.p, *z = mg.pcg_z.p, *part = mg.pcg_part.p;
-  if (!mixed) {
-    hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, z);
-    mu_cycle<double>(mg, 0, z, r, 1, true, s);
-  } else {
-    hipLaunchKernelGGL(k_d2f, G(n), dim3(kBlock), 0, s, n, r, F.f.b.p);
-    hipLaunchKernelGGL(k_zero<float>, G(n), dim3(kBlock), 0, s, n, F.f.u.p);
-    mu_cycle<float>(mg, 0, F.f.u.p, F.f.b.p, 1, true, s);
-    hipLaunchKernelGGL(k_f2d, G(n), dim3(kBlock), 0, s, n, F.f.u.p, z);
-  }
-  hipLaunchKernelGGL(k_pcg_dot, dim3(kPcgBlocks), dim3(kBlock), 0, s, n, r, z, part);
-  DEC_ELL_WIDTH_SWITCH(F.width, hipLaunchKernelGGL((k_pcg_dir<W>), dim3(kPcgBlocks), dim3(kBlock), 0, s, n, F.width,
-                                                   F.col.p, F.d.val.p, F.rowid.p, z, mg.pcg_pold.p, mg.pcg_p.p,
-                                                   mg.pcg_q.p, part, mg.pcg_sc.p, part + kPcgBlocks));
-  hipLaunchKernelGGL(k_pcg_update, dim3(kPcgBlocks), dim3(kBlock), 0, s, n, x, r, mg.pcg_p.p, mg.pcg_q.p,
-                     mg.pcg_pold.p, part + kPcgBlocks, mg.pcg_sc.p, part + 2 * kPcgBlocks);
+  double *part_pq = part + kPcgDotBlocks, *part_rr = part_pq + ng;
+  if (!mixed) mu_cycle<double>(mg, 0, z, r, 1, true, s);
+  else mu_cycle<float>(mg, 0, F.f.u.p, F.f.b.p, 1, true, s);
+  hipLaunchKernelGGL(k_pcg_dot, dim3(kPcgDotBlocks), dim3(kBlock), 0, s, n, r, mixed ? F.f.u.p : nullptr, z, part);
+  DEC_ELL_WIDTH_SWITCH(F.width, hipLaunchKernelGGL((k_pcg_dir<W>), dim3(ng), dim3(kBlock), 0, s, n,
+                                                   F.width, F.vcol.p, F.d.vval.p, static_cast<const int *>(nullptr),
+                                                   z, mg.pcg_p.p,
+                                                   mg.pcg_q.p, part, kPcgDotBlocks, mg.pcg_sc.p, part_pq));
+  hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(kBlock), 0, s, part_pq, ng, mg.pcg_sc.p);
+  hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(kBlock), 0, s, n, x, r, mg.pcg_p.p, mg.pcg_q.p, mg.pcg_sc.p,
+                     part_rr, mixed ? nullptr : z, mixed ? F.f.b.p : nullptr, mixed ? F.f.u.p : nullptr);
 }
 
 // Solve with the preconditioned CG (variants DEC_SOLVE_PCG_F64 / _MIXED).
@@ -2120,16 +2170,16 @@ void pcg_solve(dec_mg_s &mg, bool mixed, double tol, int max_iters, double *u_ca
     *relres = 0.0;
     return;
   }
+  int nb = dec::nblocks(n, kBlock);
   if (mg.pcg_r.n < static_cast<size_t>(n)) {
-    for (auto *v : {&mg.pcg_r, &mg.pcg_z, &mg.pcg_pold, &mg.pcg_p, &mg.pcg_q}) v->alloc(n);
-    mg.pcg_part.alloc(3 * kPcgBlocks);
-    mg.pcg_sc.alloc(2);
+    for (auto *v : {&mg.pcg_r, &mg.pcg_z, &mg.pcg_p, &mg.pcg_q}) v->alloc(n);
+    mg.pcg_part.alloc(kPcgDotBlocks + static_cast<size_t>(dec::xcd_grid(n)) + nb);
+    mg.pcg_sc.alloc(3);
   }
   FineIO io = fine_in(mg, u_caller, b_caller, true, s);
   double *x = io.u;
   const double *b = io.b;
   mg.learned_active = false;
-  int nb = dec::nblocks(n, kBlock);
   if (mg.rpart.n < static_cast<size_t>(nb)) fail(DEC_ERR_ARG, "multigrid: residual partials");
   // a norm on its way to the host: the partial sums part[0..np) added into
   // ring slot k with an event
@@ -2147,12 +2197,15 @@ void pcg_solve(dec_mg_s &mg, bool mixed, double tol, int max_iters, double *u_ca
   // the true residual b - A x into F.d.r, its norm to slot k
   auto true_residual = [&](int k) {
     DEC_ELL_WIDTH_SWITCH(F.width, hipLaunchKernelGGL((k_residual_norm<W>), dim3(nb), dim3(kBlock), 0, s, n, F.width,
-                                                     F.col.p, F.d.val.p, F.rowid.p, b, x, F.d.r.p, mg.rpart.p));
+                                                     F.vcol.p, F.d.vval.p, static_cast<const int *>(nullptr), b, x,
+                                                     F.d.r.p, mg.rpart.p));
     send(k, mg.rpart.p, nb);
   };
+  // CG from the true residual in F.d.r
   auto restart = [&]() {
-    hipLaunchKernelGGL(k_copy<double>, G(n), dim3(kBlock), 0, s, n, F.d.r.p, mg.pcg_r.p);
-    hipLaunchKernelGGL(k_pcg_start, G(n), dim3(kBlock), 0, s, n, mg.pcg_pold.p, mg.pcg_sc.p);
+    hipLaunchKernelGGL(k_pcg_start, G(n), dim3(kBlock), 0, s, n, F.d.r.p, mg.pcg_r.p, mg.pcg_p.p, mg.pcg_q.p,
+                       mg.pcg_sc.p, mixed ? nullptr : mg.pcg_z.p, mixed ? F.f.b.p : nullptr,
+                       mixed ? F.f.u.p : nullptr);
   };
   int enq = 0;
   bool enq_checked = true;
@@ -2162,7 +2215,7 @@ void pcg_solve(dec_mg_s &mg, bool mixed, double tol, int max_iters, double *u_ca
       replay(mg, mixed ? 3 : 2, x, b, [&](hipStream_t cs) { pcg_iteration(mg, mixed, x, cs); }, s);
     else
       pcg_iteration(mg, mixed, x, s);
-    if (check) send(enq, mg.pcg_part.p + 2 * kPcgBlocks, kPcgBlocks);
+    if (check) send(enq, mg.pcg_part.p + kPcgDotBlocks + dec::xcd_grid(n), nb);
     enq_checked = check;
   };
   true_residual(0);
@@ -2193,7 +2246,7 @@ void pcg_solve(dec_mg_s &mg, bool mixed, double tol, int max_iters, double *u_ca
     if (enq == kc && !enq_checked) {
       // queued ahead without its norm; nothing after it is queued, so its
       // partial sums are still in place
-      send(kc, mg.pcg_part.p + 2 * kPcgBlocks, kPcgBlocks);
+      send(kc, mg.pcg_part.p + kPcgDotBlocks + dec::xcd_grid(n), nb);
       enq_checked = true;
     }
     if (kc + 1 <= max_iters && rr * std::pow(1e-3, kc + 1 - c) > tol) enqueue(false);
