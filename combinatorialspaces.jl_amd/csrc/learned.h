@@ -23,6 +23,9 @@ struct NnLevel {
   // of internal number i), for the smoother inside the V-cycles
   Op pi;
   DBuf<float> Di;
+  // perm on the device and pi's columns as vertex numbers (both empty when
+  // the numbering is vertex order)
+  DBuf<int32_t> perm_d, pi_vcol;
   NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr, const std::vector<int32_t> *perm);
 };
 

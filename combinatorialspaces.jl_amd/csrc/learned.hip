@@ -90,7 +90,17 @@ __global__ void k_scale_partial(int n, const float *__restrict__ r, double *__re
   __shared__ double sh[TB];
   const float *rb = r + static_cast<size_t>(blockIdx.y) * n;
   double acc = 0.0;
-  for (int i = blockIdx.x * TB + threadIdx.x; i < n; i += kScaleBlocks * TB) acc += static_cast<double>(rb[i]) * rb[i];
+  constexpr int stride = kScaleBlocks * TB;
+  int i = blockIdx.x * TB + threadIdx.x;
+  // eight loads in flight, added in the order of i
+  for (; i + 7 * stride < n; i += 8 * stride) {
+    float a[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) a[u] = rb[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc += static_cast<double>(a[u]) * a[u];
+  }
+  for (; i < n; i += stride) acc += static_cast<double>(rb[i]) * rb[i];
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int w = TB / 2; w > 0; w >>= 1) {
@@ -243,16 +253,18 @@ __global__ void k_output(int n, const float *__restrict__ x, const AT *__restric
 }
 
 // ---- inference: the smoother inside V-cycles -------------------------------
-// Three kernels for one residual (B = 1): x and P x; layer 1 recomputed for
-// every neighbour from (x, P x) and layer 2 on MFMA; layer 3 on MFMA with the
-// output head, adding c to u.  h1 and h3 never reach memory.  Each wavefront
-// takes 64 rows at a time, one row per lane for the gathers (64 rows of
-// loads in flight per wavefront), and passes the rows [h, P h] through LDS
-// into the A operands of v_mfma_f32_16x16x16bf16 (four 16-row tiles; lane
-// (li, lg) of tile q takes row 16q + li, K entries 4lg..4lg+3 of each K
-// step).  Expressions, summation orders and BF16 rounding points are those
-// of k_input / k_spmm / k_dense_bf16 / k_output, so that u gains exactly the
-// c of the BF16 forward.
+// Three kernels for one residual (B = 1) after the scale's partial sums.
+// Levels up to kInfSmallRows rows: x and P x; layer 1 recomputed for every
+// neighbour from (x, P x) with layer 2 on MFMA; layer 3 on MFMA with the
+// output head, adding c to u (16 rows per wavefront).  Larger levels: x, P x
+// and the layer-1 rows h1 (stored); layer 2 gathering h1 and storing h2;
+// layer 3 gathering h2 and adding c to u (64 rows per wavefront, four lanes
+// to a row for the gathers).  The rows [h, P h] pass through LDS into the A
+// operands of v_mfma_f32_16x16x16bf16 (four 16-row tiles; lane (li, lg) of
+// tile q takes row 16q + li, K entries 4lg..4lg+3 of each K step).
+// Expressions, summation orders and BF16 rounding points are those of
+// k_input / k_spmm / k_dense_bf16 / k_output, so that u gains exactly the c
+// of the BF16 forward.
 
 constexpr int kInfWaves = TB / 64;
 // levels up to this many rows use the 16-rows-per-wavefront layer kernels
@@ -380,8 +392,7 @@ __global__ void k_inf_input(int n, int pw, const int *__restrict__ pcol, const f
   }
 }
 
-// One row per lane: the row's own h (BF16) and m = P h (FP32 sums over the
-// neighbours' rows, ascending ELL order, rounded to BF16) into its LDS row.
+// m += pv * (the 32 BF16 values of a row)
 __device__ __forceinline__ void add_row(const uint4 *hc, float pv, float *m) {
 #pragma unroll
   for (int k4 = 0; k4 < 4; k4++) {
@@ -392,72 +403,117 @@ __device__ __forceinline__ void add_row(const uint4 *hc, float pv, float *m) {
   }
 }
 
+// uint4 of BF16 values, or zeros where the slot is a pad (c < 0): the
+// product pv * 0 is ±0, and adding ±0 leaves a sum unchanged (the sums start
+// at +0 and never become -0), so a pad adds nothing, without a branch
+__device__ __forceinline__ uint4 keep_if(uint4 a, bool keep) {
+  unsigned int mk = keep ? 0xffffffffu : 0u;
+  return make_uint4(a.x & mk, a.y & mk, a.z & mk, a.w & mk);
+}
+
+// k_inf_input for the large levels, with the layer-1 rows stored four lanes
+// to a row (a store instruction writes 16 whole rows).  A lane first forms
+// (x, P x) of its own row; lane l then computes features 8 (l & 3) .. +7 of
+// row 16 g + (l >> 2) for g = 0..3 (the same operations as k_inf_input).
+// rcol/rperm: r's index of each slot's column and of each row (r in another
+// numbering than the rows; NULL: pcol and the row itself).  Wavefronts run
+// independently after the scale; tiles are dealt to XCDs in contiguous ranges.
 template <int W>
-__device__ __forceinline__ void gather_rows(int n, int pw, int v, const int *__restrict__ pcol,
+__global__ void __launch_bounds__(TB) k_inf_input_c(int n, const int *__restrict__ pcol, const float *__restrict__ pval,
+                                                    const int *__restrict__ rcol, const int *__restrict__ rperm,
+                                                    const float *__restrict__ r, const double *__restrict__ part,
+                                                    float *__restrict__ s, float2 *__restrict__ xp,
+                                                    const float *__restrict__ th, unsigned short *__restrict__ h1) {
+  __shared__ float s_sh;
+  if (threadIdx.x == 0) {
+    double acc = 0.0;
+    for (int j = 0; j < kScaleBlocks; j++) acc += part[j];
+    double m = acc / n;
+    float sv = m > 0 ? static_cast<float>(sqrt(m)) : 1.0f;
+    s_sh = sv;
+    if (blockIdx.x == 0) s[0] = sv;
+  }
+  __syncthreads();
+  int lane = threadIdx.x & 63;
+  int t = dec_ell::xcd_block() * kInfWaves + (threadIdx.x >> 6);
+  if (t * 64 >= n) return;
+  int v = t * 64 + lane;
+  size_t vv = static_cast<size_t>(min(v, n - 1));
+  const int *cols = rcol ? rcol : pcol;
+  float inv = 1.0f / s_sh;
+  int rv = rperm ? rperm[vv] : static_cast<int>(vv);
+  int c[W];
+  float pv[W], rc[W];
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    c[k] = cols[k * static_cast<size_t>(n) + vv];
+    pv[k] = pval[k * static_cast<size_t>(n) + vv];
+  }
+  float rown = r[rv];
+#pragma unroll
+  for (int k = 0; k < W; k++) rc[k] = r[c[k] < 0 ? rv : c[k]];
+  float xv = rown * inv;
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    float tk = pv[k] * (rc[k] * inv);
+    acc = c[k] >= 0 ? acc + tk : acc;
+  }
+  if (v < n) xp[v] = make_float2(xv, acc);
+  const int sub = lane >> 2, q = lane & 3;
+  float w0[8], w1[8], b1[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    w0[e] = th[OW1 + 8 * q + e];
+    w1[e] = th[OW1 + H + 8 * q + e];
+    b1[e] = th[OB1 + 8 * q + e];
+  }
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    int src = 16 * g + sub;
+    float xg = __shfl(xv, src), ag = __shfl(acc, src);
+    float hh[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      float z = xg * w0[e] + ag * w1[e] + b1[e];
+      hh[e] = z > 0.0f ? z : 0.0f;
+    }
+    int row = t * 64 + src;
+    if (row < n) *reinterpret_cast<uint4 *>(h1 + static_cast<size_t>(row) * H + 8 * q) = pack8(hh);
+  }
+}
+
+// One row per lane, for rows of any width: the row's own h (BF16) and
+// m = P h (FP32 sums over the neighbours' rows, ascending ELL order, rounded
+// to BF16) into its LDS row; rows past the end (live false) get a zero row.
+__device__ __forceinline__ void gather_rows(int n, int pw, int v, bool live, const int *__restrict__ pcol,
                                             const float *__restrict__ pval, const unsigned short *__restrict__ h,
                                             unsigned short *my) {
-  const uint4 *hv = reinterpret_cast<const uint4 *>(h + static_cast<size_t>(v) * H);
   uint4 *dst = reinterpret_cast<uint4 *>(my);
+  if (!live) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) dst[k] = make_uint4(0u, 0u, 0u, 0u);
+    return;
+  }
+  const uint4 *hv = reinterpret_cast<const uint4 *>(h + static_cast<size_t>(v) * H);
 #pragma unroll
   for (int k = 0; k < 4; k++) dst[k] = hv[k];
   float m[H];
 #pragma unroll
   for (int j = 0; j < H; j++) m[j] = 0.0f;
-  if constexpr (W > 0) {
-    // all slots' columns and weights, then the neighbours' rows two at a time
-    int c[W];
-    float pv[W];
-#pragma unroll
-    for (int k = 0; k < W; k++) {
-      size_t qq = static_cast<size_t>(k) * n + v;
-      c[k] = pcol[qq];
-      pv[k] = pval[qq];
-    }
-#pragma unroll
-    for (int k = 0; k < W; k += 2) {
-      uint4 a[4], b[4];
-      const uint4 *ha = reinterpret_cast<const uint4 *>(h + static_cast<size_t>(c[k] < 0 ? v : c[k]) * H);
-#pragma unroll
-      for (int k4 = 0; k4 < 4; k4++) a[k4] = ha[k4];
-      if (k + 1 < W) {
-        const uint4 *hb = reinterpret_cast<const uint4 *>(h + static_cast<size_t>(c[k + 1] < 0 ? v : c[k + 1]) * H);
-#pragma unroll
-        for (int k4 = 0; k4 < 4; k4++) b[k4] = hb[k4];
-      }
-      if (c[k] >= 0) add_row(a, pv[k], m);
-      if (k + 1 < W && c[k + 1] >= 0) add_row(b, pv[k + 1], m);
-    }
-#pragma unroll
-    for (int k4 = 0; k4 < 4; k4++) dst[4 + k4] = pack8(m + 8 * k4);
-    return;
-  }
   for (int k = 0; k < pw; k++) {
     size_t qq = static_cast<size_t>(k) * n + v;
     int c = pcol[qq];
     if (c < 0) break;
-    float pv = pval[qq];
-    const uint4 *hc = reinterpret_cast<const uint4 *>(h + static_cast<size_t>(c) * H);
-#pragma unroll
-    for (int k4 = 0; k4 < 4; k4++) {
-      float f[8];
-      unpack8(hc[k4], f);
-#pragma unroll
-      for (int e = 0; e < 8; e++) m[8 * k4 + e] = m[8 * k4 + e] + pv * f[e];
-    }
+    add_row(reinterpret_cast<const uint4 *>(h + static_cast<size_t>(c) * H), pval[qq], m);
   }
 #pragma unroll
   for (int k4 = 0; k4 < 4; k4++) dst[4 + k4] = pack8(m + 8 * k4);
 }
 
-__device__ __forceinline__ void zero_row(unsigned short *my) {
-  uint4 *dst = reinterpret_cast<uint4 *>(my);
-#pragma unroll
-  for (int k = 0; k < 8; k++) dst[k] = make_uint4(0u, 0u, 0u, 0u);
-}
-
-// h2 = relu([h1, P h1] W2 + b2) from the stored layer-1 rows: each lane
-// gathers its row's neighbours' h1 (the sums of k_spmm, in slot order).
-template <int W>
+// h2 = relu([h1, P h1] W2 + b2) from the stored layer-1 rows, for rows wider
+// than the wave-granular kernels take: each lane gathers its row's
+// neighbours' h1 (the sums of k_spmm, in slot order).
 __global__ void __launch_bounds__(TB) k_inf_layer2s(int n, int pw, const int *__restrict__ pcol,
                                                     const float *__restrict__ pval,
                                                     const unsigned short *__restrict__ h1,
@@ -475,8 +531,7 @@ __global__ void __launch_bounds__(TB) k_inf_layer2s(int n, int pw, const int *__
     int v = t * 64 + lane;
     bool live = t < nsteps && v < n;
     unsigned short *my = rows + lane * kRowPitch;
-    if (live) gather_rows<W>(n, pw, v, pcol, pval, h1, my);
-    else zero_row(my);
+    gather_rows(n, pw, v, live, pcol, pval, h1, my);
     __syncthreads();
     float4_t acc[4][2];
     mfma_rows(rows, li, lg, Bw, acc);
@@ -502,8 +557,7 @@ __global__ void __launch_bounds__(TB) k_inf_layer2s(int n, int pw, const int *__
 }
 
 // h3 = relu([h2, P h2] W3 + b3); y = alpha x + h3 wo + bo (ascending sum over
-// the 32 features); u += (s / D) y.
-template <int W>
+// the 32 features); u += (s / D) y.  Rows of any width, one per lane.
 __global__ void __launch_bounds__(TB) k_inf_layer3(int n, int pw, const int *__restrict__ pcol,
                                                    const float *__restrict__ pval, const unsigned short *__restrict__ h2,
                                                    const float2 *__restrict__ xo, const float *__restrict__ th,
@@ -530,10 +584,8 @@ __global__ void __launch_bounds__(TB) k_inf_layer3(int n, int pw, const int *__r
       xv = xo[v].x;
       dv = D[v];
       uv = u[v];
-      gather_rows<W>(n, pw, v, pcol, pval, h2, my);
-    } else {
-      zero_row(my);
     }
+    gather_rows(n, pw, v, live, pcol, pval, h2, my);
     __syncthreads();
     float4_t acc[4][2];
     mfma_rows(rows, li, lg, Bw, acc);
@@ -559,120 +611,182 @@ __global__ void __launch_bounds__(TB) k_inf_layer3(int n, int pw, const int *__r
   }
 }
 
-// Layer 2 with the row gathers done four lanes to a row.  A lane-per-row
-// gather of 64-byte rows makes every load instruction touch 64 cache lines
-// (16 bytes of each); here the wavefront's 64 rows are taken 16 at a time,
-// lane l serving row 16g + (l >> 2) and its features 8 (l & 3) .. 8 (l & 3) + 7,
-// so one instruction reads 16 whole rows.  A row's columns and weights are
-// loaded by the lane that owns the row (lane l owns row l) and passed on by
-// cross-lane reads.  Each feature's sum still runs over the slots in
-// ascending order with the same operations, so the results are
-// k_inf_layer2s' bit for bit.
-
 // The wavefront's rows t*64 .. t*64+63 into its LDS rows: the own row h into
-// columns 0..31 and m = P h (BF16 of the FP32 sums) into columns 32..63.
-// c/pv: this lane's own row's slots (c = -1 pads, and every slot of a row
-// past the end).
+// columns 0..31 and m = P h (BF16 of the FP32 sums) into columns 32..63, four
+// lanes to a row.  A lane-per-row gather of 64-byte rows makes every load
+// instruction touch 64 cache lines (16 bytes of each); here the rows are
+// taken 16 at a time, lane l serving row 16g + (l >> 2) and its features
+// 8 (l & 3) .. 8 (l & 3) + 7, so one instruction reads 16 whole rows.  A
+// group's slots are gathered together (W loads in flight) and summed into 8
+// registers.  A row's columns and weights are loaded by the lane that owns
+// the row (lane l owns row l; c = -1 pads, and every slot of a row past the
+// end) and passed on by cross-lane reads.  Each feature's sum runs over the
+// slots in ascending order with the operations of k_spmm.
 template <int W>
-__device__ __forceinline__ void coop_rows(int n, int t, int lane, const int (&c)[W], const float (&pv)[W],
-                                          const unsigned short *__restrict__ h, unsigned short *rows) {
+__device__ __forceinline__ void coop_rows_g(int n, int t, int lane, const int (&c)[W], const float (&pv)[W],
+                                            const unsigned short *__restrict__ h, unsigned short *rows) {
   const int sub = lane >> 2, part = lane & 3;
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     int r = t * 64 + 16 * g + sub;
-    uint4 own = make_uint4(0u, 0u, 0u, 0u);
-    if (r < n) own = *reinterpret_cast<const uint4 *>(h + static_cast<size_t>(r) * H + 8 * part);
-    *reinterpret_cast<uint4 *>(rows + (16 * g + sub) * kRowPitch + 8 * part) = own;
-  }
-  float m[4][8];
+    int cg[W];
+    float vg[W];
 #pragma unroll
-  for (int g = 0; g < 4; g++)
-#pragma unroll
-    for (int e = 0; e < 8; e++) m[g][e] = 0.0f;
-#pragma unroll
-  for (int k = 0; k < W; k++) {
-    int cg[4];
-    float vg[4];
-#pragma unroll
-    for (int g = 0; g < 4; g++) {
-      cg[g] = __shfl(c[k], 16 * g + sub);
-      vg[g] = __shfl(pv[k], 16 * g + sub);
+    for (int k = 0; k < W; k++) {
+      cg[k] = __shfl(c[k], 16 * g + sub);
+      vg[k] = __shfl(pv[k], 16 * g + sub);
     }
-    uint4 a[4];
+    uint4 own = *reinterpret_cast<const uint4 *>(h + static_cast<size_t>(min(r, n - 1)) * H + 8 * part);
+    uint4 a[W];
 #pragma unroll
-    for (int g = 0; g < 4; g++)
-      a[g] = *reinterpret_cast<const uint4 *>(h + static_cast<size_t>(cg[g] < 0 ? 0 : cg[g]) * H + 8 * part);
+    for (int k = 0; k < W; k++)
+      a[k] = keep_if(*reinterpret_cast<const uint4 *>(h + static_cast<size_t>(cg[k] < 0 ? 0 : cg[k]) * H + 8 * part),
+                     cg[k] >= 0);
+    float m[8];
 #pragma unroll
-    for (int g = 0; g < 4; g++) {
-      if (cg[g] < 0) continue;
+    for (int e = 0; e < 8; e++) m[e] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
       float f[8];
-      unpack8(a[g], f);
+      unpack8(a[k], f);
 #pragma unroll
-      for (int e = 0; e < 8; e++) m[g][e] = m[g][e] + vg[g] * f[e];
+      for (int e = 0; e < 8; e++) m[e] = m[e] + vg[k] * f[e];
     }
+    *reinterpret_cast<uint4 *>(rows + (16 * g + sub) * kRowPitch + 8 * part) = keep_if(own, r < n);
+    *reinterpret_cast<uint4 *>(rows + (16 * g + sub) * kRowPitch + H + 8 * part) = pack8(m);
   }
-#pragma unroll
-  for (int g = 0; g < 4; g++)
-    *reinterpret_cast<uint4 *>(rows + (16 * g + sub) * kRowPitch + H + 8 * part) = pack8(m[g]);
 }
 
 // this lane's row's slots (v = its row; rows past the end get only pads)
 template <int W>
 __device__ __forceinline__ void own_slots(int n, int v, const int *__restrict__ pcol, const float *__restrict__ pval,
                                           int (&c)[W], float (&pv)[W]) {
+  size_t vv = static_cast<size_t>(min(v, n - 1));
 #pragma unroll
   for (int k = 0; k < W; k++) {
-    c[k] = -1;
-    pv[k] = 0.0f;
-    if (v < n) {
-      size_t q = static_cast<size_t>(k) * n + v;
-      c[k] = pcol[q];
-      pv[k] = pval[q];
-    }
+    int cq = pcol[k * static_cast<size_t>(n) + vv];
+    c[k] = v < n ? cq : -1;
+    pv[k] = pval[k * static_cast<size_t>(n) + vv];
+  }
+}
+
+// Wave-granular forms of the two layers: one 64-row tile per one-wavefront
+// workgroup.  The LDS rows are the wavefront's own, so the steps are ordered
+// by a wavefront-scope fence (LDS operations of one wavefront complete in
+// order) instead of a workgroup barrier, and each 16-row MFMA tile's output
+// goes to LDS as soon as it is formed.  Tiles are dealt to XCDs in
+// contiguous ranges.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// acc[jt] = rows 16q .. 16q+15 of the tile times W (64 x 32)
+__device__ __forceinline__ void mfma_tile(const unsigned short *rows, int q, int li, int lg, const short4_t Bw[2][4],
+                                          float4_t acc[2]) {
+  const unsigned short *r = rows + (16 * q + li) * kRowPitch + 4 * lg;
+  short4_t A0 = lds4(r), A1 = lds4(r + 16), A2 = lds4(r + 32), A3 = lds4(r + 48);
+#pragma unroll
+  for (int jt = 0; jt < 2; jt++) {
+    float4_t c = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A0, Bw[jt][0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1, Bw[jt][1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A2, Bw[jt][2], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A3, Bw[jt][3], c, 0, 0, 0);
+    acc[jt] = c;
   }
 }
 
 template <int W>
-__global__ void __launch_bounds__(TB) k_inf_layer2c(int n, const int *__restrict__ pcol, const float *__restrict__ pval,
+__global__ void __launch_bounds__(64) k_inf_layer2w(int n, const int *__restrict__ pcol, const float *__restrict__ pval,
                                                     const unsigned short *__restrict__ h1,
                                                     const float *__restrict__ th, unsigned short *__restrict__ h2) {
-  __shared__ __align__(16) unsigned short rows_s[kInfWaves][64 * kRowPitch];
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __shared__ __align__(16) unsigned short rows[64 * kRowPitch];
+  int lane = threadIdx.x;
   int li = lane & 15, lg = lane >> 4;
   const int sub = lane >> 2, part = lane & 3;
-  unsigned short *rows = rows_s[w];
+  int t = dec_ell::xcd_block();
+  if (t >= (n + 63) / 64) return;
   short4_t Bw[2][4];
   load_wB(th + OW2, li, lg, Bw);
   float bias0 = th[OB2 + li], bias1 = th[OB2 + 16 + li];
-  int nsteps = (n + 63) / 64;
-  for (int base = blockIdx.x * kInfWaves; base < nsteps; base += gridDim.x * kInfWaves) {
-    int t = base + w;
-    int c[W];
-    float pv[W];
-    own_slots<W>(n, t * 64 + lane, pcol, pval, c, pv);
-    coop_rows<W>(n, t, lane, c, pv, h1, rows);
-    __syncthreads();
-    float4_t acc[4][2];
-    mfma_rows(rows, li, lg, Bw, acc);
-    __syncthreads();
+  int c[W];
+  float pv[W];
+  own_slots<W>(n, t * 64 + lane, pcol, pval, c, pv);
+  coop_rows_g<W>(n, t, lane, c, pv, h1, rows);
+  wave_sync();
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+  for (int q = 0; q < 4; q++) {
+    float4_t acc[2];
+    mfma_tile(rows, q, li, lg, Bw, acc);
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        unsigned short *o = rows + (16 * q + 4 * lg + i) * kRowPitch;
-        float z0 = acc[q][0][i] + bias0, z1 = acc[q][1][i] + bias1;
-        o[li] = f2bf(z0 > 0.0f ? z0 : 0.0f);
-        o[16 + li] = f2bf(z1 > 0.0f ? z1 : 0.0f);
-      }
-    __syncthreads();
-#pragma unroll
-    for (int g = 0; g < 4; g++) {
-      int r = t * 64 + 16 * g + sub;
-      if (r < n)
-        *reinterpret_cast<uint4 *>(h2 + static_cast<size_t>(r) * H + 8 * part) =
-            *reinterpret_cast<const uint4 *>(rows + (16 * g + sub) * kRowPitch + 8 * part);
+    for (int i = 0; i < 4; i++) {
+      unsigned short *o = rows + (16 * q + 4 * lg + i) * kRowPitch;
+      float z0 = acc[0][i] + bias0, z1 = acc[1][i] + bias1;
+      o[li] = f2bf(z0 > 0.0f ? z0 : 0.0f);
+      o[16 + li] = f2bf(z1 > 0.0f ? z1 : 0.0f);
     }
-    __syncthreads();
+  }
+  wave_sync();
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    int r = t * 64 + 16 * g + sub;
+    if (r < n)
+      *reinterpret_cast<uint4 *>(h2 + static_cast<size_t>(r) * H + 8 * part) =
+          *reinterpret_cast<const uint4 *>(rows + (16 * g + sub) * kRowPitch + 8 * part);
+  }
+}
+
+// prod (64 x (H + 1) floats) shares the LDS of rows: tile q's products cover
+// bytes below the start of tile q + 1's rows, and are written after tile q's
+// rows were read.
+template <int W>
+__global__ void __launch_bounds__(64) k_inf_layer3w(int n, const int *__restrict__ pcol, const float *__restrict__ pval,
+                                                    const unsigned short *__restrict__ h2,
+                                                    const float2 *__restrict__ xo, const float *__restrict__ th,
+                                                    const float *__restrict__ s, const float *__restrict__ D,
+                                                    const int *__restrict__ uperm, float *__restrict__ u) {
+  __shared__ __align__(16) unsigned short rows[64 * kRowPitch];
+  static_assert(16 * (H + 1) * 4 <= 16 * kRowPitch * 2, "products of a tile overlap the next tile's rows");
+  int lane = threadIdx.x;
+  int li = lane & 15, lg = lane >> 4;
+  float *prod = reinterpret_cast<float *>(rows);
+  int t = dec_ell::xcd_block();
+  if (t >= (n + 63) / 64) return;
+  short4_t Bw[2][4];
+  load_wB(th + OW3, li, lg, Bw);
+  float bias0 = th[OB3 + li], bias1 = th[OB3 + 16 + li];
+  float wo0 = th[OWO + li], wo1 = th[OWO + 16 + li];
+  float alpha = th[OAL], bo = th[OBO], sc = s[0];
+  int v = t * 64 + lane;
+  int vv = min(v, n - 1);
+  int c[W];
+  float pv[W];
+  own_slots<W>(n, v, pcol, pval, c, pv);
+  int ui = uperm ? uperm[vv] : vv;
+  float xv = xo[vv].x, dv = D[vv], uv = u[ui];
+  coop_rows_g<W>(n, t, lane, c, pv, h2, rows);
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    float4_t acc[2];
+    mfma_tile(rows, q, li, lg, Bw, acc);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      float *o = prod + (16 * q + 4 * lg + i) * (H + 1);
+      float z0 = acc[0][i] + bias0, z1 = acc[1][i] + bias1;
+      o[li] = bf2f(f2bf(z0 > 0.0f ? z0 : 0.0f)) * wo0;
+      o[16 + li] = bf2f(f2bf(z1 > 0.0f ? z1 : 0.0f)) * wo1;
+    }
+  }
+  wave_sync();
+  if (v < n) {
+    const float *o = prod + lane * (H + 1);
+    float hw = 0.0f;
+    for (int j = 0; j < H; j++) hw = hw + o[j];
+    float yy = alpha * xv + hw + bo;
+    u[ui] = uv + (sc / dv) * yy;
   }
 }
 
@@ -1277,7 +1391,7 @@ NnLevel::NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr, const st
     // in P's order, so each row's sum is the vertex-order row's
     std::vector<int32_t> iperm(n);
     for (int i = 0; i < n; i++) iperm[perm ? (*perm)[i] : i] = i;
-    HostCsr Q;
+    HostCsr Q, Qv;
     Q.m = Q.n = n;
     Q.ptr.assign(n + 1, 0);
     std::vector<float> di(n);
@@ -1292,6 +1406,15 @@ NnLevel::NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr, const st
     }
     to_ell(Q, pi);
     Di.upload(di);
+    if (perm) {
+      perm_d.upload(*perm);
+      Qv = Q;
+      for (int i = 0; i < n; i++)
+        for (int k = Q.ptr[i]; k < Q.ptr[i + 1]; k++) Qv.col[k] = (*perm)[Q.col[k]];
+      Op tmp;
+      to_ell(Qv, tmp);
+      pi_vcol = std::move(tmp.col);
+    }
   }
   to_ell(transpose(P), pt);
   if (R && Pr) {
@@ -1507,21 +1630,41 @@ void nn_bind(dec_nn_s &nn, const std::vector<int> &sizes) {
 
 void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s, bool internal) {
   const NnLevel &NL = level_of(nn, lvl);
+  int n = NL.n;
+  reserve_inference(nn, n);
+  // the scale's partial sums (over r in the caller's order); the next kernel
+  // adds them (a last-workgroup ticket would need a device-scope fence per
+  // workgroup, which writes back the XCD's L2)
+  hipLaunchKernelGGL(k_scale_partial, dim3(kScaleBlocks, 1), dim3(TB), 0, s, n, r, nn.spart.p);
+  if (n > kInfSmallRows && NL.pi.width >= 1 && NL.pi.width <= 8) {
+    // Large levels run over the multigrid level's own numbering, whose BFS
+    // blocks keep a row's neighbours close together (the vertex order of a
+    // subdivided mesh puts a third of them a level's size away).  A caller in
+    // vertex order has r read and u updated through perm; every row's sums
+    // are those of vertex order (same slots, same order).
+    bool vorder = !internal && NL.perm_d.n;
+    const int *rcol = vorder ? NL.pi_vcol.p : nullptr;
+    const int *rperm = vorder ? NL.perm_d.p : nullptr;
+    int tiles = (n + 63) / 64;
+    dim3 tg((tiles + 7) / 8 * 8);
+    DEC_ELL_WIDTH_SWITCH(NL.pi.width, if constexpr (W > 0) {
+      hipLaunchKernelGGL((k_inf_input_c<W>), dim3(dec::xcd_grid(n, TB)), dim3(TB), 0, s, n, NL.pi.col.p,
+                         NL.pi.val.p, rcol, rperm, r, nn.spart.p, nn.s.p, nn.ix.p, nn.th.p, nn.ih1.p);
+      hipLaunchKernelGGL((k_inf_layer2w<W>), tg, dim3(64), 0, s, n, NL.pi.col.p, NL.pi.val.p, nn.ih1.p, nn.th.p,
+                         nn.ih2.p);
+      hipLaunchKernelGGL((k_inf_layer3w<W>), tg, dim3(64), 0, s, n, NL.pi.col.p, NL.pi.val.p, nn.ih2.p, nn.ix.p,
+                         nn.th.p, nn.s.p, NL.Di.p, rperm, u);
+    });
+    return;
+  }
   struct {
     const NnLevel::Op &p;
     const DBuf<float> &D;
   } L{internal ? NL.pi : NL.p, internal ? NL.Di : NL.D};
-  int n = NL.n;
-  reserve_inference(nn, n);
-  // the scale's partial sums; the next kernel adds them (a last-workgroup
-  // ticket would need a device-scope fence per workgroup, which writes back
-  // the XCD's L2)
-  hipLaunchKernelGGL(k_scale_partial, dim3(kScaleBlocks, 1), dim3(TB), 0, s, n, r, nn.spart.p);
   bool big = n > kInfSmallRows;
-  unsigned short *h1 = big ? nn.ih1.p : nullptr;
   DEC_ELL_WIDTH_SWITCH(L.p.width, hipLaunchKernelGGL((k_inf_input<W>), dim3(dec::nblocks(n, TB)), dim3(TB), 0, s, n,
                                                       L.p.width, L.p.col.p, L.p.val.p, r, nn.spart.p, nn.s.p,
-                                                      nn.ix.p, nn.th.p, h1));
+                                                      nn.ix.p, nn.th.p, big ? nn.ih1.p : nullptr));
   if (!big) {
     int tiles = (n + 15) / 16;
     int blocks = std::max(1, (tiles + kInfWaves - 1) / kInfWaves);
@@ -1530,22 +1673,13 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s, b
     hipLaunchKernelGGL(k_inf_layer3_t, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ih2.p,
                        nn.ix.p, nn.th.p, nn.s.p, L.D.p, u);
   } else {
+    // rows wider than 8 slots: one lane per row
     int steps = (n + 63) / 64;
     int blocks = std::max(1, (steps + kInfWaves - 1) / kInfWaves);
-    // layer 2 gathers four lanes to a row where the rows' width is a
-    // template constant (measured 11.9 against 15.1 µs at 163,842 rows, equal
-    // at 655,362); layer 3 keeps a row per lane (32.8 against 39.6 µs at
-    // 655,362 rows with four lanes to a row)
-    DEC_ELL_WIDTH_SWITCH(L.p.width, {
-      if constexpr (W > 0)
-        hipLaunchKernelGGL((k_inf_layer2c<W>), dim3(blocks), dim3(TB), 0, s, n, L.p.col.p, L.p.val.p, nn.ih1.p,
-                           nn.th.p, nn.ih2.p);
-      else
-        hipLaunchKernelGGL((k_inf_layer2s<W>), dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p,
-                           nn.ih1.p, nn.th.p, nn.ih2.p);
-      hipLaunchKernelGGL((k_inf_layer3<W>), dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p,
-                         nn.ih2.p, nn.ix.p, nn.th.p, nn.s.p, L.D.p, u);
-    });
+    hipLaunchKernelGGL(k_inf_layer2s, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ih1.p,
+                       nn.th.p, nn.ih2.p);
+    hipLaunchKernelGGL(k_inf_layer3, dim3(blocks), dim3(TB), 0, s, n, L.p.width, L.p.col.p, L.p.val.p, nn.ih2.p,
+                       nn.ix.p, nn.th.p, nn.s.p, L.D.p, u);
   }
 }
 
