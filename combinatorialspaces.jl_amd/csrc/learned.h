@@ -26,6 +26,15 @@ struct NnLevel {
   // perm on the device and pi's columns as vertex numbers (both empty when
   // the numbering is vertex order)
   DBuf<int32_t> perm_d, pi_vcol;
+  // Halo lists of pi's 64-row tiles (large levels): tile t's rows gather
+  // from its own rows and from the rows hrow[hoff[t] .. hoff[t+1]) (hrow_v:
+  // their vertex numbers, with perm); lidx (slot-major like pi) holds each
+  // slot's column as a tile-local index, 0..63 the tile's own rows, 64 + j
+  // its j-th halo row, 255 a pad.  hmax: the largest halo, -1 when the lists
+  // were not built (a halo above 191 rows, or rows wider than 8 slots).
+  DBuf<uint8_t> lidx;
+  DBuf<int32_t> hoff, hrow, hrow_v;
+  int hmax = -1;
   NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr, const std::vector<int32_t> *perm);
 };
 

@@ -91,16 +91,16 @@ __global__ void k_scale_partial(int n, const float *__restrict__ r, double *__re
   const float *rb = r + static_cast<size_t>(blockIdx.y) * n;
   double acc = 0.0;
   constexpr int stride = kScaleBlocks * TB;
-  int i = blockIdx.x * TB + threadIdx.x;
-  // eight loads in flight, added in the order of i
-  for (; i + 7 * stride < n; i += 8 * stride) {
-    float a[8];
+  // 32 loads in flight (one round trip up to 1,048,576 rows), added in the
+  // order of i; past the end a clamped load adds +0, which leaves the sum of
+  // squares unchanged
+  for (int i = blockIdx.x * TB + threadIdx.x; i < n; i += 32 * stride) {
+    float a[32];
 #pragma unroll
-    for (int u = 0; u < 8; u++) a[u] = rb[i + u * stride];
+    for (int u = 0; u < 32; u++) a[u] = rb[min(i + u * stride, n - 1)];
 #pragma unroll
-    for (int u = 0; u < 8; u++) acc += static_cast<double>(a[u]) * a[u];
+    for (int u = 0; u < 32; u++) acc += i + u * stride < n ? static_cast<double>(a[u]) * a[u] : 0.0;
   }
-  for (; i < n; i += stride) acc += static_cast<double>(rb[i]) * rb[i];
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int w = TB / 2; w > 0; w >>= 1) {
@@ -270,6 +270,9 @@ constexpr int kInfWaves = TB / 64;
 // levels up to this many rows use the 16-rows-per-wavefront layer kernels
 constexpr int kInfSmallRows = 50000;
 constexpr int kRowPitch = 2 * H + 8;  // bf16 per LDS row: [h | P h] plus padding
+// tile-local indices of the halo kernels are bytes: 64 own rows, at most
+// kHaloMax halo rows, 255 for a pad
+constexpr size_t kHaloMax = 191;
 
 __device__ __forceinline__ void load_wB(const float *__restrict__ W, int li, int lg, short4_t B[2][4]) {
 #pragma unroll
@@ -738,13 +741,17 @@ __global__ void __launch_bounds__(64) k_inf_layer2w(int n, const int *__restrict
   }
 }
 
+// x of (x, P x) or of x alone
+__device__ __forceinline__ float x_of(float2 a) { return a.x; }
+__device__ __forceinline__ float x_of(float a) { return a; }
+
 // prod (64 x (H + 1) floats) shares the LDS of rows: tile q's products cover
 // bytes below the start of tile q + 1's rows, and are written after tile q's
 // rows were read.
-template <int W>
+template <int W, class XT>
 __global__ void __launch_bounds__(64) k_inf_layer3w(int n, const int *__restrict__ pcol, const float *__restrict__ pval,
                                                     const unsigned short *__restrict__ h2,
-                                                    const float2 *__restrict__ xo, const float *__restrict__ th,
+                                                    const XT *__restrict__ xo, const float *__restrict__ th,
                                                     const float *__restrict__ s, const float *__restrict__ D,
                                                     const int *__restrict__ uperm, float *__restrict__ u) {
   __shared__ __align__(16) unsigned short rows[64 * kRowPitch];
@@ -765,7 +772,7 @@ __global__ void __launch_bounds__(64) k_inf_layer3w(int n, const int *__restrict
   float pv[W];
   own_slots<W>(n, v, pcol, pval, c, pv);
   int ui = uperm ? uperm[vv] : vv;
-  float xv = xo[vv].x, dv = D[vv], uv = u[ui];
+  float xv = x_of(xo[vv]), dv = D[vv], uv = u[ui];
   coop_rows_g<W>(n, t, lane, c, pv, h2, rows);
   wave_sync();
 #pragma unroll
@@ -787,6 +794,106 @@ __global__ void __launch_bounds__(64) k_inf_layer3w(int n, const int *__restrict
     for (int j = 0; j < H; j++) hw = hw + o[j];
     float yy = alpha * xv + hw + bo;
     u[ui] = uv + (sc / dv) * yy;
+  }
+}
+
+// ---- layer 1 of large levels with a staged halo -----------------------------
+// Each 64-row tile (one wavefront) copies r of its own rows and of its halo
+// rows (hrow, at most kHaloMax) into LDS and reads every neighbour's r from
+// there; a slot costs 5 bytes (its weight and a one-byte tile-local index)
+// instead of 8.  The sums and their order are those of k_inf_input_c.
+
+// The halo's values of x into LDS: at most kHaloMax entries, 64 a time.
+// Lanes past the end load and store the last entry again (the same value to
+// the same place), so no load or store sits under a branch: a load under a
+// branch makes the wavefront wait for it before going on.
+__device__ __forceinline__ void stage_halo_f(int lane, int h0, int nh, const int *__restrict__ hrow,
+                                             const float *__restrict__ x, float *dst) {
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    int i = min(lane + 64 * j, max(nh - 1, 0));
+    dst[i] = x[hrow[h0 + i]];
+  }
+}
+
+// this lane's row's slots as tile-local indices (255: pad, and every slot of
+// a row past the end) and weights
+template <int W>
+__device__ __forceinline__ void own_lslots(int n, int v, const uint8_t *__restrict__ lidx,
+                                           const float *__restrict__ pval, int (&l)[W], float (&pv)[W]) {
+  size_t vv = static_cast<size_t>(min(v, n - 1));
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    int lq = lidx[k * static_cast<size_t>(n) + vv];
+    l[k] = v < n ? lq : 255;
+    pv[k] = pval[k * static_cast<size_t>(n) + vv];
+  }
+}
+
+// x = r / s and the layer-1 rows h1 (the values of k_inf_input_c), reading r
+// of the tile's own rows and halo from LDS.  rperm / hrow_r: r's index of
+// each own row (NULL: the row itself) and of each halo row.
+template <int W>
+__global__ void __launch_bounds__(TB) k_inf_input_h(int n, const uint8_t *__restrict__ lidx,
+                                                    const float *__restrict__ pval, const int *__restrict__ hoff,
+                                                    const int *__restrict__ hrow_r, const int *__restrict__ rperm,
+                                                    const float *__restrict__ r, const double *__restrict__ part,
+                                                    float *__restrict__ s, float *__restrict__ xo,
+                                                    const float *__restrict__ th, unsigned short *__restrict__ h1) {
+  __shared__ float s_sh;
+  __shared__ float rs_s[kInfWaves][64 + kHaloMax];
+  if (threadIdx.x == 0) {
+    double acc = 0.0;
+    for (int j = 0; j < kScaleBlocks; j++) acc += part[j];
+    double m = acc / n;
+    float sv = m > 0 ? static_cast<float>(sqrt(m)) : 1.0f;
+    s_sh = sv;
+    if (blockIdx.x == 0) s[0] = sv;
+  }
+  __syncthreads();
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int t = dec_ell::xcd_block() * kInfWaves + w;
+  if (t * 64 >= n) return;
+  float *rs = rs_s[w];
+  int v = t * 64 + lane;
+  int vv = min(v, n - 1);
+  int h0 = hoff[t], nh = hoff[t + 1] - h0;
+  float rown = r[rperm ? rperm[vv] : vv];
+  int l[W];
+  float pv[W];
+  own_lslots<W>(n, v, lidx, pval, l, pv);
+  stage_halo_f(lane, h0, nh, hrow_r, r, rs + 64);
+  rs[lane] = rown;
+  wave_sync();
+  float inv = 1.0f / s_sh;
+  float xv = rown * inv;
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    float tk = pv[k] * (rs[l[k] < 255 ? l[k] : lane] * inv);
+    acc = l[k] < 255 ? acc + tk : acc;
+  }
+  if (v < n) xo[v] = xv;
+  const int sub = lane >> 2, q = lane & 3;
+  float w0[8], w1[8], b1[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    w0[e] = th[OW1 + 8 * q + e];
+    w1[e] = th[OW1 + H + 8 * q + e];
+    b1[e] = th[OB1 + 8 * q + e];
+  }
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    int src = 16 * g + sub;
+    float xg = __shfl(xv, src), ag = __shfl(acc, src);
+    float hh[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      float z = xg * w0[e] + ag * w1[e] + b1[e];
+      hh[e] = z > 0.0f ? z : 0.0f;
+    }
+    int row = t * 64 + src;
+    if (row < n) *reinterpret_cast<uint4 *>(h1 + static_cast<size_t>(row) * H + 8 * q) = pack8(hh);
   }
 }
 
@@ -1361,6 +1468,49 @@ static HostCsr transpose(const HostCsr &A) {
   return T;
 }
 
+// The halo lists of the 64-row tiles of Q (rows and columns in one
+// numbering; perm maps it to vertex numbers, NULL: the identity).
+static void build_halos(const HostCsr &Q, const std::vector<int32_t> *perm, NnLevel &L) {
+  int n = Q.m, w = 0;
+  for (int i = 0; i < n; i++) w = std::max(w, Q.ptr[i + 1] - Q.ptr[i]);
+  if (w < 1 || w > 8) return;
+  int tiles = (n + 63) / 64;
+  std::vector<int32_t> hoff(tiles + 1, 0), hrow, hrow_v, loc(n, -1), h;
+  std::vector<uint8_t> lid(static_cast<size_t>(w) * n, 255);
+  int hmax = 0;
+  for (int t = 0; t < tiles; t++) {
+    int r0 = 64 * t, r1 = std::min(n, r0 + 64);
+    h.clear();
+    for (int i = r0; i < r1; i++)
+      for (int k = Q.ptr[i]; k < Q.ptr[i + 1]; k++)
+        if (Q.col[k] < r0 || Q.col[k] >= r1) h.push_back(Q.col[k]);
+    std::sort(h.begin(), h.end());
+    h.erase(std::unique(h.begin(), h.end()), h.end());
+    if (h.size() > kHaloMax) return;
+    hmax = std::max(hmax, static_cast<int>(h.size()));
+    for (size_t j = 0; j < h.size(); j++) loc[h[j]] = 64 + static_cast<int>(j);
+    for (int i = r0; i < r1; i++)
+      for (int k = Q.ptr[i]; k < Q.ptr[i + 1]; k++) {
+        int c = Q.col[k];
+        lid[static_cast<size_t>(k - Q.ptr[i]) * n + i] = static_cast<uint8_t>(c >= r0 && c < r1 ? c - r0 : loc[c]);
+      }
+    for (int c : h) {
+      loc[c] = -1;
+      hrow.push_back(c);
+      hrow_v.push_back(perm ? (*perm)[c] : c);
+    }
+    hoff[t + 1] = static_cast<int32_t>(hrow.size());
+  }
+  // one more entry: the last tile's (clamped) loads stay in bounds
+  hrow.push_back(0);
+  hrow_v.push_back(0);
+  L.lidx.upload(lid);
+  L.hoff.upload(hoff);
+  L.hrow.upload(hrow);
+  L.hrow_v.upload(hrow_v);
+  L.hmax = hmax;
+}
+
 NnLevel::NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr, const std::vector<int32_t> *perm) {
   n = A.m;
   HostCsr P;
@@ -1415,6 +1565,7 @@ NnLevel::NnLevel(const HostCsr &A, const HostCsr *R, const HostCsr *Pr, const st
       to_ell(Qv, tmp);
       pi_vcol = std::move(tmp.col);
     }
+    if (n > kInfSmallRows) build_halos(Q, perm, *this);
   }
   to_ell(transpose(P), pt);
   if (R && Pr) {
@@ -1647,12 +1798,25 @@ void nn_smooth(dec_nn_s &nn, int lvl, const float *r, float *u, hipStream_t s, b
     const int *rperm = vorder ? NL.perm_d.p : nullptr;
     int tiles = (n + 63) / 64;
     dim3 tg((tiles + 7) / 8 * 8);
+    if (NL.hmax >= 0) {
+      float *x = reinterpret_cast<float *>(nn.ix.p);
+      DEC_ELL_WIDTH_SWITCH(NL.pi.width, if constexpr (W > 0) {
+        hipLaunchKernelGGL((k_inf_input_h<W>), dim3(dec::xcd_grid(n, TB)), dim3(TB), 0, s, n, NL.lidx.p,
+                           NL.pi.val.p, NL.hoff.p, vorder ? NL.hrow_v.p : NL.hrow.p, rperm, r, nn.spart.p, nn.s.p,
+                           x, nn.th.p, nn.ih1.p);
+        hipLaunchKernelGGL((k_inf_layer2w<W>), tg, dim3(64), 0, s, n, NL.pi.col.p, NL.pi.val.p, nn.ih1.p, nn.th.p,
+                           nn.ih2.p);
+        hipLaunchKernelGGL((k_inf_layer3w<W, float>), tg, dim3(64), 0, s, n, NL.pi.col.p, NL.pi.val.p, nn.ih2.p, x,
+                           nn.th.p, nn.s.p, NL.Di.p, rperm, u);
+      });
+      return;
+    }
     DEC_ELL_WIDTH_SWITCH(NL.pi.width, if constexpr (W > 0) {
       hipLaunchKernelGGL((k_inf_input_c<W>), dim3(dec::xcd_grid(n, TB)), dim3(TB), 0, s, n, NL.pi.col.p,
                          NL.pi.val.p, rcol, rperm, r, nn.spart.p, nn.s.p, nn.ix.p, nn.th.p, nn.ih1.p);
       hipLaunchKernelGGL((k_inf_layer2w<W>), tg, dim3(64), 0, s, n, NL.pi.col.p, NL.pi.val.p, nn.ih1.p, nn.th.p,
                          nn.ih2.p);
-      hipLaunchKernelGGL((k_inf_layer3w<W>), tg, dim3(64), 0, s, n, NL.pi.col.p, NL.pi.val.p, nn.ih2.p, nn.ix.p,
+      hipLaunchKernelGGL((k_inf_layer3w<W, float2>), tg, dim3(64), 0, s, n, NL.pi.col.p, NL.pi.val.p, nn.ih2.p, nn.ix.p,
                          nn.th.p, nn.s.p, NL.Di.p, rperm, u);
     });
     return;
