@@ -405,6 +405,30 @@ multigrid_vcycles(u::ROCVector{Float64}, b::ROCVector{Float64}, md::HIPMultigrid
 multigrid_wcycles(u::ROCVector{Float64}, b::ROCVector{Float64}, md::HIPMultigridData, cycles, alg=Krylov.cg) =
   multigrid_μ_cycles(u, b, md, cycles, alg, 2)
 
+# A u = b to ‖b - A u‖₂/‖b‖₂ ≤ tol (checked in float64), by dec_mg_solve.  No
+# counterpart in the reference (its solves run a fixed number of cycles,
+# src/Multigrid.jl:814-855).  variant: :plain (float64 V-cycles), :mixed
+# (float32 V-cycles in float64 iterative refinement), :pcg / :pcg_mixed
+# (conjugate gradients preconditioned by one float64 / float32 V-cycle); the
+# smoother is the hybrid block Gauss-Seidel.  Returns (u, cycles or CG
+# iterations, relative residual).
+const SOLVE_VARIANTS = Dict(:plain => Int32(0), :mixed => Int32(1), :pcg => Int32(2), :pcg_mixed => Int32(3))
+
+function multigrid_solve(b::ROCVector{Float64}, md::HIPMultigridData; u=nothing, tol=1e-8, max_cycles=200,
+                         variant=:plain)
+  set_smoother!(md, :block_gauss_seidel; coarse_direct=true)
+  v = get(SOLVE_VARIANTS, variant) do
+    error("unsupported solve variant $variant")
+  end
+  x = u === nothing ? fill!(similar(b), 0) : copy(u)
+  done = Ref{Int32}(0)
+  rr = Ref{Float64}(0)
+  check(ccall((:dec_mg_solve, libdec_hip), Cint,
+    (Ptr{Cvoid}, Int32, Float64, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ptr{Float64}, Ptr{Cvoid}),
+    md.handle, v, tol, Int32(max_cycles), pointer(x), pointer(b), done, rr, stream_ptr()))
+  x, Int(done[]), rr[]
+end
+
 function full_multigrid(b::ROCVector{Float64}, md::HIPMultigridData, cycles, alg=Krylov.cg, μ=1)
   set_smoother!(md, alg)
   u = similar(b)
