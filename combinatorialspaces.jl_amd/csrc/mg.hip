@@ -1098,8 +1098,9 @@ struct dec_mg_s {
   bool use_graph = true;
   // residual norms of a solve on their way to the host (pinned ring + events)
   DBuf<double> rpart;  // per-workgroup partial sums of the finest residual's r²
+  // (slot kNormSlots: ||b||^2 of the solve)
   double *hnorm = nullptr;
-  hipEvent_t nev[kNormSlots] = {};
+  hipEvent_t nev[kNormSlots + 1] = {};
   // learned smoother (float32 cycles, levels of learned_min_rows to
   // learned_max_rows rows; learned_max_rows <= 0: no upper bound)
   dec_nn_s *nn = nullptr;
@@ -1565,7 +1566,7 @@ void finish_hierarchy(dec_mg_s &mg, const std::vector<dec::HostCsr> &As, std::ve
   mg.part2.alloc(kDotBlocks);
   mg.scal.alloc(8);
   mg.rpart.alloc(static_cast<size_t>(dec::nblocks(mg.lv[0]->n, kBlock)));
-  if (hipHostMalloc(reinterpret_cast<void **>(&mg.hnorm), sizeof(double) * kNormSlots, hipHostMallocDefault) !=
+  if (hipHostMalloc(reinterpret_cast<void **>(&mg.hnorm), sizeof(double) * (kNormSlots + 1), hipHostMallocDefault) !=
       hipSuccess)
     fail(DEC_ERR_OOM, "multigrid: pinned host buffer");
   for (auto &e : mg.nev)
@@ -2051,6 +2052,19 @@ void gcheck(hipError_t e, const char *what) {
   if (e != hipSuccess) fail(DEC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// ||b||^2 on its way to the host (pinned slot kNormSlots with its event), and
+// its square root once it has arrived
+void send_bnorm(dec_mg_s &mg, int n, const double *b, hipStream_t s) {
+  dot(mg, n, b, b, mg.scal.p + kNormSlots, nullptr, s);
+  gcheck(hipMemcpyAsync(mg.hnorm + kNormSlots, mg.scal.p + kNormSlots, sizeof(double), hipMemcpyDeviceToHost, s),
+         "hipMemcpyAsync");
+  gcheck(hipEventRecord(mg.nev[kNormSlots], s), "hipEventRecord");
+}
+double read_bnorm(dec_mg_s &mg) {
+  gcheck(hipEventSynchronize(mg.nev[kNormSlots]), "hipEventSynchronize");
+  return std::sqrt(mg.hnorm[kNormSlots]);
+}
+
 // body(stream) replayed from the hipGraph in slot i, captured for (u, b) and
 // the current smoother settings (key); captured again when any of them changes.
 template <class Body>
@@ -2098,12 +2112,23 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_call
            int *done_cycles, double *relres, hipStream_t s) {
   Level &F = *mg.lv[0];
   int n = F.n;
-  double bn = norm2(mg, n, b_caller, s);
-  if (bn == 0.0) {
-    hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, u_caller);
-    *done_cycles = 0;
-    *relres = 0.0;
-    return;
+  // With the iterate in the hierarchy's own numbering, the caller's u stays
+  // untouched until the end, so the first cycles are queued before the host
+  // has read ||b|| and the starting residual (one wait instead of two, with
+  // the GPU busy during it); a b of zero or a start that already meets the
+  // tolerance then returns without them.
+  const bool ahead = !F.identity;
+  double bn = 0.0;
+  if (ahead) {
+    send_bnorm(mg, n, b_caller, s);
+  } else {
+    bn = norm2(mg, n, b_caller, s);
+    if (bn == 0.0) {
+      hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, u_caller);
+      *done_cycles = 0;
+      *relres = 0.0;
+      return;
+    }
   }
   FineIO io = fine_in(mg, u_caller, b_caller, true, s);
   double *u = io.u;
@@ -2156,8 +2181,26 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_call
   // stays above 1000 tol after the queued cycles, the cycle after the check
   // is queued too (no V-cycle cuts the residual 1000x).
   post(0, true);
+  if (ahead) {
+    // what the loop below queues first for a start far from the solution
+    if (max_cycles >= 1) enqueue_cycle(true);
+    if (max_cycles >= 2) enqueue_cycle(false);
+    bn = read_bnorm(mg);
+    if (bn == 0.0) {
+      hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, u_caller);
+      *done_cycles = 0;
+      *relres = 0.0;
+      return;
+    }
+  }
   double rr = read_norm(0), prev = rr, rho = -1.0;
   int c = 0;
+  if (ahead && rr <= tol) {
+    mg.learned_cycles = 0;
+    *done_cycles = 0;
+    *relres = rr;
+    return;
+  }
   while (rr > tol && c < max_cycles) {
     int kc = c + 1;
     // (the prediction takes each cycle to cut twice as much as the last ones
@@ -2172,7 +2215,7 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_call
       post(kc, true);
       enq_checked = true;
     }
-    if (kc + 1 <= max_cycles && rr * std::pow(1e-3, kc + 1 - c) > tol) enqueue_cycle(false);
+    if (enq < kc + 1 && kc + 1 <= max_cycles && rr * std::pow(1e-3, kc + 1 - c) > tol) enqueue_cycle(false);
     double rk = read_norm(kc);
     rho = std::min(0.95, std::max(1e-4, std::pow(rk / rr, 1.0 / (kc - c))));
     // a learned cycle that cut the residual by less than 0.5x hands the rest
@@ -2231,12 +2274,20 @@ void pcg_solve(dec_mg_s &mg, bool mixed, double tol, int max_iters, double *u_ca
                int *done_iters, double *relres, hipStream_t s) {
   Level &F = *mg.lv[0];
   int n = F.n;
-  double bn = norm2(mg, n, b_caller, s);
-  if (bn == 0.0) {
-    hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, u_caller);
-    *done_iters = 0;
-    *relres = 0.0;
-    return;
+  // the first iterations are queued before the host reads ||b|| and the
+  // starting residual, as in solve()
+  const bool ahead = !F.identity;
+  double bn = 0.0;
+  if (ahead) {
+    send_bnorm(mg, n, b_caller, s);
+  } else {
+    bn = norm2(mg, n, b_caller, s);
+    if (bn == 0.0) {
+      hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, u_caller);
+      *done_iters = 0;
+      *relres = 0.0;
+      return;
+    }
   }
   int nb = dec::nblocks(n, kBlock);
   if (mg.pcg_r.n < static_cast<size_t>(n)) {
@@ -2288,9 +2339,25 @@ void pcg_solve(dec_mg_s &mg, bool mixed, double tol, int max_iters, double *u_ca
   };
   true_residual(0);
   restart();
+  if (ahead) {
+    if (max_iters >= 1) enqueue(true);
+    if (max_iters >= 2) enqueue(false);
+    bn = read_bnorm(mg);
+    if (bn == 0.0) {
+      hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, u_caller);
+      *done_iters = 0;
+      *relres = 0.0;
+      return;
+    }
+  }
   double rr = read_norm(0), rho = -1.0;
   int c = 0;
   bool stalled = false, verified = rr <= tol;
+  if (ahead && verified) {
+    *done_iters = 0;
+    *relres = rr;
+    return;
+  }
   while (c < max_iters && !verified) {
     if (rr <= tol) {
       // the recursive residual meets the tolerance: confirm it on the
@@ -2317,7 +2384,7 @@ void pcg_solve(dec_mg_s &mg, bool mixed, double tol, int max_iters, double *u_ca
       send(kc, mg.pcg_part.p + kPcgDotBlocks + dec::xcd_grid(n), nb);
       enq_checked = true;
     }
-    if (kc + 1 <= max_iters && rr * std::pow(1e-3, kc + 1 - c) > tol) enqueue(false);
+    if (enq < kc + 1 && kc + 1 <= max_iters && rr * std::pow(1e-3, kc + 1 - c) > tol) enqueue(false);
     double rk = read_norm(kc);
     double rate = std::pow(rk / rr, 1.0 / (kc - c));
     rr = rk;

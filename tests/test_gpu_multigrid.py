@@ -304,6 +304,27 @@ def test_pcg_solve(O, cs, sphere5_pair, variant):
     assert it < cyc, (it, cyc)
 
 
+@pytest.mark.parametrize("pair", ["sphere_pair", "sphere5_pair"])
+@pytest.mark.parametrize("variant", ["plain", "mixed", "pcg", "pcg_mixed"])
+def test_solve_zero_rhs_and_converged_start(O, cs, request, pair, variant):
+    """b = 0 gives u = 0 after no cycle, and a start that already meets the
+    tolerance comes back unchanged after no cycle.  On the renumbered
+    hierarchy (sphere5_pair) the library queues its first cycles before it
+    reads ||b|| and the starting residual, so these exits discard work
+    already queued; on sphere_pair (vertex order) it reads first."""
+    import torch
+
+    H, md = request.getfixturevalue(pair)
+    n = H.A[0].n
+    u = torch.tensor(O.uniform_form(5, n), device="cuda")
+    u, c, rr = cs.solve(md, torch.zeros(n, dtype=torch.float64, device="cuda"), u=u, tol=1e-8, variant=variant)
+    assert c == 0 and rr == 0.0 and not u.any()
+    f = torch.tensor(sphere_rhs(O, H.duals[0], 3), device="cuda")
+    x, _, _ = cs.solve(md, f, tol=1e-10, variant=variant)
+    x2, c2, rr2 = cs.solve(md, f, u=x.clone(), tol=1e-8, variant=variant)
+    assert c2 == 0 and rr2 <= 1e-8 and torch.equal(x2, x)
+
+
 def test_hierarchy_from_matrices(O, cs, sphere5_pair):
     """MultigridData(ops, rs, ps, steps) (Multigrid.jl:545-546): the oracle's own
     CSR matrices handed to the library give the oracle's cycles (CG smoother
