@@ -502,6 +502,17 @@ def learned(torch, cs, md, fd, npairs=10000, train_levels=3, steps=2000, batch=1
     app_ms = a.elapsed_time(b) / reps
     out["inference_ms_per_application_fine"] = app_ms
     out["dense_layer_TFLOPs_fine"] = 2 * 2 * 64 * 32 * n0 / (app_ms * 1e-3) / 1e12
+    # algorithmic bytes of the application, per row with w slots (icosphere
+    # vertices have 5 or 6 neighbours, w = 6): the scale's read of r (4);
+    # layer 1: a one-byte index and a weight per slot, r, x and h1 (32 BF16)
+    # written; layers 2 and 3: column and weight per slot and the own h row,
+    # h2 written; x, D and u read, u written; perm read in layers 1 and 3 (the
+    # vertex-order call)
+    w = 6
+    per_row = 4 + (5 * w + 4 + 4 + 64) + (8 * w + 64 + 64) + (8 * w + 64 + 4 + 4 + 8) + 8
+    out["inference_alg_MB_fine"] = per_row * n0 / 1e6
+    out["inference_GBps_fine"] = per_row * n0 / (app_ms * 1e-3) / 1e9
+    out["inference_frac_hbm_peak_fine"] = out["inference_GBps_fine"] / 8000.0
     # levels that use the network: all (one or two applications); the two
     # finest; the three below the finest (the finest keeps Gauss-Seidel)
     # one application per level at each level size (internal numbering, as in the cycles)
