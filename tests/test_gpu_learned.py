@@ -105,6 +105,57 @@ def test_inference_kernels_match_forward(O, cs, base, levels):
         assert torch.equal(u, u0 + c), (lv, n, (u - (u0 + c)).abs().max().item())
 
 
+def test_inference_with_wide_halos(O, cs):
+    """A level above 50,000 rows whose 64-row tiles name more than 191 other
+    rows runs layer 1 without the staged halo (the direct gathers).  The
+    graph is a ring with five random matchings, so the breadth-first blocks
+    cannot keep a row's neighbours together.  The vertex-order call still adds
+    exactly the forward's c."""
+    import scipy.sparse as sp
+    import torch
+
+    rng = np.random.default_rng(3)
+    n, agg = 60000, 60
+    rows = [np.arange(n), np.arange(n)]
+    cols = [(np.arange(n) + 1) % n, (np.arange(n) - 1) % n]
+    for _ in range(5):
+        p = rng.permutation(n)
+        rows += [p[0::2], p[1::2]]
+        cols += [p[1::2], p[0::2]]
+    W = sp.csr_matrix((np.ones(sum(r.size for r in rows)), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(n, n))
+    W.data[:] = 1.0  # duplicate edges count once
+    W.setdiag(0)
+    W.eliminate_zeros()
+    deg = np.asarray(W.sum(axis=1)).ravel()
+    A = (sp.diags(deg + 0.5) - W).tocsr()
+    A.sort_indices()
+    R = sp.csr_matrix((np.full(n, 1.0 / agg), (np.arange(n) // agg, np.arange(n))), shape=(n // agg, n))
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), np.arange(n) // agg)), shape=(n, n // agg))
+    # the coarse operator only has to be a valid level: a ring of n / agg rows
+    m = n // agg
+    Ac = (sp.diags(np.full(m, 2.5)) - sp.diags(np.ones(m - 1), 1) - sp.diags(np.ones(m - 1), -1)).tocsr()
+    Ac.sort_indices()
+    csr = lambda M: (M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.astype(np.float64))  # noqa: E731
+    md = cs.MultigridData.from_matrices([csr(A), csr(Ac)], [csr(R)], [csr(P)], steps=2)
+    blk, nb = md.blocks(0)
+    # the full 64-row blocks (contiguous in the level's numbering) name far
+    # more than 191 rows outside them, so the tiles over them do too
+    size = np.bincount(blk, minlength=nb)
+    B = sp.csr_matrix((np.ones(n), (blk, np.arange(n))), shape=(nb, n))
+    reach = (B @ (A != 0)).tocsr()
+    halo = np.diff(reach.indptr) - size
+    full = size == 64
+    assert full.sum() >= 100 and halo[full].min() > 250, (full.sum(), halo[full].min())
+    nn = cs.LearnedSmoother(md, seed=8)
+    nn.set_params(random_params(nn, 4))
+    r = torch.tensor(O.uniform_form(30, n), dtype=torch.float32, device="cuda")
+    u0 = torch.tensor(O.uniform_form(60, n), dtype=torch.float32, device="cuda")
+    c = nn.forward(0, r.clone())
+    u = nn.smooth(0, r, u0.clone())
+    assert torch.equal(u, u0 + c), (u - (u0 + c)).abs().max().item()
+
+
 @pytest.mark.parametrize("kind", ["residual", "error"])
 def test_training_gain_and_solve(O, cs, kind):
     """Training lowers the loss on held-out V-cycle pairs compared with the
