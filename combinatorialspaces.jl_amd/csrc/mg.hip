@@ -2188,6 +2188,7 @@ void solve(dec_mg_s &mg, int variant, double tol, int max_cycles, double *u_call
     bn = read_bnorm(mg);
     if (bn == 0.0) {
       hipLaunchKernelGGL(k_zero<double>, G(n), dim3(kBlock), 0, s, n, u_caller);
+      mg.learned_cycles = 0;
       *done_cycles = 0;
       *relres = 0.0;
       return;
