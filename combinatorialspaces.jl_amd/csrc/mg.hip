@@ -624,6 +624,100 @@ __global__ void k_cgf_p(CgState *st, int c, int n, double *__restrict__ p, const
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kDotBlocks * kBlock) p[i] = r[i] + beta * p[i];
 }
 
+// The CG smoother (the recurrence of k_cgf_*) on a level of at most
+// kCgSmallRows rows, as one workgroup of kCgThreads threads: thread t owns
+// rows t + kCgThreads i (i < R) and keeps their x and r in registers; p and
+// A p live in LDS, so the product gathers from LDS (one row at a time, which
+// keeps its loads within the registers of 1024 threads), and each dot product is
+// one workgroup reduction in a fixed order (waves, then the waves' sums).
+// One launch per smoothing in place of 4 + 4 steps launches.  Its dot
+// products add in another order than k_dot_partial's partition, so its
+// iterates agree with those of the multi-launch smoother to rounding.
+constexpr int kCgSmallRows = 4000;  // p and A p in 64,000 bytes of LDS
+constexpr int kCgThreads = 1024;
+
+__device__ double wg_sum(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off, 64);
+  int w = threadIdx.x >> 6;
+  __syncthreads();  // every thread has read red[] of the previous sum
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < kCgThreads / 64; k++) t = t + red[k];
+  return t;
+}
+
+template <int W, int R>
+__global__ void __launch_bounds__(kCgThreads)
+    k_cg_smooth_small(int n, int width, const int *__restrict__ vcol, const double *__restrict__ vval,
+                      const double *__restrict__ b, double *__restrict__ u, int steps) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double *p = reinterpret_cast<double *>(smem_raw);
+  double *Ap = p + n;
+  __shared__ double red[kCgThreads / 64];
+  double x[R], r[R];
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    int i = static_cast<int>(threadIdx.x) + k * kCgThreads;
+    x[k] = 0.0;
+    r[k] = 0.0;
+    if (i < n) {
+      r[k] = b[i] - dec_ell::row<W, double, double>(n, width, i, vcol, vval, u);
+      p[i] = r[k];
+      acc = acc + r[k] * r[k];
+    }
+  }
+  double gamma = wg_sum(acc, red);
+  double rnorm = sqrt(gamma);
+  const double sq = 1.4901161193847656e-08, eps = 2.220446049250313e-16;
+  const double eps_stop = sq + sq * rnorm;
+  double pnorm2 = gamma;
+  bool done = (gamma == 0.0) || (rnorm <= eps_stop);
+  for (int it = 0; it < steps && !done; it++) {
+    __syncthreads();  // p complete
+    double pap = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < R; k++) {
+      int i = static_cast<int>(threadIdx.x) + k * kCgThreads;
+      if (i < n) {
+        double api = dec_ell::row<W, double, double>(n, width, i, vcol, vval, p);
+        Ap[i] = api;
+        pap = pap + p[i] * api;
+      }
+    }
+    double pAp = wg_sum(pap, red);
+    if (pAp <= eps * pnorm2 && fabs(pAp) <= eps * pnorm2) break;
+    double a = gamma / pAp;
+    double rr = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      int i = static_cast<int>(threadIdx.x) + k * kCgThreads;
+      if (i < n) {
+        x[k] = x[k] + a * p[i];
+        r[k] = r[k] - a * Ap[i];
+        rr = rr + r[k] * r[k];
+      }
+    }
+    double gn = wg_sum(rr, red);
+    double rn = sqrt(gn);
+    if (rn <= eps_stop || rn + 1.0 <= 1.0) break;
+    double beta = gn / gamma;
+    pnorm2 = gn + beta * beta * pnorm2;
+    gamma = gn;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      int i = static_cast<int>(threadIdx.x) + k * kCgThreads;
+      if (i < n) p[i] = r[k] + beta * p[i];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    int i = static_cast<int>(threadIdx.x) + k * kCgThreads;
+    if (i < n) u[i] = x[k] + u[i];
+  }
+}
+
 __global__ void k_axpy_final(int n, const double *__restrict__ x, double *__restrict__ u) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) u[i] = x[i] + u[i];
@@ -1811,8 +1905,19 @@ void smooth(dec_mg_s &mg, int lvl, Level &L, const T *b, T *u, bool post, bool u
     }
   }
   if (mg.alg == DEC_SMOOTH_CG) {
-    if constexpr (sizeof(T) == 8) cg_smooth(mg, L, b, u, steps, s);
-    else fail(DEC_ERR_UNSUPPORTED, "the CG smoother runs in float64 only");
+    if constexpr (sizeof(T) == 8) {
+      if (L.n <= kCgSmallRows && L.width >= 1 && L.width <= 8) {
+        size_t lds = 2 * static_cast<size_t>(L.n) * sizeof(double);  // p and A p
+        DEC_ELL_WIDTH_SWITCH(L.width, if constexpr (W > 0) {
+          hipLaunchKernelGGL((k_cg_smooth_small<W, (kCgSmallRows + kCgThreads - 1) / kCgThreads>), dim3(1), dim3(kCgThreads), lds,
+                             s, L.n, L.width, L.vcol.p, L.d.vval.p, b, u, steps);
+        });
+      } else {
+        cg_smooth(mg, L, b, u, steps, s);
+      }
+    } else {
+      fail(DEC_ERR_UNSUPPORTED, "the CG smoother runs in float64 only");
+    }
     return;
   }
   if (mg.alg == DEC_SMOOTH_BLOCK_GS) {
