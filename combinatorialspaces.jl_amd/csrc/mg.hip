@@ -4,8 +4,8 @@
 //
 // Per level the operator is held in slot-major ELL with rows sorted by
 // (colour, vertex): a multicolour Gauss-Seidel pass over one colour is then a
-// contiguous range of rows.  The same storage serves residuals and the CG
-// smoother (rows write back through their vertex id).
+// contiguous range of rows.  The same slots with rows in vertex order serve
+// residuals and the CG smoother.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -537,101 +537,212 @@ struct CgState {
   int done, iter;
 };
 
-// The scalar steps run inside the vector kernels: every block reduces the
-// kDotBlocks partial sums of the previous launch itself with block_sum's tree,
-// so each holds the value a separate k_sum_partials launch would give, and
-// block 0 alone writes the scalars back.  The partial sums keep k_dot_partial's
-// partition (kDotBlocks blocks, grid-stride over the form index) and every
-// element update is the unfused one, so the iterates are those of the
-// sequence dot → scalar kernel → update, bit for bit, in fewer launches.
+// Levels above kCgSmallRows rows: two launches per iteration.  The scalar
+// steps run inside the vector kernels: every block reduces all partial sums of
+// the previous launch itself, in one fixed order, so all blocks hold the same
+// γ, β and α, and block 0 alone writes the scalars back.  p is double-
+// buffered: the direction launch of iteration it > 0 forms p = r + β p_old for
+// its own rows into the other buffer, and its product evaluates r + β p_old at
+// every column, the expression the column's own row stores, so A p is the
+// product of the stored p without a launch in between.  The last step launch
+// adds the correction to u.  Against the four-launch sequence (product, dot,
+// α step, β step) the per-row values are the same expressions; only the dot
+// products add in another partition, so the iterates agree to rounding.
+constexpr int kCgBlocksMax = 1024;  // partial sums per reduction
 
-// x = 0, p = r, and the partial sums of r·r.
-__global__ void k_cgf_start(int n, double *__restrict__ x, const double *__restrict__ r, double *__restrict__ p,
-                            double *__restrict__ part) {
+inline int cg_blocks(int n) { return std::min(kCgBlocksMax, dec::nblocks(n, kBlock)); }
+
+// Σ part[0..np) in a fixed order, the same in every workgroup: thread t adds
+// a run of ceil(np / kBlock) consecutive partials, then block_sum's tree
+// (every thread has read the result before a later block_sum of the kernel
+// writes the same LDS array)
+__device__ double sum_partials(const double *__restrict__ part, int np) {
+  constexpr int kRun = 16;  // runs up to this long load all their partials at once
+  int per = (np + kBlock - 1) / kBlock;
+  int j0 = min(np, static_cast<int>(threadIdx.x) * per), j1 = min(np, j0 + per);
   double acc = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kDotBlocks * kBlock) {
-    double ri = r[i];
-    x[i] = 0.0;
+  if (per <= kRun) {
+    double v[kRun];
+#pragma unroll
+    for (int k = 0; k < kRun; k++) v[k] = part[min(j0 + k, np - 1)];
+#pragma unroll
+    for (int k = 0; k < kRun; k++)
+      if (j0 + k < j1) acc = acc + v[k];
+  } else {
+    for (int j = j0; j < j1; j++) acc = acc + part[j];
+  }
+  double s = block_sum<double>(acc);
+  __syncthreads();
+  return s;
+}
+
+// (A p)_i with p = r + β q evaluated at each column, summed in slot order.
+template <int W>
+__device__ __forceinline__ double row_dir(int n, int width, int i, const int *__restrict__ col,
+                                          const double *__restrict__ val, const double *__restrict__ r,
+                                          const double *__restrict__ q, double beta) {
+  double acc = 0.0;
+  if constexpr (W > 0) {
+    int c[W];
+    double v[W], rv[W], qv[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      size_t o = static_cast<size_t>(k) * n + i;
+      c[k] = col[o];
+      v[k] = val[o];
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      int cc = c[k] < 0 ? 0 : c[k];
+      rv[k] = r[cc];
+      qv[k] = q[cc];
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++)
+      if (c[k] >= 0) acc = acc + v[k] * (rv[k] + beta * qv[k]);
+  } else {
+    for (int k = 0; k < width; k++) {
+      size_t o = static_cast<size_t>(k) * n + i;
+      int cc = col[o];
+      if (cc < 0) break;
+      acc = acc + val[o] * (r[cc] + beta * q[cc]);
+    }
+  }
+  return acc;
+}
+
+// r = b - A u, p = r, x = 0, and the partial sums of r·r.
+template <int W>
+__global__ void __launch_bounds__(kBlock)
+    k_cg2_start(int n, int width, const int *__restrict__ vcol, const double *__restrict__ vval,
+                const double *__restrict__ b, const double *__restrict__ u, double *__restrict__ r,
+                double *__restrict__ p, double *__restrict__ x, double *__restrict__ part) {
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double ri = b[i] - dec_ell::row<W, double, double>(n, width, i, vcol, vval, u);
+    r[i] = ri;
     p[i] = ri;
+    x[i] = 0.0;
     acc = acc + ri * ri;
   }
   double sum = block_sum<double>(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = sum;
 }
 
-// γ = r·r, ‖r‖, the stopping threshold, and the immediate exit.
-__global__ void k_cgf_init(CgState *st, const double *__restrict__ part) {
-  double g = block_sum<double>(threadIdx.x < kDotBlocks ? part[threadIdx.x] : 0.0);
-  if (threadIdx.x != 0) return;
-  st->gamma[0] = g;
-  st->rnorm = sqrt(g);
-  const double sq = 1.4901161193847656e-08;
-  st->eps_stop = sq + sq * st->rnorm;
-  st->pnorm2[0] = g;
-  st->iter = 0;
-  st->done = (g == 0.0) || (st->rnorm <= st->eps_stop);
+// Iteration it's direction: at it = 0, γ = r·r, ‖r‖, the stopping threshold
+// and the immediate exit; at it > 0, ‖r‖ and the solved exit, β = γ_new / γ,
+// ‖p‖² and γ to slot it & 1, and p = r + β p_old.  Then A p and the partial
+// sums of p·Ap.
+template <int W>
+__global__ void __launch_bounds__(kBlock)
+    k_cg2_dir(CgState *st, int it, int n, int width, const int *__restrict__ vcol, const double *__restrict__ vval,
+              const double *__restrict__ r, const double *__restrict__ pold, double *__restrict__ p,
+              double *__restrict__ Ap, const double *__restrict__ part_rr, int nb, double *__restrict__ part_pap) {
+  if (it > 0 && st->done) return;
+  double g = sum_partials(part_rr, nb);
+  double rn = sqrt(g);
+  bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  double beta = 0.0;
+  if (it == 0) {
+    const double sq = 1.4901161193847656e-08;
+    double es = sq + sq * rn;
+    bool stop = (g == 0.0) || (rn <= es);
+    if (lead) {
+      st->gamma[0] = g;
+      st->rnorm = rn;
+      st->eps_stop = es;
+      st->pnorm2[0] = g;
+      st->iter = 0;
+      st->done = stop;
+    }
+    if (stop) return;
+  } else {
+    int c = (it - 1) & 1;
+    if (rn <= st->eps_stop || rn + 1.0 <= 1.0) {
+      if (lead) {
+        st->rnorm = rn;
+        st->iter += 1;
+        st->done = 2;  // solved: p is left as is
+      }
+      return;
+    }
+    beta = g / st->gamma[c];
+    if (lead) {
+      st->rnorm = rn;
+      st->iter += 1;
+      st->beta = beta;
+      st->pnorm2[c ^ 1] = g + beta * beta * st->pnorm2[c];
+      st->gamma[c ^ 1] = g;
+    }
+  }
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double pi, api;
+    if (it == 0) {
+      pi = pold[i];
+      api = dec_ell::row<W, double, double>(n, width, i, vcol, vval, pold);
+    } else {
+      pi = r[i] + beta * pold[i];
+      p[i] = pi;
+      api = row_dir<W>(n, width, i, vcol, vval, r, pold, beta);
+    }
+    Ap[i] = api;
+    acc = acc + pi * api;
+  }
+  double sum = block_sum<double>(acc);
+  if (threadIdx.x == 0) part_pap[blockIdx.x] = sum;
 }
 
-// α = γ / p·Ap (zero-curvature exit), x += α p, r -= α Ap, partials of r·r.
-__global__ void k_cgf_xr(CgState *st, int c, int n, double *__restrict__ x, const double *__restrict__ p,
-                         double *__restrict__ r, const double *__restrict__ Ap, const double *__restrict__ part,
-                         double *__restrict__ part2) {
-  if (st->done) return;
+// Iteration it's step: α = γ / p·Ap (zero-curvature exit), x += α p,
+// r -= α Ap and the partial sums of r·r; on the last iteration u += x
+// instead (x as it stands when an exit came earlier).
+__global__ void __launch_bounds__(kBlock)
+    k_cg2_step(CgState *st, int it, int last, int n, double *__restrict__ x, const double *__restrict__ p,
+               double *__restrict__ r, const double *__restrict__ Ap, const double *__restrict__ part_pap, int nb,
+               double *__restrict__ part_rr, double *__restrict__ u) {
   const double eps = 2.220446049250313e-16;
-  double pAp = block_sum<double>(threadIdx.x < kDotBlocks ? part[threadIdx.x] : 0.0);
-  __syncthreads();  // every thread has read pAp before the second block_sum below
-  double pn2 = st->pnorm2[c];
-  if (pAp <= eps * pn2 && fabs(pAp) <= eps * pn2) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
+  bool stop = st->done != 0;
+  double a = 0.0;
+  if (!stop) {
+    double pAp = sum_partials(part_pap, nb);
+    int c = it & 1;
+    double pn2 = st->pnorm2[c];
+    if (pAp <= eps * pn2 && fabs(pAp) <= eps * pn2) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
+      stop = true;
+    } else {
+      a = st->gamma[c] / pAp;
+      if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = a;
+    }
+  }
+  int stride = gridDim.x * kBlock;
+  if (last) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+      double xi = stop ? x[i] : x[i] + a * p[i];
+      u[i] = xi + u[i];
+    }
     return;
   }
-  double a = st->gamma[c] / pAp;
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = a;
+  if (stop) return;
   double acc = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kDotBlocks * kBlock) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
     x[i] = x[i] + a * p[i];
     double ri = r[i] - a * Ap[i];
     r[i] = ri;
     acc = acc + ri * ri;
   }
   double sum = block_sum<double>(acc);
-  if (threadIdx.x == 0) part2[blockIdx.x] = sum;
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = sum;
 }
 
-// ‖r‖ and the solved exit, β = γ_new / γ, ‖p‖² and γ to the other slot, p = r + β p.
-__global__ void k_cgf_p(CgState *st, int c, int n, double *__restrict__ p, const double *__restrict__ r,
-                        const double *__restrict__ part2) {
-  if (st->done) return;
-  double gn = block_sum<double>(threadIdx.x < kDotBlocks ? part2[threadIdx.x] : 0.0);
-  double rn = sqrt(gn);
-  bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  if (rn <= st->eps_stop || rn + 1.0 <= 1.0) {
-    if (lead) {
-      st->rnorm = rn;
-      st->iter += 1;
-      st->done = 2;  // solved: p is left as is
-    }
-    return;
-  }
-  double beta = gn / st->gamma[c];
-  if (lead) {
-    st->rnorm = rn;
-    st->iter += 1;
-    st->beta = beta;
-    st->pnorm2[c ^ 1] = gn + beta * beta * st->pnorm2[c];
-    st->gamma[c ^ 1] = gn;
-  }
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kDotBlocks * kBlock) p[i] = r[i] + beta * p[i];
-}
-
-// The CG smoother (the recurrence of k_cgf_*) on a level of at most
+// The CG smoother (the recurrence of k_cg2_*) on a level of at most
 // kCgSmallRows rows, as one workgroup of kCgThreads threads: thread t owns
 // rows t + kCgThreads i (i < R) and keeps their x and r in registers; p and
 // A p live in LDS, so the product gathers from LDS (one row at a time, which
 // keeps its loads within the registers of 1024 threads), and each dot product is
 // one workgroup reduction in a fixed order (waves, then the waves' sums).
-// One launch per smoothing in place of 4 + 4 steps launches.  Its dot
-// products add in another order than k_dot_partial's partition, so its
+// One launch per smoothing in place of 1 + 2 steps launches.  Its dot
+// products add in another order than the multi-launch partition, so its
 // iterates agree with those of the multi-launch smoother to rounding.
 constexpr int kCgSmallRows = 4000;  // p and A p in 64,000 bytes of LDS
 constexpr int kCgThreads = 1024;
@@ -718,11 +829,6 @@ __global__ void __launch_bounds__(kCgThreads)
   }
 }
 
-__global__ void k_axpy_final(int n, const double *__restrict__ x, double *__restrict__ u) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) u[i] = x[i] + u[i];
-}
-
 __global__ void k_d2f(int n, const double *__restrict__ x, float *__restrict__ y) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = static_cast<float>(x[i]);
@@ -756,29 +862,6 @@ __global__ void k_f2d(int n, const float *__restrict__ c, double *__restrict__ z
 // (float64 preconditioner), or the float32 r and a zero float32 iterate
 // (float32 preconditioner).
 
-// Σ part[0..np) in a fixed order, the same in every workgroup: thread t adds
-// a run of ceil(np / kBlock) consecutive partials, then block_sum's tree
-// (every thread has read the result before a later block_sum of the kernel
-// writes the same LDS array)
-__device__ double sum_partials(const double *__restrict__ part, int np) {
-  constexpr int kRun = 16;  // runs up to this long load all their partials at once
-  int per = (np + kBlock - 1) / kBlock;
-  int j0 = min(np, static_cast<int>(threadIdx.x) * per), j1 = min(np, j0 + per);
-  double acc = 0.0;
-  if (per <= kRun) {
-    double v[kRun];
-#pragma unroll
-    for (int k = 0; k < kRun; k++) v[k] = part[min(j0 + k, np - 1)];
-#pragma unroll
-    for (int k = 0; k < kRun; k++)
-      if (j0 + k < j1) acc = acc + v[k];
-  } else {
-    for (int j = j0; j < j1; j++) acc = acc + part[j];
-  }
-  double s = block_sum<double>(acc);
-  __syncthreads();
-  return s;
-}
 
 // kPcgDotBlocks partial sums of r·z over a grid-stride loop (z32 non-null:
 // z = the float32 preconditioner's result, stored in float64 first)
@@ -1168,7 +1251,7 @@ struct dec_mg_s {
   double omega = 2.0 / 3.0;
   DBuf<double> part, part2, scal;  // reduction partials, scalar outputs
   DBuf<CgState> cg;
-  DBuf<double> cgx, cgp, cgAp;  // CG vectors (finest size)
+  DBuf<double> cgx, cgp, cgp2, cgAp;  // CG vectors (finest size; p double-buffered)
   int fuse_from = -1;           // first level handled by the fused coarse kernel
   DBuf<LevelDesc<double>> desc64;
   DBuf<LevelDesc<float>> desc32;
@@ -1656,8 +1739,8 @@ void finish_hierarchy(dec_mg_s &mg, const std::vector<dec::HostCsr> &As, std::ve
     mg.hdesc64 = d64;
     mg.hdesc32 = d32;
   }
-  mg.part.alloc(kDotBlocks);
-  mg.part2.alloc(kDotBlocks);
+  mg.part.alloc(kCgBlocksMax);  // also the dot partials (kDotBlocks of them)
+  mg.part2.alloc(kCgBlocksMax);
   mg.scal.alloc(8);
   mg.rpart.alloc(static_cast<size_t>(dec::nblocks(mg.lv[0]->n, kBlock)));
   if (hipHostMalloc(reinterpret_cast<void **>(&mg.hnorm), sizeof(double) * (kNormSlots + 1), hipHostMallocDefault) !=
@@ -1669,6 +1752,7 @@ void finish_hierarchy(dec_mg_s &mg, const std::vector<dec::HostCsr> &As, std::ve
   int nf = mg.lv[0]->n;
   mg.cgx.alloc(nf);
   mg.cgp.alloc(nf);
+  mg.cgp2.alloc(nf);
   mg.cgAp.alloc(nf);
 }
 
@@ -1811,26 +1895,27 @@ void dot(dec_mg_s &mg, int n, const double *x, const double *y, double *out, con
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, mg.part.p, out, done);
 }
 
-// cg(A, b, u; itmax = steps) as in Multigrid.jl:860 (float64 only).  Four
-// launches per iteration: the product, the partial sums of p·Ap, then the
-// α and β steps each with its update and the next partial sums.
+// cg(A, b, u; itmax = steps) as in Multigrid.jl:860 (float64 only), on a
+// level above kCgSmallRows rows: 1 + 2 steps launches (the residual with the
+// start, then a direction and a step launch per iteration).
 void cg_smooth(dec_mg_s &mg, Level &L, const double *b, double *u, int steps, hipStream_t s) {
-  int n = L.n;
+  if (steps <= 0) return;
+  int n = L.n, nb = cg_blocks(n);
   CgState *st = mg.cg.p;
-  int *done = reinterpret_cast<int *>(reinterpret_cast<char *>(st) + offsetof(CgState, done));
-  double *x = mg.cgx.p, *p = mg.cgp.p, *Ap = mg.cgAp.p, *r = L.d.t.p;
-  residual<double>(L, b, u, r, s);
-  hipLaunchKernelGGL(k_cgf_start, dim3(kDotBlocks), dim3(kBlock), 0, s, n, x, r, p, mg.part.p);
-  hipLaunchKernelGGL(k_cgf_init, dim3(1), dim3(kBlock), 0, s, st, mg.part.p);
+  double *x = mg.cgx.p, *Ap = mg.cgAp.p, *r = L.d.t.p;
+  double *pb[2] = {mg.cgp.p, mg.cgp2.p};
+  DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_cg2_start<W>), dim3(nb), dim3(kBlock), 0, s, n, L.width,
+                                                   L.vcol.p, L.d.vval.p, b, u, r, pb[0], x, mg.part.p));
   for (int it = 0; it < steps; it++) {
-    DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_spmv_rows<W, double>), G(n), dim3(kBlock), 0, s, n, L.width,
-                                                     L.col.p, L.d.val.p, L.rowid.p, p, Ap, done));
-    hipLaunchKernelGGL(k_dot_partial<double>, dim3(kDotBlocks), dim3(kBlock), 0, s, n, p, Ap, mg.part.p, done);
-    hipLaunchKernelGGL(k_cgf_xr, dim3(kDotBlocks), dim3(kBlock), 0, s, st, it & 1, n, x, p, r, Ap, mg.part.p,
-                       mg.part2.p);
-    hipLaunchKernelGGL(k_cgf_p, dim3(kDotBlocks), dim3(kBlock), 0, s, st, it & 1, n, p, r, mg.part2.p);
+    // iteration it's p is pb[it & 1]; the direction launch forms it from
+    // pb[(it - 1) & 1] (at it = 0, pb[0] already holds p = r and is only read)
+    double *pold = it == 0 ? pb[0] : pb[(it + 1) & 1], *pnew = it == 0 ? pb[1] : pb[it & 1];
+    DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_cg2_dir<W>), dim3(nb), dim3(kBlock), 0, s, st, it, n,
+                                                     L.width, L.vcol.p, L.d.vval.p, r, pold, pnew, Ap, mg.part.p,
+                                                     nb, mg.part2.p));
+    hipLaunchKernelGGL(k_cg2_step, dim3(nb), dim3(kBlock), 0, s, st, it, it + 1 == steps ? 1 : 0, n, x,
+                       pb[it & 1], r, Ap, mg.part2.p, nb, mg.part.p, u);
   }
-  hipLaunchKernelGGL(k_axpy_final, G(n), dim3(kBlock), 0, s, n, x, u);
 }
 
 inline bool learned_on(const dec_mg_s &mg) { return mg.nn && mg.learned && mg.learned_active; }
