@@ -807,6 +807,8 @@ def main():
                     help="time only the operator set on a tetrahedral mesh and print its JSON")
     ap.add_argument("--solve-only", action="store_true",
                     help="time only the Poisson solves on levels 3->8 and print their JSON")
+    ap.add_argument("--reference-calls-only", action="store_true",
+                    help="time only the reference's own multigrid calls (CG smoother) and print their JSON")
     ap.add_argument("--n3d", type=int, default=64, help="cubes per side of the Kuhn box of the 3-D leg")
     ap.add_argument("--traffic-per-op", action="store_true",
                     help="measure the HBM traffic of each operator of the step (two counter passes each)")
@@ -832,6 +834,11 @@ def main():
         return
     if args.solve_only:
         print(json.dumps({"poisson": poisson(torch, cs)[0]}), flush=True)
+        return
+    if args.reference_calls_only:
+        _, md, fd = poisson(torch, cs)
+        del md
+        print(json.dumps({"multigrid_reference_calls": mg_reference_calls(torch, cs, fd)}), flush=True)
         return
     t = time.perf_counter()
     mesh = cs.icosphere(args.k)

@@ -737,15 +737,16 @@ __global__ void __launch_bounds__(kBlock)
 
 // The CG smoother (the recurrence of k_cg2_*) on a level of at most
 // kCgSmallRows rows, as one workgroup of kCgThreads threads: thread t owns
-// rows t + kCgThreads i (i < R) and keeps their x and r in registers; p and
-// A p live in LDS, so the product gathers from LDS (one row at a time, which
-// keeps its loads within the registers of 1024 threads), and each dot product is
-// one workgroup reduction in a fixed order (waves, then the waves' sums).
-// One launch per smoothing in place of 1 + 2 steps launches.  Its dot
-// products add in another order than the multi-launch partition, so its
-// iterates agree with those of the multi-launch smoother to rounding.
-constexpr int kCgSmallRows = 4000;  // p and A p in 64,000 bytes of LDS
+// rows t + kCgThreads k (k < R) and holds their slots, values, x, r and A p
+// in registers for the whole smoothing; p lives in LDS, so after the first
+// residual every product is LDS gathers only.  Each dot product is one
+// workgroup reduction in a fixed order (waves, then the waves' sums).  One
+// launch per smoothing in place of 1 + 2 steps launches.  Its dot products
+// add in another order than the multi-launch partition, so its iterates
+// agree with those of the multi-launch smoother to rounding.
 constexpr int kCgThreads = 1024;
+constexpr int kCgRowsPerThread = 3;  // slots and values of 3 rows of up to 7 fit in 128 VGPRs
+constexpr int kCgSmallRows = kCgThreads * kCgRowsPerThread;
 
 __device__ double wg_sum(double v, double *red) {
   for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off, 64);
@@ -758,23 +759,48 @@ __device__ double wg_sum(double v, double *red) {
   return t;
 }
 
+// Σ_s v[s] x[c[s]] over the slots with c[s] ≥ 0, in slot order (the sum of
+// dec_ell::row).
+template <int W>
+__device__ __forceinline__ double row_regs(const int (&c)[W], const double (&v)[W], const double *x) {
+  double xv[W];
+#pragma unroll
+  for (int s = 0; s < W; s++) xv[s] = x[c[s] < 0 ? 0 : c[s]];
+  double acc = 0.0;
+#pragma unroll
+  for (int s = 0; s < W; s++)
+    if (c[s] >= 0) acc = acc + v[s] * xv[s];
+  return acc;
+}
+
 template <int W, int R>
 __global__ void __launch_bounds__(kCgThreads)
-    k_cg_smooth_small(int n, int width, const int *__restrict__ vcol, const double *__restrict__ vval,
+    k_cg_smooth_small(int n, const int *__restrict__ vcol, const double *__restrict__ vval,
                       const double *__restrict__ b, double *__restrict__ u, int steps) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double *p = reinterpret_cast<double *>(smem_raw);
-  double *Ap = p + n;
   __shared__ double red[kCgThreads / 64];
-  double x[R], r[R];
+  int c[R][W];
+  double v[R][W], x[R], r[R], Ap[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    int i = static_cast<int>(threadIdx.x) + k * kCgThreads;
+#pragma unroll
+    for (int s = 0; s < W; s++) {
+      size_t o = static_cast<size_t>(s) * n + (i < n ? i : 0);
+      c[k][s] = i < n ? vcol[o] : -1;
+      v[k][s] = vval[o];
+    }
+  }
   double acc = 0.0;
 #pragma unroll
   for (int k = 0; k < R; k++) {
     int i = static_cast<int>(threadIdx.x) + k * kCgThreads;
     x[k] = 0.0;
     r[k] = 0.0;
+    Ap[k] = 0.0;
     if (i < n) {
-      r[k] = b[i] - dec_ell::row<W, double, double>(n, width, i, vcol, vval, u);
+      r[k] = b[i] - row_regs<W>(c[k], v[k], u);
       p[i] = r[k];
       acc = acc + r[k] * r[k];
     }
@@ -788,13 +814,12 @@ __global__ void __launch_bounds__(kCgThreads)
   for (int it = 0; it < steps && !done; it++) {
     __syncthreads();  // p complete
     double pap = 0.0;
-#pragma unroll 1
+#pragma unroll
     for (int k = 0; k < R; k++) {
       int i = static_cast<int>(threadIdx.x) + k * kCgThreads;
       if (i < n) {
-        double api = dec_ell::row<W, double, double>(n, width, i, vcol, vval, p);
-        Ap[i] = api;
-        pap = pap + p[i] * api;
+        Ap[k] = row_regs<W>(c[k], v[k], p);
+        pap = pap + p[i] * Ap[k];
       }
     }
     double pAp = wg_sum(pap, red);
@@ -806,7 +831,7 @@ __global__ void __launch_bounds__(kCgThreads)
       int i = static_cast<int>(threadIdx.x) + k * kCgThreads;
       if (i < n) {
         x[k] = x[k] + a * p[i];
-        r[k] = r[k] - a * Ap[i];
+        r[k] = r[k] - a * Ap[k];
         rr = rr + r[k] * r[k];
       }
     }
@@ -1991,11 +2016,11 @@ void smooth(dec_mg_s &mg, int lvl, Level &L, const T *b, T *u, bool post, bool u
   }
   if (mg.alg == DEC_SMOOTH_CG) {
     if constexpr (sizeof(T) == 8) {
-      if (L.n <= kCgSmallRows && L.width >= 1 && L.width <= 8) {
-        size_t lds = 2 * static_cast<size_t>(L.n) * sizeof(double);  // p and A p
+      if (L.n <= kCgSmallRows && L.width >= 1 && L.width <= 7) {
+        size_t lds = static_cast<size_t>(L.n) * sizeof(double);  // p
         DEC_ELL_WIDTH_SWITCH(L.width, if constexpr (W > 0) {
-          hipLaunchKernelGGL((k_cg_smooth_small<W, (kCgSmallRows + kCgThreads - 1) / kCgThreads>), dim3(1), dim3(kCgThreads), lds,
-                             s, L.n, L.width, L.vcol.p, L.d.vval.p, b, u, steps);
+          hipLaunchKernelGGL((k_cg_smooth_small<W, kCgRowsPerThread>), dim3(1), dim3(kCgThreads), lds, s, L.n,
+                             L.vcol.p, L.d.vval.p, b, u, steps);
         });
       } else {
         cg_smooth(mg, L, b, u, steps, s);
