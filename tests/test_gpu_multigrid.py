@@ -115,6 +115,35 @@ def test_smoothers_match_oracle(O, cs, sphere_pair, alg):
     assert np.linalg.norm(u.cpu().numpy() - u_o) <= 1e-10 * np.linalg.norm(u_o)
 
 
+@pytest.mark.parametrize("steps", [1, 2, 3])
+def test_cg_smoother_forms_match_oracle(O, cs, sphere5_pair, steps):
+    """The Krylov CG smoother (Multigrid.jl:860) on levels 10,242 .. 12: the
+    finest level runs the multi-workgroup form (two launches per iteration),
+    the levels of at most 3,072 rows the one-workgroup form.  The dot products
+    add in another order than the oracle's, so u agrees to a tolerance.  A
+    zero right-hand side stops CG at once and leaves u zero.
+
+    Steps are kept at 3 or fewer: the 12-row level has three distinct nonzero
+    eigenvalues, so CG converges there in 3 steps, and further steps on its
+    singular, slightly inconsistent system divide by a p·Ap that rounding
+    sets, which moves u along the constants by an amount no tolerance covers.
+    This happens in the oracle too: u's mean is 16 at 5 steps, 8e3 at 8 and
+    1e8 at 40."""
+    import torch
+
+    H, md = sphere5_pair
+    L = H.A[0]
+    b = rhs(O, L, 11)
+    b = b - b.mean()
+    u_o = H.mu_cycles(np.zeros(L.n), b, 2, alg=O.CG, steps=steps)
+    u = torch.zeros(L.n, dtype=torch.float64, device="cuda")
+    cs.multigrid_vcycles(u, torch.tensor(b, device="cuda"), md, 2, cs.cg, steps=steps)
+    assert np.linalg.norm(u.cpu().numpy() - u_o) <= 1e-8 * np.linalg.norm(u_o)
+    z = torch.zeros(L.n, dtype=torch.float64, device="cuda")
+    cs.multigrid_vcycles(z, torch.zeros_like(z), md, 1, cs.cg, steps=steps)
+    assert torch.count_nonzero(z).item() == 0
+
+
 @pytest.mark.parametrize("which", ["grid", "sphere"])
 def test_gauss_seidel_colouring(which, grid_pair, sphere_pair):
     """Every level's colouring is proper for the level's operator pattern and
