@@ -748,14 +748,19 @@ constexpr int kCgThreads = 1024;
 constexpr int kCgRowsPerThread = 3;  // slots and values of 3 rows of up to 7 fit in 128 VGPRs
 constexpr int kCgSmallRows = kCgThreads * kCgRowsPerThread;
 
-__device__ double wg_sum(double v, double *red) {
+// The workgroup's sum of v: each wave's shuffle tree, then the 16 wave sums
+// in order, read by every thread.  Successive calls alternate between the two
+// halves of red[2][16]: a thread writes a half again only two calls later,
+// after the barrier of the call between, by which time every thread has read
+// it, so one barrier per call suffices.
+__device__ double wg_sum(double v, double (*red)[kCgThreads / 64], int &half) {
   for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off, 64);
-  int w = threadIdx.x >> 6;
-  __syncthreads();  // every thread has read red[] of the previous sum
-  if ((threadIdx.x & 63) == 0) red[w] = v;
+  double *h = red[half];
+  half ^= 1;
+  if ((threadIdx.x & 63) == 0) h[threadIdx.x >> 6] = v;
   __syncthreads();
   double t = 0.0;
-  for (int k = 0; k < kCgThreads / 64; k++) t = t + red[k];
+  for (int k = 0; k < kCgThreads / 64; k++) t = t + h[k];
   return t;
 }
 
@@ -779,7 +784,8 @@ __global__ void __launch_bounds__(kCgThreads)
                       const double *__restrict__ b, double *__restrict__ u, int steps) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double *p = reinterpret_cast<double *>(smem_raw);
-  __shared__ double red[kCgThreads / 64];
+  __shared__ double red[2][kCgThreads / 64];
+  int half = 0;
   int c[R][W];
   double v[R][W], x[R], r[R], Ap[R];
 #pragma unroll
@@ -805,7 +811,7 @@ __global__ void __launch_bounds__(kCgThreads)
       acc = acc + r[k] * r[k];
     }
   }
-  double gamma = wg_sum(acc, red);
+  double gamma = wg_sum(acc, red, half);
   double rnorm = sqrt(gamma);
   const double sq = 1.4901161193847656e-08, eps = 2.220446049250313e-16;
   const double eps_stop = sq + sq * rnorm;
@@ -822,7 +828,7 @@ __global__ void __launch_bounds__(kCgThreads)
         pap = pap + p[i] * Ap[k];
       }
     }
-    double pAp = wg_sum(pap, red);
+    double pAp = wg_sum(pap, red, half);
     if (pAp <= eps * pnorm2 && fabs(pAp) <= eps * pnorm2) break;
     double a = gamma / pAp;
     double rr = 0.0;
@@ -835,7 +841,7 @@ __global__ void __launch_bounds__(kCgThreads)
         rr = rr + r[k] * r[k];
       }
     }
-    double gn = wg_sum(rr, red);
+    double gn = wg_sum(rr, red, half);
     double rn = sqrt(gn);
     if (rn <= eps_stop || rn + 1.0 <= 1.0) break;
     double beta = gn / gamma;
