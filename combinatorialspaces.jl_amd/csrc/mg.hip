@@ -611,6 +611,17 @@ __device__ __forceinline__ double row_dir(int n, int width, int i, const int *__
   return acc;
 }
 
+// *f as read by thread 0, the same for every thread of the block.  Block 0's
+// thread 0 may write *f later in the same launch; a block that reads that
+// value takes the exit its own scalars would take, and reading once keeps the
+// waves of a block on one path through its barriers.
+__device__ int block_flag(const int *f) {
+  __shared__ int v;
+  if (threadIdx.x == 0) v = *f;
+  __syncthreads();
+  return v;
+}
+
 // r = b - A u, p = r, x = 0, and the partial sums of r·r.
 template <int W>
 __global__ void __launch_bounds__(kBlock)
@@ -638,7 +649,7 @@ __global__ void __launch_bounds__(kBlock)
     k_cg2_dir(CgState *st, int it, int n, int width, const int *__restrict__ vcol, const double *__restrict__ vval,
               const double *__restrict__ r, const double *__restrict__ pold, double *__restrict__ p,
               double *__restrict__ Ap, const double *__restrict__ part_rr, int nb, double *__restrict__ part_pap) {
-  if (it > 0 && st->done) return;
+  if (it > 0 && block_flag(&st->done)) return;
   double g = sum_partials(part_rr, nb);
   double rn = sqrt(g);
   bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -701,7 +712,7 @@ __global__ void __launch_bounds__(kBlock)
                double *__restrict__ r, const double *__restrict__ Ap, const double *__restrict__ part_pap, int nb,
                double *__restrict__ part_rr, double *__restrict__ u) {
   const double eps = 2.220446049250313e-16;
-  bool stop = st->done != 0;
+  bool stop = block_flag(&st->done) != 0;
   double a = 0.0;
   if (!stop) {
     double pAp = sum_partials(part_pap, nb);
