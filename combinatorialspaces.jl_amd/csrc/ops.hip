@@ -493,6 +493,17 @@ __global__ void k_wedge_pd(int ne, int width, const int *__restrict__ col, const
 constexpr int kCgChunk = 8;  // most iterations between host reads of the stop flag
 enum { CS_BB = 0, CS_RR, CS_DONE, CS_ITER, CS_ALPHA, CS_OMEGA, CS_RZ0, CS_RZ1, CS_COUNT };
 
+// The stop flag as read by thread 0, the same for every thread of the block.
+// Block 0's thread 0 may set the flag later in the same launch; a block that
+// reads it set takes the exit its own scalars would take, and reading once
+// keeps the waves of a block on one path through its barriers.
+__device__ bool block_stopped(const double *sc) {
+  __shared__ int v;
+  if (threadIdx.x == 0) v = sc[CS_DONE] != 0.0;
+  __syncthreads();
+  return v != 0;
+}
+
 // ---- fused conjugate gradients: two launches per iteration ----
 // Every block of a launch reduces the previous launch's kFcBlocks partial sums
 // itself, in one fixed order, so all blocks hold the same scalars and no
@@ -570,7 +581,7 @@ __global__ __launch_bounds__(256) void k_fcg_spmv(int n, int width, const int *_
                                                   double *__restrict__ partA, double *__restrict__ sc, int it,
                                                   int last, double rtol) {
   __shared__ double lds[512];
-  if (sc[CS_DONE] != 0.0) return;
+  if (block_stopped(sc)) return;
   double rz, rr;
   fc_sum(partB, rz, rr, lds);
   double bb = it == 0 ? rr : sc[CS_BB];
@@ -628,7 +639,7 @@ __global__ __launch_bounds__(256) void k_fcg_update(int n, const T *__restrict__
                                                     const double *__restrict__ partA, double *__restrict__ partB,
                                                     double *__restrict__ sc, int it) {
   __shared__ double lds[512];
-  if (sc[CS_DONE] != 0.0) return;
+  if (block_stopped(sc)) return;
   double pq, unused;
   fc_sum(partA, pq, unused, lds);
   if (pq == 0.0) {
@@ -679,7 +690,7 @@ __global__ __launch_bounds__(256) void k_fbi_dir(int n, const T *__restrict__ r,
                                                  const double *__restrict__ partR, double *__restrict__ sc, int it,
                                                  int last, double rtol) {
   __shared__ double lds[512];
-  if (sc[CS_DONE] != 0.0) return;
+  if (block_stopped(sc)) return;
   double rho, rr;
   fc_sum(partR, rho, rr, lds);
   double bb = it == 0 ? rr : sc[CS_BB];
@@ -712,7 +723,7 @@ __global__ __launch_bounds__(256) void k_fbi_spmv(int n, int width, const int *_
                                                   T *__restrict__ q, const T *__restrict__ a, int with_qq,
                                                   double *__restrict__ part, const double *__restrict__ sc) {
   __shared__ double lds[512];
-  if (sc[CS_DONE] != 0.0) return;
+  if (block_stopped(sc)) return;
   double s1 = 0.0, s2 = 0.0;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kFcBlocks * 256) {
     T qi = dec_ell::row<W, T, T>(n, width, i, col, val, xin);
@@ -729,7 +740,7 @@ __global__ __launch_bounds__(256) void k_fbi_half(int n, T *__restrict__ yz, con
                                                   const T *__restrict__ dinv, T *__restrict__ x, T *__restrict__ r,
                                                   const double *__restrict__ partV, double *__restrict__ sc, int it) {
   __shared__ double lds[512];
-  if (sc[CS_DONE] != 0.0) return;
+  if (block_stopped(sc)) return;
   double rv, unused;
   fc_sum(partV, rv, unused, lds);
   if (rv == 0.0) {
@@ -755,7 +766,7 @@ __global__ __launch_bounds__(256) void k_fbi_full(int n, const T *__restrict__ z
                                                   const double *__restrict__ partT, double *__restrict__ partR,
                                                   double *__restrict__ sc, int it) {
   __shared__ double lds[512];
-  if (sc[CS_DONE] != 0.0) return;
+  if (block_stopped(sc)) return;
   double ts, tt;
   fc_sum(partT, ts, tt, lds);
   if (tt == 0.0 || ts == 0.0) {
