@@ -650,6 +650,29 @@ __global__ void __launch_bounds__(kBlock)
               const double *__restrict__ r, const double *__restrict__ pold, double *__restrict__ p,
               double *__restrict__ Ap, const double *__restrict__ part_rr, int nb, double *__restrict__ part_pap) {
   if (it > 0 && block_flag(&st->done)) return;
+  const int stride = gridDim.x * kBlock, i0 = blockIdx.x * kBlock + threadIdx.x;
+  // The thread's first row is loaded before the scalar step, which does not
+  // need it, so its loads are in flight while the partial sums are reduced.
+  constexpr int WP = W > 0 ? W : 1;
+  int c0[WP];
+  double v0[WP], rc0[WP], qc0[WP], ri0 = 0.0, qi0 = 0.0;
+  if constexpr (W > 0) {
+    int ia = min(i0, n - 1);
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      size_t o = static_cast<size_t>(k) * n + ia;
+      c0[k] = vcol[o];
+      v0[k] = vval[o];
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      int cc = c0[k] < 0 ? 0 : c0[k];
+      rc0[k] = r[cc];
+      qc0[k] = pold[cc];
+    }
+    ri0 = r[ia];
+    qi0 = pold[ia];
+  }
   double g = sum_partials(part_rr, nb);
   double rn = sqrt(g);
   bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -687,7 +710,30 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   double acc = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  int i = i0;
+  if constexpr (W > 0) {
+    // the first row from the registers, with the sums of dec_ell::row and
+    // row_dir (slot order, the same products)
+    if (i0 < n) {
+      double pi, api = 0.0;
+      if (it == 0) {
+        pi = qi0;
+#pragma unroll
+        for (int k = 0; k < W; k++)
+          if (c0[k] >= 0) api = api + v0[k] * qc0[k];
+      } else {
+        pi = ri0 + beta * qi0;
+        p[i0] = pi;
+#pragma unroll
+        for (int k = 0; k < W; k++)
+          if (c0[k] >= 0) api = api + v0[k] * (rc0[k] + beta * qc0[k]);
+      }
+      Ap[i0] = api;
+      acc = acc + pi * api;
+    }
+    i += stride;
+  }
+  for (; i < n; i += stride) {
     double pi, api;
     if (it == 0) {
       pi = pold[i];
@@ -705,13 +751,24 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Iteration it's step: α = γ / p·Ap (zero-curvature exit), x += α p,
-// r -= α Ap and the partial sums of r·r; on the last iteration u += x
-// instead (x as it stands when an exit came earlier).
+// r -= α Ap and the partial sums of r·r; on the last iteration (LAST) u += x
+// instead (x as it stands when an exit came earlier).  The thread's first
+// element is loaded before the scalar step, so those loads are in flight
+// while the partial sums are reduced.
+template <bool LAST>
 __global__ void __launch_bounds__(kBlock)
-    k_cg2_step(CgState *st, int it, int last, int n, double *__restrict__ x, const double *__restrict__ p,
+    k_cg2_step(CgState *st, int it, int n, double *__restrict__ x, const double *__restrict__ p,
                double *__restrict__ r, const double *__restrict__ Ap, const double *__restrict__ part_pap, int nb,
                double *__restrict__ part_rr, double *__restrict__ u) {
   const double eps = 2.220446049250313e-16;
+  const int stride = gridDim.x * kBlock, i0 = blockIdx.x * kBlock + threadIdx.x, ia = min(i0, n - 1);
+  double x0 = x[ia], p0 = p[ia], r0 = 0.0, a0 = 0.0, u0 = 0.0;
+  if constexpr (LAST) {
+    u0 = u[ia];
+  } else {
+    r0 = r[ia];
+    a0 = Ap[ia];
+  }
   bool stop = block_flag(&st->done) != 0;
   double a = 0.0;
   if (!stop) {
@@ -726,9 +783,12 @@ __global__ void __launch_bounds__(kBlock)
       if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = a;
     }
   }
-  int stride = gridDim.x * kBlock;
-  if (last) {
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+  if constexpr (LAST) {
+    if (i0 < n) {
+      double xi = stop ? x0 : x0 + a * p0;
+      u[i0] = xi + u0;
+    }
+    for (int i = i0 + stride; i < n; i += stride) {
       double xi = stop ? x[i] : x[i] + a * p[i];
       u[i] = xi + u[i];
     }
@@ -736,7 +796,13 @@ __global__ void __launch_bounds__(kBlock)
   }
   if (stop) return;
   double acc = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+  if (i0 < n) {
+    x[i0] = x0 + a * p0;
+    double ri = r0 - a * a0;
+    r[i0] = ri;
+    acc = acc + ri * ri;
+  }
+  for (int i = i0 + stride; i < n; i += stride) {
     x[i] = x[i] + a * p[i];
     double ri = r[i] - a * Ap[i];
     r[i] = ri;
@@ -1955,8 +2021,12 @@ void cg_smooth(dec_mg_s &mg, Level &L, const double *b, double *u, int steps, hi
     DEC_ELL_WIDTH_SWITCH(L.width, hipLaunchKernelGGL((k_cg2_dir<W>), dim3(nb), dim3(kBlock), 0, s, st, it, n,
                                                      L.width, L.vcol.p, L.d.vval.p, r, pold, pnew, Ap, mg.part.p,
                                                      nb, mg.part2.p));
-    hipLaunchKernelGGL(k_cg2_step, dim3(nb), dim3(kBlock), 0, s, st, it, it + 1 == steps ? 1 : 0, n, x,
-                       pb[it & 1], r, Ap, mg.part2.p, nb, mg.part.p, u);
+    if (it + 1 == steps)
+      hipLaunchKernelGGL(k_cg2_step<true>, dim3(nb), dim3(kBlock), 0, s, st, it, n, x, pb[it & 1], r, Ap,
+                         mg.part2.p, nb, mg.part.p, u);
+    else
+      hipLaunchKernelGGL(k_cg2_step<false>, dim3(nb), dim3(kBlock), 0, s, st, it, n, x, pb[it & 1], r, Ap,
+                         mg.part2.p, nb, mg.part.p, u);
   }
 }
 
